@@ -1,0 +1,131 @@
+"""ctypes mirror of ``include/abcmb.h`` and loaders for the native libraries.
+
+The GPU library (``libabcmb_hip.so``) is mandatory for every hot-path call:
+``hip_lib()`` raises when it is missing instead of falling back to any CPU
+code.  The host background producer (``libabcmb_host.so``) is plain C++.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+HIP_LIB_PATH = os.path.join(_HERE, "libabcmb_hip.so")
+HOST_LIB_PATH = os.path.join(_HERE, "libabcmb_host.so")
+
+TAU_TAB_N = 10000
+TAU_TAB_LO = -33.0
+N_PT_FIELDS = 21
+PT_FIELDS = (
+    "delta_m", "theta_b_prime", "metric_eta", "metric_h_prime", "metric_eta_prime",
+    "metric_alpha", "metric_alpha_prime",
+    "cdm_delta", "b_delta", "b_theta",
+    "g_delta", "g_theta", "g_sigma", "g_G0", "g_G2",
+    "ur_delta", "ur_theta", "ur_sigma",
+    "ncdm_delta", "ncdm_theta", "ncdm_sigma",
+)
+PT_INDEX = {n: i for i, n in enumerate(PT_FIELDS)}
+
+_D = C.c_double
+_I = C.c_int32
+_P = C.c_void_p
+
+
+class AbcmbCosmo(C.Structure):
+    _fields_ = [(n, _D) for n in (
+        "h", "omega_b", "omega_cdm", "omega_Lambda",
+        "A_s", "n_s", "k_pivot", "TCMB0", "YHe",
+        "N_ur", "T_ur", "N_ncdm", "T_ncdm", "m_ncdm",
+        "R_nu", "R_b", "om", "omega_r", "omega_m",
+        "tau0",
+        "xe_x0", "xe_dx", "xe_lna_last", "xe_last",
+        "tm_x0", "tm_dx", "tm_lna_last", "tm_last",
+        "lna_rec", "lna_transfer_start", "z_reion")] + [
+        ("has_ncdm", _I), ("n_q", _I), ("xe_n", _I), ("tm_n", _I),
+        ("n_save", _I), ("nk", _I),
+        ("tau_tab", _P), ("xe_tab", _P), ("tm_tab", _P),
+        ("lna_save", _P), ("k", _P), ("pt", _P)]
+
+
+class AbcmbPertItem(C.Structure):
+    _fields_ = [("k", _D), ("lna_start", _D), ("rtol", _D), ("atol", _D),
+                ("cosmo", _I), ("ik", _I), ("pad0", _I), ("pad1", _I)]
+
+
+class AbcmbOdeSettings(C.Structure):
+    _fields_ = [(n, _D) for n in ("pcoeff", "icoeff", "dcoeff", "dt0", "safety",
+                                  "factormin", "factormax", "newton_kappa")] + [
+        (n, _I) for n in ("max_steps", "max_newton", "l_g", "l_pol", "l_ur", "l_ncdm", "n_q", "n")]
+
+
+assert C.sizeof(AbcmbCosmo) == 31 * 8 + 6 * 4 + 6 * 8
+assert C.sizeof(AbcmbPertItem) == 48
+
+_host = None
+_hip = None
+
+
+class HipLibraryMissing(RuntimeError):
+    pass
+
+
+def host_lib():
+    global _host
+    if _host is None:
+        if not os.path.exists(HOST_LIB_PATH):
+            raise RuntimeError(f"{HOST_LIB_PATH} is not built; run __graft_entry__.build()")
+        lib = C.CDLL(HOST_LIB_PATH)
+        lib.abcmb_host_background.argtypes = [C.POINTER(AbcmbCosmo)] + [_D] * 6
+        lib.abcmb_host_recomb_points.argtypes = []
+        lib.abcmb_host_los_background.argtypes = [C.POINTER(AbcmbCosmo), _P, C.c_int, _P]
+        lib.abcmb_host_lna_start.argtypes = [C.POINTER(AbcmbCosmo), _P, C.c_int, _D, _D, _P]
+        lib.abcmb_host_eval.argtypes = [C.POINTER(AbcmbCosmo), _P, C.c_int, _P]
+        _host = lib
+    return _host
+
+
+HIP_SYMBOLS = (
+    "abcmb_last_error", "abcmb_device_init", "abcmb_pert_workspace_size", "abcmb_pert_fwd",
+    "abcmb_sources_workspace_size", "abcmb_sources", "abcmb_los_cl", "abcmb_apply_operator",
+    "abcmb_lensing_clpp", "abcmb_lens_workspace_size", "abcmb_lens", "abcmb_pk_lin",
+)
+
+
+def hip_lib():
+    """Load libabcmb_hip.so; raise (never fall back) when it is absent."""
+    global _hip
+    if _hip is None:
+        if not os.path.exists(HIP_LIB_PATH):
+            raise HipLibraryMissing(
+                f"{HIP_LIB_PATH} is missing: the perturbation/LOS path only runs through the "
+                "HIP library (build it with __graft_entry__.build())")
+        lib = C.CDLL(HIP_LIB_PATH)
+        lib.abcmb_last_error.restype = C.c_char_p
+        lib.abcmb_pert_workspace_size.restype = C.c_size_t
+        lib.abcmb_pert_workspace_size.argtypes = [C.c_int]
+        lib.abcmb_pert_fwd.argtypes = [_P, C.c_int, _P, C.c_int, C.POINTER(AbcmbOdeSettings),
+                                       _P, _P, _P, _P, C.c_size_t, _P]
+        lib.abcmb_sources_workspace_size.restype = C.c_size_t
+        lib.abcmb_sources_workspace_size.argtypes = [C.c_int, C.c_int]
+        lib.abcmb_sources.argtypes = [_P, _P, C.c_int, C.c_int, _P, _P, C.c_int, _P, _P,
+                                      _P, C.c_size_t, _P]
+        lib.abcmb_los_cl.argtypes = [_P, C.c_int, _P, _D, _P, C.c_int, _P, _P, _P, C.c_int,
+                                     C.c_int, _P, C.c_int, _D, _D, _D, C.c_int, _P, _P, _P]
+        lib.abcmb_apply_operator.argtypes = [_P, C.c_int, C.c_int, _P, C.c_int, _P, _P]
+        lib.abcmb_lensing_clpp.argtypes = [C.POINTER(AbcmbCosmo), _P, _P, _P, _P, C.c_int, _D,
+                                           _P, C.c_int, _P, _P]
+        lib.abcmb_lens_workspace_size.restype = C.c_size_t
+        lib.abcmb_lens_workspace_size.argtypes = [C.c_int, C.c_int]
+        lib.abcmb_lens.argtypes = [_P, _P, C.c_int, _P, _P, C.c_int, _P, _P, C.c_size_t, _P]
+        lib.abcmb_pk_lin.argtypes = [_P, _P, C.c_int, C.c_int, _P, C.c_int, _D, _D, _D, _P, _P]
+        _hip = lib
+    return _hip
+
+
+def check(rc: int, what: str):
+    if rc != 0:
+        msg = ""
+        if _hip is not None:
+            m = _hip.abcmb_last_error()
+            msg = m.decode() if m else ""
+        raise RuntimeError(f"{what} failed with status {rc}: {msg}")

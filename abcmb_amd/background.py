@@ -1,0 +1,94 @@
+"""Host background: the tables the GPU perturbation/LOS path reads.
+
+``Background`` plays the role of the reference's ``Background`` object
+(``abcmb/background.py:420-768``) for this library: it owns the conformal
+time table, the x_e / T_m histories, tau_0, lna_rec, lna_transfer_start and
+z_reion, produced by the native host producer ``libabcmb_host.so``.  The
+recombination history there is a Saha + Peebles stand-in for HyRex (the
+reference's HyRex cannot run in this environment); everything downstream of
+the tables follows the reference's formulas.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import abi
+
+
+def _ptr(a: np.ndarray):
+    return a.ctypes.data_as(C.c_void_p)
+
+
+class Background:
+    """Background of one cosmology (host-resident FP64 tables)."""
+
+    def __init__(self, params: dict, has_ncdm: bool, n_q: int = 3, k_pivot: float = 0.05,
+                 input_tau_reion: bool = True, xe_pad: int = 0):
+        lib = abi.host_lib()
+        self.params = params
+        n_rec = lib.abcmb_host_recomb_points()
+        self.tau_tab = np.zeros(abi.TAU_TAB_N)
+        self.xe_tab = np.zeros(n_rec + xe_pad)
+        self.tm_tab = np.zeros(n_rec + xe_pad)
+        c = abi.AbcmbCosmo()
+        p = params
+        c.h, c.omega_b, c.omega_cdm, c.omega_Lambda = p["h"], p["omega_b"], p["omega_cdm"], p["omega_Lambda"]
+        c.A_s, c.n_s, c.k_pivot, c.TCMB0, c.YHe = p["A_s"], p["n_s"], k_pivot, p["TCMB0"], p["YHe"]
+        c.N_ur, c.T_ur = p["N_nu_massless"], p["T_nu_massless"]
+        c.N_ncdm, c.T_ncdm, c.m_ncdm = p["N_nu_massive"], p["T_nu_massive"], p["m_nu_massive"]
+        c.R_nu, c.R_b, c.om, c.omega_r, c.omega_m = p["R_nu"], p["R_b"], p["om"], p["omega_r"], p["omega_m"]
+        c.has_ncdm = 1 if has_ncdm else 0
+        c.n_q = n_q if has_ncdm else 0
+        c.xe_n = c.tm_n = n_rec + xe_pad
+        c.tau_tab = _ptr(self.tau_tab).value
+        c.xe_tab = _ptr(self.xe_tab).value
+        c.tm_tab = _ptr(self.tm_tab).value
+        if input_tau_reion:
+            tau_re, z_re = p["tau_reion"], 0.0
+        else:
+            tau_re, z_re = -1.0, p["z_reion"]
+        rc = lib.abcmb_host_background(C.byref(c), tau_re, z_re, p["Delta_z_reion"], p["z_reion_He"],
+                                       p["Delta_z_reion_He"], p["exp_reion"])
+        if rc != 0:
+            raise RuntimeError(f"abcmb_host_background failed ({rc})")
+        self.cosmo = c
+        self.n_rec = n_rec
+
+    # scalar accessors mirroring the reference's attribute names
+    @property
+    def tau0(self): return self.cosmo.tau0
+    @property
+    def lna_rec(self): return self.cosmo.lna_rec
+    @property
+    def lna_transfer_start(self): return self.cosmo.lna_transfer_start
+    @property
+    def z_reion(self): return self.cosmo.z_reion
+
+    def los_vectors(self, lna: np.ndarray) -> np.ndarray:
+        """[6, n]: tau, g, g', aH, exp(-kappa), aH' aH on ``lna``."""
+        lna = np.ascontiguousarray(lna, dtype=np.float64)
+        out = np.zeros((6, lna.size))
+        rc = abi.host_lib().abcmb_host_los_background(C.byref(self.cosmo), _ptr(lna), lna.size, _ptr(out))
+        if rc != 0:
+            raise RuntimeError("abcmb_host_los_background failed")
+        return out
+
+    def lna_start(self, k: np.ndarray, R_tc: float, R_large: float) -> np.ndarray:
+        k = np.ascontiguousarray(k, dtype=np.float64)
+        out = np.zeros(k.size)
+        rc = abi.host_lib().abcmb_host_lna_start(C.byref(self.cosmo), _ptr(k), k.size, R_tc, R_large, _ptr(out))
+        if rc != 0:
+            raise RuntimeError("abcmb_host_lna_start failed")
+        return out
+
+    def eval(self, lna: np.ndarray) -> dict:
+        """aH, tau, tau_c, cs2, x_e, T_m, R at ``lna`` (host evaluation)."""
+        lna = np.ascontiguousarray(np.atleast_1d(lna), dtype=np.float64)
+        out = np.zeros((7, lna.size))
+        abi.host_lib().abcmb_host_eval(C.byref(self.cosmo), _ptr(lna), lna.size, _ptr(out))
+        return dict(zip(("aH", "tau", "tau_c", "cs2", "xe", "Tm", "R"), out))
+
+    def tau(self, lna):
+        return self.eval(lna)["tau"]

@@ -1,0 +1,55 @@
+"""Build the native libraries in-tree.
+
+``libabcmb_hip.so``  - HIP kernels + C ABI, gfx950 only (hipcc).
+``libabcmb_host.so`` - host background producer (g++ with OpenMP).
+"""
+from __future__ import annotations
+
+import glob
+import os
+import subprocess
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+CSRC = os.path.join(HERE, "csrc")
+ROOT = os.path.dirname(HERE)
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+
+HIP_FLAGS = [
+    "--offload-arch=gfx950", "-O3", "-fPIC", "-shared", "-std=c++17",
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    "-ffp-contract=fast", "-munsafe-fp-atomics",
+    f"-I{os.path.join(ROOT, 'include')}",
+]
+
+
+def _run(cmd):
+    print(" ".join(cmd), flush=True)
+    subprocess.run(cmd, check=True)
+
+
+def build_host():
+    out = os.path.join(HERE, "libabcmb_host.so")
+    _run(["g++", "-O2", "-fPIC", "-shared", "-std=c++17", "-fopenmp",
+          f"-I{os.path.join(ROOT, 'include')}",
+          os.path.join(CSRC, "host_background.cpp"), "-o", out])
+    return out
+
+
+def build_hip():
+    out = os.path.join(HERE, "libabcmb_hip.so")
+    srcs = sorted(glob.glob(os.path.join(CSRC, "*.hip")))
+    tmp = out + ".tmp"
+    _run([HIPCC] + HIP_FLAGS + srcs + ["-o", tmp])
+    os.replace(tmp, out)
+    return out
+
+
+def build_all():
+    build_host()
+    build_hip()
+
+
+if __name__ == "__main__":
+    what = sys.argv[1] if len(sys.argv) > 1 else "all"
+    {"host": build_host, "hip": build_hip, "all": build_all}[what]()

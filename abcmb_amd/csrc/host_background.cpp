@@ -1,0 +1,512 @@
+// Host-side background producer for the perturbation + LOS path.
+//
+// Produces, per cosmology and in FP64, everything the GPU hot path reads
+// about the homogeneous universe: the conformal-time table, the free-electron
+// fraction and matter temperature histories, reionization, the visibility
+// function, and the discrete grid choices the reference makes
+// (lna_rec, lna_transfer_start, per-k start times).
+//
+// Reference behaviour followed:
+//   closed-form rho/P/H/aH          abcmb/species.py (rho/P of each fluid),
+//                                   abcmb/background.py:101-208
+//   tau(ln a) table                 abcmb/background.py:252-347
+//   x_e, T_m, tau_c lookups         abcmb/background.py:568-680 (incl. the
+//                                   fast_interp span quirk, :599-601, :653-655)
+//   reionization                    abcmb/background.py:939-1017
+//   kappa, exp(-kappa), visibility  abcmb/background.py:682-768
+//   lna_rec, lna_transfer_start     abcmb/background.py:555-564
+//   start time per k                abcmb/perturbations.py:119-159, :279
+// The recombination history is NOT HyRex (abcmb/hyrex/*): it is a Saha +
+// Peebles three-level-atom stand-in on HyRex's own ln a grid
+// (uniform 5e-4 from -ln 8001), because the reference cannot run here.
+#include <cmath>
+#include <cstring>
+#include <vector>
+#include <algorithm>
+
+#include "../../include/abcmb.h"
+
+namespace {
+
+constexpr double C_CM = 29979245800.0;
+constexpr double C_MPC = 9.71561e-15;
+constexpr double H0_OVER_H = 3.24078e-18;
+constexpr double HBAR = 6.582119569509e-16;
+constexpr double GN = 1.18980205e-40;
+constexpr double KB = 8.617343e-5;
+constexpr double MP = 938271999.0;
+constexpr double ME = 510998.9461;
+constexpr double MH = MP + ME + 13.5982860719383;
+constexpr double MHE = 3.72839e9;
+constexpr double MU_E = MP * ME / MH;
+constexpr double E21 = 10.198714553953742;
+constexpr double RYD = 13.598286071938324;
+constexpr double SIGMA_T = 6.652458734e-25;
+constexpr double R2S1S = 8.2206;
+const double PI = 3.14159265358979323846;
+const double STEF = PI * PI / (60.0 * HBAR * HBAR * HBAR * C_CM * C_CM);
+const double LYA_FREQ = RYD * 0.75 / (2.0 * PI * HBAR);
+const double RHO_UNIT = 3.0 * H0_OVER_H * H0_OVER_H / 8.0 / PI / GN;
+const double HC3 = (C_CM * HBAR) * (C_CM * HBAR) * (C_CM * HBAR);
+
+const double Q5[5] = {0.583165, 2.0, 4.0, 7.26582, 13.0};
+const double W5[5] = {0.0081201, 0.689407, 2.8063, 2.05156, 0.12681};
+
+struct Rho {
+  double de, cdm, b, g, ur, ncdm, p_ncdm;
+};
+
+inline Rho rho_all(const AbcmbCosmo& c, double lna) {
+  Rho r;
+  double a = std::exp(lna);
+  double a3 = a * a * a, a4 = a3 * a;
+  r.de = c.omega_Lambda * RHO_UNIT;
+  r.cdm = c.omega_cdm * RHO_UNIT / a3;
+  r.b = c.omega_b * RHO_UNIT / a3;
+  double t04 = c.TCMB0 * c.TCMB0 * c.TCMB0 * c.TCMB0;
+  r.g = PI * PI / 15.0 * t04 / a4 / HC3;
+  double tu4 = c.T_ur * c.T_ur * c.T_ur * c.T_ur;
+  r.ur = c.N_ur * 2.0 * 7.0 / 8.0 * PI * PI / 30.0 * tu4 * t04 / a4 / HC3;
+  r.ncdm = 0.0;
+  r.p_ncdm = 0.0;
+  if (c.has_ncdm) {
+    double T = c.T_ncdm * c.TCMB0 / a;
+    double x = c.m_ncdm / T;
+    double ir = 0.0, ip = 0.0;
+    for (int i = 0; i < 5; ++i) {
+      double q = Q5[i];
+      double eps = std::sqrt(q * q + x * x);
+      ir += W5[i] * (1.0 + std::exp(-q)) / (q * q) * eps;
+      ip += W5[i] * (1.0 + std::exp(-q)) / eps;
+    }
+    double T4 = T * T * T * T;
+    r.ncdm = c.N_ncdm * 4.0 * T4 / (PI * PI) * ir / (HBAR * HBAR * HBAR) / (C_CM * C_CM * C_CM);
+    r.p_ncdm = c.N_ncdm * 4.0 / 3.0 * T4 / (PI * PI) * ip / (HBAR * HBAR * HBAR) / (C_CM * C_CM * C_CM);
+  }
+  return r;
+}
+
+inline double rho_tot(const Rho& r) { return r.de + r.cdm + r.b + r.g + r.ur + r.ncdm; }
+inline double p_tot(const Rho& r) { return -r.de + r.g / 3.0 + r.ur / 3.0 + r.p_ncdm; }
+
+inline double hubble(const AbcmbCosmo& c, double lna) {
+  return std::sqrt(8.0 * PI * GN * rho_tot(rho_all(c, lna)) / 3.0);
+}
+inline double aH(const AbcmbCosmo& c, double lna) { return std::exp(lna) * hubble(c, lna) / C_MPC; }
+inline double aH_prime(const AbcmbCosmo& c, double lna) {
+  Rho r = rho_all(c, lna);
+  double a = std::exp(lna);
+  double ah = a * std::sqrt(8.0 * PI * GN * rho_tot(r) / 3.0) / C_MPC;
+  return -4.0 * PI * GN * a * a / 3.0 / ah * (rho_tot(r) + 3.0 * p_tot(r)) / (C_MPC * C_MPC);
+}
+inline double nH(const AbcmbCosmo& c, double lna) {
+  double a = std::exp(lna);
+  return (1.0 - c.YHe) * 3.0 * c.omega_b * H0_OVER_H * H0_OVER_H / 8.0 / PI / GN / MH / (a * a * a);
+}
+
+// fast_interp of abcmb/ABCMBTools.py:268-308 (index clip to [eps, n-1-eps]).
+inline double fast_interp(double x, double lo, double hi, const double* f, int n) {
+  double i = (x - lo) / (hi - lo) * (n - 1);
+  const double eps = 1e-6;
+  if (i < eps) i = eps;
+  if (i > n - 1.0 - eps) i = n - 1.0 - eps;
+  int il = (int)std::floor(i);
+  int iu = std::min(il + 1, n - 1);
+  double wu = i - il;
+  return (1.0 - wu) * f[il] + wu * f[iu];
+}
+// slope d/dx of the same interpolant (0 where the index is clipped)
+inline double fast_interp_slope(double x, double lo, double hi, const double* f, int n) {
+  double i = (x - lo) / (hi - lo) * (n - 1);
+  const double eps = 1e-6;
+  if (i < eps || i > n - 1.0 - eps) return 0.0;
+  int il = (int)std::floor(i);
+  int iu = std::min(il + 1, n - 1);
+  return (f[iu] - f[il]) * (n - 1) / (hi - lo);
+}
+
+inline double tau_of(const AbcmbCosmo& c, double lna) {
+  return fast_interp(lna, ABCMB_TAU_TAB_LO, 0.0, c.tau_tab, ABCMB_TAU_TAB_N);
+}
+inline double xe_of(const AbcmbCosmo& c, double lna) {
+  if (lna < c.xe_x0) return c.xe_tab[0];
+  if (lna >= c.xe_lna_last) return c.xe_last;
+  return fast_interp(lna, c.xe_x0, c.xe_x0 + c.xe_n * c.xe_dx, c.xe_tab, c.xe_n);
+}
+inline double dxe_dlna(const AbcmbCosmo& c, double lna) {
+  if (lna < c.xe_x0 || lna >= c.xe_lna_last) return 0.0;
+  return fast_interp_slope(lna, c.xe_x0, c.xe_x0 + c.xe_n * c.xe_dx, c.xe_tab, c.xe_n);
+}
+inline double tau_c_of(const AbcmbCosmo& c, double lna) {
+  double a = std::exp(lna);
+  double ne = nH(c, lna) * xe_of(c, lna);
+  return 1.0 / a / ne / SIGMA_T / C_CM * C_MPC;
+}
+inline double gamma_compton(double xe, double T, double YHe) {
+  double fhe = YHe * MH / MHE / (1.0 - YHe);
+  return xe / (1.0 + fhe + xe) * 8.0 * SIGMA_T * (4.0 * STEF) * T * T * T * T / (3.0 * ME);
+}
+inline double tm_of(const AbcmbCosmo& c, double lna) {
+  if (lna < c.tm_x0) {
+    double T = c.TCMB0 / std::exp(lna);
+    double xe = xe_of(c, lna);
+    return T * (1.0 - hubble(c, lna) / gamma_compton(xe, T, c.YHe));
+  }
+  if (lna >= c.tm_lna_last) return c.tm_last;
+  return fast_interp(lna, c.tm_x0, c.tm_x0 + c.tm_n * c.tm_dx, c.tm_tab, c.tm_n);
+}
+inline double cs2_of(const AbcmbCosmo& c, double lna) {
+  Rho r = rho_all(c, lna);
+  double tm = tm_of(c, lna);
+  double tg = c.TCMB0 / std::exp(lna);
+  double xe = xe_of(c, lna);
+  double mu = MH / ((1.0 + xe) * (1.0 - c.YHe) + MH / MHE * c.YHe);
+  double R = 4.0 * r.g / 3.0 / r.b;
+  return tm / mu * (5.0 / 3.0 - 2.0 / 3.0 * mu * R / ME / aH(c, lna) / tau_c_of(c, lna) * (tg / tm - 1.0));
+}
+
+// 8-point Gauss-Legendre on [-1, 1]
+const double GLX[8] = {-0.9602898564975363, -0.7966664774136267, -0.5255324099163290, -0.1834346424956498,
+                       0.1834346424956498, 0.5255324099163290, 0.7966664774136267, 0.9602898564975363};
+const double GLW[8] = {0.1012285362903763, 0.2223810344533745, 0.3137066458778873, 0.3626837833783620,
+                       0.3626837833783620, 0.3137066458778873, 0.2223810344533745, 0.1012285362903763};
+
+template <class F>
+double gl8(F f, double a, double b) {
+  double m = 0.5 * (a + b), hw = 0.5 * (b - a), s = 0.0;
+  for (int i = 0; i < 8; ++i) s += GLW[i] * f(m + hw * GLX[i]);
+  return s * hw;
+}
+
+// ---- conformal time table (abcmb/background.py:252-319) ----
+void tau_table(AbcmbCosmo& c, double* tab) {
+  const double lna_cut = -16.1;
+  auto approx = [&](double l) { return std::exp(l) / (H0_OVER_H / C_MPC) / std::sqrt(c.omega_r); };
+  auto inv_aH = [&](double l) { return 1.0 / aH(c, l); };
+  const int n = ABCMB_TAU_TAB_N;
+  double prev_l = lna_cut, prev_tau = approx(lna_cut);
+  for (int i = 0; i < n; ++i) {
+    double l = ABCMB_TAU_TAB_LO + (0.0 - ABCMB_TAU_TAB_LO) * i / (n - 1);
+    if (i == n - 1) l = 0.0;
+    if (l <= lna_cut) {
+      tab[i] = approx(l);
+      continue;
+    }
+    // integrate from prev_l to l in sub-intervals of at most 2e-3
+    int nsub = std::max(1, (int)std::ceil((l - prev_l) / 2e-3));
+    double h = (l - prev_l) / nsub, t = prev_tau;
+    for (int s = 0; s < nsub; ++s) t += gl8(inv_aH, prev_l + s * h, prev_l + (s + 1) * h);
+    tab[i] = t;
+    prev_l = l;
+    prev_tau = t;
+  }
+}
+
+// ---- recombination stand-in: Saha + Peebles TLA on HyRex's grid ----
+const double REC_DLNA = 5.0e-4;
+const double REC_Z0 = 8000.0;
+int rec_points() {
+  // jnp.arange(-ln(1+z0), 0, 5e-4) (abcmb/hyrex/hyrex.py:139)
+  double lo = -std::log(1.0 + REC_Z0);
+  return (int)std::ceil((0.0 - lo) / REC_DLNA);
+}
+
+// Saha ionization: given T (eV), nH (cm^-3), returns x_e including helium.
+double saha_xe(double T, double nh, double fhe, double* xp_out) {
+  double lam = std::pow(2.0 * PI * MU_E * T, 1.5) / std::pow(2.0 * PI * HBAR * C_CM, 3.0);
+  double lame = std::pow(2.0 * PI * ME * T, 1.5) / std::pow(2.0 * PI * HBAR * C_CM, 3.0);
+  double sH = lam * std::exp(-RYD / T);
+  double sHe1 = 4.0 * lame * std::exp(-24.587387 / T);
+  double sHe2 = lame * std::exp(-54.417760 / T);
+  double lo = 0.0, hi = 1.0 + 2.0 * fhe;
+  double xp = 1.0;
+  for (int it = 0; it < 200; ++it) {
+    double xe = 0.5 * (lo + hi);
+    double ne = xe * nh;
+    xp = sH / (ne + sH);
+    double r1 = sHe1 / ne, r2 = sHe2 / ne;   // HeII/HeI, HeIII/HeII
+    double y0 = 1.0 / (1.0 + r1 + r1 * r2);
+    double yII = r1 * y0, yIII = r1 * r2 * y0;
+    double f = xp + fhe * (yII + 2.0 * yIII) - xe;
+    if (f > 0) lo = xe; else hi = xe;
+  }
+  if (xp_out) *xp_out = xp;
+  return 0.5 * (lo + hi);
+}
+
+double alpha_b(double T) {  // case-B recombination coefficient with RECFAST fudge 1.14
+  double t4 = T / KB * 1.0e-4;
+  return 1.14 * 4.309e-13 * std::pow(t4, -0.6166) / (1.0 + 0.6703 * std::pow(t4, 0.5300));
+}
+double beta_b(double T) {
+  double lam3 = std::pow(2.0 * PI * MU_E * T, 1.5) / std::pow(2.0 * PI * HBAR * C_CM, 3.0);
+  return alpha_b(T) * lam3 * std::exp(-RYD / 4.0 / T);
+}
+
+struct RecOut { std::vector<double> xe, tm; };
+
+void recombination(const AbcmbCosmo& c, int N, double* xe, double* tm) {
+  double lo = -std::log(1.0 + REC_Z0);
+  double fhe = c.YHe * MH / MHE / (1.0 - c.YHe);
+  bool tla = false;
+  double xp = 1.0, T_m = 0.0;
+  for (int j = 0; j < N; ++j) {
+    double lna = lo + j * REC_DLNA;
+    double a = std::exp(lna);
+    double Tg = c.TCMB0 / a;
+    double nh = nH(c, lna);
+    double H = hubble(c, lna);
+    double xp_saha;
+    double xe_saha = saha_xe(Tg, nh, fhe, &xp_saha);
+    if (!tla) {
+      xe[j] = xe_saha;
+      xp = xp_saha;
+      double gc = gamma_compton(xe_saha, Tg, c.YHe);
+      T_m = Tg * (1.0 - H / gc);
+      tm[j] = T_m;
+      if (xp_saha < 0.99) tla = true;
+      continue;
+    }
+    // advance x_p and T_m from lna - dlna to lna with backward-Euler substeps
+    const int NS = 8;
+    double h = REC_DLNA / NS;
+    double lprev = lna - REC_DLNA;
+    for (int s = 1; s <= NS; ++s) {
+      double l = lprev + s * h;
+      double al = std::exp(l);
+      double tg = c.TCMB0 / al;
+      double n1 = nH(c, l);
+      double Hs = hubble(c, l);
+      // helium from Saha given the current electron density (lagged)
+      double ne_guess = (xp + 0.0) * n1;
+      double lame = std::pow(2.0 * PI * ME * tg, 1.5) / std::pow(2.0 * PI * HBAR * C_CM, 3.0);
+      double r1 = 4.0 * lame * std::exp(-24.587387 / tg) / std::max(ne_guess, 1e-300);
+      double xhe = fhe * r1 / (1.0 + r1);
+      // Newton on F(x) = x - xp_old - h * dxdlna(x)
+      double x = xp, xold = xp;
+      for (int it = 0; it < 30; ++it) {
+        auto rhs = [&](double xx, double* dr) {
+          double xe_ = xx + xhe;
+          double K = std::pow(C_CM / LYA_FREQ, 3) / (8.0 * PI * Hs);
+          double x1s = std::max(1.0 - xx, 1e-30);
+          double lam_a = 1.0 / (K * n1 * x1s);
+          double bb = beta_b(tg);
+          double Cp = (R2S1S + lam_a) / (R2S1S + lam_a + bb);
+          double ab = alpha_b(T_m);
+          double f = -Cp * (ab * n1 * xe_ * xx - bb * (1.0 - xx) * std::exp(-E21 / tg)) / Hs;
+          // derivative wrt x, C treated as constant
+          *dr = -Cp * (ab * n1 * (xe_ + xx) + bb * std::exp(-E21 / tg)) / Hs;
+          return f;
+        };
+        double d;
+        double f = rhs(x, &d);
+        double F = x - xold - h * f;
+        double dF = 1.0 - h * d;
+        double dx = -F / dF;
+        x += dx;
+        if (x < 0) x = 1e-12;
+        if (std::fabs(dx) < 1e-14 * std::max(x, 1e-10)) break;
+      }
+      xp = x;
+      double xe_ = xp + xhe;
+      double gc = gamma_compton(xe_, tg, c.YHe) / Hs;
+      // backward Euler for dT/dlna = -2T + gc (Tg - T)
+      T_m = (T_m + h * gc * tg) / (1.0 + h * (2.0 + gc));
+    }
+    double lame = std::pow(2.0 * PI * ME * Tg, 1.5) / std::pow(2.0 * PI * HBAR * C_CM, 3.0);
+    double r1 = 4.0 * lame * std::exp(-24.587387 / Tg) / std::max(xp * nh, 1e-300);
+    xe[j] = xp + fhe * r1 / (1.0 + r1);
+    tm[j] = T_m;
+  }
+}
+
+// ---- reionization (abcmb/background.py:939-1017) ----
+double xe_reion(double lna, double z_re, double dz, double zHe, double dzHe, double ex, double YHe) {
+  double fHe = YHe / 4.0 / (1.0 - YHe);
+  double z = 1.0 / std::exp(lna) - 1.0;
+  double y = std::pow(1.0 + z, ex);
+  double yre = std::pow(1.0 + z_re, ex);
+  double dy = ex * std::pow(1.0 + z_re, ex - 1.0) * dz;
+  double xH = (1.0 + fHe) / 2.0 * (1.0 + std::tanh((yre - y) / dy));
+  double xHe = fHe / 2.0 * (1.0 + std::tanh((zHe - z) / dzHe));
+  return xH + xHe;
+}
+double tau_reion_of(const AbcmbCosmo& c, double z_re, double dz, double zHe, double dzHe, double ex) {
+  const int n = 2000;
+  double s = 0.0, prev = 0.0;
+  for (int i = 0; i < n; ++i) {
+    double l = -5.0 + 5.0 * i / (n - 1);
+    double ne = nH(c, l) * xe_reion(l, z_re, dz, zHe, dzHe, ex, c.YHe);
+    double gam = std::exp(l) * ne * SIGMA_T * C_CM / C_MPC;
+    double f = gam / aH(c, l);
+    if (i > 0) s += 0.5 * (f + prev) * (5.0 / (n - 1));
+    prev = f;
+  }
+  return s;
+}
+
+// kappa(lna) = int_lna^0 dl / (tau_c aH): composite GL on a node grid
+struct Kappa {
+  std::vector<double> nodes, cum;  // cum[i] = kappa(nodes[i])
+  const AbcmbCosmo* c;
+  double integrand(double l) const { return 1.0 / tau_c_of(*c, l) / aH(*c, l); }
+  void build(const AbcmbCosmo& cc) {
+    c = &cc;
+    const double lo = -10.0;
+    const int n = 40001;  // 2.5e-4 spacing
+    nodes.resize(n);
+    cum.assign(n, 0.0);
+    for (int i = 0; i < n; ++i) nodes[i] = lo + (0.0 - lo) * i / (n - 1);
+    nodes[n - 1] = 0.0;
+    for (int i = n - 2; i >= 0; --i)
+      cum[i] = cum[i + 1] + gl8([&](double l) { return integrand(l); }, nodes[i], nodes[i + 1]);
+  }
+  double operator()(double l) const {
+    const int n = (int)nodes.size();
+    if (l <= nodes[0]) return cum[0];
+    if (l >= 0.0) return 0.0;
+    double fi = (l - nodes[0]) / (nodes[1] - nodes[0]);
+    int i = std::min((int)fi, n - 2);
+    return cum[i + 1] + gl8([&](double x) { return integrand(x); }, l, nodes[i + 1]);
+  }
+};
+
+double expmkappa(const Kappa& K, double l) { return l < -10.0 ? 0.0 : std::exp(-K(l)); }
+
+thread_local Kappa* g_kappa_cache = nullptr;
+
+}  // namespace
+
+extern "C" int abcmb_host_recomb_points(void) { return rec_points(); }
+
+extern "C" int abcmb_host_background(AbcmbCosmo* c, double tau_reion, double z_reion_in,
+                                     double Delta_z_reion, double z_reion_He,
+                                     double Delta_z_reion_He, double exp_reion) {
+  if (!c || !c->tau_tab || !c->xe_tab || !c->tm_tab) return ABCMB_EINVAL;
+  int N = rec_points();
+  if (c->xe_n < N || c->tm_n != c->xe_n) return ABCMB_EINVAL;
+  double* tau = const_cast<double*>(c->tau_tab);
+  double* xe = const_cast<double*>(c->xe_tab);
+  double* tm = const_cast<double*>(c->tm_tab);
+  tau_table(*c, tau);
+  c->tau0 = fast_interp(0.0, ABCMB_TAU_TAB_LO, 0.0, c->tau_tab, ABCMB_TAU_TAB_N);
+  recombination(*c, N, xe, tm);
+  double x0 = -std::log(1.0 + REC_Z0);
+  c->xe_x0 = c->tm_x0 = x0;
+  c->xe_dx = c->tm_dx = REC_DLNA;
+  c->xe_lna_last = c->tm_lna_last = x0 + (N - 1) * REC_DLNA;
+  // reionization: solve tau_reion(z) = target (Newton/bisection), or take z
+  double zr = z_reion_in;
+  if (tau_reion > 0.0) {
+    double lo = 0.0, hi = 50.0;
+    for (int it = 0; it < 200; ++it) {
+      double mid = 0.5 * (lo + hi);
+      double t = tau_reion_of(*c, mid, Delta_z_reion, z_reion_He, Delta_z_reion_He, exp_reion);
+      if (t < tau_reion) lo = mid; else hi = mid;
+      if (hi - lo < 1e-13) break;
+    }
+    zr = 0.5 * (lo + hi);
+  }
+  c->z_reion = zr;
+  for (int j = 0; j < N; ++j) {
+    double l = x0 + j * REC_DLNA;
+    xe[j] += xe_reion(l, zr, Delta_z_reion, z_reion_He, Delta_z_reion_He, exp_reion, c->YHe);
+  }
+  for (int j = N; j < c->xe_n; ++j) { xe[j] = xe[N - 1]; tm[j] = tm[N - 1]; }
+  c->xe_last = xe[N - 1];
+  c->tm_last = tm[N - 1];
+  // lna_rec: argmax of the visibility on linspace(-8, -4, 1500)
+  Kappa K;
+  K.build(*c);
+  {
+    int best = 0;
+    double bv = -1.0;
+    for (int i = 0; i < 1500; ++i) {
+      double l = -8.0 + 4.0 * i / 1499.0;
+      double g = expmkappa(K, l) / tau_c_of(*c, l);
+      if (g > bv) { bv = g; best = i; }
+    }
+    c->lna_rec = -8.0 + 4.0 * best / 1499.0;
+  }
+  {
+    int best = 0;
+    double bv = 1e300;
+    for (int i = 0; i < 5000; ++i) {
+      double l = -15.0 + 9.0 * i / 4999.0;
+      double v = aH(*c, l) * tau_c_of(*c, l) - 0.008;
+      v = v * v;
+      if (v < bv) { bv = v; best = i; }
+    }
+    c->lna_transfer_start = -15.0 + 9.0 * best / 4999.0;
+  }
+  return ABCMB_OK;
+}
+
+extern "C" int abcmb_host_los_background(const AbcmbCosmo* c, const double* lna, int n, double* out) {
+  if (!c || !lna || !out || n <= 0) return ABCMB_EINVAL;
+  Kappa K;
+  K.build(*c);
+  for (int i = 0; i < n; ++i) {
+    double l = lna[i];
+    double tc = tau_c_of(*c, l);
+    double ah = aH(*c, l);
+    double em = expmkappa(K, l);
+    double g = em / tc;
+    // d g / d lna: exp(-kappa) gains 1/(tau_c aH); tau_c ~ a^2 / x_e
+    double xe = xe_of(*c, l);
+    double dlntc = 2.0 - dxe_dlna(*c, l) / xe;
+    double gp = (l < -10.0) ? 0.0 : g * (1.0 / (tc * ah) - dlntc);
+    out[0 * n + i] = tau_of(*c, l);
+    out[1 * n + i] = g;
+    out[2 * n + i] = gp;
+    out[3 * n + i] = ah;
+    out[4 * n + i] = em;
+    out[5 * n + i] = aH_prime(*c, l) * ah;
+  }
+  return ABCMB_OK;
+}
+
+extern "C" int abcmb_host_lna_start(const AbcmbCosmo* c, const double* k, int nk, double R_tc,
+                                    double R_large, double* out) {
+  if (!c || !k || !out) return ABCMB_EINVAL;
+  const int n = 10000;
+  std::vector<double> grid(n), f1(n), ah(n);
+  for (int i = 0; i < n; ++i) {
+    grid[i] = -20.0 + 10.0 * i / (n - 1);
+    ah[i] = aH(*c, grid[i]);
+    f1[i] = tau_c_of(*c, grid[i]) * ah[i];
+  }
+  // numpy/jax interp: clamp outside, linear inside (xp ascending)
+  auto interp = [&](double x, const std::vector<double>& xp) {
+    if (x <= xp[0]) return grid[0];
+    if (x >= xp[n - 1]) return grid[n - 1];
+    int i = (int)(std::upper_bound(xp.begin(), xp.end(), x) - xp.begin()) - 1;
+    double t = (x - xp[i]) / (xp[i + 1] - xp[i]);
+    return grid[i] + t * (grid[i + 1] - grid[i]);
+  };
+  double l1 = interp(R_tc, f1);
+  std::vector<double> f2(n);
+  for (int j = 0; j < nk; ++j) {
+    for (int i = 0; i < n; ++i) f2[i] = k[j] / ah[i];
+    double l2 = interp(R_large, f2);
+    double l = std::min(l1, l2);
+    out[j] = std::min(l, -10.0);
+  }
+  return ABCMB_OK;
+}
+
+extern "C" int abcmb_host_eval(const AbcmbCosmo* c, const double* lna, int n, double* out) {
+  if (!c || !lna || !out) return ABCMB_EINVAL;
+  for (int i = 0; i < n; ++i) {
+    double l = lna[i];
+    Rho r = rho_all(*c, l);
+    out[0 * n + i] = aH(*c, l);
+    out[1 * n + i] = tau_of(*c, l);
+    out[2 * n + i] = tau_c_of(*c, l);
+    out[3 * n + i] = cs2_of(*c, l);
+    out[4 * n + i] = xe_of(*c, l);
+    out[5 * n + i] = tm_of(*c, l);
+    out[6 * n + i] = 4.0 * r.g / 3.0 / r.b;
+  }
+  return ABCMB_OK;
+}

@@ -1,0 +1,206 @@
+/*
+ * abcmb.h - C ABI of the MI355X perturbation + line-of-sight library
+ * (libabcmb_hip.so) and of its host-side background producer
+ * (libabcmb_host.so).
+ *
+ * The reference (mshettyFE/ABCMB) is a JAX package with no FFI; its
+ * boundary for this path is a set of Python method calls.  Each entry point
+ * below replaces one of them and cites it (paths relative to the reference
+ * repository root).  Conventions:
+ *   - every pointer named *_dev is device memory owned by the caller;
+ *     host structs are read at call time only;
+ *   - nothing here allocates device memory: scratch comes in through
+ *     `workspace` / `ws_bytes` (query the size first);
+ *   - calls are stream-ordered on `stream` (a hipStream_t passed as void*);
+ *   - return value 0 = success, otherwise an ABCMB_E* code; the message is
+ *     available from abcmb_last_error() (thread-local).
+ */
+#ifndef ABCMB_H
+#define ABCMB_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define ABCMB_OK 0
+#define ABCMB_EINVAL 1      /* bad argument / shape / layout */
+#define ABCMB_EHIP 2        /* HIP runtime error */
+#define ABCMB_EWORKSPACE 3  /* workspace too small */
+
+/* per-k ODE status words (AbcmbPertItem.status output) */
+#define ABCMB_K_OK 0
+#define ABCMB_K_MAX_STEPS 1   /* max_steps reached before ln a = 0 */
+#define ABCMB_K_NONFINITE 2   /* NaN/Inf in state or error norm */
+
+#define ABCMB_TAU_TAB_N 10000      /* tau(ln a) table on [-33, 0] */
+#define ABCMB_TAU_TAB_LO (-33.0)
+#define ABCMB_N_PT_FIELDS 21       /* fields of the perturbation table */
+
+/* Perturbation-table field order inside AbcmbCosmo.pt ([field][n_save][nk]). */
+enum {
+  ABCMB_PT_DELTA_M = 0, ABCMB_PT_THETA_B_PRIME, ABCMB_PT_ETA, ABCMB_PT_H_PRIME,
+  ABCMB_PT_ETA_PRIME, ABCMB_PT_ALPHA, ABCMB_PT_ALPHA_PRIME,
+  ABCMB_PT_CDM_DELTA, ABCMB_PT_B_DELTA, ABCMB_PT_B_THETA,
+  ABCMB_PT_G_DELTA, ABCMB_PT_G_THETA, ABCMB_PT_G_SIGMA, ABCMB_PT_G_G0, ABCMB_PT_G_G2,
+  ABCMB_PT_UR_DELTA, ABCMB_PT_UR_THETA, ABCMB_PT_UR_SIGMA,
+  ABCMB_PT_NCDM_DELTA, ABCMB_PT_NCDM_THETA, ABCMB_PT_NCDM_SIGMA
+};
+
+/*
+ * One cosmology: parameters (input + derived, abcmb/main.py:368-612), the
+ * background tables the hot path reads, and where its perturbation table
+ * goes.  Pointers are device pointers for the HIP entry points and host
+ * pointers for the host producer.
+ */
+typedef struct AbcmbCosmo {
+  double h, omega_b, omega_cdm, omega_Lambda;
+  double A_s, n_s, k_pivot, TCMB0, YHe;
+  double N_ur, T_ur, N_ncdm, T_ncdm, m_ncdm;
+  double R_nu, R_b, om, omega_r, omega_m;
+  double tau0;                                   /* conformal time today, Mpc */
+  double xe_x0, xe_dx, xe_lna_last, xe_last;     /* x_e table metadata */
+  double tm_x0, tm_dx, tm_lna_last, tm_last;     /* T_m table metadata */
+  double lna_rec, lna_transfer_start, z_reion;
+  int32_t has_ncdm;   /* MassiveNeutrino species present */
+  int32_t n_q;        /* momentum bins of its hierarchy (3 or 5) */
+  int32_t xe_n, tm_n; /* allocated table lengths (lookups span all of them) */
+  int32_t n_save, nk; /* saved ln a points, PT k-modes */
+  const double* tau_tab;   /* ABCMB_TAU_TAB_N */
+  const double* xe_tab;    /* xe_n */
+  const double* tm_tab;    /* tm_n */
+  const double* lna_save;  /* n_save, ascending, last = 0 */
+  const double* k;         /* nk, Mpc^-1 */
+  double* pt;              /* [ABCMB_N_PT_FIELDS][n_save][nk] */
+} AbcmbCosmo;
+
+/* One (cosmology, k-mode) ODE work item. */
+typedef struct AbcmbPertItem {
+  double k, lna_start, rtol, atol;
+  int32_t cosmo, ik, pad0, pad1;
+} AbcmbPertItem;
+
+/* Integrator settings shared by a launch (abcmb/model_specs.py:57-68,
+ * abcmb/perturbations.py:285-312). */
+typedef struct AbcmbOdeSettings {
+  double pcoeff, icoeff, dcoeff, dt0;
+  double safety, factormin, factormax, newton_kappa;
+  int32_t max_steps, max_newton;
+  int32_t l_g, l_pol, l_ur, l_ncdm; /* hierarchy cut-offs */
+  int32_t n_q, n;                   /* massive bins (0 if absent), state size */
+} AbcmbOdeSettings;
+
+/* ---------------- host background producer (libabcmb_host.so) ---------- */
+
+/*
+ * Fill the background tables and scalars of `c` from its parameters:
+ * conformal time table (abcmb/background.py:252-319), a recombination
+ * history x_e/T_m (stand-in for HyRex, abcmb/hyrex/hyrex.py:92-207) with the
+ * tanh reionization added (abcmb/background.py:939-1017), lna_rec and
+ * lna_transfer_start (abcmb/background.py:555-564).  c->tau_tab, xe_tab and
+ * tm_tab must point to writable host arrays of ABCMB_TAU_TAB_N, xe_n and
+ * tm_n doubles (xe_n = tm_n >= abcmb_host_recomb_points()).
+ * tau_reion > 0 selects the tau-input reionization branch.
+ */
+int abcmb_host_background(AbcmbCosmo* c, double tau_reion, double z_reion_in,
+                          double Delta_z_reion, double z_reion_He,
+                          double Delta_z_reion_He, double exp_reion);
+int abcmb_host_recomb_points(void);
+
+/*
+ * Background vectors on the LOS grid lna[0..n-1] (abcmb/spectrum.py:640-651):
+ * out[0..5][n] = tau, g, dg/dlna, aH, exp(-kappa), aH' * aH.
+ */
+int abcmb_host_los_background(const AbcmbCosmo* c, const double* lna, int n,
+                              double* out);
+/* Per-k starting ln a (abcmb/perturbations.py:119-159 and :279). */
+int abcmb_host_lna_start(const AbcmbCosmo* c, const double* k, int nk,
+                         double R_tc, double R_large, double* out);
+/* Background at arbitrary ln a: out[0..6][n] = aH, tau, tau_c, cs2, x_e, T_m, R. */
+int abcmb_host_eval(const AbcmbCosmo* c, const double* lna, int n, double* out);
+
+/* ---------------- GPU library (libabcmb_hip.so) ---------------------------- */
+
+const char* abcmb_last_error(void);
+int abcmb_device_init(void);
+
+/*
+ * Perturbation ODE for every work item: adaptive ESDIRK (Kvaerno5) with
+ * PID control and chord Newton in FP64, one k-mode per workgroup, dense
+ * output at the cosmology's n_save points, fused output-table assembly.
+ * Replaces PerturbationEvolver.full_evolution (abcmb/perturbations.py:75-117)
+ * with evolution_one_k (:250-316) and make_output_table (:318-404).
+ * cosmos_dev: n_cosmo device-resident AbcmbCosmo; items_dev: n_items;
+ * status_dev/steps_dev: int32[n_items] outputs (status, accepted+rejected
+ * steps); state_dev: optional [n_items][n_save][n] raw states (NULL = off).
+ */
+size_t abcmb_pert_workspace_size(int n_items);
+int abcmb_pert_fwd(const AbcmbCosmo* cosmos_dev, int n_cosmo,
+                   const AbcmbPertItem* items_dev, int n_items,
+                   const AbcmbOdeSettings* settings,
+                   int32_t* status_dev, int32_t* steps_dev, double* state_dev,
+                   void* workspace, size_t ws_bytes, void* stream);
+
+/*
+ * Cubic-spline interpolation (not-a-knot, in log10 k) of the 10 LOS fields
+ * of a perturbation table from its k grid to the transfer grid, fused with
+ * the source functions S_T0, S_T1, S_T2, S_E and the LOS quadrature weight
+ * (abcmb/spectrum.py:664-696, :755-758).
+ * bgv_dev: [6][n_save-1] from abcmb_host_los_background; kt_dev: [nkt].
+ * src_dev out: [4][n_save-1][nkt].  scales: sw, isw, dop, pol.
+ */
+size_t abcmb_sources_workspace_size(int nk, int n_save);
+int abcmb_sources(const double* pt_dev, const double* k_dev, int nk, int n_save,
+                  const double* bgv_dev, const double* kt_dev, int nkt,
+                  const double* scales4, double* src_dev,
+                  void* workspace, size_t ws_bytes, void* stream);
+
+/*
+ * Line-of-sight transfer Delta_l(k) for each sampled multipole and the C_l
+ * k-quadrature (abcmb/spectrum.py:616-797, Bessel lookups :48-109).
+ * bes_dev: tables [3][n_lt][nx] (phi0, phi1, phi2), bes_rng_dev: [3][n_lt][2]
+ * x ranges, bes_l_dev: int32[n_lt] multipoles; ell_idx_dev: int32[n_ell]
+ * rows of the Bessel tables to use; tau_los_dev: tau at the n_save-1 LOS
+ * points.  precision: 0 = FP64, 1 = FP32.  cl_dev out: [3][n_ell]
+ * (TT, TE, EE); delta_dev (optional) out: [4][n_ell][nkt].
+ */
+int abcmb_los_cl(const double* src_dev, int n_tau, const double* tau_los_dev,
+                 double tau0, const double* kt_dev, int nkt,
+                 const double* bes_dev, const double* bes_rng_dev,
+                 const int32_t* bes_l_dev, int n_lt, int nx,
+                 const int32_t* ell_idx_dev, int n_ell,
+                 double A_s, double n_s, double k_pivot, int precision,
+                 double* cl_dev, double* delta_dev, void* stream);
+
+/* Dense linear map out[m][r] = sum_c op[r][c] in[m][c] (the l-spline of
+ * abcmb/spectrum.py:598-601 applied as a precomputed operator). */
+int abcmb_apply_operator(const double* op_dev, int rows, int cols,
+                         const double* in_dev, int n_vec, double* out_dev,
+                         void* stream);
+
+/* Limber lensing potential C_l^phiphi (abcmb/spectrum.py:349-435). */
+int abcmb_lensing_clpp(const AbcmbCosmo* c_host, const double* pt_dev,
+                       const double* lna_dev, const double* aH_dev,
+                       const double* tau_dev, int n_lna, double chi_rec,
+                       const double* ells_dev, int n_l, double* clpp_dev,
+                       void* stream);
+
+/* Correlation-function lensing of TT/TE/EE (+BB) (abcmb/spectrum.py:437-572,
+ * Wigner-d recursions abcmb/ABCMBTools.py:16-185).  cl_in: [3][n_l] unlensed
+ * on l = 2..n_l+1; out: [4][n_l] lensed TT, TE, EE, BB. */
+size_t abcmb_lens_workspace_size(int n_mu, int n_l);
+int abcmb_lens(const double* cl_in_dev, const double* clpp_dev, int n_l,
+               const double* mu_dev, const double* w_dev, int n_mu,
+               double* out_dev, void* workspace, size_t ws_bytes, void* stream);
+
+/* Linear matter power spectrum at z = 0 (abcmb/spectrum.py:268-302). */
+int abcmb_pk_lin(const double* pt_dev, const double* k_dev, int nk, int n_save,
+                 const double* kq_dev, int nq, double A_s, double n_s,
+                 double k_pivot, double* pk_dev, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* ABCMB_H */

@@ -1,0 +1,147 @@
+"""ctypes wrapper of the CPU oracle -- TEST INFRASTRUCTURE ONLY.
+
+Only ``tests/``, ``__graft_entry__.smoke()`` and the CPU-baseline leg of
+``bench.py`` use this module, and only as the checker.  The functions
+restate the reference's hot path in FP64 C (``oracle/abcmb_oracle.c``, with
+the reference file:line of each piece cited there).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "liboracle.so")
+_lib = None
+_P = C.c_void_p
+_D = C.c_double
+_I = C.c_int
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        from abcmb_amd import abi
+        L = C.CDLL(LIB_PATH)
+        L.oracle_pert.argtypes = [_P, _P, _I, C.POINTER(abi.AbcmbOdeSettings), _P, _P, _P]
+        L.oracle_rhs.argtypes = [C.POINTER(abi.AbcmbCosmo), C.POINTER(abi.AbcmbOdeSettings), _D, _D, _P, _P]
+        L.oracle_ic.argtypes = [C.POINTER(abi.AbcmbCosmo), C.POINTER(abi.AbcmbOdeSettings), _D, _D, _P]
+        L.oracle_bg.argtypes = [C.POINTER(abi.AbcmbCosmo), _D, _P]
+        L.oracle_kvaerno5_tableau.argtypes = [_P, _P, _P, _P]
+        L.oracle_spline.argtypes = [_P, _P, _I, _P, _I, _P]
+        L.oracle_sources.argtypes = [_P, _P, _I, _I, _P, _P, _P, _I, _P, _P]
+        L.oracle_los_cl.argtypes = [_P, _I, _P, _D, _P, _I, _P, _P, _P, _I, _I, _P, _I, _D, _D, _D, _P, _P]
+        L.oracle_pk.argtypes = [_P, _P, _I, _I, _P, _P, _I, _D, _D, _D, _P]
+        L.oracle_clpp.argtypes = [C.POINTER(abi.AbcmbCosmo), _P, _P, _P, _P, _I, _D, _P, _I, _P]
+        L.oracle_wigner.argtypes = [_I, _I, _P, _I, _I, _P]
+        L.oracle_lens.argtypes = [_P, _P, _I, _P, _P, _I, _P]
+        _lib = L
+    return _lib
+
+
+def _p(a):
+    return a.ctypes.data_as(C.c_void_p) if a is not None else None
+
+
+def _c(a, dtype=np.float64):
+    return np.ascontiguousarray(a, dtype=dtype)
+
+
+def kvaerno5_tableau():
+    A = np.zeros(49); b = np.zeros(7); bh = np.zeros(7); c = np.zeros(7)
+    lib().oracle_kvaerno5_tableau(_p(A), _p(b), _p(bh), _p(c))
+    return A.reshape(7, 7), b, bh, c
+
+
+def rhs(cosmo, settings, k, lna, y):
+    y = _c(y); dy = np.zeros_like(y)
+    lib().oracle_rhs(C.byref(cosmo), C.byref(settings), k, lna, _p(y), _p(dy))
+    return dy
+
+
+def ic(cosmo, settings, k, lna):
+    y = np.zeros(settings.n)
+    lib().oracle_ic(C.byref(cosmo), C.byref(settings), k, lna, _p(y))
+    return y
+
+
+def bg(cosmo, lna):
+    out = np.zeros(6)
+    lib().oracle_bg(C.byref(cosmo), lna, _p(out))
+    return dict(zip(("aH", "tau", "tau_c", "cs2", "xe", "Tm"), out))
+
+
+def pert(cosmos_arr, items_arr, settings, want_states=False):
+    """cosmos_arr: ctypes array of AbcmbCosmo (host pointers, pt set); items_arr: ctypes array."""
+    n = len(items_arr)
+    status = np.zeros(n, np.int32)
+    steps = np.zeros(n, np.int32)
+    states = None
+    if want_states:
+        ns = cosmos_arr[0].n_save
+        states = np.zeros((n, ns, settings.n))
+    lib().oracle_pert(C.cast(cosmos_arr, C.c_void_p), C.cast(items_arr, C.c_void_p), n,
+                      C.byref(settings), _p(status), _p(steps), _p(states))
+    return status, steps, states
+
+
+def spline(x, y, xq):
+    x = _c(x); y = _c(y); xq = _c(xq); out = np.zeros(xq.size)
+    lib().oracle_spline(_p(x), _p(y), x.size, _p(xq), xq.size, _p(out))
+    return out
+
+
+def sources(pt, k, n_save, bgv, lna_save, kt, scales=(1, 1, 1, 1)):
+    pt = _c(pt); k = _c(k); bgv = _c(bgv); lna_save = _c(lna_save); kt = _c(kt)
+    sc = _c(scales)
+    out = np.zeros((4, n_save - 1, kt.size))
+    lib().oracle_sources(_p(pt), _p(k), k.size, n_save, _p(bgv), _p(lna_save), _p(kt), kt.size, _p(sc), _p(out))
+    return out
+
+
+def los_cl(src, tau_los, tau0, kt, bes, rng, bes_l, ell_idx, A_s, n_s, k_pivot, want_delta=False):
+    src = _c(src); tau_los = _c(tau_los); kt = _c(kt); bes = _c(bes); rng = _c(rng)
+    bes_l = _c(bes_l, np.int32); ell_idx = _c(ell_idx, np.int32)
+    n_ell = ell_idx.size
+    cl = np.zeros((3, n_ell))
+    delta = np.zeros((4, n_ell, kt.size)) if want_delta else None
+    lib().oracle_los_cl(_p(src), src.shape[1], _p(tau_los), tau0, _p(kt), kt.size, _p(bes), _p(rng), _p(bes_l),
+                        bes.shape[1], bes.shape[2], _p(ell_idx), n_ell, A_s, n_s, k_pivot, _p(cl), _p(delta))
+    return (cl, delta) if want_delta else cl
+
+
+def pk(pt, k, n_save, lna_save, kq, A_s, n_s, k_pivot):
+    pt = _c(pt); k = _c(k); lna_save = _c(lna_save); kq = _c(kq); out = np.zeros(kq.size)
+    lib().oracle_pk(_p(pt), _p(k), k.size, n_save, _p(lna_save), _p(kq), kq.size, A_s, n_s, k_pivot, _p(out))
+    return out
+
+
+def clpp(cosmo, pt, lna_ax, aH_ax, tau_ax, chi_rec, ells):
+    pt = _c(pt); lna_ax = _c(lna_ax); aH_ax = _c(aH_ax); tau_ax = _c(tau_ax); ells = _c(ells)
+    out = np.zeros(ells.size)
+    lib().oracle_clpp(C.byref(cosmo), _p(pt), _p(lna_ax), _p(aH_ax), _p(tau_ax), lna_ax.size, chi_rec,
+                      _p(ells), ells.size, _p(out))
+    return out
+
+
+def wigner(m, n, mu, n_l):
+    mu = _c(mu); out = np.zeros((mu.size, n_l))
+    lib().oracle_wigner(m, n, _p(mu), mu.size, n_l, _p(out))
+    return out
+
+
+def lens(cl_in, clpp_, mu, w):
+    cl_in = _c(cl_in); clpp_ = _c(clpp_); mu = _c(mu); w = _c(w)
+    n_l = clpp_.size
+    out = np.zeros((4, n_l))
+    lib().oracle_lens(_p(cl_in), _p(clpp_), n_l, _p(mu), _p(w), mu.size, _p(out))
+    return out
