@@ -86,7 +86,7 @@ def host_lib():
 
 HIP_SYMBOLS = (
     "abcmb_last_error", "abcmb_device_init", "abcmb_pert_workspace_size", "abcmb_pert_fwd",
-    "abcmb_sources_workspace_size", "abcmb_sources", "abcmb_los_cl", "abcmb_apply_operator",
+    "abcmb_sources_workspace_size", "abcmb_sources", "abcmb_los_workspace_size", "abcmb_los_cl", "abcmb_apply_operator",
     "abcmb_lensing_clpp", "abcmb_lens_workspace_size", "abcmb_lens", "abcmb_pk_lin",
 )
 
@@ -107,17 +107,20 @@ def hip_lib():
                                        _P, _P, _P, _P, C.c_size_t, _P]
         lib.abcmb_sources_workspace_size.restype = C.c_size_t
         lib.abcmb_sources_workspace_size.argtypes = [C.c_int, C.c_int]
-        lib.abcmb_sources.argtypes = [_P, _P, C.c_int, C.c_int, _P, _P, C.c_int, _P, _P,
+        lib.abcmb_sources.argtypes = [_P, _P, C.c_int, C.c_int, _P, _D, _P, C.c_int, _P, _P,
                                       _P, C.c_size_t, _P]
+        lib.abcmb_los_workspace_size.restype = C.c_size_t
+        lib.abcmb_los_workspace_size.argtypes = [C.c_int, C.c_int]
         lib.abcmb_los_cl.argtypes = [_P, C.c_int, _P, _D, _P, C.c_int, _P, _P, _P, C.c_int,
-                                     C.c_int, _P, C.c_int, _D, _D, _D, C.c_int, _P, _P, _P]
+                                     C.c_int, _P, C.c_int, _D, _D, _D, C.c_int, _P, _P, _P,
+                                     C.c_size_t, _P]
         lib.abcmb_apply_operator.argtypes = [_P, C.c_int, C.c_int, _P, C.c_int, _P, _P]
         lib.abcmb_lensing_clpp.argtypes = [C.POINTER(AbcmbCosmo), _P, _P, _P, _P, C.c_int, _D,
                                            _P, C.c_int, _P, _P]
         lib.abcmb_lens_workspace_size.restype = C.c_size_t
         lib.abcmb_lens_workspace_size.argtypes = [C.c_int, C.c_int]
         lib.abcmb_lens.argtypes = [_P, _P, C.c_int, _P, _P, C.c_int, _P, _P, C.c_size_t, _P]
-        lib.abcmb_pk_lin.argtypes = [_P, _P, C.c_int, C.c_int, _P, C.c_int, _D, _D, _D, _P, _P]
+        lib.abcmb_pk_lin.argtypes = [_P, _P, C.c_int, C.c_int, _P, _P, C.c_int, _D, _D, _D, _P, _P]
         _hip = lib
     return _hip
 

@@ -1,0 +1,380 @@
+// CMB lensing on the GPU.
+//
+//   abcmb_lensing_clpp : Limber C_l^phiphi (abcmb/spectrum.py:349-435), one
+//                        thread per multipole, trapezoid over 500 ln a.
+//   abcmb_lens         : correlation-function lensing of TT/TE/EE plus BB
+//                        (abcmb/spectrum.py:437-572).  The reduced Wigner
+//                        d^l_mn(mu) are generated by the three-term recursion in
+//                        l (abcmb/ABCMBTools.py:16-185) in registers, one
+//                        Gauss-Legendre node per thread, never stored:
+//                          pass 1: Cgl(mu), Cgl2(mu)  (d11, d1-1)
+//                          pass 2: xi, xi+, xi-, xi_x (12 d's, X-functions)
+//                          pass 3: back-transform sum_mu w xi d (d00, d20,
+//                                  d22, d2-2), reduced per block in l order
+//                                  (deterministic), then across blocks.
+#include <cmath>
+#include <cstdint>
+
+#include "common.h"
+
+namespace abcmb {
+namespace {
+
+using namespace cst;
+
+// ---- Limber lensing potential ----
+__device__ double interp_col(double x, const double* __restrict__ xp, const double* __restrict__ fp, int n,
+                             int stride) {
+  if (x <= xp[0]) return fp[0];
+  if (x >= xp[n - 1]) return fp[(size_t)(n - 1) * stride];
+  int a = 0, b = n - 1;
+  while (b - a > 1) { int mid = (a + b) >> 1; if (xp[mid] <= x) a = mid; else b = mid; }
+  double t = (x - xp[a]) / (xp[a + 1] - xp[a]);
+  return fp[(size_t)a * stride] + t * (fp[(size_t)(a + 1) * stride] - fp[(size_t)a * stride]);
+}
+
+struct LimberArgs {
+  double tau0, Om, OL, A_s, n_s, kp, chi_rec;
+  int n_save, nk;
+  const double* lna_save;
+  const double* k;
+};
+
+__global__ void clpp_kernel(LimberArgs A, const double* __restrict__ pt, const double* __restrict__ lna_ax,
+                            const double* __restrict__ aH_ax, const double* __restrict__ tau_ax, int n_lna,
+                            const double* __restrict__ ells, int n_l, double* __restrict__ out) {
+  int il = blockIdx.x * blockDim.x + threadIdx.x;
+  if (il >= n_l) return;
+  const double* dm = pt + (size_t)ABCMB_PT_DELTA_M * A.n_save * A.nk;
+  double ell = ells[il];
+  double lna_floor = lna_ax[n_lna - 2];
+  double s = 0.0, prev = 0.0;
+  for (int i = 0; i < n_lna; ++i) {
+    double lna = lna_ax[i];
+    double v = 0.0;
+    if (lna < 0.0) {
+      double chi = A.tau0 - tau_ax[i];
+      double kk = (ell + 0.5) / chi;
+      double win = (A.chi_rec - chi) / A.chi_rec / chi;
+      double a = exp(lna);
+      double z = 1. / a - 1.;
+      double aH = aH_ax[i];
+      double z3 = (1. + z) * (1. + z) * (1. + z);
+      double omz = (A.Om * z3) / ((A.Om * z3) + A.OL);
+      double lq = -log(1. + z);
+      int ka, kb;
+      double t;
+      if (kk <= A.k[0]) { ka = kb = 0; t = 0.0; }
+      else if (kk >= A.k[A.nk - 1]) { ka = kb = A.nk - 1; t = 0.0; }
+      else {
+        ka = 0; kb = A.nk - 1;
+        while (kb - ka > 1) { int mid = (ka + kb) >> 1; if (A.k[mid] <= kk) ka = mid; else kb = mid; }
+        t = (kk - A.k[ka]) / (A.k[kb] - A.k[ka]);
+      }
+      double da = interp_col(lq, A.lna_save, dm + ka, A.n_save, A.nk);
+      double db = interp_col(lq, A.lna_save, dm + kb, A.n_save, A.nk);
+      double d = da + t * (db - da);
+      double pk = d * d * (A.A_s * pow(kk / A.kp, A.n_s - 1.) * (2 * PI * PI / (kk * kk * kk)));
+      double pl = 9. / 8. / (PI * PI) * omz * omz * (aH * aH * aH * aH) * pk / kk;
+      v = chi / aH * win * win * pl;
+    }
+    (void)lna_floor;
+    if (i > 0) s += 0.5 * (prev + v) * (lna_ax[i] - lna_ax[i - 1]);
+    prev = v;
+  }
+  double e5 = ell + 0.5;
+  out[il] = 8. * PI * PI / (e5 * e5 * e5) * s;
+}
+
+// ---- Wigner d recursion tables ----
+// pairs: 0 d00, 1 d11, 2 d1-1, 3 d2-2, 4 d22, 5 d31, 6 d40, 7 d3-3, 8 d4-4, 9 d20, 10 d3-1, 11 d4-2
+__constant__ int PAIR_M[12] = {0, 1, 1, 2, 2, 3, 4, 3, 4, 2, 3, 4};
+__constant__ int PAIR_N[12] = {0, 1, -1, -2, 2, 1, 0, -3, -4, 0, -1, -2};
+constexpr int NPAIR = 12;
+
+// coef[q][l][3] = (A, B, C) of d^{l+1} = (A mu + B) d^l + C d^{l-1} in the
+// normalised recursion; lfac[l][5] = sqrt(2/(2l+1)) and the four X-function
+// square roots.  l = 0 .. LT-1 with LT = n_l + 3.
+__global__ void wigner_coef_kernel(int LT, double* __restrict__ coef, double* __restrict__ lfac) {
+  int idx = blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx < NPAIR * LT) {
+    int q = idx / LT, li = idx % LT;
+    double l = li, m = PAIR_M[q], n = PAIR_N[q];
+    double nA = sqrt((2 * l + 3) / (2 * l + 1));
+    double nC = sqrt((2 * l + 3) / (2 * l - 1));
+    double den = sqrt((l + 1) * (l + 1) - m * m) * sqrt((l + 1) * (l + 1) - n * n);
+    double A = nA * (l + 1) * (2 * l + 1) / den;
+    double B = -A * m * n / l / (l + 1);
+    double C = -nC * sqrt(l * l - m * m) * sqrt(l * l - n * n) / den * (l + 1) / l;
+    // the reference maps non-finite coefficients to 0 (nan_to_num)
+    coef[(size_t)idx * 3 + 0] = isfinite(A) ? A : 0.0;
+    coef[(size_t)idx * 3 + 1] = isfinite(B) ? B : 0.0;
+    coef[(size_t)idx * 3 + 2] = isfinite(C) ? C : 0.0;
+  }
+  if (idx < LT) {
+    double l = idx;
+    lfac[idx * 5 + 0] = sqrt(2. / (2. * l + 1.));
+    lfac[idx * 5 + 1] = sqrt(fmax((l + 2) * (l - 1) * l * (l + 1), 0.0));
+    lfac[idx * 5 + 2] = sqrt(fmax((l + 2) * (l - 1), 0.0));
+    lfac[idx * 5 + 3] = sqrt(fmax((l + 3) * (l - 2), 0.0));
+    lfac[idx * 5 + 4] = sqrt(fmax((l + 4) * (l + 3) * (l - 2) * (l - 3), 0.0));
+  }
+}
+
+__device__ double fact(int n) {
+  double f = 1.0;
+  for (int i = 2; i <= n; ++i) f *= i;
+  return f;
+}
+
+// d^m_{mn}(mu) in the normalised recursion (times sqrt((2m+1)/2))
+__device__ double wigner_start(int m, int n, double mu) {
+  double beta = acos(mu);
+  double norm = sqrt((2 * m + 1) / 2.0) * sqrt(fact(2 * m) / (fact(m + n) * fact(m - n)));
+  double c = cos(beta / 2.), s = -sin(beta / 2.);
+  double v = norm;
+  for (int i = 0; i < m + n; ++i) v *= c;
+  for (int i = 0; i < m - n; ++i) v *= s;
+  return v;
+}
+
+struct Rec {
+  double dl, dlm1;
+};
+
+// advance pair q from l to l+1
+__device__ __forceinline__ void rec_step(Rec& r, const double* __restrict__ cq, int l, double mu) {
+  double A = cq[l * 3 + 0], B = cq[l * 3 + 1], C = cq[l * 3 + 2];
+  double dn = A * mu * r.dl + B * r.dl + C * r.dlm1;
+  r.dlm1 = r.dl;
+  r.dl = dn;
+}
+
+// pass 1: Cgl(mu), Cgl2(mu)
+__global__ void cgl_kernel(const double* __restrict__ clpp, int n_l, const double* __restrict__ mu_arr, int n_mu,
+                           const double* __restrict__ coef, const double* __restrict__ lfac, int LT,
+                           double* __restrict__ cgl) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n_mu) return;
+  double mu = mu_arr[i];
+  const double* c11 = coef + (size_t)1 * LT * 3;
+  const double* c1m = coef + (size_t)2 * LT * 3;
+  Rec a{wigner_start(1, 1, mu), 0.0}, b{wigner_start(1, -1, mu), 0.0};
+  rec_step(a, c11, 1, mu);
+  rec_step(b, c1m, 1, mu);
+  double s1 = 0.0, s2 = 0.0;
+  for (int j = 0; j < n_l; ++j) {
+    int l = j + 2;
+    double nf = lfac[l * 5];
+    double f = (2. * l + 1) * l * (l + 1) * clpp[j];
+    s1 += f * (a.dl * nf);
+    s2 += f * (b.dl * nf);
+    rec_step(a, c11, l, mu);
+    rec_step(b, c1m, l, mu);
+  }
+  cgl[i] = s1 / 4. / PI;
+  cgl[n_mu + i] = s2 / 4. / PI;
+}
+
+// pass 2: correlation functions
+__global__ void ksi_kernel(const double* __restrict__ cl_in, const double* __restrict__ cgl, int n_l,
+                           const double* __restrict__ mu_arr, int n_mu, const double* __restrict__ coef,
+                           const double* __restrict__ lfac, int LT, double* __restrict__ ksi) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n_mu) return;
+  const double* TT = cl_in;
+  const double* TE = cl_in + n_l;
+  const double* EE = cl_in + 2 * n_l;
+  double mu = mu_arr[i];
+  double s2 = cgl[n_mu - 1] - cgl[i];
+  double C2 = cgl[n_mu + i];
+  Rec R[NPAIR];
+#pragma unroll
+  for (int q = 0; q < NPAIR; ++q) R[q] = Rec{0.0, 0.0};
+  // d00 from l=0, d1n from l=1, d2n from l=2 are live at l=2; d3n, d4n start later
+  R[0] = Rec{wigner_start(0, 0, mu), 0.0};
+  rec_step(R[0], coef + 0, 0, mu);
+  rec_step(R[0], coef + 0, 1, mu);
+  R[1] = Rec{wigner_start(1, 1, mu), 0.0};
+  rec_step(R[1], coef + (size_t)1 * LT * 3, 1, mu);
+  R[2] = Rec{wigner_start(1, -1, mu), 0.0};
+  rec_step(R[2], coef + (size_t)2 * LT * 3, 1, mu);
+  R[3] = Rec{wigner_start(2, -2, mu), 0.0};
+  R[4] = Rec{wigner_start(2, 2, mu), 0.0};
+  R[9] = Rec{wigner_start(2, 0, mu), 0.0};
+  double e2 = exp(2. * s2 / 4.), e4 = exp(4. * s2 / 4.), e83 = exp(8. / 3. * s2 / 4.);
+  double e203 = exp(20. / 3. * s2 / 4.), e10 = exp(10. * s2 / 4.);
+  double a0 = 0, ap = 0, am = 0, ax = 0;
+  for (int j = 0; j < n_l; ++j) {
+    int l = j + 2;
+    if (l == 3) {
+      R[5] = Rec{wigner_start(3, 1, mu), 0.0};
+      R[7] = Rec{wigner_start(3, -3, mu), 0.0};
+      R[10] = Rec{wigner_start(3, -1, mu), 0.0};
+    }
+    if (l == 4) {
+      R[6] = Rec{wigner_start(4, 0, mu), 0.0};
+      R[8] = Rec{wigner_start(4, -4, mu), 0.0};
+      R[11] = Rec{wigner_start(4, -2, mu), 0.0};
+    }
+    const double* lf = lfac + l * 5;
+    double nf = lf[0];
+    double d[NPAIR];
+#pragma unroll
+    for (int q = 0; q < NPAIR; ++q) d[q] = R[q].dl * nf;
+    double dl = l;
+    double llp1 = dl * (dl + 1);
+    double E = exp(-llp1 * s2 / 4);
+    double X000 = E;
+    double X000p = -llp1 / 4. * X000;
+    double X220 = 1. / 4. * lf[1] * E * e2;
+    double X022 = E * e4;
+    double X022p = -(llp1 - 4) / 4 * X022;
+    double X121 = -1. / 2. * lf[2] * E * e83;
+    double X132 = -1. / 2. * lf[3] * E * e203;
+    double X242 = 1. / 4. * lf[4] * E * e10;
+    double d00 = d[0], d11 = d[1], d1m1 = d[2], d2m2 = d[3], d22 = d[4], d31 = d[5];
+    double d40 = d[6], d3m3 = d[7], d4m4 = d[8], d20 = d[9], d3m1 = d[10], d4m2 = d[11];
+    double pre = 2. * dl + 1;
+    a0 += pre * TT[j] * (X000 * X000 * d00 + 8. / dl / (dl + 1) * C2 * X000p * X000p * d1m1 +
+                         C2 * C2 * (X000p * X000p * d00 + X220 * X220 * d2m2));
+    ap += pre * EE[j] * (X022 * X022 * d22 + 2 * C2 * X132 * X121 * d31 +
+                         C2 * C2 * (X022p * X022p * d22 + X242 * X220 * d40));
+    am += pre * EE[j] * (X022 * X022 * d2m2 + C2 * (X121 * X121 * d1m1 + X132 * X132 * d3m3) +
+                         1. / 2. * C2 * C2 * (2 * X022p * X022p * d2m2 + X220 * X220 * d00 + X242 * X242 * d4m4));
+    ax += pre * TE[j] * (X022 * X000 * d20 + C2 * 2 * X000p / sqrt(llp1) * (X121 * d11 + X132 * d3m1) +
+                         1. / 2. * C2 * C2 * ((2 * X022p * X000p + X220 * X220) * d20 + X220 * X242 * d4m2));
+#pragma unroll
+    for (int q = 0; q < NPAIR; ++q)
+      if (l >= PAIR_M[q]) rec_step(R[q], coef + (size_t)q * LT * 3, l, mu);
+  }
+  ksi[0 * n_mu + i] = a0 / 4. / PI;
+  ksi[1 * n_mu + i] = ap / 4. / PI;
+  ksi[2 * n_mu + i] = am / 4. / PI;
+  ksi[3 * n_mu + i] = ax / 4. / PI;
+}
+
+// pass 3: per-block partial back-transform sums, part[blk][4][n_l]
+__global__ void __launch_bounds__(64) back_kernel(const double* __restrict__ ksi, int n_l,
+                                                  const double* __restrict__ mu_arr, const double* __restrict__ w_arr,
+                                                  int n_mu, const double* __restrict__ coef,
+                                                  const double* __restrict__ lfac, int LT, double* __restrict__ part) {
+  int i = blockIdx.x * 64 + threadIdx.x;
+  bool live = i < n_mu;
+  double mu = live ? mu_arr[i] : 0.0;
+  double w = live ? w_arr[i] : 0.0;
+  double k0 = live ? ksi[0 * n_mu + i] * w : 0.0, kp = live ? ksi[1 * n_mu + i] * w : 0.0;
+  double km = live ? ksi[2 * n_mu + i] * w : 0.0, kx = live ? ksi[3 * n_mu + i] * w : 0.0;
+  Rec r00{wigner_start(0, 0, mu), 0.0}, r20{wigner_start(2, 0, mu), 0.0};
+  Rec r22{wigner_start(2, 2, mu), 0.0}, r2m2{wigner_start(2, -2, mu), 0.0};
+  const double* c00 = coef;
+  const double* c2m2 = coef + (size_t)3 * LT * 3;
+  const double* c22 = coef + (size_t)4 * LT * 3;
+  const double* c20 = coef + (size_t)9 * LT * 3;
+  rec_step(r00, c00, 0, mu);
+  rec_step(r00, c00, 1, mu);
+  double* P = part + (size_t)blockIdx.x * 4 * n_l;
+  for (int j = 0; j < n_l; ++j) {
+    int l = j + 2;
+    double nf = lfac[l * 5];
+    double d00 = r00.dl * nf, d20 = r20.dl * nf, d22 = r22.dl * nf, d2m2 = r2m2.dl * nf;
+    double vt = k0 * d00, vx = kx * d20, ve = kp * d22 + km * d2m2, vb = kp * d22 - km * d2m2;
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+      vt += __shfl_xor(vt, o, 64);
+      vx += __shfl_xor(vx, o, 64);
+      ve += __shfl_xor(ve, o, 64);
+      vb += __shfl_xor(vb, o, 64);
+    }
+    if (threadIdx.x == 0) {
+      P[0 * n_l + j] = vt;
+      P[1 * n_l + j] = vx;
+      P[2 * n_l + j] = ve;
+      P[3 * n_l + j] = vb;
+    }
+    rec_step(r00, c00, l, mu);
+    rec_step(r20, c20, l, mu);
+    rec_step(r22, c22, l, mu);
+    rec_step(r2m2, c2m2, l, mu);
+  }
+}
+
+__global__ void back_final_kernel(const double* __restrict__ part, int nblk, int n_l, double* __restrict__ out) {
+  int idx = blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= 4 * n_l) return;
+  double s = 0.0;
+  for (int b = 0; b < nblk; ++b) s += part[(size_t)b * 4 * n_l + idx];
+  int X = idx / n_l;
+  double f = X < 2 ? 2 * PI : 1. / 2. * 2 * PI;
+  out[idx] = s * f;
+}
+
+}  // namespace
+}  // namespace abcmb
+
+using namespace abcmb;
+
+extern "C" int abcmb_lensing_clpp(const AbcmbCosmo* c, const double* pt_dev, const double* lna_dev,
+                                  const double* aH_dev, const double* tau_dev, int n_lna, double chi_rec,
+                                  const double* ells_dev, int n_l, double* clpp_dev, void* stream) {
+  ABCMB_REQUIRE(c && pt_dev && lna_dev && aH_dev && tau_dev && ells_dev && clpp_dev, "null argument");
+  ABCMB_REQUIRE(n_lna >= 2 && n_l > 0 && c->n_save > 1 && c->nk > 1, "bad sizes");
+  LimberArgs A;
+  A.tau0 = c->tau0;
+  A.Om = c->omega_m / (c->h * c->h);
+  A.OL = c->omega_Lambda / (c->h * c->h);
+  A.A_s = c->A_s;
+  A.n_s = c->n_s;
+  A.kp = c->k_pivot;
+  A.chi_rec = chi_rec;
+  A.n_save = c->n_save;
+  A.nk = c->nk;
+  A.lna_save = c->lna_save;
+  A.k = c->k;
+  hipLaunchKernelGGL(clpp_kernel, dim3((n_l + 127) / 128), dim3(128), 0, (hipStream_t)stream, A, pt_dev, lna_dev,
+                     aH_dev, tau_dev, n_lna, ells_dev, n_l, clpp_dev);
+  ABCMB_HIP_CHECK(hipGetLastError());
+  return ABCMB_OK;
+}
+
+static size_t lens_ws_parts(int n_mu, int n_l, size_t* off) {
+  int LT = n_l + 3;
+  size_t o = 0;
+  off[0] = o; o += (size_t)NPAIR * LT * 3;   // coef
+  off[1] = o; o += (size_t)LT * 5;           // lfac
+  off[2] = o; o += (size_t)2 * n_mu;         // cgl
+  off[3] = o; o += (size_t)4 * n_mu;         // ksi
+  off[4] = o; o += (size_t)((n_mu + 63) / 64) * 4 * n_l;  // partials
+  return o;
+}
+
+extern "C" size_t abcmb_lens_workspace_size(int n_mu, int n_l) {
+  size_t off[5];
+  return sizeof(double) * lens_ws_parts(n_mu, n_l, off);
+}
+
+extern "C" int abcmb_lens(const double* cl_in_dev, const double* clpp_dev, int n_l, const double* mu_dev,
+                          const double* w_dev, int n_mu, double* out_dev, void* workspace, size_t ws_bytes,
+                          void* stream) {
+  ABCMB_REQUIRE(cl_in_dev && clpp_dev && mu_dev && w_dev && out_dev && workspace, "null argument");
+  ABCMB_REQUIRE(n_l >= 3 && n_mu >= 2, "bad sizes");
+  ABCMB_REQUIRE(ws_bytes >= abcmb_lens_workspace_size(n_mu, n_l), "workspace too small");
+  hipStream_t s = (hipStream_t)stream;
+  size_t off[5];
+  lens_ws_parts(n_mu, n_l, off);
+  double* ws = (double*)workspace;
+  double* coef = ws + off[0];
+  double* lfac = ws + off[1];
+  double* cgl = ws + off[2];
+  double* ksi = ws + off[3];
+  double* part = ws + off[4];
+  int LT = n_l + 3;
+  int nblk = (n_mu + 63) / 64;
+  hipLaunchKernelGGL(wigner_coef_kernel, dim3((NPAIR * LT + 255) / 256), dim3(256), 0, s, LT, coef, lfac);
+  hipLaunchKernelGGL(cgl_kernel, dim3(nblk), dim3(64), 0, s, clpp_dev, n_l, mu_dev, n_mu, coef, lfac, LT, cgl);
+  hipLaunchKernelGGL(ksi_kernel, dim3(nblk), dim3(64), 0, s, cl_in_dev, cgl, n_l, mu_dev, n_mu, coef, lfac, LT, ksi);
+  hipLaunchKernelGGL(back_kernel, dim3(nblk), dim3(64), 0, s, ksi, n_l, mu_dev, w_dev, n_mu, coef, lfac, LT, part);
+  hipLaunchKernelGGL(back_final_kernel, dim3((4 * n_l + 255) / 256), dim3(256), 0, s, part, nblk, n_l, out_dev);
+  ABCMB_HIP_CHECK(hipGetLastError());
+  return ABCMB_OK;
+}

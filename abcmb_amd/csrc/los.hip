@@ -1,0 +1,380 @@
+// Source functions, line-of-sight transfer, C_l k-quadrature, l-spline
+// operator and P(k), all on the GPU.
+//
+//   abcmb_sources  : not-a-knot cubic spline (in log10 k) of the 10 LOS
+//                    fields from the PT k-grid to the transfer grid
+//                    (abcmb/spectrum.py:664-678), fused with the source
+//                    functions S_T0, S_T1, S_T2, S_E (:681-696) and the
+//                    trapezoid-in-ln-a weight / aH (:755-772).
+//   abcmb_los_cl   : Delta_l(k) = sum_i S(k, i) phi_l(k (tau0 - tau_i)) on the
+//                    tabulated Bessel functions (:697-783, lookups :48-109)
+//                    and the C_l trapezoid in k (:785-797).  FP64 or FP32.
+//   abcmb_apply_operator : the l-spline (:598-601) as a precomputed linear map.
+//   abcmb_pk_lin   : P(k, z=0) (:250-302).
+#include <cmath>
+#include <cstdint>
+
+#include "common.h"
+
+namespace abcmb {
+namespace {
+
+using namespace cst;
+
+// fields of the perturbation table used by the sources, in spline order
+__constant__ int LOS_FIELDS[10] = {ABCMB_PT_G_DELTA, ABCMB_PT_B_THETA, ABCMB_PT_THETA_B_PRIME, ABCMB_PT_G_SIGMA,
+                                   ABCMB_PT_G_G0,    ABCMB_PT_G_G2,   ABCMB_PT_ETA,           ABCMB_PT_ETA_PRIME,
+                                   ABCMB_PT_ALPHA,   ABCMB_PT_ALPHA_PRIME};
+
+// ---- not-a-knot spline: data-independent Thomas factors on log10 k ----
+// ws layout: x[nk], h[nk], w[nk] (multipliers), bp[nk] (pivots), cc[nk] (super)
+__global__ void spline_factor_kernel(const double* __restrict__ k, int nk, double* __restrict__ ws) {
+  if (blockIdx.x != 0 || threadIdx.x != 0) return;
+  double* x = ws;
+  double* h = x + nk;
+  double* w = h + nk;
+  double* bp = w + nk;
+  double* cc = bp + nk;
+  for (int i = 0; i < nk; ++i) x[i] = log10(k[i]);
+  for (int i = 0; i < nk - 1; ++i) h[i] = x[i + 1] - x[i];
+  int m = nk - 2;
+  // rows r = 0..m-1 correspond to M_{r+1}
+  for (int r = 0; r < m; ++r) {
+    int i = r + 1;
+    double a = h[i - 1], b = 2.0 * (h[i - 1] + h[i]), c = h[i];
+    if (r == 0) { b = h[0] * (1.0 + h[0] / h[1]) + 2.0 * (h[0] + h[1]); c = h[1] - h[0] * h[0] / h[1]; }
+    if (r == m - 1) {
+      double h2 = h[nk - 2], h3 = h[nk - 3];
+      a = h3 - h2 * h2 / h3;
+      b = 2.0 * (h3 + h2) + h2 * (1.0 + h2 / h3);
+      if (m == 1) c = 0.0;
+    }
+    if (r == 0) { w[r] = 0.0; bp[r] = b; }
+    else { w[r] = a / bp[r - 1]; bp[r] = b - w[r] * cc[r - 1]; }
+    cc[r] = c;
+  }
+}
+
+// M for one (field, save-point) row.  out M: [10][nk][n_tau]
+__global__ void spline_m_kernel(const double* __restrict__ pt, int nk, int n_save, const double* __restrict__ ws,
+                                double* __restrict__ Mout) {
+  int n_tau = n_save - 1;
+  int s = blockIdx.x * blockDim.x + threadIdx.x;
+  int f = blockIdx.y;
+  if (s >= n_tau) return;
+  const double* h = ws + nk;
+  const double* w = h + nk;
+  const double* bp = w + nk;
+  const double* cc = bp + nk;
+  const double* y = pt + ((size_t)LOS_FIELDS[f] * n_save + s) * nk;
+  double* M = Mout + (size_t)f * nk * n_tau + s;  // M[f][i][s] at M[i * n_tau]
+  int m = nk - 2;
+  // forward sweep: d'_r stored in M_{r+1}
+  double dprev = 0.0;
+  for (int r = 0; r < m; ++r) {
+    int i = r + 1;
+    double d = 6.0 * ((y[i + 1] - y[i]) / h[i] - (y[i] - y[i - 1]) / h[i - 1]);
+    if (r > 0) d -= w[r] * dprev;
+    M[(size_t)i * n_tau] = d;
+    dprev = d;
+  }
+  double nxt = M[(size_t)m * n_tau] / bp[m - 1];
+  M[(size_t)m * n_tau] = nxt;
+  for (int r = m - 2; r >= 0; --r) {
+    double v = (M[(size_t)(r + 1) * n_tau] - cc[r] * nxt) / bp[r];
+    M[(size_t)(r + 1) * n_tau] = v;
+    nxt = v;
+  }
+  double M1 = M[(size_t)1 * n_tau], M2 = M[(size_t)2 * n_tau];
+  M[0] = M1 * (1.0 + h[0] / h[1]) - h[0] / h[1] * M2;
+  double Ma = M[(size_t)(nk - 2) * n_tau], Mb = M[(size_t)(nk - 3) * n_tau];
+  M[(size_t)(nk - 1) * n_tau] = Ma * (1.0 + h[nk - 2] / h[nk - 3]) - h[nk - 2] / h[nk - 3] * Mb;
+}
+
+// sources at (s, kt).  bgv: [6][n_tau] (tau, g, g', aH, exp(-kappa), aH' aH)
+__global__ void sources_kernel(const double* __restrict__ pt, int nk, int n_save, const double* __restrict__ ws,
+                               const double* __restrict__ Mb, const double* __restrict__ bgv,
+                               const double* __restrict__ kt, int nkt, double sw, double isw, double dop, double pol,
+                               double dl, double* __restrict__ src) {
+  int n_tau = n_save - 1;
+  int j = blockIdx.x * blockDim.x + threadIdx.x;
+  int s = blockIdx.y;
+  if (j >= nkt) return;
+  const double* x = ws;
+  double kk = kt[j];
+  double xq = log10(kk);
+  int lo;
+  if (xq <= x[0]) lo = 0;
+  else if (xq >= x[nk - 1]) lo = nk - 2;
+  else {
+    int a = 0, b = nk - 1;
+    while (b - a > 1) { int mid = (a + b) >> 1; if (x[mid] <= xq) a = mid; else b = mid; }
+    lo = a;
+  }
+  double hh = x[lo + 1] - x[lo];
+  double A = (x[lo + 1] - xq) / hh, B = (xq - x[lo]) / hh;
+  double cA = (A * A * A - A) * hh * hh / 6.0, cB = (B * B * B - B) * hh * hh / 6.0;
+  double v[10];
+#pragma unroll
+  for (int f = 0; f < 10; ++f) {
+    const double* y = pt + ((size_t)LOS_FIELDS[f] * n_save + s) * nk;
+    const double* M = Mb + (size_t)f * nk * n_tau + s;
+    v[f] = A * y[lo] + B * y[lo + 1] + cA * M[(size_t)lo * n_tau] + cB * M[(size_t)(lo + 1) * n_tau];
+  }
+  double g = bgv[1 * n_tau + s], gp = bgv[2 * n_tau + s], aH = bgv[3 * n_tau + s], em = bgv[4 * n_tau + s],
+         aHd = bgv[5 * n_tau + s];
+  double wq = (s == 0 ? 0.5 * dl : dl) / aH;
+  double dg = v[0], tb = v[1], tbp = v[2], sg = v[3], g0 = v[4], g2 = v[5], eta = v[6], etap = v[7], al = v[8],
+         alp = v[9];
+  double sT0 = sw * g * (dg / 4. + aH * alp) +
+               isw * (g * (eta - aH * alp - 2. * aH * al) + 2. * em * (aH * etap - aHd * al - aH * aH * alp)) +
+               dop * (aH * (g * ((tbp / (kk * kk)) + alp) + gp * ((tb / (kk * kk)) + al)));
+  double sT1 = isw * em * ((aH * alp + 2. * aH * al - eta) * kk);
+  double sT2 = pol * g * (2 * sg + g0 + g2) / 8.;
+  double sE = sqrt(6.0) * g * (2 * sg + g0 + g2) / 8.;
+  size_t o = (size_t)s * nkt + j, plane = (size_t)n_tau * nkt;
+  src[0 * plane + o] = wq * sT0;
+  src[1 * plane + o] = wq * sT1;
+  src[2 * plane + o] = wq * sT2;
+  src[3 * plane + o] = wq * sE;
+}
+
+// ---- Bessel lookups (abcmb/spectrum.py:48-109) ----
+template <typename T>
+__device__ __forceinline__ T fast_interp_t(T x, T lo, T hi, const double* __restrict__ f, int n) {
+  T i = (x - lo) / (hi - lo) * (T)(n - 1);
+  const T eps = (T)1e-6;
+  i = fmin(fmax(i, eps), (T)n - (T)1 - eps);
+  int il = (int)floor(i);
+  int iu = min(il + 1, n - 1);
+  T wu = i - (T)il;
+  return ((T)1 - wu) * (T)f[il] + wu * (T)f[iu];
+}
+
+template <typename T>
+__device__ __forceinline__ T j_asym(T l, T x) {
+  const T pi = (T)PI;
+  T nu = l + (T)0.5;
+  T r = sqrt(x * x - nu * nu);
+  T Q = r - nu * pi / (T)2 + nu * asin(nu / x);
+  T J = sqrt((T)2 / pi / r) * cos(Q - pi / (T)4);
+  return sqrt(pi / (T)2 / x) * J;
+}
+
+template <typename T>
+__device__ __forceinline__ T phi_eval(int which, T l, const double* __restrict__ tab, T xmin, T xmax, int nx, T x) {
+  if (x < xmin) return (T)0;
+  if (x >= xmax) {
+    if (which == 0) return j_asym<T>(l, x);
+    if (which == 1) return l / x * j_asym<T>(l, x) - j_asym<T>(l + (T)1, x);
+    return (((T)3 * l * (l - (T)1) - (T)2 * x * x) * j_asym<T>(l, x) + (T)6 * x * j_asym<T>(l + (T)1, x)) /
+           (T)2 / (x * x);
+  }
+  return fast_interp_t<T>(x, xmin, xmax, tab, nx);
+}
+
+// one block per (k chunk, l): partial C_l sums into part[3][n_ell][n_chunk]
+template <typename T, int NTH>
+__global__ void __launch_bounds__(NTH) los_cl_kernel(const double* __restrict__ src, int n_tau,
+                                                     const double* __restrict__ tau_los, double tau0,
+                                                     const double* __restrict__ kt, int nkt,
+                                                     const double* __restrict__ bes, const double* __restrict__ rng,
+                                                     const int32_t* __restrict__ bes_l, int n_lt, int nx,
+                                                     const int32_t* __restrict__ ell_idx, int n_ell, double A_s,
+                                                     double n_s, double k_pivot, double* __restrict__ part,
+                                                     double* __restrict__ delta) {
+  __shared__ double red[3][NTH / 64];
+  int e = blockIdx.y;
+  int j = blockIdx.x * NTH + threadIdx.x;
+  int li = ell_idx[e];
+  T l = (T)bes_l[li];
+  const double* t0 = bes + ((size_t)0 * n_lt + li) * nx;
+  const double* t1 = bes + ((size_t)1 * n_lt + li) * nx;
+  const double* t2 = bes + ((size_t)2 * n_lt + li) * nx;
+  T x0lo = (T)rng[(0 * n_lt + li) * 2], x0hi = (T)rng[(0 * n_lt + li) * 2 + 1];
+  T x1lo = (T)rng[(1 * n_lt + li) * 2], x1hi = (T)rng[(1 * n_lt + li) * 2 + 1];
+  T x2lo = (T)rng[(2 * n_lt + li) * 2], x2hi = (T)rng[(2 * n_lt + li) * 2 + 1];
+  T epsf = sqrt((T)3 / (T)8 * (l + (T)2) * (l + (T)1) * l * (l - (T)1));
+  T D0 = 0, D1 = 0, D2 = 0, D3 = 0;
+  double ctt = 0, cte = 0, cee = 0;
+  if (j < nkt) {
+    T kk = (T)kt[j];
+    size_t plane = (size_t)n_tau * nkt;
+    for (int s = 0; s < n_tau; ++s) {
+      T x = (T)(tau0 - tau_los[s]) * kk;
+      size_t o = (size_t)s * nkt + j;
+      T p0 = phi_eval<T>(0, l, t0, x0lo, x0hi, nx, x);
+      T p1 = phi_eval<T>(1, l, t1, x1lo, x1hi, nx, x);
+      T p2 = phi_eval<T>(2, l, t2, x2lo, x2hi, nx, x);
+      D0 += (T)src[0 * plane + o] * p0;
+      D1 += (T)src[1 * plane + o] * p1;
+      D2 += (T)src[2 * plane + o] * p2;
+      D3 += (T)src[3 * plane + o] * (p0 / (x * x) * epsf);
+    }
+    if (delta) {
+      delta[((size_t)0 * n_ell + e) * nkt + j] = D0;
+      delta[((size_t)1 * n_ell + e) * nkt + j] = D1;
+      delta[((size_t)2 * n_ell + e) * nkt + j] = D2;
+      delta[((size_t)3 * n_ell + e) * nkt + j] = D3;
+    }
+    // trapezoid weight of node j
+    double kd = kt[j];
+    double wj = 0.5 * ((j + 1 < nkt ? kt[j + 1] : kd) - (j > 0 ? kt[j - 1] : kd));
+    double P = 4. * PI * A_s * pow(kd / k_pivot, n_s - 1.) / kd;
+    double Tt = (double)D0 + (double)D1 + (double)D2, E = (double)D3;
+    ctt = wj * P * Tt * Tt;
+    cte = wj * P * Tt * E;
+    cee = wj * P * E * E;
+  }
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    ctt += __shfl_xor(ctt, o, 64);
+    cte += __shfl_xor(cte, o, 64);
+    cee += __shfl_xor(cee, o, 64);
+  }
+  int wv = threadIdx.x >> 6;
+  if ((threadIdx.x & 63) == 0) { red[0][wv] = ctt; red[1][wv] = cte; red[2][wv] = cee; }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double a = 0, b = 0, c = 0;
+    for (int i = 0; i < NTH / 64; ++i) { a += red[0][i]; b += red[1][i]; c += red[2][i]; }
+    int nch = gridDim.x;
+    part[((size_t)0 * n_ell + e) * nch + blockIdx.x] = a;
+    part[((size_t)1 * n_ell + e) * nch + blockIdx.x] = b;
+    part[((size_t)2 * n_ell + e) * nch + blockIdx.x] = c;
+  }
+}
+
+__global__ void sum_parts_kernel(const double* __restrict__ part, int n_rows, int nch, double* __restrict__ out) {
+  int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= n_rows) return;
+  double s = 0.0;
+  for (int c = 0; c < nch; ++c) s += part[(size_t)r * nch + c];
+  out[r] = s;
+}
+
+__global__ void apply_operator_kernel(const double* __restrict__ op, int rows, int cols,
+                                      const double* __restrict__ in, int n_vec, double* __restrict__ out) {
+  int r = blockIdx.x * blockDim.x + threadIdx.x;
+  int m = blockIdx.y;
+  if (r >= rows) return;
+  const double* a = op + (size_t)r * cols;
+  const double* v = in + (size_t)m * cols;
+  double s = 0.0;
+  for (int c = 0; c < cols; ++c) s += a[c] * v[c];
+  out[(size_t)m * rows + r] = s;
+}
+
+__device__ double interp_clamped(double x, const double* __restrict__ xp, const double* __restrict__ fp, int n,
+                                 int stride) {
+  if (x <= xp[0]) return fp[0];
+  if (x >= xp[n - 1]) return fp[(size_t)(n - 1) * stride];
+  int a = 0, b = n - 1;
+  while (b - a > 1) { int mid = (a + b) >> 1; if (xp[mid] <= x) a = mid; else b = mid; }
+  double t = (x - xp[a]) / (xp[a + 1] - xp[a]);
+  return fp[(size_t)a * stride] + t * (fp[(size_t)(a + 1) * stride] - fp[(size_t)a * stride]);
+}
+
+__global__ void pk_kernel(const double* __restrict__ pt, const double* __restrict__ k, int nk, int n_save,
+                          const double* __restrict__ lna_save, const double* __restrict__ kq, int nq, double A_s,
+                          double n_s, double kp, double* __restrict__ out) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nq) return;
+  const double* dm = pt + (size_t)ABCMB_PT_DELTA_M * n_save * nk;
+  double x = kq[i];
+  // delta_m at ln a = 0 for the two bracketing k columns, then linear in k
+  int a, b;
+  double t;
+  if (x <= k[0]) { a = b = 0; t = 0.0; }
+  else if (x >= k[nk - 1]) { a = b = nk - 1; t = 0.0; }
+  else {
+    a = 0; b = nk - 1;
+    while (b - a > 1) { int mid = (a + b) >> 1; if (k[mid] <= x) a = mid; else b = mid; }
+    t = (x - k[a]) / (k[b] - k[a]);
+  }
+  double va = interp_clamped(0.0, lna_save, dm + a, n_save, nk);
+  double vb = interp_clamped(0.0, lna_save, dm + b, n_save, nk);
+  double d = va + t * (vb - va);
+  out[i] = d * d * (A_s * pow(x / kp, n_s - 1.) * (2 * PI * PI / (x * x * x)));
+}
+
+}  // namespace
+}  // namespace abcmb
+
+using namespace abcmb;
+
+extern "C" size_t abcmb_sources_workspace_size(int nk, int n_save) {
+  return sizeof(double) * ((size_t)5 * nk + (size_t)10 * nk * (n_save - 1) + 64);
+}
+
+extern "C" int abcmb_sources(const double* pt_dev, const double* k_dev, int nk, int n_save, const double* bgv_dev,
+                             double dlna, const double* kt_dev, int nkt, const double* scales4, double* src_dev,
+                             void* workspace, size_t ws_bytes, void* stream) {
+  ABCMB_REQUIRE(pt_dev && k_dev && bgv_dev && kt_dev && src_dev && workspace, "null argument");
+  ABCMB_REQUIRE(nk >= 5 && n_save >= 3 && nkt >= 2, "bad sizes nk=%d n_save=%d nkt=%d", nk, n_save, nkt);
+  ABCMB_REQUIRE(ws_bytes >= abcmb_sources_workspace_size(nk, n_save), "workspace too small");
+  double sc[4] = {1, 1, 1, 1};
+  if (scales4) for (int i = 0; i < 4; ++i) sc[i] = scales4[i];
+  hipStream_t s = (hipStream_t)stream;
+  double* ws = (double*)workspace;
+  double* Mb = ws + 5 * (size_t)nk;
+  int n_tau = n_save - 1;
+  hipLaunchKernelGGL(spline_factor_kernel, dim3(1), dim3(64), 0, s, k_dev, nk, ws);
+  hipLaunchKernelGGL(spline_m_kernel, dim3((n_tau + 63) / 64, 10), dim3(64), 0, s, pt_dev, nk, n_save, ws, Mb);
+  hipLaunchKernelGGL(sources_kernel, dim3((nkt + 255) / 256, n_tau), dim3(256), 0, s, pt_dev, nk, n_save, ws, Mb,
+                     bgv_dev, kt_dev, nkt, sc[0], sc[1], sc[2], sc[3], dlna, src_dev);
+  ABCMB_HIP_CHECK(hipGetLastError());
+  return ABCMB_OK;
+}
+
+extern "C" size_t abcmb_los_workspace_size(int n_ell, int nkt) {
+  return sizeof(double) * (size_t)3 * n_ell * ((nkt + 255) / 256);
+}
+
+extern "C" int abcmb_los_cl(const double* src_dev, int n_tau, const double* tau_los_dev, double tau0,
+                            const double* kt_dev, int nkt, const double* bes_dev, const double* bes_rng_dev,
+                            const int32_t* bes_l_dev, int n_lt, int nx, const int32_t* ell_idx_dev, int n_ell,
+                            double A_s, double n_s, double k_pivot, int precision, double* cl_dev,
+                            double* delta_dev, void* workspace, size_t ws_bytes, void* stream) {
+  ABCMB_REQUIRE(src_dev && tau_los_dev && kt_dev && bes_dev && bes_rng_dev && bes_l_dev && ell_idx_dev && cl_dev,
+                "null argument");
+  ABCMB_REQUIRE(n_tau > 0 && nkt > 1 && n_lt > 0 && nx > 1 && n_ell > 0, "bad sizes");
+  ABCMB_REQUIRE(precision == 0 || precision == 1, "precision must be 0 (FP64) or 1 (FP32)");
+  ABCMB_REQUIRE(workspace && ws_bytes >= abcmb_los_workspace_size(n_ell, nkt), "workspace too small");
+  hipStream_t s = (hipStream_t)stream;
+  constexpr int NTH = 256;
+  int nch = (nkt + NTH - 1) / NTH;
+  double* part = (double*)workspace;
+  dim3 grid(nch, n_ell);
+  if (precision == 0)
+    hipLaunchKernelGGL((los_cl_kernel<double, NTH>), grid, dim3(NTH), 0, s, src_dev, n_tau, tau_los_dev, tau0, kt_dev,
+                       nkt, bes_dev, bes_rng_dev, bes_l_dev, n_lt, nx, ell_idx_dev, n_ell, A_s, n_s, k_pivot, part,
+                       delta_dev);
+  else
+    hipLaunchKernelGGL((los_cl_kernel<float, NTH>), grid, dim3(NTH), 0, s, src_dev, n_tau, tau_los_dev, tau0, kt_dev,
+                       nkt, bes_dev, bes_rng_dev, bes_l_dev, n_lt, nx, ell_idx_dev, n_ell, A_s, n_s, k_pivot, part,
+                       delta_dev);
+  hipLaunchKernelGGL(sum_parts_kernel, dim3((3 * n_ell + 255) / 256), dim3(256), 0, s, part, 3 * n_ell, nch, cl_dev);
+  ABCMB_HIP_CHECK(hipGetLastError());
+  return ABCMB_OK;
+}
+
+extern "C" int abcmb_apply_operator(const double* op_dev, int rows, int cols, const double* in_dev, int n_vec,
+                                    double* out_dev, void* stream) {
+  ABCMB_REQUIRE(op_dev && in_dev && out_dev && rows > 0 && cols > 0 && n_vec > 0, "bad operator arguments");
+  hipLaunchKernelGGL(apply_operator_kernel, dim3((rows + 255) / 256, n_vec), dim3(256), 0, (hipStream_t)stream,
+                     op_dev, rows, cols, in_dev, n_vec, out_dev);
+  ABCMB_HIP_CHECK(hipGetLastError());
+  return ABCMB_OK;
+}
+
+extern "C" int abcmb_pk_lin(const double* pt_dev, const double* k_dev, int nk, int n_save,
+                            const double* lna_save_dev, const double* kq_dev, int nq, double A_s, double n_s,
+                            double k_pivot, double* pk_dev, void* stream) {
+  ABCMB_REQUIRE(pt_dev && k_dev && lna_save_dev && kq_dev && pk_dev && nk > 1 && n_save > 1 && nq > 0,
+                "bad P(k) arguments");
+  hipLaunchKernelGGL(pk_kernel, dim3((nq + 255) / 256), dim3(256), 0, (hipStream_t)stream, pt_dev, k_dev, nk,
+                     n_save, lna_save_dev, kq_dev, nq, A_s, n_s, k_pivot, pk_dev);
+  ABCMB_HIP_CHECK(hipGetLastError());
+  return ABCMB_OK;
+}

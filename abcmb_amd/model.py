@@ -1,0 +1,353 @@
+"""``Model`` / ``Output`` / ``PerturbationTable``: the drop-in API.
+
+Mirrors the reference's user-facing surface (abcmb/main.py:29-184,
+:614-650; abcmb/perturbations.py:406-450): ``Model(user_species, **specs)``
+then ``model(params) -> Output`` with the same field names and units
+(C_l dimensionless, P(k) in Mpc^3, k in Mpc^-1).  Background and
+recombination tables come from the host producer; the perturbation ODE,
+sources, line-of-sight transfer, C_l quadrature, l-spline, lensing and
+P(k) run in ``libabcmb_hip.so`` on the GPU.  ``Model.run_batch`` runs many
+cosmologies with one ODE launch for all their k-modes.
+
+Host-side preparation (``Model.prepare``) and device execution
+(``Model.execute``) are separate so that every cosmology's inputs are fixed
+before any kernel runs.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import abi, bessel, splines
+from . import specs as S
+from .background import Background
+from .params import add_derived_parameters
+
+LCDM_SPECIES = ("DarkEnergy", "ColdDarkMatter", "Baryon", "Photon", "MasslessNeutrino")
+
+
+class MassiveNeutrino:
+    """Marker for the massive-neutrino user species (abcmb/species.py:762-1090)."""
+    name = "MassiveNeutrino"
+
+
+def _torch():
+    import torch
+    if not torch.cuda.is_available():
+        raise RuntimeError("abcmb_amd needs a ROCm GPU: torch.cuda.is_available() is False")
+    return torch
+
+
+@dataclass
+class CosmoPlan:
+    """Everything the GPU path needs for one cosmology, prepared on the host."""
+    params: dict
+    bg: Background
+    lna_save: np.ndarray
+    lna_start: np.ndarray
+    rtol: np.ndarray
+    atol: np.ndarray
+    bgv: np.ndarray               # [6][n_save-1] LOS background vectors
+    lens_lna: np.ndarray = None   # [500] Limber ln a axis
+    lens_aH: np.ndarray = None
+    lens_tau: np.ndarray = None
+    chi_rec: float = 0.0
+
+
+class PerturbationTable:
+    """Perturbation table (abcmb/perturbations.py:406-450); arrays are (Nlna, Nk) views."""
+
+    def __init__(self, k, lna, pt, has_ncdm):
+        self.k = k
+        self.lna = lna
+        self.table = pt  # [21][Nlna][Nk]
+        I = abi.PT_INDEX
+        self.delta_m = pt[I["delta_m"]]
+        self.theta_b_prime = pt[I["theta_b_prime"]]
+        self.metric_eta = pt[I["metric_eta"]]
+        self.metric_h_prime = pt[I["metric_h_prime"]]
+        self.metric_eta_prime = pt[I["metric_eta_prime"]]
+        self.metric_alpha = pt[I["metric_alpha"]]
+        self.metric_alpha_prime = pt[I["metric_alpha_prime"]]
+        sp = {
+            "DarkEnergy": {},
+            "ColdDarkMatter": {"delta": pt[I["cdm_delta"]]},
+            "Baryon": {"delta": pt[I["b_delta"]], "theta": pt[I["b_theta"]]},
+            "Photon": {"delta": pt[I["g_delta"]], "theta": pt[I["g_theta"]], "sigma": pt[I["g_sigma"]],
+                       "G0": pt[I["g_G0"]], "G2": pt[I["g_G2"]]},
+            "MasslessNeutrino": {"delta": pt[I["ur_delta"]], "theta": pt[I["ur_theta"]],
+                                 "sigma": pt[I["ur_sigma"]]},
+        }
+        if has_ncdm:
+            sp["MassiveNeutrino"] = {"delta": pt[I["ncdm_delta"]], "theta": pt[I["ncdm_theta"]],
+                                     "sigma": pt[I["ncdm_sigma"]]}
+        self.species_perturbations = sp
+
+
+@dataclass
+class Output:
+    """Result of one cosmology (abcmb/main.py:614-650) plus ClBB and C_l^phiphi."""
+    ClTT: object
+    ClTE: object
+    ClEE: object
+    Pk: object
+    l: np.ndarray
+    k: np.ndarray
+    BG: Background
+    PT: PerturbationTable
+    params: dict
+    ClBB: object = None
+    Clpp: object = None
+    steps: object = None
+
+
+class Model:
+    """GPU drop-in for ``abcmb.Model`` on the perturbation + LOS path."""
+
+    def __init__(self, user_species=None, los_precision: str = "fp64", **kwargs):
+        kwargs.pop("adjoint", None)  # reference option for diffrax AD; not used here
+        specs = S.load_specs(kwargs)
+        self.specs = specs
+        names = list(LCDM_SPECIES)
+        self.has_ncdm = False
+        for s in (user_species or ()):
+            nm = getattr(s, "name", s if isinstance(s, str) else getattr(s, "__name__", ""))
+            if nm != "MassiveNeutrino":
+                raise NotImplementedError(f"user species {nm!r}: only MassiveNeutrino is supported")
+            names.append("MassiveNeutrino")
+            self.has_ncdm = True
+        self.species = tuple(names)
+        self.layout = S.state_layout(specs, self.has_ncdm)
+        k_pt, k_pk = S.get_k_axis_perturbations(specs)
+        if specs.get("k_axis_perturbations") is not None:
+            k_pt = np.asarray(specs["k_axis_perturbations"], dtype=np.float64)
+        self.k_pt = np.ascontiguousarray(k_pt)
+        self.k_pk = np.ascontiguousarray(k_pk)
+        self.k_transfer = np.ascontiguousarray(S.get_k_axis_transfer(specs))
+        self.n_save = int(specs["n_lna_save"])
+        if specs["l_min"] != 2:
+            raise NotImplementedError("l_min must be 2")
+        # multipoles (abcmb/spectrum.py:180-233)
+        self.bes_l, self.bes_tab, self.bes_rng = bessel.load_tables()
+        l_min, l_max = specs["l_min"], specs["l_max"]
+        i_lo = int(np.where(self.bes_l <= l_min)[0][-1])
+        self.ells = np.arange(l_min, l_max + 1)
+        if specs["lensing"]:
+            lens_lmax = l_max + 500
+            i_hi = int(np.where(self.bes_l >= lens_lmax)[0][0])
+            self.lensing_ells = np.arange(l_min, lens_lmax + 1)
+            mu, w = np.polynomial.legendre.leggauss(lens_lmax + 70)
+            self.mu = np.concatenate((mu, [1.0]))
+            self.w = np.concatenate((w, [0.0]))
+        else:
+            i_hi = int(np.where(self.bes_l >= l_max)[0][0])
+            self.lensing_ells = self.ells
+            self.mu = self.w = None
+        self.ell_idx = np.arange(i_lo, i_hi + 1, dtype=np.int32)
+        self.ell_sampled = self.bes_l[self.ell_idx].astype(np.float64)
+        self.ell_op = splines.operator(self.ell_sampled, self.lensing_ells.astype(np.float64))
+        self.scales = np.array([specs["scale_sw"], specs["scale_isw"], specs["scale_dop"], specs["scale_pol"]],
+                               dtype=np.float64)
+        lay = self.layout
+        self.settings = abi.AbcmbOdeSettings(
+            pcoeff=specs["pcoeff_PE"], icoeff=specs["icoeff_PE"], dcoeff=specs["dcoeff_PE"], dt0=1e-2,
+            safety=0.9, factormin=0.2, factormax=10.0, newton_kappa=1e-2,
+            max_steps=int(specs["max_steps_PE"]), max_newton=10,
+            l_g=lay.l_g, l_pol=lay.l_pol, l_ur=lay.l_ur, l_ncdm=lay.l_ncdm, n_q=lay.n_q, n=lay.n)
+        if los_precision not in ("fp64", "fp32"):
+            raise ValueError("los_precision must be 'fp64' or 'fp32'")
+        self.los_precision = 0 if los_precision == "fp64" else 1
+        self._dev = None
+
+    # ------------------------------------------------------------------ host side
+    def derived(self, params: dict) -> dict:
+        return add_derived_parameters(params, self.species, self.specs)
+
+    def prepare(self, full_params: dict) -> CosmoPlan:
+        sp = self.specs
+        bg = Background(full_params, self.has_ncdm, n_q=self.layout.n_q, k_pivot=sp["k_pivot"],
+                        input_tau_reion=sp["input_tau_reion"])
+        lna_save = np.linspace(bg.lna_transfer_start, 0.0, self.n_save)
+        ls = bg.lna_start(self.k_pt, sp["R_tc"], sp["R_large"])
+        big = self.k_pt > sp["k_split_PE"]
+        rtol = np.where(big, sp["rtol_large_k_PE"], sp["rtol_small_k_PE"])
+        atol = np.where(big, sp["atol_large_k_PE"], sp["atol_small_k_PE"])
+        bgv = bg.los_vectors(lna_save[:-1])
+        plan = CosmoPlan(full_params, bg, lna_save, ls, rtol, atol, bgv)
+        if sp["lensing"]:
+            lens_lna = np.linspace(bg.lna_rec, 0.0, 500)
+            ev = bg.eval(lens_lna)
+            plan.lens_lna = lens_lna
+            plan.lens_aH = ev["aH"]
+            plan.lens_tau = ev["tau"]
+            plan.chi_rec = bg.tau0 - float(bg.eval(np.array([bg.lna_rec]))["tau"][0])
+        return plan
+
+    # ------------------------------------------------------------------ device side
+    def _device_constants(self):
+        if self._dev is None:
+            torch = _torch()
+            lib = abi.hip_lib()
+            abi.check(lib.abcmb_device_init(), "abcmb_device_init")
+            dev = torch.device("cuda")
+            t = lambda a, dt=torch.float64: torch.as_tensor(np.ascontiguousarray(a), dtype=dt, device=dev)
+            self._dev = dict(
+                k_pt=t(self.k_pt), k_pk=t(self.k_pk), kt=t(self.k_transfer),
+                bes=t(self.bes_tab), rng=t(self.bes_rng), bes_l=t(self.bes_l, torch.int32),
+                ell_idx=t(self.ell_idx, torch.int32), ell_op=t(self.ell_op),
+                lens_ells=t(self.lensing_ells.astype(np.float64)),
+                mu=t(self.mu) if self.mu is not None else None,
+                w=t(self.w) if self.w is not None else None,
+                scales=self.scales,
+            )
+        return self._dev
+
+    def run_batch(self, params_list, derived: bool = False):
+        """Outputs for several cosmologies; one ODE launch covers all their k-modes."""
+        plist = [p if derived else self.derived(p) for p in params_list]
+        plans = [self.prepare(p) for p in plist]
+        return self.execute(plans)
+
+    def execute(self, plans, want_state: bool = False, sync: bool = True):
+        torch = _torch()
+        lib = abi.hip_lib()
+        D = self._device_constants()
+        dev = torch.device("cuda")
+        stream = torch.cuda.current_stream().cuda_stream
+        nk, ns = self.k_pt.size, self.n_save
+        ncos = len(plans)
+        keep = []  # host/device buffers that must outlive the launches
+
+        def dt(a, dtype=torch.float64):
+            x = torch.as_tensor(np.ascontiguousarray(a), dtype=dtype).to(dev, non_blocking=False)
+            keep.append(x)
+            return x
+
+        cos_arr = (abi.AbcmbCosmo * ncos)()
+        pts = []
+        for i, pl in enumerate(plans):
+            bg = pl.bg
+            c = abi.AbcmbCosmo.from_buffer_copy(bg.cosmo)
+            c.tau_tab = dt(bg.tau_tab).data_ptr()
+            c.xe_tab = dt(bg.xe_tab).data_ptr()
+            c.tm_tab = dt(bg.tm_tab).data_ptr()
+            c.lna_save = dt(pl.lna_save).data_ptr()
+            c.k = D["k_pt"].data_ptr()
+            c.n_save, c.nk = ns, nk
+            pt = torch.empty((abi.N_PT_FIELDS, ns, nk), dtype=torch.float64, device=dev)
+            pts.append(pt)
+            c.pt = pt.data_ptr()
+            cos_arr[i] = c
+        cos_dev = dt(np.frombuffer(bytes(cos_arr), dtype=np.uint8), torch.uint8)
+        # work items: all (cosmology, k), largest k first
+        n_items = ncos * nk
+        items = (abi.AbcmbPertItem * n_items)()
+        order = np.argsort(-self.k_pt, kind="stable")
+        m = 0
+        for ik in order:
+            for ci, pl in enumerate(plans):
+                items[m] = abi.AbcmbPertItem(k=self.k_pt[ik], lna_start=pl.lna_start[ik], rtol=pl.rtol[ik],
+                                             atol=pl.atol[ik], cosmo=ci, ik=int(ik))
+                m += 1
+        items_dev = dt(np.frombuffer(bytes(items), dtype=np.uint8), torch.uint8)
+        status = torch.zeros(n_items, dtype=torch.int32, device=dev)
+        steps = torch.zeros(n_items, dtype=torch.int32, device=dev)
+        state = None
+        if want_state:
+            state = torch.zeros((n_items, ns, self.layout.n), dtype=torch.float64, device=dev)
+        st = self.settings
+        abi.check(lib.abcmb_pert_fwd(cos_dev.data_ptr(), ncos, items_dev.data_ptr(), n_items, C.byref(st),
+                                     status.data_ptr(), steps.data_ptr(),
+                                     state.data_ptr() if state is not None else None, None, 0, stream),
+                  "abcmb_pert_fwd")
+        outs = []
+        nkt = self.k_transfer.size
+        n_ell = self.ell_idx.size
+        n_tau = ns - 1
+        ws_src = torch.empty(lib.abcmb_sources_workspace_size(nk, ns) // 8 + 1, dtype=torch.float64, device=dev)
+        ws_los = torch.empty(lib.abcmb_los_workspace_size(n_ell, nkt) // 8 + 1, dtype=torch.float64, device=dev)
+        src = torch.empty((4, n_tau, nkt), dtype=torch.float64, device=dev)
+        nl = self.lensing_ells.size
+        ws_lens = None
+        if self.specs["lensing"]:
+            ws_lens = torch.empty(lib.abcmb_lens_workspace_size(self.mu.size, nl) // 8 + 1, dtype=torch.float64,
+                                  device=dev)
+        sc = self.scales.ctypes.data_as(C.c_void_p)
+        cls_dev = []
+        for i, pl in enumerate(plans):
+            p = pl.params
+            pt = pts[i]
+            bgv = dt(pl.bgv)
+            dl = float(pl.lna_save[-1] - pl.lna_save[-2])
+            abi.check(lib.abcmb_sources(pt.data_ptr(), D["k_pt"].data_ptr(), nk, ns, bgv.data_ptr(), dl,
+                                        D["kt"].data_ptr(), nkt, sc, src.data_ptr(), ws_src.data_ptr(),
+                                        ws_src.numel() * 8, stream), "abcmb_sources")
+            cl_raw = torch.empty((3, n_ell), dtype=torch.float64, device=dev)
+            abi.check(lib.abcmb_los_cl(src.data_ptr(), n_tau, bgv.data_ptr(), pl.bg.tau0, D["kt"].data_ptr(), nkt,
+                                       D["bes"].data_ptr(), D["rng"].data_ptr(), D["bes_l"].data_ptr(),
+                                       self.bes_l.size, self.bes_tab.shape[2], D["ell_idx"].data_ptr(), n_ell,
+                                       p["A_s"], p["n_s"], self.specs["k_pivot"], self.los_precision,
+                                       cl_raw.data_ptr(), None, ws_los.data_ptr(), ws_los.numel() * 8, stream),
+                      "abcmb_los_cl")
+            cl_full = torch.empty((3, nl), dtype=torch.float64, device=dev)
+            abi.check(lib.abcmb_apply_operator(D["ell_op"].data_ptr(), nl, n_ell, cl_raw.data_ptr(), 3,
+                                               cl_full.data_ptr(), stream), "abcmb_apply_operator")
+            clpp = lensed = None
+            if self.specs["lensing"]:
+                c = cos_arr[i]
+                lna_ax, aH_ax, tau_ax = dt(pl.lens_lna), dt(pl.lens_aH), dt(pl.lens_tau)
+                clpp = torch.empty(nl, dtype=torch.float64, device=dev)
+                abi.check(lib.abcmb_lensing_clpp(C.byref(c), pt.data_ptr(), lna_ax.data_ptr(), aH_ax.data_ptr(),
+                                                 tau_ax.data_ptr(), pl.lens_lna.size, pl.chi_rec,
+                                                 D["lens_ells"].data_ptr(), nl, clpp.data_ptr(), stream),
+                          "abcmb_lensing_clpp")
+                lensed = torch.empty((4, nl), dtype=torch.float64, device=dev)
+                abi.check(lib.abcmb_lens(cl_full.data_ptr(), clpp.data_ptr(), nl, D["mu"].data_ptr(),
+                                         D["w"].data_ptr(), self.mu.size, lensed.data_ptr(), ws_lens.data_ptr(),
+                                         ws_lens.numel() * 8, stream), "abcmb_lens")
+            pk = torch.empty(self.k_pk.size, dtype=torch.float64, device=dev)
+            lna_save_dev = dt(pl.lna_save)
+            abi.check(lib.abcmb_pk_lin(pt.data_ptr(), D["k_pt"].data_ptr(), nk, ns, lna_save_dev.data_ptr(),
+                                       D["k_pk"].data_ptr(), self.k_pk.size, p["A_s"], p["n_s"],
+                                       self.specs["k_pivot"], pk.data_ptr(), stream), "abcmb_pk_lin")
+            cls_dev.append((cl_raw, cl_full, clpp, lensed, pk))
+        if sync:
+            torch.cuda.synchronize()
+        st_h = status.cpu().numpy()
+        steps_h = steps.cpu().numpy()
+        bad = np.nonzero(st_h != abi_k_ok())[0]
+        if bad.size:
+            raise RuntimeError(f"perturbation ODE failed for {bad.size} (cosmology, k) items; status codes "
+                               f"{np.unique(st_h[bad]).tolist()} (1 = max_steps_PE reached, 2 = non-finite)")
+        n_l = self.ells.size
+        # per-cosmology step counts in PT k order
+        steps_ck = np.zeros((ncos, nk), dtype=np.int32)
+        m = 0
+        for ik in order:
+            for ci in range(ncos):
+                steps_ck[ci, ik] = steps_h[m]
+                m += 1
+        for i, pl in enumerate(plans):
+            cl_raw, cl_full, clpp, lensed, pk = cls_dev[i]
+            if lensed is not None:
+                TT, TE, EE, BB = lensed[0, :n_l], lensed[1, :n_l], lensed[2, :n_l], lensed[3, :n_l]
+            else:
+                TT, TE, EE, BB = cl_full[0, :n_l], cl_full[1, :n_l], cl_full[2, :n_l], None
+            PT = PerturbationTable(self.k_pt, pl.lna_save, pts[i], self.has_ncdm)
+            out = Output(TT, TE, EE, pk, self.ells, self.k_pk, pl.bg, PT, pl.params, ClBB=BB, Clpp=clpp,
+                         steps=steps_ck[i])
+            out.cl_raw = cl_raw
+            out.cl_unlensed = cl_full
+            outs.append(out)
+        if want_state:
+            return outs, state, order
+        return outs
+
+    def __call__(self, params: dict = {}) -> Output:
+        return self.run_batch([params])[0]
+
+
+def abi_k_ok():
+    return 0

@@ -147,13 +147,13 @@ int abcmb_pert_fwd(const AbcmbCosmo* cosmos_dev, int n_cosmo,
  * Cubic-spline interpolation (not-a-knot, in log10 k) of the 10 LOS fields
  * of a perturbation table from its k grid to the transfer grid, fused with
  * the source functions S_T0, S_T1, S_T2, S_E and the LOS quadrature weight
- * (abcmb/spectrum.py:664-696, :755-758).
+ * (abcmb/spectrum.py:664-696, :755-758); dlna is the save-grid spacing.
  * bgv_dev: [6][n_save-1] from abcmb_host_los_background; kt_dev: [nkt].
  * src_dev out: [4][n_save-1][nkt].  scales: sw, isw, dop, pol.
  */
 size_t abcmb_sources_workspace_size(int nk, int n_save);
 int abcmb_sources(const double* pt_dev, const double* k_dev, int nk, int n_save,
-                  const double* bgv_dev, const double* kt_dev, int nkt,
+                  const double* bgv_dev, double dlna, const double* kt_dev, int nkt,
                   const double* scales4, double* src_dev,
                   void* workspace, size_t ws_bytes, void* stream);
 
@@ -166,13 +166,15 @@ int abcmb_sources(const double* pt_dev, const double* k_dev, int nk, int n_save,
  * points.  precision: 0 = FP64, 1 = FP32.  cl_dev out: [3][n_ell]
  * (TT, TE, EE); delta_dev (optional) out: [4][n_ell][nkt].
  */
+size_t abcmb_los_workspace_size(int n_ell, int nkt);
 int abcmb_los_cl(const double* src_dev, int n_tau, const double* tau_los_dev,
                  double tau0, const double* kt_dev, int nkt,
                  const double* bes_dev, const double* bes_rng_dev,
                  const int32_t* bes_l_dev, int n_lt, int nx,
                  const int32_t* ell_idx_dev, int n_ell,
                  double A_s, double n_s, double k_pivot, int precision,
-                 double* cl_dev, double* delta_dev, void* stream);
+                 double* cl_dev, double* delta_dev,
+                 void* workspace, size_t ws_bytes, void* stream);
 
 /* Dense linear map out[m][r] = sum_c op[r][c] in[m][c] (the l-spline of
  * abcmb/spectrum.py:598-601 applied as a precomputed operator). */
@@ -197,7 +199,7 @@ int abcmb_lens(const double* cl_in_dev, const double* clpp_dev, int n_l,
 
 /* Linear matter power spectrum at z = 0 (abcmb/spectrum.py:268-302). */
 int abcmb_pk_lin(const double* pt_dev, const double* k_dev, int nk, int n_save,
-                 const double* kq_dev, int nq, double A_s, double n_s,
+                 const double* lna_save_dev, const double* kq_dev, int nq, double A_s, double n_s,
                  double k_pivot, double* pk_dev, void* stream);
 
 #ifdef __cplusplus

@@ -398,6 +398,8 @@ static double rms_scaled(const double* e, const double* y0, const double* y1, in
 }
 
 /* out_states: [n_save][n] raw saved states; returns status */
+static double* g_trace = NULL;  /* optional per-step trace (single-threaded debug) */
+static int g_trace_cap = 0;
 static int ode_one_k(const Ctx* x, double lna0, double rtol, double atol, const AbcmbOdeSettings* st,
                      int n_save, const double* lna_save, double* out_states, int* nsteps) {
   int n = x->n;
@@ -457,12 +459,12 @@ static int ode_one_k(const Ctx* x, double lna0, double rtol, double atol, const 
         for (int i = 0; i < n; ++i) z[i] += r[i];
         double nrm = rms_scaled(r, z, NULL, n, rtol, atol);
         if (!isfinite(nrm)) break;
-        if (it == 0) {
-          if (nrm < st->newton_kappa * 1e-2) { conv = 1; break; }
-        } else {
+        if (nrm < 1e-12) { conv = 1; break; }
+        if (it > 0) {
           double th = nrm / prev;
-          if (th >= 1.0) break;
-          if (th / (1.0 - th) * nrm < st->newton_kappa) { conv = 1; break; }
+          if (th > 2.0) break;
+          double fac = nrm * th / (1.0 - th);
+          if (fac > 0.0 && fac < st->newton_kappa) { conv = 1; break; }
         }
         prev = nrm;
       }
@@ -481,6 +483,10 @@ static int ode_one_k(const Ctx* x, double lna0, double rtol, double atol, const 
       if (!isfinite(enorm)) enorm = INFINITY;
     }
     int accept = enorm < 1.0;
+    if (g_trace && steps - 1 < g_trace_cap) {
+      double* tr = g_trace + (size_t)(steps - 1) * 4;
+      tr[0] = t; tr[1] = hh; tr[2] = enorm; tr[3] = ok;
+    }
     double inv = enorm == 0.0 ? INFINITY : 1.0 / enorm;
     double fmin = accept ? 1.0 : st->factormin;
     double fac = st->safety * pow(inv, b1 / 5.0) * pow(prev_inv, b2 / 5.0);
@@ -601,6 +607,18 @@ int oracle_pert(const AbcmbCosmo* cosmos, const AbcmbPertItem* items, int n_item
     if (!states) free(S);
   }
   return 0;
+}
+
+int oracle_trace(const AbcmbCosmo* c, const AbcmbOdeSettings* st, const AbcmbPertItem* I, double* trace, int cap) {
+  Ctx x;
+  ctx_init(&x, c, st, I->k);
+  double* S = (double*)malloc(sizeof(double) * (size_t)c->n_save * st->n);
+  g_trace = trace; g_trace_cap = cap;
+  int nst = 0;
+  ode_one_k(&x, I->lna_start, I->rtol, I->atol, st, c->n_save, c->lna_save, S, &nst);
+  g_trace = NULL;
+  free(S);
+  return nst;
 }
 
 int oracle_rhs(const AbcmbCosmo* c, const AbcmbOdeSettings* st, double k, double lna, const double* y, double* dy) {
