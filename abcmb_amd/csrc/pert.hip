@@ -14,11 +14,12 @@
 //     background-scalar) terms plus its coupling to the two metric sums
 //     h' and eta' (a rank-2 part of A).  Background scalars at a given
 //     ln a are computed once per evaluation time and shared through LDS;
-//   - M = I - h*gamma*A(t_n) is assembled in LDS (column-major, leading
-//     dimension NT) and inverted in place by Gauss-Jordan with partial
-//     pivoting (implicit row permutation), so each chord correction is a
-//     dense mat-vec across the workgroup instead of a serial triangular
-//     solve;
+//   - the chord matrix M = I - h*gamma*A(t_n) is never formed: A is block
+//     diagonal (one block per Boltzmann hierarchy) plus the rank-2 metric
+//     coupling, so each thread inverts its row of the small diagonal block
+//     in registers and M^-1 is applied with the Woodbury identity;
+//   - the background scalars at the step's six abscissae are evaluated in
+//     parallel, one lane per time, once per step;
 //   - stage derivatives, state and error live in registers of the row's
 //     thread; workgroup sums (metric sums, error norms, pivot search) use
 //     wavefront shuffles plus one LDS slot per wavefront.
@@ -127,7 +128,7 @@ __device__ __forceinline__ double bg_xe(const AbcmbCosmo& c, double lna) {
   return fast_interp(lna, c.xe_x0, c.xe_x0 + c.xe_n * c.xe_dx, c.xe_tab, c.xe_n);
 }
 
-__device__ void bg_eval(const AbcmbCosmo& c, double lna, Bg& b) {
+__device__ __noinline__ void bg_eval(const AbcmbCosmo& c, double lna, Bg& b) {
   bg_rho(c, lna, b);
   b.tau = fast_interp(lna, ABCMB_TAU_TAB_LO, 0.0, c.tau_tab, ABCMB_TAU_TAB_N);
   double xe = bg_xe(c, lna);
@@ -168,7 +169,7 @@ struct Layout {
   int ig, ip, iu, inc;
 };
 
-__device__ void row_desc(const Layout& L, double k, int r, Row& w) {
+__device__ __noinline__ void row_desc(const Layout& L, double k, int r, Row& w) {
   for (int j = 0; j < 5; ++j) { w.col[j] = r < L.n ? r : 0; w.cs[j] = 0.0; w.si[j] = SC_ZERO; }
   w.uh = w.ue = 0.0;
   w.whi = w.wei = SC_ZERO;
@@ -254,7 +255,7 @@ __device__ void row_desc(const Layout& L, double k, int r, Row& w) {
 }
 
 // scalars of A(ln a) shared by all rows (written by thread 0)
-__device__ void fill_scalars(const AbcmbCosmo& c, const Layout& L, double k, const Bg& b, double* sc) {
+__device__ __noinline__ void fill_scalars(const AbcmbCosmo& c, const Layout& L, double k, const Bg& b, double* sc) {
   double ah = b.aH;
   sc[SC_ONE] = 1.0;
   sc[SC_IAH] = 1.0 / ah;
@@ -285,168 +286,198 @@ __device__ void fill_scalars(const AbcmbCosmo& c, const Layout& L, double k, con
   sc[SC_ZERO] = 0.0;
 }
 
-// ---------------------------------------------------------------- workgroup reductions
-template <int NT>
-__device__ __forceinline__ double wave_sum(double v) {
+// ---------------------------------------------------------------- the kernel
+// A(ln a) = B + u_h v_h^T + u_e v_e^T.  B is block diagonal with blocks
+// {eta}, {delta_c}, {delta_b, theta_b, F_0..F_lg, G_0..G_lp}, {N_0..N_lu} and
+// one block per massive-neutrino momentum bin; the rank-2 part is the metric
+// coupling through h' and eta' (column weights v = d h'/dy, d eta'/dy, row
+// weights u = d y'_r / d h', d y'_r / d eta').  Each thread keeps its row of
+// (I - g B_blk)^-1 in registers, built by Gauss-Jordan inside the block in
+// natural order (the block pivots stay >= 1 for these matrices; a pivot
+// below 1e-12 is reported as a failed step), and (I - g A)^-1 is applied
+// with the Woodbury identity through a 2x2 capacitance matrix.
+
+constexpr int NTIMES = 6;  // t and the distinct stage abscissae t + c_s h
+__constant__ double TIME_C[NTIMES] = {0.0, 0.52, 1.230333209967908, 0.8957659843500759, 0.43639360985864756,
+                                      1.0};
+__constant__ int STAGE_SLOT[7] = {0, 1, 2, 3, 4, 5, 5};
+constexpr int NBLK = 10;
+
+struct Blk {
+  int id, start, size, lr;
+};
+
+__device__ Blk row_block(const Layout& L, int r) {
+  Blk b;
+  if (r == 0) { b.id = 0; b.start = 0; b.size = 1; }
+  else if (r == 1) { b.id = 1; b.start = 1; b.size = 1; }
+  else if (r < L.iu) { b.id = 2; b.start = 2; b.size = L.iu - 2; }
+  else if (r < L.inc) { b.id = 3; b.start = L.iu; b.size = L.lu + 1; }
+  else if (r < L.n) {
+    int i = (r - L.inc) / (L.ln + 1);
+    b.id = 4 + i; b.start = L.inc + i * (L.ln + 1); b.size = L.ln + 1;
+  } else { b.id = NBLK - 1; b.start = r; b.size = 0; }
+  b.lr = r - b.start;
+  return b;
+}
+
+template <int NT, int BMAX>
+struct Smem {
+  double sc[NTIMES][SC_COUNT];
+  double zl[NT + BMAX];
+  double rl[NT + BMAX];
+  double uh[NT + BMAX];
+  double ue[NT + BMAX];
+  double pbuf[2][NBLK][BMAX];
+  double red[8 * (NT / 64)];
+  double Fs[7][NT];  // stage derivatives, one column per thread
+};
+
+// workgroup sum of NV values (wave shuffles + one LDS slot per wave)
+template <int NT, int NV>
+__device__ __forceinline__ void block_sum(double (&v)[NV], double* red) {
 #pragma unroll
-  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-  return v;
-}
-
-template <int NT>
-__device__ __forceinline__ void block_sum2(double& a, double& b, double* red) {
-  a = wave_sum<NT>(a);
-  b = wave_sum<NT>(b);
-  if constexpr (NT > 64) {
-    int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    __syncthreads();
-    if (lane == 0) { red[2 * w] = a; red[2 * w + 1] = b; }
-    __syncthreads();
-    a = 0.0; b = 0.0;
-    for (int i = 0; i < NT / 64; ++i) { a += red[2 * i]; b += red[2 * i + 1]; }
-  }
-}
-
-template <int NT>
-__device__ __forceinline__ double block_sum(double a, double* red) {
-  double b = 0.0;
-  block_sum2<NT>(a, b, red);
-  return a;
-}
-
-// argmax of v (>= 0 for candidates, -1 otherwise); returns the winning row
-template <int NT>
-__device__ __forceinline__ int block_argmax(double v, int idx, double* red) {
+  for (int q = 0; q < NV; ++q) {
 #pragma unroll
-  for (int o = 32; o > 0; o >>= 1) {
-    double ov = __shfl_xor(v, o, 64);
-    int oi = __shfl_xor(idx, o, 64);
-    if (ov > v || (ov == v && oi < idx)) { v = ov; idx = oi; }
+    for (int o = 32; o > 0; o >>= 1) v[q] += __shfl_xor(v[q], o, 64);
   }
   if constexpr (NT > 64) {
-    int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    constexpr int NW = NT / 64;
+    int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
     __syncthreads();
-    if (lane == 0) { red[2 * w] = v; red[2 * w + 1] = (double)idx; }
+    if (lane == 0) {
+#pragma unroll
+      for (int q = 0; q < NV; ++q) red[q * NW + wv] = v[q];
+    }
     __syncthreads();
-    v = red[0]; idx = (int)red[1];
-    for (int i = 1; i < NT / 64; ++i) {
-      double ov = red[2 * i];
-      int oi = (int)red[2 * i + 1];
-      if (ov > v || (ov == v && oi < idx)) { v = ov; idx = oi; }
+#pragma unroll
+    for (int q = 0; q < NV; ++q) {
+      double s = 0.0;
+      for (int i = 0; i < NW; ++i) s += red[q * NW + i];
+      v[q] = s;
     }
   }
-  return v > 0.0 ? idx : -1;
 }
 
-// ---------------------------------------------------------------- the kernel
-struct Shared {
-  double* S;      // NT x n, column-major: S[c*NT + r]
-  double* sc;     // SC_COUNT
-  double* zl;     // NT
-  double* rl;     // NT residual
-  double* whv;    // NT
-  double* wev;    // NT
-  double* red;    // 2*(NT/64)
-  int* perm;      // NT: pivot row of column p
-  int* pivoted;   // NT flags
-};
+// background scalars of A at t + TIME_C[i] * hh, i < cnt (one lane per time)
+template <int NT, int BMAX>
+__device__ void set_times(Smem<NT, BMAX>& sm, const AbcmbCosmo& c, const Layout& L, double k, double t, double hh,
+                          int cnt) {
+  __syncthreads();
+  int r = threadIdx.x;
+  if (r < cnt) {
+    Bg b;
+    bg_eval(c, t + TIME_C[r] * hh, b);
+    fill_scalars(c, L, k, b, sm.sc[r]);
+  }
+  __syncthreads();
+}
 
-template <int NT>
-struct Ctx {
-  const AbcmbCosmo* c;
-  Layout L;
+struct RowState {
   Row w;
-  double k;
-  Shared sh;
+  Blk B;
+  int lc[5];  // term columns local to the block
+  int n;
 };
 
-// set per-time scalars; whv/wev of this thread's row (as a column)
-template <int NT>
-__device__ void set_time(Ctx<NT>& x, double t, double& wh_own, double& we_own) {
-  int r = threadIdx.x;
-  Bg b;
-  bg_eval(*x.c, t, b);
+// f_r(t_slot, z)
+template <int NT, int BMAX>
+__device__ __forceinline__ double rhs_eval(Smem<NT, BMAX>& sm, const RowState& R, int slot, double z) {
+  const int r = threadIdx.x;
+  sm.zl[r] = z;
   __syncthreads();
-  if (r == 0) fill_scalars(*x.c, x.L, x.k, b, x.sh.sc);
-  __syncthreads();
-  wh_own = x.w.whs * x.sh.sc[x.w.whi];
-  we_own = x.w.wes * x.sh.sc[x.w.wei];
-}
-
-// f_r(t, z) with z already in registers; writes z to LDS first
-template <int NT>
-__device__ double rhs_row(Ctx<NT>& x, double z, double wh_own, double we_own) {
-  int r = threadIdx.x;
-  x.sh.zl[r] = z;
-  double hp = wh_own * z, ep = we_own * z;
-  __syncthreads();
-  block_sum2<NT>(hp, ep, x.sh.red);
+  const double* sc = sm.sc[slot];
+  double v[2] = {R.w.whs * sc[R.w.whi] * z, R.w.wes * sc[R.w.wei] * z};
+  block_sum<NT, 2>(v, sm.red);
   double acc = 0.0;
 #pragma unroll
-  for (int j = 0; j < 5; ++j) acc += x.w.cs[j] * x.sh.sc[x.w.si[j]] * x.sh.zl[x.w.col[j]];
-  acc += x.w.uh * hp + x.w.ue * ep;
-  __syncthreads();  // zl reused by the caller
-  return r < x.L.n ? acc : 0.0;
+  for (int j = 0; j < 5; ++j) acc += R.w.cs[j] * sc[R.w.si[j]] * sm.zl[R.w.col[j]];
+  acc += R.w.uh * v[0] + R.w.ue * v[1];
+  return r < R.n ? acc : 0.0;
 }
 
-// assemble M = I - g*A(t) (scalars must be set) and invert in place
-template <int NT>
-__device__ bool build_invert(Ctx<NT>& x, double g, double wh_own, double we_own) {
-  const int n = x.L.n;
-  const int r = threadIdx.x;
-  double* S = x.sh.S;
-  x.sh.whv[r] = wh_own;
-  x.sh.wev[r] = we_own;
-  x.sh.pivoted[r] = 0;
-  __syncthreads();
-  if (r < n) {
-    double uh = x.w.uh, ue = x.w.ue;
-    for (int c = 0; c < n; ++c) S[c * NT + r] = (r == c ? 1.0 : 0.0) - g * (uh * x.sh.whv[c] + ue * x.sh.wev[c]);
+template <int BMAX>
+struct Inv {
+  double row[BMAX];   // this thread's row of (I - g B_blk)^-1 (block-local columns)
+  double Yh, Ye;      // rows of S^-1 g u_h, S^-1 g u_e
+  double c00, c01, c10, c11;  // inverse capacitance
+  double Vh, Ve;      // this row's weights in h', eta' at t_n
+};
+
+// build and invert S = I - g B(t_n) block-wise, then the Woodbury terms
+template <int NT, int BMAX>
+__device__ bool factor(Smem<NT, BMAX>& sm, const RowState& R, int maxb, double g, Inv<BMAX>& F) {
+  const double* sc = sm.sc[0];
+  const Blk& B = R.B;
 #pragma unroll
-    for (int j = 0; j < 5; ++j) S[x.w.col[j] * NT + r] -= g * x.w.cs[j] * x.sh.sc[x.w.si[j]];
+  for (int c = 0; c < BMAX; ++c) F.row[c] = (c == B.lr) ? 1.0 : 0.0;
+#pragma unroll
+  for (int j = 0; j < 5; ++j) {
+    double v = g * R.w.cs[j] * sc[R.w.si[j]];
+#pragma unroll
+    for (int c = 0; c < BMAX; ++c) F.row[c] -= (R.lc[j] == c) ? v : 0.0;
   }
-  __syncthreads();
-  bool ok = true;
-  int mine_pivoted = 0;
-  for (int p = 0; p < n; ++p) {
-    double cand = (r < n && !mine_pivoted) ? fabs(S[p * NT + r]) : -1.0;
-    int prow = block_argmax<NT>(cand, r, x.sh.red);
-    if (prow < 0) { ok = false; break; }
-    double ap = S[p * NT + prow];
-    double f = (r < n) ? S[p * NT + r] : 0.0;
-    __syncthreads();
-    double inv = 1.0 / ap;
-    if (r == prow) {
-      mine_pivoted = 1;
-      x.sh.perm[p] = prow;
-      for (int c = 0; c < n; ++c) S[c * NT + r] = (c == p) ? inv : S[c * NT + r] * inv;
-    }
-    __syncthreads();
-    if (r < n && r != prow) {
-      for (int c = 0; c < n; ++c) {
-        double pv = S[c * NT + prow];
-        S[c * NT + r] = (c == p) ? -f * pv : S[c * NT + r] - f * pv;
+  double bad = 0.0;
+  for (int p = 0; p < maxb; ++p) {
+    double* pb = sm.pbuf[p & 1][B.id];
+    const bool in = p < B.size;
+    if (in && B.lr == p) {
+      double d = 0.0;
+#pragma unroll
+      for (int c = 0; c < BMAX; ++c) d = (c == p) ? F.row[c] : d;
+      if (!(fabs(d) > 1e-12)) bad = 1.0;
+      double inv = 1.0 / d;
+#pragma unroll
+      for (int c = 0; c < BMAX; ++c) {
+        double v = (c == p) ? inv : F.row[c] * inv;
+        F.row[c] = v;
+        pb[c] = v;
       }
     }
     __syncthreads();
+    if (in && B.lr != p && B.lr < B.size) {
+      double f = 0.0;
+#pragma unroll
+      for (int c = 0; c < BMAX; ++c) f = (c == p) ? F.row[c] : f;
+#pragma unroll
+      for (int c = 0; c < BMAX; ++c) {
+        double pv = pb[c];
+        F.row[c] = (c == p) ? -f * pv : F.row[c] - f * pv;
+      }
+    }
   }
-  return ok;
+  double yh = 0.0, ye = 0.0;
+#pragma unroll
+  for (int c = 0; c < BMAX; ++c) {
+    yh += F.row[c] * sm.uh[B.start + c];
+    ye += F.row[c] * sm.ue[B.start + c];
+  }
+  F.Yh = g * yh;
+  F.Ye = g * ye;
+  F.Vh = R.w.whs * sc[R.w.whi];
+  F.Ve = R.w.wes * sc[R.w.wei];
+  double v[5] = {F.Vh * F.Yh, F.Vh * F.Ye, F.Ve * F.Yh, F.Ve * F.Ye, bad};
+  block_sum<NT, 5>(v, sm.red);
+  double a = 1.0 - v[0], b = -v[1], cc = -v[2], d = 1.0 - v[3];
+  double det = a * d - b * cc;
+  F.c00 = d / det; F.c01 = -b / det; F.c10 = -cc / det; F.c11 = a / det;
+  return v[4] == 0.0 && isfinite(F.c00) && isfinite(F.c11) && det != 0.0;
 }
 
-// x_i = (M^-1 b)_i with b in LDS rl[]; implicit permutation perm[]
-template <int NT>
-__device__ double apply_inverse(Ctx<NT>& x, double b_own) {
-  const int n = x.L.n;
+// x = (I - g A(t_n))^-1 b
+template <int NT, int BMAX>
+__device__ __forceinline__ double solve(Smem<NT, BMAX>& sm, const RowState& R, const Inv<BMAX>& F, double b) {
   const int r = threadIdx.x;
-  x.sh.rl[r] = b_own;
+  sm.rl[r] = r < R.n ? b : 0.0;
   __syncthreads();
-  double acc = 0.0;
-  if (r < n) {
-    int pr = x.sh.perm[r];
-    for (int p = 0; p < n; ++p) acc += x.sh.S[p * NT + pr] * x.sh.rl[x.sh.perm[p]];
-  }
-  __syncthreads();
-  return acc;
+  double z = 0.0;
+#pragma unroll
+  for (int c = 0; c < BMAX; ++c) z += F.row[c] * sm.rl[R.B.start + c];
+  double v[2] = {F.Vh * z, F.Ve * z};
+  block_sum<NT, 2>(v, sm.red);
+  double a = F.c00 * v[0] + F.c01 * v[1];
+  double bb = F.c10 * v[0] + F.c11 * v[1];
+  return r < R.n ? z + F.Yh * a + F.Ye * bb : 0.0;
 }
 
 __device__ __forceinline__ double rms_term(double e, double y0, double y1, double rtol, double atol) {
@@ -456,7 +487,7 @@ __device__ __forceinline__ double rms_term(double e, double y0, double y1, doubl
 }
 
 // initial conditions (abcmb/perturbations.py:161-199 and species y_ini)
-__device__ double ic_row(const AbcmbCosmo& c, const Layout& L, double k, double ti, int r) {
+__device__ __noinline__ double ic_row(const AbcmbCosmo& c, const Layout& L, double k, double ti, int r) {
   double om = c.om, Rn = c.R_nu, Rb = c.R_b;
   double kt = k * ti;
   double dg = -kt * kt / 3. * (1. - om * ti / 5.);
@@ -488,7 +519,7 @@ __device__ double ic_row(const AbcmbCosmo& c, const Layout& L, double k, double 
 
 // output-table fields at one saved point from the state in LDS (zl)
 // (abcmb/perturbations.py:318-404; output_perturbations in abcmb/species.py)
-__device__ double output_field(const AbcmbCosmo& c, const Layout& L, double k, double lna, const double* y, int f) {
+__device__ __noinline__ double output_field(const AbcmbCosmo& c, const Layout& L, double k, double lna, const double* y, int f) {
   Bg b;
   bg_eval(c, lna, b);
   double a = b.a, aH = b.aH;
@@ -544,60 +575,57 @@ __device__ double output_field(const AbcmbCosmo& c, const Layout& L, double k, d
   return y[0] / aH - 2. * al - 3. * KG / aH * srs / (k * k);  // ABCMB_PT_ALPHA_PRIME
 }
 
-template <int NT>
+template <int NT, int BMAX>
 __global__ void __launch_bounds__(NT) pert_fwd_kernel(const AbcmbCosmo* __restrict__ cosmos,
                                                       const AbcmbPertItem* __restrict__ items,
                                                       AbcmbOdeSettings st, int32_t* __restrict__ status_out,
                                                       int32_t* __restrict__ steps_out, double* __restrict__ state_out) {
-  extern __shared__ double lds[];
+  __shared__ Smem<NT, BMAX> sm;
   const int r = threadIdx.x;
   const int item = blockIdx.x;
   const AbcmbPertItem I = items[item];
-  Ctx<NT> x;
-  x.c = cosmos + I.cosmo;
-  const AbcmbCosmo& c = *x.c;
-  x.k = I.k;
-  Layout& L = x.L;
+  const AbcmbCosmo& c = cosmos[I.cosmo];
+  const double k = I.k;
+  Layout L;
   L.lg = st.l_g; L.lp = st.l_pol; L.lu = st.l_ur; L.ln = st.l_ncdm; L.nq = st.n_q; L.n = st.n;
   L.ig = 4; L.ip = 4 + L.lg + 1; L.iu = L.ip + L.lp + 1; L.inc = L.iu + L.lu + 1;
   const int n = L.n;
-  double* p = lds;
-  x.sh.S = p; p += NT * n;
-  x.sh.sc = p; p += SC_COUNT;
-  x.sh.zl = p; p += NT;
-  x.sh.rl = p; p += NT;
-  x.sh.whv = p; p += NT;
-  x.sh.wev = p; p += NT;
-  x.sh.red = p; p += 2 * (NT / 64) + 2;
-  x.sh.perm = (int*)p; p += NT / 2;
-  x.sh.pivoted = (int*)p;
-  row_desc(L, x.k, r, x.w);
+  RowState R;
+  row_desc(L, k, r, R.w);
+  R.B = row_block(L, r);
+  R.n = n;
+#pragma unroll
+  for (int j = 0; j < 5; ++j) R.lc[j] = R.w.col[j] - R.B.start;
+  const int maxb = max(L.iu - 2, max(L.lu + 1, L.nq > 0 ? L.ln + 1 : 1));
+  sm.zl[r] = 0.0; sm.rl[r] = 0.0;
+  sm.uh[r] = r < n ? R.w.uh : 0.0;
+  sm.ue[r] = r < n ? R.w.ue : 0.0;
+  if (r < BMAX) { sm.zl[NT + r] = 0.0; sm.rl[NT + r] = 0.0; sm.uh[NT + r] = 0.0; sm.ue[NT + r] = 0.0; }
+
   const double rtol = I.rtol, atol = I.atol;
   const int n_save = c.n_save;
   const double* __restrict__ lna_save = c.lna_save;
 
   double t = I.lna_start;
   double ti = fast_interp(t, ABCMB_TAU_TAB_LO, 0.0, c.tau_tab, ABCMB_TAU_TAB_N);
-  double y = r < n ? ic_row(c, L, x.k, ti, r) : 0.0;
-  double wh, we;
-  set_time<NT>(x, t, wh, we);
-  double f0 = rhs_row<NT>(x, y, wh, we);
+  double y = r < n ? ic_row(c, L, k, ti, r) : 0.0;
+  set_times(sm, c, L, k, t, 0.0, 1);
+  double f0 = rhs_eval(sm, R, 0, y);
   double h = st.dt0;
   const double b1 = st.pcoeff + st.icoeff + st.dcoeff, b2 = -(st.pcoeff + 2.0 * st.dcoeff);
   double prev_inv = 1.0;
-  bool sc_at_t = true;
   int isave = 0, steps = 0, status = ABCMB_K_OK;
-  double F[7];
+  Inv<BMAX> FI;
 
   auto save_point = [&](double ys, double ls) {
-    x.sh.zl[r] = ys;
+    __syncthreads();
+    sm.zl[r] = ys;
     __syncthreads();
     if (state_out && r < n) state_out[((size_t)item * n_save + isave) * n + r] = ys;
     if (r < ABCMB_N_PT_FIELDS) {
-      double v = output_field(c, L, x.k, ls, x.sh.zl, r);
+      double v = output_field(c, L, k, ls, sm.zl, r);
       c.pt[((size_t)r * n_save + isave) * c.nk + I.ik] = v;
     }
-    __syncthreads();
   };
 
   while (isave < n_save && lna_save[isave] <= t) { save_point(y, lna_save[isave]); ++isave; }
@@ -609,30 +637,28 @@ __global__ void __launch_bounds__(NT) pert_fwd_kernel(const AbcmbCosmo* __restri
     const double hh = tn - t;
     ++steps;
     const double g = hh * KV_G;
-    // Jacobian at t (scalars of t are still current after an accepted step)
-    if (!sc_at_t) set_time<NT>(x, t, wh, we);
-    bool ok = build_invert<NT>(x, g, wh, we);
-    F[0] = f0;
+    set_times(sm, c, L, k, t, hh, NTIMES);
+    bool ok = factor(sm, R, maxb, g, FI);
+    sm.Fs[0][r] = f0;
     double z = y;
-#pragma unroll
+#pragma unroll 1
     for (int s = 1; s < 7; ++s) {
       if (!ok) break;
       double acc = 0.0;
-#pragma unroll
-      for (int j = 0; j < s; ++j) acc += KV_A[s][j] * F[j];
+      for (int j = 0; j < s; ++j) acc += KV_A[s][j] * sm.Fs[j][r];
       const double base = y + hh * acc;
-      z = base + g * F[s - 1];
-      set_time<NT>(x, t + KV_C[s] * hh, wh, we);
+      z = base + g * sm.Fs[s - 1][r];
+      const int slot = STAGE_SLOT[s];
       bool conv = false;
       double prev = 0.0;
       for (int it = 0; it < st.max_newton; ++it) {
-        double fz = rhs_row<NT>(x, z, wh, we);
-        double res = -(z - base - g * fz);
-        double dz = apply_inverse<NT>(x, r < n ? res : 0.0);
+        double fz = rhs_eval(sm, R, slot, z);
+        double dz = solve(sm, R, FI, -(z - base - g * fz));
         z += dz;
-        double term = 0.0;
-        if (r < n) { double v = dz / (atol + rtol * fabs(z)); term = v * v; }
-        double nrm = sqrt(block_sum<NT>(term, x.sh.red) / n);
+        double v[1] = {0.0};
+        if (r < n) { double q = dz / (atol + rtol * fabs(z)); v[0] = q * q; }
+        block_sum<NT, 1>(v, sm.red);
+        double nrm = sqrt(v[0] / n);
         if (!isfinite(nrm)) break;
         if (nrm < 1e-12) { conv = true; break; }
         if (it > 0) {
@@ -644,16 +670,17 @@ __global__ void __launch_bounds__(NT) pert_fwd_kernel(const AbcmbCosmo* __restri
         prev = nrm;
       }
       if (!conv) { ok = false; break; }
-      F[s] = (z - base) / g;
+      sm.Fs[s][r] = (z - base) / g;
     }
     double enorm = INFINITY;
-    double ynew = z;
+    const double ynew = z;
     if (ok) {
       double acc = 0.0;
 #pragma unroll
-      for (int j = 0; j < 7; ++j) acc += KV_E[j] * F[j];
-      double term = r < n ? rms_term(hh * acc, y, ynew, rtol, atol) : 0.0;
-      enorm = sqrt(block_sum<NT>(term, x.sh.red) / n);
+      for (int j = 0; j < 7; ++j) acc += KV_E[j] * sm.Fs[j][r];
+      double v[1] = {r < n ? rms_term(hh * acc, y, ynew, rtol, atol) : 0.0};
+      block_sum<NT, 1>(v, sm.red);
+      enorm = sqrt(v[0] / n);
       if (!isfinite(enorm)) enorm = INFINITY;
     }
     const bool accept = enorm < 1.0;
@@ -663,8 +690,7 @@ __global__ void __launch_bounds__(NT) pert_fwd_kernel(const AbcmbCosmo* __restri
     if (!(fac >= fmin)) fac = fmin;
     if (fac > st.factormax) fac = st.factormax;
     if (accept) {
-      set_time<NT>(x, tn, wh, we);
-      double fnew = rhs_row<NT>(x, ynew, wh, we);
+      double fnew = rhs_eval(sm, R, 5, ynew);
       while (isave < n_save && lna_save[isave] <= tn) {
         double th = (lna_save[isave] - t) / hh;
         double h00 = 2 * th * th * th - 3 * th * th + 1, h10 = th * th * th - 2 * th * th + th;
@@ -677,9 +703,6 @@ __global__ void __launch_bounds__(NT) pert_fwd_kernel(const AbcmbCosmo* __restri
       y = ynew;
       f0 = fnew;
       prev_inv = inv;
-      sc_at_t = true;
-    } else {
-      sc_at_t = false;
     }
     h = hh * fac;
   }
@@ -689,21 +712,11 @@ __global__ void __launch_bounds__(NT) pert_fwd_kernel(const AbcmbCosmo* __restri
   }
 }
 
-template <int NT>
-size_t pert_lds_bytes(int n) {
-  return sizeof(double) * ((size_t)NT * n + SC_COUNT + 4 * NT + 2 * (NT / 64) + 2) + sizeof(int) * 2 * NT;
-}
-
-template <int NT>
+template <int NT, int BMAX>
 int launch(const AbcmbCosmo* cosmos, const AbcmbPertItem* items, int n_items, const AbcmbOdeSettings& st,
            int32_t* status, int32_t* steps, double* state, hipStream_t stream) {
-  size_t lds = pert_lds_bytes<NT>(st.n);
-  ABCMB_REQUIRE(lds <= 160 * 1024, "state size %d needs %zu bytes of LDS", st.n, lds);
-  if (lds > 64 * 1024)
-    ABCMB_HIP_CHECK(hipFuncSetAttribute((const void*)pert_fwd_kernel<NT>,
-                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-  hipLaunchKernelGGL(pert_fwd_kernel<NT>, dim3(n_items), dim3(NT), lds, stream, cosmos, items, st, status, steps,
-                     state);
+  hipLaunchKernelGGL((pert_fwd_kernel<NT, BMAX>), dim3(n_items), dim3(NT), 0, stream, cosmos, items, st, status,
+                     steps, state);
   ABCMB_HIP_CHECK(hipGetLastError());
   return ABCMB_OK;
 }
@@ -731,10 +744,20 @@ extern "C" int abcmb_pert_fwd(const AbcmbCosmo* cosmos_dev, int n_cosmo, const A
   int n_expect = 4 + (st.l_g + 1) + (st.l_pol + 1) + (st.l_ur + 1) + st.n_q * (st.l_ncdm + 1);
   ABCMB_REQUIRE(st.n == n_expect, "settings.n=%d does not match the layout (%d)", st.n, n_expect);
   ABCMB_REQUIRE(st.max_newton > 0 && st.max_steps > 0, "bad iteration limits");
+  int maxb = 2 + (st.l_g + 1) + (st.l_pol + 1);
+  if (st.l_ur + 1 > maxb) maxb = st.l_ur + 1;
+  if (st.n_q > 0 && st.l_ncdm + 1 > maxb) maxb = st.l_ncdm + 1;
+  ABCMB_REQUIRE(maxb <= 64, "a hierarchy block of %d rows exceeds 64", maxb);
   hipStream_t s = (hipStream_t)stream;
-  if (st.n <= 64) return launch<64>(cosmos_dev, items_dev, n_items, st, status_dev, steps_dev, state_dev, s);
-  if (st.n <= 128) return launch<128>(cosmos_dev, items_dev, n_items, st, status_dev, steps_dev, state_dev, s);
-  if (st.n <= 192) return launch<192>(cosmos_dev, items_dev, n_items, st, status_dev, steps_dev, state_dev, s);
+  if (maxb <= 32) {
+    if (st.n <= 64) return launch<64, 32>(cosmos_dev, items_dev, n_items, st, status_dev, steps_dev, state_dev, s);
+    if (st.n <= 128) return launch<128, 32>(cosmos_dev, items_dev, n_items, st, status_dev, steps_dev, state_dev, s);
+    if (st.n <= 192) return launch<192, 32>(cosmos_dev, items_dev, n_items, st, status_dev, steps_dev, state_dev, s);
+  } else {
+    if (st.n <= 64) return launch<64, 64>(cosmos_dev, items_dev, n_items, st, status_dev, steps_dev, state_dev, s);
+    if (st.n <= 128) return launch<128, 64>(cosmos_dev, items_dev, n_items, st, status_dev, steps_dev, state_dev, s);
+    if (st.n <= 192) return launch<192, 64>(cosmos_dev, items_dev, n_items, st, status_dev, steps_dev, state_dev, s);
+  }
   set_error("state size %d exceeds 192", st.n);
   return ABCMB_EINVAL;
 }
