@@ -54,7 +54,7 @@ class AbcmbPertItem(C.Structure):
 
 class AbcmbOdeSettings(C.Structure):
     _fields_ = [(n, _D) for n in ("pcoeff", "icoeff", "dcoeff", "dt0", "safety",
-                                  "factormin", "factormax", "newton_kappa")] + [
+                                  "factormin", "factormax", "newton_kappa", "jac_offset")] + [
         (n, _I) for n in ("max_steps", "max_newton", "l_g", "l_pol", "l_ur", "l_ncdm", "n_q", "n")]
 
 
@@ -75,7 +75,7 @@ def host_lib():
         if not os.path.exists(HOST_LIB_PATH):
             raise RuntimeError(f"{HOST_LIB_PATH} is not built; run __graft_entry__.build()")
         lib = C.CDLL(HOST_LIB_PATH)
-        lib.abcmb_host_background.argtypes = [C.POINTER(AbcmbCosmo)] + [_D] * 6
+        lib.abcmb_host_background.argtypes = [C.POINTER(AbcmbCosmo)] + [_D] * 6 + [C.POINTER(_I)]
         lib.abcmb_host_recomb_points.argtypes = []
         lib.abcmb_host_los_background.argtypes = [C.POINTER(AbcmbCosmo), _P, C.c_int, _P]
         lib.abcmb_host_lna_start.argtypes = [C.POINTER(AbcmbCosmo), _P, C.c_int, _D, _D, _P]
@@ -104,23 +104,20 @@ def hip_lib():
         lib.abcmb_pert_workspace_size.restype = C.c_size_t
         lib.abcmb_pert_workspace_size.argtypes = [C.c_int]
         lib.abcmb_pert_fwd.argtypes = [_P, C.c_int, _P, C.c_int, C.POINTER(AbcmbOdeSettings),
-                                       _P, _P, _P, _P, C.c_size_t, _P]
+                                       _P, _P, _P, _P, _P, C.c_size_t, _P]
         lib.abcmb_sources_workspace_size.restype = C.c_size_t
-        lib.abcmb_sources_workspace_size.argtypes = [C.c_int, C.c_int]
-        lib.abcmb_sources.argtypes = [_P, _P, C.c_int, C.c_int, _P, _D, _P, C.c_int, _P, _P,
-                                      _P, C.c_size_t, _P]
+        lib.abcmb_sources_workspace_size.argtypes = [C.c_int, C.c_int, C.c_int]
+        lib.abcmb_sources.argtypes = [_P, C.c_int, C.c_int, C.c_int, _P, _P, C.c_int, _P, _P, _P, C.c_size_t, _P]
         lib.abcmb_los_workspace_size.restype = C.c_size_t
-        lib.abcmb_los_workspace_size.argtypes = [C.c_int, C.c_int]
-        lib.abcmb_los_cl.argtypes = [_P, C.c_int, _P, _D, _P, C.c_int, _P, _P, _P, C.c_int,
-                                     C.c_int, _P, C.c_int, _D, _D, _D, C.c_int, _P, _P, _P,
-                                     C.c_size_t, _P]
+        lib.abcmb_los_workspace_size.argtypes = [C.c_int, C.c_int, C.c_int]
+        lib.abcmb_los_cl.argtypes = [_P, C.c_int, _P, C.c_int, _P, _P, C.c_int, _P, _P, _P, C.c_int, C.c_int,
+                                     _P, C.c_int, C.c_int, _P, _P, _P, C.c_size_t, _P]
         lib.abcmb_apply_operator.argtypes = [_P, C.c_int, C.c_int, _P, C.c_int, _P, _P]
-        lib.abcmb_lensing_clpp.argtypes = [C.POINTER(AbcmbCosmo), _P, _P, _P, _P, C.c_int, _D,
-                                           _P, C.c_int, _P, _P]
+        lib.abcmb_lensing_clpp.argtypes = [_P, C.c_int, _P, C.c_int, _P, C.c_int, _P, _P]
         lib.abcmb_lens_workspace_size.restype = C.c_size_t
-        lib.abcmb_lens_workspace_size.argtypes = [C.c_int, C.c_int]
-        lib.abcmb_lens.argtypes = [_P, _P, C.c_int, _P, _P, C.c_int, _P, _P, C.c_size_t, _P]
-        lib.abcmb_pk_lin.argtypes = [_P, _P, C.c_int, C.c_int, _P, _P, C.c_int, _D, _D, _D, _P, _P]
+        lib.abcmb_lens_workspace_size.argtypes = [C.c_int, C.c_int, C.c_int]
+        lib.abcmb_lens.argtypes = [C.c_int, _P, _P, C.c_int, _P, _P, C.c_int, _P, _P, C.c_size_t, _P]
+        lib.abcmb_pk_lin.argtypes = [_P, C.c_int, _P, C.c_int, _P, _P]
         _hip = lib
     return _hip
 

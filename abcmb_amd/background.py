@@ -25,7 +25,7 @@ class Background:
     """Background of one cosmology (host-resident FP64 tables)."""
 
     def __init__(self, params: dict, has_ncdm: bool, n_q: int = 3, k_pivot: float = 0.05,
-                 input_tau_reion: bool = True, xe_pad: int = 0):
+                 input_tau_reion: bool = True, xe_pad: int = 0, rec_switch: int = -1):
         lib = abi.host_lib()
         self.params = params
         n_rec = lib.abcmb_host_recomb_points()
@@ -49,10 +49,12 @@ class Background:
             tau_re, z_re = p["tau_reion"], 0.0
         else:
             tau_re, z_re = -1.0, p["z_reion"]
+        sw = C.c_int32(rec_switch)
         rc = lib.abcmb_host_background(C.byref(c), tau_re, z_re, p["Delta_z_reion"], p["z_reion_He"],
-                                       p["Delta_z_reion_He"], p["exp_reion"])
+                                       p["Delta_z_reion_He"], p["exp_reion"], C.byref(sw))
         if rc != 0:
             raise RuntimeError(f"abcmb_host_background failed ({rc})")
+        self.rec_switch = sw.value
         self.cosmo = c
         self.n_rec = n_rec
 

@@ -193,7 +193,7 @@ void tau_table(AbcmbCosmo& c, double* tab) {
       continue;
     }
     // integrate from prev_l to l in sub-intervals of at most 2e-3
-    int nsub = std::max(1, (int)std::ceil((l - prev_l) / 2e-3));
+    int nsub = std::max(1, (int)std::ceil((l - prev_l) / 4e-3));
     double h = (l - prev_l) / nsub, t = prev_tau;
     for (int s = 0; s < nsub; ++s) t += gl8(inv_aH, prev_l + s * h, prev_l + (s + 1) * h);
     tab[i] = t;
@@ -220,7 +220,7 @@ double saha_xe(double T, double nh, double fhe, double* xp_out) {
   double sHe2 = lame * std::exp(-54.417760 / T);
   double lo = 0.0, hi = 1.0 + 2.0 * fhe;
   double xp = 1.0;
-  for (int it = 0; it < 200; ++it) {
+  for (int it = 0; it < 64; ++it) {
     double xe = 0.5 * (lo + hi);
     double ne = xe * nh;
     xp = sH / (ne + sH);
@@ -243,9 +243,7 @@ double beta_b(double T) {
   return alpha_b(T) * lam3 * std::exp(-RYD / 4.0 / T);
 }
 
-struct RecOut { std::vector<double> xe, tm; };
-
-void recombination(const AbcmbCosmo& c, int N, double* xe, double* tm) {
+void recombination(const AbcmbCosmo& c, int N, double* xe, double* tm, int* switch_io) {
   double lo = -std::log(1.0 + REC_Z0);
   double fhe = c.YHe * MH / MHE / (1.0 - c.YHe);
   bool tla = false;
@@ -264,7 +262,10 @@ void recombination(const AbcmbCosmo& c, int N, double* xe, double* tm) {
       double gc = gamma_compton(xe_saha, Tg, c.YHe);
       T_m = Tg * (1.0 - H / gc);
       tm[j] = T_m;
-      if (xp_saha < 0.99) tla = true;
+      if ((*switch_io < 0 && xp_saha < 0.99) || (*switch_io >= 0 && j + 1 >= *switch_io)) {
+        tla = true;
+        if (*switch_io < 0) *switch_io = j + 1;
+      }
       continue;
     }
     // advance x_p and T_m from lna - dlna to lna with backward-Euler substeps
@@ -282,24 +283,19 @@ void recombination(const AbcmbCosmo& c, int N, double* xe, double* tm) {
       double lame = std::pow(2.0 * PI * ME * tg, 1.5) / std::pow(2.0 * PI * HBAR * C_CM, 3.0);
       double r1 = 4.0 * lame * std::exp(-24.587387 / tg) / std::max(ne_guess, 1e-300);
       double xhe = fhe * r1 / (1.0 + r1);
-      // Newton on F(x) = x - xp_old - h * dxdlna(x)
+      // Newton on F(x) = x - xp_old - h * dxdlna(x); rates depend on T only
+      const double K = std::pow(C_CM / LYA_FREQ, 3) / (8.0 * PI * Hs);
+      const double bb = beta_b(tg);
+      const double ab = alpha_b(T_m);
+      const double boltz = std::exp(-E21 / tg);
       double x = xp, xold = xp;
       for (int it = 0; it < 30; ++it) {
-        auto rhs = [&](double xx, double* dr) {
-          double xe_ = xx + xhe;
-          double K = std::pow(C_CM / LYA_FREQ, 3) / (8.0 * PI * Hs);
-          double x1s = std::max(1.0 - xx, 1e-30);
-          double lam_a = 1.0 / (K * n1 * x1s);
-          double bb = beta_b(tg);
-          double Cp = (R2S1S + lam_a) / (R2S1S + lam_a + bb);
-          double ab = alpha_b(T_m);
-          double f = -Cp * (ab * n1 * xe_ * xx - bb * (1.0 - xx) * std::exp(-E21 / tg)) / Hs;
-          // derivative wrt x, C treated as constant
-          *dr = -Cp * (ab * n1 * (xe_ + xx) + bb * std::exp(-E21 / tg)) / Hs;
-          return f;
-        };
-        double d;
-        double f = rhs(x, &d);
+        double xe_ = x + xhe;
+        double x1s = std::max(1.0 - x, 1e-30);
+        double lam_a = 1.0 / (K * n1 * x1s);
+        double Cp = (R2S1S + lam_a) / (R2S1S + lam_a + bb);
+        double f = -Cp * (ab * n1 * xe_ * x - bb * (1.0 - x) * boltz) / Hs;
+        double d = -Cp * (ab * n1 * (xe_ + x) + bb * boltz) / Hs;
         double F = x - xold - h * f;
         double dF = 1.0 - h * d;
         double dx = -F / dF;
@@ -353,7 +349,7 @@ struct Kappa {
   void build(const AbcmbCosmo& cc) {
     c = &cc;
     const double lo = -10.0;
-    const int n = 40001;  // 2.5e-4 spacing
+    const int n = 20001;  // 5e-4 spacing
     nodes.resize(n);
     cum.assign(n, 0.0);
     for (int i = 0; i < n; ++i) nodes[i] = lo + (0.0 - lo) * i / (n - 1);
@@ -373,15 +369,13 @@ struct Kappa {
 
 double expmkappa(const Kappa& K, double l) { return l < -10.0 ? 0.0 : std::exp(-K(l)); }
 
-thread_local Kappa* g_kappa_cache = nullptr;
-
 }  // namespace
 
 extern "C" int abcmb_host_recomb_points(void) { return rec_points(); }
 
 extern "C" int abcmb_host_background(AbcmbCosmo* c, double tau_reion, double z_reion_in,
                                      double Delta_z_reion, double z_reion_He,
-                                     double Delta_z_reion_He, double exp_reion) {
+                                     double Delta_z_reion_He, double exp_reion, int32_t* rec_switch) {
   if (!c || !c->tau_tab || !c->xe_tab || !c->tm_tab) return ABCMB_EINVAL;
   int N = rec_points();
   if (c->xe_n < N || c->tm_n != c->xe_n) return ABCMB_EINVAL;
@@ -390,7 +384,9 @@ extern "C" int abcmb_host_background(AbcmbCosmo* c, double tau_reion, double z_r
   double* tm = const_cast<double*>(c->tm_tab);
   tau_table(*c, tau);
   c->tau0 = fast_interp(0.0, ABCMB_TAU_TAB_LO, 0.0, c->tau_tab, ABCMB_TAU_TAB_N);
-  recombination(*c, N, xe, tm);
+  int sw = rec_switch ? *rec_switch : -1;
+  recombination(*c, N, xe, tm, &sw);
+  if (rec_switch) *rec_switch = sw;
   double x0 = -std::log(1.0 + REC_Z0);
   c->xe_x0 = c->tm_x0 = x0;
   c->xe_dx = c->tm_dx = REC_DLNA;

@@ -40,11 +40,25 @@ struct LimberArgs {
   const double* k;
 };
 
-__global__ void clpp_kernel(LimberArgs A, const double* __restrict__ pt, const double* __restrict__ lna_ax,
-                            const double* __restrict__ aH_ax, const double* __restrict__ tau_ax, int n_lna,
-                            const double* __restrict__ ells, int n_l, double* __restrict__ out) {
+// lens_all: [B][4][n_lna] = ln a axis, aH, tau, and chi(lna_rec) in [3][0]
+__global__ void clpp_kernel(const AbcmbCosmo* __restrict__ cosmos, const double* __restrict__ lens_all, int n_lna,
+                            const double* __restrict__ ells, int n_l, double* __restrict__ out_all) {
   int il = blockIdx.x * blockDim.x + threadIdx.x;
+  int ci = blockIdx.y;
   if (il >= n_l) return;
+  const AbcmbCosmo& c = cosmos[ci];
+  const double* lna_ax = lens_all + (size_t)ci * 4 * n_lna;
+  const double* aH_ax = lna_ax + n_lna;
+  const double* tau_ax = lna_ax + 2 * n_lna;
+  double* out = out_all + (size_t)ci * n_l;
+  const double* pt = c.pt;
+  LimberArgs A;
+  A.tau0 = c.tau0;
+  A.Om = c.omega_m / (c.h * c.h);
+  A.OL = c.omega_Lambda / (c.h * c.h);
+  A.A_s = c.A_s; A.n_s = c.n_s; A.kp = c.k_pivot;
+  A.chi_rec = lna_ax[3 * n_lna];
+  A.n_save = c.n_save; A.nk = c.nk; A.lna_save = c.lna_save; A.k = c.k;
   const double* dm = pt + (size_t)ABCMB_PT_DELTA_M * A.n_save * A.nk;
   double ell = ells[il];
   double lna_floor = lna_ax[n_lna - 2];
@@ -151,11 +165,14 @@ __device__ __forceinline__ void rec_step(Rec& r, const double* __restrict__ cq, 
 }
 
 // pass 1: Cgl(mu), Cgl2(mu)
-__global__ void cgl_kernel(const double* __restrict__ clpp, int n_l, const double* __restrict__ mu_arr, int n_mu,
+__global__ void cgl_kernel(const double* __restrict__ clpp_all, int n_l, const double* __restrict__ mu_arr, int n_mu,
                            const double* __restrict__ coef, const double* __restrict__ lfac, int LT,
-                           double* __restrict__ cgl) {
+                           double* __restrict__ cgl_all) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
+  int ci = blockIdx.y;
   if (i >= n_mu) return;
+  const double* clpp = clpp_all + (size_t)ci * n_l;
+  double* cgl = cgl_all + (size_t)ci * 2 * n_mu;
   double mu = mu_arr[i];
   const double* c11 = coef + (size_t)1 * LT * 3;
   const double* c1m = coef + (size_t)2 * LT * 3;
@@ -177,11 +194,15 @@ __global__ void cgl_kernel(const double* __restrict__ clpp, int n_l, const doubl
 }
 
 // pass 2: correlation functions
-__global__ void ksi_kernel(const double* __restrict__ cl_in, const double* __restrict__ cgl, int n_l,
+__global__ void ksi_kernel(const double* __restrict__ cl_all, const double* __restrict__ cgl_all, int n_l,
                            const double* __restrict__ mu_arr, int n_mu, const double* __restrict__ coef,
-                           const double* __restrict__ lfac, int LT, double* __restrict__ ksi) {
+                           const double* __restrict__ lfac, int LT, double* __restrict__ ksi_all) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
+  int ci = blockIdx.y;
   if (i >= n_mu) return;
+  const double* cl_in = cl_all + (size_t)ci * 3 * n_l;
+  const double* cgl = cgl_all + (size_t)ci * 2 * n_mu;
+  double* ksi = ksi_all + (size_t)ci * 4 * n_mu;
   const double* TT = cl_in;
   const double* TE = cl_in + n_l;
   const double* EE = cl_in + 2 * n_l;
@@ -255,11 +276,14 @@ __global__ void ksi_kernel(const double* __restrict__ cl_in, const double* __res
 }
 
 // pass 3: per-block partial back-transform sums, part[blk][4][n_l]
-__global__ void __launch_bounds__(64) back_kernel(const double* __restrict__ ksi, int n_l,
+__global__ void __launch_bounds__(64) back_kernel(const double* __restrict__ ksi_all, int n_l,
                                                   const double* __restrict__ mu_arr, const double* __restrict__ w_arr,
                                                   int n_mu, const double* __restrict__ coef,
-                                                  const double* __restrict__ lfac, int LT, double* __restrict__ part) {
+                                                  const double* __restrict__ lfac, int LT, double* __restrict__ part_all) {
   int i = blockIdx.x * 64 + threadIdx.x;
+  int ci = blockIdx.y;
+  const double* ksi = ksi_all + (size_t)ci * 4 * n_mu;
+  double* part = part_all + (size_t)ci * gridDim.x * 4 * n_l;
   bool live = i < n_mu;
   double mu = live ? mu_arr[i] : 0.0;
   double w = live ? w_arr[i] : 0.0;
@@ -299,9 +323,12 @@ __global__ void __launch_bounds__(64) back_kernel(const double* __restrict__ ksi
   }
 }
 
-__global__ void back_final_kernel(const double* __restrict__ part, int nblk, int n_l, double* __restrict__ out) {
+__global__ void back_final_kernel(const double* __restrict__ part_all, int nblk, int n_l, double* __restrict__ out_all) {
   int idx = blockIdx.x * blockDim.x + threadIdx.x;
+  int ci = blockIdx.y;
   if (idx >= 4 * n_l) return;
+  const double* part = part_all + (size_t)ci * nblk * 4 * n_l;
+  double* out = out_all + (size_t)ci * 4 * n_l;
   double s = 0.0;
   for (int b = 0; b < nblk; ++b) s += part[(size_t)b * 4 * n_l + idx];
   int X = idx / n_l;
@@ -314,54 +341,41 @@ __global__ void back_final_kernel(const double* __restrict__ part, int nblk, int
 
 using namespace abcmb;
 
-extern "C" int abcmb_lensing_clpp(const AbcmbCosmo* c, const double* pt_dev, const double* lna_dev,
-                                  const double* aH_dev, const double* tau_dev, int n_lna, double chi_rec,
+extern "C" int abcmb_lensing_clpp(const AbcmbCosmo* cosmos_dev, int n_cosmo, const double* lens_dev, int n_lna,
                                   const double* ells_dev, int n_l, double* clpp_dev, void* stream) {
-  ABCMB_REQUIRE(c && pt_dev && lna_dev && aH_dev && tau_dev && ells_dev && clpp_dev, "null argument");
-  ABCMB_REQUIRE(n_lna >= 2 && n_l > 0 && c->n_save > 1 && c->nk > 1, "bad sizes");
-  LimberArgs A;
-  A.tau0 = c->tau0;
-  A.Om = c->omega_m / (c->h * c->h);
-  A.OL = c->omega_Lambda / (c->h * c->h);
-  A.A_s = c->A_s;
-  A.n_s = c->n_s;
-  A.kp = c->k_pivot;
-  A.chi_rec = chi_rec;
-  A.n_save = c->n_save;
-  A.nk = c->nk;
-  A.lna_save = c->lna_save;
-  A.k = c->k;
-  hipLaunchKernelGGL(clpp_kernel, dim3((n_l + 127) / 128), dim3(128), 0, (hipStream_t)stream, A, pt_dev, lna_dev,
-                     aH_dev, tau_dev, n_lna, ells_dev, n_l, clpp_dev);
+  ABCMB_REQUIRE(cosmos_dev && lens_dev && ells_dev && clpp_dev, "null argument");
+  ABCMB_REQUIRE(n_cosmo > 0 && n_lna >= 2 && n_l > 0, "bad sizes");
+  hipLaunchKernelGGL(clpp_kernel, dim3((n_l + 127) / 128, n_cosmo), dim3(128), 0, (hipStream_t)stream, cosmos_dev,
+                     lens_dev, n_lna, ells_dev, n_l, clpp_dev);
   ABCMB_HIP_CHECK(hipGetLastError());
   return ABCMB_OK;
 }
 
-static size_t lens_ws_parts(int n_mu, int n_l, size_t* off) {
+static size_t lens_ws_parts(int n_cosmo, int n_mu, int n_l, size_t* off) {
   int LT = n_l + 3;
   size_t o = 0;
-  off[0] = o; o += (size_t)NPAIR * LT * 3;   // coef
-  off[1] = o; o += (size_t)LT * 5;           // lfac
-  off[2] = o; o += (size_t)2 * n_mu;         // cgl
-  off[3] = o; o += (size_t)4 * n_mu;         // ksi
-  off[4] = o; o += (size_t)((n_mu + 63) / 64) * 4 * n_l;  // partials
+  off[0] = o; o += (size_t)NPAIR * LT * 3;                               // coef
+  off[1] = o; o += (size_t)LT * 5;                                       // lfac
+  off[2] = o; o += (size_t)n_cosmo * 2 * n_mu;                           // cgl
+  off[3] = o; o += (size_t)n_cosmo * 4 * n_mu;                           // ksi
+  off[4] = o; o += (size_t)n_cosmo * ((n_mu + 63) / 64) * 4 * n_l;       // partials
   return o;
 }
 
-extern "C" size_t abcmb_lens_workspace_size(int n_mu, int n_l) {
+extern "C" size_t abcmb_lens_workspace_size(int n_cosmo, int n_mu, int n_l) {
   size_t off[5];
-  return sizeof(double) * lens_ws_parts(n_mu, n_l, off);
+  return sizeof(double) * lens_ws_parts(n_cosmo, n_mu, n_l, off);
 }
 
-extern "C" int abcmb_lens(const double* cl_in_dev, const double* clpp_dev, int n_l, const double* mu_dev,
+extern "C" int abcmb_lens(int n_cosmo, const double* cl_in_dev, const double* clpp_dev, int n_l, const double* mu_dev,
                           const double* w_dev, int n_mu, double* out_dev, void* workspace, size_t ws_bytes,
                           void* stream) {
   ABCMB_REQUIRE(cl_in_dev && clpp_dev && mu_dev && w_dev && out_dev && workspace, "null argument");
-  ABCMB_REQUIRE(n_l >= 3 && n_mu >= 2, "bad sizes");
-  ABCMB_REQUIRE(ws_bytes >= abcmb_lens_workspace_size(n_mu, n_l), "workspace too small");
+  ABCMB_REQUIRE(n_cosmo > 0 && n_l >= 3 && n_mu >= 2, "bad sizes");
+  ABCMB_REQUIRE(ws_bytes >= abcmb_lens_workspace_size(n_cosmo, n_mu, n_l), "workspace too small");
   hipStream_t s = (hipStream_t)stream;
   size_t off[5];
-  lens_ws_parts(n_mu, n_l, off);
+  lens_ws_parts(n_cosmo, n_mu, n_l, off);
   double* ws = (double*)workspace;
   double* coef = ws + off[0];
   double* lfac = ws + off[1];
@@ -371,10 +385,14 @@ extern "C" int abcmb_lens(const double* cl_in_dev, const double* clpp_dev, int n
   int LT = n_l + 3;
   int nblk = (n_mu + 63) / 64;
   hipLaunchKernelGGL(wigner_coef_kernel, dim3((NPAIR * LT + 255) / 256), dim3(256), 0, s, LT, coef, lfac);
-  hipLaunchKernelGGL(cgl_kernel, dim3(nblk), dim3(64), 0, s, clpp_dev, n_l, mu_dev, n_mu, coef, lfac, LT, cgl);
-  hipLaunchKernelGGL(ksi_kernel, dim3(nblk), dim3(64), 0, s, cl_in_dev, cgl, n_l, mu_dev, n_mu, coef, lfac, LT, ksi);
-  hipLaunchKernelGGL(back_kernel, dim3(nblk), dim3(64), 0, s, ksi, n_l, mu_dev, w_dev, n_mu, coef, lfac, LT, part);
-  hipLaunchKernelGGL(back_final_kernel, dim3((4 * n_l + 255) / 256), dim3(256), 0, s, part, nblk, n_l, out_dev);
+  hipLaunchKernelGGL(cgl_kernel, dim3(nblk, n_cosmo), dim3(64), 0, s, clpp_dev, n_l, mu_dev, n_mu, coef, lfac, LT,
+                     cgl);
+  hipLaunchKernelGGL(ksi_kernel, dim3(nblk, n_cosmo), dim3(64), 0, s, cl_in_dev, cgl, n_l, mu_dev, n_mu, coef, lfac,
+                     LT, ksi);
+  hipLaunchKernelGGL(back_kernel, dim3(nblk, n_cosmo), dim3(64), 0, s, ksi, n_l, mu_dev, w_dev, n_mu, coef, lfac, LT,
+                     part);
+  hipLaunchKernelGGL(back_final_kernel, dim3((4 * n_l + 255) / 256, n_cosmo), dim3(256), 0, s, part, nblk, n_l,
+                     out_dev);
   ABCMB_HIP_CHECK(hipGetLastError());
   return ABCMB_OK;
 }

@@ -28,8 +28,13 @@ __constant__ int LOS_FIELDS[10] = {ABCMB_PT_G_DELTA, ABCMB_PT_B_THETA, ABCMB_PT_
 
 // ---- not-a-knot spline: data-independent Thomas factors on log10 k ----
 // ws layout: x[nk], h[nk], w[nk] (multipliers), bp[nk] (pivots), cc[nk] (super)
-__global__ void spline_factor_kernel(const double* __restrict__ k, int nk, double* __restrict__ ws) {
-  if (blockIdx.x != 0 || threadIdx.x != 0) return;
+__global__ void spline_factor_kernel(const AbcmbCosmo* __restrict__ cosmos, int n_cosmo, size_t ws_stride,
+                                     double* __restrict__ ws_all) {
+  int ci = blockIdx.x * blockDim.x + threadIdx.x;
+  if (ci >= n_cosmo) return;
+  const double* k = cosmos[ci].k;
+  int nk = cosmos[ci].nk;
+  double* ws = ws_all + (size_t)ci * ws_stride;
   double* x = ws;
   double* h = x + nk;
   double* w = h + nk;
@@ -56,12 +61,17 @@ __global__ void spline_factor_kernel(const double* __restrict__ k, int nk, doubl
 }
 
 // M for one (field, save-point) row.  out M: [10][nk][n_tau]
-__global__ void spline_m_kernel(const double* __restrict__ pt, int nk, int n_save, const double* __restrict__ ws,
-                                double* __restrict__ Mout) {
+__global__ void spline_m_kernel(const AbcmbCosmo* __restrict__ cosmos, int n_save, size_t ws_stride,
+                                const double* __restrict__ ws_all, size_t m_stride, double* __restrict__ M_all) {
   int n_tau = n_save - 1;
   int s = blockIdx.x * blockDim.x + threadIdx.x;
   int f = blockIdx.y;
+  int ci = blockIdx.z;
   if (s >= n_tau) return;
+  const double* pt = cosmos[ci].pt;
+  int nk = cosmos[ci].nk;
+  const double* ws = ws_all + (size_t)ci * ws_stride;
+  double* Mout = M_all + (size_t)ci * m_stride;
   const double* h = ws + nk;
   const double* w = h + nk;
   const double* bp = w + nk;
@@ -92,14 +102,23 @@ __global__ void spline_m_kernel(const double* __restrict__ pt, int nk, int n_sav
 }
 
 // sources at (s, kt).  bgv: [6][n_tau] (tau, g, g', aH, exp(-kappa), aH' aH)
-__global__ void sources_kernel(const double* __restrict__ pt, int nk, int n_save, const double* __restrict__ ws,
-                               const double* __restrict__ Mb, const double* __restrict__ bgv,
-                               const double* __restrict__ kt, int nkt, double sw, double isw, double dop, double pol,
-                               double dl, double* __restrict__ src) {
+__global__ void sources_kernel(const AbcmbCosmo* __restrict__ cosmos, int n_save, size_t ws_stride,
+                               const double* __restrict__ ws_all, size_t m_stride, const double* __restrict__ M_all,
+                               const double* __restrict__ bgv_all, const double* __restrict__ kt, int nkt, double sw,
+                               double isw, double dop, double pol, double* __restrict__ src_all) {
   int n_tau = n_save - 1;
   int j = blockIdx.x * blockDim.x + threadIdx.x;
   int s = blockIdx.y;
+  int ci = blockIdx.z;
   if (j >= nkt) return;
+  const AbcmbCosmo& c = cosmos[ci];
+  const double* pt = c.pt;
+  int nk = c.nk;
+  const double* ws = ws_all + (size_t)ci * ws_stride;
+  const double* Mb = M_all + (size_t)ci * m_stride;
+  const double* bgv = bgv_all + (size_t)ci * 6 * n_tau;
+  double* src = src_all + (size_t)ci * 4 * n_tau * nkt;
+  double dl = c.lna_save[n_save - 1] - c.lna_save[n_save - 2];
   const double* x = ws;
   double kk = kt[j];
   double xq = log10(kk);
@@ -175,17 +194,24 @@ __device__ __forceinline__ T phi_eval(int which, T l, const double* __restrict__
 
 // one block per (k chunk, l): partial C_l sums into part[3][n_ell][n_chunk]
 template <typename T, int NTH>
-__global__ void __launch_bounds__(NTH) los_cl_kernel(const double* __restrict__ src, int n_tau,
-                                                     const double* __restrict__ tau_los, double tau0,
+__global__ void __launch_bounds__(NTH) los_cl_kernel(const AbcmbCosmo* __restrict__ cosmos,
+                                                     const double* __restrict__ src_all, int n_tau,
+                                                     const double* __restrict__ bgv_all,
                                                      const double* __restrict__ kt, int nkt,
                                                      const double* __restrict__ bes, const double* __restrict__ rng,
                                                      const int32_t* __restrict__ bes_l, int n_lt, int nx,
-                                                     const int32_t* __restrict__ ell_idx, int n_ell, double A_s,
-                                                     double n_s, double k_pivot, double* __restrict__ part,
-                                                     double* __restrict__ delta) {
+                                                     const int32_t* __restrict__ ell_idx, int n_ell,
+                                                     double* __restrict__ part_all, double* __restrict__ delta_all) {
   __shared__ double red[3][NTH / 64];
   int e = blockIdx.y;
+  int ci = blockIdx.z;
   int j = blockIdx.x * NTH + threadIdx.x;
+  const AbcmbCosmo& cc = cosmos[ci];
+  const double tau0 = cc.tau0, A_s = cc.A_s, n_s = cc.n_s, k_pivot = cc.k_pivot;
+  const double* src = src_all + (size_t)ci * 4 * n_tau * nkt;
+  const double* tau_los = bgv_all + (size_t)ci * 6 * n_tau;  // row 0 of the LOS background vectors
+  double* part = part_all + (size_t)ci * 3 * n_ell * gridDim.x;
+  double* delta = delta_all ? delta_all + (size_t)ci * 4 * n_ell * nkt : nullptr;
   int li = ell_idx[e];
   T l = (T)bes_l[li];
   const double* t0 = bes + ((size_t)0 * n_lt + li) * nx;
@@ -275,11 +301,18 @@ __device__ double interp_clamped(double x, const double* __restrict__ xp, const 
   return fp[(size_t)a * stride] + t * (fp[(size_t)(a + 1) * stride] - fp[(size_t)a * stride]);
 }
 
-__global__ void pk_kernel(const double* __restrict__ pt, const double* __restrict__ k, int nk, int n_save,
-                          const double* __restrict__ lna_save, const double* __restrict__ kq, int nq, double A_s,
-                          double n_s, double kp, double* __restrict__ out) {
+__global__ void pk_kernel(const AbcmbCosmo* __restrict__ cosmos, const double* __restrict__ kq, int nq,
+                          double* __restrict__ out_all) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
+  int ci = blockIdx.y;
   if (i >= nq) return;
+  const AbcmbCosmo& cc = cosmos[ci];
+  const double* pt = cc.pt;
+  const double* k = cc.k;
+  const double* lna_save = cc.lna_save;
+  int nk = cc.nk, n_save = cc.n_save;
+  double A_s = cc.A_s, n_s = cc.n_s, kp = cc.k_pivot;
+  double* out = out_all + (size_t)ci * nq;
   const double* dm = pt + (size_t)ABCMB_PT_DELTA_M * n_save * nk;
   double x = kq[i];
   // delta_m at ln a = 0 for the two bracketing k columns, then linear in k
@@ -303,58 +336,63 @@ __global__ void pk_kernel(const double* __restrict__ pt, const double* __restric
 
 using namespace abcmb;
 
-extern "C" size_t abcmb_sources_workspace_size(int nk, int n_save) {
-  return sizeof(double) * ((size_t)5 * nk + (size_t)10 * nk * (n_save - 1) + 64);
+static size_t src_ws_stride(int nk_max) { return (size_t)5 * nk_max + 8; }
+static size_t src_m_stride(int nk_max, int n_save) { return (size_t)10 * nk_max * (n_save - 1); }
+
+extern "C" size_t abcmb_sources_workspace_size(int n_cosmo, int nk_max, int n_save) {
+  return sizeof(double) * (size_t)n_cosmo * (src_ws_stride(nk_max) + src_m_stride(nk_max, n_save));
 }
 
-extern "C" int abcmb_sources(const double* pt_dev, const double* k_dev, int nk, int n_save, const double* bgv_dev,
-                             double dlna, const double* kt_dev, int nkt, const double* scales4, double* src_dev,
-                             void* workspace, size_t ws_bytes, void* stream) {
-  ABCMB_REQUIRE(pt_dev && k_dev && bgv_dev && kt_dev && src_dev && workspace, "null argument");
-  ABCMB_REQUIRE(nk >= 5 && n_save >= 3 && nkt >= 2, "bad sizes nk=%d n_save=%d nkt=%d", nk, n_save, nkt);
-  ABCMB_REQUIRE(ws_bytes >= abcmb_sources_workspace_size(nk, n_save), "workspace too small");
+extern "C" int abcmb_sources(const AbcmbCosmo* cosmos_dev, int n_cosmo, int nk_max, int n_save,
+                             const double* bgv_dev, const double* kt_dev, int nkt, const double* scales4,
+                             double* src_dev, void* workspace, size_t ws_bytes, void* stream) {
+  ABCMB_REQUIRE(cosmos_dev && bgv_dev && kt_dev && src_dev && workspace, "null argument");
+  ABCMB_REQUIRE(n_cosmo > 0 && nk_max >= 5 && n_save >= 3 && nkt >= 2, "bad sizes nk_max=%d n_save=%d nkt=%d",
+                nk_max, n_save, nkt);
+  ABCMB_REQUIRE(ws_bytes >= abcmb_sources_workspace_size(n_cosmo, nk_max, n_save), "workspace too small");
   double sc[4] = {1, 1, 1, 1};
   if (scales4) for (int i = 0; i < 4; ++i) sc[i] = scales4[i];
   hipStream_t s = (hipStream_t)stream;
+  size_t wss = src_ws_stride(nk_max), ms = src_m_stride(nk_max, n_save);
   double* ws = (double*)workspace;
-  double* Mb = ws + 5 * (size_t)nk;
+  double* Mb = ws + (size_t)n_cosmo * wss;
   int n_tau = n_save - 1;
-  hipLaunchKernelGGL(spline_factor_kernel, dim3(1), dim3(64), 0, s, k_dev, nk, ws);
-  hipLaunchKernelGGL(spline_m_kernel, dim3((n_tau + 63) / 64, 10), dim3(64), 0, s, pt_dev, nk, n_save, ws, Mb);
-  hipLaunchKernelGGL(sources_kernel, dim3((nkt + 255) / 256, n_tau), dim3(256), 0, s, pt_dev, nk, n_save, ws, Mb,
-                     bgv_dev, kt_dev, nkt, sc[0], sc[1], sc[2], sc[3], dlna, src_dev);
+  hipLaunchKernelGGL(spline_factor_kernel, dim3((n_cosmo + 63) / 64), dim3(64), 0, s, cosmos_dev, n_cosmo, wss, ws);
+  hipLaunchKernelGGL(spline_m_kernel, dim3((n_tau + 63) / 64, 10, n_cosmo), dim3(64), 0, s, cosmos_dev, n_save, wss,
+                     ws, ms, Mb);
+  hipLaunchKernelGGL(sources_kernel, dim3((nkt + 255) / 256, n_tau, n_cosmo), dim3(256), 0, s, cosmos_dev, n_save,
+                     wss, ws, ms, Mb, bgv_dev, kt_dev, nkt, sc[0], sc[1], sc[2], sc[3], src_dev);
   ABCMB_HIP_CHECK(hipGetLastError());
   return ABCMB_OK;
 }
 
-extern "C" size_t abcmb_los_workspace_size(int n_ell, int nkt) {
-  return sizeof(double) * (size_t)3 * n_ell * ((nkt + 255) / 256);
+extern "C" size_t abcmb_los_workspace_size(int n_cosmo, int n_ell, int nkt) {
+  return sizeof(double) * (size_t)n_cosmo * 3 * n_ell * ((nkt + 255) / 256);
 }
 
-extern "C" int abcmb_los_cl(const double* src_dev, int n_tau, const double* tau_los_dev, double tau0,
-                            const double* kt_dev, int nkt, const double* bes_dev, const double* bes_rng_dev,
-                            const int32_t* bes_l_dev, int n_lt, int nx, const int32_t* ell_idx_dev, int n_ell,
-                            double A_s, double n_s, double k_pivot, int precision, double* cl_dev,
+extern "C" int abcmb_los_cl(const AbcmbCosmo* cosmos_dev, int n_cosmo, const double* src_dev, int n_tau,
+                            const double* bgv_dev, const double* kt_dev, int nkt, const double* bes_dev,
+                            const double* bes_rng_dev, const int32_t* bes_l_dev, int n_lt, int nx,
+                            const int32_t* ell_idx_dev, int n_ell, int precision, double* cl_dev,
                             double* delta_dev, void* workspace, size_t ws_bytes, void* stream) {
-  ABCMB_REQUIRE(src_dev && tau_los_dev && kt_dev && bes_dev && bes_rng_dev && bes_l_dev && ell_idx_dev && cl_dev,
-                "null argument");
-  ABCMB_REQUIRE(n_tau > 0 && nkt > 1 && n_lt > 0 && nx > 1 && n_ell > 0, "bad sizes");
+  ABCMB_REQUIRE(cosmos_dev && src_dev && bgv_dev && kt_dev && bes_dev && bes_rng_dev && bes_l_dev && ell_idx_dev &&
+                    cl_dev, "null argument");
+  ABCMB_REQUIRE(n_cosmo > 0 && n_tau > 0 && nkt > 1 && n_lt > 0 && nx > 1 && n_ell > 0, "bad sizes");
   ABCMB_REQUIRE(precision == 0 || precision == 1, "precision must be 0 (FP64) or 1 (FP32)");
-  ABCMB_REQUIRE(workspace && ws_bytes >= abcmb_los_workspace_size(n_ell, nkt), "workspace too small");
+  ABCMB_REQUIRE(workspace && ws_bytes >= abcmb_los_workspace_size(n_cosmo, n_ell, nkt), "workspace too small");
   hipStream_t s = (hipStream_t)stream;
   constexpr int NTH = 256;
   int nch = (nkt + NTH - 1) / NTH;
   double* part = (double*)workspace;
-  dim3 grid(nch, n_ell);
+  dim3 grid(nch, n_ell, n_cosmo);
   if (precision == 0)
-    hipLaunchKernelGGL((los_cl_kernel<double, NTH>), grid, dim3(NTH), 0, s, src_dev, n_tau, tau_los_dev, tau0, kt_dev,
-                       nkt, bes_dev, bes_rng_dev, bes_l_dev, n_lt, nx, ell_idx_dev, n_ell, A_s, n_s, k_pivot, part,
-                       delta_dev);
+    hipLaunchKernelGGL((los_cl_kernel<double, NTH>), grid, dim3(NTH), 0, s, cosmos_dev, src_dev, n_tau, bgv_dev,
+                       kt_dev, nkt, bes_dev, bes_rng_dev, bes_l_dev, n_lt, nx, ell_idx_dev, n_ell, part, delta_dev);
   else
-    hipLaunchKernelGGL((los_cl_kernel<float, NTH>), grid, dim3(NTH), 0, s, src_dev, n_tau, tau_los_dev, tau0, kt_dev,
-                       nkt, bes_dev, bes_rng_dev, bes_l_dev, n_lt, nx, ell_idx_dev, n_ell, A_s, n_s, k_pivot, part,
-                       delta_dev);
-  hipLaunchKernelGGL(sum_parts_kernel, dim3((3 * n_ell + 255) / 256), dim3(256), 0, s, part, 3 * n_ell, nch, cl_dev);
+    hipLaunchKernelGGL((los_cl_kernel<float, NTH>), grid, dim3(NTH), 0, s, cosmos_dev, src_dev, n_tau, bgv_dev,
+                       kt_dev, nkt, bes_dev, bes_rng_dev, bes_l_dev, n_lt, nx, ell_idx_dev, n_ell, part, delta_dev);
+  hipLaunchKernelGGL(sum_parts_kernel, dim3((3 * n_ell * n_cosmo + 255) / 256), dim3(256), 0, s, part,
+                     3 * n_ell * n_cosmo, nch, cl_dev);
   ABCMB_HIP_CHECK(hipGetLastError());
   return ABCMB_OK;
 }
@@ -368,13 +406,11 @@ extern "C" int abcmb_apply_operator(const double* op_dev, int rows, int cols, co
   return ABCMB_OK;
 }
 
-extern "C" int abcmb_pk_lin(const double* pt_dev, const double* k_dev, int nk, int n_save,
-                            const double* lna_save_dev, const double* kq_dev, int nq, double A_s, double n_s,
-                            double k_pivot, double* pk_dev, void* stream) {
-  ABCMB_REQUIRE(pt_dev && k_dev && lna_save_dev && kq_dev && pk_dev && nk > 1 && n_save > 1 && nq > 0,
-                "bad P(k) arguments");
-  hipLaunchKernelGGL(pk_kernel, dim3((nq + 255) / 256), dim3(256), 0, (hipStream_t)stream, pt_dev, k_dev, nk,
-                     n_save, lna_save_dev, kq_dev, nq, A_s, n_s, k_pivot, pk_dev);
+extern "C" int abcmb_pk_lin(const AbcmbCosmo* cosmos_dev, int n_cosmo, const double* kq_dev, int nq, double* pk_dev,
+                            void* stream) {
+  ABCMB_REQUIRE(cosmos_dev && kq_dev && pk_dev && n_cosmo > 0 && nq > 0, "bad P(k) arguments");
+  hipLaunchKernelGGL(pk_kernel, dim3((nq + 255) / 256, n_cosmo), dim3(256), 0, (hipStream_t)stream, cosmos_dev,
+                     kq_dev, nq, pk_dev);
   ABCMB_HIP_CHECK(hipGetLastError());
   return ABCMB_OK;
 }

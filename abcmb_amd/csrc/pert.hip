@@ -287,6 +287,7 @@ __device__ __noinline__ void fill_scalars(const AbcmbCosmo& c, const Layout& L, 
 }
 
 // ---------------------------------------------------------------- the kernel
+// The chord matrix is I - g A(t_n + jac_offset h) (the oracle uses the same time).
 // A(ln a) = B + u_h v_h^T + u_e v_e^T.  B is block diagonal with blocks
 // {eta}, {delta_c}, {delta_b, theta_b, F_0..F_lg, G_0..G_lp}, {N_0..N_lu} and
 // one block per massive-neutrino momentum bin; the rank-2 part is the metric
@@ -297,9 +298,10 @@ __device__ __noinline__ void fill_scalars(const AbcmbCosmo& c, const Layout& L, 
 // below 1e-12 is reported as a failed step), and (I - g A)^-1 is applied
 // with the Woodbury identity through a 2x2 capacitance matrix.
 
-constexpr int NTIMES = 6;  // t and the distinct stage abscissae t + c_s h
-__constant__ double TIME_C[NTIMES] = {0.0, 0.52, 1.230333209967908, 0.8957659843500759, 0.43639360985864756,
-                                      1.0};
+constexpr int NTIMES = 7;  // t, the distinct stage abscissae t + c_s h, and the chord-Jacobian time
+constexpr int JAC_SLOT = 6;
+__constant__ double TIME_C[NTIMES - 1] = {0.0, 0.52, 1.230333209967908, 0.8957659843500759, 0.43639360985864756,
+                                          1.0};
 __constant__ int STAGE_SLOT[7] = {0, 1, 2, 3, 4, 5, 5};
 constexpr int NBLK = 10;
 
@@ -362,12 +364,12 @@ __device__ __forceinline__ void block_sum(double (&v)[NV], double* red) {
 // background scalars of A at t + TIME_C[i] * hh, i < cnt (one lane per time)
 template <int NT, int BMAX>
 __device__ void set_times(Smem<NT, BMAX>& sm, const AbcmbCosmo& c, const Layout& L, double k, double t, double hh,
-                          int cnt) {
+                          int cnt, double jac_offset) {
   __syncthreads();
   int r = threadIdx.x;
   if (r < cnt) {
     Bg b;
-    bg_eval(c, t + TIME_C[r] * hh, b);
+    bg_eval(c, t + (r == JAC_SLOT ? jac_offset : TIME_C[r]) * hh, b);
     fill_scalars(c, L, k, b, sm.sc[r]);
   }
   __syncthreads();
@@ -407,7 +409,7 @@ struct Inv {
 // build and invert S = I - g B(t_n) block-wise, then the Woodbury terms
 template <int NT, int BMAX>
 __device__ bool factor(Smem<NT, BMAX>& sm, const RowState& R, int maxb, double g, Inv<BMAX>& F) {
-  const double* sc = sm.sc[0];
+  const double* sc = sm.sc[JAC_SLOT];
   const Blk& B = R.B;
 #pragma unroll
   for (int c = 0; c < BMAX; ++c) F.row[c] = (c == B.lr) ? 1.0 : 0.0;
@@ -579,7 +581,8 @@ template <int NT, int BMAX>
 __global__ void __launch_bounds__(NT) pert_fwd_kernel(const AbcmbCosmo* __restrict__ cosmos,
                                                       const AbcmbPertItem* __restrict__ items,
                                                       AbcmbOdeSettings st, int32_t* __restrict__ status_out,
-                                                      int32_t* __restrict__ steps_out, double* __restrict__ state_out) {
+                                                      int32_t* __restrict__ steps_out, int32_t* __restrict__ iters_out,
+                                                      double* __restrict__ state_out) {
   __shared__ Smem<NT, BMAX> sm;
   const int r = threadIdx.x;
   const int item = blockIdx.x;
@@ -609,12 +612,12 @@ __global__ void __launch_bounds__(NT) pert_fwd_kernel(const AbcmbCosmo* __restri
   double t = I.lna_start;
   double ti = fast_interp(t, ABCMB_TAU_TAB_LO, 0.0, c.tau_tab, ABCMB_TAU_TAB_N);
   double y = r < n ? ic_row(c, L, k, ti, r) : 0.0;
-  set_times(sm, c, L, k, t, 0.0, 1);
+  set_times(sm, c, L, k, t, 0.0, 1, st.jac_offset);
   double f0 = rhs_eval(sm, R, 0, y);
   double h = st.dt0;
   const double b1 = st.pcoeff + st.icoeff + st.dcoeff, b2 = -(st.pcoeff + 2.0 * st.dcoeff);
   double prev_inv = 1.0;
-  int isave = 0, steps = 0, status = ABCMB_K_OK;
+  int isave = 0, steps = 0, iters = 0, status = ABCMB_K_OK;
   Inv<BMAX> FI;
 
   auto save_point = [&](double ys, double ls) {
@@ -637,7 +640,7 @@ __global__ void __launch_bounds__(NT) pert_fwd_kernel(const AbcmbCosmo* __restri
     const double hh = tn - t;
     ++steps;
     const double g = hh * KV_G;
-    set_times(sm, c, L, k, t, hh, NTIMES);
+    set_times(sm, c, L, k, t, hh, NTIMES, st.jac_offset);
     bool ok = factor(sm, R, maxb, g, FI);
     sm.Fs[0][r] = f0;
     double z = y;
@@ -652,6 +655,7 @@ __global__ void __launch_bounds__(NT) pert_fwd_kernel(const AbcmbCosmo* __restri
       bool conv = false;
       double prev = 0.0;
       for (int it = 0; it < st.max_newton; ++it) {
+        ++iters;
         double fz = rhs_eval(sm, R, slot, z);
         double dz = solve(sm, R, FI, -(z - base - g * fz));
         z += dz;
@@ -709,14 +713,15 @@ __global__ void __launch_bounds__(NT) pert_fwd_kernel(const AbcmbCosmo* __restri
   if (r == 0) {
     status_out[item] = status;
     steps_out[item] = steps;
+    if (iters_out) iters_out[item] = iters;
   }
 }
 
 template <int NT, int BMAX>
 int launch(const AbcmbCosmo* cosmos, const AbcmbPertItem* items, int n_items, const AbcmbOdeSettings& st,
-           int32_t* status, int32_t* steps, double* state, hipStream_t stream) {
+           int32_t* status, int32_t* steps, int32_t* iters, double* state, hipStream_t stream) {
   hipLaunchKernelGGL((pert_fwd_kernel<NT, BMAX>), dim3(n_items), dim3(NT), 0, stream, cosmos, items, st, status,
-                     steps, state);
+                     steps, iters, state);
   ABCMB_HIP_CHECK(hipGetLastError());
   return ABCMB_OK;
 }
@@ -731,8 +736,8 @@ extern "C" size_t abcmb_pert_workspace_size(int n_items) {
 
 extern "C" int abcmb_pert_fwd(const AbcmbCosmo* cosmos_dev, int n_cosmo, const AbcmbPertItem* items_dev,
                               int n_items, const AbcmbOdeSettings* settings, int32_t* status_dev,
-                              int32_t* steps_dev, double* state_dev, void* workspace, size_t ws_bytes,
-                              void* stream) {
+                              int32_t* steps_dev, int32_t* iters_dev, double* state_dev, void* workspace,
+                              size_t ws_bytes, void* stream) {
   using namespace abcmb;
   (void)workspace;
   (void)ws_bytes;
@@ -750,13 +755,13 @@ extern "C" int abcmb_pert_fwd(const AbcmbCosmo* cosmos_dev, int n_cosmo, const A
   ABCMB_REQUIRE(maxb <= 64, "a hierarchy block of %d rows exceeds 64", maxb);
   hipStream_t s = (hipStream_t)stream;
   if (maxb <= 32) {
-    if (st.n <= 64) return launch<64, 32>(cosmos_dev, items_dev, n_items, st, status_dev, steps_dev, state_dev, s);
-    if (st.n <= 128) return launch<128, 32>(cosmos_dev, items_dev, n_items, st, status_dev, steps_dev, state_dev, s);
-    if (st.n <= 192) return launch<192, 32>(cosmos_dev, items_dev, n_items, st, status_dev, steps_dev, state_dev, s);
+    if (st.n <= 64) return launch<64, 32>(cosmos_dev, items_dev, n_items, st, status_dev, steps_dev, iters_dev, state_dev, s);
+    if (st.n <= 128) return launch<128, 32>(cosmos_dev, items_dev, n_items, st, status_dev, steps_dev, iters_dev, state_dev, s);
+    if (st.n <= 192) return launch<192, 32>(cosmos_dev, items_dev, n_items, st, status_dev, steps_dev, iters_dev, state_dev, s);
   } else {
-    if (st.n <= 64) return launch<64, 64>(cosmos_dev, items_dev, n_items, st, status_dev, steps_dev, state_dev, s);
-    if (st.n <= 128) return launch<128, 64>(cosmos_dev, items_dev, n_items, st, status_dev, steps_dev, state_dev, s);
-    if (st.n <= 192) return launch<192, 64>(cosmos_dev, items_dev, n_items, st, status_dev, steps_dev, state_dev, s);
+    if (st.n <= 64) return launch<64, 64>(cosmos_dev, items_dev, n_items, st, status_dev, steps_dev, iters_dev, state_dev, s);
+    if (st.n <= 128) return launch<128, 64>(cosmos_dev, items_dev, n_items, st, status_dev, steps_dev, iters_dev, state_dev, s);
+    if (st.n <= 192) return launch<192, 64>(cosmos_dev, items_dev, n_items, st, status_dev, steps_dev, iters_dev, state_dev, s);
   }
   set_error("state size %d exceeds 192", st.n);
   return ABCMB_EINVAL;

@@ -54,6 +54,7 @@ class CosmoPlan:
     lens_aH: np.ndarray = None
     lens_tau: np.ndarray = None
     chi_rec: float = 0.0
+    k: np.ndarray = None          # PT k-grid of this cosmology (Mpc^-1)
 
 
 class PerturbationTable:
@@ -153,29 +154,46 @@ class Model:
         lay = self.layout
         self.settings = abi.AbcmbOdeSettings(
             pcoeff=specs["pcoeff_PE"], icoeff=specs["icoeff_PE"], dcoeff=specs["dcoeff_PE"], dt0=1e-2,
-            safety=0.9, factormin=0.2, factormax=10.0, newton_kappa=1e-2,
+            safety=0.9, factormin=0.2, factormax=10.0, newton_kappa=1e-2, jac_offset=0.6,
             max_steps=int(specs["max_steps_PE"]), max_newton=10,
             l_g=lay.l_g, l_pol=lay.l_pol, l_ur=lay.l_ur, l_ncdm=lay.l_ncdm, n_q=lay.n_q, n=lay.n)
         if los_precision not in ("fp64", "fp32"):
             raise ValueError("los_precision must be 'fp64' or 'fp32'")
         self.los_precision = 0 if los_precision == "fp64" else 1
+        self.chunk = 128   # cosmologies per downstream launch (bounds the source buffers)
+        self.timing = None  # list: per-launch pert_fwd event timing and work counts when set
         self._dev = None
 
     # ------------------------------------------------------------------ host side
     def derived(self, params: dict) -> dict:
         return add_derived_parameters(params, self.species, self.specs)
 
-    def prepare(self, full_params: dict) -> CosmoPlan:
+    def prepare(self, full_params: dict, freeze: CosmoPlan = None, k: np.ndarray = None) -> CosmoPlan:
+        """Host background and ODE inputs of one cosmology.
+
+        ``freeze``: reuse that plan's discrete grid choices (save grid from
+        lna_transfer_start, lna_rec, per-k start times, recombination switch)
+        so that finite differences see a smooth function of the parameters.
+        ``k``: PT k-grid of this cosmology (default: the model's).
+        """
         sp = self.specs
+        kk = np.ascontiguousarray(self.k_pt if k is None else k, dtype=np.float64)
         bg = Background(full_params, self.has_ncdm, n_q=self.layout.n_q, k_pivot=sp["k_pivot"],
-                        input_tau_reion=sp["input_tau_reion"])
-        lna_save = np.linspace(bg.lna_transfer_start, 0.0, self.n_save)
-        ls = bg.lna_start(self.k_pt, sp["R_tc"], sp["R_large"])
-        big = self.k_pt > sp["k_split_PE"]
+                        input_tau_reion=sp["input_tau_reion"],
+                        rec_switch=freeze.bg.rec_switch if freeze is not None else -1)
+        if freeze is not None:
+            bg.cosmo.lna_rec = freeze.bg.lna_rec
+            bg.cosmo.lna_transfer_start = freeze.bg.lna_transfer_start
+            lna_save = freeze.lna_save
+            ls = freeze.lna_start
+        else:
+            lna_save = np.linspace(bg.lna_transfer_start, 0.0, self.n_save)
+            ls = bg.lna_start(kk, sp["R_tc"], sp["R_large"])
+        big = kk > sp["k_split_PE"]
         rtol = np.where(big, sp["rtol_large_k_PE"], sp["rtol_small_k_PE"])
         atol = np.where(big, sp["atol_large_k_PE"], sp["atol_small_k_PE"])
         bgv = bg.los_vectors(lna_save[:-1])
-        plan = CosmoPlan(full_params, bg, lna_save, ls, rtol, atol, bgv)
+        plan = CosmoPlan(full_params, bg, lna_save, ls, rtol, atol, bgv, k=kk)
         if sp["lensing"]:
             lens_lna = np.linspace(bg.lna_rec, 0.0, 500)
             ev = bg.eval(lens_lna)
@@ -184,6 +202,17 @@ class Model:
             plan.lens_tau = ev["tau"]
             plan.chi_rec = bg.tau0 - float(bg.eval(np.array([bg.lna_rec]))["tau"][0])
         return plan
+
+    def prepare_many(self, params_list, freeze_list=None, k_list=None, threads: int = None):
+        """``prepare`` for many cosmologies on a host thread pool (the producer releases the GIL)."""
+        import os
+        from concurrent.futures import ThreadPoolExecutor
+        n = len(params_list)
+        fz = freeze_list or [None] * n
+        ks = k_list or [None] * n
+        threads = threads or min(16, os.cpu_count() or 1)
+        with ThreadPoolExecutor(threads) as ex:
+            return list(ex.map(lambda i: self.prepare(params_list[i], fz[i], ks[i]), range(n)))
 
     # ------------------------------------------------------------------ device side
     def _device_constants(self):
@@ -207,8 +236,7 @@ class Model:
     def run_batch(self, params_list, derived: bool = False):
         """Outputs for several cosmologies; one ODE launch covers all their k-modes."""
         plist = [p if derived else self.derived(p) for p in params_list]
-        plans = [self.prepare(p) for p in plist]
-        return self.execute(plans)
+        return self.execute(self.prepare_many(plist))
 
     def execute(self, plans, want_state: bool = False, sync: bool = True):
         torch = _torch()
@@ -216,7 +244,7 @@ class Model:
         D = self._device_constants()
         dev = torch.device("cuda")
         stream = torch.cuda.current_stream().cuda_stream
-        nk, ns = self.k_pt.size, self.n_save
+        ns = self.n_save
         ncos = len(plans)
         keep = []  # host/device buffers that must outlive the launches
 
@@ -227,14 +255,19 @@ class Model:
 
         cos_arr = (abi.AbcmbCosmo * ncos)()
         pts = []
+        kdev = []
         for i, pl in enumerate(plans):
+            kk = pl.k if pl.k is not None else self.k_pt
+            kd = D["k_pt"] if kk is self.k_pt else dt(kk)
+            kdev.append(kd)
+            nk = kk.size
             bg = pl.bg
             c = abi.AbcmbCosmo.from_buffer_copy(bg.cosmo)
             c.tau_tab = dt(bg.tau_tab).data_ptr()
             c.xe_tab = dt(bg.xe_tab).data_ptr()
             c.tm_tab = dt(bg.tm_tab).data_ptr()
             c.lna_save = dt(pl.lna_save).data_ptr()
-            c.k = D["k_pt"].data_ptr()
+            c.k = kd.data_ptr()
             c.n_save, c.nk = ns, nk
             pt = torch.empty((abi.N_PT_FIELDS, ns, nk), dtype=torch.float64, device=dev)
             pts.append(pt)
@@ -242,107 +275,116 @@ class Model:
             cos_arr[i] = c
         cos_dev = dt(np.frombuffer(bytes(cos_arr), dtype=np.uint8), torch.uint8)
         # work items: all (cosmology, k), largest k first
-        n_items = ncos * nk
+        allk = [(pl.k if pl.k is not None else self.k_pt) for pl in plans]
+        flat = [(-allk[ci][ik], ci, ik) for ci in range(ncos) for ik in range(allk[ci].size)]
+        flat.sort()
+        n_items = len(flat)
         items = (abi.AbcmbPertItem * n_items)()
-        order = np.argsort(-self.k_pt, kind="stable")
-        m = 0
-        for ik in order:
-            for ci, pl in enumerate(plans):
-                items[m] = abi.AbcmbPertItem(k=self.k_pt[ik], lna_start=pl.lna_start[ik], rtol=pl.rtol[ik],
-                                             atol=pl.atol[ik], cosmo=ci, ik=int(ik))
-                m += 1
+        for m, (_, ci, ik) in enumerate(flat):
+            pl = plans[ci]
+            items[m] = abi.AbcmbPertItem(k=allk[ci][ik], lna_start=pl.lna_start[ik], rtol=pl.rtol[ik],
+                                         atol=pl.atol[ik], cosmo=ci, ik=int(ik))
         items_dev = dt(np.frombuffer(bytes(items), dtype=np.uint8), torch.uint8)
         status = torch.zeros(n_items, dtype=torch.int32, device=dev)
         steps = torch.zeros(n_items, dtype=torch.int32, device=dev)
+        iters = torch.zeros(n_items, dtype=torch.int32, device=dev)
         state = None
         if want_state:
             state = torch.zeros((n_items, ns, self.layout.n), dtype=torch.float64, device=dev)
         st = self.settings
+        ev0 = ev1 = None
+        if self.timing is not None:
+            ev0 = torch.cuda.Event(enable_timing=True)
+            ev1 = torch.cuda.Event(enable_timing=True)
+            ev0.record()
         abi.check(lib.abcmb_pert_fwd(cos_dev.data_ptr(), ncos, items_dev.data_ptr(), n_items, C.byref(st),
-                                     status.data_ptr(), steps.data_ptr(),
+                                     status.data_ptr(), steps.data_ptr(), iters.data_ptr(),
                                      state.data_ptr() if state is not None else None, None, 0, stream),
                   "abcmb_pert_fwd")
+        if ev1 is not None:
+            ev1.record()
         outs = []
         nkt = self.k_transfer.size
         n_ell = self.ell_idx.size
         n_tau = ns - 1
-        ws_src = torch.empty(lib.abcmb_sources_workspace_size(nk, ns) // 8 + 1, dtype=torch.float64, device=dev)
-        ws_los = torch.empty(lib.abcmb_los_workspace_size(n_ell, nkt) // 8 + 1, dtype=torch.float64, device=dev)
-        src = torch.empty((4, n_tau, nkt), dtype=torch.float64, device=dev)
         nl = self.lensing_ells.size
-        ws_lens = None
-        if self.specs["lensing"]:
-            ws_lens = torch.empty(lib.abcmb_lens_workspace_size(self.mu.size, nl) // 8 + 1, dtype=torch.float64,
-                                  device=dev)
+        nq = self.k_pk.size
+        lensing = bool(self.specs["lensing"])
+        nkmax = max(a.size for a in allk)
         sc = self.scales.ctypes.data_as(C.c_void_p)
-        cls_dev = []
-        for i, pl in enumerate(plans):
-            p = pl.params
-            pt = pts[i]
-            bgv = dt(pl.bgv)
-            dl = float(pl.lna_save[-1] - pl.lna_save[-2])
-            abi.check(lib.abcmb_sources(pt.data_ptr(), D["k_pt"].data_ptr(), nk, ns, bgv.data_ptr(), dl,
-                                        D["kt"].data_ptr(), nkt, sc, src.data_ptr(), ws_src.data_ptr(),
-                                        ws_src.numel() * 8, stream), "abcmb_sources")
-            cl_raw = torch.empty((3, n_ell), dtype=torch.float64, device=dev)
-            abi.check(lib.abcmb_los_cl(src.data_ptr(), n_tau, bgv.data_ptr(), pl.bg.tau0, D["kt"].data_ptr(), nkt,
+        cl_raw = torch.empty((ncos, 3, n_ell), dtype=torch.float64, device=dev)
+        cl_full = torch.empty((ncos, 3, nl), dtype=torch.float64, device=dev)
+        pk = torch.empty((ncos, nq), dtype=torch.float64, device=dev)
+        clpp = lensed = None
+        bgv_all = dt(np.stack([pl.bgv for pl in plans]))
+        if lensing:
+            lens_h = np.zeros((ncos, 4, plans[0].lens_lna.size))
+            for i, pl in enumerate(plans):
+                lens_h[i, 0], lens_h[i, 1], lens_h[i, 2] = pl.lens_lna, pl.lens_aH, pl.lens_tau
+                lens_h[i, 3, 0] = pl.chi_rec
+            lens_all = dt(lens_h)
+            clpp = torch.empty((ncos, nl), dtype=torch.float64, device=dev)
+            lensed = torch.empty((ncos, 4, nl), dtype=torch.float64, device=dev)
+        csz = C.sizeof(abi.AbcmbCosmo)
+        chunk = max(1, min(ncos, self.chunk))
+        bc = min(chunk, ncos)
+        ws_src = torch.empty(lib.abcmb_sources_workspace_size(bc, nkmax, ns) // 8 + 1, dtype=torch.float64, device=dev)
+        ws_los = torch.empty(lib.abcmb_los_workspace_size(bc, n_ell, nkt) // 8 + 1, dtype=torch.float64, device=dev)
+        src = torch.empty((bc, 4, n_tau, nkt), dtype=torch.float64, device=dev)
+        ws_lens = None
+        if lensing:
+            ws_lens = torch.empty(lib.abcmb_lens_workspace_size(bc, self.mu.size, nl) // 8 + 1, dtype=torch.float64,
+                                  device=dev)
+        for c0 in range(0, ncos, chunk):
+            nb = min(chunk, ncos - c0)
+            cp = cos_dev.data_ptr() + c0 * csz
+            abi.check(lib.abcmb_sources(cp, nb, nkmax, ns, bgv_all[c0].data_ptr(), D["kt"].data_ptr(), nkt, sc,
+                                        src.data_ptr(), ws_src.data_ptr(), ws_src.numel() * 8, stream),
+                      "abcmb_sources")
+            abi.check(lib.abcmb_los_cl(cp, nb, src.data_ptr(), n_tau, bgv_all[c0].data_ptr(), D["kt"].data_ptr(), nkt,
                                        D["bes"].data_ptr(), D["rng"].data_ptr(), D["bes_l"].data_ptr(),
                                        self.bes_l.size, self.bes_tab.shape[2], D["ell_idx"].data_ptr(), n_ell,
-                                       p["A_s"], p["n_s"], self.specs["k_pivot"], self.los_precision,
-                                       cl_raw.data_ptr(), None, ws_los.data_ptr(), ws_los.numel() * 8, stream),
-                      "abcmb_los_cl")
-            cl_full = torch.empty((3, nl), dtype=torch.float64, device=dev)
-            abi.check(lib.abcmb_apply_operator(D["ell_op"].data_ptr(), nl, n_ell, cl_raw.data_ptr(), 3,
-                                               cl_full.data_ptr(), stream), "abcmb_apply_operator")
-            clpp = lensed = None
-            if self.specs["lensing"]:
-                c = cos_arr[i]
-                lna_ax, aH_ax, tau_ax = dt(pl.lens_lna), dt(pl.lens_aH), dt(pl.lens_tau)
-                clpp = torch.empty(nl, dtype=torch.float64, device=dev)
-                abi.check(lib.abcmb_lensing_clpp(C.byref(c), pt.data_ptr(), lna_ax.data_ptr(), aH_ax.data_ptr(),
-                                                 tau_ax.data_ptr(), pl.lens_lna.size, pl.chi_rec,
-                                                 D["lens_ells"].data_ptr(), nl, clpp.data_ptr(), stream),
+                                       self.los_precision, cl_raw[c0].data_ptr(), None, ws_los.data_ptr(),
+                                       ws_los.numel() * 8, stream), "abcmb_los_cl")
+            abi.check(lib.abcmb_apply_operator(D["ell_op"].data_ptr(), nl, n_ell, cl_raw[c0].data_ptr(), 3 * nb,
+                                               cl_full[c0].data_ptr(), stream), "abcmb_apply_operator")
+            if lensing:
+                abi.check(lib.abcmb_lensing_clpp(cp, nb, lens_all[c0].data_ptr(), plans[0].lens_lna.size,
+                                                 D["lens_ells"].data_ptr(), nl, clpp[c0].data_ptr(), stream),
                           "abcmb_lensing_clpp")
-                lensed = torch.empty((4, nl), dtype=torch.float64, device=dev)
-                abi.check(lib.abcmb_lens(cl_full.data_ptr(), clpp.data_ptr(), nl, D["mu"].data_ptr(),
-                                         D["w"].data_ptr(), self.mu.size, lensed.data_ptr(), ws_lens.data_ptr(),
+                abi.check(lib.abcmb_lens(nb, cl_full[c0].data_ptr(), clpp[c0].data_ptr(), nl, D["mu"].data_ptr(),
+                                         D["w"].data_ptr(), self.mu.size, lensed[c0].data_ptr(), ws_lens.data_ptr(),
                                          ws_lens.numel() * 8, stream), "abcmb_lens")
-            pk = torch.empty(self.k_pk.size, dtype=torch.float64, device=dev)
-            lna_save_dev = dt(pl.lna_save)
-            abi.check(lib.abcmb_pk_lin(pt.data_ptr(), D["k_pt"].data_ptr(), nk, ns, lna_save_dev.data_ptr(),
-                                       D["k_pk"].data_ptr(), self.k_pk.size, p["A_s"], p["n_s"],
-                                       self.specs["k_pivot"], pk.data_ptr(), stream), "abcmb_pk_lin")
-            cls_dev.append((cl_raw, cl_full, clpp, lensed, pk))
+            abi.check(lib.abcmb_pk_lin(cp, nb, D["k_pk"].data_ptr(), nq, pk[c0].data_ptr(), stream), "abcmb_pk_lin")
         if sync:
             torch.cuda.synchronize()
         st_h = status.cpu().numpy()
         steps_h = steps.cpu().numpy()
+        if self.timing is not None:
+            self.timing.append(dict(ms=ev0.elapsed_time(ev1), steps=int(steps_h.sum()),
+                                    iters=int(iters.sum().item()), items=n_items))
         bad = np.nonzero(st_h != abi_k_ok())[0]
         if bad.size:
             raise RuntimeError(f"perturbation ODE failed for {bad.size} (cosmology, k) items; status codes "
                                f"{np.unique(st_h[bad]).tolist()} (1 = max_steps_PE reached, 2 = non-finite)")
         n_l = self.ells.size
         # per-cosmology step counts in PT k order
-        steps_ck = np.zeros((ncos, nk), dtype=np.int32)
-        m = 0
-        for ik in order:
-            for ci in range(ncos):
-                steps_ck[ci, ik] = steps_h[m]
-                m += 1
+        steps_ck = [np.zeros(a.size, dtype=np.int32) for a in allk]
+        for m, (_, ci, ik) in enumerate(flat):
+            steps_ck[ci][ik] = steps_h[m]
         for i, pl in enumerate(plans):
-            cl_raw, cl_full, clpp, lensed, pk = cls_dev[i]
-            if lensed is not None:
-                TT, TE, EE, BB = lensed[0, :n_l], lensed[1, :n_l], lensed[2, :n_l], lensed[3, :n_l]
+            if lensing:
+                TT, TE, EE, BB = lensed[i, 0, :n_l], lensed[i, 1, :n_l], lensed[i, 2, :n_l], lensed[i, 3, :n_l]
             else:
-                TT, TE, EE, BB = cl_full[0, :n_l], cl_full[1, :n_l], cl_full[2, :n_l], None
-            PT = PerturbationTable(self.k_pt, pl.lna_save, pts[i], self.has_ncdm)
-            out = Output(TT, TE, EE, pk, self.ells, self.k_pk, pl.bg, PT, pl.params, ClBB=BB, Clpp=clpp,
-                         steps=steps_ck[i])
-            out.cl_raw = cl_raw
-            out.cl_unlensed = cl_full
+                TT, TE, EE, BB = cl_full[i, 0, :n_l], cl_full[i, 1, :n_l], cl_full[i, 2, :n_l], None
+            PT = PerturbationTable(allk[i], pl.lna_save, pts[i], self.has_ncdm)
+            out = Output(TT, TE, EE, pk[i], self.ells, self.k_pk, pl.bg, PT, pl.params, ClBB=BB,
+                         Clpp=clpp[i] if lensing else None, steps=steps_ck[i])
+            out.cl_raw = cl_raw[i]
+            out.cl_unlensed = cl_full[i]
             outs.append(out)
         if want_state:
-            return outs, state, order
+            return outs, state, flat
         return outs
 
     def __call__(self, params: dict = {}) -> Output:

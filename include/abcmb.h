@@ -87,6 +87,7 @@ typedef struct AbcmbPertItem {
 typedef struct AbcmbOdeSettings {
   double pcoeff, icoeff, dcoeff, dt0;
   double safety, factormin, factormax, newton_kappa;
+  double jac_offset;                /* chord Jacobian at t_n + jac_offset * h */
   int32_t max_steps, max_newton;
   int32_t l_g, l_pol, l_ur, l_ncdm; /* hierarchy cut-offs */
   int32_t n_q, n;                   /* massive bins (0 if absent), state size */
@@ -102,11 +103,16 @@ typedef struct AbcmbOdeSettings {
  * lna_transfer_start (abcmb/background.py:555-564).  c->tau_tab, xe_tab and
  * tm_tab must point to writable host arrays of ABCMB_TAU_TAB_N, xe_n and
  * tm_n doubles (xe_n = tm_n >= abcmb_host_recomb_points()).
- * tau_reion > 0 selects the tau-input reionization branch.
+ * tau_reion > 0 selects the tau-input reionization branch.  rec_switch
+ * (in/out, may be NULL): grid index where the hydrogen history leaves Saha
+ * equilibrium; < 0 on input lets the producer choose it (and reports it),
+ * >= 0 forces it (finite differences hold it fixed, like every other
+ * discrete grid choice).
  */
 int abcmb_host_background(AbcmbCosmo* c, double tau_reion, double z_reion_in,
                           double Delta_z_reion, double z_reion_He,
-                          double Delta_z_reion_He, double exp_reion);
+                          double Delta_z_reion_He, double exp_reion,
+                          int32_t* rec_switch);
 int abcmb_host_recomb_points(void);
 
 /*
@@ -134,26 +140,34 @@ int abcmb_device_init(void);
  * with evolution_one_k (:250-316) and make_output_table (:318-404).
  * cosmos_dev: n_cosmo device-resident AbcmbCosmo; items_dev: n_items;
  * status_dev/steps_dev: int32[n_items] outputs (status, accepted+rejected
- * steps); state_dev: optional [n_items][n_save][n] raw states (NULL = off).
+ * steps); iters_dev (optional): int32[n_items] chord-Newton iterations;
+ * state_dev: optional [n_items][n_save][n] raw states (NULL = off).
  */
 size_t abcmb_pert_workspace_size(int n_items);
 int abcmb_pert_fwd(const AbcmbCosmo* cosmos_dev, int n_cosmo,
                    const AbcmbPertItem* items_dev, int n_items,
                    const AbcmbOdeSettings* settings,
-                   int32_t* status_dev, int32_t* steps_dev, double* state_dev,
-                   void* workspace, size_t ws_bytes, void* stream);
+                   int32_t* status_dev, int32_t* steps_dev, int32_t* iters_dev,
+                   double* state_dev, void* workspace, size_t ws_bytes, void* stream);
+
+/*
+ * The entry points below work on a batch of n_cosmo cosmologies described by
+ * the device-resident AbcmbCosmo array (their perturbation tables, k-grids,
+ * save grids and parameters); per-cosmology buffers are contiguous blocks of
+ * the stated shape, one after another.
+ */
 
 /*
  * Cubic-spline interpolation (not-a-knot, in log10 k) of the 10 LOS fields
- * of a perturbation table from its k grid to the transfer grid, fused with
+ * of each perturbation table from its k grid to the transfer grid, fused with
  * the source functions S_T0, S_T1, S_T2, S_E and the LOS quadrature weight
- * (abcmb/spectrum.py:664-696, :755-758); dlna is the save-grid spacing.
- * bgv_dev: [6][n_save-1] from abcmb_host_los_background; kt_dev: [nkt].
- * src_dev out: [4][n_save-1][nkt].  scales: sw, isw, dop, pol.
+ * w_i / aH_i (abcmb/spectrum.py:664-696, :755-772).
+ * bgv_dev: [B][6][n_save-1] from abcmb_host_los_background; kt_dev: [nkt];
+ * src_dev out: [B][4][n_save-1][nkt]; scales4: sw, isw, dop, pol (host).
  */
-size_t abcmb_sources_workspace_size(int nk, int n_save);
-int abcmb_sources(const double* pt_dev, const double* k_dev, int nk, int n_save,
-                  const double* bgv_dev, double dlna, const double* kt_dev, int nkt,
+size_t abcmb_sources_workspace_size(int n_cosmo, int nk_max, int n_save);
+int abcmb_sources(const AbcmbCosmo* cosmos_dev, int n_cosmo, int nk_max, int n_save,
+                  const double* bgv_dev, const double* kt_dev, int nkt,
                   const double* scales4, double* src_dev,
                   void* workspace, size_t ws_bytes, void* stream);
 
@@ -162,17 +176,16 @@ int abcmb_sources(const double* pt_dev, const double* k_dev, int nk, int n_save,
  * k-quadrature (abcmb/spectrum.py:616-797, Bessel lookups :48-109).
  * bes_dev: tables [3][n_lt][nx] (phi0, phi1, phi2), bes_rng_dev: [3][n_lt][2]
  * x ranges, bes_l_dev: int32[n_lt] multipoles; ell_idx_dev: int32[n_ell]
- * rows of the Bessel tables to use; tau_los_dev: tau at the n_save-1 LOS
- * points.  precision: 0 = FP64, 1 = FP32.  cl_dev out: [3][n_ell]
- * (TT, TE, EE); delta_dev (optional) out: [4][n_ell][nkt].
+ * rows of the Bessel tables to use.  precision: 0 = FP64, 1 = FP32.
+ * cl_dev out: [B][3][n_ell] (TT, TE, EE); delta_dev (optional) out:
+ * [B][4][n_ell][nkt].
  */
-size_t abcmb_los_workspace_size(int n_ell, int nkt);
-int abcmb_los_cl(const double* src_dev, int n_tau, const double* tau_los_dev,
-                 double tau0, const double* kt_dev, int nkt,
+size_t abcmb_los_workspace_size(int n_cosmo, int n_ell, int nkt);
+int abcmb_los_cl(const AbcmbCosmo* cosmos_dev, int n_cosmo, const double* src_dev,
+                 int n_tau, const double* bgv_dev, const double* kt_dev, int nkt,
                  const double* bes_dev, const double* bes_rng_dev,
                  const int32_t* bes_l_dev, int n_lt, int nx,
-                 const int32_t* ell_idx_dev, int n_ell,
-                 double A_s, double n_s, double k_pivot, int precision,
+                 const int32_t* ell_idx_dev, int n_ell, int precision,
                  double* cl_dev, double* delta_dev,
                  void* workspace, size_t ws_bytes, void* stream);
 
@@ -182,25 +195,27 @@ int abcmb_apply_operator(const double* op_dev, int rows, int cols,
                          const double* in_dev, int n_vec, double* out_dev,
                          void* stream);
 
-/* Limber lensing potential C_l^phiphi (abcmb/spectrum.py:349-435). */
-int abcmb_lensing_clpp(const AbcmbCosmo* c_host, const double* pt_dev,
-                       const double* lna_dev, const double* aH_dev,
-                       const double* tau_dev, int n_lna, double chi_rec,
+/* Limber lensing potential C_l^phiphi (abcmb/spectrum.py:349-435).
+ * lens_dev: [B][4][n_lna] = ln a axis (lna_rec .. 0), aH and tau on it, and
+ * chi(lna_rec) at [3][0]; ells_dev: [n_l]; clpp_dev out: [B][n_l]. */
+int abcmb_lensing_clpp(const AbcmbCosmo* cosmos_dev, int n_cosmo,
+                       const double* lens_dev, int n_lna,
                        const double* ells_dev, int n_l, double* clpp_dev,
                        void* stream);
 
 /* Correlation-function lensing of TT/TE/EE (+BB) (abcmb/spectrum.py:437-572,
- * Wigner-d recursions abcmb/ABCMBTools.py:16-185).  cl_in: [3][n_l] unlensed
- * on l = 2..n_l+1; out: [4][n_l] lensed TT, TE, EE, BB. */
-size_t abcmb_lens_workspace_size(int n_mu, int n_l);
-int abcmb_lens(const double* cl_in_dev, const double* clpp_dev, int n_l,
+ * Wigner-d recursions abcmb/ABCMBTools.py:16-185).  cl_in: [B][3][n_l]
+ * unlensed on l = 2..n_l+1; clpp: [B][n_l]; mu, w: Gauss-Legendre nodes with
+ * mu = 1 (weight 0) appended; out: [B][4][n_l] lensed TT, TE, EE, BB. */
+size_t abcmb_lens_workspace_size(int n_cosmo, int n_mu, int n_l);
+int abcmb_lens(int n_cosmo, const double* cl_in_dev, const double* clpp_dev, int n_l,
                const double* mu_dev, const double* w_dev, int n_mu,
                double* out_dev, void* workspace, size_t ws_bytes, void* stream);
 
-/* Linear matter power spectrum at z = 0 (abcmb/spectrum.py:268-302). */
-int abcmb_pk_lin(const double* pt_dev, const double* k_dev, int nk, int n_save,
-                 const double* lna_save_dev, const double* kq_dev, int nq, double A_s, double n_s,
-                 double k_pivot, double* pk_dev, void* stream);
+/* Linear matter power spectrum at z = 0 (abcmb/spectrum.py:268-302):
+ * pk_dev out [B][nq] on the query grid kq_dev [nq]. */
+int abcmb_pk_lin(const AbcmbCosmo* cosmos_dev, int n_cosmo, const double* kq_dev, int nq,
+                 double* pk_dev, void* stream);
 
 #ifdef __cplusplus
 }

@@ -13,9 +13,12 @@
  * details below are this project's and are stated in the design notes:
  *   - error order 5 in the PID exponents, safety 0.9, factor in
  *     [0.2 (rejected) or 1 (accepted), 10];
- *   - chord Newton on each implicit stage with the matrix I - h*gamma*J(t_n),
- *     J the exact (linear) Jacobian, stage predictor base + h*gamma*F_{i-1},
- *     convergence theta/(1-theta)*|dz| < kappa, divergence theta >= 1;
+ *   - chord Newton on each implicit stage with the matrix
+ *     I - h*gamma*J(t_n + jac_offset*h) (jac_offset = 0.6: the middle of the
+ *     stage abscissae 0..1.23), J the exact (linear) Jacobian, stage
+ *     predictor base + h*gamma*F_{i-1}, convergence
+ *     theta/(1-theta)*|dz| < kappa after two or more iterations (or
+ *     |dz| < 1e-12), divergence theta > 2;
  *   - stage derivative F_i = (z_i - base_i) / (h*gamma);
  *   - cubic Hermite dense output with exact end derivatives.
  * Linear solves use LU with partial pivoting (the GPU uses Gauss-Jordan
@@ -437,7 +440,7 @@ static int ode_one_k(const Ctx* x, double lna0, double rtol, double atol, const 
     for (int j = 0; j < n; ++j) {
       memset(e, 0, sizeof(double) * n);
       e[j] = 1.0;
-      rhs(x, t, e, tmp);
+      rhs(x, t + st->jac_offset * hh, e, tmp);
       for (int i = 0; i < n; ++i) M[i * n + j] = (i == j ? 1.0 : 0.0) - hh * KV_G * tmp[i];
     }
     int ok = lu_factor(M, n, piv) == 0;

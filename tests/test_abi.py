@@ -43,7 +43,7 @@ def test_struct_layouts_match_header():
     # the ctypes mirrors must match the C sizes used by the kernels
     assert C.sizeof(abi.AbcmbCosmo) == 320
     assert C.sizeof(abi.AbcmbPertItem) == 48
-    assert C.sizeof(abi.AbcmbOdeSettings) == 8 * 8 + 8 * 4
+    assert C.sizeof(abi.AbcmbOdeSettings) == 9 * 8 + 8 * 4
 
 
 def test_hot_path_refuses_missing_library(monkeypatch, tmp_path):

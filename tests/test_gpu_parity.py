@@ -1,0 +1,172 @@
+"""GPU parity: the HIP path through the C ABI against the CPU oracle.
+
+Sizes follow the workloads: config 1 (200 k, massless nu, unlensed l <= 1000)
+end to end; config 2 (massive nu, 500 k, lensed l <= 2500) on a k subset of
+the ODE and through size-independent properties at full size; lensing end to
+end at l <= 1000.  Tolerances:
+  - ODE: the same accepted/rejected step count on at least 95% of the
+    k-modes (the chord-Newton iterates of the GPU (Gauss-Jordan block inverse
+    + Woodbury) and of the oracle (LU with pivoting) differ at rounding level,
+    which can flip an accept/reject decision that sits on the threshold) and
+    perturbation-table fields within 1e-4 of each field's maximum;
+  - C_l (FP64 LOS): relative 1e-6 on TT and EE, 1e-6 of max|TE| on TE;
+  - lensed C_l: relative 1e-6 (TT, EE), 1e-6 of max on TE and BB;
+  - P(k): relative 1e-7.
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+P0 = {"h": 0.6736, "omega_b": 0.02237, "omega_cdm": 0.1200, "A_s": 2.1e-9, "n_s": 0.9649, "tau_reion": 0.0544}
+
+
+def _say(msg):
+    print(msg, flush=True)
+
+
+def _same_steps(a, b):
+    a = np.asarray(a); b = np.asarray(b)
+    return np.mean(a == b) >= 0.95 and np.max(np.abs(a - b) / b) <= 0.05
+
+
+@pytest.fixture(scope="module")
+def c1_run():
+    from abcmb_amd.model import Model
+    from oracle import pipeline
+    m = Model(l_max=1000, l_max_g=12, l_max_pol_g=12, l_max_massless_nu=12,
+              k_axis_perturbations=np.geomspace(1e-4, 0.3, 200))
+    plan = m.prepare(m.derived(dict(P0, Neff=3.044)))
+    _say("case: config1 fp64 end to end")
+    g = m.execute([plan])[0]
+    o = pipeline.run(m, plan)
+    return m, plan, g, o
+
+
+def test_config1_ode_parity(c1_run):
+    m, plan, g, o = c1_run
+    assert _same_steps(g.steps, o["steps"])
+    pt_g = g.PT.table.cpu().numpy()
+    pt_o = o["pt"]
+    scale = np.abs(pt_o).max(axis=1, keepdims=True) + 1e-300
+    assert np.max(np.abs(pt_g - pt_o) / scale) < 1e-4
+
+
+def test_config1_cl_and_pk_parity(c1_run):
+    m, plan, g, o = c1_run
+    clg = g.cl_unlensed.cpu().numpy()
+    clo = o["cl_full"]
+    assert np.max(np.abs(clg[0] / clo[0] - 1)) < 1e-6
+    assert np.max(np.abs(clg[2] / clo[2] - 1)) < 1e-6
+    assert np.max(np.abs(clg[1] - clo[1])) < 1e-6 * np.max(np.abs(clo[1]))
+    assert np.max(np.abs(g.Pk.cpu().numpy() / o["pk"] - 1)) < 1e-7
+    # the API returns l = 2..l_max
+    assert g.l[0] == 2 and g.l[-1] == 1000 and g.ClTT.shape[0] == 999
+
+
+def test_config1_fp32_los_error(c1_run):
+    from abcmb_amd.model import Model
+    m, plan, g, o = c1_run
+    m32 = Model(l_max=1000, l_max_g=12, l_max_pol_g=12, l_max_massless_nu=12,
+                k_axis_perturbations=np.geomspace(1e-4, 0.3, 200), los_precision="fp32")
+    _say("case: config1 fp32 LOS")
+    g32 = m32.execute([plan])[0]
+    c64 = g.cl_unlensed.cpu().numpy()
+    c32 = g32.cl_unlensed.cpu().numpy()
+    err_tt = np.max(np.abs(c32[0] / c64[0] - 1))
+    err_ee = np.max(np.abs(c32[2] / c64[2] - 1))
+    err_te = np.max(np.abs(c32[1] - c64[1])) / np.max(np.abs(c64[1]))
+    _say(f"fp32 LOS vs fp64: TT {err_tt:.2e} EE {err_ee:.2e} TE {err_te:.2e}")
+    assert err_tt < 1e-3 and err_ee < 1e-3 and err_te < 1e-3
+
+
+def test_batch_equals_single(c1_run):
+    from abcmb_amd.model import Model
+    m, plan, g, o = c1_run
+    p2 = m.derived(dict(P0, Neff=3.044, omega_b=0.0215, h=0.70))
+    plan2 = m.prepare(p2)
+    _say("case: batch of 3 cosmologies")
+    outs = m.execute([plan2, plan, plan2])
+    assert np.array_equal(outs[1].ClTT.cpu().numpy(), g.ClTT.cpu().numpy())
+    assert np.array_equal(outs[1].PT.table.cpu().numpy(), g.PT.table.cpu().numpy())
+    single2 = m.execute([plan2])[0]
+    assert np.array_equal(outs[0].ClEE.cpu().numpy(), single2.ClEE.cpu().numpy())
+    assert np.array_equal(outs[2].ClEE.cpu().numpy(), single2.ClEE.cpu().numpy())
+    assert not np.array_equal(outs[0].ClTT.cpu().numpy(), outs[1].ClTT.cpu().numpy())
+
+
+def test_lensing_parity_l1000():
+    from abcmb_amd.model import Model
+    from oracle import pipeline
+    m = Model(l_max=1000, lensing=True, l_max_g=12, l_max_pol_g=12, l_max_massless_nu=12,
+              k_axis_perturbations=np.geomspace(1e-4, 0.6, 220))
+    plan = m.prepare(m.derived(dict(P0, Neff=3.044)))
+    _say("case: lensing l<=1000")
+    g = m.execute([plan])[0]
+    o = pipeline.run(m, plan)
+    assert _same_steps(g.steps, o["steps"])
+    clpp_g = g.Clpp.cpu().numpy()
+    assert np.max(np.abs(clpp_g / o["clpp"] - 1)) < 1e-6
+    lens = o["lensed"][:, :g.l.size]
+    assert np.max(np.abs(g.ClTT.cpu().numpy() / lens[0] - 1)) < 1e-6
+    assert np.max(np.abs(g.ClEE.cpu().numpy() / lens[2] - 1)) < 1e-6
+    assert np.max(np.abs(g.ClTE.cpu().numpy() - lens[1])) < 1e-6 * np.max(np.abs(lens[1]))
+    assert np.max(np.abs(g.ClBB.cpu().numpy() - lens[3])) < 1e-6 * np.max(np.abs(lens[3]))
+    # lensing smooths the acoustic peaks by a few per cent and makes BB > 0
+    unl = g.cl_unlensed.cpu().numpy()[0, :g.l.size]
+    assert 1e-4 < np.max(np.abs(g.ClTT.cpu().numpy() / unl - 1)) < 0.1
+    assert np.all(g.ClBB.cpu().numpy()[10:] > 0)
+
+
+@pytest.fixture(scope="module")
+def c2_run():
+    from abcmb_amd.model import Model, MassiveNeutrino
+    m = Model(user_species=(MassiveNeutrino,), k_axis_perturbations=np.geomspace(1e-5, 0.5, 500), lensing=True)
+    plan = m.prepare(m.derived(dict(P0, N_nu_massive=1, m_nu_massive=0.06)))
+    _say("case: config2 massive nu, 500 k, lensed l<=2500")
+    g = m.execute([plan])[0]
+    return m, plan, g
+
+
+def test_config2_massive_nu_ode_parity(c2_run):
+    from oracle import pipeline
+    m, plan, g = c2_run
+    assert m.layout.n == 100
+    sub = np.arange(0, 500, 25)
+    o = pipeline.run(m, plan, k_subset=sub)
+    assert _same_steps(g.steps[sub], o["steps"])
+    ptg = g.PT.table.cpu().numpy()[:, :, sub]
+    pto = o["pt"][:, :, sub]
+    scale = np.abs(pto).max(axis=1, keepdims=True) + 1e-300
+    assert np.max(np.abs(ptg - pto) / scale) < 1e-4
+
+
+def test_config2_full_size_properties(c2_run):
+    m, plan, g = c2_run
+    tt, ee, bb, te = (x.cpu().numpy() for x in (g.ClTT, g.ClEE, g.ClBB, g.ClTE))
+    l = g.l.astype(float)
+    assert tt.shape == (2499,) and np.all(np.isfinite(tt)) and np.all(tt > 0) and np.all(ee > 0)
+    assert np.all(bb[20:] > 0) and np.max(bb / ee) < 0.1
+    dtt = l * (l + 1) * tt / (2 * np.pi) * (2.7255e6) ** 2
+    assert 200 < l[np.argmax(dtt)] < 240
+    # the massive species' density contrast tracks CDM on large scales today
+    pt = g.PT.species_perturbations
+    dn = pt["MassiveNeutrino"]["delta"].cpu().numpy()[-1, :50]
+    dc = pt["ColdDarkMatter"]["delta"].cpu().numpy()[-1, :50]
+    assert np.max(np.abs(dn / dc - 1)) < 0.05
+    assert np.all(g.Pk.cpu().numpy() > 0)
+
+
+def test_max_steps_raises():
+    from abcmb_amd.model import Model
+    m = Model(l_max=1000, l_max_g=12, l_max_pol_g=12, l_max_massless_nu=12, max_steps_PE=10,
+              k_axis_perturbations=np.geomspace(1e-4, 0.3, 20))
+    plan = m.prepare(m.derived(dict(P0, Neff=3.044)))
+    _say("case: max_steps_PE=10 must raise")
+    with pytest.raises(RuntimeError, match="max_steps"):
+        m.execute([plan])
+
+
+def test_native_library_is_the_path():
+    from abcmb_amd import abi
+    assert abi._hip is not None and abi._hip._name.endswith("libabcmb_hip.so")
