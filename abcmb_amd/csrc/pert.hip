@@ -298,6 +298,8 @@ __device__ __noinline__ void fill_scalars(const AbcmbCosmo& c, const Layout& L, 
 // below 1e-12 is reported as a failed step), and (I - g A)^-1 is applied
 // with the Woodbury identity through a 2x2 capacitance matrix.
 
+// chord updates below this (in units of the error tolerance) are rounding noise
+constexpr double NEWTON_SMALL = 1e-9;
 constexpr int NTIMES = 7;  // t, the distinct stage abscissae t + c_s h, and the chord-Jacobian time
 constexpr int JAC_SLOT = 6;
 __constant__ double TIME_C[NTIMES - 1] = {0.0, 0.52, 1.230333209967908, 0.8957659843500759, 0.43639360985864756,
@@ -638,6 +640,7 @@ __global__ void __launch_bounds__(NT) pert_fwd_kernel(const AbcmbCosmo* __restri
     double tn = t + h;
     if (tn > 0.0) tn = 0.0;
     const double hh = tn - t;
+    if (!(hh > 0.0)) { status = ABCMB_K_STEP_UNDERFLOW; break; }
     ++steps;
     const double g = hh * KV_G;
     set_times(sm, c, L, k, t, hh, NTIMES, st.jac_offset);
@@ -664,7 +667,7 @@ __global__ void __launch_bounds__(NT) pert_fwd_kernel(const AbcmbCosmo* __restri
         block_sum<NT, 1>(v, sm.red);
         double nrm = sqrt(v[0] / n);
         if (!isfinite(nrm)) break;
-        if (nrm < 1e-12) { conv = true; break; }
+        if (nrm < NEWTON_SMALL) { conv = true; break; }
         if (it > 0) {
           double th = nrm / prev;
           if (th > 2.0) break;

@@ -365,8 +365,10 @@ class Model:
                                     iters=int(iters.sum().item()), items=n_items))
         bad = np.nonzero(st_h != abi_k_ok())[0]
         if bad.size:
+            where = [(flat[b][1], float(allk[flat[b][1]][flat[b][2]])) for b in bad[:8]]
             raise RuntimeError(f"perturbation ODE failed for {bad.size} (cosmology, k) items; status codes "
-                               f"{np.unique(st_h[bad]).tolist()} (1 = max_steps_PE reached, 2 = non-finite)")
+                               f"{np.unique(st_h[bad]).tolist()} (1 = max_steps_PE reached, 2 = non-finite, 3 = step-size underflow); "
+                               f"first (cosmology, k): {where}")
         n_l = self.ells.size
         # per-cosmology step counts in PT k order
         steps_ck = [np.zeros(a.size, dtype=np.int32) for a in allk]

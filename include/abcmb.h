@@ -34,6 +34,7 @@ extern "C" {
 #define ABCMB_K_OK 0
 #define ABCMB_K_MAX_STEPS 1   /* max_steps reached before ln a = 0 */
 #define ABCMB_K_NONFINITE 2   /* NaN/Inf in state or error norm */
+#define ABCMB_K_STEP_UNDERFLOW 3  /* step size fell below the spacing of ln a */
 
 #define ABCMB_TAU_TAB_N 10000      /* tau(ln a) table on [-33, 0] */
 #define ABCMB_TAU_TAB_LO (-33.0)

@@ -18,7 +18,7 @@
  *     stage abscissae 0..1.23), J the exact (linear) Jacobian, stage
  *     predictor base + h*gamma*F_{i-1}, convergence
  *     theta/(1-theta)*|dz| < kappa after two or more iterations (or
- *     |dz| < 1e-12), divergence theta > 2;
+ *     |dz| < 1e-9, the rounding floor), divergence theta > 2;
  *   - stage derivative F_i = (z_i - base_i) / (h*gamma);
  *   - cubic Hermite dense output with exact end derivatives.
  * Linear solves use LU with partial pivoting (the GPU uses Gauss-Jordan
@@ -358,6 +358,10 @@ static void lu_solve(const double* A, int n, const int* piv, double* b) {
   }
 }
 
+/* chord updates below this (in units of the error tolerance) are rounding noise:
+ * u / rtol is ~2e-11 for rtol = 1e-5, so a smaller threshold could stall */
+#define NEWTON_SMALL 1e-9
+
 /* ---------- Kvaerno5 tableau (Kvaerno 2004) ---------- */
 static const double KV_G = 0.26;
 static const double KV_C[7] = {0.0, 0.52, 1.230333209967908, 0.8957659843500759, 0.43639360985864756, 1.0, 1.0};
@@ -435,6 +439,7 @@ static int ode_one_k(const Ctx* x, double lna0, double rtol, double atol, const 
     double tn = t + h;
     if (tn > 0.0) tn = 0.0;
     double hh = tn - t;
+    if (!(hh > 0.0)) { status = ABCMB_K_STEP_UNDERFLOW; break; }
     ++steps;
     /* M = I - hh*gamma*J(t) with J columns from the linear RHS */
     for (int j = 0; j < n; ++j) {
@@ -462,7 +467,7 @@ static int ode_one_k(const Ctx* x, double lna0, double rtol, double atol, const 
         for (int i = 0; i < n; ++i) z[i] += r[i];
         double nrm = rms_scaled(r, z, NULL, n, rtol, atol);
         if (!isfinite(nrm)) break;
-        if (nrm < 1e-12) { conv = 1; break; }
+        if (nrm < NEWTON_SMALL) { conv = 1; break; }
         if (it > 0) {
           double th = nrm / prev;
           if (th > 2.0) break;
