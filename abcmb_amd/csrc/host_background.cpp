@@ -51,6 +51,7 @@ const double HC3 = (C_CM * HBAR) * (C_CM * HBAR) * (C_CM * HBAR);
 
 const double Q5[5] = {0.583165, 2.0, 4.0, 7.26582, 13.0};
 const double W5[5] = {0.0081201, 0.689407, 2.8063, 2.05156, 0.12681};
+const double EQ5[5] = {std::exp(-Q5[0]), std::exp(-Q5[1]), std::exp(-Q5[2]), std::exp(-Q5[3]), std::exp(-Q5[4])};
 
 struct Rho {
   double de, cdm, b, g, ur, ncdm, p_ncdm;
@@ -76,8 +77,8 @@ inline Rho rho_all(const AbcmbCosmo& c, double lna) {
     for (int i = 0; i < 5; ++i) {
       double q = Q5[i];
       double eps = std::sqrt(q * q + x * x);
-      ir += W5[i] * (1.0 + std::exp(-q)) / (q * q) * eps;
-      ip += W5[i] * (1.0 + std::exp(-q)) / eps;
+      ir += W5[i] * (1.0 + EQ5[i]) / (q * q) * eps;
+      ip += W5[i] * (1.0 + EQ5[i]) / eps;
     }
     double T4 = T * T * T * T;
     r.ncdm = c.N_ncdm * 4.0 * T4 / (PI * PI) * ir / (HBAR * HBAR * HBAR) / (C_CM * C_CM * C_CM);
@@ -171,6 +172,16 @@ const double GLX[8] = {-0.9602898564975363, -0.7966664774136267, -0.525532409916
 const double GLW[8] = {0.1012285362903763, 0.2223810344533745, 0.3137066458778873, 0.3626837833783620,
                        0.3626837833783620, 0.3137066458778873, 0.2223810344533745, 0.1012285362903763};
 
+const double GL4X[4] = {-0.8611363115940526, -0.3399810435848563, 0.3399810435848563, 0.8611363115940526};
+const double GL4W[4] = {0.3478548451374538, 0.6521451548625461, 0.6521451548625461, 0.3478548451374538};
+
+template <class F>
+double gl4(F f, double a, double b) {
+  double m = 0.5 * (a + b), hw = 0.5 * (b - a), s = 0.0;
+  for (int i = 0; i < 4; ++i) s += GL4W[i] * f(m + hw * GL4X[i]);
+  return s * hw;
+}
+
 template <class F>
 double gl8(F f, double a, double b) {
   double m = 0.5 * (a + b), hw = 0.5 * (b - a), s = 0.0;
@@ -195,7 +206,7 @@ void tau_table(AbcmbCosmo& c, double* tab) {
     // integrate from prev_l to l in sub-intervals of at most 2e-3
     int nsub = std::max(1, (int)std::ceil((l - prev_l) / 4e-3));
     double h = (l - prev_l) / nsub, t = prev_tau;
-    for (int s = 0; s < nsub; ++s) t += gl8(inv_aH, prev_l + s * h, prev_l + (s + 1) * h);
+    for (int s = 0; s < nsub; ++s) t += gl4(inv_aH, prev_l + s * h, prev_l + (s + 1) * h);
     tab[i] = t;
     prev_l = l;
     prev_tau = t;
@@ -269,7 +280,7 @@ void recombination(const AbcmbCosmo& c, int N, double* xe, double* tm, int* swit
       continue;
     }
     // advance x_p and T_m from lna - dlna to lna with backward-Euler substeps
-    const int NS = 8;
+    const int NS = 2;
     double h = REC_DLNA / NS;
     double lprev = lna - REC_DLNA;
     for (int s = 1; s <= NS; ++s) {
@@ -355,7 +366,7 @@ struct Kappa {
     for (int i = 0; i < n; ++i) nodes[i] = lo + (0.0 - lo) * i / (n - 1);
     nodes[n - 1] = 0.0;
     for (int i = n - 2; i >= 0; --i)
-      cum[i] = cum[i + 1] + gl8([&](double l) { return integrand(l); }, nodes[i], nodes[i + 1]);
+      cum[i] = cum[i + 1] + gl4([&](double l) { return integrand(l); }, nodes[i], nodes[i + 1]);
   }
   double operator()(double l) const {
     const int n = (int)nodes.size();
@@ -363,7 +374,7 @@ struct Kappa {
     if (l >= 0.0) return 0.0;
     double fi = (l - nodes[0]) / (nodes[1] - nodes[0]);
     int i = std::min((int)fi, n - 2);
-    return cum[i + 1] + gl8([&](double x) { return integrand(x); }, l, nodes[i + 1]);
+    return cum[i + 1] + gl4([&](double x) { return integrand(x); }, l, nodes[i + 1]);
   }
 };
 
@@ -394,14 +405,23 @@ extern "C" int abcmb_host_background(AbcmbCosmo* c, double tau_reion, double z_r
   // reionization: solve tau_reion(z) = target (Newton/bisection), or take z
   double zr = z_reion_in;
   if (tau_reion > 0.0) {
-    double lo = 0.0, hi = 50.0;
-    for (int it = 0; it < 200; ++it) {
-      double mid = 0.5 * (lo + hi);
-      double t = tau_reion_of(*c, mid, Delta_z_reion, z_reion_He, Delta_z_reion_He, exp_reion);
-      if (t < tau_reion) lo = mid; else hi = mid;
-      if (hi - lo < 1e-13) break;
+    auto f = [&](double z) {
+      return tau_reion_of(*c, z, Delta_z_reion, z_reion_He, Delta_z_reion_He, exp_reion) - tau_reion;
+    };
+    double lo = 0.0, hi = 50.0, flo = f(lo), fhi = f(hi);
+    for (int it = 0; it < 8; ++it) {  // bracket
+      double mid = 0.5 * (lo + hi), fm = f(mid);
+      if ((fm < 0) == (flo < 0)) { lo = mid; flo = fm; } else { hi = mid; fhi = fm; }
     }
-    zr = 0.5 * (lo + hi);
+    double a = lo, fa = flo, b = hi, fb = fhi;  // safeguarded secant (Illinois)
+    for (int it = 0; it < 60 && std::fabs(b - a) > 1e-12; ++it) {
+      double z = b - fb * (b - a) / (fb - fa);
+      double fz = f(z);
+      if ((fz < 0) == (fb < 0)) { fa *= 0.5; } else { a = b; fa = fb; }
+      b = z; fb = fz;
+      if (fz == 0.0) break;
+    }
+    zr = b;
   }
   c->z_reion = zr;
   for (int j = 0; j < N; ++j) {
