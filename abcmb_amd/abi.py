@@ -58,7 +58,16 @@ class AbcmbOdeSettings(C.Structure):
         (n, _I) for n in ("max_steps", "max_newton", "l_g", "l_pol", "l_ur", "l_ncdm", "n_q", "n")]
 
 
-assert C.sizeof(AbcmbCosmo) == 31 * 8 + 6 * 4 + 6 * 8
+N_TANGENT_DIRS = 5
+COSMO_DOUBLES = 31
+
+
+class AbcmbTangent(C.Structure):
+    _fields_ = [("par", _P), ("tau_tab", _P), ("xe_tab", _P), ("tm_tab", _P), ("pt", _P)]
+
+
+assert C.sizeof(AbcmbCosmo) == COSMO_DOUBLES * 8 + 6 * 4 + 6 * 8
+assert C.sizeof(AbcmbTangent) == 40
 assert C.sizeof(AbcmbPertItem) == 48
 
 _host = None
@@ -85,7 +94,7 @@ def host_lib():
 
 
 HIP_SYMBOLS = (
-    "abcmb_last_error", "abcmb_device_init", "abcmb_pert_workspace_size", "abcmb_pert_fwd",
+    "abcmb_last_error", "abcmb_device_init", "abcmb_pert_workspace_size", "abcmb_pert_fwd", "abcmb_pert_tangent",
     "abcmb_sources_workspace_size", "abcmb_sources", "abcmb_los_workspace_size", "abcmb_los_cl", "abcmb_apply_operator",
     "abcmb_lensing_clpp", "abcmb_lens_workspace_size", "abcmb_lens", "abcmb_pk_lin",
 )
@@ -105,6 +114,8 @@ def hip_lib():
         lib.abcmb_pert_workspace_size.argtypes = [C.c_int]
         lib.abcmb_pert_fwd.argtypes = [_P, C.c_int, _P, C.c_int, C.POINTER(AbcmbOdeSettings),
                                        _P, _P, _P, _P, _P, C.c_size_t, _P]
+        lib.abcmb_pert_tangent.argtypes = [_P, _P, C.c_int, _P, C.c_int, C.POINTER(AbcmbOdeSettings),
+                                           _P, _P, _P, _P]
         lib.abcmb_sources_workspace_size.restype = C.c_size_t
         lib.abcmb_sources_workspace_size.argtypes = [C.c_int, C.c_int, C.c_int]
         lib.abcmb_sources.argtypes = [_P, C.c_int, C.c_int, C.c_int, _P, _P, C.c_int, _P, _P, _P, C.c_size_t, _P]

@@ -24,9 +24,11 @@
 //     thread; workgroup sums (metric sums, error norms, pivot search) use
 //     wavefront shuffles plus one LDS slot per wavefront.
 #include <cmath>
+#include <cstddef>
 #include <cstdint>
 
 #include "common.h"
+#include "dual.h"
 
 namespace abcmb {
 namespace {
@@ -68,88 +70,142 @@ enum {
   SC_COUNT
 };
 
+// ---------------------------------------------------------------- cosmology fields
+// AbcmbCosmo's double fields in declaration order; the tangent inputs index
+// them the same way.
+enum CosmoField {
+  F_H = 0, F_OMEGA_B, F_OMEGA_CDM, F_OMEGA_LAMBDA, F_A_S, F_N_S, F_K_PIVOT, F_TCMB0, F_YHE,
+  F_N_UR, F_T_UR, F_N_NCDM, F_T_NCDM, F_M_NCDM, F_R_NU, F_R_B, F_OM, F_OMEGA_R, F_OMEGA_M, F_TAU0,
+  F_XE_X0, F_XE_DX, F_XE_LNA_LAST, F_XE_LAST, F_TM_X0, F_TM_DX, F_TM_LNA_LAST, F_TM_LAST,
+  F_LNA_REC, F_LNA_TRANSFER_START, F_Z_REION, F_COUNT
+};
+static_assert(F_COUNT == ABCMB_COSMO_DOUBLES, "field list out of step with AbcmbCosmo");
+static_assert(offsetof(AbcmbCosmo, z_reion) == (F_COUNT - 1) * sizeof(double), "AbcmbCosmo doubles are not packed");
+
+// A cosmology seen by the kernels: its fields and tables, with their
+// derivatives along N parameter directions when N > 0.
+template <int N>
+struct Cosmo {
+  const AbcmbCosmo& c;
+  const AbcmbTangent* t;  // null when N == 0
+  __device__ __forceinline__ D<N> operator()(int f) const {
+    D<N> r(reinterpret_cast<const double*>(&c)[f]);
+    if constexpr (N > 0) {
+      ABCMB_DLOOP(N) r.d[i_] = t->par[i_ * ABCMB_COSMO_DOUBLES + f];
+    }
+    return r;
+  }
+  __device__ __forceinline__ const double* dtau() const { if constexpr (N > 0) return t->tau_tab; else return nullptr; }
+  __device__ __forceinline__ const double* dxe() const { if constexpr (N > 0) return t->xe_tab; else return nullptr; }
+  __device__ __forceinline__ const double* dtm() const { if constexpr (N > 0) return t->tm_tab; else return nullptr; }
+};
+
+// table entry i of f (and of its tangent tables fd[N][n])
+template <int N>
+__device__ __forceinline__ D<N> tab_at(const double* __restrict__ f, const double* __restrict__ fd, int n, int i) {
+  D<N> r(f[i]);
+  if constexpr (N > 0) { ABCMB_DLOOP(N) r.d[i_] = fd[i_ * n + i]; }
+  return r;
+}
+
 // ---------------------------------------------------------------- background
-__device__ __forceinline__ double fast_interp(double x, double lo, double hi, const double* __restrict__ f, int n) {
+// linear interpolation on a uniform grid with the reference's end clipping
+// (abcmb/ABCMBTools.py:268-308); the abscissa does not depend on the parameters
+template <int N>
+__device__ __forceinline__ D<N> fast_interp(double x, double lo, double hi, const double* __restrict__ f,
+                                            const double* __restrict__ fd, int n) {
   double i = (x - lo) / (hi - lo) * (n - 1);
   const double eps = 1e-6;
   i = fmin(fmax(i, eps), n - 1.0 - eps);
   int il = (int)floor(i);
   int iu = min(il + 1, n - 1);
   double wu = i - il;
-  return (1.0 - wu) * f[il] + wu * f[iu];
+  D<N> r((1.0 - wu) * f[il] + wu * f[iu]);
+  if constexpr (N > 0) { ABCMB_DLOOP(N) r.d[i_] = (1.0 - wu) * fd[i_ * n + il] + wu * fd[i_ * n + iu]; }
+  return r;
 }
 
+template <int N>
 struct Bg {
-  double a, aH, tau, tauc, cs2, R;
-  double rc, rb, rg, ru, rn, pn;
-  double Tn, xn;
+  double a;  // exp(ln a): the time grid does not move with the parameters
+  D<N> aH, tau, tauc, cs2, R;
+  D<N> rc, rb, rg, ru, rn, pn;
+  D<N> Tn, xn;
 };
 
 __device__ __forceinline__ double rho_unit() { return 3.0 * H0_OVER_H * H0_OVER_H / 8.0 / PI / GN; }
 
 // densities, aH (abcmb/background.py:101-187; species rho at abcmb/species.py)
-__device__ void bg_rho(const AbcmbCosmo& c, double lna, Bg& b) {
+template <int N>
+__device__ void bg_rho(const Cosmo<N>& P, double lna, Bg<N>& b) {
   const double hc3 = (C_CM * HBAR) * (C_CM * HBAR) * (C_CM * HBAR);
   double a = exp(lna);
   double a3 = a * a * a, a4 = a3 * a;
-  double t04 = c.TCMB0 * c.TCMB0 * c.TCMB0 * c.TCMB0;
-  double tu4 = c.T_ur * c.T_ur * c.T_ur * c.T_ur;
+  const D<N> T0 = P(F_TCMB0), Tu = P(F_T_UR);
+  D<N> t04 = T0 * T0 * T0 * T0;
+  D<N> tu4 = Tu * Tu * Tu * Tu;
   b.a = a;
-  double rde = c.omega_Lambda * rho_unit();
-  b.rc = c.omega_cdm * rho_unit() / a3;
-  b.rb = c.omega_b * rho_unit() / a3;
+  D<N> rde = P(F_OMEGA_LAMBDA) * rho_unit();
+  b.rc = P(F_OMEGA_CDM) * rho_unit() / a3;
+  b.rb = P(F_OMEGA_B) * rho_unit() / a3;
   b.rg = PI * PI / 15.0 * t04 / a4 / hc3;
-  b.ru = c.N_ur * 2.0 * 7.0 / 8.0 * PI * PI / 30.0 * tu4 * t04 / a4 / hc3;
-  b.rn = b.pn = 0.0;
-  b.Tn = c.T_ncdm * c.TCMB0 / a;
-  b.xn = c.m_ncdm / b.Tn;
-  if (c.has_ncdm) {
-    double ir = 0.0, ip = 0.0;
+  b.ru = P(F_N_UR) * 2.0 * 7.0 / 8.0 * PI * PI / 30.0 * tu4 * t04 / a4 / hc3;
+  b.rn = D<N>(0.0);
+  b.pn = D<N>(0.0);
+  b.Tn = P(F_T_NCDM) * T0 / a;
+  b.xn = P(F_M_NCDM) / b.Tn;
+  if (P.c.has_ncdm) {
+    D<N> ir(0.0), ip(0.0);
 #pragma unroll
     for (int i = 0; i < 5; ++i) {
       double q = Q5[i];
-      double e = sqrt(q * q + b.xn * b.xn);
+      D<N> e = dsqrt(q * q + b.xn * b.xn);
       ir += (1.0 + exp(-q)) / (q * q) * e * W5[i];
       ip += (1.0 + exp(-q)) / e * W5[i];
     }
-    double T4 = b.Tn * b.Tn * b.Tn * b.Tn;
-    double pre = T4 / (PI * PI) / (HBAR * HBAR * HBAR) / (C_CM * C_CM * C_CM);
-    b.rn = c.N_ncdm * 4.0 * pre * ir;
-    b.pn = c.N_ncdm * 4.0 / 3.0 * pre * ip;
+    D<N> T4 = b.Tn * b.Tn * b.Tn * b.Tn;
+    D<N> pre = T4 / (PI * PI) / (HBAR * HBAR * HBAR) / (C_CM * C_CM * C_CM);
+    b.rn = P(F_N_NCDM) * 4.0 * pre * ir;
+    b.pn = P(F_N_NCDM) * 4.0 / 3.0 * pre * ip;
   }
-  double rt = rde + b.rc + b.rb + b.rg + b.ru + b.rn;
-  b.aH = a * sqrt(8.0 * PI * GN * rt / 3.0) / C_MPC;
+  D<N> rt = rde + b.rc + b.rb + b.rg + b.ru + b.rn;
+  b.aH = a * dsqrt(8.0 * PI * GN * rt / 3.0) / C_MPC;
 }
 
 // x_e lookup with the reference's span quirk (abcmb/background.py:568-603)
-__device__ __forceinline__ double bg_xe(const AbcmbCosmo& c, double lna) {
-  if (lna < c.xe_x0) return c.xe_tab[0];
-  if (lna >= c.xe_lna_last) return c.xe_last;
-  return fast_interp(lna, c.xe_x0, c.xe_x0 + c.xe_n * c.xe_dx, c.xe_tab, c.xe_n);
+template <int N>
+__device__ __forceinline__ D<N> bg_xe(const Cosmo<N>& P, double lna) {
+  const AbcmbCosmo& c = P.c;
+  if (lna < c.xe_x0) return tab_at<N>(c.xe_tab, P.dxe(), c.xe_n, 0);
+  if (lna >= c.xe_lna_last) return P(F_XE_LAST);
+  return fast_interp<N>(lna, c.xe_x0, c.xe_x0 + c.xe_n * c.xe_dx, c.xe_tab, P.dxe(), c.xe_n);
 }
 
-__device__ __noinline__ void bg_eval(const AbcmbCosmo& c, double lna, Bg& b) {
-  bg_rho(c, lna, b);
-  b.tau = fast_interp(lna, ABCMB_TAU_TAB_LO, 0.0, c.tau_tab, ABCMB_TAU_TAB_N);
-  double xe = bg_xe(c, lna);
-  double nH = (1 - c.YHe) * 3.0 * c.omega_b * H0_OVER_H * H0_OVER_H / 8 / PI / GN / MH / (b.a * b.a * b.a);
+template <int N>
+__device__ __noinline__ void bg_eval(const Cosmo<N>& P, double lna, Bg<N>& b) {
+  const AbcmbCosmo& c = P.c;
+  bg_rho(P, lna, b);
+  b.tau = fast_interp<N>(lna, ABCMB_TAU_TAB_LO, 0.0, c.tau_tab, P.dtau(), ABCMB_TAU_TAB_N);
+  D<N> xe = bg_xe(P, lna);
+  const D<N> YHe = P(F_YHE);
+  D<N> nH = (1 - YHe) * 3.0 * P(F_OMEGA_B) * H0_OVER_H * H0_OVER_H / 8 / PI / GN / MH / (b.a * b.a * b.a);
   b.tauc = 1.0 / b.a / (nH * xe) / SIGMA_T / C_CM * C_MPC;
   // T_m (abcmb/background.py:605-657)
-  double Tg = c.TCMB0 / b.a;
-  double Tm;
+  D<N> Tg = P(F_TCMB0) / b.a;
+  D<N> Tm;
   if (lna < c.tm_x0) {
-    double fhe = c.YHe * MH / MHE / (1 - c.YHe);
+    D<N> fhe = YHe * MH / MHE / (1 - YHe);
     double stef = PI * PI / (60.0 * HBAR * HBAR * HBAR * C_CM * C_CM);
-    double gc = xe / (1.0 + fhe + xe) * 8.0 * SIGMA_T * (4.0 * stef) * Tg * Tg * Tg * Tg / (3.0 * ME);
-    double H = b.aH * C_MPC / b.a;
+    D<N> gc = xe / (1.0 + fhe + xe) * 8.0 * SIGMA_T * (4.0 * stef) * Tg * Tg * Tg * Tg / (3.0 * ME);
+    D<N> H = b.aH * C_MPC / b.a;
     Tm = Tg * (1.0 - H / gc);
   } else if (lna >= c.tm_lna_last) {
-    Tm = c.tm_last;
+    Tm = P(F_TM_LAST);
   } else {
-    Tm = fast_interp(lna, c.tm_x0, c.tm_x0 + c.tm_n * c.tm_dx, c.tm_tab, c.tm_n);
+    Tm = fast_interp<N>(lna, c.tm_x0, c.tm_x0 + c.tm_n * c.tm_dx, c.tm_tab, P.dtm(), c.tm_n);
   }
   // c_s^2 (abcmb/species.py:1163-1220)
-  double mu = MH / ((1.0 + xe) * (1.0 - c.YHe) + MH / MHE * c.YHe);
+  D<N> mu = MH / ((1.0 + xe) * (1.0 - YHe) + MH / MHE * YHe);
   b.R = 4.0 * b.rg / 3.0 / b.rb;
   b.cs2 = Tm / mu * (5.0 / 3.0 - 2.0 / 3.0 * mu * b.R / ME / b.aH / b.tauc * (Tg / Tm - 1.0));
 }
@@ -254,36 +310,45 @@ __device__ __noinline__ void row_desc(const Layout& L, double k, int r, Row& w) 
   }
 }
 
-// scalars of A(ln a) shared by all rows (written by thread 0)
-__device__ __noinline__ void fill_scalars(const AbcmbCosmo& c, const Layout& L, double k, const Bg& b, double* sc) {
-  double ah = b.aH;
-  sc[SC_ONE] = 1.0;
-  sc[SC_IAH] = 1.0 / ah;
-  sc[SC_ITC] = 1.0 / (ah * b.tauc);
-  sc[SC_ITAU] = 1.0 / (ah * b.tau);
-  sc[SC_CS2] = b.cs2 / ah;
-  sc[SC_RTC] = b.R / (b.tauc * ah);
-  double kh = 2.0 / (ah * ah) * 4.0 * PI * GN * b.a * b.a / (C_MPC * C_MPC);
-  sc[SC_H_ETA] = 2.0 / (ah * ah) * k * k;
-  sc[SC_H_C] = kh * b.rc;
-  sc[SC_H_B] = kh * b.rb;
-  sc[SC_H_G] = kh * b.rg;
-  sc[SC_H_U] = kh * b.ru;
-  double ke = 4.0 * PI * GN * b.a * b.a / ah / (k * k) / (C_MPC * C_MPC);
-  sc[SC_E_B] = ke * b.rb;
-  sc[SC_E_G] = ke * (b.rg + b.rg / 3.0);
-  sc[SC_E_U] = ke * (b.ru + b.ru / 3.0);
-  double pre = 0.0;
-  if (c.has_ncdm) pre = c.N_ncdm * b.Tn * b.Tn * b.Tn * b.Tn / (PI * PI) / (HBAR * HBAR * HBAR) / (C_CM * C_CM * C_CM);
+
+// scalars of A(ln a) shared by all rows: sc[c][slot] holds the value (c = 0)
+// and the derivative along direction c - 1
+template <int N>
+__device__ __noinline__ void fill_scalars(const Cosmo<N>& P, const Layout& L, double k, const Bg<N>& b,
+                                          double (*sc)[SC_COUNT]) {
+  auto put = [&](int s, const D<N>& x) {
+    sc[0][s] = x.v;
+    if constexpr (N > 0) { ABCMB_DLOOP(N) sc[1 + i_][s] = x.d[i_]; }
+  };
+  const D<N> ah = b.aH;
+  put(SC_ONE, D<N>(1.0));
+  put(SC_IAH, 1.0 / ah);
+  put(SC_ITC, 1.0 / (ah * b.tauc));
+  put(SC_ITAU, 1.0 / (ah * b.tau));
+  put(SC_CS2, b.cs2 / ah);
+  put(SC_RTC, b.R / (b.tauc * ah));
+  D<N> kh = 2.0 / (ah * ah) * 4.0 * PI * GN * b.a * b.a / (C_MPC * C_MPC);
+  put(SC_H_ETA, 2.0 / (ah * ah) * k * k);
+  put(SC_H_C, kh * b.rc);
+  put(SC_H_B, kh * b.rb);
+  put(SC_H_G, kh * b.rg);
+  put(SC_H_U, kh * b.ru);
+  D<N> ke = 4.0 * PI * GN * b.a * b.a / ah / (k * k) / (C_MPC * C_MPC);
+  put(SC_E_B, ke * b.rb);
+  put(SC_E_G, ke * (b.rg + b.rg / 3.0));
+  put(SC_E_U, ke * (b.ru + b.ru / 3.0));
+  D<N> pre(0.0);
+  if (P.c.has_ncdm)
+    pre = P(F_N_NCDM) * b.Tn * b.Tn * b.Tn * b.Tn / (PI * PI) / (HBAR * HBAR * HBAR) / (C_CM * C_CM * C_CM);
   for (int i = 0; i < 5; ++i) {
     double q = i < L.nq ? (L.nq == 5 ? Q5[i] : Q3[i]) : 1.0;
     double wq = i < L.nq ? (L.nq == 5 ? W5[i] : W3[i]) : 0.0;
-    double e = sqrt(q * q + b.xn * b.xn);
-    sc[SC_P0 + i] = q / e / ah;
-    sc[SC_HN0 + i] = kh * 4.0 * pre * wq * (1.0 + exp(-q)) * e / (q * q);
+    D<N> e = dsqrt(q * q + b.xn * b.xn);
+    put(SC_P0 + i, q / e / ah);
+    put(SC_HN0 + i, kh * 4.0 * pre * wq * (1.0 + exp(-q)) * e / (q * q));
   }
-  sc[SC_EN] = ke * 4.0 * pre;
-  sc[SC_ZERO] = 0.0;
+  put(SC_EN, ke * 4.0 * pre);
+  put(SC_ZERO, D<N>(0.0));
 }
 
 // ---------------------------------------------------------------- the kernel
@@ -297,9 +362,23 @@ __device__ __noinline__ void fill_scalars(const AbcmbCosmo& c, const Layout& L, 
 // natural order (the block pivots stay >= 1 for these matrices; a pivot
 // below 1e-12 is reported as a failed step), and (I - g A)^-1 is applied
 // with the Woodbury identity through a 2x2 capacitance matrix.
+//
+// With N tangent directions every state quantity is a dual number D<N>.  A
+// depends on the parameters only through the background scalars, so the
+// tangent of a stage solves (I - g A) zdot = basedot + g Adot z; it is
+// iterated with the same chord matrix alongside the value (and beyond, until
+// it has converged too).  Step sizes and acceptance are decided on the value
+// alone, which stays identical to the primal kernel's.
 
 // chord updates below this (in units of the error tolerance) are rounding noise
 constexpr double NEWTON_SMALL = 1e-9;
+// Tangents: once the value has converged, the tangents keep iterating (value
+// fixed) until each direction's weighted update is below TANGENT_RTOL of its
+// weighted size, so they solve the differentiated stage equations rather
+// than the derivative of a truncated chord iteration; at most TANGENT_EXTRA
+// iterations beyond max_newton.
+constexpr double TANGENT_RTOL = 1e-7;
+constexpr int TANGENT_EXTRA = 8;
 constexpr int NTIMES = 7;  // t, the distinct stage abscissae t + c_s h, and the chord-Jacobian time
 constexpr int JAC_SLOT = 6;
 __constant__ double TIME_C[NTIMES - 1] = {0.0, 0.52, 1.230333209967908, 0.8957659843500759, 0.43639360985864756,
@@ -325,16 +404,18 @@ __device__ Blk row_block(const Layout& L, int r) {
   return b;
 }
 
-template <int NT, int BMAX>
+template <int NT, int BMAX, int N>
 struct Smem {
-  double sc[NTIMES][SC_COUNT];
-  double zl[NT + BMAX];
-  double rl[NT + BMAX];
+  static constexpr int NC = 1 + N;                        // value + tangents
+  static constexpr int NRED = (2 * NC > 8 ? 2 * NC : 8);  // widest workgroup sum
+  double sc[NTIMES][NC][SC_COUNT];
+  double zl[NC][NT + BMAX];
+  double rl[NC][NT + BMAX];
   double uh[NT + BMAX];
   double ue[NT + BMAX];
   double pbuf[2][NBLK][BMAX];
-  double red[8 * (NT / 64)];
-  double Fs[7][NT];  // stage derivatives, one column per thread
+  double red[NRED * (NT / 64)];
+  double Fs[7][NC][NT];  // stage derivatives, one column per thread
 };
 
 // workgroup sum of NV values (wave shuffles + one LDS slot per wave)
@@ -364,15 +445,15 @@ __device__ __forceinline__ void block_sum(double (&v)[NV], double* red) {
 }
 
 // background scalars of A at t + TIME_C[i] * hh, i < cnt (one lane per time)
-template <int NT, int BMAX>
-__device__ void set_times(Smem<NT, BMAX>& sm, const AbcmbCosmo& c, const Layout& L, double k, double t, double hh,
+template <int NT, int BMAX, int N>
+__device__ void set_times(Smem<NT, BMAX, N>& sm, const Cosmo<N>& P, const Layout& L, double k, double t, double hh,
                           int cnt, double jac_offset) {
   __syncthreads();
   int r = threadIdx.x;
   if (r < cnt) {
-    Bg b;
-    bg_eval(c, t + (r == JAC_SLOT ? jac_offset : TIME_C[r]) * hh, b);
-    fill_scalars(c, L, k, b, sm.sc[r]);
+    Bg<N> b;
+    bg_eval(P, t + (r == JAC_SLOT ? jac_offset : TIME_C[r]) * hh, b);
+    fill_scalars(P, L, k, b, sm.sc[r]);
   }
   __syncthreads();
 }
@@ -384,20 +465,65 @@ struct RowState {
   int n;
 };
 
-// f_r(t_slot, z)
-template <int NT, int BMAX>
-__device__ __forceinline__ double rhs_eval(Smem<NT, BMAX>& sm, const RowState& R, int slot, double z) {
+template <int NT, int BMAX, int N>
+__device__ __forceinline__ void put_col(double (*buf)[NT + BMAX], int r, const D<N>& x) {
+  buf[0][r] = x.v;
+  if constexpr (N > 0) { ABCMB_DLOOP(N) buf[1 + i_][r] = x.d[i_]; }
+}
+
+template <int NT, int BMAX, int N>
+__device__ __forceinline__ D<N> get_F(const Smem<NT, BMAX, N>& sm, int s, int r) {
+  D<N> x;
+  x.v = sm.Fs[s][0][r];
+  if constexpr (N > 0) { ABCMB_DLOOP(N) x.d[i_] = sm.Fs[s][1 + i_][r]; }
+  return x;
+}
+
+template <int NT, int BMAX, int N>
+__device__ __forceinline__ void put_F(Smem<NT, BMAX, N>& sm, int s, int r, const D<N>& x) {
+  sm.Fs[s][0][r] = x.v;
+  if constexpr (N > 0) { ABCMB_DLOOP(N) sm.Fs[s][1 + i_][r] = x.d[i_]; }
+}
+
+// f_r(t_slot, z): value sum_j cs_j sc_j z_j + u_h h' + u_e eta'; each tangent
+// adds the derivative of the scalars times the value state
+template <int NT, int BMAX, int N>
+__device__ __forceinline__ D<N> rhs_eval(Smem<NT, BMAX, N>& sm, const RowState& R, int slot, const D<N>& z) {
   const int r = threadIdx.x;
-  sm.zl[r] = z;
+  put_col<NT, BMAX, N>(sm.zl, r, z);
   __syncthreads();
-  const double* sc = sm.sc[slot];
-  double v[2] = {R.w.whs * sc[R.w.whi] * z, R.w.wes * sc[R.w.wei] * z};
-  block_sum<NT, 2>(v, sm.red);
+  const double* sc = sm.sc[slot][0];
+  double v[2 * (1 + N)];
+  v[0] = R.w.whs * sc[R.w.whi] * z.v;
+  v[1] = R.w.wes * sc[R.w.wei] * z.v;
+  if constexpr (N > 0) {
+#pragma unroll
+    for (int i = 0; i < N; ++i) {
+      const double* sd = sm.sc[slot][1 + i];
+      v[2 + 2 * i] = R.w.whs * (sd[R.w.whi] * z.v + sc[R.w.whi] * z.d[i]);
+      v[3 + 2 * i] = R.w.wes * (sd[R.w.wei] * z.v + sc[R.w.wei] * z.d[i]);
+    }
+  }
+  block_sum<NT, 2 * (1 + N)>(v, sm.red);
+  D<N> f;
   double acc = 0.0;
 #pragma unroll
-  for (int j = 0; j < 5; ++j) acc += R.w.cs[j] * sc[R.w.si[j]] * sm.zl[R.w.col[j]];
+  for (int j = 0; j < 5; ++j) acc += R.w.cs[j] * sc[R.w.si[j]] * sm.zl[0][R.w.col[j]];
   acc += R.w.uh * v[0] + R.w.ue * v[1];
-  return r < R.n ? acc : 0.0;
+  f.v = r < R.n ? acc : 0.0;
+  if constexpr (N > 0) {
+#pragma unroll
+    for (int i = 0; i < N; ++i) {
+      const double* sd = sm.sc[slot][1 + i];
+      double ad = 0.0;
+#pragma unroll
+      for (int j = 0; j < 5; ++j)
+        ad += R.w.cs[j] * (sd[R.w.si[j]] * sm.zl[0][R.w.col[j]] + sc[R.w.si[j]] * sm.zl[1 + i][R.w.col[j]]);
+      ad += R.w.uh * v[2 + 2 * i] + R.w.ue * v[3 + 2 * i];
+      f.d[i] = r < R.n ? ad : 0.0;
+    }
+  }
+  return f;
 }
 
 template <int BMAX>
@@ -409,9 +535,9 @@ struct Inv {
 };
 
 // build and invert S = I - g B(t_n) block-wise, then the Woodbury terms
-template <int NT, int BMAX>
-__device__ bool factor(Smem<NT, BMAX>& sm, const RowState& R, int maxb, double g, Inv<BMAX>& F) {
-  const double* sc = sm.sc[JAC_SLOT];
+template <int NT, int BMAX, int N>
+__device__ bool factor(Smem<NT, BMAX, N>& sm, const RowState& R, int maxb, double g, Inv<BMAX>& F) {
+  const double* sc = sm.sc[JAC_SLOT][0];
   const Blk& B = R.B;
 #pragma unroll
   for (int c = 0; c < BMAX; ++c) F.row[c] = (c == B.lr) ? 1.0 : 0.0;
@@ -468,20 +594,36 @@ __device__ bool factor(Smem<NT, BMAX>& sm, const RowState& R, int maxb, double g
   return v[4] == 0.0 && isfinite(F.c00) && isfinite(F.c11) && det != 0.0;
 }
 
-// x = (I - g A(t_n))^-1 b
-template <int NT, int BMAX>
-__device__ __forceinline__ double solve(Smem<NT, BMAX>& sm, const RowState& R, const Inv<BMAX>& F, double b) {
+// x = (I - g A(t_n))^-1 b, value and tangents with the same factors
+template <int NT, int BMAX, int N>
+__device__ __forceinline__ D<N> solve(Smem<NT, BMAX, N>& sm, const RowState& R, const Inv<BMAX>& F, const D<N>& b) {
+  constexpr int NC = 1 + N;
   const int r = threadIdx.x;
-  sm.rl[r] = r < R.n ? b : 0.0;
+  put_col<NT, BMAX, N>(sm.rl, r, r < R.n ? b : D<N>(0.0));
   __syncthreads();
-  double z = 0.0;
+  double z[NC];
+  double v[2 * NC];
 #pragma unroll
-  for (int c = 0; c < BMAX; ++c) z += F.row[c] * sm.rl[R.B.start + c];
-  double v[2] = {F.Vh * z, F.Ve * z};
-  block_sum<NT, 2>(v, sm.red);
-  double a = F.c00 * v[0] + F.c01 * v[1];
-  double bb = F.c10 * v[0] + F.c11 * v[1];
-  return r < R.n ? z + F.Yh * a + F.Ye * bb : 0.0;
+  for (int q = 0; q < NC; ++q) {
+    double s = 0.0;
+#pragma unroll
+    for (int c = 0; c < BMAX; ++c) s += F.row[c] * sm.rl[q][R.B.start + c];
+    z[q] = s;
+    v[2 * q] = F.Vh * s;
+    v[2 * q + 1] = F.Ve * s;
+  }
+  block_sum<NT, 2 * NC>(v, sm.red);
+  double out[NC];
+#pragma unroll
+  for (int q = 0; q < NC; ++q) {
+    double a = F.c00 * v[2 * q] + F.c01 * v[2 * q + 1];
+    double bb = F.c10 * v[2 * q] + F.c11 * v[2 * q + 1];
+    out[q] = r < R.n ? z[q] + F.Yh * a + F.Ye * bb : 0.0;
+  }
+  D<N> x;
+  x.v = out[0];
+  if constexpr (N > 0) { ABCMB_DLOOP(N) x.d[i_] = out[1 + i_]; }
+  return x;
 }
 
 __device__ __forceinline__ double rms_term(double e, double y0, double y1, double rtol, double atol) {
@@ -491,17 +633,18 @@ __device__ __forceinline__ double rms_term(double e, double y0, double y1, doubl
 }
 
 // initial conditions (abcmb/perturbations.py:161-199 and species y_ini)
-__device__ __noinline__ double ic_row(const AbcmbCosmo& c, const Layout& L, double k, double ti, int r) {
-  double om = c.om, Rn = c.R_nu, Rb = c.R_b;
-  double kt = k * ti;
-  double dg = -kt * kt / 3. * (1. - om * ti / 5.);
-  double thb = -k * k * k * k * ti * ti * ti / 36. * (1. - 3. * (1. + 5. * Rb - Rn) / 20. / (1. - Rn) * om * ti);
-  double tn = -k * kt * kt * kt / 36. / (4. * Rn + 15.) *
-              (4. * Rn + 11. + 12. - 3. * (8. * Rn * Rn + 50. * Rn + 275.) / 20. / (2. * Rn + 15.) * ti * om);
-  double sn = kt * kt / (45. + 12. * Rn) * 2. * (1. + (4. * Rn - 5.) / 4. / (2. * Rn + 15.) * ti * om);
+template <int N>
+__device__ __noinline__ D<N> ic_row(const Cosmo<N>& P, const Layout& L, double k, D<N> ti, int r) {
+  const D<N> om = P(F_OM), Rn = P(F_R_NU), Rb = P(F_R_B);
+  D<N> kt = k * ti;
+  D<N> dg = -kt * kt / 3. * (1. - om * ti / 5.);
+  D<N> thb = -k * k * k * k * ti * ti * ti / 36. * (1. - 3. * (1. + 5. * Rb - Rn) / 20. / (1. - Rn) * om * ti);
+  D<N> tn = -k * kt * kt * kt / 36. / (4. * Rn + 15.) *
+            (4. * Rn + 11. + 12. - 3. * (8. * Rn * Rn + 50. * Rn + 275.) / 20. / (2. * Rn + 15.) * ti * om);
+  D<N> sn = kt * kt / (45. + 12. * Rn) * 2. * (1. + (4. * Rn - 5.) / 4. / (2. * Rn + 15.) * ti * om);
   if (r == 0)
     return 1. - k * k * ti * ti / 12. / (15. + 4. * Rn) *
-                    (5. + 4. * Rn - (16. * Rn * Rn + 280. * Rn + 325) / 10. / (2. * Rn + 15.) * ti * om);
+                    (5. + 4. * Rn - (16. * Rn * Rn + 280. * Rn + 325.) / 10. / (2. * Rn + 15.) * ti * om);
   if (r == 1 || r == 2) return -kt * kt / 4. * (1. - om * ti / 5.);
   if (r == 3) return thb;
   if (r == L.ig) return dg;
@@ -518,78 +661,90 @@ __device__ __noinline__ double ic_row(const AbcmbCosmo& c, const Layout& L, doub
     if (l == 1) return tn / 3. * f;
     if (l == 2) return sn / 2. * f;
   }
-  return 0.0;
+  return D<N>(0.0);
 }
 
-// output-table fields at one saved point from the state in LDS (zl)
+// output-table fields at one saved point from the state in LDS (ys[c][row])
 // (abcmb/perturbations.py:318-404; output_perturbations in abcmb/species.py)
-__device__ __noinline__ double output_field(const AbcmbCosmo& c, const Layout& L, double k, double lna, const double* y, int f) {
-  Bg b;
-  bg_eval(c, lna, b);
-  double a = b.a, aH = b.aH;
-  double nrd = 0.0, nrt = 0.0, nrs = 0.0;
+template <int N, int LD>
+__device__ __noinline__ D<N> output_field(const Cosmo<N>& P, const Layout& L, double k, double lna,
+                                          const double (*ys)[LD], int f) {
+  auto y = [&](int i) {
+    D<N> x(ys[0][i]);
+    if constexpr (N > 0) { ABCMB_DLOOP(N) x.d[i_] = ys[1 + i_][i]; }
+    return x;
+  };
+  Bg<N> b;
+  bg_eval(P, lna, b);
+  const double a = b.a;
+  const D<N> aH = b.aH;
+  D<N> nrd(0.0), nrt(0.0), nrs(0.0);
   if (L.nq > 0) {
     int nl = L.ln + 1;
-    double s0 = 0, s1 = 0, s2 = 0;
+    D<N> s0(0.0), s1(0.0), s2(0.0);
     for (int i = 0; i < L.nq; ++i) {
       double q = L.nq == 5 ? Q5[i] : Q3[i];
       double wq = L.nq == 5 ? W5[i] : W3[i];
-      double e = sqrt(q * q + b.xn * b.xn);
-      const double* P = y + L.inc + i * nl;
-      s0 += wq * (1.0 + exp(-q)) * e / (q * q) * P[0];
-      s1 += wq * (1.0 + exp(-q)) / q * P[1];
-      s2 += wq * (1.0 + exp(-q)) / e * P[2];
+      D<N> e = dsqrt(q * q + b.xn * b.xn);
+      const int p0 = L.inc + i * nl;
+      s0 += wq * (1.0 + exp(-q)) * e / (q * q) * y(p0);
+      s1 += wq * (1.0 + exp(-q)) / q * y(p0 + 1);
+      s2 += wq * (1.0 + exp(-q)) / e * y(p0 + 2);
     }
-    double pre = c.N_ncdm * b.Tn * b.Tn * b.Tn * b.Tn / (PI * PI) / (HBAR * HBAR * HBAR) / (C_CM * C_CM * C_CM);
+    D<N> pre = P(F_N_NCDM) * b.Tn * b.Tn * b.Tn * b.Tn / (PI * PI) / (HBAR * HBAR * HBAR) / (C_CM * C_CM * C_CM);
     nrd = s0 * 4.0 * pre; nrt = s1 * 4.0 * pre; nrs = s2 * 8.0 / 3.0 * pre;
   }
   switch (f) {
     case ABCMB_PT_THETA_B_PRIME:
-      return -y[3] + b.cs2 / aH * (k * k * y[2]) + b.R / aH / b.tauc * (y[L.ig + 1] - y[3]);
-    case ABCMB_PT_ETA: return y[0];
-    case ABCMB_PT_CDM_DELTA: return y[1];
-    case ABCMB_PT_B_DELTA: return y[2];
-    case ABCMB_PT_B_THETA: return y[3];
-    case ABCMB_PT_G_DELTA: return y[L.ig];
-    case ABCMB_PT_G_THETA: return y[L.ig + 1];
-    case ABCMB_PT_G_SIGMA: return y[L.ig + 2];
-    case ABCMB_PT_G_G0: return y[L.ip];
-    case ABCMB_PT_G_G2: return y[L.ip + 2];
-    case ABCMB_PT_UR_DELTA: return y[L.iu];
-    case ABCMB_PT_UR_THETA: return y[L.iu + 1];
-    case ABCMB_PT_UR_SIGMA: return y[L.iu + 2];
-    case ABCMB_PT_NCDM_DELTA: return L.nq > 0 ? nrd / b.rn : 0.0;
-    case ABCMB_PT_NCDM_THETA: return L.nq > 0 ? nrt / (b.rn + b.pn) : 0.0;
-    case ABCMB_PT_NCDM_SIGMA: return L.nq > 0 ? nrs / (b.rn + b.pn) : 0.0;
+      return -y(3) + b.cs2 / aH * (k * k * y(2)) + b.R / aH / b.tauc * (y(L.ig + 1) - y(3));
+    case ABCMB_PT_ETA: return y(0);
+    case ABCMB_PT_CDM_DELTA: return y(1);
+    case ABCMB_PT_B_DELTA: return y(2);
+    case ABCMB_PT_B_THETA: return y(3);
+    case ABCMB_PT_G_DELTA: return y(L.ig);
+    case ABCMB_PT_G_THETA: return y(L.ig + 1);
+    case ABCMB_PT_G_SIGMA: return y(L.ig + 2);
+    case ABCMB_PT_G_G0: return y(L.ip);
+    case ABCMB_PT_G_G2: return y(L.ip + 2);
+    case ABCMB_PT_UR_DELTA: return y(L.iu);
+    case ABCMB_PT_UR_THETA: return y(L.iu + 1);
+    case ABCMB_PT_UR_SIGMA: return y(L.iu + 2);
+    case ABCMB_PT_NCDM_DELTA: return L.nq > 0 ? nrd / b.rn : D<N>(0.0);
+    case ABCMB_PT_NCDM_THETA: return L.nq > 0 ? nrt / (b.rn + b.pn) : D<N>(0.0);
+    case ABCMB_PT_NCDM_SIGMA: return L.nq > 0 ? nrs / (b.rn + b.pn) : D<N>(0.0);
     default: break;
   }
-  double srd = b.rc * y[1] + b.rb * y[2] + b.rg * y[L.ig] + b.ru * y[L.iu];
-  double srt = b.rb * y[3] + (b.rg + b.rg / 3.) * y[L.ig + 1] + (b.ru + b.ru / 3.) * y[L.iu + 1];
-  double srs = (b.rg + b.rg / 3.) * y[L.ig + 2] + (b.ru + b.ru / 3.) * y[L.iu + 2];
-  double srdm = b.rc * y[1] + b.rb * y[2], srm = b.rc + b.rb;
+  D<N> srd = b.rc * y(1) + b.rb * y(2) + b.rg * y(L.ig) + b.ru * y(L.iu);
+  D<N> srt = b.rb * y(3) + (b.rg + b.rg / 3.) * y(L.ig + 1) + (b.ru + b.ru / 3.) * y(L.iu + 1);
+  D<N> srs = (b.rg + b.rg / 3.) * y(L.ig + 2) + (b.ru + b.ru / 3.) * y(L.iu + 2);
+  D<N> srdm = b.rc * y(1) + b.rb * y(2), srm = b.rc + b.rb;
   if (L.nq > 0) { srd += nrd; srt += nrt; srs += nrs; srdm += nrd; srm += b.rn; }
   if (f == ABCMB_PT_DELTA_M) return srdm / srm;
   double KG = 4. * PI * GN * a * a / (C_MPC * C_MPC);
-  double hp = 2. / (aH * aH) * (k * k * y[0] + KG * srd);
-  double ep = KG / aH * srt / (k * k);
+  D<N> hp = 2. / (aH * aH) * (k * k * y(0) + KG * srd);
+  D<N> ep = KG / aH * srt / (k * k);
   if (f == ABCMB_PT_H_PRIME) return hp;
   if (f == ABCMB_PT_ETA_PRIME) return ep;
-  double al = aH * (hp + 6. * ep) / 2. / (k * k);
+  D<N> al = aH * (hp + 6. * ep) / 2. / (k * k);
   if (f == ABCMB_PT_ALPHA) return al;
-  return y[0] / aH - 2. * al - 3. * KG / aH * srs / (k * k);  // ABCMB_PT_ALPHA_PRIME
+  return y(0) / aH - 2. * al - 3. * KG / aH * srs / (k * k);  // ABCMB_PT_ALPHA_PRIME
 }
 
-template <int NT, int BMAX>
+template <int NT, int BMAX, int N>
 __global__ void __launch_bounds__(NT) pert_fwd_kernel(const AbcmbCosmo* __restrict__ cosmos,
+                                                      const AbcmbTangent* __restrict__ tangents,
                                                       const AbcmbPertItem* __restrict__ items,
                                                       AbcmbOdeSettings st, int32_t* __restrict__ status_out,
                                                       int32_t* __restrict__ steps_out, int32_t* __restrict__ iters_out,
                                                       double* __restrict__ state_out) {
-  __shared__ Smem<NT, BMAX> sm;
+  __shared__ Smem<NT, BMAX, N> sm;
   const int r = threadIdx.x;
   const int item = blockIdx.x;
   const AbcmbPertItem I = items[item];
   const AbcmbCosmo& c = cosmos[I.cosmo];
+  const AbcmbTangent* T = nullptr;
+  if constexpr (N > 0) T = tangents + I.cosmo;
+  const Cosmo<N> P{c, T};
   const double k = I.k;
   Layout L;
   L.lg = st.l_g; L.lp = st.l_pol; L.lu = st.l_ur; L.ln = st.l_ncdm; L.nq = st.n_q; L.n = st.n;
@@ -602,34 +757,42 @@ __global__ void __launch_bounds__(NT) pert_fwd_kernel(const AbcmbCosmo* __restri
 #pragma unroll
   for (int j = 0; j < 5; ++j) R.lc[j] = R.w.col[j] - R.B.start;
   const int maxb = max(L.iu - 2, max(L.lu + 1, L.nq > 0 ? L.ln + 1 : 1));
-  sm.zl[r] = 0.0; sm.rl[r] = 0.0;
+  for (int q = 0; q < 1 + N; ++q) {
+    sm.zl[q][r] = 0.0; sm.rl[q][r] = 0.0;
+    if (r < BMAX) { sm.zl[q][NT + r] = 0.0; sm.rl[q][NT + r] = 0.0; }
+  }
   sm.uh[r] = r < n ? R.w.uh : 0.0;
   sm.ue[r] = r < n ? R.w.ue : 0.0;
-  if (r < BMAX) { sm.zl[NT + r] = 0.0; sm.rl[NT + r] = 0.0; sm.uh[NT + r] = 0.0; sm.ue[NT + r] = 0.0; }
+  if (r < BMAX) { sm.uh[NT + r] = 0.0; sm.ue[NT + r] = 0.0; }
 
   const double rtol = I.rtol, atol = I.atol;
   const int n_save = c.n_save;
   const double* __restrict__ lna_save = c.lna_save;
 
   double t = I.lna_start;
-  double ti = fast_interp(t, ABCMB_TAU_TAB_LO, 0.0, c.tau_tab, ABCMB_TAU_TAB_N);
-  double y = r < n ? ic_row(c, L, k, ti, r) : 0.0;
-  set_times(sm, c, L, k, t, 0.0, 1, st.jac_offset);
-  double f0 = rhs_eval(sm, R, 0, y);
+  const D<N> ti = fast_interp<N>(t, ABCMB_TAU_TAB_LO, 0.0, c.tau_tab, P.dtau(), ABCMB_TAU_TAB_N);
+  D<N> y = r < n ? ic_row(P, L, k, ti, r) : D<N>(0.0);
+  set_times(sm, P, L, k, t, 0.0, 1, st.jac_offset);
+  D<N> f0 = rhs_eval(sm, R, 0, y);
   double h = st.dt0;
   const double b1 = st.pcoeff + st.icoeff + st.dcoeff, b2 = -(st.pcoeff + 2.0 * st.dcoeff);
   double prev_inv = 1.0;
   int isave = 0, steps = 0, iters = 0, status = ABCMB_K_OK;
   Inv<BMAX> FI;
 
-  auto save_point = [&](double ys, double ls) {
+  auto save_point = [&](const D<N>& ys, double ls) {
     __syncthreads();
-    sm.zl[r] = ys;
+    put_col<NT, BMAX, N>(sm.zl, r, ys);
     __syncthreads();
-    if (state_out && r < n) state_out[((size_t)item * n_save + isave) * n + r] = ys;
+    if (state_out && r < n) state_out[((size_t)item * n_save + isave) * n + r] = ys.v;
     if (r < ABCMB_N_PT_FIELDS) {
-      double v = output_field(c, L, k, ls, sm.zl, r);
-      c.pt[((size_t)r * n_save + isave) * c.nk + I.ik] = v;
+      D<N> v = output_field<N, NT + BMAX>(P, L, k, ls, sm.zl, r);
+      const size_t o = ((size_t)r * n_save + isave) * c.nk + I.ik;
+      c.pt[o] = v.v;
+      if constexpr (N > 0) {
+        const size_t dstride = (size_t)ABCMB_N_PT_FIELDS * n_save * c.nk;
+        ABCMB_DLOOP(N) T->pt[i_ * dstride + o] = v.d[i_];
+      }
     }
   };
 
@@ -643,49 +806,71 @@ __global__ void __launch_bounds__(NT) pert_fwd_kernel(const AbcmbCosmo* __restri
     if (!(hh > 0.0)) { status = ABCMB_K_STEP_UNDERFLOW; break; }
     ++steps;
     const double g = hh * KV_G;
-    set_times(sm, c, L, k, t, hh, NTIMES, st.jac_offset);
+    set_times(sm, P, L, k, t, hh, NTIMES, st.jac_offset);
     bool ok = factor(sm, R, maxb, g, FI);
-    sm.Fs[0][r] = f0;
-    double z = y;
+    put_F(sm, 0, r, f0);
+    D<N> z = y;
 #pragma unroll 1
     for (int s = 1; s < 7; ++s) {
       if (!ok) break;
-      double acc = 0.0;
-      for (int j = 0; j < s; ++j) acc += KV_A[s][j] * sm.Fs[j][r];
-      const double base = y + hh * acc;
-      z = base + g * sm.Fs[s - 1][r];
+      D<N> acc(0.0);
+      for (int j = 0; j < s; ++j) acc += KV_A[s][j] * get_F(sm, j, r);
+      const D<N> base = y + hh * acc;
+      z = base + g * get_F(sm, s - 1, r);
       const int slot = STAGE_SLOT[s];
-      bool conv = false;
+      bool conv = false;      // value converged
+      bool tconv = N == 0;    // tangents converged
       double prev = 0.0;
-      for (int it = 0; it < st.max_newton; ++it) {
+      for (int it = 0;; ++it) {
+        if (!conv && it >= st.max_newton) break;
         ++iters;
-        double fz = rhs_eval(sm, R, slot, z);
-        double dz = solve(sm, R, FI, -(z - base - g * fz));
-        z += dz;
-        double v[1] = {0.0};
-        if (r < n) { double q = dz / (atol + rtol * fabs(z)); v[0] = q * q; }
-        block_sum<NT, 1>(v, sm.red);
-        double nrm = sqrt(v[0] / n);
-        if (!isfinite(nrm)) break;
-        if (nrm < NEWTON_SMALL) { conv = true; break; }
-        if (it > 0) {
-          double th = nrm / prev;
-          if (th > 2.0) break;
-          double fac = nrm * th / (1.0 - th);
-          if (fac > 0.0 && fac < st.newton_kappa) { conv = true; break; }
+        D<N> fz = rhs_eval(sm, R, slot, z);
+        D<N> dz = solve(sm, R, FI, -(z - base - g * fz));
+        if (!conv) {
+          z += dz;
+        } else if constexpr (N > 0) {  // the value is final: iterate the tangents alone
+          ABCMB_DLOOP(N) z.d[i_] += dz.d[i_];
         }
-        prev = nrm;
+        double v[1 + 2 * N];
+        v[0] = 0.0;
+        if (!conv && r < n) { double q = dz.v / (atol + rtol * fabs(z.v)); v[0] = q * q; }
+        if constexpr (N > 0) {
+          const double w = r < n ? 1.0 / (atol + rtol * fabs(z.v)) : 0.0;
+          ABCMB_DLOOP(N) {
+            v[1 + 2 * i_] = (dz.d[i_] * w) * (dz.d[i_] * w);
+            v[2 + 2 * i_] = (z.d[i_] * w) * (z.d[i_] * w);
+          }
+        }
+        block_sum<NT, 1 + 2 * N>(v, sm.red);
+        if (!conv) {
+          double nrm = sqrt(v[0] / n);
+          if (!isfinite(nrm)) break;
+          if (nrm < NEWTON_SMALL) {
+            conv = true;
+          } else if (it > 0) {
+            double th = nrm / prev;
+            if (th > 2.0) break;
+            double fac = nrm * th / (1.0 - th);
+            if (fac > 0.0 && fac < st.newton_kappa) conv = true;
+          }
+          prev = nrm;
+        }
+        if constexpr (N > 0) {
+          tconv = true;
+          ABCMB_DLOOP(N) tconv = tconv && v[1 + 2 * i_] <= TANGENT_RTOL * TANGENT_RTOL * v[2 + 2 * i_];
+        }
+        if (conv && (tconv || it + 1 >= st.max_newton + TANGENT_EXTRA)) break;
       }
       if (!conv) { ok = false; break; }
-      sm.Fs[s][r] = (z - base) / g;
+      put_F(sm, s, r, (z - base) / g);
     }
     double enorm = INFINITY;
-    const double ynew = z;
+    const D<N> ynew = z;
     if (ok) {
       double acc = 0.0;
 #pragma unroll
-      for (int j = 0; j < 7; ++j) acc += KV_E[j] * sm.Fs[j][r];
-      double v[1] = {r < n ? rms_term(hh * acc, y, ynew, rtol, atol) : 0.0};
+      for (int j = 0; j < 7; ++j) acc += KV_E[j] * sm.Fs[j][0][r];
+      double v[1] = {r < n ? rms_term(hh * acc, y.v, ynew.v, rtol, atol) : 0.0};
       block_sum<NT, 1>(v, sm.red);
       enorm = sqrt(v[0] / n);
       if (!isfinite(enorm)) enorm = INFINITY;
@@ -697,12 +882,12 @@ __global__ void __launch_bounds__(NT) pert_fwd_kernel(const AbcmbCosmo* __restri
     if (!(fac >= fmin)) fac = fmin;
     if (fac > st.factormax) fac = st.factormax;
     if (accept) {
-      double fnew = rhs_eval(sm, R, 5, ynew);
+      D<N> fnew = rhs_eval(sm, R, 5, ynew);
       while (isave < n_save && lna_save[isave] <= tn) {
         double th = (lna_save[isave] - t) / hh;
         double h00 = 2 * th * th * th - 3 * th * th + 1, h10 = th * th * th - 2 * th * th + th;
         double h01 = -2 * th * th * th + 3 * th * th, h11 = th * th * th - th * th;
-        double ys = h00 * y + h10 * hh * f0 + h01 * ynew + h11 * hh * fnew;
+        D<N> ys = h00 * y + h10 * hh * f0 + h01 * ynew + h11 * hh * fnew;
         save_point(ys, lna_save[isave]);
         ++isave;
       }
@@ -720,13 +905,43 @@ __global__ void __launch_bounds__(NT) pert_fwd_kernel(const AbcmbCosmo* __restri
   }
 }
 
-template <int NT, int BMAX>
-int launch(const AbcmbCosmo* cosmos, const AbcmbPertItem* items, int n_items, const AbcmbOdeSettings& st,
-           int32_t* status, int32_t* steps, int32_t* iters, double* state, hipStream_t stream) {
-  hipLaunchKernelGGL((pert_fwd_kernel<NT, BMAX>), dim3(n_items), dim3(NT), 0, stream, cosmos, items, st, status,
-                     steps, iters, state);
+template <int NT, int BMAX, int N>
+int launch(const AbcmbCosmo* cosmos, const AbcmbTangent* tangents, const AbcmbPertItem* items, int n_items,
+           const AbcmbOdeSettings& st, int32_t* status, int32_t* steps, int32_t* iters, double* state,
+           hipStream_t stream) {
+  hipLaunchKernelGGL((pert_fwd_kernel<NT, BMAX, N>), dim3(n_items), dim3(NT), 0, stream, cosmos, tangents, items,
+                     st, status, steps, iters, state);
   ABCMB_HIP_CHECK(hipGetLastError());
   return ABCMB_OK;
+}
+
+// checks shared by both entry points; sets the block bound
+int check_settings(const AbcmbOdeSettings& st, int& maxb) {
+  ABCMB_REQUIRE(st.l_g >= 5 && st.l_pol >= 4 && st.l_ur >= 5, "hierarchy cut-offs too small");
+  ABCMB_REQUIRE(st.n_q == 0 || ((st.n_q == 3 || st.n_q == 5) && st.l_ncdm >= 4), "bad massive-nu layout");
+  int n_expect = 4 + (st.l_g + 1) + (st.l_pol + 1) + (st.l_ur + 1) + st.n_q * (st.l_ncdm + 1);
+  ABCMB_REQUIRE(st.n == n_expect, "settings.n=%d does not match the layout (%d)", st.n, n_expect);
+  ABCMB_REQUIRE(st.max_newton > 0 && st.max_steps > 0, "bad iteration limits");
+  maxb = 2 + (st.l_g + 1) + (st.l_pol + 1);
+  if (st.l_ur + 1 > maxb) maxb = st.l_ur + 1;
+  if (st.n_q > 0 && st.l_ncdm + 1 > maxb) maxb = st.l_ncdm + 1;
+  ABCMB_REQUIRE(maxb <= 64, "a hierarchy block of %d rows exceeds 64", maxb);
+  ABCMB_REQUIRE(st.n <= 192, "state size %d exceeds 192", st.n);
+  return ABCMB_OK;
+}
+
+template <int N>
+int dispatch(const AbcmbCosmo* cosmos, const AbcmbTangent* tangents, const AbcmbPertItem* items, int n_items,
+             const AbcmbOdeSettings& st, int maxb, int32_t* status, int32_t* steps, int32_t* iters, double* state,
+             hipStream_t s) {
+  if (maxb <= 32) {
+    if (st.n <= 64) return launch<64, 32, N>(cosmos, tangents, items, n_items, st, status, steps, iters, state, s);
+    if (st.n <= 128) return launch<128, 32, N>(cosmos, tangents, items, n_items, st, status, steps, iters, state, s);
+    return launch<192, 32, N>(cosmos, tangents, items, n_items, st, status, steps, iters, state, s);
+  }
+  if (st.n <= 64) return launch<64, 64, N>(cosmos, tangents, items, n_items, st, status, steps, iters, state, s);
+  if (st.n <= 128) return launch<128, 64, N>(cosmos, tangents, items, n_items, st, status, steps, iters, state, s);
+  return launch<192, 64, N>(cosmos, tangents, items, n_items, st, status, steps, iters, state, s);
 }
 
 }  // namespace
@@ -746,26 +961,22 @@ extern "C" int abcmb_pert_fwd(const AbcmbCosmo* cosmos_dev, int n_cosmo, const A
   (void)ws_bytes;
   ABCMB_REQUIRE(cosmos_dev && items_dev && settings && status_dev && steps_dev, "null argument");
   ABCMB_REQUIRE(n_cosmo > 0 && n_items > 0, "empty launch");
-  const AbcmbOdeSettings& st = *settings;
-  ABCMB_REQUIRE(st.l_g >= 5 && st.l_pol >= 4 && st.l_ur >= 5, "hierarchy cut-offs too small");
-  ABCMB_REQUIRE(st.n_q == 0 || ((st.n_q == 3 || st.n_q == 5) && st.l_ncdm >= 4), "bad massive-nu layout");
-  int n_expect = 4 + (st.l_g + 1) + (st.l_pol + 1) + (st.l_ur + 1) + st.n_q * (st.l_ncdm + 1);
-  ABCMB_REQUIRE(st.n == n_expect, "settings.n=%d does not match the layout (%d)", st.n, n_expect);
-  ABCMB_REQUIRE(st.max_newton > 0 && st.max_steps > 0, "bad iteration limits");
-  int maxb = 2 + (st.l_g + 1) + (st.l_pol + 1);
-  if (st.l_ur + 1 > maxb) maxb = st.l_ur + 1;
-  if (st.n_q > 0 && st.l_ncdm + 1 > maxb) maxb = st.l_ncdm + 1;
-  ABCMB_REQUIRE(maxb <= 64, "a hierarchy block of %d rows exceeds 64", maxb);
-  hipStream_t s = (hipStream_t)stream;
-  if (maxb <= 32) {
-    if (st.n <= 64) return launch<64, 32>(cosmos_dev, items_dev, n_items, st, status_dev, steps_dev, iters_dev, state_dev, s);
-    if (st.n <= 128) return launch<128, 32>(cosmos_dev, items_dev, n_items, st, status_dev, steps_dev, iters_dev, state_dev, s);
-    if (st.n <= 192) return launch<192, 32>(cosmos_dev, items_dev, n_items, st, status_dev, steps_dev, iters_dev, state_dev, s);
-  } else {
-    if (st.n <= 64) return launch<64, 64>(cosmos_dev, items_dev, n_items, st, status_dev, steps_dev, iters_dev, state_dev, s);
-    if (st.n <= 128) return launch<128, 64>(cosmos_dev, items_dev, n_items, st, status_dev, steps_dev, iters_dev, state_dev, s);
-    if (st.n <= 192) return launch<192, 64>(cosmos_dev, items_dev, n_items, st, status_dev, steps_dev, iters_dev, state_dev, s);
-  }
-  set_error("state size %d exceeds 192", st.n);
-  return ABCMB_EINVAL;
+  int maxb = 0;
+  int rc = check_settings(*settings, maxb);
+  if (rc != ABCMB_OK) return rc;
+  return dispatch<0>(cosmos_dev, nullptr, items_dev, n_items, *settings, maxb, status_dev, steps_dev, iters_dev,
+                     state_dev, (hipStream_t)stream);
+}
+
+extern "C" int abcmb_pert_tangent(const AbcmbCosmo* cosmos_dev, const AbcmbTangent* tangents_dev, int n_cosmo,
+                                  const AbcmbPertItem* items_dev, int n_items, const AbcmbOdeSettings* settings,
+                                  int32_t* status_dev, int32_t* steps_dev, int32_t* iters_dev, void* stream) {
+  using namespace abcmb;
+  ABCMB_REQUIRE(cosmos_dev && tangents_dev && items_dev && settings && status_dev && steps_dev, "null argument");
+  ABCMB_REQUIRE(n_cosmo > 0 && n_items > 0, "empty launch");
+  int maxb = 0;
+  int rc = check_settings(*settings, maxb);
+  if (rc != ABCMB_OK) return rc;
+  return dispatch<ABCMB_TANGENT_DIRS>(cosmos_dev, tangents_dev, items_dev, n_items, *settings, maxb, status_dev,
+                                      steps_dev, iters_dev, nullptr, (hipStream_t)stream);
 }

@@ -16,9 +16,19 @@ Gradients:
     the base point; ``batched=True`` puts all evaluations of all cosmologies
     into one ODE launch, ``batched=False`` is the plain path (one cosmology
     per launch).
+  - ``tangent_gradients``: one tangent-linear ODE launch for all cosmologies
+    gives the perturbation tables and their exact derivatives along the five
+    parameters that reach the perturbation equations (omega_b, omega_cdm, h,
+    tau_reio, m_nu); the host inputs' derivatives come from central
+    differences of the (cheap, deterministic) host producer.  The
+    downstream (sources, LOS, lensing) is then differenced along each
+    direction with the tables moved along their tangents, and ln A_s, n_s
+    (which enter only the primordial spectrum) are differenced there too.
 """
 from __future__ import annotations
 
+import copy
+import dataclasses
 import math
 
 import numpy as np
@@ -27,6 +37,7 @@ NAMES = ("omega_b", "omega_cdm", "h", "ln10As", "n_s", "tau_reion", "m_nu")
 RANGES = np.array([(0.020, 0.024), (0.10, 0.14), (0.62, 0.74), (2.9, 3.2), (0.92, 1.0), (0.03, 0.09), (0.06, 0.3)])
 FIDUCIAL = np.array([0.02237, 0.1200, 0.6736, math.log(1e10 * 2.1e-9), 0.9649, 0.0544, 0.06])
 FD_REL_STEP = 1e-4
+ODE_DIRS = (0, 1, 2, 5, 6)   # parameters that reach the perturbation equations
 
 
 def theta_to_params(theta) -> dict:
@@ -62,11 +73,14 @@ class LensedClLikelihood:
 
     def loglike(self, outs, mock_index):
         import torch
-        vals = []
-        for o, i in zip(outs, mock_index):
-            cl = torch.stack([o.ClTT, o.ClTE, o.ClEE])
-            vals.append(-0.5 * torch.sum((cl - self.mock[i]) ** 2 * self.inv_var[i]))
-        return torch.stack(vals)
+        cl = torch.stack([torch.stack([o.ClTT, o.ClTE, o.ClEE]) for o in outs])
+        return self.loglike_cl(cl, mock_index)
+
+    def loglike_cl(self, cl, mock_index):
+        """L for spectra cl[B][3][n_l] (TT, TE, EE on l = 2..l_max) against mocks ``mock_index``."""
+        import torch
+        idx = torch.as_tensor(np.asarray(mock_index, dtype=np.int64), device=cl.device)
+        return -0.5 * torch.sum((cl - self.mock[idx]) ** 2 * self.inv_var[idx], dim=(1, 2))
 
 
 def fd_gradients(model, like, thetas, k_grids=None, batched=True, mock_index=None):
@@ -103,5 +117,71 @@ def fd_gradients(model, like, thetas, k_grids=None, batched=True, mock_index=Non
             L[i] = float(like.loglike(o, [all_slots[i]])[0])
     Lb = L[:nc]
     Lp = L[nc:].reshape(nc, 7, 2)
+    grad = (Lp[:, :, 0] - Lp[:, :, 1]) / (2 * steps)
+    return Lb, grad
+
+
+def with_primordial(plan, A_s: float, n_s: float):
+    """Copy of ``plan`` with another primordial spectrum (shares every table)."""
+    from . import abi
+    bg = copy.copy(plan.bg)
+    bg.cosmo = abi.AbcmbCosmo.from_buffer_copy(plan.bg.cosmo)
+    bg.cosmo.A_s, bg.cosmo.n_s = A_s, n_s
+    return dataclasses.replace(plan, params=dict(plan.params, A_s=A_s, n_s=n_s), bg=bg)
+
+
+def tangent_gradients(model, like, thetas, k_grids=None, mock_index=None):
+    """d L / d theta for each row of ``thetas`` from one tangent-linear ODE launch."""
+    from .model import tangent_seeds
+    thetas = np.atleast_2d(thetas)
+    nc = thetas.shape[0]
+    mock_index = list(range(nc)) if mock_index is None else list(mock_index)
+    k_grids = k_grids or [None] * nc
+    steps = FD_REL_STEP * np.abs(thetas)
+    base_plans = model.prepare_many([model.derived(theta_to_params(t)) for t in thetas], k_list=k_grids)
+    pm_params, freeze, ks = [], [], []
+    for c in range(nc):
+        for p in ODE_DIRS:
+            for sgn in (+1, -1):
+                t = thetas[c].copy()
+                t[p] += sgn * steps[c, p]
+                pm_params.append(model.derived(theta_to_params(t)))
+                freeze.append(base_plans[c])
+                ks.append(k_grids[c])
+    pm = model.prepare_many(pm_params, freeze_list=freeze, k_list=ks)
+    nd = len(ODE_DIRS)
+    pm = [pm[c * 2 * nd:(c + 1) * 2 * nd] for c in range(nc)]
+    seeds = [tangent_seeds(pm[c][0::2], pm[c][1::2], [steps[c, p] for p in ODE_DIRS]) for c in range(nc)]
+    outs, dpts = model.execute_tangent(base_plans, seeds)
+    Lb = like.loglike(outs, mock_index).cpu().numpy()
+
+    # downstream along each direction: 14 evaluations per cosmology, in groups
+    per = 2 * 7
+    group = max(1, model.chunk // per)
+    Lp = np.zeros((nc, 7, 2))
+    for c0 in range(0, nc, group):
+        cs = range(c0, min(nc, c0 + group))
+        plans, pts, kdev, slots = [], [], [], []
+        for c in cs:
+            pt = outs[c].PT.table
+            kd = outs[c].PT.k_dev
+            for p in range(7):
+                for si, sgn in enumerate((+1, -1)):
+                    h = sgn * steps[c, p]
+                    if p in ODE_DIRS:
+                        d = ODE_DIRS.index(p)
+                        plans.append(pm[c][2 * d + si])
+                        pts.append(pt + h * dpts[c][d])
+                    else:
+                        t = thetas[c].copy()
+                        t[p] += h
+                        q = theta_to_params(t)
+                        plans.append(with_primordial(base_plans[c], q["A_s"], q["n_s"]))
+                        pts.append(pt)
+                    kdev.append(kd)
+                    slots.append(mock_index[c])
+        ds = model._downstream(plans, pts, kdev)
+        L = like.loglike_cl(model.observed_cl(ds), slots).cpu().numpy().reshape(len(cs), 7, 2)
+        Lp[c0:c0 + len(cs)] = L
     grad = (Lp[:, :, 0] - Lp[:, :, 1]) / (2 * steps)
     return Lb, grad

@@ -57,6 +57,53 @@ class CosmoPlan:
     k: np.ndarray = None          # PT k-grid of this cosmology (Mpc^-1)
 
 
+@dataclass
+class TangentSeeds:
+    """Derivatives of one cosmology's host inputs along the tangent directions.
+
+    par: [N_TANGENT_DIRS][31] derivatives of the AbcmbCosmo double fields;
+    tau, xe, tm: [N_TANGENT_DIRS][len] derivatives of the tables.  Directions
+    beyond the ones a caller asks for are zero.
+    """
+    par: np.ndarray
+    tau: np.ndarray
+    xe: np.ndarray
+    tm: np.ndarray
+
+
+def cosmo_doubles(c) -> np.ndarray:
+    """The 31 double fields of an AbcmbCosmo, in declaration order."""
+    return np.frombuffer(bytes(c), dtype=np.float64, count=abi.COSMO_DOUBLES).copy()
+
+
+def tangent_seeds(plus, minus, steps) -> TangentSeeds:
+    """Central differences of the host producer between plans at theta +- step.
+
+    ``plus[d]``/``minus[d]`` were prepared with every grid frozen at the base
+    plan, so the tables share their abscissae and the differences are smooth
+    functions of the parameters.  The producer (background integrals,
+    recombination, reionization root) is deterministic and converged to
+    rounding, so a relative step of 1e-4 leaves differences accurate to ~1e-8.
+    """
+    nd = abi.N_TANGENT_DIRS
+    if not (len(plus) == len(minus) == len(steps) <= nd):
+        raise ValueError(f"need at most {nd} directions with a plus and a minus plan each")
+    p0 = plus[0].bg
+    par = np.zeros((nd, abi.COSMO_DOUBLES))
+    tau = np.zeros((nd, p0.tau_tab.size))
+    xe = np.zeros((nd, p0.xe_tab.size))
+    tm = np.zeros((nd, p0.tm_tab.size))
+    for d, (pp, pm, h) in enumerate(zip(plus, minus, steps)):
+        if pp.bg.xe_tab.size != xe.shape[1] or pm.bg.xe_tab.size != xe.shape[1]:
+            raise ValueError("plus/minus plans have different recombination grids")
+        s = 1.0 / (2.0 * h)
+        par[d] = (cosmo_doubles(pp.bg.cosmo) - cosmo_doubles(pm.bg.cosmo)) * s
+        tau[d] = (pp.bg.tau_tab - pm.bg.tau_tab) * s
+        xe[d] = (pp.bg.xe_tab - pm.bg.xe_tab) * s
+        tm[d] = (pp.bg.tm_tab - pm.bg.tm_tab) * s
+    return TangentSeeds(par, tau, xe, tm)
+
+
 class PerturbationTable:
     """Perturbation table (abcmb/perturbations.py:406-450); arrays are (Nlna, Nk) views."""
 
@@ -238,87 +285,151 @@ class Model:
         plist = [p if derived else self.derived(p) for p in params_list]
         return self.execute(self.prepare_many(plist))
 
-    def execute(self, plans, want_state: bool = False, sync: bool = True):
+    def _cosmo_array(self, plans, pts, kdev, dt, tables=True):
+        """AbcmbCosmo structs (device pointers) for ``plans`` with perturbation tables ``pts``."""
+        ns = self.n_save
+        cos = []
+        for pl, pt, kd in zip(plans, pts, kdev):
+            bg = pl.bg
+            c = abi.AbcmbCosmo.from_buffer_copy(bg.cosmo)
+            if tables:
+                c.tau_tab = dt(bg.tau_tab).data_ptr()
+                c.xe_tab = dt(bg.xe_tab).data_ptr()
+                c.tm_tab = dt(bg.tm_tab).data_ptr()
+            else:   # downstream kernels read parameters, k, the save grid and the table only
+                c.tau_tab = c.xe_tab = c.tm_tab = None
+            c.lna_save = dt(pl.lna_save).data_ptr()
+            c.k = kd.data_ptr()
+            c.n_save, c.nk = ns, int(kd.numel())
+            c.pt = pt.data_ptr()
+            cos.append(c)
+        return cos
+
+    @staticmethod
+    def _struct_array(torch, dev, structs, dt):
+        arr = (type(structs[0]) * len(structs))(*structs)
+        return dt(np.frombuffer(bytes(arr), dtype=np.uint8), torch.uint8)
+
+    def _run_ode(self, plans, tangents=None, want_state: bool = False):
+        """Perturbation tables of ``plans`` (one ODE launch for all their k-modes).
+
+        ``tangents``: one ``TangentSeeds`` per plan selects the tangent-linear
+        kernel, which also returns the tables' derivatives along the seeds'
+        directions ([N_TANGENT_DIRS][21][n_save][nk] per cosmology).
+        """
         torch = _torch()
         lib = abi.hip_lib()
         D = self._device_constants()
         dev = torch.device("cuda")
         stream = torch.cuda.current_stream().cuda_stream
         ns = self.n_save
-        ncos = len(plans)
-        keep = []  # host/device buffers that must outlive the launches
+        keep = []
 
         def dt(a, dtype=torch.float64):
             x = torch.as_tensor(np.ascontiguousarray(a), dtype=dtype).to(dev, non_blocking=False)
             keep.append(x)
             return x
 
-        cos_arr = (abi.AbcmbCosmo * ncos)()
-        pts = []
-        kdev = []
-        for i, pl in enumerate(plans):
-            kk = pl.k if pl.k is not None else self.k_pt
-            kd = D["k_pt"] if kk is self.k_pt else dt(kk)
-            kdev.append(kd)
-            nk = kk.size
-            bg = pl.bg
-            c = abi.AbcmbCosmo.from_buffer_copy(bg.cosmo)
-            c.tau_tab = dt(bg.tau_tab).data_ptr()
-            c.xe_tab = dt(bg.xe_tab).data_ptr()
-            c.tm_tab = dt(bg.tm_tab).data_ptr()
-            c.lna_save = dt(pl.lna_save).data_ptr()
-            c.k = kd.data_ptr()
-            c.n_save, c.nk = ns, nk
-            pt = torch.empty((abi.N_PT_FIELDS, ns, nk), dtype=torch.float64, device=dev)
-            pts.append(pt)
-            c.pt = pt.data_ptr()
-            cos_arr[i] = c
-        cos_dev = dt(np.frombuffer(bytes(cos_arr), dtype=np.uint8), torch.uint8)
-        # work items: all (cosmology, k), largest k first
         allk = [(pl.k if pl.k is not None else self.k_pt) for pl in plans]
-        flat = [(-allk[ci][ik], ci, ik) for ci in range(ncos) for ik in range(allk[ci].size)]
+        kdev = [D["k_pt"] if kk is self.k_pt else dt(kk) for kk in allk]
+        pts = [torch.empty((abi.N_PT_FIELDS, ns, kk.size), dtype=torch.float64, device=dev) for kk in allk]
+        cos = self._cosmo_array(plans, pts, kdev, dt)
+        cos_dev = self._struct_array(torch, dev, cos, dt)
+        # work items: all (cosmology, k), largest k first
+        flat = [(-allk[ci][ik], ci, ik) for ci in range(len(plans)) for ik in range(allk[ci].size)]
         flat.sort()
         n_items = len(flat)
-        items = (abi.AbcmbPertItem * n_items)()
-        for m, (_, ci, ik) in enumerate(flat):
-            pl = plans[ci]
-            items[m] = abi.AbcmbPertItem(k=allk[ci][ik], lna_start=pl.lna_start[ik], rtol=pl.rtol[ik],
-                                         atol=pl.atol[ik], cosmo=ci, ik=int(ik))
-        items_dev = dt(np.frombuffer(bytes(items), dtype=np.uint8), torch.uint8)
+        items = [abi.AbcmbPertItem(k=allk[ci][ik], lna_start=plans[ci].lna_start[ik], rtol=plans[ci].rtol[ik],
+                                   atol=plans[ci].atol[ik], cosmo=ci, ik=int(ik)) for _, ci, ik in flat]
+        items_dev = self._struct_array(torch, dev, items, dt)
         status = torch.zeros(n_items, dtype=torch.int32, device=dev)
         steps = torch.zeros(n_items, dtype=torch.int32, device=dev)
         iters = torch.zeros(n_items, dtype=torch.int32, device=dev)
         state = None
         if want_state:
             state = torch.zeros((n_items, ns, self.layout.n), dtype=torch.float64, device=dev)
-        st = self.settings
+        dpts = None
+        if tangents is not None:
+            nd = abi.N_TANGENT_DIRS
+            dpts = [torch.empty((nd, abi.N_PT_FIELDS, ns, kk.size), dtype=torch.float64, device=dev)
+                    for kk in allk]
+            tans = [abi.AbcmbTangent(par=dt(sd.par).data_ptr(), tau_tab=dt(sd.tau).data_ptr(),
+                                     xe_tab=dt(sd.xe).data_ptr(), tm_tab=dt(sd.tm).data_ptr(), pt=dp.data_ptr())
+                    for sd, dp in zip(tangents, dpts)]
+            tan_dev = self._struct_array(torch, dev, tans, dt)
         ev0 = ev1 = None
         if self.timing is not None:
             ev0 = torch.cuda.Event(enable_timing=True)
             ev1 = torch.cuda.Event(enable_timing=True)
             ev0.record()
-        abi.check(lib.abcmb_pert_fwd(cos_dev.data_ptr(), ncos, items_dev.data_ptr(), n_items, C.byref(st),
-                                     status.data_ptr(), steps.data_ptr(), iters.data_ptr(),
-                                     state.data_ptr() if state is not None else None, None, 0, stream),
-                  "abcmb_pert_fwd")
+        st = self.settings
+        if tangents is None:
+            abi.check(lib.abcmb_pert_fwd(cos_dev.data_ptr(), len(plans), items_dev.data_ptr(), n_items, C.byref(st),
+                                         status.data_ptr(), steps.data_ptr(), iters.data_ptr(),
+                                         state.data_ptr() if state is not None else None, None, 0, stream),
+                      "abcmb_pert_fwd")
+        else:
+            abi.check(lib.abcmb_pert_tangent(cos_dev.data_ptr(), tan_dev.data_ptr(), len(plans), items_dev.data_ptr(),
+                                             n_items, C.byref(st), status.data_ptr(), steps.data_ptr(),
+                                             iters.data_ptr(), stream), "abcmb_pert_tangent")
         if ev1 is not None:
             ev1.record()
-        outs = []
+        torch.cuda.synchronize()
+        st_h = status.cpu().numpy()
+        steps_h = steps.cpu().numpy()
+        if self.timing is not None:
+            self.timing.append(dict(ms=ev0.elapsed_time(ev1), steps=int(steps_h.sum()),
+                                    iters=int(iters.sum().item()), items=n_items,
+                                    kernel="tangent" if tangents is not None else "primal"))
+        bad = np.nonzero(st_h != abi_k_ok())[0]
+        if bad.size:
+            where = [(flat[b][1], float(allk[flat[b][1]][flat[b][2]])) for b in bad[:8]]
+            raise RuntimeError(f"perturbation ODE failed for {bad.size} (cosmology, k) items; status codes "
+                               f"{np.unique(st_h[bad]).tolist()} (1 = max_steps_PE reached, 2 = non-finite, "
+                               f"3 = step-size underflow); first (cosmology, k): {where}")
+        steps_ck = [np.zeros(a.size, dtype=np.int32) for a in allk]
+        for m, (_, ci, ik) in enumerate(flat):
+            steps_ck[ci][ik] = steps_h[m]
+        return dict(pts=pts, dpts=dpts, kdev=kdev, allk=allk, steps=steps_ck, state=state, flat=flat, keep=keep)
+
+    def _downstream(self, plans, pts, kdev):
+        """Sources, LOS C_l, l-spline, lensing and P(k) for ``plans`` with tables ``pts``.
+
+        Runs in chunks of ``self.chunk`` cosmologies.  Returns [B][3][n_ell]
+        sampled C_l, [B][3][n_l] C_l on every l, and when lensing is on
+        [B][n_l] C_l^phiphi and [B][4][n_l] lensed TT, TE, EE, BB; [B][nq] P(k).
+        """
+        torch = _torch()
+        lib = abi.hip_lib()
+        D = self._device_constants()
+        dev = torch.device("cuda")
+        stream = torch.cuda.current_stream().cuda_stream
+        keep = []
+
+        def dt(a, dtype=torch.float64):
+            x = torch.as_tensor(np.ascontiguousarray(a), dtype=dtype).to(dev, non_blocking=False)
+            keep.append(x)
+            return x
+
+        ns = self.n_save
+        ncos = len(plans)
         nkt = self.k_transfer.size
         n_ell = self.ell_idx.size
         n_tau = ns - 1
         nl = self.lensing_ells.size
         nq = self.k_pk.size
         lensing = bool(self.specs["lensing"])
-        nkmax = max(a.size for a in allk)
+        nkmax = max(int(k.numel()) for k in kdev)
         sc = self.scales.ctypes.data_as(C.c_void_p)
+        cos_dev = self._struct_array(torch, dev, self._cosmo_array(plans, pts, kdev, dt, tables=False), dt)
         cl_raw = torch.empty((ncos, 3, n_ell), dtype=torch.float64, device=dev)
         cl_full = torch.empty((ncos, 3, nl), dtype=torch.float64, device=dev)
         pk = torch.empty((ncos, nq), dtype=torch.float64, device=dev)
         clpp = lensed = None
         bgv_all = dt(np.stack([pl.bgv for pl in plans]))
+        n_lens = plans[0].lens_lna.size if lensing else 0
         if lensing:
-            lens_h = np.zeros((ncos, 4, plans[0].lens_lna.size))
+            lens_h = np.zeros((ncos, 4, n_lens))
             for i, pl in enumerate(plans):
                 lens_h[i, 0], lens_h[i, 1], lens_h[i, 2] = pl.lens_lna, pl.lens_aH, pl.lens_tau
                 lens_h[i, 3, 0] = pl.chi_rec
@@ -327,14 +438,14 @@ class Model:
             lensed = torch.empty((ncos, 4, nl), dtype=torch.float64, device=dev)
         csz = C.sizeof(abi.AbcmbCosmo)
         chunk = max(1, min(ncos, self.chunk))
-        bc = min(chunk, ncos)
-        ws_src = torch.empty(lib.abcmb_sources_workspace_size(bc, nkmax, ns) // 8 + 1, dtype=torch.float64, device=dev)
-        ws_los = torch.empty(lib.abcmb_los_workspace_size(bc, n_ell, nkt) // 8 + 1, dtype=torch.float64, device=dev)
-        src = torch.empty((bc, 4, n_tau, nkt), dtype=torch.float64, device=dev)
+        ws_src = torch.empty(lib.abcmb_sources_workspace_size(chunk, nkmax, ns) // 8 + 1, dtype=torch.float64,
+                             device=dev)
+        ws_los = torch.empty(lib.abcmb_los_workspace_size(chunk, n_ell, nkt) // 8 + 1, dtype=torch.float64, device=dev)
+        src = torch.empty((chunk, 4, n_tau, nkt), dtype=torch.float64, device=dev)
         ws_lens = None
         if lensing:
-            ws_lens = torch.empty(lib.abcmb_lens_workspace_size(bc, self.mu.size, nl) // 8 + 1, dtype=torch.float64,
-                                  device=dev)
+            ws_lens = torch.empty(lib.abcmb_lens_workspace_size(chunk, self.mu.size, nl) // 8 + 1,
+                                  dtype=torch.float64, device=dev)
         for c0 in range(0, ncos, chunk):
             nb = min(chunk, ncos - c0)
             cp = cos_dev.data_ptr() + c0 * csz
@@ -349,45 +460,62 @@ class Model:
             abi.check(lib.abcmb_apply_operator(D["ell_op"].data_ptr(), nl, n_ell, cl_raw[c0].data_ptr(), 3 * nb,
                                                cl_full[c0].data_ptr(), stream), "abcmb_apply_operator")
             if lensing:
-                abi.check(lib.abcmb_lensing_clpp(cp, nb, lens_all[c0].data_ptr(), plans[0].lens_lna.size,
+                abi.check(lib.abcmb_lensing_clpp(cp, nb, lens_all[c0].data_ptr(), n_lens,
                                                  D["lens_ells"].data_ptr(), nl, clpp[c0].data_ptr(), stream),
                           "abcmb_lensing_clpp")
                 abi.check(lib.abcmb_lens(nb, cl_full[c0].data_ptr(), clpp[c0].data_ptr(), nl, D["mu"].data_ptr(),
                                          D["w"].data_ptr(), self.mu.size, lensed[c0].data_ptr(), ws_lens.data_ptr(),
                                          ws_lens.numel() * 8, stream), "abcmb_lens")
             abi.check(lib.abcmb_pk_lin(cp, nb, D["k_pk"].data_ptr(), nq, pk[c0].data_ptr(), stream), "abcmb_pk_lin")
-        if sync:
-            torch.cuda.synchronize()
-        st_h = status.cpu().numpy()
-        steps_h = steps.cpu().numpy()
-        if self.timing is not None:
-            self.timing.append(dict(ms=ev0.elapsed_time(ev1), steps=int(steps_h.sum()),
-                                    iters=int(iters.sum().item()), items=n_items))
-        bad = np.nonzero(st_h != abi_k_ok())[0]
-        if bad.size:
-            where = [(flat[b][1], float(allk[flat[b][1]][flat[b][2]])) for b in bad[:8]]
-            raise RuntimeError(f"perturbation ODE failed for {bad.size} (cosmology, k) items; status codes "
-                               f"{np.unique(st_h[bad]).tolist()} (1 = max_steps_PE reached, 2 = non-finite, 3 = step-size underflow); "
-                               f"first (cosmology, k): {where}")
+        # the host-side structs and staging buffers must outlive the launches
+        torch.cuda.current_stream().synchronize()
+        return dict(cl_raw=cl_raw, cl_full=cl_full, clpp=clpp, lensed=lensed, pk=pk)
+
+    def observed_cl(self, ds):
+        """[B][3][n_l] TT, TE, EE on l = 2..l_max (lensed when lensing is on) from a downstream result."""
         n_l = self.ells.size
-        # per-cosmology step counts in PT k order
-        steps_ck = [np.zeros(a.size, dtype=np.int32) for a in allk]
-        for m, (_, ci, ik) in enumerate(flat):
-            steps_ck[ci][ik] = steps_h[m]
+        if self.specs["lensing"]:
+            return ds["lensed"][:, :3, :n_l]
+        return ds["cl_full"][:, :, :n_l]
+
+    def _outputs(self, plans, run, ds):
+        n_l = self.ells.size
+        lensing = bool(self.specs["lensing"])
+        outs = []
         for i, pl in enumerate(plans):
             if lensing:
-                TT, TE, EE, BB = lensed[i, 0, :n_l], lensed[i, 1, :n_l], lensed[i, 2, :n_l], lensed[i, 3, :n_l]
+                L = ds["lensed"]
+                TT, TE, EE, BB = L[i, 0, :n_l], L[i, 1, :n_l], L[i, 2, :n_l], L[i, 3, :n_l]
             else:
-                TT, TE, EE, BB = cl_full[i, 0, :n_l], cl_full[i, 1, :n_l], cl_full[i, 2, :n_l], None
-            PT = PerturbationTable(allk[i], pl.lna_save, pts[i], self.has_ncdm)
-            out = Output(TT, TE, EE, pk[i], self.ells, self.k_pk, pl.bg, PT, pl.params, ClBB=BB,
-                         Clpp=clpp[i] if lensing else None, steps=steps_ck[i])
-            out.cl_raw = cl_raw[i]
-            out.cl_unlensed = cl_full[i]
+                F = ds["cl_full"]
+                TT, TE, EE, BB = F[i, 0, :n_l], F[i, 1, :n_l], F[i, 2, :n_l], None
+            PT = PerturbationTable(run["allk"][i], pl.lna_save, run["pts"][i], self.has_ncdm)
+            PT.k_dev = run["kdev"][i]
+            out = Output(TT, TE, EE, ds["pk"][i], self.ells, self.k_pk, pl.bg, PT, pl.params, ClBB=BB,
+                         Clpp=ds["clpp"][i] if lensing else None, steps=run["steps"][i])
+            out.cl_raw = ds["cl_raw"][i]
+            out.cl_unlensed = ds["cl_full"][i]
             outs.append(out)
-        if want_state:
-            return outs, state, flat
         return outs
+
+    def execute(self, plans, want_state: bool = False):
+        """Outputs of prepared cosmologies: one ODE launch, then the batched downstream kernels."""
+        run = self._run_ode(plans, want_state=want_state)
+        ds = self._downstream(plans, run["pts"], run["kdev"])
+        outs = self._outputs(plans, run, ds)
+        if want_state:
+            return outs, run["state"], run["flat"]
+        return outs
+
+    def execute_tangent(self, plans, seeds):
+        """Outputs of ``plans`` plus the derivatives of their perturbation tables.
+
+        ``seeds``: one ``TangentSeeds`` per plan.  Returns (outputs, dpts) with
+        dpts[i] of shape [N_TANGENT_DIRS][21][n_save][nk_i] on the GPU.
+        """
+        run = self._run_ode(plans, tangents=seeds)
+        ds = self._downstream(plans, run["pts"], run["kdev"])
+        return self._outputs(plans, run, ds), run["dpts"]
 
     def __call__(self, params: dict = {}) -> Output:
         return self.run_batch([params])[0]

@@ -83,6 +83,24 @@ typedef struct AbcmbPertItem {
   int32_t cosmo, ik, pad0, pad1;
 } AbcmbPertItem;
 
+/*
+ * Forward-mode tangents of one cosmology's inputs along ABCMB_TANGENT_DIRS
+ * parameter directions (directions a caller does not need carry zeros).
+ * par[d][f] is the derivative of the f-th double field of AbcmbCosmo (in
+ * declaration order, h .. z_reion) along direction d; the tables are the
+ * derivatives of AbcmbCosmo's tables on their own (fixed) abscissae.  The
+ * tangent of the perturbation table comes back in pt.
+ */
+#define ABCMB_TANGENT_DIRS 5
+#define ABCMB_COSMO_DOUBLES 31
+typedef struct AbcmbTangent {
+  const double* par;      /* [ABCMB_TANGENT_DIRS][ABCMB_COSMO_DOUBLES] */
+  const double* tau_tab;  /* [ABCMB_TANGENT_DIRS][ABCMB_TAU_TAB_N] */
+  const double* xe_tab;   /* [ABCMB_TANGENT_DIRS][xe_n] */
+  const double* tm_tab;   /* [ABCMB_TANGENT_DIRS][tm_n] */
+  double* pt;             /* out: [ABCMB_TANGENT_DIRS][ABCMB_N_PT_FIELDS][n_save][nk] */
+} AbcmbTangent;
+
 /* Integrator settings shared by a launch (abcmb/model_specs.py:57-68,
  * abcmb/perturbations.py:285-312). */
 typedef struct AbcmbOdeSettings {
@@ -150,6 +168,25 @@ int abcmb_pert_fwd(const AbcmbCosmo* cosmos_dev, int n_cosmo,
                    const AbcmbOdeSettings* settings,
                    int32_t* status_dev, int32_t* steps_dev, int32_t* iters_dev,
                    double* state_dev, void* workspace, size_t ws_bytes, void* stream);
+
+/*
+ * Tangent-linear perturbation ODE: the same integration as abcmb_pert_fwd
+ * (same step sequence, chord matrices and iteration counts, same values in
+ * cosmos[].pt) carrying, with every state row, its derivatives along
+ * ABCMB_TANGENT_DIRS parameter directions.  The step sizes are held fixed
+ * (the derivative of the discrete scheme at the chosen steps), the chord
+ * iterations act on value and tangents together, the initial conditions,
+ * background scalars and output-table fields are differentiated exactly.
+ * Replaces jax.jacfwd through PerturbationEvolver.full_evolution
+ * (abcmb/perturbations.py:75-117, :250-404) for the parameters that reach
+ * the perturbation equations.  tangents_dev: n_cosmo device-resident
+ * AbcmbTangent (inputs and the tangent-table output pointer).
+ */
+int abcmb_pert_tangent(const AbcmbCosmo* cosmos_dev, const AbcmbTangent* tangents_dev, int n_cosmo,
+                       const AbcmbPertItem* items_dev, int n_items,
+                       const AbcmbOdeSettings* settings,
+                       int32_t* status_dev, int32_t* steps_dev, int32_t* iters_dev,
+                       void* stream);
 
 /*
  * The entry points below work on a batch of n_cosmo cosmologies described by

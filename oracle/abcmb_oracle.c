@@ -407,8 +407,34 @@ static double rms_scaled(const double* e, const double* y0, const double* y1, in
 /* out_states: [n_save][n] raw saved states; returns status */
 static double* g_trace = NULL;  /* optional per-step trace (single-threaded debug) */
 static int g_trace_cap = 0;
+
+/* Step script: with mode SCRIPT_RECORD every attempted step writes
+ * {hh, accepted, chord iterations of stages 1..6, t + hh} (9 doubles); with
+ * SCRIPT_REPLAY the integration takes exactly those steps and iteration
+ * counts, skipping error control and convergence tests.  Differencing
+ * replays at nearby parameters gives the derivative of the discrete map at
+ * a frozen step sequence, which is what a tangent-linear integrator computes. */
+enum { SCRIPT_NONE = 0, SCRIPT_RECORD = 1, SCRIPT_REPLAY = 2, SCRIPT_REPLAY_CONVERGED = 3 };
+#define SCRIPT_W 9
+typedef struct {
+  int mode, cap, len;
+  double* rec;        /* [cap][SCRIPT_W] */
+  const Ctx* jac;     /* replay: build the chord matrix from this cosmology (NULL: the integrated one) */
+} Script;
+
+static int ode_one_k_s(const Ctx* x, double lna0, double rtol, double atol, const AbcmbOdeSettings* st,
+                       int n_save, const double* lna_save, double* out_states, int* nsteps, Script* sc);
 static int ode_one_k(const Ctx* x, double lna0, double rtol, double atol, const AbcmbOdeSettings* st,
                      int n_save, const double* lna_save, double* out_states, int* nsteps) {
+  return ode_one_k_s(x, lna0, rtol, atol, st, n_save, lna_save, out_states, nsteps, NULL);
+}
+
+static int ode_one_k_s(const Ctx* x, double lna0, double rtol, double atol, const AbcmbOdeSettings* st,
+                       int n_save, const double* lna_save, double* out_states, int* nsteps, Script* sc) {
+  const int converged = sc && sc->mode == SCRIPT_REPLAY_CONVERGED;
+  const int replay = sc && (sc->mode == SCRIPT_REPLAY || converged);
+  const Ctx* xj = (replay && sc->jac) ? sc->jac : x;
+  const int record = sc && sc->mode == SCRIPT_RECORD;
   int n = x->n;
   double* mem = (double*)malloc(sizeof(double) * (size_t)(n * n + 20 * n));
   int* piv = (int*)malloc(sizeof(int) * n);
@@ -436,16 +462,24 @@ static int ode_one_k(const Ctx* x, double lna0, double rtol, double atol, const 
   }
   while (t < 0.0) {
     if (steps >= st->max_steps) { status = ABCMB_K_MAX_STEPS; break; }
+    const double* srec = NULL;
+    if (replay) {
+      if (steps >= sc->len) { status = ABCMB_K_MAX_STEPS; break; }
+      srec = sc->rec + (size_t)steps * SCRIPT_W;
+      if (srec[1] == 0.0) { ++steps; continue; }  /* a rejected step leaves the state alone */
+    }
     double tn = t + h;
     if (tn > 0.0) tn = 0.0;
     double hh = tn - t;
+    if (replay) { hh = srec[0]; tn = srec[8]; }
     if (!(hh > 0.0)) { status = ABCMB_K_STEP_UNDERFLOW; break; }
     ++steps;
+    int its[7] = {0, 0, 0, 0, 0, 0, 0};
     /* M = I - hh*gamma*J(t) with J columns from the linear RHS */
     for (int j = 0; j < n; ++j) {
       memset(e, 0, sizeof(double) * n);
       e[j] = 1.0;
-      rhs(x, t + st->jac_offset * hh, e, tmp);
+      rhs(xj, t + st->jac_offset * hh, e, tmp);
       for (int i = 0; i < n; ++i) M[i * n + j] = (i == j ? 1.0 : 0.0) - hh * KV_G * tmp[i];
     }
     int ok = lu_factor(M, n, piv) == 0;
@@ -460,11 +494,22 @@ static int ode_one_k(const Ctx* x, double lna0, double rtol, double atol, const 
       }
       int conv = 0;
       double prev = 0.0;
-      for (int it = 0; it < st->max_newton; ++it) {
+      const int nit = converged ? 200 : replay ? (int)srec[1 + s] : st->max_newton;
+      for (int it = 0; it < nit; ++it) {
+        ++its[s];
         rhs(x, ts, z, tmp);
         for (int i = 0; i < n; ++i) r[i] = -(z[i] - base[i] - hh * KV_G * tmp[i]);
         lu_solve(M, n, piv, r);
         for (int i = 0; i < n; ++i) z[i] += r[i];
+        if (converged) {  /* to rounding: the stage equations solved exactly */
+          double nrm = rms_scaled(r, z, NULL, n, rtol, atol);
+          conv = 1;
+          /* past the recorded iterations: stop when done or at the rounding floor */
+          if (it + 1 >= (int)srec[1 + s] && (nrm < 1e-10 || nrm >= prev)) break;
+          prev = nrm;
+          continue;
+        }
+        if (replay) { conv = 1; continue; }
         double nrm = rms_scaled(r, z, NULL, n, rtol, atol);
         if (!isfinite(nrm)) break;
         if (nrm < NEWTON_SMALL) { conv = 1; break; }
@@ -490,7 +535,14 @@ static int ode_one_k(const Ctx* x, double lna0, double rtol, double atol, const 
       enorm = rms_scaled(err, y, ynew, n, rtol, atol);
       if (!isfinite(enorm)) enorm = INFINITY;
     }
-    int accept = enorm < 1.0;
+    int accept = replay ? 1 : enorm < 1.0;
+    if (record && steps - 1 < sc->cap) {
+      double* w = sc->rec + (size_t)(steps - 1) * SCRIPT_W;
+      w[0] = hh; w[1] = accept;
+      for (int q = 1; q < 7; ++q) w[1 + q] = its[q];
+      w[8] = tn;
+      sc->len = steps;
+    }
     if (g_trace && steps - 1 < g_trace_cap) {
       double* tr = g_trace + (size_t)(steps - 1) * 4;
       tr[0] = t; tr[1] = hh; tr[2] = enorm; tr[3] = ok;
@@ -613,6 +665,47 @@ int oracle_pert(const AbcmbCosmo* cosmos, const AbcmbPertItem* items, int n_item
       for (int f = 0; f < ABCMB_N_PT_FIELDS; ++f) c->pt[((size_t)f * ns + s) * c->nk + I->ik] = o[f];
     }
     if (!states) free(S);
+  }
+  return 0;
+}
+
+/* oracle_pert with a step script per item: mode 1 records into
+ * scripts[n_items][cap][9] (and lens[n_items]), mode 2 replays them with the
+ * recorded chord-iteration counts, mode 3 replays the steps with every stage
+ * solved to rounding.  jac_cosmos (may be NULL): replays build their chord
+ * matrices from these cosmologies instead of the integrated ones. */
+int oracle_pert_scripted(const AbcmbCosmo* cosmos, const AbcmbCosmo* jac_cosmos, const AbcmbPertItem* items,
+                         int n_items, const AbcmbOdeSettings* st, int mode, double* scripts, int32_t* lens, int cap,
+                         int32_t* status, int32_t* steps) {
+#pragma omp parallel for schedule(dynamic, 1)
+  for (int it = 0; it < n_items; ++it) {
+    const AbcmbPertItem* I = &items[it];
+    const AbcmbCosmo* c = &cosmos[I->cosmo];
+    Ctx x;
+    ctx_init(&x, c, st, I->k);
+    int ns = c->n_save;
+    double* S = (double*)malloc(sizeof(double) * (size_t)ns * st->n);
+    Ctx xj;
+    if (jac_cosmos) ctx_init(&xj, &jac_cosmos[I->cosmo], st, I->k);
+    Script sc;
+    sc.jac = jac_cosmos ? &xj : NULL;
+    sc.mode = mode;
+    sc.cap = cap;
+    sc.len = mode >= SCRIPT_REPLAY ? lens[it] : 0;
+    sc.rec = scripts + (size_t)it * cap * SCRIPT_W;
+    int nst = 0;
+    status[it] = ode_one_k_s(&x, I->lna_start, I->rtol, I->atol, st, ns, c->lna_save, S, &nst, &sc);
+    if (mode == SCRIPT_RECORD) {
+      lens[it] = sc.len;
+      if (nst > cap) status[it] = ABCMB_K_MAX_STEPS;
+    }
+    steps[it] = nst;
+    double o[ABCMB_N_PT_FIELDS];
+    for (int s = 0; s < ns; ++s) {
+      output_fields(&x, c->lna_save[s], S + (size_t)s * st->n, o);
+      for (int f = 0; f < ABCMB_N_PT_FIELDS; ++f) c->pt[((size_t)f * ns + s) * c->nk + I->ik] = o[f];
+    }
+    free(S);
   }
   return 0;
 }

@@ -33,6 +33,8 @@ def lib():
         from abcmb_amd import abi
         L = C.CDLL(LIB_PATH)
         L.oracle_pert.argtypes = [_P, _P, _I, C.POINTER(abi.AbcmbOdeSettings), _P, _P, _P]
+        L.oracle_pert_scripted.argtypes = [_P, _P, _P, _I, C.POINTER(abi.AbcmbOdeSettings), _I, _P, _P, _I, _P,
+                                           _P]
         L.oracle_rhs.argtypes = [C.POINTER(abi.AbcmbCosmo), C.POINTER(abi.AbcmbOdeSettings), _D, _D, _P, _P]
         L.oracle_ic.argtypes = [C.POINTER(abi.AbcmbCosmo), C.POINTER(abi.AbcmbOdeSettings), _D, _D, _P]
         L.oracle_bg.argtypes = [C.POINTER(abi.AbcmbCosmo), _D, _P]
@@ -92,6 +94,29 @@ def pert(cosmos_arr, items_arr, settings, want_states=False):
     lib().oracle_pert(C.cast(cosmos_arr, C.c_void_p), C.cast(items_arr, C.c_void_p), n,
                       C.byref(settings), _p(status), _p(steps), _p(states))
     return status, steps, states
+
+
+SCRIPT_W = 9   # doubles per recorded step: hh, accepted, chord iterations of stages 1..6, t + hh
+
+
+def pert_scripted(cosmos_arr, items_arr, settings, mode, scripts, lens, jac_cosmos=None):
+    """oracle_pert that records (mode 1) or replays each item's step script.
+
+    Mode 2 replays the recorded chord-iteration counts, mode 3 solves every
+    replayed stage to rounding.  scripts: float64 [n_items][cap][SCRIPT_W],
+    lens: int32 [n_items] (written when recording); jac_cosmos: cosmologies
+    whose chord matrices the replay uses (default: the integrated ones).
+    """
+    n = len(items_arr)
+    status = np.zeros(n, np.int32)
+    steps = np.zeros(n, np.int32)
+    assert scripts.dtype == np.float64 and scripts.flags.c_contiguous and scripts.shape[0] == n
+    assert lens.dtype == np.int32 and lens.size == n
+    jc = C.cast(jac_cosmos, C.c_void_p) if jac_cosmos is not None else None
+    lib().oracle_pert_scripted(C.cast(cosmos_arr, C.c_void_p), jc, C.cast(items_arr, C.c_void_p), n,
+                               C.byref(settings), mode, _p(scripts), _p(lens), scripts.shape[1],
+                               _p(status), _p(steps))
+    return status, steps
 
 
 def spline(x, y, xq):

@@ -50,3 +50,58 @@ def run(model, plan, want_states=False, lensing=None, k_subset=None):
         out.update(clpp=clpp, lensed=lensed)
     out["pk"] = O.pk(pt, kk, ns, plan.lna_save, model.k_pk, p["A_s"], p["n_s"], model.specs["k_pivot"])
     return out
+
+
+def _pert_inputs(model, plan, ks, pt):
+    kk = np.ascontiguousarray(plan.k if plan.k is not None else model.k_pt)
+    c = abi.AbcmbCosmo.from_buffer_copy(plan.bg.cosmo)
+    lna = np.ascontiguousarray(plan.lna_save)
+    c.lna_save, c.k, c.pt = lna.ctypes.data, kk.ctypes.data, pt.ctypes.data
+    c.n_save, c.nk = model.n_save, kk.size
+    cos = (abi.AbcmbCosmo * 1)(c)
+    items = (abi.AbcmbPertItem * ks.size)()
+    for m, ik in enumerate(ks):
+        items[m] = abi.AbcmbPertItem(k=kk[ik], lna_start=plan.lna_start[ik], rtol=plan.rtol[ik],
+                                     atol=plan.atol[ik], cosmo=0, ik=int(ik))
+    return cos, items, (lna, kk)
+
+
+def frozen_step_tangents(model, base, plus, minus, steps, k_subset, mode="chord"):
+    """Derivatives of the perturbation table at the base plan's step sequence.
+
+    Records every k-mode's steps and chord-iteration counts at ``base``,
+    replays them at ``plus[d]`` / ``minus[d]`` (plans at theta +- steps[d]
+    with frozen grids) and takes central differences.
+    mode "chord": the recorded iteration counts with the base's chord
+    matrices (the derivative a tangent-linear integrator computes);
+    "own_chord": the recorded counts with each replay's own chord matrices;
+    "converged": every stage solved to rounding (the derivative of the
+    implicit Runge-Kutta map at the frozen steps).
+    Returns (pt, dpt): [21][n_save][nk] and [len(plus)][21][n_save][nk],
+    filled on k_subset.
+    """
+    kk = base.k if base.k is not None else model.k_pt
+    ks = np.asarray(k_subset)
+    nk, ns = kk.size, model.n_save
+    cap = int(model.settings.max_steps)
+    scripts = np.zeros((ks.size, cap, O.SCRIPT_W))
+    lens = np.zeros(ks.size, np.int32)
+    pt = np.zeros((abi.N_PT_FIELDS, ns, nk))
+    cos, items, keep = _pert_inputs(model, base, ks, pt)
+    st, _ = O.pert_scripted(cos, items, model.settings, 1, scripts, lens)
+    rmode = 3 if mode == "converged" else 2
+    jac = cos if mode == "chord" else None
+    if np.any(st != 0):
+        raise RuntimeError(f"oracle record failed: status {np.unique(st)}")
+    dpt = np.zeros((len(plus), abi.N_PT_FIELDS, ns, nk))
+    for d, (pp, pm, h) in enumerate(zip(plus, minus, steps)):
+        tabs = []
+        for pl in (pp, pm):
+            t = np.zeros((abi.N_PT_FIELDS, ns, nk))
+            c2, it2, keep2 = _pert_inputs(model, pl, ks, t)
+            st2, _ = O.pert_scripted(c2, it2, model.settings, rmode, scripts, lens, jac_cosmos=jac)
+            if np.any(st2 != 0):
+                raise RuntimeError(f"oracle replay failed: status {np.unique(st2)}")
+            tabs.append(t)
+        dpt[d] = (tabs[0] - tabs[1]) / (2.0 * h)
+    return pt, dpt
