@@ -130,9 +130,15 @@ def with_primordial(plan, A_s: float, n_s: float):
     return dataclasses.replace(plan, params=dict(plan.params, A_s=A_s, n_s=n_s), bg=bg)
 
 
-def tangent_gradients(model, like, thetas, k_grids=None, mock_index=None):
-    """d L / d theta for each row of ``thetas`` from one tangent-linear ODE launch."""
+def tangent_gradients(model, like, thetas, k_grids=None, mock_index=None, phases=None):
+    """d L / d theta for each row of ``thetas`` from one tangent-linear ODE launch.
+
+    ``phases`` (optional dict) receives the wall time of the host preparation,
+    the tangent-linear ODE plus base downstream, and the directional downstream.
+    """
+    import time
     from .model import tangent_seeds
+    t_start = time.perf_counter()
     thetas = np.atleast_2d(thetas)
     nc = thetas.shape[0]
     mock_index = list(range(nc)) if mock_index is None else list(mock_index)
@@ -152,8 +158,10 @@ def tangent_gradients(model, like, thetas, k_grids=None, mock_index=None):
     nd = len(ODE_DIRS)
     pm = [pm[c * 2 * nd:(c + 1) * 2 * nd] for c in range(nc)]
     seeds = [tangent_seeds(pm[c][0::2], pm[c][1::2], [steps[c, p] for p in ODE_DIRS]) for c in range(nc)]
+    t_host = time.perf_counter()
     outs, dpts = model.execute_tangent(base_plans, seeds)
     Lb = like.loglike(outs, mock_index).cpu().numpy()
+    t_ode = time.perf_counter()
 
     # downstream along each direction: 14 evaluations per cosmology, in groups
     per = 2 * 7
@@ -184,4 +192,7 @@ def tangent_gradients(model, like, thetas, k_grids=None, mock_index=None):
         L = like.loglike_cl(model.observed_cl(ds), slots).cpu().numpy().reshape(len(cs), 7, 2)
         Lp[c0:c0 + len(cs)] = L
     grad = (Lp[:, :, 0] - Lp[:, :, 1]) / (2 * steps)
+    if phases is not None:
+        phases.update(host_s=t_host - t_start, tangent_ode_s=t_ode - t_host,
+                      directional_downstream_s=time.perf_counter() - t_ode)
     return Lb, grad

@@ -141,8 +141,9 @@ def main():
                                  n_mocks=args.batch, seed=1)
     print(f"# setup {time.perf_counter() - t_setup:.1f} s", file=sys.stderr, flush=True)
 
+    phases = {}
     if args.gradient == "tangent":
-        batched = lambda: LK.tangent_gradients(model, like, thetas, k_grids=k_grids)
+        batched = lambda: LK.tangent_gradients(model, like, thetas, k_grids=k_grids, phases=phases)
     else:
         batched = lambda: LK.fd_gradients(model, like, thetas, k_grids=k_grids, batched=True)
     for w in range(args.warmup):
@@ -201,7 +202,8 @@ def main():
                                 "central finite differences, 15 evaluations, grids frozen, all in one ODE launch"),
                    "plain_path": "one cosmology per launch, FP64, central finite differences (15 evaluations)",
                    "plain_path_grad_evals_per_s": plain, "speedup_vs_plain": value / plain,
-                   "grad_diff_vs_plain_rel_to_norm": agree, "grad_diff_vs_plain_max_rel": agree_rel},
+                   "grad_diff_vs_plain_rel_to_norm": agree, "grad_diff_vs_plain_max_rel": agree_rel,
+                   "phases_s": phases},
         "roofline": roof,
         "cpu_baseline": cpu,
     }

@@ -422,15 +422,41 @@ typedef struct {
   const Ctx* jac;     /* replay: build the chord matrix from this cosmology (NULL: the integrated one) */
 } Script;
 
-static int ode_one_k_s(const Ctx* x, double lna0, double rtol, double atol, const AbcmbOdeSettings* st,
-                       int n_save, const double* lna_save, double* out_states, int* nsteps, Script* sc);
-static int ode_one_k(const Ctx* x, double lna0, double rtol, double atol, const AbcmbOdeSettings* st,
-                     int n_save, const double* lna_save, double* out_states, int* nsteps) {
-  return ode_one_k_s(x, lna0, rtol, atol, st, n_save, lna_save, out_states, nsteps, NULL);
+/* Tangent-linear companion of the integration (forward mode along nd
+ * parameter directions).  The parameter derivative of the right-hand side at
+ * a fixed state, dA/dtheta_d y, is the central difference of rhs() between
+ * the cosmologies xp[d] and xm[d] at theta +- h[d] (rhs is linear in y and
+ * smooth in theta, so this is exact to O(h^2)); everything else is exact
+ * linear algebra on the recorded step sequence: each stage tangent solves
+ * (I - g A(t_s)) zdot = basedot + g dA/dtheta z to rounding (iterated with
+ * the step's chord matrix), with z the value's stage.  The saved tangent
+ * states go to ys_dot[nd][n_save][n]. */
+typedef struct {
+  int nd;
+  const Ctx* xp;
+  const Ctx* xm;
+  const double* h;
+  double* ys_dot;
+} Tangent;
+
+/* out = dA/dtheta_d y at lna */
+static void rhs_dtheta(const Tangent* tg, int d, double lna, const double* y, double* out, double* work, int n) {
+  rhs(&tg->xp[d], lna, y, out);
+  rhs(&tg->xm[d], lna, y, work);
+  for (int i = 0; i < n; ++i) out[i] = (out[i] - work[i]) / (2.0 * tg->h[d]);
 }
 
 static int ode_one_k_s(const Ctx* x, double lna0, double rtol, double atol, const AbcmbOdeSettings* st,
-                       int n_save, const double* lna_save, double* out_states, int* nsteps, Script* sc) {
+                       int n_save, const double* lna_save, double* out_states, int* nsteps, Script* sc,
+                       const Tangent* tg);
+static int ode_one_k(const Ctx* x, double lna0, double rtol, double atol, const AbcmbOdeSettings* st,
+                     int n_save, const double* lna_save, double* out_states, int* nsteps) {
+  return ode_one_k_s(x, lna0, rtol, atol, st, n_save, lna_save, out_states, nsteps, NULL, NULL);
+}
+
+static int ode_one_k_s(const Ctx* x, double lna0, double rtol, double atol, const AbcmbOdeSettings* st,
+                       int n_save, const double* lna_save, double* out_states, int* nsteps, Script* sc,
+                       const Tangent* tg) {
   const int converged = sc && sc->mode == SCRIPT_REPLAY_CONVERGED;
   const int replay = sc && (sc->mode == SCRIPT_REPLAY || converged);
   const Ctx* xj = (replay && sc->jac) ? sc->jac : x;
@@ -455,9 +481,31 @@ static int ode_one_k_s(const Ctx* x, double lna0, double rtol, double atol, cons
   double b1 = st->pcoeff + st->icoeff + st->dcoeff, b2 = -(st->pcoeff + 2.0 * st->dcoeff);
   double prev_inv = 1.0;
   rhs(x, t, y, f0);
+  /* tangent state: per direction y', f0', F'[7], stage base', z', f_new' (12 n) */
+  const int nd = tg ? tg->nd : 0;
+  double* tmem = nd ? (double*)malloc(sizeof(double) * ((size_t)nd * 12 * n + 2 * (size_t)n)) : NULL;
+  double* tw1 = tmem ? tmem + (size_t)nd * 12 * n : NULL;
+  double* tw2 = tw1 ? tw1 + n : NULL;
+#define TYD(d) (tmem + (size_t)(d) * 12 * n)
+#define TF0(d) (TYD(d) + n)
+#define TFS(d, s_) (TYD(d) + (size_t)(2 + (s_)) * n)
+#define TBASE(d) (TYD(d) + 9 * n)
+#define TZ(d) (TYD(d) + 10 * n)
+#define TFN(d) (TYD(d) + 11 * n)
+  for (int d = 0; d < nd; ++d) {
+    /* initial conditions and their derivative at the (fixed) start time */
+    initial_conditions(&tg->xp[d], lna0, TZ(d));
+    initial_conditions(&tg->xm[d], lna0, TBASE(d));
+    for (int i = 0; i < n; ++i) TYD(d)[i] = (TZ(d)[i] - TBASE(d)[i]) / (2.0 * tg->h[d]);
+    rhs(x, t, TYD(d), TF0(d));
+    rhs_dtheta(tg, d, t, y, tw1, tw2, n);
+    for (int i = 0; i < n; ++i) TF0(d)[i] += tw1[i];
+  }
   int isave = 0, steps = 0, status = ABCMB_K_OK;
   while (isave < n_save && lna_save[isave] <= t) {  /* save points at t0 */
     memcpy(out_states + (size_t)isave * n, y, sizeof(double) * n);
+    for (int d = 0; d < nd; ++d)
+      memcpy(tg->ys_dot + ((size_t)d * n_save + isave) * n, TYD(d), sizeof(double) * n);
     ++isave;
   }
   while (t < 0.0) {
@@ -523,6 +571,35 @@ static int ode_one_k_s(const Ctx* x, double lna0, double rtol, double atol, cons
       }
       if (!conv) { ok = 0; break; }
       for (int i = 0; i < n; ++i) F[s * n + i] = (z[i] - base[i]) / (hh * KV_G);
+      for (int d = 0; d < nd; ++d) {
+        double* bd = TBASE(d);
+        double* zd = TZ(d);
+        if (s == 1) memcpy(TFS(d, 0), TF0(d), sizeof(double) * n);
+        for (int i = 0; i < n; ++i) {
+          double acc = 0.0;
+          for (int j = 0; j < s; ++j) acc += KV_A[s][j] * TFS(d, j)[i];
+          bd[i] = TYD(d)[i] + hh * acc;
+          zd[i] = bd[i] + hh * KV_G * TFS(d, s - 1)[i];
+        }
+        double* adz = tw1;   /* g dA/dtheta z, fixed during the iteration */
+        rhs_dtheta(tg, d, ts, z, adz, tw2, n);
+        double prevt = INFINITY;
+        for (int it = 0; it < 400; ++it) {
+          rhs(x, ts, zd, tmp);
+          for (int i = 0; i < n; ++i) r[i] = -(zd[i] - bd[i] - hh * KV_G * (tmp[i] + adz[i]));
+          lu_solve(M, n, piv, r);
+          double nr = 0.0, nz = 0.0;
+          for (int i = 0; i < n; ++i) {
+            zd[i] += r[i];
+            double w = 1.0 / (atol + rtol * fabs(z[i]));
+            nr += (r[i] * w) * (r[i] * w);
+            nz += (zd[i] * w) * (zd[i] * w);
+          }
+          if (nr <= 1e-30 * nz || (it > 2 && nr >= prevt)) break;  /* to rounding */
+          prevt = nr;
+        }
+        for (int i = 0; i < n; ++i) TFS(d, s)[i] = (zd[i] - bd[i]) / (hh * KV_G);
+      }
     }
     double enorm = INFINITY;
     if (ok) {
@@ -554,17 +631,31 @@ static int ode_one_k_s(const Ctx* x, double lna0, double rtol, double atol, cons
     if (fac > st->factormax) fac = st->factormax;
     if (accept) {
       rhs(x, tn, ynew, fnew);
+      for (int d = 0; d < nd; ++d) {  /* z' of the last stage is the new tangent state */
+        rhs(x, tn, TZ(d), TFN(d));
+        rhs_dtheta(tg, d, tn, ynew, tw1, tw2, n);
+        for (int i = 0; i < n; ++i) TFN(d)[i] += tw1[i];
+      }
       while (isave < n_save && lna_save[isave] <= tn) {
         double th = (lna_save[isave] - t) / hh;
         double h00 = 2 * th * th * th - 3 * th * th + 1, h10 = th * th * th - 2 * th * th + th;
         double h01 = -2 * th * th * th + 3 * th * th, h11 = th * th * th - th * th;
         double* o = out_states + (size_t)isave * n;
         for (int i = 0; i < n; ++i) o[i] = h00 * y[i] + h10 * hh * f0[i] + h01 * ynew[i] + h11 * hh * fnew[i];
+        for (int d = 0; d < nd; ++d) {
+          double* od = tg->ys_dot + ((size_t)d * n_save + isave) * n;
+          for (int i = 0; i < n; ++i)
+            od[i] = h00 * TYD(d)[i] + h10 * hh * TF0(d)[i] + h01 * TZ(d)[i] + h11 * hh * TFN(d)[i];
+        }
         ++isave;
       }
       t = tn;
       memcpy(y, ynew, sizeof(double) * n);
       memcpy(f0, fnew, sizeof(double) * n);
+      for (int d = 0; d < nd; ++d) {
+        memcpy(TYD(d), TZ(d), sizeof(double) * n);
+        memcpy(TF0(d), TFN(d), sizeof(double) * n);
+      }
       prev_inv = inv;
       int bad = 0;
       for (int i = 0; i < n; ++i) if (!isfinite(y[i])) bad = 1;
@@ -575,6 +666,13 @@ static int ode_one_k_s(const Ctx* x, double lna0, double rtol, double atol, cons
   *nsteps = steps;
   free(mem);
   free(piv);
+  free(tmem);
+#undef TYD
+#undef TF0
+#undef TFS
+#undef TBASE
+#undef TZ
+#undef TFN
   return status;
 }
 
@@ -694,7 +792,7 @@ int oracle_pert_scripted(const AbcmbCosmo* cosmos, const AbcmbCosmo* jac_cosmos,
     sc.len = mode >= SCRIPT_REPLAY ? lens[it] : 0;
     sc.rec = scripts + (size_t)it * cap * SCRIPT_W;
     int nst = 0;
-    status[it] = ode_one_k_s(&x, I->lna_start, I->rtol, I->atol, st, ns, c->lna_save, S, &nst, &sc);
+    status[it] = ode_one_k_s(&x, I->lna_start, I->rtol, I->atol, st, ns, c->lna_save, S, &nst, &sc, NULL);
     if (mode == SCRIPT_RECORD) {
       lens[it] = sc.len;
       if (nst > cap) status[it] = ABCMB_K_MAX_STEPS;
@@ -704,6 +802,47 @@ int oracle_pert_scripted(const AbcmbCosmo* cosmos, const AbcmbCosmo* jac_cosmos,
     for (int s = 0; s < ns; ++s) {
       output_fields(&x, c->lna_save[s], S + (size_t)s * st->n, o);
       for (int f = 0; f < ABCMB_N_PT_FIELDS; ++f) c->pt[((size_t)f * ns + s) * c->nk + I->ik] = o[f];
+    }
+    free(S);
+  }
+  return 0;
+}
+
+/* Tangent-linear perturbation tables of one cosmology: base->pt gets the
+ * values, dpt[nd][21][n_save][nk] their derivatives along nd directions whose
+ * cosmologies at theta +- h[d] are plus[d] / minus[d] (same grids as base).
+ * The output fields are linear in the state, so their derivative is the
+ * field of the tangent state plus the central difference of the field at the
+ * fixed state. */
+int oracle_pert_tangent(const AbcmbCosmo* base, const AbcmbCosmo* plus, const AbcmbCosmo* minus,
+                        const double* h, int nd, const AbcmbPertItem* items, int n_items,
+                        const AbcmbOdeSettings* st, double* dpt, int32_t* status, int32_t* steps) {
+#pragma omp parallel for schedule(dynamic, 1)
+  for (int it = 0; it < n_items; ++it) {
+    const AbcmbPertItem* I = &items[it];
+    Ctx x, xp[8], xm[8];
+    ctx_init(&x, base, st, I->k);
+    for (int d = 0; d < nd && d < 8; ++d) { ctx_init(&xp[d], &plus[d], st, I->k); ctx_init(&xm[d], &minus[d], st, I->k); }
+    int ns = base->n_save, n = st->n;
+    double* S = (double*)malloc(sizeof(double) * (size_t)ns * n * (1 + nd));
+    Tangent tg = {nd, xp, xm, h, S + (size_t)ns * n};
+    int nst = 0;
+    status[it] = nd <= 8 ? ode_one_k_s(&x, I->lna_start, I->rtol, I->atol, st, ns, base->lna_save, S, &nst, NULL, &tg)
+                         : ABCMB_K_NONFINITE;
+    steps[it] = nst;
+    double o[ABCMB_N_PT_FIELDS], op[ABCMB_N_PT_FIELDS], om[ABCMB_N_PT_FIELDS];
+    const size_t plane = (size_t)ABCMB_N_PT_FIELDS * ns * base->nk;
+    for (int s = 0; s < ns; ++s) {
+      const double* ys = S + (size_t)s * n;
+      output_fields(&x, base->lna_save[s], ys, o);
+      for (int f = 0; f < ABCMB_N_PT_FIELDS; ++f) base->pt[((size_t)f * ns + s) * base->nk + I->ik] = o[f];
+      for (int d = 0; d < nd && d < 8; ++d) {
+        output_fields(&x, base->lna_save[s], tg.ys_dot + ((size_t)d * ns + s) * n, o);
+        output_fields(&xp[d], base->lna_save[s], ys, op);
+        output_fields(&xm[d], base->lna_save[s], ys, om);
+        for (int f = 0; f < ABCMB_N_PT_FIELDS; ++f)
+          dpt[d * plane + ((size_t)f * ns + s) * base->nk + I->ik] = o[f] + (op[f] - om[f]) / (2.0 * h[d]);
+      }
     }
     free(S);
   }

@@ -35,6 +35,7 @@ def lib():
         L.oracle_pert.argtypes = [_P, _P, _I, C.POINTER(abi.AbcmbOdeSettings), _P, _P, _P]
         L.oracle_pert_scripted.argtypes = [_P, _P, _P, _I, C.POINTER(abi.AbcmbOdeSettings), _I, _P, _P, _I, _P,
                                            _P]
+        L.oracle_pert_tangent.argtypes = [_P, _P, _P, _P, _I, _P, _I, C.POINTER(abi.AbcmbOdeSettings), _P, _P, _P]
         L.oracle_rhs.argtypes = [C.POINTER(abi.AbcmbCosmo), C.POINTER(abi.AbcmbOdeSettings), _D, _D, _P, _P]
         L.oracle_ic.argtypes = [C.POINTER(abi.AbcmbCosmo), C.POINTER(abi.AbcmbOdeSettings), _D, _D, _P]
         L.oracle_bg.argtypes = [C.POINTER(abi.AbcmbCosmo), _D, _P]
@@ -116,6 +117,19 @@ def pert_scripted(cosmos_arr, items_arr, settings, mode, scripts, lens, jac_cosm
     lib().oracle_pert_scripted(C.cast(cosmos_arr, C.c_void_p), jc, C.cast(items_arr, C.c_void_p), n,
                                C.byref(settings), mode, _p(scripts), _p(lens), scripts.shape[1],
                                _p(status), _p(steps))
+    return status, steps
+
+
+def pert_tangent(base_arr, plus_arr, minus_arr, h, items_arr, settings, dpt):
+    """Tangent-linear tables: values into base_arr[0].pt, derivatives into dpt [nd][21][n_save][nk]."""
+    n = len(items_arr)
+    h = _c(h)
+    status = np.zeros(n, np.int32)
+    steps = np.zeros(n, np.int32)
+    assert dpt.dtype == np.float64 and dpt.flags.c_contiguous and dpt.shape[0] == h.size
+    lib().oracle_pert_tangent(C.cast(base_arr, C.c_void_p), C.cast(plus_arr, C.c_void_p),
+                              C.cast(minus_arr, C.c_void_p), _p(h), h.size, C.cast(items_arr, C.c_void_p), n,
+                              C.byref(settings), _p(dpt), _p(status), _p(steps))
     return status, steps
 
 

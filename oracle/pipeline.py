@@ -36,19 +36,32 @@ def run(model, plan, want_states=False, lensing=None, k_subset=None):
     out = dict(pt=pt, status=status, steps=steps, states=states, cosmo=c, keep=(lna, kk))
     if k_subset is not None:
         return out
+    out.update(downstream(model, plan, pt, lensing=lensing))
+    return out
+
+
+def downstream(model, plan, pt, lensing=None):
+    """Sources, LOS C_l, l-spline, lensing and P(k) of one cosmology from its table ``pt``."""
+    kk = np.ascontiguousarray(plan.k if plan.k is not None else model.k_pt)
+    ns = model.n_save
+    pt = np.ascontiguousarray(pt)
     p = plan.params
+    c = abi.AbcmbCosmo.from_buffer_copy(plan.bg.cosmo)
+    lna = np.ascontiguousarray(plan.lna_save)
+    c.lna_save, c.k, c.pt = lna.ctypes.data, kk.ctypes.data, pt.ctypes.data
+    c.n_save, c.nk = ns, kk.size
     src = O.sources(pt, kk, ns, plan.bgv, plan.lna_save, model.k_transfer, model.scales)
     cl_raw = O.los_cl(src, plan.bgv[0], plan.bg.tau0, model.k_transfer, model.bes_tab, model.bes_rng, model.bes_l,
-                      model.ell_idx, p["A_s"], p["n_s"], model.specs["k_pivot"])
+                      model.ell_idx, c.A_s, c.n_s, model.specs["k_pivot"])
     cl_full = cl_raw @ model.ell_op.T
-    out.update(src=src, cl_raw=cl_raw, cl_full=cl_full)
+    out = dict(src=src, cl_raw=cl_raw, cl_full=cl_full)
     do_lens = model.specs["lensing"] if lensing is None else lensing
     if do_lens:
         clpp = O.clpp(c, pt, plan.lens_lna, plan.lens_aH, plan.lens_tau, plan.chi_rec,
                       model.lensing_ells.astype(np.float64))
         lensed = O.lens(cl_full, clpp, model.mu, model.w)
         out.update(clpp=clpp, lensed=lensed)
-    out["pk"] = O.pk(pt, kk, ns, plan.lna_save, model.k_pk, p["A_s"], p["n_s"], model.specs["k_pivot"])
+    out["pk"] = O.pk(pt, kk, ns, plan.lna_save, model.k_pk, c.A_s, c.n_s, model.specs["k_pivot"])
     return out
 
 
@@ -105,3 +118,32 @@ def frozen_step_tangents(model, base, plus, minus, steps, k_subset, mode="chord"
             tabs.append(t)
         dpt[d] = (tabs[0] - tabs[1]) / (2.0 * h)
     return pt, dpt
+
+
+def tangent_tables(model, base, plus, minus, steps, k_subset):
+    """Tangent-linear perturbation tables of ``base`` on ``k_subset`` (oracle).
+
+    Same step sequence as the plain integration; each stage tangent solved
+    to rounding; the right-hand side's parameter derivative by central
+    differences between ``plus[d]`` and ``minus[d]`` (theta +- steps[d],
+    grids frozen at ``base``).  Returns (pt [21][n_save][nk],
+    dpt [nd][21][n_save][nk], steps per k of the subset).
+    """
+    kk = base.k if base.k is not None else model.k_pt
+    ks = np.asarray(k_subset)
+    nk, ns, nd = kk.size, model.n_save, len(plus)
+    pt = np.zeros((abi.N_PT_FIELDS, ns, nk))
+    cos, items, keep = _pert_inputs(model, base, ks, pt)
+    pcs, mcs, keeps = [], [], []
+    for pl_list, out in ((plus, pcs), (minus, mcs)):
+        for pl in pl_list:
+            c2, _, k2 = _pert_inputs(model, pl, ks, pt)
+            out.append(c2[0])
+            keeps.append(k2)
+    parr = (abi.AbcmbCosmo * nd)(*pcs)
+    marr = (abi.AbcmbCosmo * nd)(*mcs)
+    dpt = np.zeros((nd, abi.N_PT_FIELDS, ns, nk))
+    st, nst = O.pert_tangent(cos, parr, marr, np.asarray(steps, dtype=np.float64), items, model.settings, dpt)
+    if np.any(st != 0):
+        raise RuntimeError(f"oracle tangent integration failed: status {np.unique(st)}")
+    return pt, dpt, nst

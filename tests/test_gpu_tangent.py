@@ -1,0 +1,139 @@
+"""GPU tangent-linear path through the C ABI against the CPU oracle.
+
+  - perturbation-table tangents of the HIP kernel against the oracle's
+    tangent-linear integration (same step sequence, stage tangents solved to
+    convergence): within 1e-4 of each field's maximum on the k-modes whose
+    step counts agree (the two step sequences can differ by a rounding-level
+    accept/reject flip, like the primal's); the kernel's values equal the
+    primal kernel's to rounding (the compiler contracts multiply-adds
+    differently when the products also feed the tangents) on those k-modes;
+  - likelihood gradients from ``tangent_gradients`` against the oracle's
+    gradient built the same way on the CPU (oracle tangent tables, oracle
+    sources/LOS/lensing differenced along each direction): 1e-4 of the
+    gradient norm;
+  - at 100x tighter ODE tolerances the tangent gradient and central finite
+    differences of full evaluations agree to 5e-4 of the gradient norm (at
+    the default tolerances their difference is the finite differences'
+    step-size noise, a few 1e-3).
+"""
+import numpy as np
+import pytest
+
+from abcmb_amd import likelihood as LK
+
+pytestmark = pytest.mark.gpu
+
+
+def _say(msg):
+    print(msg, flush=True)
+
+
+def _pm(m, th, base, k=None):
+    steps = LK.FD_REL_STEP * np.abs(th)
+    plus, minus = [], []
+    for p in LK.ODE_DIRS:
+        for sgn, lst in ((1, plus), (-1, minus)):
+            t = th.copy()
+            t[p] += sgn * steps[p]
+            lst.append(m.prepare(m.derived(LK.theta_to_params(t)), freeze=base, k=k))
+    return plus, minus, [steps[p] for p in LK.ODE_DIRS]
+
+
+def _norm_err(g, ref, theta):
+    """per-parameter |g - ref| relative to the gradient norm in units of each parameter."""
+    scale = np.linalg.norm(ref * theta, axis=1, keepdims=True) / np.abs(theta)
+    return np.abs(g - ref) / scale
+
+
+@pytest.mark.parametrize("massive", [False, True])
+def test_tangent_tables_match_oracle(massive):
+    from abcmb_amd.model import Model, MassiveNeutrino, tangent_seeds
+    from oracle import pipeline
+    kk = np.geomspace(1e-4, 0.3, 8)
+    m = Model(user_species=(MassiveNeutrino,) if massive else (), l_max=500, k_axis_perturbations=kk)
+    th = LK.FIDUCIAL.copy()
+    base = m.prepare(m.derived(LK.theta_to_params(th)))
+    plus, minus, sd = _pm(m, th, base)
+    _say(f"case: tangent tables vs oracle, massive={massive}")
+    outs, dpts = m.execute_tangent([base], [tangent_seeds(plus, minus, sd)])
+    prim = m.execute([base])[0]
+    pt_t, pt_p = outs[0].PT.table.cpu().numpy(), prim.PT.table.cpu().numpy()
+    sp = outs[0].steps == prim.steps
+    assert np.mean(sp) >= 0.75
+    scale = np.abs(pt_p).max(axis=(1, 2), keepdims=True) + 1e-300
+    assert np.max(np.abs(pt_t - pt_p)[..., sp] / scale) < 1e-10
+    pt_o, dpt_o, steps_o = pipeline.tangent_tables(m, base, plus, minus, sd, np.arange(kk.size))
+    same = outs[0].steps == steps_o
+    assert np.mean(same) >= 0.75
+    dg = dpts[0].cpu().numpy()[..., same]
+    do = dpt_o[..., same]
+    scale = np.abs(do).max(axis=(2, 3), keepdims=True) + 1e-300
+    err = np.abs(dg - do) / scale
+    _say(f"max tangent error {err.max():.2e}")
+    assert err.max() < 1e-4
+
+
+def test_tangent_gradient_matches_oracle():
+    from abcmb_amd.model import Model, MassiveNeutrino
+    from abcmb_amd.likelihood import with_primordial
+    from oracle import pipeline
+    kk = np.geomspace(1e-4, 0.6, 24)
+    m = Model(user_species=(MassiveNeutrino,), lensing=True, l_max=1000, k_axis_perturbations=kk)
+    thetas = LK.latin_hypercube(2, seed=3)
+    base0 = m.prepare(m.derived(LK.theta_to_params(thetas[0])))
+    g0 = m.execute([base0])[0]
+    like = LK.LensedClLikelihood(m, [g0.ClTT.cpu().numpy(), g0.ClTE.cpu().numpy(), g0.ClEE.cpu().numpy()],
+                                 n_mocks=2, seed=5)
+    _say("case: tangent gradients, 2 cosmologies, lensed l<=1000, 24 k")
+    Lg, gg = LK.tangent_gradients(m, like, thetas)
+    mock = like.mock.cpu().numpy()
+    ivar = like.inv_var.cpu().numpy()
+    n_l = m.ells.size
+
+    def loglike(out, slot):
+        cl = out["lensed"][:3, :n_l]
+        return -0.5 * np.sum((cl - mock[slot]) ** 2 * ivar[slot])
+
+    go = np.zeros_like(gg)
+    for c, th in enumerate(thetas):
+        base = m.prepare(m.derived(LK.theta_to_params(th)))
+        plus, minus, sd = _pm(m, th, base)
+        pt, dpt, _ = pipeline.tangent_tables(m, base, plus, minus, sd, np.arange(kk.size))
+        steps = LK.FD_REL_STEP * np.abs(th)
+        for p in range(7):
+            L = []
+            for sgn in (+1, -1):
+                if p in LK.ODE_DIRS:
+                    d = LK.ODE_DIRS.index(p)
+                    plan = (plus if sgn > 0 else minus)[d]
+                    tab = pt + sgn * steps[p] * dpt[d]
+                else:
+                    t = th.copy()
+                    t[p] += sgn * steps[p]
+                    q = LK.theta_to_params(t)
+                    plan, tab = with_primordial(base, q["A_s"], q["n_s"]), pt
+                L.append(loglike(pipeline.downstream(m, plan, tab), c))
+            go[c, p] = (L[0] - L[1]) / (2 * steps[p])
+    err = _norm_err(gg, go, thetas)
+    _say(f"gradient vs oracle, relative to the norm: {err.max():.2e}")
+    assert err.max() < 1e-4
+
+
+def test_tangent_and_fd_gradients_converge_together():
+    from abcmb_amd.model import Model, MassiveNeutrino
+    tol = 1e-2
+    m = Model(user_species=(MassiveNeutrino,), lensing=True, l_max=2500, max_steps_PE=30000,
+              rtol_small_k_PE=1e-5 * tol, rtol_large_k_PE=1e-4 * tol, atol_small_k_PE=1e-10 * tol,
+              atol_large_k_PE=1e-6 * tol)
+    thetas = LK.latin_hypercube(2, seed=0)
+    kg = LK.uneven_k_grids(2, seed=0)
+    fid = m.execute([m.prepare(m.derived(LK.theta_to_params(LK.FIDUCIAL)))])[0]
+    like = LK.LensedClLikelihood(m, [fid.ClTT.cpu().numpy(), fid.ClTE.cpu().numpy(), fid.ClEE.cpu().numpy()],
+                                 n_mocks=2, seed=1)
+    _say("case: tangent vs finite-difference gradients at 100x tighter tolerances")
+    Lf, gf = LK.fd_gradients(m, like, thetas, k_grids=kg, batched=True)
+    Lt, gt = LK.tangent_gradients(m, like, thetas, k_grids=kg)
+    assert np.allclose(Lf, Lt, rtol=1e-6, atol=0)
+    err = _norm_err(gt, gf, thetas)
+    _say(f"tangent vs FD, relative to the norm: {err.max():.2e}")
+    assert err.max() < 5e-4
