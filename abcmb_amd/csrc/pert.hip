@@ -373,11 +373,13 @@ __device__ __noinline__ void fill_scalars(const Cosmo<N>& P, const Layout& L, do
 // chord updates below this (in units of the error tolerance) are rounding noise
 constexpr double NEWTON_SMALL = 1e-9;
 // Tangents: once the value has converged, the tangents keep iterating (value
-// fixed) until each direction's weighted update is below TANGENT_RTOL of its
-// weighted size, so they solve the differentiated stage equations rather
-// than the derivative of a truncated chord iteration; at most TANGENT_EXTRA
-// iterations beyond max_newton.
-constexpr double TANGENT_RTOL = 1e-7;
+// fixed) until each direction's estimated remaining error -- its weighted
+// update times rate / (1 - rate), with the value's contraction rate, since
+// value and tangents run the same linear iteration -- is below TANGENT_RTOL
+// of its weighted size.  They then solve the differentiated stage equations
+// rather than the derivative of a truncated chord iteration.  At most
+// TANGENT_EXTRA iterations beyond max_newton.
+constexpr double TANGENT_RTOL = 1e-6;
 constexpr int TANGENT_EXTRA = 8;
 constexpr int NTIMES = 7;  // t, the distinct stage abscissae t + c_s h, and the chord-Jacobian time
 constexpr int JAC_SLOT = 6;
@@ -821,6 +823,7 @@ __global__ void __launch_bounds__(NT) pert_fwd_kernel(const AbcmbCosmo* __restri
       bool conv = false;      // value converged
       bool tconv = N == 0;    // tangents converged
       double prev = 0.0;
+      double rate = 1.0;      // the value's latest contraction-rate estimate
       for (int it = 0;; ++it) {
         if (!conv && it >= st.max_newton) break;
         ++iters;
@@ -850,14 +853,18 @@ __global__ void __launch_bounds__(NT) pert_fwd_kernel(const AbcmbCosmo* __restri
           } else if (it > 0) {
             double th = nrm / prev;
             if (th > 2.0) break;
+            rate = th;
             double fac = nrm * th / (1.0 - th);
             if (fac > 0.0 && fac < st.newton_kappa) conv = true;
           }
           prev = nrm;
         }
         if constexpr (N > 0) {
+          // remaining error ~ update * rate / (1 - rate) when the iteration contracts
+          const double amp = rate < 1.0 ? rate / (1.0 - rate) : 1.0;
+          const double lim = (TANGENT_RTOL / amp) * (TANGENT_RTOL / amp);
           tconv = true;
-          ABCMB_DLOOP(N) tconv = tconv && v[1 + 2 * i_] <= TANGENT_RTOL * TANGENT_RTOL * v[2 + 2 * i_];
+          ABCMB_DLOOP(N) tconv = tconv && v[1 + 2 * i_] <= lim * v[2 + 2 * i_];
         }
         if (conv && (tconv || it + 1 >= st.max_newton + TANGENT_EXTRA)) break;
       }

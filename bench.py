@@ -113,6 +113,19 @@ def cpu_baseline(model, thetas, k_grids, seconds_budget=20.0):
                        f"central-difference gradient")}
 
 
+def copy_calibration(nbytes: int):
+    """Five copies of ``nbytes`` between two device buffers (FETCH_SIZE / WRITE_SIZE reference)."""
+    import torch
+    n = nbytes // 8
+    src = torch.rand(n, dtype=torch.float64, device="cuda")
+    dst = torch.empty_like(src)
+    torch.cuda.synchronize()
+    for _ in range(5):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes_per_launch": n * 8, "launches": 5}))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -123,7 +136,13 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--gradient", choices=("tangent", "fd"), default="tangent",
                     help="batched gradient method (the plain path is always finite differences)")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
+                    help="only run a device-to-device copy of BYTES (5 times): the reference for "
+                         "scaling HBM byte counters")
     args = ap.parse_args()
+    if args.copy_calibration:
+        copy_calibration(args.copy_calibration)
+        return
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only; multi-GPU sharding is described in "
                                    "DESIGN.md and not launched"}))
