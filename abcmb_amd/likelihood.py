@@ -130,20 +130,11 @@ def with_primordial(plan, A_s: float, n_s: float):
     return dataclasses.replace(plan, params=dict(plan.params, A_s=A_s, n_s=n_s), bg=bg)
 
 
-def tangent_gradients(model, like, thetas, k_grids=None, mock_index=None, phases=None):
-    """d L / d theta for each row of ``thetas`` from one tangent-linear ODE launch.
-
-    ``phases`` (optional dict) receives the wall time of the host preparation,
-    the tangent-linear ODE plus base downstream, and the directional downstream.
-    """
-    import time
+def _tangent_inputs(model, thetas, k_grids, steps):
+    """Host side of the tangent gradients: base plans, plans at theta +- step along
+    the ODE directions (grids frozen at the base), and the tangent seeds."""
     from .model import tangent_seeds
-    t_start = time.perf_counter()
-    thetas = np.atleast_2d(thetas)
     nc = thetas.shape[0]
-    mock_index = list(range(nc)) if mock_index is None else list(mock_index)
-    k_grids = k_grids or [None] * nc
-    steps = FD_REL_STEP * np.abs(thetas)
     base_plans = model.prepare_many([model.derived(theta_to_params(t)) for t in thetas], k_list=k_grids)
     pm_params, freeze, ks = [], [], []
     for c in range(nc):
@@ -158,12 +149,14 @@ def tangent_gradients(model, like, thetas, k_grids=None, mock_index=None, phases
     nd = len(ODE_DIRS)
     pm = [pm[c * 2 * nd:(c + 1) * 2 * nd] for c in range(nc)]
     seeds = [tangent_seeds(pm[c][0::2], pm[c][1::2], [steps[c, p] for p in ODE_DIRS]) for c in range(nc)]
-    t_host = time.perf_counter()
-    outs, dpts = model.execute_tangent(base_plans, seeds)
-    Lb = like.loglike(outs, mock_index).cpu().numpy()
-    t_ode = time.perf_counter()
+    return base_plans, pm, seeds
 
-    # downstream along each direction: 14 evaluations per cosmology, in groups
+
+def _directional_losses(model, like, thetas, steps, base_plans, pm, outs, dpts, mock_index):
+    """L at theta +- step along all seven directions ([nc][7][2]): the downstream on
+    tables moved along their tangents (ODE directions) or with another primordial
+    spectrum (ln A_s, n_s)."""
+    nc = thetas.shape[0]
     per = 2 * 7
     group = max(1, model.chunk // per)
     Lp = np.zeros((nc, 7, 2))
@@ -189,10 +182,50 @@ def tangent_gradients(model, like, thetas, k_grids=None, mock_index=None, phases
                     kdev.append(kd)
                     slots.append(mock_index[c])
         ds = model._downstream(plans, pts, kdev)
-        L = like.loglike_cl(model.observed_cl(ds), slots).cpu().numpy().reshape(len(cs), 7, 2)
-        Lp[c0:c0 + len(cs)] = L
+        Lp[c0:c0 + len(cs)] = like.loglike_cl(model.observed_cl(ds), slots).cpu().numpy().reshape(len(cs), 7, 2)
+    return Lp
+
+
+def tangent_gradients(model, like, thetas, k_grids=None, mock_index=None, phases=None, chunk: int = 64):
+    """d L / d theta for each row of ``thetas`` from tangent-linear ODE launches.
+
+    The cosmologies go through in chunks of ``chunk`` (one ODE launch each);
+    a host thread prepares the next chunks' background tables and tangent
+    seeds while the GPU works on the current one.  ``phases`` (optional
+    dict) receives the host time not hidden behind GPU work, the time of the
+    tangent-linear ODE launches with their base downstream, and of the
+    directional downstream.
+    """
+    import time
+    from concurrent.futures import ThreadPoolExecutor
+    t_start = time.perf_counter()
+    thetas = np.atleast_2d(thetas)
+    nc = thetas.shape[0]
+    mock_index = list(range(nc)) if mock_index is None else list(mock_index)
+    k_grids = list(k_grids) if k_grids is not None else [None] * nc
+    steps = FD_REL_STEP * np.abs(thetas)
+    bounds = [(c0, min(nc, c0 + chunk)) for c0 in range(0, nc, max(1, chunk))]
+    Lb = np.zeros(nc)
+    Lp = np.zeros((nc, 7, 2))
+    t_host = t_ode = t_dir = 0.0
+    with ThreadPoolExecutor(1) as host:
+        futs = [host.submit(_tangent_inputs, model, thetas[a:b], k_grids[a:b], steps[a:b]) for a, b in bounds]
+        for (a, b), fut in zip(bounds, futs):
+            t0 = time.perf_counter()
+            base_plans, pm, seeds = fut.result()
+            t1 = time.perf_counter()
+            outs, dpts = model.execute_tangent(base_plans, seeds)
+            Lb[a:b] = like.loglike(outs, mock_index[a:b]).cpu().numpy()
+            t2 = time.perf_counter()
+            Lp[a:b] = _directional_losses(model, like, thetas[a:b], steps[a:b], base_plans, pm, outs, dpts,
+                                          mock_index[a:b])
+            del outs, dpts
+            t3 = time.perf_counter()
+            t_host += t1 - t0
+            t_ode += t2 - t1
+            t_dir += t3 - t2
     grad = (Lp[:, :, 0] - Lp[:, :, 1]) / (2 * steps)
     if phases is not None:
-        phases.update(host_s=t_host - t_start, tangent_ode_s=t_ode - t_host,
-                      directional_downstream_s=time.perf_counter() - t_ode)
+        phases.update(host_exposed_s=t_host, tangent_ode_s=t_ode, directional_downstream_s=t_dir,
+                      total_s=time.perf_counter() - t_start, chunk=chunk)
     return Lb, grad

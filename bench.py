@@ -7,11 +7,13 @@ ranges, seed 0), each with its own uneven log-k grid of 300-800 modes in
 [7.05e-6, 0.62] Mpc^-1 (seed 0), one massive neutrino species (3 momentum
 bins x l <= 17, state size 100), lensed TT/TE/EE to l = 2500 in FP64 (ODE
 and LOS), Gaussian log-likelihood against a per-cosmology seeded mock.
-One step = the gradients of all ``--batch`` cosmologies: one tangent-linear
-ODE launch for all cosmologies (perturbation tables and their derivatives
-along omega_b, omega_cdm, h, tau_reio, m_nu), then the downstream differenced
-along all seven directions (``--gradient fd`` instead: central finite
-differences, 15 full model evaluations per gradient, all in one ODE launch).
+One step = the gradients of all ``--batch`` cosmologies: tangent-linear ODE
+launches of ``--chunk`` cosmologies each (perturbation tables and their
+derivatives along omega_b, omega_cdm, h, tau_reio, m_nu), each followed by
+the downstream differenced along all seven directions, with the next chunk's
+host preparation overlapping the GPU work (``--gradient fd`` instead:
+central finite differences, 15 full model evaluations per gradient, all in
+one ODE launch).
 
 Also reported: the plain path (one cosmology per launch, FP64, central
 finite differences) on a few of the same cosmologies and its agreement with
@@ -134,6 +136,7 @@ def main():
     ap.add_argument("--batch", type=int, default=256)
     ap.add_argument("--plain", type=int, default=2, help="cosmologies timed on the plain path")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--chunk", type=int, default=64, help="cosmologies per tangent-linear ODE launch")
     ap.add_argument("--gradient", choices=("tangent", "fd"), default="tangent",
                     help="batched gradient method (the plain path is always finite differences)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
@@ -162,7 +165,7 @@ def main():
 
     phases = {}
     if args.gradient == "tangent":
-        batched = lambda: LK.tangent_gradients(model, like, thetas, k_grids=k_grids, phases=phases)
+        batched = lambda: LK.tangent_gradients(model, like, thetas, k_grids=k_grids, phases=phases, chunk=args.chunk)
     else:
         batched = lambda: LK.fd_gradients(model, like, thetas, k_grids=k_grids, batched=True)
     for w in range(args.warmup):
@@ -216,8 +219,8 @@ def main():
         "vs_baseline": None, "dtype": "f64", "data": "synthetic",
         "config": {"workload": f"batched lensed-Cl likelihood gradients, {args.batch} Latin-hypercube cosmologies "
                                f"(massive nu, uneven 300-800 k-grids, l<=2500, FP64 ODE + FP64 LOS)",
-                   "gradient": ("tangent-linear ODE (5 directions, one launch) + downstream differenced along 7 "
-                                "directions" if args.gradient == "tangent" else
+                   "gradient": (f"tangent-linear ODE (5 directions, one launch per {args.chunk} cosmologies) + "
+                                "downstream differenced along 7 directions" if args.gradient == "tangent" else
                                 "central finite differences, 15 evaluations, grids frozen, all in one ODE launch"),
                    "plain_path": "one cosmology per launch, FP64, central finite differences (15 evaluations)",
                    "plain_path_grad_evals_per_s": plain, "speedup_vs_plain": value / plain,
