@@ -180,16 +180,12 @@ __device__ __forceinline__ T j_asym(T l, T x) {
   return sqrt(pi / (T)2 / x) * J;
 }
 
+// phi_0 = j_l, phi_1 = j_l', phi_2 = (3 j_l'' + j_l) / 2 from the asymptotic j_l, j_{l+1}
 template <typename T>
-__device__ __forceinline__ T phi_eval(int which, T l, const double* __restrict__ tab, T xmin, T xmax, int nx, T x) {
-  if (x < xmin) return (T)0;
-  if (x >= xmax) {
-    if (which == 0) return j_asym<T>(l, x);
-    if (which == 1) return l / x * j_asym<T>(l, x) - j_asym<T>(l + (T)1, x);
-    return (((T)3 * l * (l - (T)1) - (T)2 * x * x) * j_asym<T>(l, x) + (T)6 * x * j_asym<T>(l + (T)1, x)) /
-           (T)2 / (x * x);
-  }
-  return fast_interp_t<T>(x, xmin, xmax, tab, nx);
+__device__ __forceinline__ T phi_asym(int which, T l, T x, T jl, T jl1) {
+  if (which == 0) return jl;
+  if (which == 1) return l / x * jl - jl1;
+  return (((T)3 * l * (l - (T)1) - (T)2 * x * x) * jl + (T)6 * x * jl1) / (T)2 / (x * x);
 }
 
 // one block per (k chunk, l): partial C_l sums into part[3][n_ell][n_chunk]
@@ -229,9 +225,14 @@ __global__ void __launch_bounds__(NTH) los_cl_kernel(const AbcmbCosmo* __restric
     for (int s = 0; s < n_tau; ++s) {
       T x = (T)(tau0 - tau_los[s]) * kk;
       size_t o = (size_t)s * nkt + j;
-      T p0 = phi_eval<T>(0, l, t0, x0lo, x0hi, nx, x);
-      T p1 = phi_eval<T>(1, l, t1, x1lo, x1hi, nx, x);
-      T p2 = phi_eval<T>(2, l, t2, x2lo, x2hi, nx, x);
+      // above a table's range the three functions share j_l(x) and j_{l+1}(x)
+      const bool a0 = x >= x0hi, a1 = x >= x1hi, a2 = x >= x2hi;
+      T jl = (T)0, jl1 = (T)0;
+      if (a0 || a1 || a2) jl = j_asym<T>(l, x);
+      if (a1 || a2) jl1 = j_asym<T>(l + (T)1, x);
+      T p0 = x < x0lo ? (T)0 : a0 ? phi_asym<T>(0, l, x, jl, jl1) : fast_interp_t<T>(x, x0lo, x0hi, t0, nx);
+      T p1 = x < x1lo ? (T)0 : a1 ? phi_asym<T>(1, l, x, jl, jl1) : fast_interp_t<T>(x, x1lo, x1hi, t1, nx);
+      T p2 = x < x2lo ? (T)0 : a2 ? phi_asym<T>(2, l, x, jl, jl1) : fast_interp_t<T>(x, x2lo, x2hi, t2, nx);
       D0 += (T)src[0 * plane + o] * p0;
       D1 += (T)src[1 * plane + o] * p1;
       D2 += (T)src[2 * plane + o] * p2;

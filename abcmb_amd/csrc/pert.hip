@@ -420,7 +420,8 @@ struct Smem {
   double Fs[7][NC][NT];  // stage derivatives, one column per thread
 };
 
-// workgroup sum of NV values (wave shuffles + one LDS slot per wave)
+// workgroup sum of NV values (wave shuffles + one LDS slot per wave); red
+// holds at least NV * NT / 64 doubles (Smem::NRED per wave)
 template <int NT, int NV>
 __device__ __forceinline__ void block_sum(double (&v)[NV], double* red) {
 #pragma unroll
@@ -444,6 +445,13 @@ __device__ __forceinline__ void block_sum(double (&v)[NV], double* red) {
       v[q] = s;
     }
   }
+}
+
+// block_sum on the workgroup's reduction buffer, checked against its size
+template <int NT, int BMAX, int N, int NV>
+__device__ __forceinline__ void wg_sum(Smem<NT, BMAX, N>& sm, double (&v)[NV]) {
+  static_assert(NV <= Smem<NT, BMAX, N>::NRED, "reduction wider than the LDS buffer");
+  block_sum<NT, NV>(v, sm.red);
 }
 
 // background scalars of A at t + TIME_C[i] * hh, i < cnt (one lane per time)
@@ -493,7 +501,8 @@ template <int NT, int BMAX, int N>
 __device__ __forceinline__ D<N> rhs_eval(Smem<NT, BMAX, N>& sm, const RowState& R, int slot, const D<N>& z) {
   const int r = threadIdx.x;
   put_col<NT, BMAX, N>(sm.zl, r, z);
-  __syncthreads();
+  // with more than one wave the reduction's first barrier orders these writes
+  if constexpr (NT == 64) __syncthreads();
   const double* sc = sm.sc[slot][0];
   double v[2 * (1 + N)];
   v[0] = R.w.whs * sc[R.w.whi] * z.v;
@@ -506,7 +515,7 @@ __device__ __forceinline__ D<N> rhs_eval(Smem<NT, BMAX, N>& sm, const RowState& 
       v[3 + 2 * i] = R.w.wes * (sd[R.w.wei] * z.v + sc[R.w.wei] * z.d[i]);
     }
   }
-  block_sum<NT, 2 * (1 + N)>(v, sm.red);
+  wg_sum(sm, v);
   D<N> f;
   double acc = 0.0;
 #pragma unroll
@@ -589,7 +598,7 @@ __device__ bool factor(Smem<NT, BMAX, N>& sm, const RowState& R, int maxb, doubl
   F.Vh = R.w.whs * sc[R.w.whi];
   F.Ve = R.w.wes * sc[R.w.wei];
   double v[5] = {F.Vh * F.Yh, F.Vh * F.Ye, F.Ve * F.Yh, F.Ve * F.Ye, bad};
-  block_sum<NT, 5>(v, sm.red);
+  wg_sum(sm, v);
   double a = 1.0 - v[0], b = -v[1], cc = -v[2], d = 1.0 - v[3];
   double det = a * d - b * cc;
   F.c00 = d / det; F.c01 = -b / det; F.c10 = -cc / det; F.c11 = a / det;
@@ -614,7 +623,7 @@ __device__ __forceinline__ D<N> solve(Smem<NT, BMAX, N>& sm, const RowState& R, 
     v[2 * q] = F.Vh * s;
     v[2 * q + 1] = F.Ve * s;
   }
-  block_sum<NT, 2 * NC>(v, sm.red);
+  wg_sum(sm, v);
   double out[NC];
 #pragma unroll
   for (int q = 0; q < NC; ++q) {
@@ -844,7 +853,7 @@ __global__ void __launch_bounds__(NT) pert_fwd_kernel(const AbcmbCosmo* __restri
             v[2 + 2 * i_] = (z.d[i_] * w) * (z.d[i_] * w);
           }
         }
-        block_sum<NT, 1 + 2 * N>(v, sm.red);
+        wg_sum(sm, v);
         if (!conv) {
           double nrm = sqrt(v[0] / n);
           if (!isfinite(nrm)) break;
@@ -878,7 +887,7 @@ __global__ void __launch_bounds__(NT) pert_fwd_kernel(const AbcmbCosmo* __restri
 #pragma unroll
       for (int j = 0; j < 7; ++j) acc += KV_E[j] * sm.Fs[j][0][r];
       double v[1] = {r < n ? rms_term(hh * acc, y.v, ynew.v, rtol, atol) : 0.0};
-      block_sum<NT, 1>(v, sm.red);
+      wg_sum(sm, v);
       enorm = sqrt(v[0] / n);
       if (!isfinite(enorm)) enorm = INFINITY;
     }
