@@ -420,15 +420,35 @@ struct Smem {
   double Fs[7][NC][NT];  // stage derivatives, one column per thread
 };
 
-// workgroup sum of NV values (wave shuffles + one LDS slot per wave); red
+// workgroup sum of NV values (wavefront sums + one LDS slot per wave); red
 // holds at least NV * NT / 64 doubles (Smem::NRED per wave)
+// v from another lane by a DPP control (rows outside ROWMASK read 0)
+template <int CTRL, int ROWMASK>
+__device__ __forceinline__ double dpp_get(double v) {
+  int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), CTRL, ROWMASK, 0xf, false);
+  int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), CTRL, ROWMASK, 0xf, false);
+  return __hiloint2double(hi, lo);
+}
+
+// sum over the 64 lanes of a wavefront with DPP moves (no LDS traffic): quad
+// sums, 8-lane and 16-lane sums by mirroring within rows, then rows 0+1 and
+// 2+3 by row broadcasts and the total in lane 63, read out as a uniform value
+__device__ __forceinline__ double wave_sum(double v) {
+  v += dpp_get<0xb1, 0xf>(v);   // quad_perm [1,0,3,2]
+  v += dpp_get<0x4e, 0xf>(v);   // quad_perm [2,3,0,1]
+  v += dpp_get<0x141, 0xf>(v);  // row_half_mirror
+  v += dpp_get<0x140, 0xf>(v);  // row_mirror
+  v += dpp_get<0x142, 0xa>(v);  // row_bcast:15 into rows 1 and 3
+  v += dpp_get<0x143, 0xc>(v);  // row_bcast:31 into rows 2 and 3
+  int lo = __builtin_amdgcn_readlane(__double2loint(v), 63);
+  int hi = __builtin_amdgcn_readlane(__double2hiint(v), 63);
+  return __hiloint2double(hi, lo);
+}
+
 template <int NT, int NV>
 __device__ __forceinline__ void block_sum(double (&v)[NV], double* red) {
 #pragma unroll
-  for (int q = 0; q < NV; ++q) {
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v[q] += __shfl_xor(v[q], o, 64);
-  }
+  for (int q = 0; q < NV; ++q) v[q] = wave_sum(v[q]);
   if constexpr (NT > 64) {
     constexpr int NW = NT / 64;
     int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
