@@ -579,31 +579,32 @@ __device__ bool factor(Smem<NT, BMAX, N>& sm, const RowState& R, int maxb, doubl
     for (int c = 0; c < BMAX; ++c) F.row[c] -= (R.lc[j] == c) ? v : 0.0;
   }
   double bad = 0.0;
-  for (int p = 0; p < maxb; ++p) {
-    double* pb = sm.pbuf[p & 1][B.id];
-    const bool in = p < B.size;
-    if (in && B.lr == p) {
-      double d = 0.0;
+  // fully unrolled over the pivot index, so F.row[p] is a fixed register
+  // (maxb is the same for every thread: the guard is uniform)
 #pragma unroll
-      for (int c = 0; c < BMAX; ++c) d = (c == p) ? F.row[c] : d;
-      if (!(fabs(d) > 1e-12)) bad = 1.0;
-      double inv = 1.0 / d;
+  for (int p = 0; p < BMAX; ++p) {
+    if (p < maxb) {
+      double* pb = sm.pbuf[p & 1][B.id];
+      const bool in = p < B.size;
+      if (in && B.lr == p) {
+        const double d = F.row[p];
+        if (!(fabs(d) > 1e-12)) bad = 1.0;
+        const double inv = 1.0 / d;
 #pragma unroll
-      for (int c = 0; c < BMAX; ++c) {
-        double v = (c == p) ? inv : F.row[c] * inv;
-        F.row[c] = v;
-        pb[c] = v;
+        for (int c = 0; c < BMAX; ++c) {
+          double v = (c == p) ? inv : F.row[c] * inv;
+          F.row[c] = v;
+          pb[c] = v;
+        }
       }
-    }
-    __syncthreads();
-    if (in && B.lr != p && B.lr < B.size) {
-      double f = 0.0;
+      __syncthreads();
+      if (in && B.lr != p && B.lr < B.size) {
+        const double f = F.row[p];
 #pragma unroll
-      for (int c = 0; c < BMAX; ++c) f = (c == p) ? F.row[c] : f;
-#pragma unroll
-      for (int c = 0; c < BMAX; ++c) {
-        double pv = pb[c];
-        F.row[c] = (c == p) ? -f * pv : F.row[c] - f * pv;
+        for (int c = 0; c < BMAX; ++c) {
+          double pv = pb[c];
+          F.row[c] = (c == p) ? -f * pv : F.row[c] - f * pv;
+        }
       }
     }
   }
