@@ -9,12 +9,17 @@
 //   abcmb_los_cl   : Delta_l(k) = sum_i S(k, i) phi_l(k (tau0 - tau_i)) on the
 //                    tabulated Bessel functions (:697-783, lookups :48-109)
 //                    and the C_l trapezoid in k (:785-797).  FP64 or FP32.
+//   abcmb_los_tangent : the same C_l in FP64 with their derivatives along the
+//                    directions of the sources and of the conformal distance
+//                    (tangent-linear LOS, exact derivatives of the Bessel
+//                    lookups), and along ln A_s and n_s.
 //   abcmb_apply_operator : the l-spline (:598-601) as a precomputed linear map.
 //   abcmb_pk_lin   : P(k, z=0) (:250-302).
 #include <cmath>
 #include <cstdint>
 
 #include "common.h"
+#include "wave.h"
 
 namespace abcmb {
 namespace {
@@ -188,8 +193,24 @@ __device__ __forceinline__ T phi_asym(int which, T l, T x, T jl, T jl1) {
   return (((T)3 * l * (l - (T)1) - (T)2 * x * x) * jl + (T)6 * x * jl1) / (T)2 / (x * x);
 }
 
+// operand rounding of the contraction: none, or to bfloat16 (round to nearest
+// even) for the BF16 variant, whose products are then accumulated in FP32 --
+// the numerics of a BF16-input, FP32-accumulate matrix contraction
+template <bool BF16>
+__device__ __forceinline__ float operand(float x) {
+  if constexpr (BF16) {
+    uint32_t u = __float_as_uint(x);
+    u += 0x7fffu + ((u >> 16) & 1u);
+    return __uint_as_float(u & 0xffff0000u);
+  } else {
+    return x;
+  }
+}
+template <bool BF16>
+__device__ __forceinline__ double operand(double x) { return x; }
+
 // one block per (k chunk, l): partial C_l sums into part[3][n_ell][n_chunk]
-template <typename T, int NTH>
+template <typename T, int NTH, bool BF16 = false>
 __global__ void __launch_bounds__(NTH) los_cl_kernel(const AbcmbCosmo* __restrict__ cosmos,
                                                      const double* __restrict__ src_all, int n_tau,
                                                      const double* __restrict__ bgv_all,
@@ -233,10 +254,10 @@ __global__ void __launch_bounds__(NTH) los_cl_kernel(const AbcmbCosmo* __restric
       T p0 = x < x0lo ? (T)0 : a0 ? phi_asym<T>(0, l, x, jl, jl1) : fast_interp_t<T>(x, x0lo, x0hi, t0, nx);
       T p1 = x < x1lo ? (T)0 : a1 ? phi_asym<T>(1, l, x, jl, jl1) : fast_interp_t<T>(x, x1lo, x1hi, t1, nx);
       T p2 = x < x2lo ? (T)0 : a2 ? phi_asym<T>(2, l, x, jl, jl1) : fast_interp_t<T>(x, x2lo, x2hi, t2, nx);
-      D0 += (T)src[0 * plane + o] * p0;
-      D1 += (T)src[1 * plane + o] * p1;
-      D2 += (T)src[2 * plane + o] * p2;
-      D3 += (T)src[3 * plane + o] * (p0 / (x * x) * epsf);
+      D0 += operand<BF16>((T)src[0 * plane + o]) * operand<BF16>(p0);
+      D1 += operand<BF16>((T)src[1 * plane + o]) * operand<BF16>(p1);
+      D2 += operand<BF16>((T)src[2 * plane + o]) * operand<BF16>(p2);
+      D3 += operand<BF16>((T)src[3 * plane + o]) * operand<BF16>(p0 / (x * x) * epsf);
     }
     if (delta) {
       delta[((size_t)0 * n_ell + e) * nkt + j] = D0;
@@ -269,6 +290,181 @@ __global__ void __launch_bounds__(NTH) los_cl_kernel(const AbcmbCosmo* __restric
     part[((size_t)0 * n_ell + e) * nch + blockIdx.x] = a;
     part[((size_t)1 * n_ell + e) * nch + blockIdx.x] = b;
     part[((size_t)2 * n_ell + e) * nch + blockIdx.x] = c;
+  }
+}
+
+// ---- tangent-linear LOS ----
+// The Bessel lookups are differentiated as the reference evaluates them: the
+// slope of the interpolated table segment inside a table's range (the
+// derivative of the linear interpolant, which finite differences approach as
+// the step shrinks) and the derivative of the asymptotic form above it.
+//
+// j_l(x) of the asymptotic form and its x-derivative: with nu = l + 1/2,
+// r = sqrt(x^2 - nu^2), Q = r - nu pi/2 + nu asin(nu/x): dQ/dx = r/x, dr/dx = x/r
+__device__ __forceinline__ void j_asym_d(double l, double x, double& j, double& dj) {
+  const double pi = PI;
+  double nu = l + 0.5;
+  double r = sqrt(x * x - nu * nu);
+  double Q = r - nu * pi / 2.0 + nu * asin(nu / x);
+  double c = cos(Q - pi / 4.0), sn = sin(Q - pi / 4.0);
+  double J = sqrt(2.0 / pi / r) * c;
+  double pre = sqrt(pi / 2.0 / x);
+  j = pre * J;
+  double dJ = sqrt(2.0 / pi) * (-0.5 / (r * sqrt(r)) * (x / r) * c - sn / sqrt(r) * (r / x));
+  dj = -j / (2.0 * x) + pre * dJ;
+}
+
+// table value and slope of the interpolated segment (zero where the index is clipped)
+__device__ __forceinline__ void table_d(double x, double lo, double hi, const double* __restrict__ f, int n,
+                                        double& v, double& dv) {
+  double i0 = (x - lo) / (hi - lo) * (n - 1);
+  const double eps = 1e-6;
+  double i = fmin(fmax(i0, eps), n - 1.0 - eps);
+  int il = (int)floor(i);
+  int iu = min(il + 1, n - 1);
+  double wu = i - il;
+  v = (1.0 - wu) * f[il] + wu * f[iu];
+  dv = (i == i0) ? (f[iu] - f[il]) * (n - 1) / (hi - lo) : 0.0;
+}
+
+// one block per (k chunk, block of LB sampled l, cosmology); a thread owns one
+// k and LB multipoles.  part: [B][NS][n_ell][n_chunk] with NS = 3 (1 + ND + 2)
+template <int NTH, int LB, int ND>
+__global__ void __launch_bounds__(NTH) los_tangent_kernel(
+    const AbcmbCosmo* __restrict__ cosmos, const double* __restrict__ src_all,
+    const double* __restrict__ dsrc_all, const double* __restrict__ dchi_all, int n_tau,
+    const double* __restrict__ bgv_all, const double* __restrict__ kt, int nkt, const double* __restrict__ bes,
+    const double* __restrict__ rng, const int32_t* __restrict__ bes_l, int n_lt, int nx,
+    const int32_t* __restrict__ ell_idx, int n_ell, double* __restrict__ part_all) {
+  constexpr int NS = 3 * (1 + ND + 2);
+  constexpr int NW = NTH / 64;
+  __shared__ double red[LB * NS][NW];
+  const int ci = blockIdx.z;
+  const int e0 = blockIdx.y * LB;
+  const int j = blockIdx.x * NTH + threadIdx.x;
+  const AbcmbCosmo& cc = cosmos[ci];
+  const double tau0 = cc.tau0;
+  const size_t plane = (size_t)n_tau * nkt;
+  const double* src = src_all + (size_t)ci * 4 * plane;
+  const double* dsrc = dsrc_all + (size_t)ci * ND * 4 * plane;
+  const double* dchi = dchi_all + (size_t)ci * ND * n_tau;
+  const double* tau_los = bgv_all + (size_t)ci * 6 * n_tau;
+  const int nch = gridDim.x;
+  double* part = part_all + (size_t)ci * NS * n_ell * nch;
+
+  double lv[LB], eps_l[LB];
+  const double* tb[LB][3];
+  double lo[LB][3], hi[LB][3];
+#pragma unroll
+  for (int b = 0; b < LB; ++b) {
+    const int li = ell_idx[min(e0 + b, n_ell - 1)];
+    const double l = bes_l[li];
+    lv[b] = l;
+    eps_l[b] = sqrt(3.0 / 8.0 * (l + 2.0) * (l + 1.0) * l * (l - 1.0));
+#pragma unroll
+    for (int w = 0; w < 3; ++w) {
+      tb[b][w] = bes + ((size_t)w * n_lt + li) * nx;
+      lo[b][w] = rng[((size_t)w * n_lt + li) * 2];
+      hi[b][w] = rng[((size_t)w * n_lt + li) * 2 + 1];
+    }
+  }
+  double T[LB], E[LB], Td[LB][ND], Ed[LB][ND];
+#pragma unroll
+  for (int b = 0; b < LB; ++b) {
+    T[b] = E[b] = 0.0;
+#pragma unroll
+    for (int d = 0; d < ND; ++d) Td[b][d] = Ed[b][d] = 0.0;
+  }
+  const double kk = j < nkt ? kt[j] : kt[nkt - 1];
+  const int jj = j < nkt ? j : nkt - 1;
+  for (int s = 0; s < n_tau; ++s) {
+    const size_t o = (size_t)s * nkt + jj;
+    const double x = (tau0 - tau_los[s]) * kk;
+    double S[4], dS[ND][4], dx[ND];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) S[q] = src[q * plane + o];
+#pragma unroll
+    for (int d = 0; d < ND; ++d) {
+      dx[d] = kk * dchi[d * n_tau + s];
+#pragma unroll
+      for (int q = 0; q < 4; ++q) dS[d][q] = dsrc[((size_t)d * 4 + q) * plane + o];
+    }
+#pragma unroll
+    for (int b = 0; b < LB; ++b) {
+      const double l = lv[b];
+      const bool a0 = x >= hi[b][0], a1 = x >= hi[b][1], a2 = x >= hi[b][2];
+      double jl = 0.0, djl = 0.0, jl1 = 0.0, djl1 = 0.0;
+      if (a0 || a1 || a2) j_asym_d(l, x, jl, djl);
+      if (a1 || a2) j_asym_d(l + 1.0, x, jl1, djl1);
+      double p0 = 0.0, q0 = 0.0, p1 = 0.0, q1 = 0.0, p2 = 0.0, q2 = 0.0;  // phi_w and d phi_w / dx
+      if (x >= lo[b][0]) {
+        if (a0) { p0 = jl; q0 = djl; }
+        else table_d(x, lo[b][0], hi[b][0], tb[b][0], nx, p0, q0);
+      }
+      if (x >= lo[b][1]) {
+        if (a1) { p1 = l / x * jl - jl1; q1 = -l / (x * x) * jl + l / x * djl - djl1; }
+        else table_d(x, lo[b][1], hi[b][1], tb[b][1], nx, p1, q1);
+      }
+      if (x >= lo[b][2]) {
+        if (a2) {
+          const double c3 = 3.0 * l * (l - 1.0) - 2.0 * x * x;
+          const double Nn = c3 * jl + 6.0 * x * jl1;
+          const double dN = -4.0 * x * jl + c3 * djl + 6.0 * jl1 + 6.0 * x * djl1;
+          p2 = Nn / 2.0 / (x * x);
+          q2 = dN / (2.0 * x * x) - Nn / (x * x * x);
+        } else {
+          table_d(x, lo[b][2], hi[b][2], tb[b][2], nx, p2, q2);
+        }
+      }
+      const double psi = p0 / (x * x) * eps_l[b];
+      const double dpsi = eps_l[b] * (q0 / (x * x) - 2.0 * p0 / (x * x * x));
+      T[b] += S[0] * p0 + S[1] * p1 + S[2] * p2;
+      E[b] += S[3] * psi;
+      const double G = S[0] * q0 + S[1] * q1 + S[2] * q2;
+      const double H = S[3] * dpsi;
+#pragma unroll
+      for (int d = 0; d < ND; ++d) {
+        Td[b][d] += dS[d][0] * p0 + dS[d][1] * p1 + dS[d][2] * p2 + G * dx[d];
+        Ed[b][d] += dS[d][3] * psi + H * dx[d];
+      }
+    }
+  }
+  // trapezoid in k with the primordial spectrum; derivatives by the product rule,
+  // d/d ln A_s = the value, d/d n_s = the value times ln(k / k_pivot)
+  const double wj = 0.5 * ((jj + 1 < nkt ? kt[jj + 1] : kk) - (jj > 0 ? kt[jj - 1] : kk));
+  const double P = j < nkt ? 4. * PI * cc.A_s * pow(kk / cc.k_pivot, cc.n_s - 1.) / kk * wj : 0.0;
+  const double lnk = log(kk / cc.k_pivot);
+  const int wv = threadIdx.x >> 6;
+#pragma unroll
+  for (int b = 0; b < LB; ++b) {
+    double cb[NS];
+    cb[0] = P * T[b] * T[b];
+    cb[1] = P * T[b] * E[b];
+    cb[2] = P * E[b] * E[b];
+#pragma unroll
+    for (int d = 0; d < ND; ++d) {
+      cb[3 + 3 * d + 0] = P * 2.0 * T[b] * Td[b][d];
+      cb[3 + 3 * d + 1] = P * (Td[b][d] * E[b] + T[b] * Ed[b][d]);
+      cb[3 + 3 * d + 2] = P * 2.0 * E[b] * Ed[b][d];
+    }
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+      cb[3 * (1 + ND) + q] = cb[q];
+      cb[3 * (2 + ND) + q] = cb[q] * lnk;
+    }
+    wave_sum(cb);
+    if ((threadIdx.x & 63) == 0) {
+#pragma unroll
+      for (int q = 0; q < NS; ++q) red[b * NS + q][wv] = cb[q];
+    }
+  }
+  __syncthreads();
+  for (int v = threadIdx.x; v < LB * NS; v += NTH) {
+    const int b = v / NS, q = v % NS;
+    if (e0 + b >= n_ell) continue;
+    double t = 0.0;
+    for (int w = 0; w < NW; ++w) t += red[v][w];
+    part[((size_t)q * n_ell + e0 + b) * nch + blockIdx.x] = t;
   }
 }
 
@@ -379,7 +575,7 @@ extern "C" int abcmb_los_cl(const AbcmbCosmo* cosmos_dev, int n_cosmo, const dou
   ABCMB_REQUIRE(cosmos_dev && src_dev && bgv_dev && kt_dev && bes_dev && bes_rng_dev && bes_l_dev && ell_idx_dev &&
                     cl_dev, "null argument");
   ABCMB_REQUIRE(n_cosmo > 0 && n_tau > 0 && nkt > 1 && n_lt > 0 && nx > 1 && n_ell > 0, "bad sizes");
-  ABCMB_REQUIRE(precision == 0 || precision == 1, "precision must be 0 (FP64) or 1 (FP32)");
+  ABCMB_REQUIRE(precision >= 0 && precision <= 2, "precision must be 0 (FP64), 1 (FP32) or 2 (BF16 operands)");
   ABCMB_REQUIRE(workspace && ws_bytes >= abcmb_los_workspace_size(n_cosmo, n_ell, nkt), "workspace too small");
   hipStream_t s = (hipStream_t)stream;
   constexpr int NTH = 256;
@@ -389,11 +585,51 @@ extern "C" int abcmb_los_cl(const AbcmbCosmo* cosmos_dev, int n_cosmo, const dou
   if (precision == 0)
     hipLaunchKernelGGL((los_cl_kernel<double, NTH>), grid, dim3(NTH), 0, s, cosmos_dev, src_dev, n_tau, bgv_dev,
                        kt_dev, nkt, bes_dev, bes_rng_dev, bes_l_dev, n_lt, nx, ell_idx_dev, n_ell, part, delta_dev);
-  else
+  else if (precision == 1)
     hipLaunchKernelGGL((los_cl_kernel<float, NTH>), grid, dim3(NTH), 0, s, cosmos_dev, src_dev, n_tau, bgv_dev,
                        kt_dev, nkt, bes_dev, bes_rng_dev, bes_l_dev, n_lt, nx, ell_idx_dev, n_ell, part, delta_dev);
+  else
+    hipLaunchKernelGGL((los_cl_kernel<float, NTH, true>), grid, dim3(NTH), 0, s, cosmos_dev, src_dev, n_tau,
+                       bgv_dev, kt_dev, nkt, bes_dev, bes_rng_dev, bes_l_dev, n_lt, nx, ell_idx_dev, n_ell, part,
+                       delta_dev);
   hipLaunchKernelGGL(sum_parts_kernel, dim3((3 * n_ell * n_cosmo + 255) / 256), dim3(256), 0, s, part,
                      3 * n_ell * n_cosmo, nch, cl_dev);
+  ABCMB_HIP_CHECK(hipGetLastError());
+  return ABCMB_OK;
+}
+
+namespace {
+constexpr int LOS_T_NTH = 128;
+constexpr int LOS_T_LB = 2;
+}
+
+extern "C" size_t abcmb_los_tangent_workspace_size(int n_cosmo, int n_dir, int n_ell, int nkt) {
+  return sizeof(double) * (size_t)n_cosmo * 3 * (1 + n_dir + 2) * n_ell * ((nkt + LOS_T_NTH - 1) / LOS_T_NTH);
+}
+
+extern "C" int abcmb_los_tangent(const AbcmbCosmo* cosmos_dev, int n_cosmo, const double* src_dev,
+                                 const double* dsrc_dev, const double* dchi_dev, int n_dir, int n_tau,
+                                 const double* bgv_dev, const double* kt_dev, int nkt, const double* bes_dev,
+                                 const double* bes_rng_dev, const int32_t* bes_l_dev, int n_lt, int nx,
+                                 const int32_t* ell_idx_dev, int n_ell, double* cl_dev, void* workspace,
+                                 size_t ws_bytes, void* stream) {
+  ABCMB_REQUIRE(cosmos_dev && src_dev && dsrc_dev && dchi_dev && bgv_dev && kt_dev && bes_dev && bes_rng_dev &&
+                    bes_l_dev && ell_idx_dev && cl_dev, "null argument");
+  ABCMB_REQUIRE(n_dir == ABCMB_TANGENT_DIRS, "n_dir must be %d", ABCMB_TANGENT_DIRS);
+  ABCMB_REQUIRE(n_cosmo > 0 && n_tau > 0 && nkt > 1 && n_lt > 0 && nx > 1 && n_ell > 0, "bad sizes");
+  ABCMB_REQUIRE(workspace && ws_bytes >= abcmb_los_tangent_workspace_size(n_cosmo, n_dir, n_ell, nkt),
+                "workspace too small");
+  hipStream_t s = (hipStream_t)stream;
+  constexpr int ND = ABCMB_TANGENT_DIRS;
+  constexpr int NS = 3 * (1 + ND + 2);
+  const int nch = (nkt + LOS_T_NTH - 1) / LOS_T_NTH;
+  double* part = (double*)workspace;
+  dim3 grid(nch, (n_ell + LOS_T_LB - 1) / LOS_T_LB, n_cosmo);
+  hipLaunchKernelGGL((los_tangent_kernel<LOS_T_NTH, LOS_T_LB, ND>), grid, dim3(LOS_T_NTH), 0, s, cosmos_dev, src_dev,
+                     dsrc_dev, dchi_dev, n_tau, bgv_dev, kt_dev, nkt, bes_dev, bes_rng_dev, bes_l_dev, n_lt, nx,
+                     ell_idx_dev, n_ell, part);
+  hipLaunchKernelGGL(sum_parts_kernel, dim3((NS * n_ell * n_cosmo + 255) / 256), dim3(256), 0, s, part,
+                     NS * n_ell * n_cosmo, nch, cl_dev);
   ABCMB_HIP_CHECK(hipGetLastError());
   return ABCMB_OK;
 }

@@ -186,12 +186,74 @@ def _directional_losses(model, like, thetas, steps, base_plans, pm, outs, dpts, 
     return Lp
 
 
-def tangent_gradients(model, like, thetas, k_grids=None, mock_index=None, phases=None, chunk: int = 64):
+def _directional_losses_linear(model, like, thetas, steps, base_plans, pm, outs, dpts, mock_index):
+    """L at theta +- step along all seven directions ([nc][7][2]) with the unlensed
+    C_l moved along their derivatives from the tangent-linear LOS
+    (``Model.downstream_tangent``); only C_l^phiphi and the lensing are
+    evaluated at theta +- step."""
+    import torch
+    nc = thetas.shape[0]
+    nd = len(ODE_DIRS)
+    n_l = model.ells.size
+    pts = [o.PT.table for o in outs]
+    kdev = [o.PT.k_dev for o in outs]
+    plus = [pm[c][0::2] for c in range(nc)]
+    minus = [pm[c][1::2] for c in range(nc)]
+    sd = [[steps[c, p] for p in ODE_DIRS] for c in range(nc)]
+    cl_t = model.downstream_tangent(base_plans, pts, kdev, dpts, plus, minus, sd)   # [nc][1+nd+2][3][nl]
+    lensing = bool(model.specs["lensing"])
+    per = 2 * 7
+    group = max(1, model.chunk // per)
+    Lp = np.zeros((nc, 7, 2))
+    for c0 in range(0, nc, group):
+        cs = range(c0, min(nc, c0 + group))
+        plans, vpts, vk, cls, slots = [], [], [], [], []
+        for c in cs:
+            for p in range(7):
+                if p in ODE_DIRS:
+                    dC = cl_t[c, 1 + ODE_DIRS.index(p)]
+                elif NAMES[p] == "ln10As":
+                    dC = cl_t[c, 1 + nd]        # d C / d ln A_s
+                else:
+                    dC = cl_t[c, 2 + nd]        # d C / d n_s
+                for si, sgn in enumerate((+1, -1)):
+                    h = sgn * steps[c, p]
+                    cls.append(cl_t[c, 0] + h * dC)
+                    slots.append(mock_index[c])
+                    if not lensing:
+                        continue
+                    if p in ODE_DIRS:
+                        d = ODE_DIRS.index(p)
+                        plans.append(pm[c][2 * d + si])
+                        vpts.append(pts[c] + h * dpts[c][d])
+                    else:
+                        t = thetas[c].copy()
+                        t[p] += h
+                        q = theta_to_params(t)
+                        plans.append(with_primordial(base_plans[c], q["A_s"], q["n_s"]))
+                        vpts.append(pts[c])
+                    vk.append(kdev[c])
+        cl = torch.stack(cls)
+        if lensing:
+            obs = model.lensed_from_cl(plans, vpts, vk, cl)[:, :3, :n_l]
+        else:
+            obs = cl[:, :, :n_l]
+        Lp[c0:c0 + len(cs)] = like.loglike_cl(obs, slots).cpu().numpy().reshape(len(cs), 7, 2)
+    return Lp
+
+
+def tangent_gradients(model, like, thetas, k_grids=None, mock_index=None, phases=None, chunk: int = 64,
+                      downstream: str = "fd"):
     """d L / d theta for each row of ``thetas`` from tangent-linear ODE launches.
 
     The cosmologies go through in chunks of ``chunk`` (one ODE launch each);
     a host thread prepares the next chunks' background tables and tangent
-    seeds while the GPU works on the current one.  ``phases`` (optional
+    seeds while the GPU works on the current one.  ``downstream``: "fd"
+    differences the whole downstream (14 evaluations per cosmology, the same
+    relative step as the plain finite-difference path); "tangent"
+    differentiates the LOS in one tangent-linear pass (the Bessel lookups as
+    interpolated segments, as automatic differentiation of the reference
+    would) and differences only the lensing.  ``phases`` (optional
     dict) receives the host time not hidden behind GPU work, the time of the
     tangent-linear ODE launches with their base downstream, and of the
     directional downstream.
@@ -217,8 +279,8 @@ def tangent_gradients(model, like, thetas, k_grids=None, mock_index=None, phases
             outs, dpts = model.execute_tangent(base_plans, seeds)
             Lb[a:b] = like.loglike(outs, mock_index[a:b]).cpu().numpy()
             t2 = time.perf_counter()
-            Lp[a:b] = _directional_losses(model, like, thetas[a:b], steps[a:b], base_plans, pm, outs, dpts,
-                                          mock_index[a:b])
+            directional = _directional_losses_linear if downstream == "tangent" else _directional_losses
+            Lp[a:b] = directional(model, like, thetas[a:b], steps[a:b], base_plans, pm, outs, dpts, mock_index[a:b])
             del outs, dpts
             t3 = time.perf_counter()
             t_host += t1 - t0

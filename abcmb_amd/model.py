@@ -204,9 +204,10 @@ class Model:
             safety=0.9, factormin=0.2, factormax=10.0, newton_kappa=1e-2, jac_offset=0.6,
             max_steps=int(specs["max_steps_PE"]), max_newton=10,
             l_g=lay.l_g, l_pol=lay.l_pol, l_ur=lay.l_ur, l_ncdm=lay.l_ncdm, n_q=lay.n_q, n=lay.n)
-        if los_precision not in ("fp64", "fp32"):
-            raise ValueError("los_precision must be 'fp64' or 'fp32'")
-        self.los_precision = 0 if los_precision == "fp64" else 1
+        precisions = {"fp64": 0, "fp32": 1, "bf16": 2}
+        if los_precision not in precisions:
+            raise ValueError("los_precision must be 'fp64', 'fp32' or 'bf16'")
+        self.los_precision = precisions[los_precision]
         self.chunk = 128   # cosmologies per downstream launch (bounds the source buffers)
         self.timing = None  # list: per-launch pert_fwd event timing and work counts when set
         self._dev = None
@@ -470,6 +471,134 @@ class Model:
         # the host-side structs and staging buffers must outlive the launches
         torch.cuda.current_stream().synchronize()
         return dict(cl_raw=cl_raw, cl_full=cl_full, clpp=clpp, lensed=lensed, pk=pk)
+
+    def _staging(self):
+        torch = _torch()
+        dev = torch.device("cuda")
+        keep = []
+
+        def dt(a, dtype=torch.float64):
+            x = torch.as_tensor(np.ascontiguousarray(a), dtype=dtype).to(dev, non_blocking=False)
+            keep.append(x)
+            return x
+        return torch, dev, keep, dt
+
+    def _run_sources(self, plans, pts, kdev):
+        """LOS source functions [B][4][n_save-1][n_kt] of ``plans`` with tables ``pts``."""
+        torch, dev, keep, dt = self._staging()
+        lib = abi.hip_lib()
+        D = self._device_constants()
+        stream = torch.cuda.current_stream().cuda_stream
+        ns, ncos, nkt = self.n_save, len(plans), self.k_transfer.size
+        nkmax = max(int(k.numel()) for k in kdev)
+        cos_dev = self._struct_array(torch, dev, self._cosmo_array(plans, pts, kdev, dt, tables=False), dt)
+        bgv_all = dt(np.stack([pl.bgv for pl in plans]))
+        src = torch.empty((ncos, 4, ns - 1, nkt), dtype=torch.float64, device=dev)
+        chunk = max(1, min(ncos, self.chunk))
+        ws = torch.empty(lib.abcmb_sources_workspace_size(chunk, nkmax, ns) // 8 + 1, dtype=torch.float64, device=dev)
+        csz = C.sizeof(abi.AbcmbCosmo)
+        sc = self.scales.ctypes.data_as(C.c_void_p)
+        for c0 in range(0, ncos, chunk):
+            nb = min(chunk, ncos - c0)
+            abi.check(lib.abcmb_sources(cos_dev.data_ptr() + c0 * csz, nb, nkmax, ns, bgv_all[c0].data_ptr(),
+                                        D["kt"].data_ptr(), nkt, sc, src[c0].data_ptr(), ws.data_ptr(),
+                                        ws.numel() * 8, stream), "abcmb_sources")
+        torch.cuda.current_stream().synchronize()
+        return src
+
+    def downstream_tangent(self, plans, pts, kdev, dpts, plus, minus, steps):
+        """Unlensed C_l of ``plans`` and their derivatives, on every l.
+
+        Along the tangent directions d the tables move along dpts[i][d] and the
+        host inputs are differenced between plus[i][d] and minus[i][d] (plans at
+        theta +- steps[i][d], grids frozen): the sources' derivatives by central
+        differences of the (linear-in-the-table) source functions, the
+        distances' from the LOS background; the LOS itself is differentiated
+        in one tangent-linear pass (``abcmb_los_tangent``).  Returns
+        [B][1 + N_TANGENT_DIRS + 2][3][n_l]: value, the directions, d/d ln A_s,
+        d/d n_s.
+        """
+        torch, dev, keep, dt = self._staging()
+        lib = abi.hip_lib()
+        D = self._device_constants()
+        stream = torch.cuda.current_stream().cuda_stream
+        nd = abi.N_TANGENT_DIRS
+        ncos, nkt, n_ell, nl = len(plans), self.k_transfer.size, self.ell_idx.size, self.lensing_ells.size
+        n_tau = self.n_save - 1
+        nv = 1 + nd + 2
+        out = torch.empty((ncos, nv, 3, nl), dtype=torch.float64, device=dev)
+        group = max(1, self.chunk // (1 + 2 * nd))
+        for c0 in range(0, ncos, group):
+            cs = range(c0, min(ncos, c0 + group))
+            vplans, vpts, vk = [], [], []
+            for c in cs:
+                vplans.append(plans[c]); vpts.append(pts[c]); vk.append(kdev[c])
+                for d in range(nd):
+                    for sgn, pl in ((+1, plus[c][d]), (-1, minus[c][d])):
+                        vplans.append(pl); vpts.append(pts[c] + (sgn * steps[c][d]) * dpts[c][d]); vk.append(kdev[c])
+            src = self._run_sources(vplans, vpts, vk)
+            del vpts
+            per = 1 + 2 * nd
+            nb = len(cs)
+            src_b = src[0::per].contiguous()
+            dsrc = torch.empty((nb, nd, 4, n_tau, nkt), dtype=torch.float64, device=dev)
+            dchi = np.zeros((nb, nd, n_tau))
+            for i, c in enumerate(cs):
+                for d in range(nd):
+                    h2 = 2.0 * steps[c][d]
+                    torch.sub(src[i * per + 1 + 2 * d], src[i * per + 2 + 2 * d], out=dsrc[i, d]).div_(h2)
+                    pp, pm = plus[c][d], minus[c][d]
+                    dchi[i, d] = ((pp.bg.tau0 - pp.bgv[0]) - (pm.bg.tau0 - pm.bgv[0])) / h2
+            del src
+            cos_b = [plans[c] for c in cs]
+            cos_dev = self._struct_array(torch, dev, self._cosmo_array(cos_b, [pts[c] for c in cs],
+                                                                       [kdev[c] for c in cs], dt, tables=False), dt)
+            bgv_b = dt(np.stack([pl.bgv for pl in cos_b]))
+            dchi_d = dt(dchi)
+            ws = torch.empty(lib.abcmb_los_tangent_workspace_size(nb, nd, n_ell, nkt) // 8 + 1, dtype=torch.float64,
+                             device=dev)
+            raw = torch.empty((nb, nv, 3, n_ell), dtype=torch.float64, device=dev)
+            abi.check(lib.abcmb_los_tangent(cos_dev.data_ptr(), nb, src_b.data_ptr(), dsrc.data_ptr(), dchi_d.data_ptr(),
+                                            nd, n_tau, bgv_b.data_ptr(), D["kt"].data_ptr(), nkt, D["bes"].data_ptr(),
+                                            D["rng"].data_ptr(), D["bes_l"].data_ptr(), self.bes_l.size,
+                                            self.bes_tab.shape[2], D["ell_idx"].data_ptr(), n_ell, raw.data_ptr(),
+                                            ws.data_ptr(), ws.numel() * 8, stream), "abcmb_los_tangent")
+            abi.check(lib.abcmb_apply_operator(D["ell_op"].data_ptr(), nl, n_ell, raw.data_ptr(), nb * nv * 3,
+                                               out[c0].data_ptr(), stream), "abcmb_apply_operator")
+            torch.cuda.current_stream().synchronize()
+        return out
+
+    def lensed_from_cl(self, plans, pts, kdev, cl_full):
+        """Lensed [B][4][n_l] from unlensed cl_full [B][3][n_l], with C_l^phiphi of ``plans`` / ``pts``."""
+        torch, dev, keep, dt = self._staging()
+        lib = abi.hip_lib()
+        D = self._device_constants()
+        stream = torch.cuda.current_stream().cuda_stream
+        ncos, nl = len(plans), self.lensing_ells.size
+        cos_dev = self._struct_array(torch, dev, self._cosmo_array(plans, pts, kdev, dt, tables=False), dt)
+        n_lens = plans[0].lens_lna.size
+        lens_h = np.zeros((ncos, 4, n_lens))
+        for i, pl in enumerate(plans):
+            lens_h[i, 0], lens_h[i, 1], lens_h[i, 2] = pl.lens_lna, pl.lens_aH, pl.lens_tau
+            lens_h[i, 3, 0] = pl.chi_rec
+        lens_all = dt(lens_h)
+        cl_full = cl_full.contiguous()
+        clpp = torch.empty((ncos, nl), dtype=torch.float64, device=dev)
+        lensed = torch.empty((ncos, 4, nl), dtype=torch.float64, device=dev)
+        chunk = max(1, min(ncos, self.chunk))
+        ws = torch.empty(lib.abcmb_lens_workspace_size(chunk, self.mu.size, nl) // 8 + 1, dtype=torch.float64,
+                         device=dev)
+        csz = C.sizeof(abi.AbcmbCosmo)
+        for c0 in range(0, ncos, chunk):
+            nb = min(chunk, ncos - c0)
+            abi.check(lib.abcmb_lensing_clpp(cos_dev.data_ptr() + c0 * csz, nb, lens_all[c0].data_ptr(), n_lens,
+                                             D["lens_ells"].data_ptr(), nl, clpp[c0].data_ptr(), stream),
+                      "abcmb_lensing_clpp")
+            abi.check(lib.abcmb_lens(nb, cl_full[c0].data_ptr(), clpp[c0].data_ptr(), nl, D["mu"].data_ptr(),
+                                     D["w"].data_ptr(), self.mu.size, lensed[c0].data_ptr(), ws.data_ptr(),
+                                     ws.numel() * 8, stream), "abcmb_lens")
+        torch.cuda.current_stream().synchronize()
+        return lensed
 
     def observed_cl(self, ds):
         """[B][3][n_l] TT, TE, EE on l = 2..l_max (lensed when lensing is on) from a downstream result."""

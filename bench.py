@@ -139,6 +139,8 @@ def main():
     ap.add_argument("--chunk", type=int, default=64, help="cosmologies per tangent-linear ODE launch")
     ap.add_argument("--gradient", choices=("tangent", "fd"), default="tangent",
                     help="batched gradient method (the plain path is always finite differences)")
+    ap.add_argument("--los-tangent", action="store_true",
+                    help="differentiate the LOS in one tangent-linear pass instead of differencing the downstream")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="only run a device-to-device copy of BYTES (5 times): the reference for "
                          "scaling HBM byte counters")
@@ -165,7 +167,8 @@ def main():
 
     phases = {}
     if args.gradient == "tangent":
-        batched = lambda: LK.tangent_gradients(model, like, thetas, k_grids=k_grids, phases=phases, chunk=args.chunk)
+        batched = lambda: LK.tangent_gradients(model, like, thetas, k_grids=k_grids, phases=phases, chunk=args.chunk,
+                                               downstream="tangent" if args.los_tangent else "fd")
     else:
         batched = lambda: LK.fd_gradients(model, like, thetas, k_grids=k_grids, batched=True)
     for w in range(args.warmup):
@@ -209,6 +212,21 @@ def main():
             "frac": achieved / FP64_PEAK_TFLOPS, "traffic": None, "kernel": kname,
             "launch_ms": ms, "flops_per_launch": flops}
 
+    # batched forward spectra (64 Latin-hypercube cosmologies with their own k grids, lensed to l = 2500,
+    # FP64 ODE + FP32 LOS): host preparation + one ODE launch + downstream
+    from abcmb_amd.model import Model, MassiveNeutrino
+    mf = Model(user_species=(MassiveNeutrino,), lensing=True, l_max=2500, max_steps_PE=4096, los_precision="fp32")
+    nf = 64
+    tf_thetas = LK.latin_hypercube(nf, seed=2)
+    tf_k = LK.uneven_k_grids(nf, seed=2)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    plans_f = mf.prepare_many([mf.derived(LK.theta_to_params(t)) for t in tf_thetas], k_list=tf_k)
+    mf.execute(plans_f)
+    torch.cuda.synchronize()
+    spectra = nf / (time.perf_counter() - t0)
+    print(f"# forward {spectra:.2f} spectra/s", file=sys.stderr, flush=True)
+
     cpu = None
     if not args.no_cpu_baseline:
         cpu = cpu_baseline(model, thetas, k_grids)
@@ -220,11 +238,16 @@ def main():
         "config": {"workload": f"batched lensed-Cl likelihood gradients, {args.batch} Latin-hypercube cosmologies "
                                f"(massive nu, uneven 300-800 k-grids, l<=2500, FP64 ODE + FP64 LOS)",
                    "gradient": (f"tangent-linear ODE (5 directions, one launch per {args.chunk} cosmologies) + "
-                                "downstream differenced along 7 directions" if args.gradient == "tangent" else
+                                + ("tangent-linear LOS, lensing differenced along 7 directions" if args.los_tangent
+                                   else "downstream differenced along 7 directions")
+                                if args.gradient == "tangent" else
                                 "central finite differences, 15 evaluations, grids frozen, all in one ODE launch"),
                    "plain_path": "one cosmology per launch, FP64, central finite differences (15 evaluations)",
                    "plain_path_grad_evals_per_s": plain, "speedup_vs_plain": value / plain,
                    "grad_diff_vs_plain_rel_to_norm": agree, "grad_diff_vs_plain_max_rel": agree_rel,
+                   "forward_spectra_per_s": spectra,
+                   "forward_workload": "64 Latin-hypercube cosmologies (seed 2), own 300-800 k grids, lensed "
+                                       "TT/TE/EE/BB to l=2500, FP64 ODE + FP32 LOS, host preparation included",
                    "phases_s": phases},
         "roofline": roof,
         "cpu_baseline": cpu,

@@ -214,7 +214,8 @@ int abcmb_sources(const AbcmbCosmo* cosmos_dev, int n_cosmo, int nk_max, int n_s
  * k-quadrature (abcmb/spectrum.py:616-797, Bessel lookups :48-109).
  * bes_dev: tables [3][n_lt][nx] (phi0, phi1, phi2), bes_rng_dev: [3][n_lt][2]
  * x ranges, bes_l_dev: int32[n_lt] multipoles; ell_idx_dev: int32[n_ell]
- * rows of the Bessel tables to use.  precision: 0 = FP64, 1 = FP32.
+ * rows of the Bessel tables to use.  precision: 0 = FP64, 1 = FP32, 2 = source
+ * and Bessel operands rounded to bfloat16 with FP32 accumulation.
  * cl_dev out: [B][3][n_ell] (TT, TE, EE); delta_dev (optional) out:
  * [B][4][n_ell][nkt].
  */
@@ -226,6 +227,27 @@ int abcmb_los_cl(const AbcmbCosmo* cosmos_dev, int n_cosmo, const double* src_de
                  const int32_t* ell_idx_dev, int n_ell, int precision,
                  double* cl_dev, double* delta_dev,
                  void* workspace, size_t ws_bytes, void* stream);
+
+/*
+ * Tangent-linear line-of-sight C_l: the sampled-l TT, TE, EE of abcmb_los_cl
+ * (FP64) and their derivatives along n_dir (= ABCMB_TANGENT_DIRS) directions
+ * of the sources and of the conformal distance chi_i = tau0 - tau_i at the
+ * LOS nodes, plus the derivatives with respect to ln A_s and n_s.  The
+ * Bessel lookups are differentiated exactly (slope of the interpolated
+ * segment, derivative of the asymptotic form).  Replaces jax.jacfwd through
+ * SpectrumSolver.Cl_one_ell (abcmb/spectrum.py:616-797).
+ * dsrc_dev: [B][n_dir][4][n_tau][nkt] derivatives of src_dev; dchi_dev:
+ * [B][n_dir][n_tau] derivatives of chi; cl_dev out: [B][1 + n_dir + 2][3][n_ell]
+ * (value, the n_dir directions, ln A_s, n_s; each TT, TE, EE).
+ */
+size_t abcmb_los_tangent_workspace_size(int n_cosmo, int n_dir, int n_ell, int nkt);
+int abcmb_los_tangent(const AbcmbCosmo* cosmos_dev, int n_cosmo, const double* src_dev,
+                      const double* dsrc_dev, const double* dchi_dev, int n_dir, int n_tau,
+                      const double* bgv_dev, const double* kt_dev, int nkt,
+                      const double* bes_dev, const double* bes_rng_dev,
+                      const int32_t* bes_l_dev, int n_lt, int nx,
+                      const int32_t* ell_idx_dev, int n_ell, double* cl_dev,
+                      void* workspace, size_t ws_bytes, void* stream);
 
 /* Dense linear map out[m][r] = sum_c op[r][c] in[m][c] (the l-spline of
  * abcmb/spectrum.py:598-601 applied as a precomputed operator). */

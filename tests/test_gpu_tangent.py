@@ -138,3 +138,45 @@ def test_tangent_and_fd_gradients_converge_together():
     err = _norm_err(gt, gf, thetas)
     _say(f"tangent vs FD, relative to the norm: {err.max():.2e}")
     assert err.max() < 5e-4
+
+
+def test_tangent_los_matches_finite_differences():
+    """``abcmb_los_tangent`` (one tangent-linear LOS pass) against central
+    differences of the FP64 LOS at theta +- h with h shrinking: the Bessel
+    lookups are differentiated as interpolated segments, so the differences
+    converge to the tangent as h -> 0 (h = 1e-6 relative: 1e-4 of each
+    derivative's maximum; omega_cdm 1e-3, where table/asymptotic branch
+    switches at x_max -- a 4e-5 jump -- enter the differences but not the
+    derivative).  The value is the FP64 LOS's to 1e-12."""
+    import torch
+    from abcmb_amd.model import Model, MassiveNeutrino
+    from abcmb_amd.likelihood import with_primordial
+    m = Model(user_species=(MassiveNeutrino,), lensing=True, l_max=2500, max_steps_PE=4096,
+              k_axis_perturbations=np.geomspace(7.05e-6, 0.62, 200))
+    thetas = LK.latin_hypercube(1, seed=4)
+    rel = 1e-6
+    steps = rel * np.abs(thetas)
+    _say("case: tangent-linear LOS vs finite differences (h = 1e-6)")
+    base_plans, pm, seeds = LK._tangent_inputs(m, thetas, [None], steps)
+    outs, dpts = m.execute_tangent(base_plans, seeds)
+    pts = [outs[0].PT.table]
+    kdev = [outs[0].PT.k_dev]
+    plus, minus = [pm[0][0::2]], [pm[0][1::2]]
+    sd = [[steps[0, p] for p in LK.ODE_DIRS]]
+    cl_t = m.downstream_tangent(base_plans, pts, kdev, dpts, plus, minus, sd).cpu().numpy()[0]
+    c0 = outs[0].cl_unlensed.cpu().numpy()
+    assert np.max(np.abs(cl_t[0][[0, 2]] / c0[[0, 2]] - 1)) < 1e-12
+    for d, p in enumerate(LK.ODE_DIRS):
+        vt = [pts[0] + sd[0][d] * dpts[0][d], pts[0] - sd[0][d] * dpts[0][d]]
+        ds = m._downstream([plus[0][d], minus[0][d]], vt, kdev * 2)
+        fd = (ds["cl_full"][0] - ds["cl_full"][1]).cpu().numpy() / (2 * sd[0][d])
+        err = np.max(np.abs(cl_t[1 + d] - fd)) / np.max(np.abs(fd))
+        _say(f"{LK.NAMES[p]}: {err:.1e}")
+        assert err < (1e-3 if LK.NAMES[p] == "omega_cdm" else 1e-4)
+    for p, row in ((3, 1 + len(LK.ODE_DIRS)), (4, 2 + len(LK.ODE_DIRS))):
+        q = [LK.theta_to_params(thetas[0] + s * steps[0, p] * np.eye(7)[p]) for s in (1, -1)]
+        ds = m._downstream([with_primordial(base_plans[0], x["A_s"], x["n_s"]) for x in q], pts * 2, kdev * 2)
+        fd = (ds["cl_full"][0] - ds["cl_full"][1]).cpu().numpy() / (2 * steps[0, p])
+        err = np.max(np.abs(cl_t[row] - fd)) / np.max(np.abs(fd))
+        _say(f"{LK.NAMES[p]}: {err:.1e}")
+        assert err < 1e-4
