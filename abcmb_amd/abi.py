@@ -112,11 +112,11 @@ def hip_lib():
         lib = C.CDLL(HIP_LIB_PATH)
         lib.abcmb_last_error.restype = C.c_char_p
         lib.abcmb_pert_workspace_size.restype = C.c_size_t
-        lib.abcmb_pert_workspace_size.argtypes = [C.c_int]
-        lib.abcmb_pert_fwd.argtypes = [_P, C.c_int, _P, C.c_int, C.POINTER(AbcmbOdeSettings),
+        lib.abcmb_pert_workspace_size.argtypes = [C.c_int, C.c_int, C.c_int]
+        lib.abcmb_pert_fwd.argtypes = [_P, C.c_int, _P, C.c_int, C.POINTER(AbcmbOdeSettings), C.c_int,
                                        _P, _P, _P, _P, _P, C.c_size_t, _P]
-        lib.abcmb_pert_tangent.argtypes = [_P, _P, C.c_int, _P, C.c_int, C.POINTER(AbcmbOdeSettings),
-                                           _P, _P, _P, _P]
+        lib.abcmb_pert_tangent.argtypes = [_P, _P, C.c_int, _P, C.c_int, C.POINTER(AbcmbOdeSettings), C.c_int,
+                                           _P, _P, _P, _P, C.c_size_t, _P]
         lib.abcmb_sources_workspace_size.restype = C.c_size_t
         lib.abcmb_sources_workspace_size.argtypes = [C.c_int, C.c_int, C.c_int]
         lib.abcmb_sources.argtypes = [_P, C.c_int, C.c_int, C.c_int, _P, _P, C.c_int, _P, _P, _P, C.c_size_t, _P]

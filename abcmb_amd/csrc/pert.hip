@@ -675,16 +675,15 @@ __device__ __noinline__ D<N> ic_row(const Cosmo<N>& P, const Layout& L, double k
 
 // output-table fields at one saved point from the state in LDS (ys[c][row])
 // (abcmb/perturbations.py:318-404; output_perturbations in abcmb/species.py)
+// b: the background at the saved point (precomputed once per cosmology)
 template <int N, int LD>
-__device__ __noinline__ D<N> output_field(const Cosmo<N>& P, const Layout& L, double k, double lna,
+__device__ __noinline__ D<N> output_field(const Cosmo<N>& P, const Layout& L, double k, const Bg<N>& b,
                                           const double (*ys)[LD], int f) {
   auto y = [&](int i) {
     D<N> x(ys[0][i]);
     if constexpr (N > 0) { ABCMB_DLOOP(N) x.d[i_] = ys[1 + i_][i]; }
     return x;
   };
-  Bg<N> b;
-  bg_eval(P, lna, b);
   const double a = b.a;
   const D<N> aH = b.aH;
   D<N> nrd(0.0), nrt(0.0), nrs(0.0);
@@ -739,11 +738,29 @@ __device__ __noinline__ D<N> output_field(const Cosmo<N>& P, const Layout& L, do
   return y(0) / aH - 2. * al - 3. * KG / aH * srs / (k * k);  // ABCMB_PT_ALPHA_PRIME
 }
 
+// the background (with its tangents) at every saved point of every cosmology,
+// shared by all the cosmology's k-modes: bgsave[cosmo][i], i < n_save_max
+template <int N>
+__global__ void bg_save_kernel(const AbcmbCosmo* __restrict__ cosmos, const AbcmbTangent* __restrict__ tangents,
+                               int n_save_max, Bg<N>* __restrict__ bgsave) {
+  const int ci = blockIdx.y;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const AbcmbCosmo& c = cosmos[ci];
+  if (i >= c.n_save || i >= n_save_max) return;
+  const AbcmbTangent* T = nullptr;
+  if constexpr (N > 0) T = tangents + ci;
+  const Cosmo<N> P{c, T};
+  Bg<N> b;
+  bg_eval(P, c.lna_save[i], b);
+  bgsave[(size_t)ci * n_save_max + i] = b;
+}
+
 template <int NT, int BMAX, int N>
 __global__ void __launch_bounds__(NT) pert_fwd_kernel(const AbcmbCosmo* __restrict__ cosmos,
                                                       const AbcmbTangent* __restrict__ tangents,
                                                       const AbcmbPertItem* __restrict__ items,
-                                                      AbcmbOdeSettings st, int32_t* __restrict__ status_out,
+                                                      AbcmbOdeSettings st, const Bg<N>* __restrict__ bgsave,
+                                                      int n_save_max, int32_t* __restrict__ status_out,
                                                       int32_t* __restrict__ steps_out, int32_t* __restrict__ iters_out,
                                                       double* __restrict__ state_out) {
   __shared__ Smem<NT, BMAX, N> sm;
@@ -777,6 +794,11 @@ __global__ void __launch_bounds__(NT) pert_fwd_kernel(const AbcmbCosmo* __restri
   const double rtol = I.rtol, atol = I.atol;
   const int n_save = c.n_save;
   const double* __restrict__ lna_save = c.lna_save;
+  if (n_save > n_save_max) {  // the background table has no room for this cosmology's saves
+    if (r == 0) { status_out[item] = ABCMB_K_BAD_INPUT; steps_out[item] = 0; if (iters_out) iters_out[item] = 0; }
+    return;
+  }
+  const Bg<N>* __restrict__ bgs = bgsave + (size_t)I.cosmo * n_save_max;
 
   double t = I.lna_start;
   const D<N> ti = fast_interp<N>(t, ABCMB_TAU_TAB_LO, 0.0, c.tau_tab, P.dtau(), ABCMB_TAU_TAB_N);
@@ -789,13 +811,13 @@ __global__ void __launch_bounds__(NT) pert_fwd_kernel(const AbcmbCosmo* __restri
   int isave = 0, steps = 0, iters = 0, status = ABCMB_K_OK;
   Inv<BMAX> FI;
 
-  auto save_point = [&](const D<N>& ys, double ls) {
+  auto save_point = [&](const D<N>& ys) {
     __syncthreads();
     put_col<NT, BMAX, N>(sm.zl, r, ys);
     __syncthreads();
     if (state_out && r < n) state_out[((size_t)item * n_save + isave) * n + r] = ys.v;
     if (r < ABCMB_N_PT_FIELDS) {
-      D<N> v = output_field<N, NT + BMAX>(P, L, k, ls, sm.zl, r);
+      D<N> v = output_field<N, NT + BMAX>(P, L, k, bgs[isave], sm.zl, r);
       const size_t o = ((size_t)r * n_save + isave) * c.nk + I.ik;
       c.pt[o] = v.v;
       if constexpr (N > 0) {
@@ -805,7 +827,7 @@ __global__ void __launch_bounds__(NT) pert_fwd_kernel(const AbcmbCosmo* __restri
     }
   };
 
-  while (isave < n_save && lna_save[isave] <= t) { save_point(y, lna_save[isave]); ++isave; }
+  while (isave < n_save && lna_save[isave] <= t) { save_point(y); ++isave; }
 
   while (t < 0.0) {
     if (steps >= st.max_steps) { status = ABCMB_K_MAX_STEPS; break; }
@@ -902,7 +924,7 @@ __global__ void __launch_bounds__(NT) pert_fwd_kernel(const AbcmbCosmo* __restri
         double h00 = 2 * th * th * th - 3 * th * th + 1, h10 = th * th * th - 2 * th * th + th;
         double h01 = -2 * th * th * th + 3 * th * th, h11 = th * th * th - th * th;
         D<N> ys = h00 * y + h10 * hh * f0 + h01 * ynew + h11 * hh * fnew;
-        save_point(ys, lna_save[isave]);
+        save_point(ys);
         ++isave;
       }
       t = tn;
@@ -921,10 +943,10 @@ __global__ void __launch_bounds__(NT) pert_fwd_kernel(const AbcmbCosmo* __restri
 
 template <int NT, int BMAX, int N>
 int launch(const AbcmbCosmo* cosmos, const AbcmbTangent* tangents, const AbcmbPertItem* items, int n_items,
-           const AbcmbOdeSettings& st, int32_t* status, int32_t* steps, int32_t* iters, double* state,
-           hipStream_t stream) {
+           const AbcmbOdeSettings& st, const Bg<N>* bgsave, int n_save_max, int32_t* status, int32_t* steps,
+           int32_t* iters, double* state, hipStream_t stream) {
   hipLaunchKernelGGL((pert_fwd_kernel<NT, BMAX, N>), dim3(n_items), dim3(NT), 0, stream, cosmos, tangents, items,
-                     st, status, steps, iters, state);
+                     st, bgsave, n_save_max, status, steps, iters, state);
   ABCMB_HIP_CHECK(hipGetLastError());
   return ABCMB_OK;
 }
@@ -945,52 +967,63 @@ int check_settings(const AbcmbOdeSettings& st, int& maxb) {
 }
 
 template <int N>
-int dispatch(const AbcmbCosmo* cosmos, const AbcmbTangent* tangents, const AbcmbPertItem* items, int n_items,
-             const AbcmbOdeSettings& st, int maxb, int32_t* status, int32_t* steps, int32_t* iters, double* state,
-             hipStream_t s) {
+int dispatch(const AbcmbCosmo* cosmos, const AbcmbTangent* tangents, int n_cosmo, const AbcmbPertItem* items,
+             int n_items, const AbcmbOdeSettings& st, int maxb, int n_save_max, void* workspace, int32_t* status,
+             int32_t* steps, int32_t* iters, double* state, hipStream_t s) {
+  Bg<N>* bgs = (Bg<N>*)workspace;
+  hipLaunchKernelGGL((bg_save_kernel<N>), dim3((n_save_max + 63) / 64, n_cosmo), dim3(64), 0, s, cosmos, tangents,
+                     n_save_max, bgs);
+  ABCMB_HIP_CHECK(hipGetLastError());
+#define ABCMB_LAUNCH(NT_, BM_) \
+  return launch<NT_, BM_, N>(cosmos, tangents, items, n_items, st, bgs, n_save_max, status, steps, iters, state, s)
   if (maxb <= 32) {
-    if (st.n <= 64) return launch<64, 32, N>(cosmos, tangents, items, n_items, st, status, steps, iters, state, s);
-    if (st.n <= 128) return launch<128, 32, N>(cosmos, tangents, items, n_items, st, status, steps, iters, state, s);
-    return launch<192, 32, N>(cosmos, tangents, items, n_items, st, status, steps, iters, state, s);
+    if (st.n <= 64) ABCMB_LAUNCH(64, 32);
+    if (st.n <= 128) ABCMB_LAUNCH(128, 32);
+    ABCMB_LAUNCH(192, 32);
   }
-  if (st.n <= 64) return launch<64, 64, N>(cosmos, tangents, items, n_items, st, status, steps, iters, state, s);
-  if (st.n <= 128) return launch<128, 64, N>(cosmos, tangents, items, n_items, st, status, steps, iters, state, s);
-  return launch<192, 64, N>(cosmos, tangents, items, n_items, st, status, steps, iters, state, s);
+  if (st.n <= 64) ABCMB_LAUNCH(64, 64);
+  if (st.n <= 128) ABCMB_LAUNCH(128, 64);
+  ABCMB_LAUNCH(192, 64);
+#undef ABCMB_LAUNCH
 }
 
 }  // namespace
 }  // namespace abcmb
 
-extern "C" size_t abcmb_pert_workspace_size(int n_items) {
-  (void)n_items;
-  return 0;
+extern "C" size_t abcmb_pert_workspace_size(int n_cosmo, int n_save_max, int n_dir) {
+  using namespace abcmb;
+  const size_t per = n_dir > 0 ? sizeof(Bg<ABCMB_TANGENT_DIRS>) : sizeof(Bg<0>);
+  return per * (size_t)(n_cosmo > 0 ? n_cosmo : 0) * (size_t)(n_save_max > 0 ? n_save_max : 0);
 }
 
 extern "C" int abcmb_pert_fwd(const AbcmbCosmo* cosmos_dev, int n_cosmo, const AbcmbPertItem* items_dev,
-                              int n_items, const AbcmbOdeSettings* settings, int32_t* status_dev,
+                              int n_items, const AbcmbOdeSettings* settings, int n_save_max, int32_t* status_dev,
                               int32_t* steps_dev, int32_t* iters_dev, double* state_dev, void* workspace,
                               size_t ws_bytes, void* stream) {
   using namespace abcmb;
-  (void)workspace;
-  (void)ws_bytes;
   ABCMB_REQUIRE(cosmos_dev && items_dev && settings && status_dev && steps_dev, "null argument");
-  ABCMB_REQUIRE(n_cosmo > 0 && n_items > 0, "empty launch");
+  ABCMB_REQUIRE(n_cosmo > 0 && n_items > 0 && n_save_max > 0, "empty launch");
+  ABCMB_REQUIRE(workspace && ws_bytes >= abcmb_pert_workspace_size(n_cosmo, n_save_max, 0), "workspace too small");
   int maxb = 0;
   int rc = check_settings(*settings, maxb);
   if (rc != ABCMB_OK) return rc;
-  return dispatch<0>(cosmos_dev, nullptr, items_dev, n_items, *settings, maxb, status_dev, steps_dev, iters_dev,
-                     state_dev, (hipStream_t)stream);
+  return dispatch<0>(cosmos_dev, nullptr, n_cosmo, items_dev, n_items, *settings, maxb, n_save_max, workspace,
+                     status_dev, steps_dev, iters_dev, state_dev, (hipStream_t)stream);
 }
 
 extern "C" int abcmb_pert_tangent(const AbcmbCosmo* cosmos_dev, const AbcmbTangent* tangents_dev, int n_cosmo,
                                   const AbcmbPertItem* items_dev, int n_items, const AbcmbOdeSettings* settings,
-                                  int32_t* status_dev, int32_t* steps_dev, int32_t* iters_dev, void* stream) {
+                                  int n_save_max, int32_t* status_dev, int32_t* steps_dev, int32_t* iters_dev,
+                                  void* workspace, size_t ws_bytes, void* stream) {
   using namespace abcmb;
   ABCMB_REQUIRE(cosmos_dev && tangents_dev && items_dev && settings && status_dev && steps_dev, "null argument");
-  ABCMB_REQUIRE(n_cosmo > 0 && n_items > 0, "empty launch");
+  ABCMB_REQUIRE(n_cosmo > 0 && n_items > 0 && n_save_max > 0, "empty launch");
+  ABCMB_REQUIRE(workspace && ws_bytes >= abcmb_pert_workspace_size(n_cosmo, n_save_max, ABCMB_TANGENT_DIRS),
+                "workspace too small");
   int maxb = 0;
   int rc = check_settings(*settings, maxb);
   if (rc != ABCMB_OK) return rc;
-  return dispatch<ABCMB_TANGENT_DIRS>(cosmos_dev, tangents_dev, items_dev, n_items, *settings, maxb, status_dev,
-                                      steps_dev, iters_dev, nullptr, (hipStream_t)stream);
+  return dispatch<ABCMB_TANGENT_DIRS>(cosmos_dev, tangents_dev, n_cosmo, items_dev, n_items, *settings, maxb,
+                                      n_save_max, workspace, status_dev, steps_dev, iters_dev, nullptr,
+                                      (hipStream_t)stream);
 }

@@ -358,6 +358,9 @@ class Model:
                                      xe_tab=dt(sd.xe).data_ptr(), tm_tab=dt(sd.tm).data_ptr(), pt=dp.data_ptr())
                     for sd, dp in zip(tangents, dpts)]
             tan_dev = self._struct_array(torch, dev, tans, dt)
+        nd = abi.N_TANGENT_DIRS if tangents is not None else 0
+        ws_bytes = lib.abcmb_pert_workspace_size(len(plans), ns, nd)
+        ws = torch.empty(ws_bytes // 8 + 1, dtype=torch.float64, device=dev)
         ev0 = ev1 = None
         if self.timing is not None:
             ev0 = torch.cuda.Event(enable_timing=True)
@@ -365,14 +368,16 @@ class Model:
             ev0.record()
         st = self.settings
         if tangents is None:
-            abi.check(lib.abcmb_pert_fwd(cos_dev.data_ptr(), len(plans), items_dev.data_ptr(), n_items, C.byref(st),
+            abi.check(lib.abcmb_pert_fwd(cos_dev.data_ptr(), len(plans), items_dev.data_ptr(), n_items, C.byref(st), ns,
                                          status.data_ptr(), steps.data_ptr(), iters.data_ptr(),
-                                         state.data_ptr() if state is not None else None, None, 0, stream),
+                                         state.data_ptr() if state is not None else None, ws.data_ptr(),
+                                         ws.numel() * 8, stream),
                       "abcmb_pert_fwd")
         else:
             abi.check(lib.abcmb_pert_tangent(cos_dev.data_ptr(), tan_dev.data_ptr(), len(plans), items_dev.data_ptr(),
-                                             n_items, C.byref(st), status.data_ptr(), steps.data_ptr(),
-                                             iters.data_ptr(), stream), "abcmb_pert_tangent")
+                                             n_items, C.byref(st), ns, status.data_ptr(), steps.data_ptr(),
+                                             iters.data_ptr(), ws.data_ptr(), ws.numel() * 8, stream),
+                      "abcmb_pert_tangent")
         if ev1 is not None:
             ev1.record()
         torch.cuda.synchronize()
@@ -387,7 +392,7 @@ class Model:
             where = [(flat[b][1], float(allk[flat[b][1]][flat[b][2]])) for b in bad[:8]]
             raise RuntimeError(f"perturbation ODE failed for {bad.size} (cosmology, k) items; status codes "
                                f"{np.unique(st_h[bad]).tolist()} (1 = max_steps_PE reached, 2 = non-finite, "
-                               f"3 = step-size underflow); first (cosmology, k): {where}")
+                               f"3 = step-size underflow, 4 = bad input); first (cosmology, k): {where}")
         steps_ck = [np.zeros(a.size, dtype=np.int32) for a in allk]
         for m, (_, ci, ik) in enumerate(flat):
             steps_ck[ci][ik] = steps_h[m]

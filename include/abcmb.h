@@ -35,6 +35,7 @@ extern "C" {
 #define ABCMB_K_MAX_STEPS 1   /* max_steps reached before ln a = 0 */
 #define ABCMB_K_NONFINITE 2   /* NaN/Inf in state or error norm */
 #define ABCMB_K_STEP_UNDERFLOW 3  /* step size fell below the spacing of ln a */
+#define ABCMB_K_BAD_INPUT 4   /* the cosmology has more saved points than n_save_max */
 
 #define ABCMB_TAU_TAB_N 10000      /* tau(ln a) table on [-33, 0] */
 #define ABCMB_TAU_TAB_LO (-33.0)
@@ -161,11 +162,14 @@ int abcmb_device_init(void);
  * status_dev/steps_dev: int32[n_items] outputs (status, accepted+rejected
  * steps); iters_dev (optional): int32[n_items] chord-Newton iterations;
  * state_dev: optional [n_items][n_save][n] raw states (NULL = off).
+ * n_save_max: upper bound of the cosmologies' n_save; the workspace
+ * (abcmb_pert_workspace_size(n_cosmo, n_save_max, 0) bytes) holds the
+ * background at every saved point, computed once per cosmology.
  */
-size_t abcmb_pert_workspace_size(int n_items);
+size_t abcmb_pert_workspace_size(int n_cosmo, int n_save_max, int n_dir);
 int abcmb_pert_fwd(const AbcmbCosmo* cosmos_dev, int n_cosmo,
                    const AbcmbPertItem* items_dev, int n_items,
-                   const AbcmbOdeSettings* settings,
+                   const AbcmbOdeSettings* settings, int n_save_max,
                    int32_t* status_dev, int32_t* steps_dev, int32_t* iters_dev,
                    double* state_dev, void* workspace, size_t ws_bytes, void* stream);
 
@@ -180,13 +184,14 @@ int abcmb_pert_fwd(const AbcmbCosmo* cosmos_dev, int n_cosmo,
  * Replaces jax.jacfwd through PerturbationEvolver.full_evolution
  * (abcmb/perturbations.py:75-117, :250-404) for the parameters that reach
  * the perturbation equations.  tangents_dev: n_cosmo device-resident
- * AbcmbTangent (inputs and the tangent-table output pointer).
+ * AbcmbTangent (inputs and the tangent-table output pointer); workspace:
+ * abcmb_pert_workspace_size(n_cosmo, n_save_max, ABCMB_TANGENT_DIRS) bytes.
  */
 int abcmb_pert_tangent(const AbcmbCosmo* cosmos_dev, const AbcmbTangent* tangents_dev, int n_cosmo,
                        const AbcmbPertItem* items_dev, int n_items,
-                       const AbcmbOdeSettings* settings,
+                       const AbcmbOdeSettings* settings, int n_save_max,
                        int32_t* status_dev, int32_t* steps_dev, int32_t* iters_dev,
-                       void* stream);
+                       void* workspace, size_t ws_bytes, void* stream);
 
 /*
  * The entry points below work on a batch of n_cosmo cosmologies described by
