@@ -246,8 +246,9 @@ def tangent_gradients(model, like, thetas, k_grids=None, mock_index=None, phases
                       downstream: str = "fd"):
     """d L / d theta for each row of ``thetas`` from tangent-linear ODE launches.
 
-    The cosmologies go through in chunks of ``chunk`` (one ODE launch each);
-    a host thread prepares the next chunks' background tables and tangent
+    The cosmologies go through in chunks of ``chunk`` (one ODE launch each,
+    the first a quarter of that); a host thread prepares the next chunks'
+    background tables and tangent
     seeds while the GPU works on the current one.  ``downstream``: "fd"
     differences the whole downstream (14 evaluations per cosmology, the same
     relative step as the plain finite-difference path); "tangent"
@@ -266,7 +267,10 @@ def tangent_gradients(model, like, thetas, k_grids=None, mock_index=None, phases
     mock_index = list(range(nc)) if mock_index is None else list(mock_index)
     k_grids = list(k_grids) if k_grids is not None else [None] * nc
     steps = FD_REL_STEP * np.abs(thetas)
-    bounds = [(c0, min(nc, c0 + chunk)) for c0 in range(0, nc, max(1, chunk))]
+    # a small first chunk, so the GPU starts while the host prepares the rest
+    chunk = max(1, chunk)
+    first = min(nc, max(1, chunk // 4))
+    bounds = [(0, first)] + [(c0, min(nc, c0 + chunk)) for c0 in range(first, nc, chunk)]
     Lb = np.zeros(nc)
     Lp = np.zeros((nc, 7, 2))
     t_host = t_ode = t_dir = 0.0
