@@ -208,9 +208,17 @@ def main():
         flops = sum(ode_flops(model, t["steps"], t["iters"]) for t in timing) / len(timing)
         kname = "pert_fwd_kernel<128, 32, 0> (FP64 VALU; peak = MI355X FP64 vector/matrix)"
     achieved = flops / (ms * 1e-3) / 1e12
+    # HBM bytes per launch of the tangent kernel, from the two PMC runs summarised in profiles/
+    traffic = None
+    tfile = os.path.join(ROOT, "profiles", "pert_tangent_traffic.json")
+    if args.gradient == "tangent" and os.path.exists(tfile):
+        t = json.load(open(tfile))
+        traffic = t["fetch_bytes_per_launch"] + t["write_bytes_per_launch"]
     roof = {"bound": "mfma", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
-            "frac": achieved / FP64_PEAK_TFLOPS, "traffic": None, "kernel": kname,
-            "launch_ms": ms, "flops_per_launch": flops}
+            "frac": achieved / FP64_PEAK_TFLOPS, "traffic": traffic, "kernel": kname,
+            "launch_ms": ms, "flops_per_launch": flops,
+            "traffic_source": "profiles/pert_tangent_traffic.json (FETCH_SIZE x2.0 + WRITE_SIZE x1.0, "
+                              "scaled by a 1 GiB copy)" if traffic else None}
 
     # batched forward spectra (64 Latin-hypercube cosmologies with their own k grids, lensed to l = 2500,
     # FP64 ODE + FP32 LOS): host preparation + one ODE launch + downstream
