@@ -410,45 +410,57 @@ __device__ Blk row_block(const Layout& L, int r) {
 template <int NT, int BMAX, int N>
 struct Smem {
   static constexpr int NC = 1 + N;                        // value + tangents
-  static constexpr int NRED = (2 * NC > 8 ? 2 * NC : 8);  // widest workgroup sum
+  // widest workgroup sum: the Newton test's 1 + 2N norms plus the 2(1 + N)
+  // metric-sum increments
+  static constexpr int NRED = (3 + 4 * N > 8 ? 3 + 4 * N : 8);
   double sc[NTIMES][NC][SC_COUNT];
   double zl[NC][NT + BMAX];
   double rl[NC][NT + BMAX];
   double uh[NT + BMAX];
   double ue[NT + BMAX];
   double pbuf[2][NBLK][BMAX];
-  double red[NRED * (NT / 64)];
+  double red[2][NRED * (NT / 64)];  // alternating halves: one barrier per sum
   double Fs[7][NC][NT];  // stage derivatives, one column per thread
 };
 
-// workgroup sum of NV values (wavefront sums + one LDS slot per wave); red
-// holds at least NV * NT / 64 doubles (Smem::NRED per wave)
+// workgroup sum of NV values: transposed wavefront sums (the row totals sit
+// in lanes 0, 16, 32, 48), one LDS slot per (value, wave), one barrier, and
+// every thread adds the waves' slots.  red holds NV * NT / 64 doubles; the
+// caller alternates between two such buffers, so a buffer is written again
+// only after the barrier of the sum in between, which every thread reaches
+// after its reads of this one.  The barrier also orders any LDS writes made
+// before the call against reads made after it.
 template <int NT, int NV>
 __device__ __forceinline__ void block_sum(double (&v)[NV], double* red) {
-  wave_sum(v);
-  if constexpr (NT > 64) {
-    constexpr int NW = NT / 64;
-    int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    __syncthreads();
-    if (lane == 0) {
+  constexpr int NW = NT / 64;
+  const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  double w[RowSums<NV>::H2];
+  wave_sum_rows(v, w);
+  if ((lane & 15) == 0) {
+    const int rho = lane >> 4;
 #pragma unroll
-      for (int q = 0; q < NV; ++q) red[q * NW + wv] = v[q];
+    for (int q = 0; q < RowSums<NV>::H2; ++q) {
+      const int i = row_sum_index<NV>(q, rho);
+      if (i >= 0) red[i * NW + wv] = w[q];
     }
-    __syncthreads();
+  }
+  __syncthreads();
 #pragma unroll
-    for (int q = 0; q < NV; ++q) {
-      double s = 0.0;
-      for (int i = 0; i < NW; ++i) s += red[q * NW + i];
-      v[q] = s;
-    }
+  for (int q = 0; q < NV; ++q) {
+    double s = 0.0;
+#pragma unroll
+    for (int i = 0; i < NW; ++i) s += red[q * NW + i];
+    v[q] = s;
   }
 }
 
-// block_sum on the workgroup's reduction buffer, checked against its size
+// block_sum on the workgroup's reduction buffers, checked against their size;
+// rs selects the half and flips
 template <int NT, int BMAX, int N, int NV>
-__device__ __forceinline__ void wg_sum(Smem<NT, BMAX, N>& sm, double (&v)[NV]) {
+__device__ __forceinline__ void wg_sum(Smem<NT, BMAX, N>& sm, int& rs, double (&v)[NV]) {
   static_assert(NV <= Smem<NT, BMAX, N>::NRED, "reduction wider than the LDS buffer");
-  block_sum<NT, NV>(v, sm.red);
+  block_sum<NT, NV>(v, sm.red[rs]);
+  rs ^= 1;
 }
 
 // background scalars of A at t + TIME_C[i] * hh, i < cnt (one lane per time)
@@ -492,32 +504,39 @@ __device__ __forceinline__ void put_F(Smem<NT, BMAX, N>& sm, int s, int r, const
   if constexpr (N > 0) { ABCMB_DLOOP(N) sm.Fs[s][1 + i_][r] = x.d[i_]; }
 }
 
-// f_r(t_slot, z): value sum_j cs_j sc_j z_j + u_h h' + u_e eta'; each tangent
-// adds the derivative of the scalars times the value state
+// this row's terms of the metric sums h' and eta' at slot, for the value and
+// each tangent: m[0] = w_h sc z, m[1] = w_e sc z, m[2 + 2i], m[3 + 2i] their
+// derivatives along direction i (linear in z, so an update dz adds the terms
+// of dz)
 template <int NT, int BMAX, int N>
-__device__ __forceinline__ D<N> rhs_eval(Smem<NT, BMAX, N>& sm, const RowState& R, int slot, const D<N>& z) {
-  const int r = threadIdx.x;
-  put_col<NT, BMAX, N>(sm.zl, r, z);
-  // with more than one wave the reduction's first barrier orders these writes
-  if constexpr (NT == 64) __syncthreads();
+__device__ __forceinline__ void metric_terms(const Smem<NT, BMAX, N>& sm, const RowState& R, int slot,
+                                             const D<N>& z, double* m) {
   const double* sc = sm.sc[slot][0];
-  double v[2 * (1 + N)];
-  v[0] = R.w.whs * sc[R.w.whi] * z.v;
-  v[1] = R.w.wes * sc[R.w.wei] * z.v;
+  m[0] = R.w.whs * sc[R.w.whi] * z.v;
+  m[1] = R.w.wes * sc[R.w.wei] * z.v;
   if constexpr (N > 0) {
 #pragma unroll
     for (int i = 0; i < N; ++i) {
       const double* sd = sm.sc[slot][1 + i];
-      v[2 + 2 * i] = R.w.whs * (sd[R.w.whi] * z.v + sc[R.w.whi] * z.d[i]);
-      v[3 + 2 * i] = R.w.wes * (sd[R.w.wei] * z.v + sc[R.w.wei] * z.d[i]);
+      m[2 + 2 * i] = R.w.whs * (sd[R.w.whi] * z.v + sc[R.w.whi] * z.d[i]);
+      m[3 + 2 * i] = R.w.wes * (sd[R.w.wei] * z.v + sc[R.w.wei] * z.d[i]);
     }
   }
-  wg_sum(sm, v);
+}
+
+// f_r(t_slot, z) from z in sm.zl and the workgroup's metric sums ms: value
+// sum_j cs_j sc_j z_j + u_h h' + u_e eta'; each tangent adds the derivative
+// of the scalars times the value state
+template <int NT, int BMAX, int N>
+__device__ __forceinline__ D<N> rhs_gather(const Smem<NT, BMAX, N>& sm, const RowState& R, int slot,
+                                           const double* ms) {
+  const int r = threadIdx.x;
+  const double* sc = sm.sc[slot][0];
   D<N> f;
   double acc = 0.0;
 #pragma unroll
   for (int j = 0; j < 5; ++j) acc += R.w.cs[j] * sc[R.w.si[j]] * sm.zl[0][R.w.col[j]];
-  acc += R.w.uh * v[0] + R.w.ue * v[1];
+  acc += R.w.uh * ms[0] + R.w.ue * ms[1];
   f.v = r < R.n ? acc : 0.0;
   if constexpr (N > 0) {
 #pragma unroll
@@ -527,11 +546,30 @@ __device__ __forceinline__ D<N> rhs_eval(Smem<NT, BMAX, N>& sm, const RowState& 
 #pragma unroll
       for (int j = 0; j < 5; ++j)
         ad += R.w.cs[j] * (sd[R.w.si[j]] * sm.zl[0][R.w.col[j]] + sc[R.w.si[j]] * sm.zl[1 + i][R.w.col[j]]);
-      ad += R.w.uh * v[2 + 2 * i] + R.w.ue * v[3 + 2 * i];
+      ad += R.w.uh * ms[2 + 2 * i] + R.w.ue * ms[3 + 2 * i];
       f.d[i] = r < R.n ? ad : 0.0;
     }
   }
   return f;
+}
+
+// z to sm.zl and the metric sums of z into ms (one workgroup sum); the sum's
+// barrier orders the zl writes before rhs_gather's reads
+template <int NT, int BMAX, int N>
+__device__ __forceinline__ void stage_state(Smem<NT, BMAX, N>& sm, const RowState& R, int& rs, int slot,
+                                            const D<N>& z, double (&ms)[2 * (1 + N)]) {
+  put_col<NT, BMAX, N>(sm.zl, threadIdx.x, z);
+  metric_terms(sm, R, slot, z, ms);
+  wg_sum(sm, rs, ms);
+}
+
+// f_r(t_slot, z) with its own metric sums
+template <int NT, int BMAX, int N>
+__device__ __forceinline__ D<N> rhs_eval(Smem<NT, BMAX, N>& sm, const RowState& R, int& rs, int slot,
+                                         const D<N>& z) {
+  double ms[2 * (1 + N)];
+  stage_state(sm, R, rs, slot, z, ms);
+  return rhs_gather(sm, R, slot, ms);
 }
 
 template <int BMAX>
@@ -544,7 +582,8 @@ struct Inv {
 
 // build and invert S = I - g B(t_n) block-wise, then the Woodbury terms
 template <int NT, int BMAX, int N>
-__device__ bool factor(Smem<NT, BMAX, N>& sm, const RowState& R, int maxb, double g, Inv<BMAX>& F) {
+__device__ bool factor(Smem<NT, BMAX, N>& sm, const RowState& R, int& rs, int maxb, double g,
+                       Inv<BMAX>& F) {
   const double* sc = sm.sc[JAC_SLOT][0];
   const Blk& B = R.B;
 #pragma unroll
@@ -596,7 +635,7 @@ __device__ bool factor(Smem<NT, BMAX, N>& sm, const RowState& R, int maxb, doubl
   F.Vh = R.w.whs * sc[R.w.whi];
   F.Ve = R.w.wes * sc[R.w.wei];
   double v[5] = {F.Vh * F.Yh, F.Vh * F.Ye, F.Ve * F.Yh, F.Ve * F.Ye, bad};
-  wg_sum(sm, v);
+  wg_sum(sm, rs, v);
   double a = 1.0 - v[0], b = -v[1], cc = -v[2], d = 1.0 - v[3];
   double det = a * d - b * cc;
   F.c00 = d / det; F.c01 = -b / det; F.c10 = -cc / det; F.c11 = a / det;
@@ -605,7 +644,8 @@ __device__ bool factor(Smem<NT, BMAX, N>& sm, const RowState& R, int maxb, doubl
 
 // x = (I - g A(t_n))^-1 b, value and tangents with the same factors
 template <int NT, int BMAX, int N>
-__device__ __forceinline__ D<N> solve(Smem<NT, BMAX, N>& sm, const RowState& R, const Inv<BMAX>& F, const D<N>& b) {
+__device__ __forceinline__ D<N> solve(Smem<NT, BMAX, N>& sm, const RowState& R, int& rs, const Inv<BMAX>& F,
+                                      const D<N>& b) {
   constexpr int NC = 1 + N;
   const int r = threadIdx.x;
   put_col<NT, BMAX, N>(sm.rl, r, r < R.n ? b : D<N>(0.0));
@@ -621,7 +661,7 @@ __device__ __forceinline__ D<N> solve(Smem<NT, BMAX, N>& sm, const RowState& R, 
     v[2 * q] = F.Vh * s;
     v[2 * q + 1] = F.Ve * s;
   }
-  wg_sum(sm, v);
+  wg_sum(sm, rs, v);
   double out[NC];
 #pragma unroll
   for (int q = 0; q < NC; ++q) {
@@ -755,14 +795,14 @@ __global__ void bg_save_kernel(const AbcmbCosmo* __restrict__ cosmos, const Abcm
   bgsave[(size_t)ci * n_save_max + i] = b;
 }
 
+// the value-only kernel with 32-row blocks fits three waves per SIMD; the
+// tangent kernel and 64-row blocks need the whole register file of one wave
 template <int NT, int BMAX, int N>
-__global__ void __launch_bounds__(NT) pert_fwd_kernel(const AbcmbCosmo* __restrict__ cosmos,
-                                                      const AbcmbTangent* __restrict__ tangents,
-                                                      const AbcmbPertItem* __restrict__ items,
-                                                      AbcmbOdeSettings st, const Bg<N>* __restrict__ bgsave,
-                                                      int n_save_max, int32_t* __restrict__ status_out,
-                                                      int32_t* __restrict__ steps_out, int32_t* __restrict__ iters_out,
-                                                      double* __restrict__ state_out) {
+__global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu((N == 0 && BMAX <= 32) ? 3 : 1)))
+pert_fwd_kernel(const AbcmbCosmo* __restrict__ cosmos, const AbcmbTangent* __restrict__ tangents,
+                const AbcmbPertItem* __restrict__ items, AbcmbOdeSettings st, const Bg<N>* __restrict__ bgsave,
+                int n_save_max, int32_t* __restrict__ status_out, int32_t* __restrict__ steps_out,
+                int32_t* __restrict__ iters_out, double* __restrict__ state_out) {
   __shared__ Smem<NT, BMAX, N> sm;
   const int r = threadIdx.x;
   const int item = blockIdx.x;
@@ -803,8 +843,9 @@ __global__ void __launch_bounds__(NT) pert_fwd_kernel(const AbcmbCosmo* __restri
   double t = I.lna_start;
   const D<N> ti = fast_interp<N>(t, ABCMB_TAU_TAB_LO, 0.0, c.tau_tab, P.dtau(), ABCMB_TAU_TAB_N);
   D<N> y = r < n ? ic_row(P, L, k, ti, r) : D<N>(0.0);
+  int rs = 0;  // reduction buffer half
   set_times(sm, P, L, k, t, 0.0, 1, st.jac_offset);
-  D<N> f0 = rhs_eval(sm, R, 0, y);
+  D<N> f0 = rhs_eval(sm, R, rs, 0, y);
   double h = st.dt0;
   const double b1 = st.pcoeff + st.icoeff + st.dcoeff, b2 = -(st.pcoeff + 2.0 * st.dcoeff);
   double prev_inv = 1.0;
@@ -838,7 +879,7 @@ __global__ void __launch_bounds__(NT) pert_fwd_kernel(const AbcmbCosmo* __restri
     ++steps;
     const double g = hh * KV_G;
     set_times(sm, P, L, k, t, hh, NTIMES, st.jac_offset);
-    bool ok = factor(sm, R, maxb, g, FI);
+    bool ok = factor(sm, R, rs, maxb, g, FI);
     put_F(sm, 0, r, f0);
     D<N> z = y;
 #pragma unroll 1
@@ -849,6 +890,8 @@ __global__ void __launch_bounds__(NT) pert_fwd_kernel(const AbcmbCosmo* __restri
       const D<N> base = y + hh * acc;
       z = base + g * get_F(sm, s - 1, r);
       const int slot = STAGE_SLOT[s];
+      double ms[2 * (1 + N)];  // metric sums of z, kept up to date by the updates
+      stage_state(sm, R, rs, slot, z, ms);
       bool conv = false;      // value converged
       bool tconv = N == 0;    // tangents converged
       double prev = 0.0;
@@ -856,14 +899,14 @@ __global__ void __launch_bounds__(NT) pert_fwd_kernel(const AbcmbCosmo* __restri
       for (int it = 0;; ++it) {
         if (!conv && it >= st.max_newton) break;
         ++iters;
-        D<N> fz = rhs_eval(sm, R, slot, z);
-        D<N> dz = solve(sm, R, FI, -(z - base - g * fz));
-        if (!conv) {
-          z += dz;
-        } else if constexpr (N > 0) {  // the value is final: iterate the tangents alone
-          ABCMB_DLOOP(N) z.d[i_] += dz.d[i_];
-        }
-        double v[1 + 2 * N];
+        D<N> fz = rhs_gather(sm, R, slot, ms);
+        D<N> dz = solve(sm, R, rs, FI, -(z - base - g * fz));
+        if (conv) dz.v = 0.0;  // the value is final: iterate the tangents alone
+        z += dz;
+        put_col<NT, BMAX, N>(sm.zl, r, z);  // for the next gather, ordered by the sum below
+        // the Newton norms, then the metric-sum increments of dz
+        constexpr int NQ = 1 + 2 * N;
+        double v[NQ + 2 * (1 + N)];
         v[0] = 0.0;
         if (!conv && r < n) { double q = dz.v / (atol + rtol * fabs(z.v)); v[0] = q * q; }
         if constexpr (N > 0) {
@@ -873,7 +916,10 @@ __global__ void __launch_bounds__(NT) pert_fwd_kernel(const AbcmbCosmo* __restri
             v[2 + 2 * i_] = (z.d[i_] * w) * (z.d[i_] * w);
           }
         }
-        wg_sum(sm, v);
+        metric_terms(sm, R, slot, dz, v + NQ);
+        wg_sum(sm, rs, v);
+#pragma unroll
+        for (int q = 0; q < 2 * (1 + N); ++q) ms[q] += v[NQ + q];
         if (!conv) {
           double nrm = sqrt(v[0] / n);
           if (!isfinite(nrm)) break;
@@ -907,7 +953,7 @@ __global__ void __launch_bounds__(NT) pert_fwd_kernel(const AbcmbCosmo* __restri
 #pragma unroll
       for (int j = 0; j < 7; ++j) acc += KV_E[j] * sm.Fs[j][0][r];
       double v[1] = {r < n ? rms_term(hh * acc, y.v, ynew.v, rtol, atol) : 0.0};
-      wg_sum(sm, v);
+      wg_sum(sm, rs, v);
       enorm = sqrt(v[0] / n);
       if (!isfinite(enorm)) enorm = INFINITY;
     }
@@ -918,7 +964,7 @@ __global__ void __launch_bounds__(NT) pert_fwd_kernel(const AbcmbCosmo* __restri
     if (!(fac >= fmin)) fac = fmin;
     if (fac > st.factormax) fac = st.factormax;
     if (accept) {
-      D<N> fnew = rhs_eval(sm, R, 5, ynew);
+      D<N> fnew = rhs_eval(sm, R, rs, 5, ynew);
       while (isave < n_save && lna_save[isave] <= tn) {
         double th = (lna_save[isave] - t) / hh;
         double h00 = 2 * th * th * th - 3 * th * th + 1, h10 = th * th * th - 2 * th * th + th;
@@ -976,6 +1022,12 @@ int dispatch(const AbcmbCosmo* cosmos, const AbcmbTangent* tangents, int n_cosmo
   ABCMB_HIP_CHECK(hipGetLastError());
 #define ABCMB_LAUNCH(NT_, BM_) \
   return launch<NT_, BM_, N>(cosmos, tangents, items, n_items, st, bgs, n_save_max, status, steps, iters, state, s)
+  // 26 rows: the photon block of the default cut-offs (l_max_g 12, l_max_pol_g 10)
+  if (maxb <= 26) {
+    if (st.n <= 64) ABCMB_LAUNCH(64, 26);
+    if (st.n <= 128) ABCMB_LAUNCH(128, 26);
+    ABCMB_LAUNCH(192, 26);
+  }
   if (maxb <= 32) {
     if (st.n <= 64) ABCMB_LAUNCH(64, 32);
     if (st.n <= 128) ABCMB_LAUNCH(128, 32);

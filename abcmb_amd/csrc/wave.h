@@ -43,4 +43,58 @@ __device__ __forceinline__ void wave_sum(double (&v)[NV]) {
   }
 }
 
+// gfx950 half-wave exchanges.  fold32(a, b): lanes 0-31 get a(l) + a(l + 32),
+// lanes 32-63 get b(l - 32) + b(l) (v_permlane32_swap exchanges the upper half
+// of a with the lower half of b).  fold16(a, b) does the same one level down
+// with v_permlane16_swap, which exchanges the odd rows of a with the even rows
+// of b: row 0 gets a's rows 0 + 1, row 1 b's rows 0 + 1, row 2 a's rows 2 + 3,
+// row 3 b's rows 2 + 3 (a row is 16 lanes).
+__device__ __forceinline__ double fold32(double a, double b) {
+  auto lo = __builtin_amdgcn_permlane32_swap(__double2loint(a), __double2loint(b), false, false);
+  auto hi = __builtin_amdgcn_permlane32_swap(__double2hiint(a), __double2hiint(b), false, false);
+  return __hiloint2double(hi[0], lo[0]) + __hiloint2double(hi[1], lo[1]);
+}
+
+__device__ __forceinline__ double fold16(double a, double b) {
+  auto lo = __builtin_amdgcn_permlane16_swap(__double2loint(a), __double2loint(b), false, false);
+  auto hi = __builtin_amdgcn_permlane16_swap(__double2hiint(a), __double2hiint(b), false, false);
+  return __hiloint2double(hi[0], lo[0]) + __hiloint2double(hi[1], lo[1]);
+}
+
+// Transposed wavefront sums of NV values.  Two folds halve the number of
+// registers twice (pairs v[j], v[j + H1], then w1[q], w1[q + H2]), so the
+// lane moves and adds of the first two levels are spent once per pair
+// instead of once per value; the last four levels are DPP moves within
+// rows.  On return every lane of row rho of w[q] holds the wavefront total of
+// v[q + (rho & 1) * H2 + (rho >> 1) * H1] when q + (rho & 1) * H2 < H1 and
+// that index is below NV, and 0 otherwise (row_sum_index).
+template <int NV>
+struct RowSums {
+  static constexpr int H1 = (NV + 1) / 2;
+  static constexpr int H2 = (H1 + 1) / 2;
+};
+
+// index of the value held by row rho of w[q], or -1 for a padding row
+template <int NV>
+__device__ __forceinline__ int row_sum_index(int q, int rho) {
+  constexpr int H1 = RowSums<NV>::H1, H2 = RowSums<NV>::H2;
+  const int j = q + ((rho & 1) ? H2 : 0);
+  const int i = j + ((rho & 2) ? H1 : 0);
+  return (j < H1 && i < NV) ? i : -1;
+}
+
+template <int NV>
+__device__ __forceinline__ void wave_sum_rows(const double (&v)[NV], double (&w)[RowSums<NV>::H2]) {
+  constexpr int H1 = RowSums<NV>::H1, H2 = RowSums<NV>::H2;
+  double w1[H1];
+#pragma unroll
+  for (int j = 0; j < H1; ++j) w1[j] = fold32(v[j], j + H1 < NV ? v[j + H1] : 0.0);
+#pragma unroll
+  for (int q = 0; q < H2; ++q) w[q] = fold16(w1[q], q + H2 < H1 ? w1[q + H2] : 0.0);
+  dpp_step<0xb1, 0xf>(w);   // quad_perm [1,0,3,2]
+  dpp_step<0x4e, 0xf>(w);   // quad_perm [2,3,0,1]
+  dpp_step<0x141, 0xf>(w);  // row_half_mirror
+  dpp_step<0x140, 0xf>(w);  // row_mirror
+}
+
 }  // namespace abcmb

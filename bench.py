@@ -70,6 +70,16 @@ def ode_flops_tangent(model, steps: int, iters: int, n_dir: int = 5) -> float:
     return ode_flops(model, steps, iters) + n_dir * per_dir
 
 
+def kernel_instance(model, n_dir: int) -> str:
+    """The ODE kernel template instance the launch selects for this layout (threads per
+    workgroup by state size, block bound by the largest hierarchy block)."""
+    lay = model.layout
+    maxb = max(lay.i_ur - 2, lay.l_ur + 1, lay.l_ncdm + 1 if lay.n_q else 1)
+    bmax = 26 if maxb <= 26 else 32 if maxb <= 32 else 64
+    nt = 64 if lay.n <= 64 else 128 if lay.n <= 128 else 192
+    return f"pert_fwd_kernel<{nt}, {bmax}, {n_dir}>"
+
+
 def build_model():
     from abcmb_amd.model import Model, MassiveNeutrino
     # max_steps_PE raised from the reference default 2048: with massive neutrinos the
@@ -203,10 +213,10 @@ def main():
     ms = sum(t["ms"] for t in timing) / len(timing)
     if args.gradient == "tangent":
         flops = sum(ode_flops_tangent(model, t["steps"], t["iters"]) for t in timing) / len(timing)
-        kname = "pert_fwd_kernel<128, 32, 5> (tangent-linear; FP64 VALU; peak = MI355X FP64 vector/matrix)"
+        kname = kernel_instance(model, 5) + " (tangent-linear; FP64 VALU; peak = MI355X FP64 vector/matrix)"
     else:
         flops = sum(ode_flops(model, t["steps"], t["iters"]) for t in timing) / len(timing)
-        kname = "pert_fwd_kernel<128, 32, 0> (FP64 VALU; peak = MI355X FP64 vector/matrix)"
+        kname = kernel_instance(model, 0) + " (FP64 VALU; peak = MI355X FP64 vector/matrix)"
     achieved = flops / (ms * 1e-3) / 1e12
     # HBM bytes per launch of the tangent kernel, from the two PMC runs summarised in profiles/
     traffic = None
@@ -217,6 +227,7 @@ def main():
     roof = {"bound": "mfma", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
             "frac": achieved / FP64_PEAK_TFLOPS, "traffic": traffic, "kernel": kname,
             "launch_ms": ms, "flops_per_launch": flops,
+            "chord_iters_per_step": sum(t["iters"] for t in timing) / max(1, sum(t["steps"] for t in timing)),
             "traffic_source": "profiles/pert_tangent_traffic.json (FETCH_SIZE x2.0 + WRITE_SIZE x1.0, "
                               "scaled by a 1 GiB copy)" if traffic else None}
 
