@@ -414,8 +414,13 @@ struct Smem {
   // metric-sum increments
   static constexpr int NRED = (3 + 4 * N > 8 ? 3 + 4 * N : 8);
   double sc[NTIMES][NC][SC_COUNT];
-  double zl[NC][NT + BMAX];
-  double rl[NC][NT + BMAX];
+  // z by component, rows padded to a multiple of 64 so that a component's
+  // column sits 512-byte strides from the next (paired reads, no address math)
+  static constexpr int LDZ = (NT + BMAX + 63) / 64 * 64;
+  double zl[NC][LDZ];
+  // right-hand sides of the block solves by row, the components of a row
+  // adjacent: one base address serves every (column, component) of a block
+  alignas(16) double rl[(NT + BMAX) * NC];
   double uh[NT + BMAX];
   double ue[NT + BMAX];
   double pbuf[2][NBLK][BMAX];
@@ -484,10 +489,18 @@ struct RowState {
   int n;
 };
 
+// x into column r of sm.zl
 template <int NT, int BMAX, int N>
-__device__ __forceinline__ void put_col(double (*buf)[NT + BMAX], int r, const D<N>& x) {
-  buf[0][r] = x.v;
-  if constexpr (N > 0) { ABCMB_DLOOP(N) buf[1 + i_][r] = x.d[i_]; }
+__device__ __forceinline__ void put_z(Smem<NT, BMAX, N>& sm, int r, const D<N>& x) {
+  sm.zl[0][r] = x.v;
+  if constexpr (N > 0) { ABCMB_DLOOP(N) sm.zl[1 + i_][r] = x.d[i_]; }
+}
+
+// x into row r of a row-major buffer of 1 + N components per row
+template <int N>
+__device__ __forceinline__ void put_row(double* buf, int r, const D<N>& x) {
+  buf[r * (1 + N)] = x.v;
+  if constexpr (N > 0) { ABCMB_DLOOP(N) buf[r * (1 + N) + 1 + i_] = x.d[i_]; }
 }
 
 template <int NT, int BMAX, int N>
@@ -558,7 +571,7 @@ __device__ __forceinline__ D<N> rhs_gather(const Smem<NT, BMAX, N>& sm, const Ro
 template <int NT, int BMAX, int N>
 __device__ __forceinline__ void stage_state(Smem<NT, BMAX, N>& sm, const RowState& R, int& rs, int slot,
                                             const D<N>& z, double (&ms)[2 * (1 + N)]) {
-  put_col<NT, BMAX, N>(sm.zl, threadIdx.x, z);
+  put_z(sm, threadIdx.x, z);
   metric_terms(sm, R, slot, z, ms);
   wg_sum(sm, rs, ms);
 }
@@ -648,7 +661,7 @@ __device__ __forceinline__ D<N> solve(Smem<NT, BMAX, N>& sm, const RowState& R, 
                                       const D<N>& b) {
   constexpr int NC = 1 + N;
   const int r = threadIdx.x;
-  put_col<NT, BMAX, N>(sm.rl, r, r < R.n ? b : D<N>(0.0));
+  put_row<N>(sm.rl, r, r < R.n ? b : D<N>(0.0));
   __syncthreads();
   double z[NC];
   double v[2 * NC];
@@ -656,7 +669,7 @@ __device__ __forceinline__ D<N> solve(Smem<NT, BMAX, N>& sm, const RowState& R, 
   for (int q = 0; q < NC; ++q) {
     double s = 0.0;
 #pragma unroll
-    for (int c = 0; c < BMAX; ++c) s += F.row[c] * sm.rl[q][R.B.start + c];
+    for (int c = 0; c < BMAX; ++c) s += F.row[c] * sm.rl[(R.B.start + c) * NC + q];
     z[q] = s;
     v[2 * q] = F.Vh * s;
     v[2 * q + 1] = F.Ve * s;
@@ -824,8 +837,8 @@ pert_fwd_kernel(const AbcmbCosmo* __restrict__ cosmos, const AbcmbTangent* __res
   for (int j = 0; j < 5; ++j) R.lc[j] = R.w.col[j] - R.B.start;
   const int maxb = max(L.iu - 2, max(L.lu + 1, L.nq > 0 ? L.ln + 1 : 1));
   for (int q = 0; q < 1 + N; ++q) {
-    sm.zl[q][r] = 0.0; sm.rl[q][r] = 0.0;
-    if (r < BMAX) { sm.zl[q][NT + r] = 0.0; sm.rl[q][NT + r] = 0.0; }
+    sm.zl[q][r] = 0.0; sm.rl[r * (1 + N) + q] = 0.0;
+    if (r < BMAX) { sm.zl[q][NT + r] = 0.0; sm.rl[(NT + r) * (1 + N) + q] = 0.0; }
   }
   sm.uh[r] = r < n ? R.w.uh : 0.0;
   sm.ue[r] = r < n ? R.w.ue : 0.0;
@@ -854,11 +867,11 @@ pert_fwd_kernel(const AbcmbCosmo* __restrict__ cosmos, const AbcmbTangent* __res
 
   auto save_point = [&](const D<N>& ys) {
     __syncthreads();
-    put_col<NT, BMAX, N>(sm.zl, r, ys);
+    put_z(sm, r, ys);
     __syncthreads();
     if (state_out && r < n) state_out[((size_t)item * n_save + isave) * n + r] = ys.v;
     if (r < ABCMB_N_PT_FIELDS) {
-      D<N> v = output_field<N, NT + BMAX>(P, L, k, bgs[isave], sm.zl, r);
+      D<N> v = output_field<N, Smem<NT, BMAX, N>::LDZ>(P, L, k, bgs[isave], sm.zl, r);
       const size_t o = ((size_t)r * n_save + isave) * c.nk + I.ik;
       c.pt[o] = v.v;
       if constexpr (N > 0) {
@@ -903,7 +916,7 @@ pert_fwd_kernel(const AbcmbCosmo* __restrict__ cosmos, const AbcmbTangent* __res
         D<N> dz = solve(sm, R, rs, FI, -(z - base - g * fz));
         if (conv) dz.v = 0.0;  // the value is final: iterate the tangents alone
         z += dz;
-        put_col<NT, BMAX, N>(sm.zl, r, z);  // for the next gather, ordered by the sum below
+        put_z(sm, r, z);  // for the next gather, ordered by the sum below
         // the Newton norms, then the metric-sum increments of dz
         constexpr int NQ = 1 + 2 * N;
         double v[NQ + 2 * (1 + N)];
