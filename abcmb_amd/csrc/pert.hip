@@ -845,6 +845,7 @@ pert_fwd_kernel(const AbcmbCosmo* __restrict__ cosmos, const AbcmbTangent* __res
   if (r < BMAX) { sm.uh[NT + r] = 0.0; sm.ue[NT + r] = 0.0; }
 
   const double rtol = I.rtol, atol = I.atol;
+  const double inv_n = 1.0 / n;
   const int n_save = c.n_save;
   const double* __restrict__ lna_save = c.lna_save;
   if (n_save > n_save_max) {  // the background table has no room for this cosmology's saves
@@ -891,6 +892,7 @@ pert_fwd_kernel(const AbcmbCosmo* __restrict__ cosmos, const AbcmbTangent* __res
     if (!(hh > 0.0)) { status = ABCMB_K_STEP_UNDERFLOW; break; }
     ++steps;
     const double g = hh * KV_G;
+    const double inv_g = 1.0 / g;
     set_times(sm, P, L, k, t, hh, NTIMES, st.jac_offset);
     bool ok = factor(sm, R, rs, maxb, g, FI);
     put_F(sm, 0, r, f0);
@@ -920,10 +922,11 @@ pert_fwd_kernel(const AbcmbCosmo* __restrict__ cosmos, const AbcmbTangent* __res
         // the Newton norms, then the metric-sum increments of dz
         constexpr int NQ = 1 + 2 * N;
         double v[NQ + 2 * (1 + N)];
-        v[0] = 0.0;
-        if (!conv && r < n) { double q = dz.v / (atol + rtol * fabs(z.v)); v[0] = q * q; }
+        // one reciprocal per row; the decisions below compare products, so the
+        // iteration spends two divisions (this one and the rate)
+        const double w = r < n ? 1.0 / (atol + rtol * fabs(z.v)) : 0.0;
+        v[0] = conv ? 0.0 : (dz.v * w) * (dz.v * w);
         if constexpr (N > 0) {
-          const double w = r < n ? 1.0 / (atol + rtol * fabs(z.v)) : 0.0;
           ABCMB_DLOOP(N) {
             v[1 + 2 * i_] = (dz.d[i_] * w) * (dz.d[i_] * w);
             v[2 + 2 * i_] = (z.d[i_] * w) * (z.d[i_] * w);
@@ -934,7 +937,7 @@ pert_fwd_kernel(const AbcmbCosmo* __restrict__ cosmos, const AbcmbTangent* __res
 #pragma unroll
         for (int q = 0; q < 2 * (1 + N); ++q) ms[q] += v[NQ + q];
         if (!conv) {
-          double nrm = sqrt(v[0] / n);
+          double nrm = sqrt(v[0] * inv_n);
           if (!isfinite(nrm)) break;
           if (nrm < NEWTON_SMALL) {
             conv = true;
@@ -942,22 +945,23 @@ pert_fwd_kernel(const AbcmbCosmo* __restrict__ cosmos, const AbcmbTangent* __res
             double th = nrm / prev;
             if (th > 2.0) break;
             rate = th;
-            double fac = nrm * th / (1.0 - th);
-            if (fac > 0.0 && fac < st.newton_kappa) conv = true;
+            // nrm th / (1 - th) in (0, kappa), with nrm, th > 0
+            if (th < 1.0 && nrm * th < st.newton_kappa * (1.0 - th)) conv = true;
           }
           prev = nrm;
         }
         if constexpr (N > 0) {
-          // remaining error ~ update * rate / (1 - rate) when the iteration contracts
-          const double amp = rate < 1.0 ? rate / (1.0 - rate) : 1.0;
-          const double lim = (TANGENT_RTOL / amp) * (TANGENT_RTOL / amp);
+          // remaining error ~ update * amp, amp = rate / (1 - rate) when the
+          // iteration contracts (else 1): converged when |dz| amp <= TANGENT_RTOL |z|
+          const double an = rate < 1.0 ? rate : 1.0, ad = rate < 1.0 ? 1.0 - rate : 1.0;
+          const double lhs = an * an, rhs = (TANGENT_RTOL * ad) * (TANGENT_RTOL * ad);
           tconv = true;
-          ABCMB_DLOOP(N) tconv = tconv && v[1 + 2 * i_] <= lim * v[2 + 2 * i_];
+          ABCMB_DLOOP(N) tconv = tconv && v[1 + 2 * i_] * lhs <= rhs * v[2 + 2 * i_];
         }
         if (conv && (tconv || it + 1 >= st.max_newton + TANGENT_EXTRA)) break;
       }
       if (!conv) { ok = false; break; }
-      put_F(sm, s, r, (z - base) / g);
+      put_F(sm, s, r, (z - base) * inv_g);
     }
     double enorm = INFINITY;
     const D<N> ynew = z;

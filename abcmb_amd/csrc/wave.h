@@ -13,6 +13,24 @@ __device__ __forceinline__ double dpp_get(double v) {
   return __hiloint2double(hi, lo);
 }
 
+// v from another lane of the same 16-lane row, for controls under which every
+// lane has a source (quad permutations, row mirrors): no old value to keep
+template <int CTRL>
+__device__ __forceinline__ double dpp_row_get(double v) {
+  int lo = __builtin_amdgcn_mov_dpp(__double2loint(v), CTRL, 0xf, 0xf, false);
+  int hi = __builtin_amdgcn_mov_dpp(__double2hiint(v), CTRL, 0xf, 0xf, false);
+  return __hiloint2double(hi, lo);
+}
+
+template <int CTRL, int NV>
+__device__ __forceinline__ void dpp_row_step(double (&v)[NV]) {
+  double t[NV];
+#pragma unroll
+  for (int q = 0; q < NV; ++q) t[q] = dpp_row_get<CTRL>(v[q]);
+#pragma unroll
+  for (int q = 0; q < NV; ++q) v[q] += t[q];
+}
+
 template <int CTRL, int ROWMASK, int NV>
 __device__ __forceinline__ void dpp_step(double (&v)[NV]) {
   double t[NV];
@@ -91,10 +109,10 @@ __device__ __forceinline__ void wave_sum_rows(const double (&v)[NV], double (&w)
   for (int j = 0; j < H1; ++j) w1[j] = fold32(v[j], j + H1 < NV ? v[j + H1] : 0.0);
 #pragma unroll
   for (int q = 0; q < H2; ++q) w[q] = fold16(w1[q], q + H2 < H1 ? w1[q + H2] : 0.0);
-  dpp_step<0xb1, 0xf>(w);   // quad_perm [1,0,3,2]
-  dpp_step<0x4e, 0xf>(w);   // quad_perm [2,3,0,1]
-  dpp_step<0x141, 0xf>(w);  // row_half_mirror
-  dpp_step<0x140, 0xf>(w);  // row_mirror
+  dpp_row_step<0xb1>(w);   // quad_perm [1,0,3,2]
+  dpp_row_step<0x4e>(w);   // quad_perm [2,3,0,1]
+  dpp_row_step<0x141>(w);  // row_half_mirror
+  dpp_row_step<0x140>(w);  // row_mirror
 }
 
 }  // namespace abcmb

@@ -20,10 +20,13 @@ Gradients:
     gives the perturbation tables and their exact derivatives along the five
     parameters that reach the perturbation equations (omega_b, omega_cdm, h,
     tau_reio, m_nu); the host inputs' derivatives come from central
-    differences of the (cheap, deterministic) host producer.  The
-    downstream (sources, LOS, lensing) is then differenced along each
-    direction with the tables moved along their tangents, and ln A_s, n_s
-    (which enter only the primordial spectrum) are differenced there too.
+    differences of the (cheap, deterministic) host producer.  By default the
+    LOS is then differentiated in one tangent-linear pass (the pointwise
+    derivative, what automatic differentiation of the reference gives) and
+    the lensing is differenced along each direction; ``downstream="fd"``
+    differences the whole downstream (sources, LOS, lensing) with the tables
+    moved along their tangents instead, which also picks up the jumps where
+    the LOS switches from the Bessel tables to their asymptotic form.
 """
 from __future__ import annotations
 
@@ -243,18 +246,19 @@ def _directional_losses_linear(model, like, thetas, steps, base_plans, pm, outs,
 
 
 def tangent_gradients(model, like, thetas, k_grids=None, mock_index=None, phases=None, chunk: int = 64,
-                      downstream: str = "fd"):
+                      downstream: str = "tangent"):
     """d L / d theta for each row of ``thetas`` from tangent-linear ODE launches.
 
     The cosmologies go through in chunks of ``chunk`` (one ODE launch each,
     the first a quarter of that); a host thread prepares the next chunks'
     background tables and tangent
-    seeds while the GPU works on the current one.  ``downstream``: "fd"
-    differences the whole downstream (14 evaluations per cosmology, the same
-    relative step as the plain finite-difference path); "tangent"
-    differentiates the LOS in one tangent-linear pass (the Bessel lookups as
-    interpolated segments, as automatic differentiation of the reference
-    would) and differences only the lensing.  ``phases`` (optional
+    seeds while the GPU works on the current one.  ``downstream``: "tangent"
+    (default) differentiates the LOS in one tangent-linear pass (the Bessel
+    lookups as interpolated segments and asymptotic forms, as automatic
+    differentiation of the reference would) and differences only the
+    lensing; "fd" differences the whole downstream (14 evaluations per
+    cosmology, the same relative step as the plain finite-difference path,
+    table/asymptotic jumps of the Bessel lookups included).  ``phases`` (optional
     dict) receives the host time not hidden behind GPU work, the time of the
     tangent-linear ODE launches with their base downstream, and of the
     directional downstream.

@@ -149,8 +149,9 @@ def main():
     ap.add_argument("--chunk", type=int, default=64, help="cosmologies per tangent-linear ODE launch")
     ap.add_argument("--gradient", choices=("tangent", "fd"), default="tangent",
                     help="batched gradient method (the plain path is always finite differences)")
-    ap.add_argument("--los-tangent", action="store_true",
-                    help="differentiate the LOS in one tangent-linear pass instead of differencing the downstream")
+    ap.add_argument("--fd-downstream", action="store_true",
+                    help="difference the whole downstream along each direction instead of differentiating the "
+                         "LOS in one tangent-linear pass")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="only run a device-to-device copy of BYTES (5 times): the reference for "
                          "scaling HBM byte counters")
@@ -178,7 +179,7 @@ def main():
     phases = {}
     if args.gradient == "tangent":
         batched = lambda: LK.tangent_gradients(model, like, thetas, k_grids=k_grids, phases=phases, chunk=args.chunk,
-                                               downstream="tangent" if args.los_tangent else "fd")
+                                               downstream="fd" if args.fd_downstream else "tangent")
     else:
         batched = lambda: LK.fd_gradients(model, like, thetas, k_grids=k_grids, batched=True)
     for w in range(args.warmup):
@@ -257,8 +258,8 @@ def main():
         "config": {"workload": f"batched lensed-Cl likelihood gradients, {args.batch} Latin-hypercube cosmologies "
                                f"(massive nu, uneven 300-800 k-grids, l<=2500, FP64 ODE + FP64 LOS)",
                    "gradient": (f"tangent-linear ODE (5 directions, one launch per {args.chunk} cosmologies) + "
-                                + ("tangent-linear LOS, lensing differenced along 7 directions" if args.los_tangent
-                                   else "downstream differenced along 7 directions")
+                                + ("downstream differenced along 7 directions" if args.fd_downstream
+                                   else "tangent-linear LOS, lensing differenced along 7 directions")
                                 if args.gradient == "tangent" else
                                 "central finite differences, 15 evaluations, grids frozen, all in one ODE launch"),
                    "plain_path": "one cosmology per launch, FP64, central finite differences (15 evaluations)",

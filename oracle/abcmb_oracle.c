@@ -1004,22 +1004,33 @@ static double j_asym(double l, double x) {
   double J = sqrt(2 / PI / sqrt(x * x - nu * nu)) * cos(Q - PI / 4);
   return sqrt(PI / 2 / x) * J;
 }
-static double bessel_phi(int which, double l, const double* tab, double xmin, double xmax, int nx, double x) {
-  if (x < xmin) return 0.0;
-  if (x >= xmax) {
+/* xb picks the branch (zero below the table, the table, the asymptotic form
+ * above it) and x is where the branch is evaluated; the reference uses xb = x.
+ * A branch frozen at other points (xb from a base cosmology) takes the jumps
+ * between table and asymptotic form out of finite differences, which then
+ * converge to the pointwise derivative (what autodiff of the reference gives). */
+static double bessel_phi_b(int which, double l, const double* tab, double xmin, double xmax, int nx, double x,
+                           double xb) {
+  if (xb < xmin) return 0.0;
+  if (xb >= xmax) {
     if (which == 0) return j_asym(l, x);
     if (which == 1) return l / x * j_asym(l, x) - j_asym(l + 1, x);
     return ((3 * l * (l - 1) - 2 * x * x) * j_asym(l, x) + 6 * x * j_asym(l + 1, x)) / 2 / (x * x);
   }
   return fast_interp(x, xmin, xmax, tab, nx);
 }
+static double bessel_phi(int which, double l, const double* tab, double xmin, double xmax, int nx, double x) {
+  return bessel_phi_b(which, l, tab, xmin, xmax, nx, x, x);
+}
 
 /* LOS + C_l (abcmb/spectrum.py:697-797).  src: [4][n_tau][nkt]; bes: [3][n_lt][nx];
- * rng: [3][n_lt][2]; cl out: [3][n_ell]; delta out (optional) [4][n_ell][nkt] */
-int oracle_los_cl(const double* src, int n_tau, const double* tau_los, double tau0, const double* kt, int nkt,
-                  const double* bes, const double* rng, const int32_t* bes_l, int n_lt, int nx,
-                  const int32_t* ell_idx, int n_ell, double A_s, double n_s, double k_pivot, double* cl,
-                  double* delta) {
+ * rng: [3][n_lt][2]; cl out: [3][n_ell]; delta out (optional) [4][n_ell][nkt];
+ * chi_branch (optional, [n_tau]): distances that pick the Bessel branches
+ * (bessel_phi_b), NULL for the reference's own choice */
+int oracle_los_cl_branch(const double* src, int n_tau, const double* tau_los, double tau0, const double* kt,
+                         int nkt, const double* bes, const double* rng, const int32_t* bes_l, int n_lt, int nx,
+                         const int32_t* ell_idx, int n_ell, double A_s, double n_s, double k_pivot, double* cl,
+                         double* delta, const double* chi_branch) {
 #pragma omp parallel for schedule(dynamic, 1)
   for (int e = 0; e < n_ell; ++e) {
     int li = ell_idx[e];
@@ -1028,11 +1039,12 @@ int oracle_los_cl(const double* src, int n_tau, const double* tau_los, double ta
     double* D = (double*)calloc(4 * (size_t)nkt, sizeof(double));
     for (int s = 0; s < n_tau; ++s) {
       double chi0 = tau0 - tau_los[s];
+      double chib = chi_branch ? chi_branch[s] : chi0;
       for (int j = 0; j < nkt; ++j) {
-        double x = chi0 * kt[j];
-        double p0 = bessel_phi(0, l, bes + ((size_t)0 * n_lt + li) * nx, rng[(0 * n_lt + li) * 2], rng[(0 * n_lt + li) * 2 + 1], nx, x);
-        double p1 = bessel_phi(1, l, bes + ((size_t)1 * n_lt + li) * nx, rng[(1 * n_lt + li) * 2], rng[(1 * n_lt + li) * 2 + 1], nx, x);
-        double p2 = bessel_phi(2, l, bes + ((size_t)2 * n_lt + li) * nx, rng[(2 * n_lt + li) * 2], rng[(2 * n_lt + li) * 2 + 1], nx, x);
+        double x = chi0 * kt[j], xb = chib * kt[j];
+        double p0 = bessel_phi_b(0, l, bes + ((size_t)0 * n_lt + li) * nx, rng[(0 * n_lt + li) * 2], rng[(0 * n_lt + li) * 2 + 1], nx, x, xb);
+        double p1 = bessel_phi_b(1, l, bes + ((size_t)1 * n_lt + li) * nx, rng[(1 * n_lt + li) * 2], rng[(1 * n_lt + li) * 2 + 1], nx, x, xb);
+        double p2 = bessel_phi_b(2, l, bes + ((size_t)2 * n_lt + li) * nx, rng[(2 * n_lt + li) * 2], rng[(2 * n_lt + li) * 2 + 1], nx, x, xb);
         double ep = p0 / (x * x) * eps_f;
         size_t o = (size_t)s * nkt + j;
         D[0 * nkt + j] += src[0 * (size_t)n_tau * nkt + o] * p0;
@@ -1067,6 +1079,14 @@ int oracle_los_cl(const double* src, int n_tau, const double* tau_los, double ta
     free(D);
   }
   return 0;
+}
+
+int oracle_los_cl(const double* src, int n_tau, const double* tau_los, double tau0, const double* kt, int nkt,
+                  const double* bes, const double* rng, const int32_t* bes_l, int n_lt, int nx,
+                  const int32_t* ell_idx, int n_ell, double A_s, double n_s, double k_pivot, double* cl,
+                  double* delta) {
+  return oracle_los_cl_branch(src, n_tau, tau_los, tau0, kt, nkt, bes, rng, bes_l, n_lt, nx, ell_idx, n_ell, A_s,
+                              n_s, k_pivot, cl, delta, NULL);
 }
 
 /* ---------- P(k) (abcmb/spectrum.py:250-302) ---------- */

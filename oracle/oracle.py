@@ -43,6 +43,8 @@ def lib():
         L.oracle_spline.argtypes = [_P, _P, _I, _P, _I, _P]
         L.oracle_sources.argtypes = [_P, _P, _I, _I, _P, _P, _P, _I, _P, _P]
         L.oracle_los_cl.argtypes = [_P, _I, _P, _D, _P, _I, _P, _P, _P, _I, _I, _P, _I, _D, _D, _D, _P, _P]
+        L.oracle_los_cl_branch.argtypes = [_P, _I, _P, _D, _P, _I, _P, _P, _P, _I, _I, _P, _I, _D, _D, _D, _P, _P,
+                                           _P]
         L.oracle_pk.argtypes = [_P, _P, _I, _I, _P, _P, _I, _D, _D, _D, _P]
         L.oracle_clpp.argtypes = [C.POINTER(abi.AbcmbCosmo), _P, _P, _P, _P, _I, _D, _P, _I, _P]
         L.oracle_wigner.argtypes = [_I, _I, _P, _I, _I, _P]
@@ -156,6 +158,20 @@ def los_cl(src, tau_los, tau0, kt, bes, rng, bes_l, ell_idx, A_s, n_s, k_pivot, 
     lib().oracle_los_cl(_p(src), src.shape[1], _p(tau_los), tau0, _p(kt), kt.size, _p(bes), _p(rng), _p(bes_l),
                         bes.shape[1], bes.shape[2], _p(ell_idx), n_ell, A_s, n_s, k_pivot, _p(cl), _p(delta))
     return (cl, delta) if want_delta else cl
+
+
+def los_cl_branch(src, tau_los, tau0, kt, bes, rng, bes_l, ell_idx, A_s, n_s, k_pivot, chi_branch):
+    """los_cl with the Bessel branches (zero / table / asymptotic form) picked at
+    chi_branch[s] * k instead of (tau0 - tau_los[s]) * k."""
+    src = _c(src); tau_los = _c(tau_los); kt = _c(kt); bes = _c(bes); rng = _c(rng)
+    bes_l = _c(bes_l, np.int32); ell_idx = _c(ell_idx, np.int32); chi_branch = _c(chi_branch)
+    assert chi_branch.size == src.shape[1]
+    n_ell = ell_idx.size
+    cl = np.zeros((3, n_ell))
+    lib().oracle_los_cl_branch(_p(src), src.shape[1], _p(tau_los), tau0, _p(kt), kt.size, _p(bes), _p(rng),
+                               _p(bes_l), bes.shape[1], bes.shape[2], _p(ell_idx), n_ell, A_s, n_s, k_pivot, _p(cl),
+                               None, _p(chi_branch))
+    return cl
 
 
 def pk(pt, k, n_save, lna_save, kq, A_s, n_s, k_pivot):

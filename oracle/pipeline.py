@@ -40,8 +40,13 @@ def run(model, plan, want_states=False, lensing=None, k_subset=None):
     return out
 
 
-def downstream(model, plan, pt, lensing=None):
-    """Sources, LOS C_l, l-spline, lensing and P(k) of one cosmology from its table ``pt``."""
+def downstream(model, plan, pt, lensing=None, chi_branch=None, cl_full=None):
+    """Sources, LOS C_l, l-spline, lensing and P(k) of one cosmology from its table ``pt``.
+
+    chi_branch: LOS distances that pick the Bessel branches (``oracle.los_cl_branch``)
+    instead of the cosmology's own; cl_full: unlensed C_l on every l to lens (with
+    C_l^phiphi of ``plan`` / ``pt``) instead of the LOS result.
+    """
     kk = np.ascontiguousarray(plan.k if plan.k is not None else model.k_pt)
     ns = model.n_save
     pt = np.ascontiguousarray(pt)
@@ -50,11 +55,15 @@ def downstream(model, plan, pt, lensing=None):
     lna = np.ascontiguousarray(plan.lna_save)
     c.lna_save, c.k, c.pt = lna.ctypes.data, kk.ctypes.data, pt.ctypes.data
     c.n_save, c.nk = ns, kk.size
-    src = O.sources(pt, kk, ns, plan.bgv, plan.lna_save, model.k_transfer, model.scales)
-    cl_raw = O.los_cl(src, plan.bgv[0], plan.bg.tau0, model.k_transfer, model.bes_tab, model.bes_rng, model.bes_l,
-                      model.ell_idx, c.A_s, c.n_s, model.specs["k_pivot"])
-    cl_full = cl_raw @ model.ell_op.T
-    out = dict(src=src, cl_raw=cl_raw, cl_full=cl_full)
+    if cl_full is None:
+        src = O.sources(pt, kk, ns, plan.bgv, plan.lna_save, model.k_transfer, model.scales)
+        los_args = (src, plan.bgv[0], plan.bg.tau0, model.k_transfer, model.bes_tab, model.bes_rng, model.bes_l,
+                    model.ell_idx, c.A_s, c.n_s, model.specs["k_pivot"])
+        cl_raw = O.los_cl(*los_args) if chi_branch is None else O.los_cl_branch(*los_args, chi_branch)
+        cl_full = cl_raw @ model.ell_op.T
+        out = dict(src=src, cl_raw=cl_raw, cl_full=cl_full)
+    else:
+        out = dict(cl_full=cl_full)
     do_lens = model.specs["lensing"] if lensing is None else lensing
     if do_lens:
         clpp = O.clpp(c, pt, plan.lens_lna, plan.lens_aH, plan.lens_tau, plan.chi_rec,

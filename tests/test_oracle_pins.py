@@ -160,3 +160,35 @@ def test_lensing_generates_positive_bb():
     out = O.lens(np.stack([tt, te, ee]), clpp, mu, w)
     assert np.all(out[3][10:300] > 0)
     assert np.max(np.abs(out[0] / tt - 1)[:300]) < 0.2
+
+
+def test_branch_frozen_los():
+    """``oracle.los_cl_branch``: with the branch distances equal to the actual ones
+    it is ``los_cl`` bit for bit; with them frozen at a base tau0, central
+    differences in tau0 converge: against a step of 1e-5 Mpc, steps of 1e-2 and
+    1e-4 Mpc differ by 2.4e-5 and 2.4e-6 of the derivative's maximum per
+    spectrum (first order in the step: the table is interpolated linearly, so
+    the differences see the slope changes between segments), the property the
+    tangent-LOS check on the GPU relies on."""
+    from abcmb_amd import bessel
+    ls, tab, rng = bessel.load_tables()
+    kt = np.geomspace(1e-4, 0.5, 600)
+    tau_los = np.linspace(250.0, 13500.0, 60)
+    tau0 = 14000.0
+    g = np.random.default_rng(3)
+    shape_t = np.exp(-((tau_los - 300.0) / 60.0) ** 2) + 0.05 * np.exp(-((tau_los - 9000.0) / 3000.0) ** 2)
+    src = np.stack([g.uniform(0.5, 1.5) * shape_t[:, None] * np.exp(-(kt[None, :] / 0.2) ** 2)
+                    * (1.0 + 0.1 * q * np.log(kt[None, :])) for q in range(4)])
+    ell_idx = np.arange(0, ls.size, 8, dtype=np.int32)
+    args = (kt, tab, rng, ls, ell_idx, 2e-9, 0.96, 0.05)
+    chi = tau0 - tau_los
+    assert np.array_equal(O.los_cl_branch(src, tau_los, tau0, *args, chi), O.los_cl(src, tau_los, tau0, *args))
+
+    def d(h):
+        f = lambda t0: O.los_cl_branch(src, tau_los, t0, *args, chi)
+        return (f(tau0 + h) - f(tau0 - h)) / (2 * h)
+
+    ref = d(1e-5)
+    scale = np.abs(ref).max(axis=1, keepdims=True)
+    assert np.max(np.abs(d(1e-2) - ref) / scale) < 5e-5
+    assert np.max(np.abs(d(1e-4) - ref) / scale) < 5e-6
