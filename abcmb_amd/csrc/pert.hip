@@ -226,7 +226,7 @@ struct Layout {
   int ig, ip, iu, inc;
 };
 
-__device__ __noinline__ void row_desc(const Layout& L, double k, int r, Row& w) {
+__device__ __forceinline__ void row_desc(const Layout& L, double k, int r, Row& w) {
   for (int j = 0; j < 5; ++j) { w.col[j] = r < L.n ? r : 0; w.cs[j] = 0.0; w.si[j] = SC_ZERO; }
   w.uh = w.ue = 0.0;
   w.whi = w.wei = SC_ZERO;
@@ -311,6 +311,13 @@ __device__ __noinline__ void row_desc(const Layout& L, double k, int r, Row& w) 
   }
 }
 
+// row_desc out of line: the value-only kernel keeps the descriptor in a stack
+// object (three waves per SIMD leave no room for it in registers); the tangent
+// kernel inlines it, so the register allocator keeps the descriptor in
+// registers or spills it to AGPRs instead of reloading it from scratch in the
+// chord loop
+__device__ __noinline__ void row_desc_call(const Layout& L, double k, int r, Row& w) { row_desc(L, k, r, w); }
+
 
 // scalars of A(ln a) shared by all rows: sc[c][slot] holds the value (c = 0)
 // and the derivative along direction c - 1
@@ -377,7 +384,8 @@ constexpr double NEWTON_SMALL = 1e-9;
 // fixed) until each direction's estimated remaining error -- its weighted
 // update times rate / (1 - rate), with the value's contraction rate, since
 // value and tangents run the same linear iteration -- is below TANGENT_RTOL
-// of its weighted size.  They then solve the differentiated stage equations
+// of its weighted size (the size of the stage predictor's tangent, summed
+// once per stage).  They then solve the differentiated stage equations
 // rather than the derivative of a truncated chord iteration.  At most
 // TANGENT_EXTRA iterations beyond max_newton.
 constexpr double TANGENT_RTOL = 1e-6;
@@ -410,9 +418,9 @@ __device__ Blk row_block(const Layout& L, int r) {
 template <int NT, int BMAX, int N>
 struct Smem {
   static constexpr int NC = 1 + N;                        // value + tangents
-  // widest workgroup sum: the Newton test's 1 + 2N norms plus the 2(1 + N)
-  // metric-sum increments
-  static constexpr int NRED = (3 + 4 * N > 8 ? 3 + 4 * N : 8);
+  // widest workgroup sum: the Newton test's 1 + N norms plus the 2(1 + N)
+  // metric-sum increments, or a stage's 2(1 + N) metric sums plus N norms
+  static constexpr int NRED = (3 + 3 * N > 8 ? 3 + 3 * N : 8);
   double sc[NTIMES][NC][SC_COUNT];
   // z by component, rows padded to a multiple of 64 so that a component's
   // column sits 512-byte strides from the next (paired reads, no address math)
@@ -593,19 +601,48 @@ struct Inv {
   double Vh, Ve;      // this row's weights in h', eta' at t_n
 };
 
-// build and invert S = I - g B(t_n) block-wise, then the Woodbury terms
+// orders LDS writes before reads among the lanes of one wavefront: a
+// wavefront's LDS operations are performed in issue order, so only the
+// compiler has to be kept from moving them across this point
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// build and invert S = I - g B(t_n) block-wise, then the Woodbury terms.
+// wl: every hierarchy block lies inside one wavefront, so a pivot row is only
+// read by lanes of its own wavefront and the pivot steps need no workgroup
+// barrier
 template <int NT, int BMAX, int N>
-__device__ bool factor(Smem<NT, BMAX, N>& sm, const RowState& R, int& rs, int maxb, double g,
+__device__ bool factor(Smem<NT, BMAX, N>& sm, const RowState& R, int& rs, int maxb, bool wl, double g,
                        Inv<BMAX>& F) {
   const double* sc = sm.sc[JAC_SLOT][0];
   const Blk& B = R.B;
+  if constexpr (7 * (1 + N) >= BMAX) {
+    // this row of S, scattered into a row of LDS and read back: the stage
+    // derivatives are dead until the stages start, and their buffer holds a
+    // row of BMAX doubles per thread (the workgroup sum at the end of this
+    // function orders the read-back before any stage writes them)
+    double* row = &sm.Fs[0][0][0] + threadIdx.x * BMAX;
 #pragma unroll
-  for (int c = 0; c < BMAX; ++c) F.row[c] = (c == B.lr) ? 1.0 : 0.0;
+    for (int c = 0; c < BMAX; ++c) row[c] = (c == B.lr) ? 1.0 : 0.0;
 #pragma unroll
-  for (int j = 0; j < 5; ++j) {
-    double v = g * R.w.cs[j] * sc[R.w.si[j]];
+    for (int j = 0; j < 5; ++j) {
+      const int cj = R.lc[j];
+      if (cj >= 0 && cj < BMAX) row[cj] -= g * R.w.cs[j] * sc[R.w.si[j]];
+    }
 #pragma unroll
-    for (int c = 0; c < BMAX; ++c) F.row[c] -= (R.lc[j] == c) ? v : 0.0;
+    for (int c = 0; c < BMAX; ++c) F.row[c] = row[c];
+  } else {
+#pragma unroll
+    for (int c = 0; c < BMAX; ++c) F.row[c] = (c == B.lr) ? 1.0 : 0.0;
+#pragma unroll
+    for (int j = 0; j < 5; ++j) {
+      double v = g * R.w.cs[j] * sc[R.w.si[j]];
+#pragma unroll
+      for (int c = 0; c < BMAX; ++c) F.row[c] -= (R.lc[j] == c) ? v : 0.0;
+    }
   }
   double bad = 0.0;
   // fully unrolled over the pivot index, so F.row[p] is a fixed register
@@ -615,25 +652,25 @@ __device__ bool factor(Smem<NT, BMAX, N>& sm, const RowState& R, int& rs, int ma
     if (p < maxb) {
       double* pb = sm.pbuf[p & 1][B.id];
       const bool in = p < B.size;
-      if (in && B.lr == p) {
+      const bool piv = in && B.lr == p;
+      if (piv) {  // the pivot row as it stands, 1/d in the pivot column
         const double d = F.row[p];
         if (!(fabs(d) > 1e-12)) bad = 1.0;
-        const double inv = 1.0 / d;
 #pragma unroll
-        for (int c = 0; c < BMAX; ++c) {
-          double v = (c == p) ? inv : F.row[c] * inv;
-          F.row[c] = v;
-          pb[c] = v;
-        }
+        for (int c = 0; c < BMAX; ++c) pb[c] = (c == p) ? 1.0 / d : F.row[c];
       }
-      __syncthreads();
-      if (in && B.lr != p && B.lr < B.size) {
-        const double f = F.row[p];
+      if (wl) wave_sync();
+      else __syncthreads();
+      if (in && B.lr < B.size) {
+        // every row of the block in one multiply-add per column: the pivot row
+        // scales itself (beta = 1/d - 1 against its own entries), the others
+        // eliminate (beta = -f/d); column p becomes 1/d, resp. -f/d
+        const double inv = pb[p];
+        const double beta = piv ? inv - 1.0 : -F.row[p] * inv;
 #pragma unroll
-        for (int c = 0; c < BMAX; ++c) {
-          double pv = pb[c];
-          F.row[c] = (c == p) ? -f * pv : F.row[c] - f * pv;
-        }
+        for (int c = 0; c < BMAX; ++c)
+          if (c != p) F.row[c] = fma(beta, pb[c], F.row[c]);
+        F.row[p] = piv ? inv : beta;
       }
     }
   }
@@ -830,12 +867,17 @@ pert_fwd_kernel(const AbcmbCosmo* __restrict__ cosmos, const AbcmbTangent* __res
   L.ig = 4; L.ip = 4 + L.lg + 1; L.iu = L.ip + L.lp + 1; L.inc = L.iu + L.lu + 1;
   const int n = L.n;
   RowState R;
-  row_desc(L, k, r, R.w);
+  if constexpr (N > 0) row_desc(L, k, r, R.w);
+  else row_desc_call(L, k, r, R.w);
   R.B = row_block(L, r);
   R.n = n;
 #pragma unroll
   for (int j = 0; j < 5; ++j) R.lc[j] = R.w.col[j] - R.B.start;
   const int maxb = max(L.iu - 2, max(L.lu + 1, L.nq > 0 ? L.ln + 1 : 1));
+  // every hierarchy block inside one wavefront (rows lo..hi)
+  auto one_wave = [](int lo, int hi) { return (lo >> 6) == (hi >> 6); };
+  bool wl = one_wave(2, L.iu - 1) && one_wave(L.iu, L.inc - 1);
+  for (int i = 0; i < L.nq; ++i) wl = wl && one_wave(L.inc + i * (L.ln + 1), L.inc + (i + 1) * (L.ln + 1) - 1);
   for (int q = 0; q < 1 + N; ++q) {
     sm.zl[q][r] = 0.0; sm.rl[r * (1 + N) + q] = 0.0;
     if (r < BMAX) { sm.zl[q][NT + r] = 0.0; sm.rl[(NT + r) * (1 + N) + q] = 0.0; }
@@ -894,7 +936,7 @@ pert_fwd_kernel(const AbcmbCosmo* __restrict__ cosmos, const AbcmbTangent* __res
     const double g = hh * KV_G;
     const double inv_g = 1.0 / g;
     set_times(sm, P, L, k, t, hh, NTIMES, st.jac_offset);
-    bool ok = factor(sm, R, rs, maxb, g, FI);
+    bool ok = factor(sm, R, rs, maxb, wl, g, FI);
     put_F(sm, 0, r, f0);
     D<N> z = y;
 #pragma unroll 1
@@ -906,7 +948,21 @@ pert_fwd_kernel(const AbcmbCosmo* __restrict__ cosmos, const AbcmbTangent* __res
       z = base + g * get_F(sm, s - 1, r);
       const int slot = STAGE_SLOT[s];
       double ms[2 * (1 + N)];  // metric sums of z, kept up to date by the updates
-      stage_state(sm, R, rs, slot, z, ms);
+      double zn[N > 0 ? N : 1];  // the predictor's scaled tangent norms (the tangents' yardstick)
+      {
+        constexpr int NM = 2 * (1 + N);
+        double sv[NM + N];
+        put_z(sm, r, z);
+        metric_terms(sm, R, slot, z, sv);
+        if constexpr (N > 0) {
+          const double w0 = r < n ? 1.0 / (atol + rtol * fabs(z.v)) : 0.0;
+          ABCMB_DLOOP(N) sv[NM + i_] = (z.d[i_] * w0) * (z.d[i_] * w0);
+        }
+        wg_sum(sm, rs, sv);
+#pragma unroll
+        for (int q = 0; q < NM; ++q) ms[q] = sv[q];
+        if constexpr (N > 0) { ABCMB_DLOOP(N) zn[i_] = sv[NM + i_]; }
+      }
       bool conv = false;      // value converged
       bool tconv = N == 0;    // tangents converged
       double prev = 0.0;
@@ -920,18 +976,13 @@ pert_fwd_kernel(const AbcmbCosmo* __restrict__ cosmos, const AbcmbTangent* __res
         z += dz;
         put_z(sm, r, z);  // for the next gather, ordered by the sum below
         // the Newton norms, then the metric-sum increments of dz
-        constexpr int NQ = 1 + 2 * N;
+        constexpr int NQ = 1 + N;
         double v[NQ + 2 * (1 + N)];
         // one reciprocal per row; the decisions below compare products, so the
         // iteration spends two divisions (this one and the rate)
         const double w = r < n ? 1.0 / (atol + rtol * fabs(z.v)) : 0.0;
         v[0] = conv ? 0.0 : (dz.v * w) * (dz.v * w);
-        if constexpr (N > 0) {
-          ABCMB_DLOOP(N) {
-            v[1 + 2 * i_] = (dz.d[i_] * w) * (dz.d[i_] * w);
-            v[2 + 2 * i_] = (z.d[i_] * w) * (z.d[i_] * w);
-          }
-        }
+        if constexpr (N > 0) { ABCMB_DLOOP(N) v[1 + i_] = (dz.d[i_] * w) * (dz.d[i_] * w); }
         metric_terms(sm, R, slot, dz, v + NQ);
         wg_sum(sm, rs, v);
 #pragma unroll
@@ -956,7 +1007,7 @@ pert_fwd_kernel(const AbcmbCosmo* __restrict__ cosmos, const AbcmbTangent* __res
           const double an = rate < 1.0 ? rate : 1.0, ad = rate < 1.0 ? 1.0 - rate : 1.0;
           const double lhs = an * an, rhs = (TANGENT_RTOL * ad) * (TANGENT_RTOL * ad);
           tconv = true;
-          ABCMB_DLOOP(N) tconv = tconv && v[1 + 2 * i_] * lhs <= rhs * v[2 + 2 * i_];
+          ABCMB_DLOOP(N) tconv = tconv && v[1 + i_] * lhs <= rhs * zn[i_];
         }
         if (conv && (tconv || it + 1 >= st.max_newton + TANGENT_EXTRA)) break;
       }
