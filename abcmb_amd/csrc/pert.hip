@@ -766,66 +766,170 @@ __device__ __noinline__ D<N> ic_row(const Cosmo<N>& P, const Layout& L, double k
 // output-table fields at one saved point from the state in LDS (ys[c][row])
 // (abcmb/perturbations.py:318-404; output_perturbations in abcmb/species.py)
 // b: the background at the saved point (precomputed once per cosmology)
+// The output fields read 12 + 3 n_q state rows: eta, delta_c, delta_b,
+// theta_b, photon F_0..F_2, polarisation G_0 and G_2, massless-nu F_0..F_2 and
+// F_0..F_2 of every massive-nu momentum bin.  Slot s of that list holds row
+// out_row(L, s); out_slot is the inverse (-1 for the other rows).
+__device__ __forceinline__ int out_rows(const Layout& L) { return 12 + 3 * L.nq; }
+
+__device__ __forceinline__ int out_slot(const Layout& L, int r) {
+  if (r < 4) return r;
+  if (r >= L.ig && r < L.ig + 3) return 4 + r - L.ig;
+  if (r == L.ip) return 7;
+  if (r == L.ip + 2) return 8;
+  if (r >= L.iu && r < L.iu + 3) return 9 + r - L.iu;
+  if (r >= L.inc && r < L.n) {
+    const int b = (r - L.inc) / (L.ln + 1), m = (r - L.inc) % (L.ln + 1);
+    if (m < 3) return 12 + 3 * b + m;
+  }
+  return -1;
+}
+
+// the rows as columns of the ODE kernel's LDS state
 template <int N, int LD>
-__device__ __noinline__ D<N> output_field(const Cosmo<N>& P, const Layout& L, double k, const Bg<N>& b,
-                                          const double (*ys)[LD], int f) {
-  auto y = [&](int i) {
+struct RowsLds {
+  const double (*ys)[LD];
+  const Layout& L;
+  __device__ __forceinline__ D<N> at(int i) const {
     D<N> x(ys[0][i]);
     if constexpr (N > 0) { ABCMB_DLOOP(N) x.d[i_] = ys[1 + i_][i]; }
     return x;
-  };
+  }
+  __device__ __forceinline__ D<N> eta() const { return at(0); }
+  __device__ __forceinline__ D<N> dc() const { return at(1); }
+  __device__ __forceinline__ D<N> db() const { return at(2); }
+  __device__ __forceinline__ D<N> tb() const { return at(3); }
+  __device__ __forceinline__ D<N> g(int m) const { return at(L.ig + m); }
+  __device__ __forceinline__ D<N> pol(int m) const { return at(L.ip + m); }
+  __device__ __forceinline__ D<N> ur(int m) const { return at(L.iu + m); }
+  __device__ __forceinline__ D<N> nc(int b, int m) const { return at(L.inc + b * (L.ln + 1) + m); }
+};
+
+// the rows in registers, by slot (bins beyond the layout's read as zero)
+template <int N>
+struct RowsReg {
+  D<N> Y[27];
+  __device__ __forceinline__ D<N> eta() const { return Y[0]; }
+  __device__ __forceinline__ D<N> dc() const { return Y[1]; }
+  __device__ __forceinline__ D<N> db() const { return Y[2]; }
+  __device__ __forceinline__ D<N> tb() const { return Y[3]; }
+  __device__ __forceinline__ D<N> g(int m) const { return Y[4 + m]; }
+  __device__ __forceinline__ D<N> pol(int m) const { return m == 0 ? Y[7] : Y[8]; }
+  __device__ __forceinline__ D<N> ur(int m) const { return Y[9 + m]; }
+  __device__ __forceinline__ D<N> nc(int b, int m) const { return Y[12 + 3 * b + m]; }
+};
+
+// every field of the perturbation table at one saved point
+// (abcmb/perturbations.py:318-404 and the species output_perturbations),
+// handed to emit(field, value)
+template <int N, typename Rows, typename Emit>
+__device__ __forceinline__ void all_fields(const Cosmo<N>& P, const Layout& L, double k, const Bg<N>& b,
+                                           const Rows& y, Emit emit) {
   const double a = b.a;
   const D<N> aH = b.aH;
   D<N> nrd(0.0), nrt(0.0), nrs(0.0);
   if (L.nq > 0) {
-    int nl = L.ln + 1;
     D<N> s0(0.0), s1(0.0), s2(0.0);
-    for (int i = 0; i < L.nq; ++i) {
-      double q = L.nq == 5 ? Q5[i] : Q3[i];
-      double wq = L.nq == 5 ? W5[i] : W3[i];
-      D<N> e = dsqrt(q * q + b.xn * b.xn);
-      const int p0 = L.inc + i * nl;
-      s0 += wq * (1.0 + exp(-q)) * e / (q * q) * y(p0);
-      s1 += wq * (1.0 + exp(-q)) / q * y(p0 + 1);
-      s2 += wq * (1.0 + exp(-q)) / e * y(p0 + 2);
+#pragma unroll
+    for (int i = 0; i < 5; ++i) {
+      if (i < L.nq) {
+        double q = L.nq == 5 ? Q5[i] : Q3[i];
+        double wq = L.nq == 5 ? W5[i] : W3[i];
+        D<N> e = dsqrt(q * q + b.xn * b.xn);
+        s0 += wq * (1.0 + exp(-q)) * e / (q * q) * y.nc(i, 0);
+        s1 += wq * (1.0 + exp(-q)) / q * y.nc(i, 1);
+        s2 += wq * (1.0 + exp(-q)) / e * y.nc(i, 2);
+      }
     }
     D<N> pre = P(F_N_NCDM) * b.Tn * b.Tn * b.Tn * b.Tn / (PI * PI) / (HBAR * HBAR * HBAR) / (C_CM * C_CM * C_CM);
     nrd = s0 * 4.0 * pre; nrt = s1 * 4.0 * pre; nrs = s2 * 8.0 / 3.0 * pre;
   }
-  switch (f) {
-    case ABCMB_PT_THETA_B_PRIME:
-      return -y(3) + b.cs2 / aH * (k * k * y(2)) + b.R / aH / b.tauc * (y(L.ig + 1) - y(3));
-    case ABCMB_PT_ETA: return y(0);
-    case ABCMB_PT_CDM_DELTA: return y(1);
-    case ABCMB_PT_B_DELTA: return y(2);
-    case ABCMB_PT_B_THETA: return y(3);
-    case ABCMB_PT_G_DELTA: return y(L.ig);
-    case ABCMB_PT_G_THETA: return y(L.ig + 1);
-    case ABCMB_PT_G_SIGMA: return y(L.ig + 2);
-    case ABCMB_PT_G_G0: return y(L.ip);
-    case ABCMB_PT_G_G2: return y(L.ip + 2);
-    case ABCMB_PT_UR_DELTA: return y(L.iu);
-    case ABCMB_PT_UR_THETA: return y(L.iu + 1);
-    case ABCMB_PT_UR_SIGMA: return y(L.iu + 2);
-    case ABCMB_PT_NCDM_DELTA: return L.nq > 0 ? nrd / b.rn : D<N>(0.0);
-    case ABCMB_PT_NCDM_THETA: return L.nq > 0 ? nrt / (b.rn + b.pn) : D<N>(0.0);
-    case ABCMB_PT_NCDM_SIGMA: return L.nq > 0 ? nrs / (b.rn + b.pn) : D<N>(0.0);
-    default: break;
-  }
-  D<N> srd = b.rc * y(1) + b.rb * y(2) + b.rg * y(L.ig) + b.ru * y(L.iu);
-  D<N> srt = b.rb * y(3) + (b.rg + b.rg / 3.) * y(L.ig + 1) + (b.ru + b.ru / 3.) * y(L.iu + 1);
-  D<N> srs = (b.rg + b.rg / 3.) * y(L.ig + 2) + (b.ru + b.ru / 3.) * y(L.iu + 2);
-  D<N> srdm = b.rc * y(1) + b.rb * y(2), srm = b.rc + b.rb;
+  emit(ABCMB_PT_THETA_B_PRIME, -y.tb() + b.cs2 / aH * (k * k * y.db()) + b.R / aH / b.tauc * (y.g(1) - y.tb()));
+  emit(ABCMB_PT_ETA, y.eta());
+  emit(ABCMB_PT_CDM_DELTA, y.dc());
+  emit(ABCMB_PT_B_DELTA, y.db());
+  emit(ABCMB_PT_B_THETA, y.tb());
+  emit(ABCMB_PT_G_DELTA, y.g(0));
+  emit(ABCMB_PT_G_THETA, y.g(1));
+  emit(ABCMB_PT_G_SIGMA, y.g(2));
+  emit(ABCMB_PT_G_G0, y.pol(0));
+  emit(ABCMB_PT_G_G2, y.pol(2));
+  emit(ABCMB_PT_UR_DELTA, y.ur(0));
+  emit(ABCMB_PT_UR_THETA, y.ur(1));
+  emit(ABCMB_PT_UR_SIGMA, y.ur(2));
+  emit(ABCMB_PT_NCDM_DELTA, L.nq > 0 ? nrd / b.rn : D<N>(0.0));
+  emit(ABCMB_PT_NCDM_THETA, L.nq > 0 ? nrt / (b.rn + b.pn) : D<N>(0.0));
+  emit(ABCMB_PT_NCDM_SIGMA, L.nq > 0 ? nrs / (b.rn + b.pn) : D<N>(0.0));
+  D<N> srd = b.rc * y.dc() + b.rb * y.db() + b.rg * y.g(0) + b.ru * y.ur(0);
+  D<N> srt = b.rb * y.tb() + (b.rg + b.rg / 3.) * y.g(1) + (b.ru + b.ru / 3.) * y.ur(1);
+  D<N> srs = (b.rg + b.rg / 3.) * y.g(2) + (b.ru + b.ru / 3.) * y.ur(2);
+  D<N> srdm = b.rc * y.dc() + b.rb * y.db(), srm = b.rc + b.rb;
   if (L.nq > 0) { srd += nrd; srt += nrt; srs += nrs; srdm += nrd; srm += b.rn; }
-  if (f == ABCMB_PT_DELTA_M) return srdm / srm;
+  emit(ABCMB_PT_DELTA_M, srdm / srm);
   double KG = 4. * PI * GN * a * a / (C_MPC * C_MPC);
-  D<N> hp = 2. / (aH * aH) * (k * k * y(0) + KG * srd);
+  D<N> hp = 2. / (aH * aH) * (k * k * y.eta() + KG * srd);
   D<N> ep = KG / aH * srt / (k * k);
-  if (f == ABCMB_PT_H_PRIME) return hp;
-  if (f == ABCMB_PT_ETA_PRIME) return ep;
+  emit(ABCMB_PT_H_PRIME, hp);
+  emit(ABCMB_PT_ETA_PRIME, ep);
   D<N> al = aH * (hp + 6. * ep) / 2. / (k * k);
-  if (f == ABCMB_PT_ALPHA) return al;
-  return y(0) / aH - 2. * al - 3. * KG / aH * srs / (k * k);  // ABCMB_PT_ALPHA_PRIME
+  emit(ABCMB_PT_ALPHA, al);
+  emit(ABCMB_PT_ALPHA_PRIME, y.eta() / aH - 2. * al - 3. * KG / aH * srs / (k * k));
+}
+
+// field f at one saved point from the LDS state (the in-kernel output path)
+template <int N, int LD>
+__device__ __noinline__ D<N> output_field(const Cosmo<N>& P, const Layout& L, double k, const Bg<N>& b,
+                                          const double (*ys)[LD], int f) {
+  D<N> out(0.0);
+  all_fields(P, L, k, b, RowsLds<N, LD>{ys, L}, [&](int g, const D<N>& v) {
+    if (g == f) out = v;
+  });
+  return out;
+}
+
+// The perturbation table in place: where the layout's output rows fit the
+// table's field slots (12 + 3 n_q <= 21, i.e. up to three momentum bins) the
+// ODE kernel stores the saved rows themselves, slot s in field slot s, with
+// no workgroup synchronisation; this kernel then turns every (k-mode, saved
+// point) of the launch's items into the table fields, one thread each.
+template <int N>
+__global__ void __launch_bounds__(64) fields_kernel(const AbcmbCosmo* __restrict__ cosmos,
+                                                    const AbcmbTangent* __restrict__ tangents,
+                                                    const AbcmbPertItem* __restrict__ items, int n_items,
+                                                    AbcmbOdeSettings st, const Bg<N>* __restrict__ bgsave,
+                                                    int n_save_max, const int32_t* __restrict__ status) {
+  const long long idx = (long long)blockIdx.x * 64 + threadIdx.x;
+  if (idx >= (long long)n_items * n_save_max) return;
+  const int item = (int)(idx / n_save_max), isave = (int)(idx % n_save_max);
+  if (status[item] != ABCMB_K_OK) return;
+  const AbcmbPertItem I = items[item];
+  const AbcmbCosmo& c = cosmos[I.cosmo];
+  const int n_save = c.n_save;
+  if (isave >= n_save) return;
+  const AbcmbTangent* T = nullptr;
+  if constexpr (N > 0) T = tangents + I.cosmo;
+  const Cosmo<N> P{c, T};
+  Layout L;
+  L.lg = st.l_g; L.lp = st.l_pol; L.lu = st.l_ur; L.ln = st.l_ncdm; L.nq = st.n_q; L.n = st.n;
+  L.ig = 4; L.ip = 4 + L.lg + 1; L.iu = L.ip + L.lp + 1; L.inc = L.iu + L.lu + 1;
+  const int nr = out_rows(L);
+  const size_t dstride = (size_t)ABCMB_N_PT_FIELDS * n_save * c.nk;
+  auto at = [&](int s) { return ((size_t)s * n_save + isave) * c.nk + I.ik; };
+  RowsReg<N> y;
+#pragma unroll
+  for (int s = 0; s < 27; ++s) {
+    y.Y[s] = D<N>(0.0);
+    if (s < ABCMB_N_PT_FIELDS && s < nr) {
+      const size_t o = at(s);
+      y.Y[s].v = c.pt[o];
+      if constexpr (N > 0) { ABCMB_DLOOP(N) y.Y[s].d[i_] = T->pt[i_ * dstride + o]; }
+    }
+  }
+  all_fields(P, L, I.k, bgsave[(size_t)I.cosmo * n_save_max + isave], y, [&](int f, const D<N>& v) {
+    const size_t o = at(f);
+    c.pt[o] = v.v;
+    if constexpr (N > 0) { ABCMB_DLOOP(N) T->pt[i_ * dstride + o] = v.d[i_]; }
+  });
 }
 
 // the background (with its tangents) at every saved point of every cosmology,
@@ -908,7 +1012,23 @@ pert_fwd_kernel(const AbcmbCosmo* __restrict__ cosmos, const AbcmbTangent* __res
   int isave = 0, steps = 0, iters = 0, status = ABCMB_K_OK;
   Inv<BMAX> FI;
 
+  // with the output rows fitting the table's field slots, a saved point is the
+  // rows themselves (fields_kernel turns them into fields after the launch)
+  const bool rows_out = out_rows(L) <= ABCMB_N_PT_FIELDS;
+  const int oslot = rows_out && r < n ? out_slot(L, r) : -1;
   auto save_point = [&](const D<N>& ys) {
+    if (rows_out) {
+      if (state_out && r < n) state_out[((size_t)item * n_save + isave) * n + r] = ys.v;
+      if (oslot >= 0) {
+        const size_t o = ((size_t)oslot * n_save + isave) * c.nk + I.ik;
+        c.pt[o] = ys.v;
+        if constexpr (N > 0) {
+          const size_t dstride = (size_t)ABCMB_N_PT_FIELDS * n_save * c.nk;
+          ABCMB_DLOOP(N) T->pt[i_ * dstride + o] = ys.d[i_];
+        }
+      }
+      return;
+    }
     __syncthreads();
     put_z(sm, r, ys);
     __syncthreads();
@@ -1062,6 +1182,12 @@ int launch(const AbcmbCosmo* cosmos, const AbcmbTangent* tangents, const AbcmbPe
   hipLaunchKernelGGL((pert_fwd_kernel<NT, BMAX, N>), dim3(n_items), dim3(NT), 0, stream, cosmos, tangents, items,
                      st, bgsave, n_save_max, status, steps, iters, state);
   ABCMB_HIP_CHECK(hipGetLastError());
+  if (12 + 3 * st.n_q <= ABCMB_N_PT_FIELDS) {  // the kernel stored rows: turn them into fields
+    const long long nthr = (long long)n_items * n_save_max;
+    hipLaunchKernelGGL((fields_kernel<N>), dim3((unsigned)((nthr + 63) / 64)), dim3(64), 0, stream, cosmos, tangents,
+                       items, n_items, st, bgsave, n_save_max, status);
+    ABCMB_HIP_CHECK(hipGetLastError());
+  }
   return ABCMB_OK;
 }
 
