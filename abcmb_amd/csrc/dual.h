@@ -100,6 +100,15 @@ template <int N> __device__ __forceinline__ D<N> operator/(double b, const D<N>&
   }
   return r;
 }
+// 1 / a with one division: the value's reciprocal, and -d / a^2 from it
+template <int N> __device__ __forceinline__ D<N> dinv(const D<N>& a) {
+  D<N> r; r.v = 1.0 / a.v;
+  if constexpr (N > 0) {
+    const double s = -r.v * r.v;
+    ABCMB_DLOOP(N) r.d[i_] = s * a.d[i_];
+  }
+  return r;
+}
 template <int N> __device__ __forceinline__ D<N>& operator+=(D<N>& a, const D<N>& b) { a = a + b; return a; }
 template <int N> __device__ __forceinline__ D<N>& operator-=(D<N>& a, const D<N>& b) { a = a - b; return a; }
 

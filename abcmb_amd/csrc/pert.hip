@@ -54,10 +54,27 @@ __constant__ double KV_E[7] = {0.13659751177640291 - 0.13855640231268224, 0.0,
                                0.62993304899016403 - 0.61943039072480676,
                                0.06962479448202728 - 0.26, 0.26};
 
-__constant__ double Q3[3] = {0.913201, 3.37517, 7.79184};
-__constant__ double W3[3] = {0.0687359, 3.31435, 2.29911};
-__constant__ double Q5[5] = {0.583165, 2.0, 4.0, 7.26582, 13.0};
-__constant__ double W5[5] = {0.0081201, 0.689407, 2.8063, 2.05156, 0.12681};
+// massive-neutrino momentum quadrature: nodes q, weights w, and 1 + e^-q at
+// the nodes (the occupation factor's numerator), folded into the weights the
+// background and the source terms use, so that no step evaluates exp(-q)
+constexpr double Q3V[3] = {0.913201, 3.37517, 7.79184};
+constexpr double W3V[3] = {0.0687359, 3.31435, 2.29911};
+constexpr double E3V[3] = {1.4012378040061892, 1.0342123017259768, 1.0004130920948129};
+constexpr double Q5V[5] = {0.583165, 2.0, 4.0, 7.26582, 13.0};
+constexpr double W5V[5] = {0.0081201, 0.689407, 2.8063, 2.05156, 0.12681};
+constexpr double E5V[5] = {1.5581290895929683, 1.1353352832366128, 1.0183156388887342, 1.0006990278317496,
+                           1.000002260329407};
+__constant__ double Q3[3] = {Q3V[0], Q3V[1], Q3V[2]};
+__constant__ double W3[3] = {W3V[0], W3V[1], W3V[2]};
+__constant__ double Q5[5] = {Q5V[0], Q5V[1], Q5V[2], Q5V[3], Q5V[4]};
+__constant__ double W5[5] = {W5V[0], W5V[1], W5V[2], W5V[3], W5V[4]};
+#define ABCMB_WR(Q, W, E, i) (E[i] * W[i] / (Q[i] * Q[i]))
+// w (1 + e^-q) / q^2 (density: times e = sqrt(q^2 + x^2)) and w (1 + e^-q) (pressure: over e)
+__constant__ double WR5[5] = {ABCMB_WR(Q5V, W5V, E5V, 0), ABCMB_WR(Q5V, W5V, E5V, 1), ABCMB_WR(Q5V, W5V, E5V, 2),
+                              ABCMB_WR(Q5V, W5V, E5V, 3), ABCMB_WR(Q5V, W5V, E5V, 4)};
+__constant__ double WP5[5] = {E5V[0] * W5V[0], E5V[1] * W5V[1], E5V[2] * W5V[2], E5V[3] * W5V[3], E5V[4] * W5V[4]};
+__constant__ double WR3[3] = {ABCMB_WR(Q3V, W3V, E3V, 0), ABCMB_WR(Q3V, W3V, E3V, 1), ABCMB_WR(Q3V, W3V, E3V, 2)};
+#undef ABCMB_WR
 
 // background scalar slots (per evaluation time)
 enum {
@@ -134,43 +151,49 @@ struct Bg {
   D<N> Tn, xn;
 };
 
-__device__ __forceinline__ double rho_unit() { return 3.0 * H0_OVER_H * H0_OVER_H / 8.0 / PI / GN; }
-
-// densities, aH (abcmb/background.py:101-187; species rho at abcmb/species.py)
+// densities, aH (abcmb/background.py:101-187; species rho at abcmb/species.py).
+// Quotients by a, by dual quantities and by constants are written as products
+// with reciprocals (one division per reciprocal), which moves results by at
+// most a few units in the last place.
 template <int N>
 __device__ void bg_rho(const Cosmo<N>& P, double lna, Bg<N>& b) {
-  const double hc3 = (C_CM * HBAR) * (C_CM * HBAR) * (C_CM * HBAR);
-  double a = exp(lna);
-  double a3 = a * a * a, a4 = a3 * a;
+  constexpr double hc3 = (C_CM * HBAR) * (C_CM * HBAR) * (C_CM * HBAR);
+  constexpr double K_RHO = 3.0 * H0_OVER_H * H0_OVER_H / 8.0 / PI / GN;  // critical density per h^2
+  constexpr double K_G = PI * PI / 15.0 / hc3;
+  constexpr double K_U = 2.0 * 7.0 / 8.0 * PI * PI / 30.0 / hc3;
+  constexpr double K_N = 1.0 / (PI * PI) / (HBAR * HBAR * HBAR) / (C_CM * C_CM * C_CM);
+  constexpr double K_H = 8.0 * PI * GN / 3.0;
+  const double a = exp(lna), ia = 1.0 / a;
+  const double ia3 = ia * ia * ia, ia4 = ia3 * ia;
   const D<N> T0 = P(F_TCMB0), Tu = P(F_T_UR);
-  D<N> t04 = T0 * T0 * T0 * T0;
-  D<N> tu4 = Tu * Tu * Tu * Tu;
+  const D<N> t04 = T0 * T0 * T0 * T0;
+  const D<N> tu4 = Tu * Tu * Tu * Tu;
   b.a = a;
-  D<N> rde = P(F_OMEGA_LAMBDA) * rho_unit();
-  b.rc = P(F_OMEGA_CDM) * rho_unit() / a3;
-  b.rb = P(F_OMEGA_B) * rho_unit() / a3;
-  b.rg = PI * PI / 15.0 * t04 / a4 / hc3;
-  b.ru = P(F_N_UR) * 2.0 * 7.0 / 8.0 * PI * PI / 30.0 * tu4 * t04 / a4 / hc3;
+  const D<N> rde = P(F_OMEGA_LAMBDA) * K_RHO;
+  b.rc = P(F_OMEGA_CDM) * (K_RHO * ia3);
+  b.rb = P(F_OMEGA_B) * (K_RHO * ia3);
+  b.rg = t04 * (K_G * ia4);
+  b.ru = P(F_N_UR) * tu4 * t04 * (K_U * ia4);
   b.rn = D<N>(0.0);
   b.pn = D<N>(0.0);
-  b.Tn = P(F_T_NCDM) * T0 / a;
-  b.xn = P(F_M_NCDM) / b.Tn;
+  b.Tn = P(F_T_NCDM) * T0 * ia;
+  b.xn = P(F_M_NCDM) * dinv(b.Tn);
   if (P.c.has_ncdm) {
     D<N> ir(0.0), ip(0.0);
+    const D<N> x2 = b.xn * b.xn;
 #pragma unroll
     for (int i = 0; i < 5; ++i) {
-      double q = Q5[i];
-      D<N> e = dsqrt(q * q + b.xn * b.xn);
-      ir += (1.0 + exp(-q)) / (q * q) * e * W5[i];
-      ip += (1.0 + exp(-q)) / e * W5[i];
+      const double q = Q5[i];
+      D<N> e = dsqrt(x2 + q * q);
+      ir += WR5[i] * e;
+      ip += WP5[i] * dinv(e);
     }
-    D<N> T4 = b.Tn * b.Tn * b.Tn * b.Tn;
-    D<N> pre = T4 / (PI * PI) / (HBAR * HBAR * HBAR) / (C_CM * C_CM * C_CM);
+    D<N> pre = b.Tn * b.Tn * b.Tn * b.Tn * K_N;
     b.rn = P(F_N_NCDM) * 4.0 * pre * ir;
-    b.pn = P(F_N_NCDM) * 4.0 / 3.0 * pre * ip;
+    b.pn = P(F_N_NCDM) * (4.0 / 3.0) * pre * ip;
   }
   D<N> rt = rde + b.rc + b.rb + b.rg + b.ru + b.rn;
-  b.aH = a * dsqrt(8.0 * PI * GN * rt / 3.0) / C_MPC;
+  b.aH = a * dsqrt(K_H * rt) * (1.0 / C_MPC);
 }
 
 // x_e lookup with the reference's span quirk (abcmb/background.py:568-603)
@@ -184,31 +207,37 @@ __device__ __forceinline__ D<N> bg_xe(const Cosmo<N>& P, double lna) {
 
 template <int N>
 __device__ __noinline__ void bg_eval(const Cosmo<N>& P, double lna, Bg<N>& b) {
+  constexpr double K_NH = 3.0 * H0_OVER_H * H0_OVER_H / 8 / PI / GN / MH;
+  constexpr double K_TC = C_MPC / SIGMA_T / C_CM;
+  constexpr double STEF = PI * PI / (60.0 * HBAR * HBAR * HBAR * C_CM * C_CM);
+  constexpr double K_GC = 8.0 * SIGMA_T * (4.0 * STEF) / (3.0 * ME);
   const AbcmbCosmo& c = P.c;
   bg_rho(P, lna, b);
+  const double ia = 1.0 / b.a;
   b.tau = fast_interp<N>(lna, ABCMB_TAU_TAB_LO, 0.0, c.tau_tab, P.dtau(), ABCMB_TAU_TAB_N);
   D<N> xe = bg_xe(P, lna);
   const D<N> YHe = P(F_YHE);
-  D<N> nH = (1 - YHe) * 3.0 * P(F_OMEGA_B) * H0_OVER_H * H0_OVER_H / 8 / PI / GN / MH / (b.a * b.a * b.a);
-  b.tauc = 1.0 / b.a / (nH * xe) / SIGMA_T / C_CM * C_MPC;
+  D<N> nH = (1 - YHe) * P(F_OMEGA_B) * (K_NH * ia * ia * ia);
+  b.tauc = dinv(nH * xe) * (ia * K_TC);
   // T_m (abcmb/background.py:605-657)
-  D<N> Tg = P(F_TCMB0) / b.a;
+  D<N> Tg = P(F_TCMB0) * ia;
   D<N> Tm;
   if (lna < c.tm_x0) {
-    D<N> fhe = YHe * MH / MHE / (1 - YHe);
-    double stef = PI * PI / (60.0 * HBAR * HBAR * HBAR * C_CM * C_CM);
-    D<N> gc = xe / (1.0 + fhe + xe) * 8.0 * SIGMA_T * (4.0 * stef) * Tg * Tg * Tg * Tg / (3.0 * ME);
-    D<N> H = b.aH * C_MPC / b.a;
-    Tm = Tg * (1.0 - H / gc);
+    D<N> fhe = YHe * (MH / MHE) * dinv(1 - YHe);
+    D<N> Tg2 = Tg * Tg;
+    D<N> gc = xe * dinv(1.0 + fhe + xe) * Tg2 * Tg2 * K_GC;
+    D<N> H = b.aH * (C_MPC * ia);
+    Tm = Tg * (1.0 - H * dinv(gc));
   } else if (lna >= c.tm_lna_last) {
     Tm = P(F_TM_LAST);
   } else {
     Tm = fast_interp<N>(lna, c.tm_x0, c.tm_x0 + c.tm_n * c.tm_dx, c.tm_tab, P.dtm(), c.tm_n);
   }
   // c_s^2 (abcmb/species.py:1163-1220)
-  D<N> mu = MH / ((1.0 + xe) * (1.0 - YHe) + MH / MHE * YHe);
-  b.R = 4.0 * b.rg / 3.0 / b.rb;
-  b.cs2 = Tm / mu * (5.0 / 3.0 - 2.0 / 3.0 * mu * b.R / ME / b.aH / b.tauc * (Tg / Tm - 1.0));
+  D<N> mu = MH * dinv((1.0 + xe) * (1.0 - YHe) + (MH / MHE) * YHe);
+  b.R = (4.0 / 3.0) * b.rg * dinv(b.rb);
+  b.cs2 = Tm * dinv(mu) *
+          (5.0 / 3.0 - (2.0 / 3.0 / ME) * mu * b.R * dinv(b.aH) * dinv(b.tauc) * (Tg * dinv(Tm) - 1.0));
 }
 
 // ---------------------------------------------------------------- row description
@@ -324,36 +353,48 @@ __device__ __noinline__ void row_desc_call(const Layout& L, double k, int r, Row
 template <int N>
 __device__ __noinline__ void fill_scalars(const Cosmo<N>& P, const Layout& L, double k, const Bg<N>& b,
                                           double (*sc)[SC_COUNT]) {
+  constexpr double K_G = 4.0 * PI * GN / (C_MPC * C_MPC);
+  constexpr double K_N = 1.0 / (PI * PI) / (HBAR * HBAR * HBAR) / (C_CM * C_CM * C_CM);
   auto put = [&](int s, const D<N>& x) {
     sc[0][s] = x.v;
     if constexpr (N > 0) { ABCMB_DLOOP(N) sc[1 + i_][s] = x.d[i_]; }
   };
-  const D<N> ah = b.aH;
+  const D<N> iah = dinv(b.aH);
+  const D<N> itc = iah * dinv(b.tauc);
   put(SC_ONE, D<N>(1.0));
-  put(SC_IAH, 1.0 / ah);
-  put(SC_ITC, 1.0 / (ah * b.tauc));
-  put(SC_ITAU, 1.0 / (ah * b.tau));
-  put(SC_CS2, b.cs2 / ah);
-  put(SC_RTC, b.R / (b.tauc * ah));
-  D<N> kh = 2.0 / (ah * ah) * 4.0 * PI * GN * b.a * b.a / (C_MPC * C_MPC);
-  put(SC_H_ETA, 2.0 / (ah * ah) * k * k);
+  put(SC_IAH, iah);
+  put(SC_ITC, itc);
+  put(SC_ITAU, iah * dinv(b.tau));
+  put(SC_CS2, b.cs2 * iah);
+  put(SC_RTC, b.R * itc);
+  const D<N> iah2 = iah * iah;
+  const double a2 = b.a * b.a;
+  D<N> kh = iah2 * (2.0 * K_G * a2);
+  put(SC_H_ETA, iah2 * (2.0 * k * k));
   put(SC_H_C, kh * b.rc);
   put(SC_H_B, kh * b.rb);
   put(SC_H_G, kh * b.rg);
   put(SC_H_U, kh * b.ru);
-  D<N> ke = 4.0 * PI * GN * b.a * b.a / ah / (k * k) / (C_MPC * C_MPC);
+  D<N> ke = iah * (K_G * a2 / (k * k));
   put(SC_E_B, ke * b.rb);
   put(SC_E_G, ke * (b.rg + b.rg / 3.0));
   put(SC_E_U, ke * (b.ru + b.ru / 3.0));
   D<N> pre(0.0);
-  if (P.c.has_ncdm)
-    pre = P(F_N_NCDM) * b.Tn * b.Tn * b.Tn * b.Tn / (PI * PI) / (HBAR * HBAR * HBAR) / (C_CM * C_CM * C_CM);
+  if (P.c.has_ncdm) pre = P(F_N_NCDM) * b.Tn * b.Tn * b.Tn * b.Tn * K_N;
+  const D<N> x2 = b.xn * b.xn;
+  const D<N> khp = kh * 4.0 * pre;
+#pragma unroll
   for (int i = 0; i < 5; ++i) {
-    double q = i < L.nq ? (L.nq == 5 ? Q5[i] : Q3[i]) : 1.0;
-    double wq = i < L.nq ? (L.nq == 5 ? W5[i] : W3[i]) : 0.0;
-    D<N> e = dsqrt(q * q + b.xn * b.xn);
-    put(SC_P0 + i, q / e / ah);
-    put(SC_HN0 + i, kh * 4.0 * pre * wq * (1.0 + exp(-q)) * e / (q * q));
+    if (i < L.nq) {
+      const double q = L.nq == 5 ? Q5[i] : Q3[i];
+      const double wr = L.nq == 5 ? WR5[i] : WR3[i];
+      D<N> e = dsqrt(x2 + q * q);
+      put(SC_P0 + i, q * dinv(e) * iah);
+      put(SC_HN0 + i, khp * (wr * e));
+    } else {  // no such bin: no row reads these slots
+      put(SC_P0 + i, D<N>(0.0));
+      put(SC_HN0 + i, D<N>(0.0));
+    }
   }
   put(SC_EN, ke * 4.0 * pre);
   put(SC_ZERO, D<N>(0.0));
