@@ -706,11 +706,16 @@ __device__ bool factor(Smem<NT, BMAX, N>& sm, const RowState& R, int& rs, int ma
         // every row of the block in one multiply-add per column: the pivot row
         // scales itself (beta = 1/d - 1 against its own entries), the others
         // eliminate (beta = -f/d); column p becomes 1/d, resp. -f/d
-        const double inv = pb[p];
+        // the whole pivot row first: one wait for all its loads, instead of the
+        // load-wait-multiply-add chain the scheduler otherwise emits
+        double pv[BMAX];
+#pragma unroll
+        for (int c = 0; c < BMAX; ++c) pv[c] = pb[c];
+        const double inv = pv[p];
         const double beta = piv ? inv - 1.0 : -F.row[p] * inv;
 #pragma unroll
         for (int c = 0; c < BMAX; ++c)
-          if (c != p) F.row[c] = fma(beta, pb[c], F.row[c]);
+          if (c != p) F.row[c] = fma(beta, pv[c], F.row[c]);
         F.row[p] = piv ? inv : beta;
       }
     }
