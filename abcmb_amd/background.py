@@ -25,7 +25,11 @@ class Background:
     """Background of one cosmology (host-resident FP64 tables)."""
 
     def __init__(self, params: dict, has_ncdm: bool, n_q: int = 3, k_pivot: float = 0.05,
-                 input_tau_reion: bool = True, xe_pad: int = 0, rec_switch: int = -1):
+                 input_tau_reion: bool = True, xe_pad: int = 0, rec_switch: int = -1,
+                 lna_rec: float = None, lna_transfer_start: float = None):
+        """``rec_switch``, ``lna_rec``, ``lna_transfer_start``: grid choices held at
+        another cosmology's (finite differences with frozen grids); the producer
+        then keeps them instead of searching for its own."""
         lib = abi.host_lib()
         self.params = params
         n_rec = lib.abcmb_host_recomb_points()
@@ -49,6 +53,8 @@ class Background:
             tau_re, z_re = p["tau_reion"], 0.0
         else:
             tau_re, z_re = -1.0, p["z_reion"]
+        if lna_rec is not None and lna_transfer_start is not None:
+            c.lna_rec, c.lna_transfer_start = lna_rec, lna_transfer_start
         sw = C.c_int32(rec_switch)
         rc = lib.abcmb_host_background(C.byref(c), tau_re, z_re, p["Delta_z_reion"], p["z_reion_He"],
                                        p["Delta_z_reion_He"], p["exp_reion"], C.byref(sw))

@@ -255,6 +255,8 @@ double beta_b(double T) {
 }
 
 void recombination(const AbcmbCosmo& c, int N, double* xe, double* tm, int* switch_io) {
+  const double hc_pi3 = std::pow(2.0 * PI * HBAR * C_CM, 3.0);
+  const double k_lya = std::pow(C_CM / LYA_FREQ, 3);
   double lo = -std::log(1.0 + REC_Z0);
   double fhe = c.YHe * MH / MHE / (1.0 - c.YHe);
   bool tla = false;
@@ -265,9 +267,9 @@ void recombination(const AbcmbCosmo& c, int N, double* xe, double* tm, int* swit
     double Tg = c.TCMB0 / a;
     double nh = nH(c, lna);
     double H = hubble(c, lna);
-    double xp_saha;
-    double xe_saha = saha_xe(Tg, nh, fhe, &xp_saha);
     if (!tla) {
+      double xp_saha;
+      double xe_saha = saha_xe(Tg, nh, fhe, &xp_saha);
       xe[j] = xe_saha;
       xp = xp_saha;
       double gc = gamma_compton(xe_saha, Tg, c.YHe);
@@ -291,11 +293,11 @@ void recombination(const AbcmbCosmo& c, int N, double* xe, double* tm, int* swit
       double Hs = hubble(c, l);
       // helium from Saha given the current electron density (lagged)
       double ne_guess = (xp + 0.0) * n1;
-      double lame = std::pow(2.0 * PI * ME * tg, 1.5) / std::pow(2.0 * PI * HBAR * C_CM, 3.0);
+      double lame = std::pow(2.0 * PI * ME * tg, 1.5) / hc_pi3;
       double r1 = 4.0 * lame * std::exp(-24.587387 / tg) / std::max(ne_guess, 1e-300);
       double xhe = fhe * r1 / (1.0 + r1);
       // Newton on F(x) = x - xp_old - h * dxdlna(x); rates depend on T only
-      const double K = std::pow(C_CM / LYA_FREQ, 3) / (8.0 * PI * Hs);
+      const double K = k_lya / (8.0 * PI * Hs);
       const double bb = beta_b(tg);
       const double ab = alpha_b(T_m);
       const double boltz = std::exp(-E21 / tg);
@@ -320,7 +322,7 @@ void recombination(const AbcmbCosmo& c, int N, double* xe, double* tm, int* swit
       // backward Euler for dT/dlna = -2T + gc (Tg - T)
       T_m = (T_m + h * gc * tg) / (1.0 + h * (2.0 + gc));
     }
-    double lame = std::pow(2.0 * PI * ME * Tg, 1.5) / std::pow(2.0 * PI * HBAR * C_CM, 3.0);
+    double lame = std::pow(2.0 * PI * ME * Tg, 1.5) / hc_pi3;
     double r1 = 4.0 * lame * std::exp(-24.587387 / Tg) / std::max(xp * nh, 1e-300);
     xe[j] = xp + fhe * r1 / (1.0 + r1);
     tm[j] = T_m;
@@ -338,19 +340,42 @@ double xe_reion(double lna, double z_re, double dz, double zHe, double dzHe, dou
   double xHe = fHe / 2.0 * (1.0 + std::tanh((zHe - z) / dzHe));
   return xH + xHe;
 }
-double tau_reion_of(const AbcmbCosmo& c, double z_re, double dz, double zHe, double dzHe, double ex) {
-  const int n = 2000;
-  double s = 0.0, prev = 0.0;
-  for (int i = 0; i < n; ++i) {
-    double l = -5.0 + 5.0 * i / (n - 1);
-    double ne = nH(c, l) * xe_reion(l, z_re, dz, zHe, dzHe, ex, c.YHe);
-    double gam = std::exp(l) * ne * SIGMA_T * C_CM / C_MPC;
-    double f = gam / aH(c, l);
-    if (i > 0) s += 0.5 * (f + prev) * (5.0 / (n - 1));
-    prev = f;
+// tau_reion(z_re) on a fixed ln a grid; the pieces of the integrand that do
+// not depend on z_re (n_H, a H, (1 + z)^ex, the helium term) are evaluated
+// once per cosmology, and each root-finder call only redoes the hydrogen
+// step -- the same operations, in the same order, as a direct evaluation
+struct ReionGrid {
+  static constexpr int n = 2000;
+  std::vector<double> nh, ea, ah, y, xhe;
+  double fHe;
+  void build(const AbcmbCosmo& c, double zHe, double dzHe, double ex) {
+    nh.resize(n); ea.resize(n); ah.resize(n); y.resize(n); xhe.resize(n);
+    fHe = c.YHe / 4.0 / (1.0 - c.YHe);
+    for (int i = 0; i < n; ++i) {
+      double l = -5.0 + 5.0 * i / (n - 1);
+      double z = 1.0 / std::exp(l) - 1.0;
+      nh[i] = nH(c, l);
+      ea[i] = std::exp(l);
+      ah[i] = aH(c, l);
+      y[i] = std::pow(1.0 + z, ex);
+      xhe[i] = fHe / 2.0 * (1.0 + std::tanh((zHe - z) / dzHe));
+    }
   }
-  return s;
-}
+  double tau(double z_re, double dz, double ex) const {
+    double yre = std::pow(1.0 + z_re, ex);
+    double dy = ex * std::pow(1.0 + z_re, ex - 1.0) * dz;
+    double s = 0.0, prev = 0.0;
+    for (int i = 0; i < n; ++i) {
+      double xH = (1.0 + fHe) / 2.0 * (1.0 + std::tanh((yre - y[i]) / dy));
+      double ne = nh[i] * (xH + xhe[i]);
+      double gam = ea[i] * ne * SIGMA_T * C_CM / C_MPC;
+      double f = gam / ah[i];
+      if (i > 0) s += 0.5 * (f + prev) * (5.0 / (n - 1));
+      prev = f;
+    }
+    return s;
+  }
+};
 
 // kappa(lna) = int_lna^0 dl / (tau_c aH): composite GL on a node grid
 struct Kappa {
@@ -405,9 +430,9 @@ extern "C" int abcmb_host_background(AbcmbCosmo* c, double tau_reion, double z_r
   // reionization: solve tau_reion(z) = target (Newton/bisection), or take z
   double zr = z_reion_in;
   if (tau_reion > 0.0) {
-    auto f = [&](double z) {
-      return tau_reion_of(*c, z, Delta_z_reion, z_reion_He, Delta_z_reion_He, exp_reion) - tau_reion;
-    };
+    ReionGrid G;
+    G.build(*c, z_reion_He, Delta_z_reion_He, exp_reion);
+    auto f = [&](double z) { return G.tau(z, Delta_z_reion, exp_reion) - tau_reion; };
     double lo = 0.0, hi = 50.0, flo = f(lo), fhi = f(hi);
     for (int it = 0; it < 8; ++it) {  // bracket
       double mid = 0.5 * (lo + hi), fm = f(mid);
@@ -431,6 +456,9 @@ extern "C" int abcmb_host_background(AbcmbCosmo* c, double tau_reion, double z_r
   for (int j = N; j < c->xe_n; ++j) { xe[j] = xe[N - 1]; tm[j] = tm[N - 1]; }
   c->xe_last = xe[N - 1];
   c->tm_last = tm[N - 1];
+  // a caller that freezes the grids at another cosmology's (both set, as
+  // negative ln a, on input) keeps them and skips the two scans below
+  if (c->lna_rec < 0.0 && c->lna_transfer_start < 0.0) return ABCMB_OK;
   // lna_rec: argmax of the visibility on linspace(-8, -4, 1500)
   Kappa K;
   K.build(*c);

@@ -228,7 +228,9 @@ class Model:
         kk = np.ascontiguousarray(self.k_pt if k is None else k, dtype=np.float64)
         bg = Background(full_params, self.has_ncdm, n_q=self.layout.n_q, k_pivot=sp["k_pivot"],
                         input_tau_reion=sp["input_tau_reion"],
-                        rec_switch=freeze.bg.rec_switch if freeze is not None else -1)
+                        rec_switch=freeze.bg.rec_switch if freeze is not None else -1,
+                        lna_rec=freeze.bg.lna_rec if freeze is not None else None,
+                        lna_transfer_start=freeze.bg.lna_transfer_start if freeze is not None else None)
         if freeze is not None:
             bg.cosmo.lna_rec = freeze.bg.lna_rec
             bg.cosmo.lna_transfer_start = freeze.bg.lna_transfer_start

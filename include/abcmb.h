@@ -127,7 +127,10 @@ typedef struct AbcmbOdeSettings {
  * (in/out, may be NULL): grid index where the hydrogen history leaves Saha
  * equilibrium; < 0 on input lets the producer choose it (and reports it),
  * >= 0 forces it (finite differences hold it fixed, like every other
- * discrete grid choice).
+ * discrete grid choice).  c->lna_rec and c->lna_transfer_start are outputs,
+ * unless both are negative on input: then they are kept (a caller that holds
+ * the grids at another cosmology's passes that cosmology's values) and their
+ * scans are skipped.
  */
 int abcmb_host_background(AbcmbCosmo* c, double tau_reion, double z_reion_in,
                           double Delta_z_reion, double z_reion_He,
