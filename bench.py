@@ -10,8 +10,9 @@ and LOS), Gaussian log-likelihood against a per-cosmology seeded mock.
 One step = the gradients of all ``--batch`` cosmologies: tangent-linear ODE
 launches of ``--chunk`` cosmologies each (perturbation tables and their
 derivatives along omega_b, omega_cdm, h, tau_reio, m_nu), each followed by
-the downstream differenced along all seven directions, with the next chunk's
-host preparation overlapping the GPU work (``--gradient fd`` instead:
+a tangent-linear LOS pass and the lensing differenced along all seven
+directions (``--fd-downstream``: the whole downstream differenced), with the
+next chunk's host preparation overlapping the GPU work (``--gradient fd`` instead:
 central finite differences, 15 full model evaluations per gradient, all in
 one ODE launch).
 
