@@ -936,31 +936,21 @@ __device__ __noinline__ D<N> output_field(const Cosmo<N>& P, const Layout& L, do
 // The perturbation table in place: where the layout's output rows fit the
 // table's field slots (12 + 3 n_q <= 21, i.e. up to three momentum bins) the
 // ODE kernel stores the saved rows themselves, slot s in field slot s, with
-// no workgroup synchronisation; this kernel then turns every (k-mode, saved
-// point) of the launch's items into the table fields, one thread each.
+// no workgroup synchronisation; after the launch every (k-mode, saved point)
+// is turned into the table fields, one thread each (fields_at).  The rows of
+// a saved point are read into registers first, so the fields can overwrite
+// them in any order.
 template <int N>
-__global__ void __launch_bounds__(64) fields_kernel(const AbcmbCosmo* __restrict__ cosmos,
-                                                    const AbcmbTangent* __restrict__ tangents,
-                                                    const AbcmbPertItem* __restrict__ items, int n_items,
-                                                    AbcmbOdeSettings st, const Bg<N>* __restrict__ bgsave,
-                                                    int n_save_max, const int32_t* __restrict__ status) {
-  const long long idx = (long long)blockIdx.x * 64 + threadIdx.x;
-  if (idx >= (long long)n_items * n_save_max) return;
-  const int item = (int)(idx / n_save_max), isave = (int)(idx % n_save_max);
-  if (status[item] != ABCMB_K_OK) return;
-  const AbcmbPertItem I = items[item];
-  const AbcmbCosmo& c = cosmos[I.cosmo];
-  const int n_save = c.n_save;
-  if (isave >= n_save) return;
-  const AbcmbTangent* T = nullptr;
-  if constexpr (N > 0) T = tangents + I.cosmo;
+__device__ __forceinline__ void fields_at(const AbcmbCosmo& c, const AbcmbTangent* T, const AbcmbOdeSettings& st,
+                                          const Bg<N>& bg, double k, int ik, int isave) {
   const Cosmo<N> P{c, T};
   Layout L;
   L.lg = st.l_g; L.lp = st.l_pol; L.lu = st.l_ur; L.ln = st.l_ncdm; L.nq = st.n_q; L.n = st.n;
   L.ig = 4; L.ip = 4 + L.lg + 1; L.iu = L.ip + L.lp + 1; L.inc = L.iu + L.lu + 1;
   const int nr = out_rows(L);
+  const int n_save = c.n_save;
   const size_t dstride = (size_t)ABCMB_N_PT_FIELDS * n_save * c.nk;
-  auto at = [&](int s) { return ((size_t)s * n_save + isave) * c.nk + I.ik; };
+  auto at = [&](int s) { return ((size_t)s * n_save + isave) * c.nk + ik; };
   RowsReg<N> y;
 #pragma unroll
   for (int s = 0; s < 27; ++s) {
@@ -971,11 +961,59 @@ __global__ void __launch_bounds__(64) fields_kernel(const AbcmbCosmo* __restrict
       if constexpr (N > 0) { ABCMB_DLOOP(N) y.Y[s].d[i_] = T->pt[i_ * dstride + o]; }
     }
   }
-  all_fields(P, L, I.k, bgsave[(size_t)I.cosmo * n_save_max + isave], y, [&](int f, const D<N>& v) {
+  all_fields(P, L, k, bg, y, [&](int f, const D<N>& v) {
     const size_t o = at(f);
     c.pt[o] = v.v;
     if constexpr (N > 0) { ABCMB_DLOOP(N) T->pt[i_ * dstride + o] = v.d[i_]; }
   });
+}
+
+// how many of a launch's items fall on each cosmology (cnt zeroed first by a
+// fill kernel, so that the sequence can sit in a captured graph)
+__global__ void zero_counts_kernel(int32_t* __restrict__ cnt, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) cnt[i] = 0;
+}
+__global__ void count_items_kernel(const AbcmbPertItem* __restrict__ items, int n_items, int32_t* __restrict__ cnt) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n_items) atomicAdd(&cnt[items[i].cosmo], 1);
+}
+
+// cosmologies whose every k-mode is in the launch (items are distinct
+// (cosmology, k) pairs): one block per (cosmology, saved point), threads along
+// k, so that the table's k-contiguous rows are read and written coalesced
+template <int N>
+__global__ void __launch_bounds__(64) fields_cosmo_kernel(const AbcmbCosmo* __restrict__ cosmos,
+                                                          const AbcmbTangent* __restrict__ tangents,
+                                                          AbcmbOdeSettings st, const Bg<N>* __restrict__ bgsave,
+                                                          int n_save_max, const int32_t* __restrict__ cnt) {
+  const int ci = blockIdx.x / n_save_max, isave = blockIdx.x % n_save_max;
+  const AbcmbCosmo& c = cosmos[ci];
+  if (isave >= c.n_save || cnt[ci] != c.nk) return;
+  const AbcmbTangent* T = nullptr;
+  if constexpr (N > 0) T = tangents + ci;
+  const Bg<N>& bg = bgsave[(size_t)ci * n_save_max + isave];
+  for (int ik = threadIdx.x; ik < c.nk; ik += 64) fields_at<N>(c, T, st, bg, c.k[ik], ik, isave);
+}
+
+// the other items (cosmologies covered in part), one thread per (item, saved point)
+template <int N>
+__global__ void __launch_bounds__(64) fields_kernel(const AbcmbCosmo* __restrict__ cosmos,
+                                                    const AbcmbTangent* __restrict__ tangents,
+                                                    const AbcmbPertItem* __restrict__ items, int n_items,
+                                                    AbcmbOdeSettings st, const Bg<N>* __restrict__ bgsave,
+                                                    int n_save_max, const int32_t* __restrict__ status,
+                                                    const int32_t* __restrict__ cnt) {
+  const long long idx = (long long)blockIdx.x * 64 + threadIdx.x;
+  if (idx >= (long long)n_items * n_save_max) return;
+  const int item = (int)(idx / n_save_max), isave = (int)(idx % n_save_max);
+  if (status[item] != ABCMB_K_OK) return;
+  const AbcmbPertItem I = items[item];
+  const AbcmbCosmo& c = cosmos[I.cosmo];
+  if (isave >= c.n_save || cnt[I.cosmo] == c.nk) return;
+  const AbcmbTangent* T = nullptr;
+  if constexpr (N > 0) T = tangents + I.cosmo;
+  fields_at<N>(c, T, st, bgsave[(size_t)I.cosmo * n_save_max + isave], I.k, I.ik, isave);
 }
 
 // the background (with its tangents) at every saved point of every cosmology,
@@ -1222,16 +1260,20 @@ pert_fwd_kernel(const AbcmbCosmo* __restrict__ cosmos, const AbcmbTangent* __res
 }
 
 template <int NT, int BMAX, int N>
-int launch(const AbcmbCosmo* cosmos, const AbcmbTangent* tangents, const AbcmbPertItem* items, int n_items,
-           const AbcmbOdeSettings& st, const Bg<N>* bgsave, int n_save_max, int32_t* status, int32_t* steps,
-           int32_t* iters, double* state, hipStream_t stream) {
+int launch(const AbcmbCosmo* cosmos, const AbcmbTangent* tangents, int n_cosmo, const AbcmbPertItem* items,
+           int n_items, const AbcmbOdeSettings& st, const Bg<N>* bgsave, int n_save_max, int32_t* cnt,
+           int32_t* status, int32_t* steps, int32_t* iters, double* state, hipStream_t stream) {
   hipLaunchKernelGGL((pert_fwd_kernel<NT, BMAX, N>), dim3(n_items), dim3(NT), 0, stream, cosmos, tangents, items,
                      st, bgsave, n_save_max, status, steps, iters, state);
   ABCMB_HIP_CHECK(hipGetLastError());
   if (12 + 3 * st.n_q <= ABCMB_N_PT_FIELDS) {  // the kernel stored rows: turn them into fields
+    hipLaunchKernelGGL(zero_counts_kernel, dim3((n_cosmo + 255) / 256), dim3(256), 0, stream, cnt, n_cosmo);
+    hipLaunchKernelGGL(count_items_kernel, dim3((n_items + 255) / 256), dim3(256), 0, stream, items, n_items, cnt);
+    hipLaunchKernelGGL((fields_cosmo_kernel<N>), dim3((unsigned)n_cosmo * n_save_max), dim3(64), 0, stream, cosmos,
+                       tangents, st, bgsave, n_save_max, cnt);
     const long long nthr = (long long)n_items * n_save_max;
     hipLaunchKernelGGL((fields_kernel<N>), dim3((unsigned)((nthr + 63) / 64)), dim3(64), 0, stream, cosmos, tangents,
-                       items, n_items, st, bgsave, n_save_max, status);
+                       items, n_items, st, bgsave, n_save_max, status, cnt);
     ABCMB_HIP_CHECK(hipGetLastError());
   }
   return ABCMB_OK;
@@ -1252,16 +1294,25 @@ int check_settings(const AbcmbOdeSettings& st, int& maxb) {
   return ABCMB_OK;
 }
 
+// the workspace: the saved-point background table, then one item count per
+// cosmology at the next 256-byte boundary
+size_t bg_table_bytes(size_t per, int n_cosmo, int n_save_max) {
+  const size_t b = per * (size_t)(n_cosmo > 0 ? n_cosmo : 0) * (size_t)(n_save_max > 0 ? n_save_max : 0);
+  return (b + 255) / 256 * 256;
+}
+
 template <int N>
 int dispatch(const AbcmbCosmo* cosmos, const AbcmbTangent* tangents, int n_cosmo, const AbcmbPertItem* items,
              int n_items, const AbcmbOdeSettings& st, int maxb, int n_save_max, void* workspace, int32_t* status,
              int32_t* steps, int32_t* iters, double* state, hipStream_t s) {
   Bg<N>* bgs = (Bg<N>*)workspace;
+  int32_t* cnt = (int32_t*)((char*)workspace + bg_table_bytes(sizeof(Bg<N>), n_cosmo, n_save_max));
   hipLaunchKernelGGL((bg_save_kernel<N>), dim3((n_save_max + 63) / 64, n_cosmo), dim3(64), 0, s, cosmos, tangents,
                      n_save_max, bgs);
   ABCMB_HIP_CHECK(hipGetLastError());
 #define ABCMB_LAUNCH(NT_, BM_) \
-  return launch<NT_, BM_, N>(cosmos, tangents, items, n_items, st, bgs, n_save_max, status, steps, iters, state, s)
+  return launch<NT_, BM_, N>(cosmos, tangents, n_cosmo, items, n_items, st, bgs, n_save_max, cnt, status, steps, \
+                             iters, state, s)
   // 26 rows: the photon block of the default cut-offs (l_max_g 12, l_max_pol_g 10)
   if (maxb <= 26) {
     if (st.n <= 64) ABCMB_LAUNCH(64, 26);
@@ -1285,7 +1336,7 @@ int dispatch(const AbcmbCosmo* cosmos, const AbcmbTangent* tangents, int n_cosmo
 extern "C" size_t abcmb_pert_workspace_size(int n_cosmo, int n_save_max, int n_dir) {
   using namespace abcmb;
   const size_t per = n_dir > 0 ? sizeof(Bg<ABCMB_TANGENT_DIRS>) : sizeof(Bg<0>);
-  return per * (size_t)(n_cosmo > 0 ? n_cosmo : 0) * (size_t)(n_save_max > 0 ? n_save_max : 0);
+  return bg_table_bytes(per, n_cosmo, n_save_max) + sizeof(int32_t) * (size_t)(n_cosmo > 0 ? n_cosmo : 0);
 }
 
 extern "C" int abcmb_pert_fwd(const AbcmbCosmo* cosmos_dev, int n_cosmo, const AbcmbPertItem* items_dev,

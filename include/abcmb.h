@@ -158,16 +158,19 @@ int abcmb_device_init(void);
 /*
  * Perturbation ODE for every work item: adaptive ESDIRK (Kvaerno5) with
  * PID control and chord Newton in FP64, one k-mode per workgroup, dense
- * output at the cosmology's n_save points, fused output-table assembly.
+ * output at the cosmology's n_save points, output-table assembly (in the
+ * kernel, or from the saved state rows by a second pass on the same stream).
  * Replaces PerturbationEvolver.full_evolution (abcmb/perturbations.py:75-117)
  * with evolution_one_k (:250-316) and make_output_table (:318-404).
- * cosmos_dev: n_cosmo device-resident AbcmbCosmo; items_dev: n_items;
+ * cosmos_dev: n_cosmo device-resident AbcmbCosmo; items_dev: n_items
+ * distinct (cosmology, k) pairs;
  * status_dev/steps_dev: int32[n_items] outputs (status, accepted+rejected
  * steps); iters_dev (optional): int32[n_items] chord-Newton iterations;
  * state_dev: optional [n_items][n_save][n] raw states (NULL = off).
  * n_save_max: upper bound of the cosmologies' n_save; the workspace
  * (abcmb_pert_workspace_size(n_cosmo, n_save_max, 0) bytes) holds the
- * background at every saved point, computed once per cosmology.
+ * background at every saved point, computed once per cosmology, and a count
+ * of the launch's items per cosmology.
  */
 size_t abcmb_pert_workspace_size(int n_cosmo, int n_save_max, int n_dir);
 int abcmb_pert_fwd(const AbcmbCosmo* cosmos_dev, int n_cosmo,
