@@ -214,3 +214,53 @@ def test_max_steps_raises():
 def test_native_library_is_the_path():
     from abcmb_amd import abi
     assert abi._hip is not None and abi._hip._name.endswith("libabcmb_hip.so")
+
+
+def test_partial_k_launch_matches_full():
+    """A launch holding only some k-modes of a cosmology (items are any distinct
+    (cosmology, k) pairs) gives those k-modes' table columns bit for bit as the
+    launch of all of them: the saved-row transform takes its item-wise pass for
+    the partly covered cosmology and its coalesced per-cosmology pass for the
+    other, fully covered one; values and the five tangent directions."""
+    import ctypes as C
+    import torch
+    from abcmb_amd import abi
+    from abcmb_amd import likelihood as LK
+    from abcmb_amd.model import Model, MassiveNeutrino
+    m = Model(user_species=(MassiveNeutrino,), lensing=False, l_max=500,
+              k_axis_perturbations=np.geomspace(1e-4, 0.3, 12))
+    thetas = LK.latin_hypercube(2, seed=11)
+    steps = LK.FD_REL_STEP * np.abs(thetas)
+    plans, pm, seeds = LK._tangent_inputs(m, thetas, [None, None], steps)
+    _say("case: partial-k launch (cosmology 0: k 1, 4, 7, 10; cosmology 1: all) vs full launch")
+    full = m._run_ode(plans, tangents=seeds)
+    lib = abi.hip_lib()
+    _, dev, keep, dt = m._staging()
+    ns, nd, nk = m.n_save, abi.N_TANGENT_DIRS, m.k_pt.size
+    kdev = [dt(m.k_pt), dt(m.k_pt)]
+    pts = [torch.full((abi.N_PT_FIELDS, ns, nk), np.nan, dtype=torch.float64, device=dev) for _ in range(2)]
+    dpts = [torch.full((nd, abi.N_PT_FIELDS, ns, nk), np.nan, dtype=torch.float64, device=dev) for _ in range(2)]
+    cos_dev = m._struct_array(torch, dev, m._cosmo_array(plans, pts, kdev, dt), dt)
+    tans = [abi.AbcmbTangent(par=dt(sd.par).data_ptr(), tau_tab=dt(sd.tau).data_ptr(), xe_tab=dt(sd.xe).data_ptr(),
+                             tm_tab=dt(sd.tm).data_ptr(), pt=dp.data_ptr()) for sd, dp in zip(seeds, dpts)]
+    tan_dev = m._struct_array(torch, dev, tans, dt)
+    sub = [1, 4, 7, 10]
+    pairs = [(0, ik) for ik in sub] + [(1, ik) for ik in range(nk)]
+    items = [abi.AbcmbPertItem(k=m.k_pt[ik], lna_start=plans[ci].lna_start[ik], rtol=plans[ci].rtol[ik],
+                               atol=plans[ci].atol[ik], cosmo=ci, ik=ik) for ci, ik in pairs]
+    items_dev = m._struct_array(torch, dev, items, dt)
+    n = len(items)
+    status = torch.zeros(n, dtype=torch.int32, device=dev)
+    stp = torch.zeros(n, dtype=torch.int32, device=dev)
+    ws = torch.empty(lib.abcmb_pert_workspace_size(2, ns, nd) // 8 + 1, dtype=torch.float64, device=dev)
+    abi.check(lib.abcmb_pert_tangent(cos_dev.data_ptr(), tan_dev.data_ptr(), 2, items_dev.data_ptr(), n,
+                                     C.byref(m.settings), ns, status.data_ptr(), stp.data_ptr(), None,
+                                     ws.data_ptr(), ws.numel() * 8, torch.cuda.current_stream().cuda_stream),
+              "abcmb_pert_tangent")
+    torch.cuda.synchronize()
+    assert int(status.abs().max()) == 0
+    assert torch.equal(pts[0][:, :, sub], full["pts"][0][:, :, sub])
+    assert torch.equal(dpts[0][..., sub], full["dpts"][0][..., sub])
+    assert torch.equal(pts[1], full["pts"][1]) and torch.equal(dpts[1], full["dpts"][1])
+    others = [ik for ik in range(nk) if ik not in sub]
+    assert torch.isnan(pts[0][:, :, others]).all()      # untouched columns stay as they were
