@@ -6,7 +6,9 @@
 // Kvaerno5 ESDIRK method, PID step control and chord Newton
 // (abcmb/perturbations.py:250-316), writes cubic-Hermite dense output at the
 // cosmology's saved ln a points and assembles the perturbation-table fields
-// in the same pass (abcmb/perturbations.py:318-404).
+// (abcmb/perturbations.py:318-404): from the saved state rows in a second
+// pass (fields_cosmo_kernel / fields_kernel) where the rows fit the table's
+// slots, otherwise in the kernel at each saved point.
 //
 // Layout on the workgroup (NT threads, one state row per thread, n <= NT):
 //   - the right-hand side is linear, y' = A(ln a) y.  Each row keeps a
@@ -18,11 +20,13 @@
 //     diagonal (one block per Boltzmann hierarchy) plus the rank-2 metric
 //     coupling, so each thread inverts its row of the small diagonal block
 //     in registers and M^-1 is applied with the Woodbury identity;
-//   - the background scalars at the step's six abscissae are evaluated in
+//   - the background scalars at the step's seven times are evaluated in
 //     parallel, one lane per time, once per step;
-//   - stage derivatives, state and error live in registers of the row's
-//     thread; workgroup sums (metric sums, error norms, pivot search) use
-//     wavefront shuffles plus one LDS slot per wavefront.
+//   - state and error live in registers of the row's thread, the stage
+//     derivatives in an LDS column per thread; workgroup sums (metric sums,
+//     Woodbury sums, Newton and error norms) are transposed wavefront sums
+//     (half-wave swaps, then DPP within rows) plus one LDS slot per
+//     (value, wavefront) and one barrier.
 #include <cmath>
 #include <cstddef>
 #include <cstdint>
