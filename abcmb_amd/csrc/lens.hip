@@ -16,6 +16,7 @@
 #include <cstdint>
 
 #include "common.h"
+#include "wave.h"
 
 namespace abcmb {
 namespace {
@@ -107,8 +108,10 @@ __constant__ int PAIR_N[12] = {0, 1, -1, -2, 2, 1, 0, -3, -4, 0, -1, -2};
 constexpr int NPAIR = 12;
 
 // coef[q][l][3] = (A, B, C) of d^{l+1} = (A mu + B) d^l + C d^{l-1} in the
-// normalised recursion; lfac[l][5] = sqrt(2/(2l+1)) and the four X-function
-// square roots.  l = 0 .. LT-1 with LT = n_l + 3.
+// normalised recursion; lfac[l][LFAC] = sqrt(2/(2l+1)), the four X-function
+// square roots, 8 / (l(l+1)) and 2 / sqrt(l(l+1)) (0 at l = 0).  l = 0 .. LT-1
+// with LT = n_l + 3.
+constexpr int LFAC = 7;
 __global__ void wigner_coef_kernel(int LT, double* __restrict__ coef, double* __restrict__ lfac) {
   int idx = blockIdx.x * blockDim.x + threadIdx.x;
   if (idx < NPAIR * LT) {
@@ -127,11 +130,14 @@ __global__ void wigner_coef_kernel(int LT, double* __restrict__ coef, double* __
   }
   if (idx < LT) {
     double l = idx;
-    lfac[idx * 5 + 0] = sqrt(2. / (2. * l + 1.));
-    lfac[idx * 5 + 1] = sqrt(fmax((l + 2) * (l - 1) * l * (l + 1), 0.0));
-    lfac[idx * 5 + 2] = sqrt(fmax((l + 2) * (l - 1), 0.0));
-    lfac[idx * 5 + 3] = sqrt(fmax((l + 3) * (l - 2), 0.0));
-    lfac[idx * 5 + 4] = sqrt(fmax((l + 4) * (l + 3) * (l - 2) * (l - 3), 0.0));
+    double* lf = lfac + idx * LFAC;
+    lf[0] = sqrt(2. / (2. * l + 1.));
+    lf[1] = sqrt(fmax((l + 2) * (l - 1) * l * (l + 1), 0.0));
+    lf[2] = sqrt(fmax((l + 2) * (l - 1), 0.0));
+    lf[3] = sqrt(fmax((l + 3) * (l - 2), 0.0));
+    lf[4] = sqrt(fmax((l + 4) * (l + 3) * (l - 2) * (l - 3), 0.0));
+    lf[5] = idx > 0 ? 8. / l / (l + 1) : 0.0;
+    lf[6] = idx > 0 ? 2. / sqrt(l * (l + 1)) : 0.0;
   }
 }
 
@@ -165,7 +171,7 @@ __device__ __forceinline__ void rec_step(Rec& r, const double* __restrict__ cq, 
 }
 
 // pass 1: Cgl(mu), Cgl2(mu)
-__global__ void cgl_kernel(const double* __restrict__ clpp_all, int n_l, const double* __restrict__ mu_arr, int n_mu,
+__global__ void __launch_bounds__(64) cgl_kernel(const double* __restrict__ clpp_all, int n_l, const double* __restrict__ mu_arr, int n_mu,
                            const double* __restrict__ coef, const double* __restrict__ lfac, int LT,
                            double* __restrict__ cgl_all) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -182,7 +188,7 @@ __global__ void cgl_kernel(const double* __restrict__ clpp_all, int n_l, const d
   double s1 = 0.0, s2 = 0.0;
   for (int j = 0; j < n_l; ++j) {
     int l = j + 2;
-    double nf = lfac[l * 5];
+    double nf = lfac[l * LFAC];
     double f = (2. * l + 1) * l * (l + 1) * clpp[j];
     s1 += f * (a.dl * nf);
     s2 += f * (b.dl * nf);
@@ -194,7 +200,7 @@ __global__ void cgl_kernel(const double* __restrict__ clpp_all, int n_l, const d
 }
 
 // pass 2: correlation functions
-__global__ void ksi_kernel(const double* __restrict__ cl_all, const double* __restrict__ cgl_all, int n_l,
+__global__ void __launch_bounds__(64) ksi_kernel(const double* __restrict__ cl_all, const double* __restrict__ cgl_all, int n_l,
                            const double* __restrict__ mu_arr, int n_mu, const double* __restrict__ coef,
                            const double* __restrict__ lfac, int LT, double* __restrict__ ksi_all) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -225,6 +231,10 @@ __global__ void ksi_kernel(const double* __restrict__ cl_all, const double* __re
   R[9] = Rec{wigner_start(2, 0, mu), 0.0};
   double e2 = exp(2. * s2 / 4.), e4 = exp(4. * s2 / 4.), e83 = exp(8. / 3. * s2 / 4.);
   double e203 = exp(20. / 3. * s2 / 4.), e10 = exp(10. * s2 / 4.);
+  // E(l) = exp(-l(l+1) s2 / 4) by products: E(l+1) = E(l) r(l) with
+  // r(l) = g^(l+1), g = exp(-s2 / 2)
+  const double g = exp(-0.5 * s2);
+  double E = exp(-1.5 * s2), r = g * g * g;
   double a0 = 0, ap = 0, am = 0, ax = 0;
   for (int j = 0; j < n_l; ++j) {
     int l = j + 2;
@@ -238,14 +248,13 @@ __global__ void ksi_kernel(const double* __restrict__ cl_all, const double* __re
       R[8] = Rec{wigner_start(4, -4, mu), 0.0};
       R[11] = Rec{wigner_start(4, -2, mu), 0.0};
     }
-    const double* lf = lfac + l * 5;
+    const double* lf = lfac + l * LFAC;
     double nf = lf[0];
     double d[NPAIR];
 #pragma unroll
     for (int q = 0; q < NPAIR; ++q) d[q] = R[q].dl * nf;
     double dl = l;
     double llp1 = dl * (dl + 1);
-    double E = exp(-llp1 * s2 / 4);
     double X000 = E;
     double X000p = -llp1 / 4. * X000;
     double X220 = 1. / 4. * lf[1] * E * e2;
@@ -257,17 +266,24 @@ __global__ void ksi_kernel(const double* __restrict__ cl_all, const double* __re
     double d00 = d[0], d11 = d[1], d1m1 = d[2], d2m2 = d[3], d22 = d[4], d31 = d[5];
     double d40 = d[6], d3m3 = d[7], d4m4 = d[8], d20 = d[9], d3m1 = d[10], d4m2 = d[11];
     double pre = 2. * dl + 1;
-    a0 += pre * TT[j] * (X000 * X000 * d00 + 8. / dl / (dl + 1) * C2 * X000p * X000p * d1m1 +
+    a0 += pre * TT[j] * (X000 * X000 * d00 + lf[5] * C2 * X000p * X000p * d1m1 +
                          C2 * C2 * (X000p * X000p * d00 + X220 * X220 * d2m2));
     ap += pre * EE[j] * (X022 * X022 * d22 + 2 * C2 * X132 * X121 * d31 +
                          C2 * C2 * (X022p * X022p * d22 + X242 * X220 * d40));
     am += pre * EE[j] * (X022 * X022 * d2m2 + C2 * (X121 * X121 * d1m1 + X132 * X132 * d3m3) +
                          1. / 2. * C2 * C2 * (2 * X022p * X022p * d2m2 + X220 * X220 * d00 + X242 * X242 * d4m4));
-    ax += pre * TE[j] * (X022 * X000 * d20 + C2 * 2 * X000p / sqrt(llp1) * (X121 * d11 + X132 * d3m1) +
+    ax += pre * TE[j] * (X022 * X000 * d20 + C2 * lf[6] * X000p * (X121 * d11 + X132 * d3m1) +
                          1. / 2. * C2 * C2 * ((2 * X022p * X000p + X220 * X220) * d20 + X220 * X242 * d4m2));
+    E *= r;
+    r *= g;
+    if (l >= 4) {
 #pragma unroll
-    for (int q = 0; q < NPAIR; ++q)
-      if (l >= PAIR_M[q]) rec_step(R[q], coef + (size_t)q * LT * 3, l, mu);
+      for (int q = 0; q < NPAIR; ++q) rec_step(R[q], coef + (size_t)q * LT * 3, l, mu);
+    } else {
+#pragma unroll
+      for (int q = 0; q < NPAIR; ++q)
+        if (l >= PAIR_M[q]) rec_step(R[q], coef + (size_t)q * LT * 3, l, mu);
+    }
   }
   ksi[0 * n_mu + i] = a0 / 4. / PI;
   ksi[1 * n_mu + i] = ap / 4. / PI;
@@ -275,51 +291,66 @@ __global__ void ksi_kernel(const double* __restrict__ cl_all, const double* __re
   ksi[3 * n_mu + i] = ax / 4. / PI;
 }
 
-// pass 3: per-block partial back-transform sums, part[blk][4][n_l]
+// pass 3: per-block partial back-transform sums, part[blk][4][n_l].  A
+// thread carries BACK_MU nodes mu (i, i + 64, ...), so each wavefront sum
+// over the block covers 64 BACK_MU nodes and the recursions of the nodes are
+// independent work between the sums.
+constexpr int BACK_MU = 4;
 __global__ void __launch_bounds__(64) back_kernel(const double* __restrict__ ksi_all, int n_l,
                                                   const double* __restrict__ mu_arr, const double* __restrict__ w_arr,
                                                   int n_mu, const double* __restrict__ coef,
                                                   const double* __restrict__ lfac, int LT, double* __restrict__ part_all) {
-  int i = blockIdx.x * 64 + threadIdx.x;
   int ci = blockIdx.y;
   const double* ksi = ksi_all + (size_t)ci * 4 * n_mu;
   double* part = part_all + (size_t)ci * gridDim.x * 4 * n_l;
-  bool live = i < n_mu;
-  double mu = live ? mu_arr[i] : 0.0;
-  double w = live ? w_arr[i] : 0.0;
-  double k0 = live ? ksi[0 * n_mu + i] * w : 0.0, kp = live ? ksi[1 * n_mu + i] * w : 0.0;
-  double km = live ? ksi[2 * n_mu + i] * w : 0.0, kx = live ? ksi[3 * n_mu + i] * w : 0.0;
-  Rec r00{wigner_start(0, 0, mu), 0.0}, r20{wigner_start(2, 0, mu), 0.0};
-  Rec r22{wigner_start(2, 2, mu), 0.0}, r2m2{wigner_start(2, -2, mu), 0.0};
+  double mu[BACK_MU], k0[BACK_MU], kp[BACK_MU], km[BACK_MU], kx[BACK_MU];
+  Rec r00[BACK_MU], r20[BACK_MU], r22[BACK_MU], r2m2[BACK_MU];
   const double* c00 = coef;
   const double* c2m2 = coef + (size_t)3 * LT * 3;
   const double* c22 = coef + (size_t)4 * LT * 3;
   const double* c20 = coef + (size_t)9 * LT * 3;
-  rec_step(r00, c00, 0, mu);
-  rec_step(r00, c00, 1, mu);
+#pragma unroll
+  for (int m = 0; m < BACK_MU; ++m) {
+    const int i = blockIdx.x * 64 * BACK_MU + m * 64 + threadIdx.x;
+    const bool live = i < n_mu;
+    mu[m] = live ? mu_arr[i] : 0.0;
+    const double w = live ? w_arr[i] : 0.0;
+    k0[m] = live ? ksi[0 * n_mu + i] * w : 0.0;
+    kp[m] = live ? ksi[1 * n_mu + i] * w : 0.0;
+    km[m] = live ? ksi[2 * n_mu + i] * w : 0.0;
+    kx[m] = live ? ksi[3 * n_mu + i] * w : 0.0;
+    r00[m] = Rec{wigner_start(0, 0, mu[m]), 0.0};
+    r20[m] = Rec{wigner_start(2, 0, mu[m]), 0.0};
+    r22[m] = Rec{wigner_start(2, 2, mu[m]), 0.0};
+    r2m2[m] = Rec{wigner_start(2, -2, mu[m]), 0.0};
+    rec_step(r00[m], c00, 0, mu[m]);
+    rec_step(r00[m], c00, 1, mu[m]);
+  }
   double* P = part + (size_t)blockIdx.x * 4 * n_l;
   for (int j = 0; j < n_l; ++j) {
     int l = j + 2;
-    double nf = lfac[l * 5];
-    double d00 = r00.dl * nf, d20 = r20.dl * nf, d22 = r22.dl * nf, d2m2 = r2m2.dl * nf;
-    double vt = k0 * d00, vx = kx * d20, ve = kp * d22 + km * d2m2, vb = kp * d22 - km * d2m2;
+    double nf = lfac[l * LFAC];
+    double vt = 0.0, vx = 0.0, ve = 0.0, vb = 0.0;
 #pragma unroll
-    for (int o = 32; o > 0; o >>= 1) {
-      vt += __shfl_xor(vt, o, 64);
-      vx += __shfl_xor(vx, o, 64);
-      ve += __shfl_xor(ve, o, 64);
-      vb += __shfl_xor(vb, o, 64);
+    for (int m = 0; m < BACK_MU; ++m) {
+      double d00 = r00[m].dl * nf, d20 = r20[m].dl * nf, d22 = r22[m].dl * nf, d2m2 = r2m2[m].dl * nf;
+      vt += k0[m] * d00;
+      vx += kx[m] * d20;
+      ve += kp[m] * d22 + km[m] * d2m2;
+      vb += kp[m] * d22 - km[m] * d2m2;
     }
-    if (threadIdx.x == 0) {
-      P[0 * n_l + j] = vt;
-      P[1 * n_l + j] = vx;
-      P[2 * n_l + j] = ve;
-      P[3 * n_l + j] = vb;
+    // transposed wavefront sums: row rho of w holds the total of value rho
+    const double v4[4] = {vt, vx, ve, vb};
+    double w[RowSums<4>::H2];
+    wave_sum_rows<4>(v4, w);
+    if ((threadIdx.x & 15) == 0) P[(threadIdx.x >> 4) * n_l + j] = w[0];
+#pragma unroll
+    for (int m = 0; m < BACK_MU; ++m) {
+      rec_step(r00[m], c00, l, mu[m]);
+      rec_step(r20[m], c20, l, mu[m]);
+      rec_step(r22[m], c22, l, mu[m]);
+      rec_step(r2m2[m], c2m2, l, mu[m]);
     }
-    rec_step(r00, c00, l, mu);
-    rec_step(r20, c20, l, mu);
-    rec_step(r22, c22, l, mu);
-    rec_step(r2m2, c2m2, l, mu);
   }
 }
 
@@ -355,7 +386,7 @@ static size_t lens_ws_parts(int n_cosmo, int n_mu, int n_l, size_t* off) {
   int LT = n_l + 3;
   size_t o = 0;
   off[0] = o; o += (size_t)NPAIR * LT * 3;                               // coef
-  off[1] = o; o += (size_t)LT * 5;                                       // lfac
+  off[1] = o; o += (size_t)LT * LFAC;                                     // lfac
   off[2] = o; o += (size_t)n_cosmo * 2 * n_mu;                           // cgl
   off[3] = o; o += (size_t)n_cosmo * 4 * n_mu;                           // ksi
   off[4] = o; o += (size_t)n_cosmo * ((n_mu + 63) / 64) * 4 * n_l;       // partials
@@ -389,9 +420,10 @@ extern "C" int abcmb_lens(int n_cosmo, const double* cl_in_dev, const double* cl
                      cgl);
   hipLaunchKernelGGL(ksi_kernel, dim3(nblk, n_cosmo), dim3(64), 0, s, cl_in_dev, cgl, n_l, mu_dev, n_mu, coef, lfac,
                      LT, ksi);
-  hipLaunchKernelGGL(back_kernel, dim3(nblk, n_cosmo), dim3(64), 0, s, ksi, n_l, mu_dev, w_dev, n_mu, coef, lfac, LT,
+  const int nblk_back = (n_mu + 64 * BACK_MU - 1) / (64 * BACK_MU);  // <= nblk: fits the part buffer
+  hipLaunchKernelGGL(back_kernel, dim3(nblk_back, n_cosmo), dim3(64), 0, s, ksi, n_l, mu_dev, w_dev, n_mu, coef, lfac, LT,
                      part);
-  hipLaunchKernelGGL(back_final_kernel, dim3((4 * n_l + 255) / 256, n_cosmo), dim3(256), 0, s, part, nblk, n_l,
+  hipLaunchKernelGGL(back_final_kernel, dim3((4 * n_l + 255) / 256, n_cosmo), dim3(256), 0, s, part, nblk_back, n_l,
                      out_dev);
   ABCMB_HIP_CHECK(hipGetLastError());
   return ABCMB_OK;
