@@ -82,7 +82,7 @@ class LensedClLikelihood:
     def loglike_cl(self, cl, mock_index):
         """L for spectra cl[B][3][n_l] (TT, TE, EE on l = 2..l_max) against mocks ``mock_index``."""
         import torch
-        idx = torch.as_tensor(np.asarray(mock_index, dtype=np.int64), device=cl.device)
+        idx = torch.as_tensor(np.asarray(mock_index, dtype=np.int64)).pin_memory().to(cl.device, non_blocking=True)
         return -0.5 * torch.sum((cl - self.mock[idx]) ** 2 * self.inv_var[idx], dim=(1, 2))
 
 
@@ -156,13 +156,14 @@ def _tangent_inputs(model, thetas, k_grids, steps):
 
 
 def _directional_losses(model, like, thetas, steps, base_plans, pm, outs, dpts, mock_index):
-    """L at theta +- step along all seven directions ([nc][7][2]): the downstream on
-    tables moved along their tangents (ODE directions) or with another primordial
-    spectrum (ln A_s, n_s)."""
+    """L at theta +- step along all seven directions ([nc][7][2], on the GPU): the
+    downstream on tables moved along their tangents (ODE directions) or with
+    another primordial spectrum (ln A_s, n_s)."""
+    import torch
     nc = thetas.shape[0]
     per = 2 * 7
     group = max(1, model.chunk // per)
-    Lp = np.zeros((nc, 7, 2))
+    Lp = []
     for c0 in range(0, nc, group):
         cs = range(c0, min(nc, c0 + group))
         plans, pts, kdev, slots = [], [], [], []
@@ -185,12 +186,12 @@ def _directional_losses(model, like, thetas, steps, base_plans, pm, outs, dpts, 
                     kdev.append(kd)
                     slots.append(mock_index[c])
         ds = model._downstream(plans, pts, kdev)
-        Lp[c0:c0 + len(cs)] = like.loglike_cl(model.observed_cl(ds), slots).cpu().numpy().reshape(len(cs), 7, 2)
-    return Lp
+        Lp.append(like.loglike_cl(model.observed_cl(ds), slots).reshape(len(cs), 7, 2))
+    return torch.cat(Lp)
 
 
 def _directional_losses_linear(model, like, thetas, steps, base_plans, pm, outs, dpts, mock_index):
-    """L at theta +- step along all seven directions ([nc][7][2]) with the unlensed
+    """L at theta +- step along all seven directions ([nc][7][2], on the GPU) with the unlensed
     C_l moved along their derivatives from the tangent-linear LOS
     (``Model.downstream_tangent``); only C_l^phiphi and the lensing are
     evaluated at theta +- step."""
@@ -205,44 +206,51 @@ def _directional_losses_linear(model, like, thetas, steps, base_plans, pm, outs,
     sd = [[steps[c, p] for p in ODE_DIRS] for c in range(nc)]
     cl_t = model.downstream_tangent(base_plans, pts, kdev, dpts, plus, minus, sd)   # [nc][1+nd+2][3][nl]
     lensing = bool(model.specs["lensing"])
+    from .model import _moved_tables
+    _, _, stage = model._staging()
+    # the column of cl_t that holds d C / d theta_p
+    col = [1 + ODE_DIRS.index(p) if p in ODE_DIRS else (1 + nd if NAMES[p] == "ln10As" else 2 + nd)
+           for p in range(7)]
+    H = np.stack([steps, -steps], axis=-1)                       # [nc][7][2]: +h, -h
+    H_d = stage.add(H)
+    hs_d = stage.add(H[:, list(ODE_DIRS)])                       # [nc][nd][2]
+    col_d = stage.add(np.array(col), torch.int64)
+    stage.flush()
+    # C(theta +- h e_p) = C + (+-h) dC/dp for every cosmology, direction and sign: [nc][7][2][3][nl]
+    cls_all = torch.addcmul(cl_t[:, 0][:, None, None], H_d[:, :, :, None, None],
+                            cl_t.index_select(1, col_d)[:, :, None])
     per = 2 * 7
     group = max(1, model.chunk // per)
-    Lp = np.zeros((nc, 7, 2))
+    Lp = []
     for c0 in range(0, nc, group):
         cs = range(c0, min(nc, c0 + group))
-        plans, vpts, vk, cls, slots = [], [], [], [], []
+        plans, vpts, vk, slots = [], [], [], []
         for c in cs:
+            moved = _moved_tables(torch, pts[c], dpts[c], hs_d[c]) if lensing else None
             for p in range(7):
-                if p in ODE_DIRS:
-                    dC = cl_t[c, 1 + ODE_DIRS.index(p)]
-                elif NAMES[p] == "ln10As":
-                    dC = cl_t[c, 1 + nd]        # d C / d ln A_s
-                else:
-                    dC = cl_t[c, 2 + nd]        # d C / d n_s
                 for si, sgn in enumerate((+1, -1)):
-                    h = sgn * steps[c, p]
-                    cls.append(cl_t[c, 0] + h * dC)
                     slots.append(mock_index[c])
                     if not lensing:
                         continue
                     if p in ODE_DIRS:
                         d = ODE_DIRS.index(p)
                         plans.append(pm[c][2 * d + si])
-                        vpts.append(pts[c] + h * dpts[c][d])
+                        vpts.append(moved[2 * d + si])
                     else:
                         t = thetas[c].copy()
-                        t[p] += h
+                        t[p] += sgn * steps[c, p]
                         q = theta_to_params(t)
                         plans.append(with_primordial(base_plans[c], q["A_s"], q["n_s"]))
                         vpts.append(pts[c])
                     vk.append(kdev[c])
-        cl = torch.stack(cls)
+        cl = cls_all[c0:c0 + len(cs)].reshape(len(cs) * per, 3, -1)
         if lensing:
             obs = model.lensed_from_cl(plans, vpts, vk, cl)[:, :3, :n_l]
         else:
             obs = cl[:, :, :n_l]
-        Lp[c0:c0 + len(cs)] = like.loglike_cl(obs, slots).cpu().numpy().reshape(len(cs), 7, 2)
-    return Lp
+        Lp.append(like.loglike_cl(obs, slots).reshape(len(cs), 7, 2))
+        del vpts, moved
+    return torch.cat(Lp)
 
 
 def tangent_gradients(model, like, thetas, k_grids=None, mock_index=None, phases=None, chunk: int = 64,
@@ -250,18 +258,19 @@ def tangent_gradients(model, like, thetas, k_grids=None, mock_index=None, phases
     """d L / d theta for each row of ``thetas`` from tangent-linear ODE launches.
 
     The cosmologies go through in chunks of ``chunk`` (one ODE launch each,
-    the first a quarter of that); a host thread prepares the next chunks'
-    background tables and tangent
-    seeds while the GPU works on the current one.  ``downstream``: "tangent"
+    the first three an eighth, a quarter and a half of that); a host thread
+    prepares the next chunks' background tables and tangent seeds while the
+    main thread enqueues the current chunk's ODE launch and downstream on the
+    current stream without waiting for the GPU, so the launches run back to
+    back.  ``downstream``: "tangent"
     (default) differentiates the LOS in one tangent-linear pass (the Bessel
     lookups as interpolated segments and asymptotic forms, as automatic
     differentiation of the reference would) and differences only the
     lensing; "fd" differences the whole downstream (14 evaluations per
     cosmology, the same relative step as the plain finite-difference path,
     table/asymptotic jumps of the Bessel lookups included).  ``phases`` (optional
-    dict) receives the host time not hidden behind GPU work, the time of the
-    tangent-linear ODE launches with their base downstream, and of the
-    directional downstream.
+    dict) receives the main thread's time waiting for host inputs, enqueuing
+    GPU work, and waiting at the end for the GPU to drain.
     """
     import time
     from concurrent.futures import ThreadPoolExecutor
@@ -271,31 +280,55 @@ def tangent_gradients(model, like, thetas, k_grids=None, mock_index=None, phases
     mock_index = list(range(nc)) if mock_index is None else list(mock_index)
     k_grids = list(k_grids) if k_grids is not None else [None] * nc
     steps = FD_REL_STEP * np.abs(thetas)
-    # a small first chunk, so the GPU starts while the host prepares the rest
+    # doubling first chunks (an eighth, a quarter, a half of ``chunk``), so
+    # the GPU starts early and each chunk's host preparation (about half the
+    # GPU time of a chunk of the same size) ends about when the GPU has
+    # finished the chunk before it
     chunk = max(1, chunk)
-    first = min(nc, max(1, chunk // 4))
-    bounds = [(0, first)] + [(c0, min(nc, c0 + chunk)) for c0 in range(first, nc, chunk)]
-    Lb = np.zeros(nc)
-    Lp = np.zeros((nc, 7, 2))
-    t_host = t_ode = t_dir = 0.0
+    bounds, c0 = [], 0
+    for size in [max(1, chunk // 8), max(1, chunk // 4), max(1, chunk // 2)] + [chunk] * nc:
+        if c0 >= nc:
+            break
+        bounds.append((c0, min(nc, c0 + size)))
+        c0 += size
+    import torch
+    # Everything goes on the current stream without a host wait: chunk i's ODE
+    # launch, its downstream and its losses are enqueued while the GPU still
+    # works on earlier chunks (the staging copies are asynchronous), so the
+    # GPU runs the launches back to back; the losses come back and the
+    # launches' status is checked once at the end.
+    Lb_d, Lp_d, handles = [], [], []
+    t_host = t_enq = 0.0
+    directional = _directional_losses_linear if downstream == "tangent" else _directional_losses
     with ThreadPoolExecutor(1) as host:
         futs = [host.submit(_tangent_inputs, model, thetas[a:b], k_grids[a:b], steps[a:b]) for a, b in bounds]
         for (a, b), fut in zip(bounds, futs):
             t0 = time.perf_counter()
             base_plans, pm, seeds = fut.result()
             t1 = time.perf_counter()
-            outs, dpts = model.execute_tangent(base_plans, seeds)
-            Lb[a:b] = like.loglike(outs, mock_index[a:b]).cpu().numpy()
-            t2 = time.perf_counter()
-            directional = _directional_losses_linear if downstream == "tangent" else _directional_losses
-            Lp[a:b] = directional(model, like, thetas[a:b], steps[a:b], base_plans, pm, outs, dpts, mock_index[a:b])
-            del outs, dpts
-            t3 = time.perf_counter()
+            h = model.launch_tangent(base_plans, seeds)
+            run = model._ode_result(h)
+            ds = model._downstream(base_plans, run["pts"], run["kdev"])
+            outs, dpts = model._outputs(base_plans, run, ds), run["dpts"]
+            Lb_d.append(like.loglike_cl(model.observed_cl(ds), mock_index[a:b]))
+            Lp_d.append(directional(model, like, thetas[a:b], steps[a:b], base_plans, pm, outs, dpts,
+                                    mock_index[a:b]))
+            # keep only what the status check reads: the tables and staging
+            # buffers go back to the allocator (their reuse is ordered after
+            # this chunk's work on the same stream)
+            handles.append({key: h[key] for key in ("done", "flat", "allk", "status", "steps", "iters", "ev",
+                                                     "tangent")})
+            del run, outs, dpts, ds, h
             t_host += t1 - t0
-            t_ode += t2 - t1
-            t_dir += t3 - t2
+            t_enq += time.perf_counter() - t1
+    t0 = time.perf_counter()
+    Lb = torch.cat(Lb_d).cpu().numpy()
+    Lp = torch.cat(Lp_d).cpu().numpy()
+    for h in handles:
+        model._check_ode(h)
+    t_drain = time.perf_counter() - t0
     grad = (Lp[:, :, 0] - Lp[:, :, 1]) / (2 * steps)
     if phases is not None:
-        phases.update(host_exposed_s=t_host, tangent_ode_s=t_ode, directional_downstream_s=t_dir,
+        phases.update(host_exposed_s=t_host, enqueue_s=t_enq, gpu_drain_s=t_drain,
                       total_s=time.perf_counter() - t_start, chunk=chunk)
     return Lb, grad

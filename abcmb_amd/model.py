@@ -320,18 +320,21 @@ class Model:
         kernel, which also returns the tables' derivatives along the seeds'
         directions ([N_TANGENT_DIRS][21][n_save][nk] per cosmology).
         """
+        return self._finish_ode(self._launch_ode(plans, tangents, want_state))
+
+    def _launch_ode(self, plans, tangents=None, want_state: bool = False):
+        """Enqueue the ODE launch of ``_run_ode`` on the current stream and return
+        a handle for ``_finish_ode``; the handle keeps every buffer the launch
+        reads or writes alive."""
         torch = _torch()
         lib = abi.hip_lib()
         D = self._device_constants()
         dev = torch.device("cuda")
         stream = torch.cuda.current_stream().cuda_stream
         ns = self.n_save
-        keep = []
-
-        def dt(a, dtype=torch.float64):
-            x = torch.as_tensor(np.ascontiguousarray(a), dtype=dtype).to(dev, non_blocking=False)
-            keep.append(x)
-            return x
+        stage = _Stage(torch, dev)
+        keep = [stage]
+        dt = stage.add
 
         allk = [(pl.k if pl.k is not None else self.k_pt) for pl in plans]
         kdev = [D["k_pt"] if kk is self.k_pt else dt(kk) for kk in allk]
@@ -363,6 +366,7 @@ class Model:
         nd = abi.N_TANGENT_DIRS if tangents is not None else 0
         ws_bytes = lib.abcmb_pert_workspace_size(len(plans), ns, nd)
         ws = torch.empty(ws_bytes // 8 + 1, dtype=torch.float64, device=dev)
+        stage.flush()
         ev0 = ev1 = None
         if self.timing is not None:
             ev0 = torch.cuda.Event(enable_timing=True)
@@ -382,13 +386,39 @@ class Model:
                       "abcmb_pert_tangent")
         if ev1 is not None:
             ev1.record()
-        torch.cuda.synchronize()
-        st_h = status.cpu().numpy()
-        steps_h = steps.cpu().numpy()
-        if self.timing is not None:
+        done = torch.cuda.Event()
+        done.record()
+        return dict(pts=pts, dpts=dpts, kdev=kdev, allk=allk, state=state, flat=flat, keep=keep, status=status,
+                    steps=steps, iters=iters, ws=ws, items_dev=items_dev, cos_dev=cos_dev, ev=(ev0, ev1), done=done,
+                    tangent=tangents is not None, tan_dev=tan_dev if tangents is not None else None)
+
+    def _finish_ode(self, h):
+        """Wait for a launch of ``_launch_ode``, check its status and return the
+        tables with the per-k step counts."""
+        run = self._ode_result(h)
+        run["steps"] = self._check_ode(h)
+        return run
+
+    @staticmethod
+    def _ode_result(h):
+        """The tables of a ``_launch_ode`` handle without waiting for them: work
+        enqueued on the launch's stream after this sees them complete."""
+        return dict(pts=h["pts"], dpts=h["dpts"], kdev=h["kdev"], allk=h["allk"], steps=None, state=h["state"],
+                    flat=h["flat"], keep=h["keep"])
+
+    def _check_ode(self, h):
+        """Wait for a ``_launch_ode`` handle and raise on any failed (cosmology, k)
+        item; returns the per-k step counts of each cosmology."""
+        torch = _torch()
+        h["done"].synchronize()
+        flat, allk, n_items = h["flat"], h["allk"], len(h["flat"])
+        st_h = h["status"].cpu().numpy()
+        steps_h = h["steps"].cpu().numpy()
+        ev0, ev1 = h["ev"]
+        if self.timing is not None and ev0 is not None:
             self.timing.append(dict(ms=ev0.elapsed_time(ev1), steps=int(steps_h.sum()),
-                                    iters=int(iters.sum().item()), items=n_items,
-                                    kernel="tangent" if tangents is not None else "primal"))
+                                    iters=int(h["iters"].sum().item()), items=n_items,
+                                    kernel="tangent" if h["tangent"] else "primal"))
         bad = np.nonzero(st_h != abi_k_ok())[0]
         if bad.size:
             where = [(flat[b][1], float(allk[flat[b][1]][flat[b][2]])) for b in bad[:8]]
@@ -398,7 +428,7 @@ class Model:
         steps_ck = [np.zeros(a.size, dtype=np.int32) for a in allk]
         for m, (_, ci, ik) in enumerate(flat):
             steps_ck[ci][ik] = steps_h[m]
-        return dict(pts=pts, dpts=dpts, kdev=kdev, allk=allk, steps=steps_ck, state=state, flat=flat, keep=keep)
+        return steps_ck
 
     def _downstream(self, plans, pts, kdev):
         """Sources, LOS C_l, l-spline, lensing and P(k) for ``plans`` with tables ``pts``.
@@ -412,12 +442,8 @@ class Model:
         D = self._device_constants()
         dev = torch.device("cuda")
         stream = torch.cuda.current_stream().cuda_stream
-        keep = []
-
-        def dt(a, dtype=torch.float64):
-            x = torch.as_tensor(np.ascontiguousarray(a), dtype=dtype).to(dev, non_blocking=False)
-            keep.append(x)
-            return x
+        stage = _Stage(torch, dev)
+        dt = stage.add
 
         ns = self.n_save
         ncos = len(plans)
@@ -454,6 +480,7 @@ class Model:
         if lensing:
             ws_lens = torch.empty(lib.abcmb_lens_workspace_size(chunk, self.mu.size, nl) // 8 + 1,
                                   dtype=torch.float64, device=dev)
+        stage.flush()
         for c0 in range(0, ncos, chunk):
             nb = min(chunk, ncos - c0)
             cp = cos_dev.data_ptr() + c0 * csz
@@ -475,24 +502,17 @@ class Model:
                                          D["w"].data_ptr(), self.mu.size, lensed[c0].data_ptr(), ws_lens.data_ptr(),
                                          ws_lens.numel() * 8, stream), "abcmb_lens")
             abi.check(lib.abcmb_pk_lin(cp, nb, D["k_pk"].data_ptr(), nq, pk[c0].data_ptr(), stream), "abcmb_pk_lin")
-        # the host-side structs and staging buffers must outlive the launches
-        torch.cuda.current_stream().synchronize()
         return dict(cl_raw=cl_raw, cl_full=cl_full, clpp=clpp, lensed=lensed, pk=pk)
 
     def _staging(self):
         torch = _torch()
         dev = torch.device("cuda")
-        keep = []
-
-        def dt(a, dtype=torch.float64):
-            x = torch.as_tensor(np.ascontiguousarray(a), dtype=dtype).to(dev, non_blocking=False)
-            keep.append(x)
-            return x
-        return torch, dev, keep, dt
+        return torch, dev, _Stage(torch, dev)
 
     def _run_sources(self, plans, pts, kdev):
         """LOS source functions [B][4][n_save-1][n_kt] of ``plans`` with tables ``pts``."""
-        torch, dev, keep, dt = self._staging()
+        torch, dev, stage = self._staging()
+        dt = stage.add
         lib = abi.hip_lib()
         D = self._device_constants()
         stream = torch.cuda.current_stream().cuda_stream
@@ -505,12 +525,12 @@ class Model:
         ws = torch.empty(lib.abcmb_sources_workspace_size(chunk, nkmax, ns) // 8 + 1, dtype=torch.float64, device=dev)
         csz = C.sizeof(abi.AbcmbCosmo)
         sc = self.scales.ctypes.data_as(C.c_void_p)
+        stage.flush()
         for c0 in range(0, ncos, chunk):
             nb = min(chunk, ncos - c0)
             abi.check(lib.abcmb_sources(cos_dev.data_ptr() + c0 * csz, nb, nkmax, ns, bgv_all[c0].data_ptr(),
                                         D["kt"].data_ptr(), nkt, sc, src[c0].data_ptr(), ws.data_ptr(),
                                         ws.numel() * 8, stream), "abcmb_sources")
-        torch.cuda.current_stream().synchronize()
         return src
 
     def downstream_tangent(self, plans, pts, kdev, dpts, plus, minus, steps):
@@ -525,7 +545,8 @@ class Model:
         [B][1 + N_TANGENT_DIRS + 2][3][n_l]: value, the directions, d/d ln A_s,
         d/d n_s.
         """
-        torch, dev, keep, dt = self._staging()
+        torch, dev, stage = self._staging()
+        dt = stage.add
         lib = abi.hip_lib()
         D = self._device_constants()
         stream = torch.cuda.current_stream().cuda_stream
@@ -535,33 +556,39 @@ class Model:
         nv = 1 + nd + 2
         out = torch.empty((ncos, nv, 3, nl), dtype=torch.float64, device=dev)
         group = max(1, self.chunk // (1 + 2 * nd))
+        per = 1 + 2 * nd
         for c0 in range(0, ncos, group):
             cs = range(c0, min(ncos, c0 + group))
-            vplans, vpts, vk = [], [], []
-            for c in cs:
-                vplans.append(plans[c]); vpts.append(pts[c]); vk.append(kdev[c])
-                for d in range(nd):
-                    for sgn, pl in ((+1, plus[c][d]), (-1, minus[c][d])):
-                        vplans.append(pl); vpts.append(pts[c] + (sgn * steps[c][d]) * dpts[c][d]); vk.append(kdev[c])
-            src = self._run_sources(vplans, vpts, vk)
-            del vpts
-            per = 1 + 2 * nd
             nb = len(cs)
-            src_b = src[0::per].contiguous()
-            dsrc = torch.empty((nb, nd, 4, n_tau, nkt), dtype=torch.float64, device=dev)
+            hs = np.array([[[steps[c][d], -steps[c][d]] for d in range(nd)] for c in cs])   # [nb][nd][2]
+            hs_d = dt(hs)
+            h2_d = dt(2.0 * hs[:, :, 0])
             dchi = np.zeros((nb, nd, n_tau))
             for i, c in enumerate(cs):
                 for d in range(nd):
-                    h2 = 2.0 * steps[c][d]
-                    torch.sub(src[i * per + 1 + 2 * d], src[i * per + 2 + 2 * d], out=dsrc[i, d]).div_(h2)
                     pp, pm = plus[c][d], minus[c][d]
-                    dchi[i, d] = ((pp.bg.tau0 - pp.bgv[0]) - (pm.bg.tau0 - pm.bgv[0])) / h2
-            del src
+                    dchi[i, d] = ((pp.bg.tau0 - pp.bgv[0]) - (pm.bg.tau0 - pm.bgv[0])) / (2.0 * steps[c][d])
             cos_b = [plans[c] for c in cs]
             cos_dev = self._struct_array(torch, dev, self._cosmo_array(cos_b, [pts[c] for c in cs],
                                                                        [kdev[c] for c in cs], dt, tables=False), dt)
             bgv_b = dt(np.stack([pl.bgv for pl in cos_b]))
             dchi_d = dt(dchi)
+            stage.flush()
+            vplans, vpts, vk = [], [], []
+            for i, c in enumerate(cs):
+                vplans.append(plans[c]); vpts.append(pts[c]); vk.append(kdev[c])
+                moved = _moved_tables(torch, pts[c], dpts[c], hs_d[i])
+                for d in range(nd):
+                    vplans += [plus[c][d], minus[c][d]]
+                    vpts += [moved[2 * d], moved[2 * d + 1]]
+                    vk += [kdev[c], kdev[c]]
+            src = self._run_sources(vplans, vpts, vk)
+            del vpts, moved
+            sv = src.view(nb, per, 4, n_tau, nkt)
+            src_b = sv[:, 0].contiguous()
+            # central differences of the sources: (S(+h) - S(-h)) / 2h
+            dsrc = torch.sub(sv[:, 1::2], sv[:, 2::2]).div_(h2_d.view(nb, nd, 1, 1, 1))
+            del src, sv
             ws = torch.empty(lib.abcmb_los_tangent_workspace_size(nb, nd, n_ell, nkt) // 8 + 1, dtype=torch.float64,
                              device=dev)
             raw = torch.empty((nb, nv, 3, n_ell), dtype=torch.float64, device=dev)
@@ -572,12 +599,12 @@ class Model:
                                             ws.data_ptr(), ws.numel() * 8, stream), "abcmb_los_tangent")
             abi.check(lib.abcmb_apply_operator(D["ell_op"].data_ptr(), nl, n_ell, raw.data_ptr(), nb * nv * 3,
                                                out[c0].data_ptr(), stream), "abcmb_apply_operator")
-            torch.cuda.current_stream().synchronize()
         return out
 
     def lensed_from_cl(self, plans, pts, kdev, cl_full):
         """Lensed [B][4][n_l] from unlensed cl_full [B][3][n_l], with C_l^phiphi of ``plans`` / ``pts``."""
-        torch, dev, keep, dt = self._staging()
+        torch, dev, stage = self._staging()
+        dt = stage.add
         lib = abi.hip_lib()
         D = self._device_constants()
         stream = torch.cuda.current_stream().cuda_stream
@@ -596,6 +623,7 @@ class Model:
         ws = torch.empty(lib.abcmb_lens_workspace_size(chunk, self.mu.size, nl) // 8 + 1, dtype=torch.float64,
                          device=dev)
         csz = C.sizeof(abi.AbcmbCosmo)
+        stage.flush()
         for c0 in range(0, ncos, chunk):
             nb = min(chunk, ncos - c0)
             abi.check(lib.abcmb_lensing_clpp(cos_dev.data_ptr() + c0 * csz, nb, lens_all[c0].data_ptr(), n_lens,
@@ -604,7 +632,6 @@ class Model:
             abi.check(lib.abcmb_lens(nb, cl_full[c0].data_ptr(), clpp[c0].data_ptr(), nl, D["mu"].data_ptr(),
                                      D["w"].data_ptr(), self.mu.size, lensed[c0].data_ptr(), ws.data_ptr(),
                                      ws.numel() * 8, stream), "abcmb_lens")
-        torch.cuda.current_stream().synchronize()
         return lensed
 
     def observed_cl(self, ds):
@@ -628,7 +655,8 @@ class Model:
             PT = PerturbationTable(run["allk"][i], pl.lna_save, run["pts"][i], self.has_ncdm)
             PT.k_dev = run["kdev"][i]
             out = Output(TT, TE, EE, ds["pk"][i], self.ells, self.k_pk, pl.bg, PT, pl.params, ClBB=BB,
-                         Clpp=ds["clpp"][i] if lensing else None, steps=run["steps"][i])
+                         Clpp=ds["clpp"][i] if lensing else None,
+                         steps=run["steps"][i] if run["steps"] is not None else None)
             out.cl_raw = ds["cl_raw"][i]
             out.cl_unlensed = ds["cl_full"][i]
             outs.append(out)
@@ -649,12 +677,89 @@ class Model:
         ``seeds``: one ``TangentSeeds`` per plan.  Returns (outputs, dpts) with
         dpts[i] of shape [N_TANGENT_DIRS][21][n_save][nk_i] on the GPU.
         """
-        run = self._run_ode(plans, tangents=seeds)
+        return self.finish_tangent(plans, self.launch_tangent(plans, seeds))
+
+    def launch_tangent(self, plans, seeds):
+        """Enqueue the tangent-linear ODE launch of ``execute_tangent`` on the
+        current stream and return its handle (``_check_ode`` waits for it)."""
+        return self._launch_ode(plans, tangents=seeds)
+
+    def finish_tangent(self, plans, handle):
+        """Wait for a ``launch_tangent`` handle, run the downstream on the current
+        stream and return (outputs, dpts) as ``execute_tangent`` does."""
+        run = self._finish_ode(handle)
         ds = self._downstream(plans, run["pts"], run["kdev"])
         return self._outputs(plans, run, ds), run["dpts"]
 
     def __call__(self, params: dict = {}) -> Output:
         return self.run_batch([params])[0]
+
+
+def _moved_tables(torch, pt, dpt, hs):
+    """pt + h dpt for every tangent direction and sign in one launch:
+    [2 * n_dir][fields][n_save][nk], direction-major, +h before -h.  ``hs``:
+    [n_dir][2] steps on the GPU."""
+    nd = dpt.shape[0]
+    return torch.addcmul(pt.unsqueeze(0).unsqueeze(0), hs.view(nd, 2, 1, 1, 1), dpt.unsqueeze(1)).view(
+        2 * nd, *pt.shape)
+
+
+def _np_dtypes():
+    import torch
+    return {torch.float64: np.float64, torch.float32: np.float32, torch.int64: np.int64, torch.int32: np.int32,
+            torch.uint8: np.uint8}
+
+
+class _DtypeMap(dict):
+    """torch dtype -> numpy dtype, built on first use (torch imports lazily)."""
+    def __missing__(self, key):
+        self.update(_np_dtypes())
+        return dict.__getitem__(self, key)
+
+
+_NP_DTYPE = _DtypeMap()
+
+
+class _Stage:
+    """Host-to-device staging that never waits for the GPU.
+
+    ``add`` packs a host array into a pinned host segment and returns the
+    device tensor it will occupy (its address is final at once, so it can go
+    into the structs the kernels read); ``flush`` enqueues one asynchronous
+    copy per segment on the current stream and must come before the first
+    kernel that reads the staged arrays.  torch's pinned-memory cache keeps a
+    host segment until its copy has run.
+    """
+    SEG = 8 << 20
+    ALIGN = 256
+
+    def __init__(self, torch, dev):
+        self.torch, self.dev = torch, dev
+        self.segs = []      # [host, device, used bytes, host as numpy]
+        self.dirty = False
+
+    def add(self, a, dtype=None):
+        torch = self.torch
+        dtype = dtype or torch.float64
+        arr = np.ascontiguousarray(a, dtype=_NP_DTYPE[dtype])
+        nb = arr.nbytes
+        if not self.segs or self.segs[-1][2] + nb > self.segs[-1][0].numel():
+            size = max(self.SEG, nb)
+            host = torch.empty(size, dtype=torch.uint8, pin_memory=True)
+            self.segs.append([host, torch.empty(size, dtype=torch.uint8, device=self.dev), 0, host.numpy()])
+        seg = self.segs[-1]
+        off = seg[2]
+        seg[3][off:off + nb] = arr.reshape(-1).view(np.uint8)     # numpy byte copy: no torch CPU threads
+        seg[2] = off + (nb + self.ALIGN - 1) // self.ALIGN * self.ALIGN
+        self.dirty = True
+        return seg[1][off:off + nb].view(dtype).view(arr.shape)
+
+    def flush(self):
+        if self.dirty:
+            for host, devb, used, _ in self.segs:
+                n = min(used, host.numel())
+                devb[:n].copy_(host[:n], non_blocking=True)
+            self.dirty = False
 
 
 def abi_k_ok():

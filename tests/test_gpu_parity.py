@@ -235,7 +235,8 @@ def test_partial_k_launch_matches_full():
     _say("case: partial-k launch (cosmology 0: k 1, 4, 7, 10; cosmology 1: all) vs full launch")
     full = m._run_ode(plans, tangents=seeds)
     lib = abi.hip_lib()
-    _, dev, keep, dt = m._staging()
+    _, dev, stage = m._staging()
+    dt = stage.add
     ns, nd, nk = m.n_save, abi.N_TANGENT_DIRS, m.k_pt.size
     kdev = [dt(m.k_pt), dt(m.k_pt)]
     pts = [torch.full((abi.N_PT_FIELDS, ns, nk), np.nan, dtype=torch.float64, device=dev) for _ in range(2)]
@@ -253,6 +254,7 @@ def test_partial_k_launch_matches_full():
     status = torch.zeros(n, dtype=torch.int32, device=dev)
     stp = torch.zeros(n, dtype=torch.int32, device=dev)
     ws = torch.empty(lib.abcmb_pert_workspace_size(2, ns, nd) // 8 + 1, dtype=torch.float64, device=dev)
+    stage.flush()
     abi.check(lib.abcmb_pert_tangent(cos_dev.data_ptr(), tan_dev.data_ptr(), 2, items_dev.data_ptr(), n,
                                      C.byref(m.settings), ns, status.data_ptr(), stp.data_ptr(), None,
                                      ws.data_ptr(), ws.numel() * 8, torch.cuda.current_stream().cuda_stream),
