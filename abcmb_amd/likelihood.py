@@ -329,6 +329,6 @@ def tangent_gradients(model, like, thetas, k_grids=None, mock_index=None, phases
     t_drain = time.perf_counter() - t0
     grad = (Lp[:, :, 0] - Lp[:, :, 1]) / (2 * steps)
     if phases is not None:
-        phases.update(host_exposed_s=t_host, enqueue_s=t_enq, gpu_drain_s=t_drain,
+        phases.update(wait_inputs_s=t_host, enqueue_s=t_enq, gpu_drain_s=t_drain,
                       total_s=time.perf_counter() - t_start, chunk=chunk)
     return Lb, grad
