@@ -210,7 +210,7 @@ __device__ __forceinline__ D<N> bg_xe(const Cosmo<N>& P, double lna) {
 }
 
 template <int N>
-__device__ __noinline__ void bg_eval(const Cosmo<N>& P, double lna, Bg<N>& b) {
+__device__ __forceinline__ void bg_eval(const Cosmo<N>& P, double lna, Bg<N>& b) {
   constexpr double K_NH = 3.0 * H0_OVER_H * H0_OVER_H / 8 / PI / GN / MH;
   constexpr double K_TC = C_MPC / SIGMA_T / C_CM;
   constexpr double STEF = PI * PI / (60.0 * HBAR * HBAR * HBAR * C_CM * C_CM);
@@ -355,7 +355,7 @@ __device__ __noinline__ void row_desc_call(const Layout& L, double k, int r, Row
 // scalars of A(ln a) shared by all rows: sc[c][slot] holds the value (c = 0)
 // and the derivative along direction c - 1
 template <int N>
-__device__ __noinline__ void fill_scalars(const Cosmo<N>& P, const Layout& L, double k, const Bg<N>& b,
+__device__ __forceinline__ void fill_scalars(const Cosmo<N>& P, const Layout& L, double k, const Bg<N>& b,
                                           double (*sc)[SC_COUNT]) {
   constexpr double K_G = 4.0 * PI * GN / (C_MPC * C_MPC);
   constexpr double K_N = 1.0 / (PI * PI) / (HBAR * HBAR * HBAR) / (C_CM * C_CM * C_CM);
@@ -402,6 +402,17 @@ __device__ __noinline__ void fill_scalars(const Cosmo<N>& P, const Layout& L, do
   }
   put(SC_EN, ke * 4.0 * pre);
   put(SC_ZERO, D<N>(0.0));
+}
+
+// the background and the scalars of A at one time, out of line: one call per
+// time and lane keeps the kernel's register budget for the chord loop, and
+// the background stays in the callee's registers instead of a stack object
+template <int N>
+__device__ __noinline__ void time_scalars(const Cosmo<N>& P, const Layout& L, double k, double lna,
+                                          double (*sc)[SC_COUNT]) {
+  Bg<N> b;
+  bg_eval(P, lna, b);
+  fill_scalars(P, L, k, b, sc);
 }
 
 // ---------------------------------------------------------------- the kernel
@@ -528,9 +539,7 @@ __device__ void set_times(Smem<NT, BMAX, N>& sm, const Cosmo<N>& P, const Layout
   __syncthreads();
   int r = threadIdx.x;
   if (r < cnt) {
-    Bg<N> b;
-    bg_eval(P, t + (r == JAC_SLOT ? jac_offset : TIME_C[r]) * hh, b);
-    fill_scalars(P, L, k, b, sm.sc[r]);
+    time_scalars(P, L, k, t + (r == JAC_SLOT ? jac_offset : TIME_C[r]) * hh, sm.sc[r]);
   }
   __syncthreads();
 }
