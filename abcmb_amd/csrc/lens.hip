@@ -24,81 +24,102 @@ namespace {
 using namespace cst;
 
 // ---- Limber lensing potential ----
-__device__ double interp_col(double x, const double* __restrict__ xp, const double* __restrict__ fp, int n,
-                             int stride) {
-  if (x <= xp[0]) return fp[0];
-  if (x >= xp[n - 1]) return fp[(size_t)(n - 1) * stride];
-  int a = 0, b = n - 1;
-  while (b - a > 1) { int mid = (a + b) >> 1; if (xp[mid] <= x) a = mid; else b = mid; }
-  double t = (x - xp[a]) / (xp[a + 1] - xp[a]);
-  return fp[(size_t)a * stride] + t * (fp[(size_t)(a + 1) * stride] - fp[(size_t)a * stride]);
-}
-
-struct LimberArgs {
-  double tau0, Om, OL, A_s, n_s, kp, chi_rec;
-  int n_save, nk;
-  const double* lna_save;
-  const double* k;
-};
-
-// lens_all: [B][4][n_lna] = ln a axis, aH, tau, and chi(lna_rec) in [3][0]
-__global__ void clpp_kernel(const AbcmbCosmo* __restrict__ cosmos, const double* __restrict__ lens_all, int n_lna,
-                            const double* __restrict__ ells, int n_l, double* __restrict__ out_all) {
-  int il = blockIdx.x * blockDim.x + threadIdx.x;
-  int ci = blockIdx.y;
-  if (il >= n_l) return;
+// lens_all: [B][4][n_lna] = ln a axis, aH, tau, and chi(lna_rec) in [3][0].
+//
+// One thread per (l, cosmology); the ln a nodes go through LDS in tiles.  The
+// Limber integrand at node i and multipole l is
+//   v = chi / aH * W^2 * 9 / (8 pi^2) * Om(z)^2 aH^4 * P(kk) / kk,
+//   P(kk) = d^2 A_s (kk / k_p)^(n_s - 1) 2 pi^2 / kk^3,  kk = (l + 1/2) / chi,
+// so v = C_i * d_i(kk)^2 * (l + 1/2)^(n_s - 5), with everything but the
+// transfer function d in a per-node factor C_i (chi^(5 - n_s) included) and
+// the power of (l + 1/2) out of the sum.  Per node the tile also holds 1/chi,
+// the ln a step of the trapezoid and the bracket of ln a(z) on the saved-time
+// grid; the k bracket of kk walks forward with i (chi falls, kk grows).
+constexpr int CLPP_TILE = 256;
+constexpr int CLPP_THREADS = 128;
+__global__ void __launch_bounds__(CLPP_THREADS) clpp_kernel(const AbcmbCosmo* __restrict__ cosmos,
+                                                            const double* __restrict__ lens_all, int n_lna,
+                                                            const double* __restrict__ ells, int n_l,
+                                                            double* __restrict__ out_all) {
+  __shared__ double s_ichi[CLPP_TILE], s_c[CLPP_TILE], s_dl[CLPP_TILE], s_u[CLPP_TILE];
+  __shared__ int s_ia[CLPP_TILE], s_ib[CLPP_TILE];
+  const int il = blockIdx.x * blockDim.x + threadIdx.x;
+  const int ci = blockIdx.y;
   const AbcmbCosmo& c = cosmos[ci];
   const double* lna_ax = lens_all + (size_t)ci * 4 * n_lna;
   const double* aH_ax = lna_ax + n_lna;
   const double* tau_ax = lna_ax + 2 * n_lna;
-  double* out = out_all + (size_t)ci * n_l;
-  const double* pt = c.pt;
-  LimberArgs A;
-  A.tau0 = c.tau0;
-  A.Om = c.omega_m / (c.h * c.h);
-  A.OL = c.omega_Lambda / (c.h * c.h);
-  A.A_s = c.A_s; A.n_s = c.n_s; A.kp = c.k_pivot;
-  A.chi_rec = lna_ax[3 * n_lna];
-  A.n_save = c.n_save; A.nk = c.nk; A.lna_save = c.lna_save; A.k = c.k;
-  const double* dm = pt + (size_t)ABCMB_PT_DELTA_M * A.n_save * A.nk;
-  double ell = ells[il];
-  double lna_floor = lna_ax[n_lna - 2];
+  const double chi_rec = lna_ax[3 * n_lna];
+  const double Om = c.omega_m / (c.h * c.h), OL = c.omega_Lambda / (c.h * c.h);
+  const double ns = c.n_s;
+  // node factor constants: 9 / (8 pi^2) * A_s * 2 pi^2 * k_p^(1 - n_s)
+  const double cnode = 9. / 8. / (PI * PI) * c.A_s * (2 * PI * PI) * pow(c.k_pivot, 1. - ns);
+  const int n_save = c.n_save, nk = c.nk;
+  const double* lna_save = c.lna_save;
+  const double* kgrid = c.k;
+  const double* dm = c.pt + (size_t)ABCMB_PT_DELTA_M * n_save * nk;
+  const double e5 = (il < n_l ? ells[il] : 0.0) + 0.5;
   double s = 0.0, prev = 0.0;
-  for (int i = 0; i < n_lna; ++i) {
-    double lna = lna_ax[i];
-    double v = 0.0;
-    if (lna < 0.0) {
-      double chi = A.tau0 - tau_ax[i];
-      double kk = (ell + 0.5) / chi;
-      double win = (A.chi_rec - chi) / A.chi_rec / chi;
-      double a = exp(lna);
-      double z = 1. / a - 1.;
-      double aH = aH_ax[i];
-      double z3 = (1. + z) * (1. + z) * (1. + z);
-      double omz = (A.Om * z3) / ((A.Om * z3) + A.OL);
-      double lq = -log(1. + z);
-      int ka, kb;
-      double t;
-      if (kk <= A.k[0]) { ka = kb = 0; t = 0.0; }
-      else if (kk >= A.k[A.nk - 1]) { ka = kb = A.nk - 1; t = 0.0; }
-      else {
-        ka = 0; kb = A.nk - 1;
-        while (kb - ka > 1) { int mid = (ka + kb) >> 1; if (A.k[mid] <= kk) ka = mid; else kb = mid; }
-        t = (kk - A.k[ka]) / (A.k[kb] - A.k[ka]);
+  int ka = 0;
+  for (int i0 = 0; i0 < n_lna; i0 += CLPP_TILE) {
+    __syncthreads();
+    for (int j = threadIdx.x; j < CLPP_TILE && i0 + j < n_lna; j += blockDim.x) {
+      const int i = i0 + j;
+      const double lna = lna_ax[i];
+      double C = 0.0, ichi = 0.0, u = 0.0;
+      int ia = 0, ib = 0;
+      if (lna < 0.0) {
+        const double chi = c.tau0 - tau_ax[i];
+        const double win = (chi_rec - chi) / chi_rec / chi;
+        const double a = exp(lna);
+        const double z = 1. / a - 1.;
+        const double aH = aH_ax[i];
+        const double z3 = (1. + z) * (1. + z) * (1. + z);
+        const double omz = (Om * z3) / ((Om * z3) + OL);
+        const double aH2 = aH * aH;
+        C = chi / aH * win * win * omz * omz * (aH2 * aH2) * cnode * pow(chi, 5. - ns);
+        ichi = 1.0 / chi;
+        const double lq = -log(1. + z);
+        if (lq <= lna_save[0]) { ia = ib = 0; }
+        else if (lq >= lna_save[n_save - 1]) { ia = ib = n_save - 1; }
+        else {
+          int lo = 0, hi = n_save - 1;
+          while (hi - lo > 1) { int mid = (lo + hi) >> 1; if (lna_save[mid] <= lq) lo = mid; else hi = mid; }
+          ia = lo; ib = lo + 1;
+          u = (lq - lna_save[lo]) / (lna_save[lo + 1] - lna_save[lo]);
+        }
       }
-      double da = interp_col(lq, A.lna_save, dm + ka, A.n_save, A.nk);
-      double db = interp_col(lq, A.lna_save, dm + kb, A.n_save, A.nk);
-      double d = da + t * (db - da);
-      double pk = d * d * (A.A_s * pow(kk / A.kp, A.n_s - 1.) * (2 * PI * PI / (kk * kk * kk)));
-      double pl = 9. / 8. / (PI * PI) * omz * omz * (aH * aH * aH * aH) * pk / kk;
-      v = chi / aH * win * win * pl;
+      s_ichi[j] = ichi; s_c[j] = C; s_u[j] = u; s_ia[j] = ia; s_ib[j] = ib;
+      s_dl[j] = i > 0 ? lna - lna_ax[i - 1] : 0.0;
     }
-    (void)lna_floor;
-    if (i > 0) s += 0.5 * (prev + v) * (lna_ax[i] - lna_ax[i - 1]);
-    prev = v;
+    __syncthreads();
+    if (il < n_l) {
+      const int m = min(CLPP_TILE, n_lna - i0);
+      for (int j = 0; j < m; ++j) {
+        double v = 0.0;
+        const double C = s_c[j];
+        if (C != 0.0) {
+          const double kk = e5 * s_ichi[j];
+          while (ka < nk - 2 && kgrid[ka + 1] <= kk) ++ka;
+          int k0, k1;
+          double t;
+          if (kk <= kgrid[0]) { k0 = k1 = 0; t = 0.0; }
+          else if (kk >= kgrid[nk - 1]) { k0 = k1 = nk - 1; t = 0.0; }
+          else { k0 = ka; k1 = ka + 1; t = (kk - kgrid[k0]) / (kgrid[k1] - kgrid[k0]); }
+          const double u = s_u[j];
+          const double* ra = dm + (size_t)s_ia[j] * nk;
+          const double* rb = dm + (size_t)s_ib[j] * nk;
+          const double da = ra[k0] + u * (rb[k0] - ra[k0]);
+          const double db = ra[k1] + u * (rb[k1] - ra[k1]);
+          const double d = da + t * (db - da);
+          v = C * (d * d);
+        }
+        if (i0 + j > 0) s += 0.5 * (prev + v) * s_dl[j];
+        prev = v;
+      }
+    }
   }
-  double e5 = ell + 0.5;
-  out[il] = 8. * PI * PI / (e5 * e5 * e5) * s;
+  if (il < n_l) out_all[(size_t)ci * n_l + il] = 8. * PI * PI / (e5 * e5 * e5) * pow(e5, ns - 5.) * s;
 }
 
 // ---- Wigner d recursion tables ----
@@ -376,8 +397,8 @@ extern "C" int abcmb_lensing_clpp(const AbcmbCosmo* cosmos_dev, int n_cosmo, con
                                   const double* ells_dev, int n_l, double* clpp_dev, void* stream) {
   ABCMB_REQUIRE(cosmos_dev && lens_dev && ells_dev && clpp_dev, "null argument");
   ABCMB_REQUIRE(n_cosmo > 0 && n_lna >= 2 && n_l > 0, "bad sizes");
-  hipLaunchKernelGGL(clpp_kernel, dim3((n_l + 127) / 128, n_cosmo), dim3(128), 0, (hipStream_t)stream, cosmos_dev,
-                     lens_dev, n_lna, ells_dev, n_l, clpp_dev);
+  hipLaunchKernelGGL(clpp_kernel, dim3((n_l + CLPP_THREADS - 1) / CLPP_THREADS, n_cosmo), dim3(CLPP_THREADS), 0,
+                     (hipStream_t)stream, cosmos_dev, lens_dev, n_lna, ells_dev, n_l, clpp_dev);
   ABCMB_HIP_CHECK(hipGetLastError());
   return ABCMB_OK;
 }
