@@ -330,7 +330,7 @@ __device__ __forceinline__ void table_d(double x, double lo, double hi, const do
 // one block per (k chunk, block of LB sampled l, cosmology); a thread owns one
 // k and LB multipoles.  part: [B][NS][n_ell][n_chunk] with NS = 3 (1 + ND + 2)
 template <int NTH, int LB, int ND>
-__global__ void __launch_bounds__(NTH) los_tangent_kernel(
+__global__ void __launch_bounds__(NTH) __attribute__((amdgpu_waves_per_eu(2))) los_tangent_kernel(
     const AbcmbCosmo* __restrict__ cosmos, const double* __restrict__ src_all,
     const double* __restrict__ dsrc_all, const double* __restrict__ dchi_all, int n_tau,
     const double* __restrict__ bgv_all, const double* __restrict__ kt, int nkt, const double* __restrict__ bes,
