@@ -356,12 +356,13 @@ __device__ __noinline__ void row_desc_call(const Layout& L, double k, int r, Row
 // and the derivative along direction c - 1
 template <int N>
 __device__ __forceinline__ void fill_scalars(const Cosmo<N>& P, const Layout& L, double k, const Bg<N>& b,
-                                          double (*sc)[SC_COUNT]) {
+                                             double* val, double* der) {
   constexpr double K_G = 4.0 * PI * GN / (C_MPC * C_MPC);
   constexpr double K_N = 1.0 / (PI * PI) / (HBAR * HBAR * HBAR) / (C_CM * C_CM * C_CM);
+  // the value into val[s] (when val is given), derivative i into der[i * SC_COUNT + s]
   auto put = [&](int s, const D<N>& x) {
-    sc[0][s] = x.v;
-    if constexpr (N > 0) { ABCMB_DLOOP(N) sc[1 + i_][s] = x.d[i_]; }
+    if (val) val[s] = x.v;
+    if constexpr (N > 0) { ABCMB_DLOOP(N) der[i_ * SC_COUNT + s] = x.d[i_]; }
   };
   const D<N> iah = dinv(b.aH);
   const D<N> itc = iah * dinv(b.tauc);
@@ -406,13 +407,33 @@ __device__ __forceinline__ void fill_scalars(const Cosmo<N>& P, const Layout& L,
 
 // the background and the scalars of A at one time, out of line: one call per
 // time and lane keeps the kernel's register budget for the chord loop, and
-// the background stays in the callee's registers instead of a stack object
+// the background stays in the callee's registers instead of a stack object.
+// sc: the scalar rows of this time, [1 + N][SC_COUNT].
 template <int N>
 __device__ __noinline__ void time_scalars(const Cosmo<N>& P, const Layout& L, double k, double lna,
                                           double (*sc)[SC_COUNT]) {
   Bg<N> b;
   bg_eval(P, lna, b);
-  fill_scalars(P, L, k, b, sc);
+  fill_scalars(P, L, k, b, sc[0], N > 0 ? sc[1] : nullptr);
+}
+
+// the same along one tangent direction d of a launch with N directions: the
+// parameters and tables of direction d as a one-direction dual (the value
+// arithmetic is the same operation sequence as in the N-direction
+// evaluation), the value row written by direction 0 only
+template <int N>
+__device__ __noinline__ void time_scalars_dir(const AbcmbCosmo& c, const AbcmbTangent* T, int d, const Layout& L,
+                                              double k, double lna, double (*sc)[SC_COUNT]) {
+  AbcmbTangent td;
+  td.par = T->par + (size_t)d * ABCMB_COSMO_DOUBLES;
+  td.tau_tab = T->tau_tab + (size_t)d * ABCMB_TAU_TAB_N;
+  td.xe_tab = T->xe_tab + (size_t)d * c.xe_n;
+  td.tm_tab = T->tm_tab + (size_t)d * c.tm_n;
+  td.pt = nullptr;
+  const Cosmo<1> P1{c, &td};
+  Bg<1> b;
+  bg_eval(P1, lna, b);
+  fill_scalars(P1, L, k, b, d == 0 ? sc[0] : nullptr, sc[1 + d]);
 }
 
 // ---------------------------------------------------------------- the kernel
@@ -537,9 +558,15 @@ template <int NT, int BMAX, int N>
 __device__ void set_times(Smem<NT, BMAX, N>& sm, const Cosmo<N>& P, const Layout& L, double k, double t, double hh,
                           int cnt, double jac_offset) {
   __syncthreads();
-  int r = threadIdx.x;
-  if (r < cnt) {
-    time_scalars(P, L, k, t + (r == JAC_SLOT ? jac_offset : TIME_C[r]) * hh, sm.sc[r]);
+  if constexpr (N == 0) {
+    const int r = threadIdx.x;
+    if (r < cnt) time_scalars(P, L, k, t + (r == JAC_SLOT ? jac_offset : TIME_C[r]) * hh, sm.sc[r]);
+  } else {
+    // one lane per (time, direction): a one-direction dual each
+    static_assert(NTIMES * N <= 64, "set_times: one wavefront holds every (time, direction) lane");
+    const int r = threadIdx.x / N, d = threadIdx.x % N;
+    if (threadIdx.x < cnt * N)
+      time_scalars_dir<N>(P.c, P.t, d, L, k, t + (r == JAC_SLOT ? jac_offset : TIME_C[r]) * hh, sm.sc[r]);
   }
   __syncthreads();
 }
