@@ -183,12 +183,41 @@ struct Rec {
   double dl, dlm1;
 };
 
-// advance pair q from l to l+1
-__device__ __forceinline__ void rec_step(Rec& r, const double* __restrict__ cq, int l, double mu) {
-  double A = cq[l * 3 + 0], B = cq[l * 3 + 1], C = cq[l * 3 + 2];
+// advance pair q from l to l+1 with its coefficients (A, B, C) at l
+__device__ __forceinline__ void rec_step_c(Rec& r, double A, double B, double C, double mu) {
   double dn = A * mu * r.dl + B * r.dl + C * r.dlm1;
   r.dlm1 = r.dl;
   r.dl = dn;
+}
+__device__ __forceinline__ void rec_step(Rec& r, const double* __restrict__ cq, int l, double mu) {
+  rec_step_c(r, cq[l * 3 + 0], cq[l * 3 + 1], cq[l * 3 + 2], mu);
+}
+
+// The l loops read the recursion coefficients and l factors of a tile of KT
+// multipoles from LDS, staged by the block (one wavefront) before the tile:
+// reads at the same address in every lane, issued together, instead of a
+// chain of dependent scalar loads per multipole.
+constexpr int KT = 16;
+struct CoefTile {
+  double c[KT][NPAIR][4];   // (A, B, C, pad) of every pair at l = l0 + i
+  double f[KT][8];          // lfac at l = l0 + i (LFAC entries, padded)
+};
+
+// stage the tile of multipoles l0 .. l0 + KT - 1 (those below LT)
+__device__ __forceinline__ void load_tile(CoefTile& T, const double* __restrict__ coef,
+                                          const double* __restrict__ lfac, int LT, int l0) {
+  __syncthreads();
+  for (int idx = threadIdx.x; idx < KT * NPAIR * 3; idx += blockDim.x) {
+    const int i = idx / (NPAIR * 3), q = (idx / 3) % NPAIR, e = idx % 3;
+    const int l = l0 + i;
+    T.c[i][q][e] = l < LT ? coef[((size_t)q * LT + l) * 3 + e] : 0.0;
+  }
+  for (int idx = threadIdx.x; idx < KT * LFAC; idx += blockDim.x) {
+    const int i = idx / LFAC, e = idx % LFAC;
+    const int l = l0 + i;
+    T.f[i][e] = l < LT ? lfac[(size_t)l * LFAC + e] : 0.0;
+  }
+  __syncthreads();
 }
 
 // pass 1: Cgl(mu), Cgl2(mu)
@@ -224,9 +253,11 @@ __global__ void __launch_bounds__(64) cgl_kernel(const double* __restrict__ clpp
 __global__ void __launch_bounds__(64) ksi_kernel(const double* __restrict__ cl_all, const double* __restrict__ cgl_all, int n_l,
                            const double* __restrict__ mu_arr, int n_mu, const double* __restrict__ coef,
                            const double* __restrict__ lfac, int LT, double* __restrict__ ksi_all) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  __shared__ CoefTile T;
+  const int i0 = blockIdx.x * blockDim.x + threadIdx.x;
+  const bool live = i0 < n_mu;
+  const int i = live ? i0 : n_mu - 1;   // spare lanes follow the last node and store nothing
   int ci = blockIdx.y;
-  if (i >= n_mu) return;
   const double* cl_in = cl_all + (size_t)ci * 3 * n_l;
   const double* cgl = cgl_all + (size_t)ci * 2 * n_mu;
   double* ksi = ksi_all + (size_t)ci * 4 * n_mu;
@@ -259,6 +290,8 @@ __global__ void __launch_bounds__(64) ksi_kernel(const double* __restrict__ cl_a
   double a0 = 0, ap = 0, am = 0, ax = 0;
   for (int j = 0; j < n_l; ++j) {
     int l = j + 2;
+    const int it = j % KT;
+    if (it == 0) load_tile(T, coef, lfac, LT, l);
     if (l == 3) {
       R[5] = Rec{wigner_start(3, 1, mu), 0.0};
       R[7] = Rec{wigner_start(3, -3, mu), 0.0};
@@ -269,7 +302,7 @@ __global__ void __launch_bounds__(64) ksi_kernel(const double* __restrict__ cl_a
       R[8] = Rec{wigner_start(4, -4, mu), 0.0};
       R[11] = Rec{wigner_start(4, -2, mu), 0.0};
     }
-    const double* lf = lfac + l * LFAC;
+    const double* lf = T.f[it];
     double nf = lf[0];
     double d[NPAIR];
 #pragma unroll
@@ -299,17 +332,19 @@ __global__ void __launch_bounds__(64) ksi_kernel(const double* __restrict__ cl_a
     r *= g;
     if (l >= 4) {
 #pragma unroll
-      for (int q = 0; q < NPAIR; ++q) rec_step(R[q], coef + (size_t)q * LT * 3, l, mu);
+      for (int q = 0; q < NPAIR; ++q) rec_step_c(R[q], T.c[it][q][0], T.c[it][q][1], T.c[it][q][2], mu);
     } else {
 #pragma unroll
       for (int q = 0; q < NPAIR; ++q)
-        if (l >= PAIR_M[q]) rec_step(R[q], coef + (size_t)q * LT * 3, l, mu);
+        if (l >= PAIR_M[q]) rec_step_c(R[q], T.c[it][q][0], T.c[it][q][1], T.c[it][q][2], mu);
     }
   }
-  ksi[0 * n_mu + i] = a0 / 4. / PI;
-  ksi[1 * n_mu + i] = ap / 4. / PI;
-  ksi[2 * n_mu + i] = am / 4. / PI;
-  ksi[3 * n_mu + i] = ax / 4. / PI;
+  if (live) {
+    ksi[0 * n_mu + i] = a0 / 4. / PI;
+    ksi[1 * n_mu + i] = ap / 4. / PI;
+    ksi[2 * n_mu + i] = am / 4. / PI;
+    ksi[3 * n_mu + i] = ax / 4. / PI;
+  }
 }
 
 // pass 3: per-block partial back-transform sums, part[blk][4][n_l].  A
