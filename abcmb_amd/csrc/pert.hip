@@ -508,7 +508,7 @@ struct Smem {
   alignas(16) double rl[(NT + BMAX) * NC];
   double uh[NT + BMAX];
   double ue[NT + BMAX];
-  double pbuf[2][NBLK][BMAX];
+  double pbuf[2][NBLK][2][BMAX];   // [buffer][block][pivot row of the pair][column]
   double red[2][NRED * (NT / 64)];  // alternating halves: one barrier per sum
   double Fs[7][NC][NT];  // stage derivatives, one column per thread
 };
@@ -726,37 +726,106 @@ __device__ bool factor(Smem<NT, BMAX, N>& sm, const RowState& R, int& rs, int ma
     }
   }
   double bad = 0.0;
-  // fully unrolled over the pivot index, so F.row[p] is a fixed register
-  // (maxb is the same for every thread: the guard is uniform)
-#pragma unroll
-  for (int p = 0; p < BMAX; ++p) {
-    if (p < maxb) {
-      double* pb = sm.pbuf[p & 1][B.id];
-      const bool in = p < B.size;
-      const bool piv = in && B.lr == p;
-      if (piv) {  // the pivot row as it stands, 1/d in the pivot column
-        const double d = F.row[p];
-        if (!(fabs(d) > 1e-12)) bad = 1.0;
-#pragma unroll
-        for (int c = 0; c < BMAX; ++c) pb[c] = (c == p) ? 1.0 / d : F.row[c];
+  if constexpr (N > 0) {
+    // Gauss-Jordan in pairs of pivots (p, p + 1): one exchange of the two pivot
+    // rows per pair instead of one row per pivot.  With the pivot rows P0, P1,
+    // the 2x2 pivot block D = [[a, b], [c, d]] and every row written as
+    // row + b0 P0 + b1 P1 outside columns p, p + 1 (one multiply-add each):
+    //   rows outside the pair:  (t0, t1) = (row_p, row_p+1) D^-1, b = -t, the
+    //                           pair's columns become -t;
+    //   row p:   D^-1 row 0 applied to (P0, P1) (b0 = d/det - 1, b1 = -b/det);
+    //   row p+1: D^-1 row 1 (b0 = -c/det, b1 = a/det - 1).
+    // The same in-place inverse as two single pivots; a block whose size is odd
+    // takes its last pivot alone.  The singularity test checks the two pivots
+    // sequential elimination would see: a and det / a.  Fully unrolled, so the
+    // pair's columns are fixed registers (maxb is the same for every thread:
+    // the guard is uniform).
+  #pragma unroll
+    for (int p = 0; p < BMAX; p += 2) {
+      if (p < maxb) {
+        double* pb0 = sm.pbuf[(p >> 1) & 1][B.id][0];
+        double* pb1 = sm.pbuf[(p >> 1) & 1][B.id][1];
+        const bool in = p < B.size;
+        const bool pair = p + 1 < B.size;
+        if (in && (B.lr == p || (pair && B.lr == p + 1))) {  // the pivot rows as they stand
+          double* dst = B.lr == p ? pb0 : pb1;
+  #pragma unroll
+          for (int c = 0; c < BMAX; ++c) dst[c] = F.row[c];
+        }
+        if (wl) wave_sync();
+        else __syncthreads();
+        if (in && B.lr < B.size) {
+          // both pivot rows first: one wait for all their loads
+          const double* src1 = pair ? pb1 : pb0;
+          double p0[BMAX], p1[BMAX];
+  #pragma unroll
+          for (int c = 0; c < BMAX; ++c) { p0[c] = pb0[c]; p1[c] = src1[c]; }
+          double b0, b1, rp, rq;
+          if (pair && p + 1 < BMAX) {
+            const double a = p0[p], b = p0[p + 1], cq = p1[p], d = p1[p + 1];
+            const double det = a * d - b * cq;
+            if (!(fabs(a) > 1e-12) || !(fabs(det) > 1e-12 * fabs(a))) bad = 1.0;
+            const double id = 1.0 / det;
+            const double m0 = F.row[p], m1 = F.row[p + 1];
+            const double t0 = (m0 * d - m1 * cq) * id, t1 = (m1 * a - m0 * b) * id;
+            if (B.lr == p) { b0 = d * id - 1.0; b1 = -b * id; rp = d * id; rq = -b * id; }
+            else if (B.lr == p + 1) { b0 = -cq * id; b1 = a * id - 1.0; rp = -cq * id; rq = a * id; }
+            else { b0 = -t0; b1 = -t1; rp = -t0; rq = -t1; }
+          } else {  // the last pivot of an odd block
+            const double dd = p0[p];
+            if (!(fabs(dd) > 1e-12)) bad = 1.0;
+            const double inv = 1.0 / dd;
+            b0 = B.lr == p ? inv - 1.0 : -F.row[p] * inv;
+            b1 = 0.0;
+            rp = B.lr == p ? inv : b0;
+            rq = 0.0;   // column p + 1 is outside this block (zero): updated below like the others
+          }
+  #pragma unroll
+          for (int c = 0; c < BMAX; ++c)
+            if (c != p && c != p + 1) F.row[c] = fma(b1, p1[c], fma(b0, p0[c], F.row[c]));
+          F.row[p] = rp;
+          if (p + 1 < BMAX) {
+            if (pair) F.row[p + 1] = rq;
+            else F.row[p + 1] = fma(b1, p1[p + 1], fma(b0, p0[p + 1], F.row[p + 1]));
+          }
+        }
       }
-      if (wl) wave_sync();
-      else __syncthreads();
-      if (in && B.lr < B.size) {
-        // every row of the block in one multiply-add per column: the pivot row
-        // scales itself (beta = 1/d - 1 against its own entries), the others
-        // eliminate (beta = -f/d); column p becomes 1/d, resp. -f/d
-        // the whole pivot row first: one wait for all its loads, instead of the
-        // load-wait-multiply-add chain the scheduler otherwise emits
-        double pv[BMAX];
-#pragma unroll
-        for (int c = 0; c < BMAX; ++c) pv[c] = pb[c];
-        const double inv = pv[p];
-        const double beta = piv ? inv - 1.0 : -F.row[p] * inv;
-#pragma unroll
-        for (int c = 0; c < BMAX; ++c)
-          if (c != p) F.row[c] = fma(beta, pv[c], F.row[c]);
-        F.row[p] = piv ? inv : beta;
+    }
+  } else {
+    // value-only kernel (three wavefronts per SIMD): one pivot at a time keeps a
+    // single pivot row in registers
+    // fully unrolled over the pivot index, so F.row[p] is a fixed register
+    // (maxb is the same for every thread: the guard is uniform)
+  #pragma unroll
+    for (int p = 0; p < BMAX; ++p) {
+      if (p < maxb) {
+        double* pb = sm.pbuf[p & 1][B.id][0];
+        const bool in = p < B.size;
+        const bool piv = in && B.lr == p;
+        if (piv) {  // the pivot row as it stands, 1/d in the pivot column
+          const double d = F.row[p];
+          if (!(fabs(d) > 1e-12)) bad = 1.0;
+  #pragma unroll
+          for (int c = 0; c < BMAX; ++c) pb[c] = (c == p) ? 1.0 / d : F.row[c];
+        }
+        if (wl) wave_sync();
+        else __syncthreads();
+        if (in && B.lr < B.size) {
+          // every row of the block in one multiply-add per column: the pivot row
+          // scales itself (beta = 1/d - 1 against its own entries), the others
+          // eliminate (beta = -f/d); column p becomes 1/d, resp. -f/d
+          // the whole pivot row first: one wait for all its loads, instead of the
+          // load-wait-multiply-add chain the scheduler otherwise emits
+          double pv[BMAX];
+  #pragma unroll
+          for (int c = 0; c < BMAX; ++c) pv[c] = pb[c];
+          const double inv = pv[p];
+          const double beta = piv ? inv - 1.0 : -F.row[p] * inv;
+  #pragma unroll
+          for (int c = 0; c < BMAX; ++c)
+            if (c != p) F.row[c] = fma(beta, pv[c], F.row[c]);
+          F.row[p] = piv ? inv : beta;
+        }
       }
     }
   }
