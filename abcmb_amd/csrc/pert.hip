@@ -726,7 +726,97 @@ __device__ bool factor(Smem<NT, BMAX, N>& sm, const RowState& R, int& rs, int ma
     }
   }
   double bad = 0.0;
-  if constexpr (N > 0) {
+  // pivots three at a time where the stage-derivative buffer (dead during the
+  // factor) holds the double-buffered pivot rows after the scattered rows
+  constexpr bool TRIPLES = N > 0 && 7 * (1 + N) >= BMAX &&
+                           7 * (1 + N) * NT >= NT * BMAX + 2 * NBLK * 3 * BMAX;
+  if constexpr (TRIPLES) {
+    // Gauss-Jordan in steps of k = min(3, rows left in the block) pivots: the
+    // k pivot rows P_m are exchanged at once, their k x k pivot block D is
+    // inverted in registers (padded to 3 x 3 with the identity), and every row
+    // of the block becomes row + sum_m b_m P_m outside the step's columns
+    // (rows outside the step: b = -t, t = (row_p .. row_p+k-1) D^-1, which
+    // also fills the step's columns with -t; pivot row m: b = D^-1 row m - e_m,
+    // the step's columns D^-1 row m) -- the same in-place inverse as k single
+    // pivots, and the singularity test checks the pivots D's elimination meets,
+    // which are the ones sequential elimination would see.
+    double* const pbase = &sm.Fs[0][0][0] + NT * BMAX;
+#pragma unroll
+    for (int p = 0; p < BMAX; p += 3) {
+      if (p < maxb) {
+        double* pb = pbase + (((p / 3) & 1) * NBLK + B.id) * 3 * BMAX;
+        const bool in = p < B.size;
+        const int kk = in ? min(3, B.size - p) : 0;
+        if (in && B.lr >= p && B.lr < p + kk) {   // the pivot rows as they stand
+          double* dst = pb + (B.lr - p) * BMAX;
+#pragma unroll
+          for (int c = 0; c < BMAX; ++c) dst[c] = F.row[c];
+        }
+        if (wl) wave_sync();
+        else __syncthreads();
+        if (in && B.lr < B.size) {
+          // rows past the step's k read row 0 again (finite; their weights are 0)
+          const double* s1 = pb + (kk > 1 ? BMAX : 0);
+          const double* s2 = pb + (kk > 2 ? 2 * BMAX : 0);
+          double p0[BMAX], p1[BMAX], p2[BMAX];
+#pragma unroll
+          for (int c = 0; c < BMAX; ++c) { p0[c] = pb[c]; p1[c] = s1[c]; p2[c] = s2[c]; }
+          // D padded with the identity, inverted in place without pivoting
+          double Dm[3][3], Di[3][3];
+#pragma unroll
+          for (int m = 0; m < 3; ++m) {
+#pragma unroll
+            for (int q = 0; q < 3; ++q) {
+              const double* pm = m == 0 ? p0 : (m == 1 ? p1 : p2);
+              Dm[m][q] = (m < kk && q < kk && p + q < BMAX) ? pm[p + q < BMAX ? p + q : 0] : (m == q ? 1.0 : 0.0);
+              Di[m][q] = m == q ? 1.0 : 0.0;
+            }
+          }
+#pragma unroll
+          for (int q = 0; q < 3; ++q) {
+            const double piv = Dm[q][q];
+            if (q < kk && !(fabs(piv) > 1e-12)) bad = 1.0;
+            const double iv = 1.0 / piv;
+#pragma unroll
+            for (int c = 0; c < 3; ++c) { Dm[q][c] *= iv; Di[q][c] *= iv; }
+#pragma unroll
+            for (int m = 0; m < 3; ++m) {
+              if (m != q) {
+                const double f = Dm[m][q];
+#pragma unroll
+                for (int c = 0; c < 3; ++c) { Dm[m][c] -= f * Dm[q][c]; Di[m][c] -= f * Di[q][c]; }
+              }
+            }
+          }
+          const int m0 = B.lr - p;
+          const bool pivrow = m0 >= 0 && m0 < kk;
+          double bt[3], rk[3];
+          double mr[3];
+#pragma unroll
+          for (int q = 0; q < 3; ++q) mr[q] = (q < kk && p + q < BMAX) ? F.row[p + q < BMAX ? p + q : 0] : 0.0;
+#pragma unroll
+          for (int q = 0; q < 3; ++q) {
+            const double t = mr[0] * Di[0][q] + mr[1] * Di[1][q] + mr[2] * Di[2][q];
+            const double dq = m0 == 0 ? Di[0][q] : (m0 == 1 ? Di[1][q] : Di[2][q]);
+            bt[q] = pivrow ? dq - (q == m0 ? 1.0 : 0.0) : -t;
+            rk[q] = pivrow ? dq : -t;
+          }
+#pragma unroll
+          for (int c = 0; c < BMAX; ++c) {
+            const bool step_col = c >= p && c < p + 3;
+            if (!step_col) F.row[c] = fma(bt[2], p2[c], fma(bt[1], p1[c], fma(bt[0], p0[c], F.row[c])));
+          }
+#pragma unroll
+          for (int q = 0; q < 3; ++q) {
+            if (p + q < BMAX) {
+              const int c = p + q;
+              F.row[c] = q < kk ? rk[q] : fma(bt[2], p2[c], fma(bt[1], p1[c], fma(bt[0], p0[c], F.row[c])));
+            }
+          }
+        }
+      }
+    }
+  } else if constexpr (N > 0) {
     // Gauss-Jordan in pairs of pivots (p, p + 1): one exchange of the two pivot
     // rows per pair instead of one row per pivot.  With the pivot rows P0, P1,
     // the 2x2 pivot block D = [[a, b], [c, d]] and every row written as
