@@ -302,11 +302,22 @@ def tangent_gradients(model, like, thetas, k_grids=None, mock_index=None, phases
     directional = _directional_losses_linear if downstream == "tangent" else _directional_losses
     with ThreadPoolExecutor(1) as host:
         futs = [host.submit(_tangent_inputs, model, thetas[a:b], k_grids[a:b], steps[a:b]) for a, b in bounds]
-        for (a, b), fut in zip(bounds, futs):
+
+        def launch(i):
+            nonlocal t_host
             t0 = time.perf_counter()
-            base_plans, pm, seeds = fut.result()
+            base_plans, pm, seeds = futs[i].result()
+            t_host += time.perf_counter() - t0
+            return base_plans, pm, model.launch_tangent(base_plans, seeds)
+
+        cur = launch(0)
+        for i, (a, b) in enumerate(bounds):
+            # the next chunk's ODE launch goes into the stream ahead of this
+            # chunk's downstream, whose many small launches the host enqueues
+            # while the GPU runs it
+            nxt = launch(i + 1) if i + 1 < len(bounds) else None
             t1 = time.perf_counter()
-            h = model.launch_tangent(base_plans, seeds)
+            base_plans, pm, h = cur
             run = model._ode_result(h)
             ds = model._downstream(base_plans, run["pts"], run["kdev"])
             outs, dpts = model._outputs(base_plans, run, ds), run["dpts"]
@@ -318,8 +329,8 @@ def tangent_gradients(model, like, thetas, k_grids=None, mock_index=None, phases
             # this chunk's work on the same stream)
             handles.append({key: h[key] for key in ("done", "flat", "allk", "status", "steps", "iters", "ev",
                                                      "tangent")})
-            del run, outs, dpts, ds, h
-            t_host += t1 - t0
+            del run, outs, dpts, ds, h, cur
+            cur = nxt
             t_enq += time.perf_counter() - t1
     t0 = time.perf_counter()
     Lb = torch.cat(Lb_d).cpu().numpy()
