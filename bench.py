@@ -258,7 +258,8 @@ def main():
         "vs_baseline": None, "dtype": "f64", "data": "synthetic",
         "config": {"workload": f"batched lensed-Cl likelihood gradients, {args.batch} Latin-hypercube cosmologies "
                                f"(massive nu, uneven 300-800 k-grids, l<=2500, FP64 ODE + FP64 LOS)",
-                   "gradient": (f"tangent-linear ODE (5 directions, one launch per {args.chunk} cosmologies) + "
+                   "gradient": (f"tangent-linear ODE (5 directions, one launch per chunk: {args.chunk // 8}, "
+                                f"{args.chunk // 4}, {args.chunk // 2}, then {args.chunk} cosmologies) + "
                                 + ("downstream differenced along 7 directions" if args.fd_downstream
                                    else "tangent-linear LOS, lensing differenced along 7 directions")
                                 if args.gradient == "tangent" else
