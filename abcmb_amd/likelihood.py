@@ -160,9 +160,13 @@ def _directional_losses(model, like, thetas, steps, base_plans, pm, outs, dpts, 
     downstream on tables moved along their tangents (ODE directions) or with
     another primordial spectrum (ln A_s, n_s)."""
     import torch
+    from .model import _moved_tables
     nc = thetas.shape[0]
     per = 2 * 7
     group = max(1, model.chunk // per)
+    _, _, stage = model._staging()
+    hs_d = stage.add(np.stack([steps, -steps], axis=-1)[:, list(ODE_DIRS)])   # [nc][nd][2]
+    stage.flush()
     Lp = []
     for c0 in range(0, nc, group):
         cs = range(c0, min(nc, c0 + group))
@@ -170,13 +174,14 @@ def _directional_losses(model, like, thetas, steps, base_plans, pm, outs, dpts, 
         for c in cs:
             pt = outs[c].PT.table
             kd = outs[c].PT.k_dev
+            moved = _moved_tables(torch, pt, dpts[c], hs_d[c])   # pt + h dpt, all directions and signs
             for p in range(7):
                 for si, sgn in enumerate((+1, -1)):
                     h = sgn * steps[c, p]
                     if p in ODE_DIRS:
                         d = ODE_DIRS.index(p)
                         plans.append(pm[c][2 * d + si])
-                        pts.append(pt + h * dpts[c][d])
+                        pts.append(moved[2 * d + si])
                     else:
                         t = thetas[c].copy()
                         t[p] += h
@@ -187,6 +192,7 @@ def _directional_losses(model, like, thetas, steps, base_plans, pm, outs, dpts, 
                     slots.append(mock_index[c])
         ds = model._downstream(plans, pts, kdev)
         Lp.append(like.loglike_cl(model.observed_cl(ds), slots).reshape(len(cs), 7, 2))
+        del pts, moved
     return torch.cat(Lp)
 
 
