@@ -155,7 +155,7 @@ def _tangent_inputs(model, thetas, k_grids, steps):
     return base_plans, pm, seeds
 
 
-def _directional_losses(model, like, thetas, steps, base_plans, pm, outs, dpts, mock_index):
+def _directional_losses(model, like, thetas, steps, base_plans, pm, pts_b, kdev_b, dpts, mock_index):
     """L at theta +- step along all seven directions ([nc][7][2], on the GPU): the
     downstream on tables moved along their tangents (ODE directions) or with
     another primordial spectrum (ln A_s, n_s)."""
@@ -172,8 +172,8 @@ def _directional_losses(model, like, thetas, steps, base_plans, pm, outs, dpts, 
         cs = range(c0, min(nc, c0 + group))
         plans, pts, kdev, slots = [], [], [], []
         for c in cs:
-            pt = outs[c].PT.table
-            kd = outs[c].PT.k_dev
+            pt = pts_b[c]
+            kd = kdev_b[c]
             moved = _moved_tables(torch, pt, dpts[c], hs_d[c])   # pt + h dpt, all directions and signs
             for p in range(7):
                 for si, sgn in enumerate((+1, -1)):
@@ -196,17 +196,16 @@ def _directional_losses(model, like, thetas, steps, base_plans, pm, outs, dpts, 
     return torch.cat(Lp)
 
 
-def _directional_losses_linear(model, like, thetas, steps, base_plans, pm, outs, dpts, mock_index):
-    """L at theta +- step along all seven directions ([nc][7][2], on the GPU) with the unlensed
-    C_l moved along their derivatives from the tangent-linear LOS
-    (``Model.downstream_tangent``); only C_l^phiphi and the lensing are
-    evaluated at theta +- step."""
+def _directional_losses_linear(model, like, thetas, steps, base_plans, pm, pts, kdev, dpts, mock_index):
+    """L at theta ([nc]) and at theta +- step along all seven directions ([nc][7][2]),
+    on the GPU, with the unlensed C_l of theta and their derivatives from the
+    tangent-linear LOS (``Model.downstream_tangent``): the base C_l are its
+    value, the moved C_l are C_l +- h dC_l; only C_l^phiphi and the lensing are
+    evaluated at each point (the base one with theta's tables and plan)."""
     import torch
     nc = thetas.shape[0]
     nd = len(ODE_DIRS)
     n_l = model.ells.size
-    pts = [o.PT.table for o in outs]
-    kdev = [o.PT.k_dev for o in outs]
     plus = [pm[c][0::2] for c in range(nc)]
     minus = [pm[c][1::2] for c in range(nc)]
     sd = [[steps[c, p] for p in ODE_DIRS] for c in range(nc)]
@@ -225,19 +224,22 @@ def _directional_losses_linear(model, like, thetas, steps, base_plans, pm, outs,
     # C(theta +- h e_p) = C + (+-h) dC/dp for every cosmology, direction and sign: [nc][7][2][3][nl]
     cls_all = torch.addcmul(cl_t[:, 0][:, None, None], H_d[:, :, :, None, None],
                             cl_t.index_select(1, col_d)[:, :, None])
-    per = 2 * 7
+    per = 1 + 2 * 7      # theta, then (direction, sign)
     group = max(1, model.chunk // per)
-    Lp = []
+    L = []
     for c0 in range(0, nc, group):
         cs = range(c0, min(nc, c0 + group))
         plans, vpts, vk, slots = [], [], [], []
         for c in cs:
             moved = _moved_tables(torch, pts[c], dpts[c], hs_d[c]) if lensing else None
+            slots += [mock_index[c]] * per
+            if not lensing:
+                continue
+            plans.append(base_plans[c])
+            vpts.append(pts[c])
+            vk.append(kdev[c])
             for p in range(7):
                 for si, sgn in enumerate((+1, -1)):
-                    slots.append(mock_index[c])
-                    if not lensing:
-                        continue
                     if p in ODE_DIRS:
                         d = ODE_DIRS.index(p)
                         plans.append(pm[c][2 * d + si])
@@ -249,14 +251,17 @@ def _directional_losses_linear(model, like, thetas, steps, base_plans, pm, outs,
                         plans.append(with_primordial(base_plans[c], q["A_s"], q["n_s"]))
                         vpts.append(pts[c])
                     vk.append(kdev[c])
-        cl = cls_all[c0:c0 + len(cs)].reshape(len(cs) * per, 3, -1)
+        n = len(cs)
+        cl = torch.cat([cl_t[c0:c0 + n, 0][:, None], cls_all[c0:c0 + n].reshape(n, per - 1, 3, -1)], dim=1)
+        cl = cl.reshape(n * per, 3, -1)
         if lensing:
             obs = model.lensed_from_cl(plans, vpts, vk, cl)[:, :3, :n_l]
         else:
             obs = cl[:, :, :n_l]
-        Lp.append(like.loglike_cl(obs, slots).reshape(len(cs), 7, 2))
+        L.append(like.loglike_cl(obs, slots).reshape(n, per))
         del vpts, moved
-    return torch.cat(Lp)
+    L = torch.cat(L)
+    return L[:, 0], L[:, 1:].reshape(nc, 7, 2)
 
 
 def tangent_gradients(model, like, thetas, k_grids=None, mock_index=None, phases=None, chunk: int = 64,
@@ -305,7 +310,6 @@ def tangent_gradients(model, like, thetas, k_grids=None, mock_index=None, phases
     # launches' status is checked once at the end.
     Lb_d, Lp_d, handles = [], [], []
     t_host = t_enq = 0.0
-    directional = _directional_losses_linear if downstream == "tangent" else _directional_losses
     with ThreadPoolExecutor(1) as host:
         futs = [host.submit(_tangent_inputs, model, thetas[a:b], k_grids[a:b], steps[a:b]) for a, b in bounds]
 
@@ -325,17 +329,25 @@ def tangent_gradients(model, like, thetas, k_grids=None, mock_index=None, phases
             t1 = time.perf_counter()
             base_plans, pm, h = cur
             run = model._ode_result(h)
-            ds = model._downstream(base_plans, run["pts"], run["kdev"])
-            outs, dpts = model._outputs(base_plans, run, ds), run["dpts"]
-            Lb_d.append(like.loglike_cl(model.observed_cl(ds), mock_index[a:b]))
-            Lp_d.append(directional(model, like, thetas[a:b], steps[a:b], base_plans, pm, outs, dpts,
-                                    mock_index[a:b]))
+            args = (model, like, thetas[a:b], steps[a:b], base_plans, pm, run["pts"], run["kdev"], run["dpts"],
+                    mock_index[a:b])
+            if downstream == "tangent":
+                # the tangent-linear LOS also gives theta's own C_l: L(theta) is
+                # one more lensing variant, no separate base downstream
+                lb, lp = _directional_losses_linear(*args)
+            else:
+                ds = model._downstream(base_plans, run["pts"], run["kdev"])
+                lb = like.loglike_cl(model.observed_cl(ds), mock_index[a:b])
+                lp = _directional_losses(*args)
+                del ds
+            Lb_d.append(lb)
+            Lp_d.append(lp)
             # keep only what the status check reads: the tables and staging
             # buffers go back to the allocator (their reuse is ordered after
             # this chunk's work on the same stream)
             handles.append({key: h[key] for key in ("done", "flat", "allk", "status", "steps", "iters", "ev",
                                                      "tangent")})
-            del run, outs, dpts, ds, h, cur
+            del run, args, h, cur
             cur = nxt
             t_enq += time.perf_counter() - t1
     t0 = time.perf_counter()
