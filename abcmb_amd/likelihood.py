@@ -231,7 +231,8 @@ def _directional_losses_linear(model, like, thetas, steps, base_plans, pm, pts, 
         cs = range(c0, min(nc, c0 + group))
         plans, vpts, vk, slots = [], [], [], []
         for c in cs:
-            moved = _moved_tables(torch, pts[c], dpts[c], hs_d[c]) if lensing else None
+            # the lensing reads only delta_m, the tables' first field: move that alone
+            moved = _moved_tables(torch, pts[c][:1], dpts[c][:, :1], hs_d[c]) if lensing else None
             slots += [mock_index[c]] * per
             if not lensing:
                 continue

@@ -268,7 +268,9 @@ int abcmb_apply_operator(const double* op_dev, int rows, int cols,
 
 /* Limber lensing potential C_l^phiphi (abcmb/spectrum.py:349-435).
  * lens_dev: [B][4][n_lna] = ln a axis (lna_rec .. 0), aH and tau on it, and
- * chi(lna_rec) at [3][0]; ells_dev: [n_l]; clpp_dev out: [B][n_l]. */
+ * chi(lna_rec) at [3][0]; ells_dev: [n_l]; clpp_dev out: [B][n_l].  Of each
+ * cosmology's perturbation table only field ABCMB_PT_DELTA_M (the first, [0])
+ * is read, so pt may point to a table holding that field alone. */
 int abcmb_lensing_clpp(const AbcmbCosmo* cosmos_dev, int n_cosmo,
                        const double* lens_dev, int n_lna,
                        const double* ells_dev, int n_l, double* clpp_dev,
