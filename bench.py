@@ -161,8 +161,8 @@ def main():
         copy_calibration(args.copy_calibration)
         return
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU only; multi-GPU sharding is described in "
-                                   "DESIGN.md and not launched"}))
+        print(json.dumps({"error": "this build runs on one GPU only; sharding the cosmologies across GPUs "
+                                   "is designed but not launched"}))
         sys.exit(2)
     import torch
     from abcmb_amd import likelihood as LK
