@@ -303,15 +303,21 @@ __global__ void __launch_bounds__(NTH) los_cl_kernel(const AbcmbCosmo* __restric
 // r = sqrt(x^2 - nu^2), Q = r - nu pi/2 + nu asin(nu/x): dQ/dx = r/x, dr/dx = x/r
 __device__ __forceinline__ void j_asym_d(double l, double x, double& j, double& dj) {
   const double pi = PI;
-  double nu = l + 0.5;
-  double r = sqrt(x * x - nu * nu);
-  double Q = r - nu * pi / 2.0 + nu * asin(nu / x);
-  double c = cos(Q - pi / 4.0), sn = sin(Q - pi / 4.0);
-  double J = sqrt(2.0 / pi / r) * c;
-  double pre = sqrt(pi / 2.0 / x);
+  constexpr double SQRT_2_PI = 0.79788456080286535588;   // sqrt(2 / pi)
+  const double nu = l + 0.5;
+  const double ix = 1.0 / x;
+  const double r = sqrt(x * x - nu * nu);
+  const double ir = 1.0 / r;
+  const double isr = 1.0 / sqrt(r);
+  const double Q = r - nu * pi / 2.0 + nu * asin(nu * ix);
+  double c, sn;
+  sincos(Q - pi / 4.0, &sn, &c);   // one argument reduction for both
+  const double J = SQRT_2_PI * isr * c;
+  const double pre = sqrt(pi / 2.0 * ix);
   j = pre * J;
-  double dJ = sqrt(2.0 / pi) * (-0.5 / (r * sqrt(r)) * (x / r) * c - sn / sqrt(r) * (r / x));
-  dj = -j / (2.0 * x) + pre * dJ;
+  // dJ/dx = sqrt(2/pi) (-c r^(-3/2) / 2 dr/dx - sin(.) r^(-1/2) dQ/dx)
+  const double dJ = SQRT_2_PI * isr * (-0.5 * ir * (x * ir) * c - sn * (r * ix));
+  dj = -0.5 * ix * j + pre * dJ;
 }
 
 // table value and slope of the interpolated segment (zero where the index is clipped)
