@@ -175,14 +175,14 @@ __device__ __forceinline__ T fast_interp_t(T x, T lo, T hi, const double* __rest
   return ((T)1 - wu) * (T)f[il] + wu * (T)f[iu];
 }
 
+// j_l(x) ~ sqrt(pi / 2x) sqrt(2 / (pi r)) cos(Q - pi/4) = cos(Q - pi/4) / sqrt(x r)
 template <typename T>
 __device__ __forceinline__ T j_asym(T l, T x) {
   const T pi = (T)PI;
   T nu = l + (T)0.5;
   T r = sqrt(x * x - nu * nu);
   T Q = r - nu * pi / (T)2 + nu * asin(nu / x);
-  T J = sqrt((T)2 / pi / r) * cos(Q - pi / (T)4);
-  return sqrt(pi / (T)2 / x) * J;
+  return cos(Q - pi / (T)4) / sqrt(x * r);
 }
 
 // phi_0 = j_l, phi_1 = j_l', phi_2 = (3 j_l'' + j_l) / 2 from the asymptotic j_l, j_{l+1}
