@@ -1346,7 +1346,12 @@ pert_fwd_kernel(const AbcmbCosmo* __restrict__ cosmos, const AbcmbTangent* __res
     for (int s = 1; s < 7; ++s) {
       if (!ok) break;
       D<N> acc(0.0);
-      for (int j = 0; j < s; ++j) acc += KV_A[s][j] * get_F(sm, j, r);
+      // all six stage derivatives read at once (one wait), the first s summed in order
+#pragma unroll
+      for (int j = 0; j < 6; ++j) {
+        const D<N> fj = get_F(sm, j, r);
+        if (j < s) acc += KV_A[s][j] * fj;
+      }
       const D<N> base = y + hh * acc;
       z = base + g * get_F(sm, s - 1, r);
       const int slot = STAGE_SLOT[s];
