@@ -1167,20 +1167,22 @@ __device__ __forceinline__ void fields_at(const AbcmbCosmo& c, const AbcmbTangen
   });
 }
 
-// how many of a launch's items fall on each cosmology (cnt zeroed first by a
-// fill kernel, so that the sequence can sit in a captured graph)
+// how many of a launch's successful items fall on each cosmology (cnt zeroed
+// first by a fill kernel, so that the sequence can sit in a captured graph)
 __global__ void zero_counts_kernel(int32_t* __restrict__ cnt, int n) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) cnt[i] = 0;
 }
-__global__ void count_items_kernel(const AbcmbPertItem* __restrict__ items, int n_items, int32_t* __restrict__ cnt) {
+__global__ void count_items_kernel(const AbcmbPertItem* __restrict__ items, int n_items,
+                                   const int32_t* __restrict__ status, int32_t* __restrict__ cnt) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n_items) atomicAdd(&cnt[items[i].cosmo], 1);
+  if (i < n_items && status[i] == ABCMB_K_OK) atomicAdd(&cnt[items[i].cosmo], 1);
 }
 
-// cosmologies whose every k-mode is in the launch (items are distinct
-// (cosmology, k) pairs): one block per (cosmology, saved point), threads along
-// k, so that the table's k-contiguous rows are read and written coalesced
+// cosmologies whose every k-mode is in the launch and integrated without
+// failure (items are distinct (cosmology, k) pairs): one block per
+// (cosmology, saved point), threads along k, so that the table's
+// k-contiguous rows are read and written coalesced
 template <int N>
 __global__ void __launch_bounds__(64) fields_cosmo_kernel(const AbcmbCosmo* __restrict__ cosmos,
                                                           const AbcmbTangent* __restrict__ tangents,
@@ -1195,7 +1197,10 @@ __global__ void __launch_bounds__(64) fields_cosmo_kernel(const AbcmbCosmo* __re
   for (int ik = threadIdx.x; ik < c.nk; ik += 64) fields_at<N>(c, T, st, bg, c.k[ik], ik, isave);
 }
 
-// the other items (cosmologies covered in part), one thread per (item, saved point)
+// the other items (cosmologies covered in part or with a failed item), one
+// thread per (item, saved point); a failed item's columns are zeroed, so that
+// work enqueued behind the launch reads defined values until the caller has
+// seen the status words
 template <int N>
 __global__ void __launch_bounds__(64) fields_kernel(const AbcmbCosmo* __restrict__ cosmos,
                                                     const AbcmbTangent* __restrict__ tangents,
@@ -1206,12 +1211,20 @@ __global__ void __launch_bounds__(64) fields_kernel(const AbcmbCosmo* __restrict
   const long long idx = (long long)blockIdx.x * 64 + threadIdx.x;
   if (idx >= (long long)n_items * n_save_max) return;
   const int item = (int)(idx / n_save_max), isave = (int)(idx % n_save_max);
-  if (status[item] != ABCMB_K_OK) return;
   const AbcmbPertItem I = items[item];
   const AbcmbCosmo& c = cosmos[I.cosmo];
   if (isave >= c.n_save || cnt[I.cosmo] == c.nk) return;
   const AbcmbTangent* T = nullptr;
   if constexpr (N > 0) T = tangents + I.cosmo;
+  if (status[item] != ABCMB_K_OK) {
+    const size_t dstride = (size_t)ABCMB_N_PT_FIELDS * c.n_save * c.nk;
+    for (int f = 0; f < ABCMB_N_PT_FIELDS; ++f) {
+      const size_t o = ((size_t)f * c.n_save + isave) * c.nk + I.ik;
+      c.pt[o] = 0.0;
+      if constexpr (N > 0) { ABCMB_DLOOP(N) T->pt[i_ * dstride + o] = 0.0; }
+    }
+    return;
+  }
   fields_at<N>(c, T, st, bgsave[(size_t)I.cosmo * n_save_max + isave], I.k, I.ik, isave);
 }
 
@@ -1472,7 +1485,8 @@ int launch(const AbcmbCosmo* cosmos, const AbcmbTangent* tangents, int n_cosmo, 
   ABCMB_HIP_CHECK(hipGetLastError());
   if (12 + 3 * st.n_q <= ABCMB_N_PT_FIELDS) {  // the kernel stored rows: turn them into fields
     hipLaunchKernelGGL(zero_counts_kernel, dim3((n_cosmo + 255) / 256), dim3(256), 0, stream, cnt, n_cosmo);
-    hipLaunchKernelGGL(count_items_kernel, dim3((n_items + 255) / 256), dim3(256), 0, stream, items, n_items, cnt);
+    hipLaunchKernelGGL(count_items_kernel, dim3((n_items + 255) / 256), dim3(256), 0, stream, items, n_items, status,
+                       cnt);
     hipLaunchKernelGGL((fields_cosmo_kernel<N>), dim3((unsigned)n_cosmo * n_save_max), dim3(64), 0, stream, cosmos,
                        tangents, st, bgsave, n_save_max, cnt);
     const long long nthr = (long long)n_items * n_save_max;
@@ -1574,3 +1588,4 @@ extern "C" int abcmb_pert_tangent(const AbcmbCosmo* cosmos_dev, const AbcmbTange
                                       n_save_max, workspace, status_dev, steps_dev, iters_dev, nullptr,
                                       (hipStream_t)stream);
 }
+

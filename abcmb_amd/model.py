@@ -691,8 +691,8 @@ class Model:
         ds = self._downstream(plans, run["pts"], run["kdev"])
         return self._outputs(plans, run, ds), run["dpts"]
 
-    def __call__(self, params: dict = {}) -> Output:
-        return self.run_batch([params])[0]
+    def __call__(self, params: dict = None) -> Output:
+        return self.run_batch([params if params is not None else {}])[0]
 
 
 def _moved_tables(torch, pt, dpt, hs):
@@ -735,7 +735,7 @@ class _Stage:
 
     def __init__(self, torch, dev):
         self.torch, self.dev = torch, dev
-        self.segs = []      # [host, device, used bytes, host as numpy]
+        self.segs = []      # [host, device, used bytes, host as numpy, bytes already copied]
         self.dirty = False
 
     def add(self, a, dtype=None):
@@ -746,7 +746,7 @@ class _Stage:
         if not self.segs or self.segs[-1][2] + nb > self.segs[-1][0].numel():
             size = max(self.SEG, nb)
             host = torch.empty(size, dtype=torch.uint8, pin_memory=True)
-            self.segs.append([host, torch.empty(size, dtype=torch.uint8, device=self.dev), 0, host.numpy()])
+            self.segs.append([host, torch.empty(size, dtype=torch.uint8, device=self.dev), 0, host.numpy(), 0])
         seg = self.segs[-1]
         off = seg[2]
         seg[3][off:off + nb] = arr.reshape(-1).view(np.uint8)     # numpy byte copy: no torch CPU threads
@@ -755,10 +755,14 @@ class _Stage:
         return seg[1][off:off + nb].view(dtype).view(arr.shape)
 
     def flush(self):
+        """Copy what was added since the last flush (each byte goes over once)."""
         if self.dirty:
-            for host, devb, used, _ in self.segs:
+            for seg in self.segs:
+                host, devb, used, done = seg[0], seg[1], seg[2], seg[4]
                 n = min(used, host.numel())
-                devb[:n].copy_(host[:n], non_blocking=True)
+                if n > done:
+                    devb[done:n].copy_(host[done:n], non_blocking=True)
+                    seg[4] = n
             self.dirty = False
 
 

@@ -1,8 +1,9 @@
 """Extract the multipole list and x-ranges of ABCMB's tabulated spherical Bessel
 functions (abcmb/bessel_tab/{l,xphi0,xphi1,xphi2}.txt, loaded at
-abcmb/spectrum.py:20-27) into bessel_ranges.json.  The table values themselves
-are regenerated with scipy by abcmb_amd.bessel; only the grid definition is
-taken from the reference data files.
+abcmb/spectrum.py:20-27) into abcmb_amd/data/bessel_ranges.json, the grid
+definition the package ships.  The table values themselves are regenerated
+with scipy by abcmb_amd.bessel; only the grid definition is taken from the
+reference data files.  Run once by hand; nothing in the package imports it.
 
 Usage: python make_bessel_ranges.py /path/to/reference/abcmb/bessel_tab
 """
@@ -19,6 +20,7 @@ for f in range(3):
     x = np.loadtxt(os.path.join(src, f"xphi{f}.txt"))
     assert x.shape == (5000, ls.size)
     out[f"range{f}"] = [[float(x[:, i].min()), float(x[:, i].max())] for i in range(ls.size)]
-with open(os.path.join(os.path.dirname(os.path.abspath(__file__)), "bessel_ranges.json"), "w") as fh:
+root = os.path.dirname(os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
+with open(os.path.join(root, "abcmb_amd", "data", "bessel_ranges.json"), "w") as fh:
     json.dump(out, fh)
 print("wrote", ls.size, "multipoles")
