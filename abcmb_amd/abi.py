@@ -97,7 +97,7 @@ HIP_SYMBOLS = (
     "abcmb_last_error", "abcmb_device_init", "abcmb_pert_workspace_size", "abcmb_pert_fwd", "abcmb_pert_tangent",
     "abcmb_sources_workspace_size", "abcmb_sources", "abcmb_los_workspace_size", "abcmb_los_cl", "abcmb_apply_operator",
     "abcmb_los_tangent_workspace_size", "abcmb_los_tangent",
-    "abcmb_lensing_clpp", "abcmb_lens_workspace_size", "abcmb_lens", "abcmb_pk_lin",
+    "abcmb_lensing_clpp", "abcmb_lens_workspace_size", "abcmb_lens", "abcmb_pk_lin", "abcmb_pert_fwd_noalloc",
 )
 
 
@@ -115,6 +115,8 @@ def hip_lib():
         lib.abcmb_pert_workspace_size.argtypes = [C.c_int, C.c_int, C.c_int]
         lib.abcmb_pert_fwd.argtypes = [_P, C.c_int, _P, C.c_int, C.POINTER(AbcmbOdeSettings), C.c_int,
                                        _P, _P, _P, _P, _P, C.c_size_t, _P]
+        lib.abcmb_pert_fwd_noalloc.argtypes = [_P, C.c_int, _P, C.c_int, C.POINTER(AbcmbOdeSettings), C.c_int,
+                                               _P, _P, _P, _P, _P]
         lib.abcmb_pert_tangent.argtypes = [_P, _P, C.c_int, _P, C.c_int, C.POINTER(AbcmbOdeSettings), C.c_int,
                                            _P, _P, _P, _P, C.c_size_t, _P]
         lib.abcmb_sources_workspace_size.restype = C.c_size_t

@@ -8,6 +8,8 @@ namespace abcmb {
 
 static thread_local char g_err[512] = {0};
 
+int scratch_address(void** out);  // pert.hip
+
 void set_error(const char* fmt, ...) {
   va_list ap;
   va_start(ap, fmt);
@@ -28,5 +30,6 @@ extern "C" int abcmb_device_init(void) {
     abcmb::set_error("device %d is %s; this library is built for gfx950 only", dev, prop.gcnArchName);
     return ABCMB_EINVAL;
   }
-  return ABCMB_OK;
+  void* scratch = nullptr;  // the allocation-free entry point's per-device scratch address
+  return abcmb::scratch_address(&scratch);
 }

@@ -1589,3 +1589,53 @@ extern "C" int abcmb_pert_tangent(const AbcmbCosmo* cosmos_dev, const AbcmbTange
                                       (hipStream_t)stream);
 }
 
+// ---------------------------------------------------------------- launch without allocation
+// A fixed scratch array per device for callers that cannot allocate (graph
+// capture, tight serving loops): it holds what abcmb_pert_fwd's workspace
+// holds (the saved-point background table and the per-cosmology item counts),
+// both fully rewritten by every call before they are read.
+namespace abcmb {
+static __device__ __align__(256) unsigned char scratch[ABCMB_NOALLOC_SCRATCH_BYTES];
+
+namespace {
+constexpr int MAX_DEVICES = 64;
+void* g_scratch_ptr[MAX_DEVICES] = {nullptr};
+}  // namespace
+
+// The device address of `scratch` on the current device (the array's name is
+// a host-side handle, not a device address).  Looked up once per device.
+int scratch_address(void** out) {
+  int dev = 0;
+  ABCMB_HIP_CHECK(hipGetDevice(&dev));
+  ABCMB_REQUIRE(dev >= 0 && dev < MAX_DEVICES, "device index %d out of range", dev);
+  void* p = __atomic_load_n(&g_scratch_ptr[dev], __ATOMIC_ACQUIRE);
+  if (!p) {
+    ABCMB_HIP_CHECK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    ABCMB_REQUIRE(p != nullptr, "no device address for the scratch array");
+    __atomic_store_n(&g_scratch_ptr[dev], p, __ATOMIC_RELEASE);
+  }
+  *out = p;
+  return ABCMB_OK;
+}
+}  // namespace abcmb
+
+extern "C" int abcmb_pert_fwd_noalloc(const AbcmbCosmo* cosmos_dev, int n_cosmo, const AbcmbPertItem* items_dev,
+                                      int n_items, const AbcmbOdeSettings* settings, int n_save_max,
+                                      int32_t* status_dev, int32_t* steps_dev, int32_t* iters_dev,
+                                      double* state_dev, void* stream) {
+  using namespace abcmb;
+  ABCMB_REQUIRE(cosmos_dev && items_dev && settings && status_dev && steps_dev, "null argument");
+  ABCMB_REQUIRE(n_cosmo > 0 && n_items > 0 && n_save_max > 0, "empty launch");
+  const size_t need = abcmb_pert_workspace_size(n_cosmo, n_save_max, 0);
+  ABCMB_REQUIRE(need <= (size_t)ABCMB_NOALLOC_SCRATCH_BYTES,
+                "launch needs %zu bytes of scratch (n_cosmo=%d, n_save_max=%d); the fixed scratch holds %d",
+                need, n_cosmo, n_save_max, (int)ABCMB_NOALLOC_SCRATCH_BYTES);
+  int maxb = 0;
+  int rc = check_settings(*settings, maxb);
+  if (rc != ABCMB_OK) return rc;
+  void* ws = nullptr;
+  rc = scratch_address(&ws);
+  if (rc != ABCMB_OK) return rc;
+  return dispatch<0>(cosmos_dev, nullptr, n_cosmo, items_dev, n_items, *settings, maxb, n_save_max, ws, status_dev,
+                     steps_dev, iters_dev, state_dev, (hipStream_t)stream);
+}

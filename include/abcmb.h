@@ -10,7 +10,8 @@
  *   - every pointer named *_dev is device memory owned by the caller;
  *     host structs are read at call time only;
  *   - nothing here allocates device memory: scratch comes in through
- *     `workspace` / `ws_bytes` (query the size first);
+ *     `workspace` / `ws_bytes` (query the size first), except for
+ *     abcmb_pert_fwd_noalloc, whose scratch is a fixed device array;
  *   - calls are stream-ordered on `stream` (a hipStream_t passed as void*);
  *   - return value 0 = success, otherwise an ABCMB_E* code; the message is
  *     available from abcmb_last_error() (thread-local).
@@ -198,6 +199,26 @@ int abcmb_pert_tangent(const AbcmbCosmo* cosmos_dev, const AbcmbTangent* tangent
                        const AbcmbOdeSettings* settings, int n_save_max,
                        int32_t* status_dev, int32_t* steps_dev, int32_t* iters_dev,
                        void* workspace, size_t ws_bytes, void* stream);
+
+/*
+ * abcmb_pert_fwd without a workspace argument and without any allocation, for
+ * callers that cannot allocate device memory (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc).  No counterpart in
+ * the reference.  Its scratch (what abcmb_pert_fwd's workspace holds) is a
+ * fixed device array of ABCMB_NOALLOC_SCRATCH_BYTES per GPU, kept until the
+ * process ends; a launch needing more (abcmb_pert_workspace_size(n_cosmo,
+ * n_save_max, 0) > ABCMB_NOALLOC_SCRATCH_BYTES, e.g. above 139 cosmologies of
+ * 500 saved points) is rejected with ABCMB_EINVAL.  Every call on one GPU
+ * shares that array, so calls that overlap on one GPU (different streams,
+ * host threads) race; calls ordered on one stream do not.  The arguments and
+ * results are those of abcmb_pert_fwd.
+ */
+#define ABCMB_NOALLOC_SCRATCH_BYTES (8 << 20)
+int abcmb_pert_fwd_noalloc(const AbcmbCosmo* cosmos_dev, int n_cosmo,
+                           const AbcmbPertItem* items_dev, int n_items,
+                           const AbcmbOdeSettings* settings, int n_save_max,
+                           int32_t* status_dev, int32_t* steps_dev, int32_t* iters_dev,
+                           double* state_dev, void* stream);
 
 /*
  * The entry points below work on a batch of n_cosmo cosmologies described by
