@@ -28,6 +28,7 @@
 #include <stdlib.h>
 #include <string.h>
 #include <stdint.h>
+#include <omp.h>
 
 #include "../include/abcmb.h"
 
@@ -1327,3 +1328,12 @@ int oracle_lens(const double* cl_in, const double* clpp, int n_l, const double* 
   free(ksi);
   return 0;
 }
+
+/* OpenMP threads of the oracle's parallel loops (the CPU-baseline timing runs
+ * it once on every host core and once on one). */
+int oracle_set_threads(int n) {
+  if (n < 1) return 1;
+  omp_set_num_threads(n);
+  return 0;
+}
+int oracle_max_threads(void) { return omp_get_max_threads(); }

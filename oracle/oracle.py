@@ -14,7 +14,9 @@ import subprocess
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_HERE, "liboracle.so")
+# ABCMB_ORACLE_LIB points the checker at another build of the same source (a
+# mutated copy, to see that the pins fail on a planted mistake)
+LIB_PATH = os.environ.get("ABCMB_ORACLE_LIB", os.path.join(_HERE, "liboracle.so"))
 _lib = None
 _P = C.c_void_p
 _D = C.c_double
@@ -49,6 +51,7 @@ def lib():
         L.oracle_clpp.argtypes = [C.POINTER(abi.AbcmbCosmo), _P, _P, _P, _P, _I, _D, _P, _I, _P]
         L.oracle_wigner.argtypes = [_I, _I, _P, _I, _I, _P]
         L.oracle_lens.argtypes = [_P, _P, _I, _P, _P, _I, _P]
+        L.oracle_set_threads.argtypes = [_I]
         _lib = L
     return _lib
 
@@ -200,3 +203,13 @@ def lens(cl_in, clpp_, mu, w):
     out = np.zeros((4, n_l))
     lib().oracle_lens(_p(cl_in), _p(clpp_), n_l, _p(mu), _p(w), mu.size, _p(out))
     return out
+
+
+def set_threads(n: int):
+    """OpenMP threads of the oracle's parallel loops."""
+    if lib().oracle_set_threads(int(n)) != 0:
+        raise ValueError(f"bad thread count {n}")
+
+
+def max_threads() -> int:
+    return int(lib().oracle_max_threads())
