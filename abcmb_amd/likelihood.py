@@ -196,12 +196,15 @@ def _directional_losses(model, like, thetas, steps, base_plans, pm, pts_b, kdev_
     return torch.cat(Lp)
 
 
-def _directional_losses_linear(model, like, thetas, steps, base_plans, pm, pts, kdev, dpts, mock_index):
+def _directional_losses_linear(model, like, thetas, steps, base_plans, pm, pts, kdev, dpts, mock_index,
+                               cl_out=None):
     """L at theta ([nc]) and at theta +- step along all seven directions ([nc][7][2]),
     on the GPU, with the unlensed C_l of theta and their derivatives from the
     tangent-linear LOS (``Model.downstream_tangent``): the base C_l are its
     value, the moved C_l are C_l +- h dC_l; only C_l^phiphi and the lensing are
-    evaluated at each point (the base one with theta's tables and plan)."""
+    evaluated at each point (the base one with theta's tables and plan).
+    ``cl_out`` (a list, optional) receives the observed C_l at theta
+    ([n][3][n_l] per group of cosmologies)."""
     import torch
     nc = thetas.shape[0]
     nd = len(ODE_DIRS)
@@ -260,13 +263,15 @@ def _directional_losses_linear(model, like, thetas, steps, base_plans, pm, pts, 
         else:
             obs = cl[:, :, :n_l]
         L.append(like.loglike_cl(obs, slots).reshape(n, per))
+        if cl_out is not None:
+            cl_out.append(obs.reshape(n, per, 3, -1)[:, 0])
         del vpts, moved
     L = torch.cat(L)
     return L[:, 0], L[:, 1:].reshape(nc, 7, 2)
 
 
 def tangent_gradients(model, like, thetas, k_grids=None, mock_index=None, phases=None, chunk: int = 64,
-                      downstream: str = "tangent"):
+                      downstream: str = "tangent", cl_out=None):
     """d L / d theta for each row of ``thetas`` from tangent-linear ODE launches.
 
     The cosmologies go through in chunks of ``chunk`` (one ODE launch each,
@@ -274,7 +279,8 @@ def tangent_gradients(model, like, thetas, k_grids=None, mock_index=None, phases
     prepares the next chunks' background tables and tangent seeds while the
     main thread enqueues the current chunk's ODE launch and downstream on the
     current stream without waiting for the GPU, so the launches run back to
-    back.  ``downstream``: "tangent"
+    back.  ``cl_out`` (a list, optional) receives the observed C_l at each
+    theta (tensors of [n][3][n_l], in order).  ``downstream``: "tangent"
     (default) differentiates the LOS in one tangent-linear pass (the Bessel
     lookups as interpolated segments and asymptotic forms, as automatic
     differentiation of the reference would) and differences only the
@@ -335,10 +341,13 @@ def tangent_gradients(model, like, thetas, k_grids=None, mock_index=None, phases
             if downstream == "tangent":
                 # the tangent-linear LOS also gives theta's own C_l: L(theta) is
                 # one more lensing variant, no separate base downstream
-                lb, lp = _directional_losses_linear(*args)
+                lb, lp = _directional_losses_linear(*args, cl_out=cl_out)
             else:
                 ds = model._downstream(base_plans, run["pts"], run["kdev"])
-                lb = like.loglike_cl(model.observed_cl(ds), mock_index[a:b])
+                obs = model.observed_cl(ds)
+                if cl_out is not None:
+                    cl_out.append(obs)
+                lb = like.loglike_cl(obs, mock_index[a:b])
                 lp = _directional_losses(*args)
                 del ds
             Lb_d.append(lb)

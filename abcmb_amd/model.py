@@ -210,6 +210,7 @@ class Model:
         self.los_precision = precisions[los_precision]
         self.chunk = 128   # cosmologies per downstream launch (bounds the source buffers)
         self.timing = None  # list: per-launch pert_fwd event timing and work counts when set
+        self.call_timing = None  # dict: entry-point name -> [(start event, end event, work)] when set
         self._dev = None
 
     # ------------------------------------------------------------------ host side
@@ -504,6 +505,19 @@ class Model:
             abi.check(lib.abcmb_pk_lin(cp, nb, D["k_pk"].data_ptr(), nq, pk[c0].data_ptr(), stream), "abcmb_pk_lin")
         return dict(cl_raw=cl_raw, cl_full=cl_full, clpp=clpp, lensed=lensed, pk=pk)
 
+    def _timed(self, name, work, fn):
+        """Run ``fn`` (one C-ABI call on the current stream), bracketed by HIP
+        events on that stream when ``call_timing`` is set."""
+        if self.call_timing is None:
+            return fn()
+        torch = _torch()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        rc = fn()
+        e1.record()
+        self.call_timing.setdefault(name, []).append((e0, e1, work))
+        return rc
+
     def _staging(self):
         torch = _torch()
         dev = torch.device("cuda")
@@ -592,11 +606,13 @@ class Model:
             ws = torch.empty(lib.abcmb_los_tangent_workspace_size(nb, nd, n_ell, nkt) // 8 + 1, dtype=torch.float64,
                              device=dev)
             raw = torch.empty((nb, nv, 3, n_ell), dtype=torch.float64, device=dev)
-            abi.check(lib.abcmb_los_tangent(cos_dev.data_ptr(), nb, src_b.data_ptr(), dsrc.data_ptr(), dchi_d.data_ptr(),
-                                            nd, n_tau, bgv_b.data_ptr(), D["kt"].data_ptr(), nkt, D["bes"].data_ptr(),
-                                            D["rng"].data_ptr(), D["bes_l"].data_ptr(), self.bes_l.size,
-                                            self.bes_tab.shape[2], D["ell_idx"].data_ptr(), n_ell, raw.data_ptr(),
-                                            ws.data_ptr(), ws.numel() * 8, stream), "abcmb_los_tangent")
+            abi.check(self._timed("abcmb_los_tangent", dict(n_cosmo=nb, n_ell=n_ell, nkt=nkt, n_tau=n_tau, n_dir=nd),
+                                  lambda: lib.abcmb_los_tangent(
+                                      cos_dev.data_ptr(), nb, src_b.data_ptr(), dsrc.data_ptr(), dchi_d.data_ptr(),
+                                      nd, n_tau, bgv_b.data_ptr(), D["kt"].data_ptr(), nkt, D["bes"].data_ptr(),
+                                      D["rng"].data_ptr(), D["bes_l"].data_ptr(), self.bes_l.size,
+                                      self.bes_tab.shape[2], D["ell_idx"].data_ptr(), n_ell, raw.data_ptr(),
+                                      ws.data_ptr(), ws.numel() * 8, stream)), "abcmb_los_tangent")
             abi.check(lib.abcmb_apply_operator(D["ell_op"].data_ptr(), nl, n_ell, raw.data_ptr(), nb * nv * 3,
                                                out[c0].data_ptr(), stream), "abcmb_apply_operator")
         return out
@@ -629,9 +645,11 @@ class Model:
             abi.check(lib.abcmb_lensing_clpp(cos_dev.data_ptr() + c0 * csz, nb, lens_all[c0].data_ptr(), n_lens,
                                              D["lens_ells"].data_ptr(), nl, clpp[c0].data_ptr(), stream),
                       "abcmb_lensing_clpp")
-            abi.check(lib.abcmb_lens(nb, cl_full[c0].data_ptr(), clpp[c0].data_ptr(), nl, D["mu"].data_ptr(),
-                                     D["w"].data_ptr(), self.mu.size, lensed[c0].data_ptr(), ws.data_ptr(),
-                                     ws.numel() * 8, stream), "abcmb_lens")
+            abi.check(self._timed("abcmb_lens", dict(n_cosmo=nb, n_mu=self.mu.size, n_l=nl),
+                                  lambda: lib.abcmb_lens(nb, cl_full[c0].data_ptr(), clpp[c0].data_ptr(), nl,
+                                                         D["mu"].data_ptr(), D["w"].data_ptr(), self.mu.size,
+                                                         lensed[c0].data_ptr(), ws.data_ptr(), ws.numel() * 8,
+                                                         stream)), "abcmb_lens")
         return lensed
 
     def observed_cl(self, ds):

@@ -37,6 +37,16 @@ sys.path.insert(0, ROOT)
 
 METRIC = "grad-evals/s, 7-param lensed Cl ℓ≤2500, vs plain GPU path at ≤1e-4 Cl err"
 FP64_PEAK_TFLOPS = 78.6
+HBM_PEAK_GBPS = 8000.0
+# algorithmic FP64 flops of the tangent-linear LOS per (cosmology, l, k, tau) node:
+# value 4 sources x 1 multiply-add = 8; per direction the source derivatives and the
+# Bessel slopes times the distance derivative, 4 x 2 multiply-adds + 1 = 17
+LOS_T_FLOPS_VALUE, LOS_T_FLOPS_PER_DIR = 8, 17
+# algorithmic FP64 flops of the lensing per (cosmology, mu, l): Wigner recursions
+# (4 flops a step; 2 in the Cgl pass, 12 in the correlation pass, 4 in the
+# back-transform) 72, Cgl sums 4, X functions 14, correlation sums 40,
+# back-transform sums 8
+LENS_FLOPS_PER_MU_L = 138
 
 
 def ode_flops(model, steps: int, iters: int) -> float:
@@ -81,49 +91,70 @@ def kernel_instance(model, n_dir: int) -> str:
     return f"pert_fwd_kernel<{nt}, {bmax}, {n_dir}>"
 
 
-def build_model():
+def build_model(tol_scale: float = 1.0):
+    """The benchmark's model; ``tol_scale`` multiplies every ODE tolerance (a
+    converged reference uses 0.01)."""
     from abcmb_amd.model import Model, MassiveNeutrino
     # max_steps_PE raised from the reference default 2048: with massive neutrinos the
     # highest modes (k ~ 0.6 Mpc^-1) need up to ~2000 steps in this integrator
-    return Model(user_species=(MassiveNeutrino,), lensing=True, l_max=2500, max_steps_PE=4096)
+    kw = {}
+    if tol_scale != 1.0:
+        from abcmb_amd import specs as S
+        d = S.load_specs({})
+        kw = {key: d[key] * tol_scale for key in ("rtol_small_k_PE", "rtol_large_k_PE", "atol_small_k_PE",
+                                                  "atol_large_k_PE")}
+    return Model(user_species=(MassiveNeutrino,), lensing=True, l_max=2500,
+                 max_steps_PE=4096 if tol_scale >= 1.0 else 32768, **kw)
 
 
-def cpu_baseline(model, thetas, k_grids, seconds_budget=20.0):
-    """Oracle (CPU FP64 restatement) on a bounded sample of one gradient evaluation."""
-    from abcmb_amd.likelihood import theta_to_params
+def cl_errors(gpu, ref, n_l):
+    """Max error of lensed TT/TE/EE/BB ([4][>= n_l] arrays, l = 2..) against ``ref``:
+    relative per l for TT, EE, BB, relative to max |TE| for TE."""
+    out = {}
+    for x, name in enumerate(("TT", "TE", "EE", "BB")[:min(len(gpu), len(ref))]):
+        g, r = np.asarray(gpu[x][:n_l]), np.asarray(ref[x][:n_l])
+        den = np.max(np.abs(r)) if name == "TE" else np.abs(r)
+        out[name] = float(np.max(np.abs(g - r) / den))
+    return out
+
+
+def cpu_baseline(model, plan, gpu_lensed):
+    """The oracle (CPU FP64 restatement of the whole path) on one cosmology of the
+    workload: one complete forward evaluation (every k-mode, sources, LOS,
+    lensing) on all host cores, and on one core a bounded sample (16 of the
+    k-modes, scaled by the k count, plus the whole downstream).  A central
+    difference gradient costs 15 forward evaluations.  Also returns the GPU's
+    lensed C_l error against the complete oracle evaluation."""
     from oracle import oracle as O
     from oracle import pipeline
-    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-    p = model.derived(theta_to_params(thetas[0]))
-    plan = model.prepare(p, k=k_grids[0])
+    threads = O.max_threads()
     nk = plan.k.size
+    t0 = time.perf_counter()
+    o = pipeline.run(model, plan)
+    t_all = time.perf_counter() - t0
+    if np.any(o["status"] != 0):
+        raise RuntimeError("oracle integration failed on the baseline cosmology")
     sub = np.linspace(0, nk - 1, 16).astype(int)
-    t0 = time.perf_counter()
-    pipeline.run(model, plan, k_subset=sub)
-    t_ode_sub = time.perf_counter() - t0
-    t_ode = t_ode_sub * nk / sub.size                   # all k-modes of one cosmology
-    # downstream (sources, LOS, lensing) once, with the ODE table from the GPU run
-    g = model.execute([plan])[0]
-    pt = g.PT.table.cpu().numpy()
-    t0 = time.perf_counter()
-    src = O.sources(pt, plan.k, model.n_save, plan.bgv, plan.lna_save, model.k_transfer, model.scales)
-    cl_raw = O.los_cl(src, plan.bgv[0], plan.bg.tau0, model.k_transfer, model.bes_tab, model.bes_rng, model.bes_l,
-                      model.ell_idx, p["A_s"], p["n_s"], model.specs["k_pivot"])
-    cl_full = cl_raw @ model.ell_op.T
-    c = plan.bg.cosmo
-    import abcmb_amd.abi as abi
-    cc = abi.AbcmbCosmo.from_buffer_copy(c)
-    lna = np.ascontiguousarray(plan.lna_save)
-    cc.lna_save, cc.k, cc.pt, cc.n_save, cc.nk = lna.ctypes.data, plan.k.ctypes.data, pt.ctypes.data, model.n_save, nk
-    clpp = O.clpp(cc, pt, plan.lens_lna, plan.lens_aH, plan.lens_tau, plan.chi_rec, model.lensing_ells.astype(float))
-    O.lens(cl_full, clpp, model.mu, model.w)
-    t_down = time.perf_counter() - t0
-    t_forward = t_ode + t_down
-    t_grad = 15 * t_forward
-    return {"value": 1.0 / t_grad, "unit": "grad-evals/s", "cores": threads, "kind": "port",
-            "sample": (f"oracle ODE on {sub.size} of {nk} k-modes ({t_ode_sub:.1f} s, scaled by k count) + full "
-                       f"sources/LOS/lensing ({t_down:.1f} s) of one cosmology; x15 forward evaluations per "
-                       f"central-difference gradient")}
+    O.set_threads(1)
+    try:
+        t0 = time.perf_counter()
+        pipeline.run(model, plan, k_subset=sub)
+        t_ode1 = (time.perf_counter() - t0) * nk / sub.size
+        t0 = time.perf_counter()
+        pipeline.downstream(model, plan, o["pt"])
+        t_down1 = time.perf_counter() - t0
+    finally:
+        O.set_threads(threads)
+    err = cl_errors(gpu_lensed, o["lensed"], model.ells.size)
+    base = {"value": 1.0 / (15 * t_all), "unit": "grad-evals/s", "cores": threads, "kind": "port",
+            "nproc": os.cpu_count(),
+            "sample": (f"one complete oracle forward evaluation of cosmology 0 ({nk} k-modes, sources, LOS, "
+                       f"lensing to l=2500) on {threads} threads: {t_all:.1f} s; x15 evaluations per "
+                       f"central-difference gradient"),
+            "one_core": {"value": 1.0 / (15 * (t_ode1 + t_down1)), "cores": 1,
+                         "sample": (f"oracle ODE on {sub.size} of {nk} k-modes on 1 thread, scaled by the k count "
+                                    f"({t_ode1:.1f} s), + the whole downstream on 1 thread ({t_down1:.1f} s); x15")}}
+    return base, err
 
 
 def copy_calibration(nbytes: int):
@@ -188,6 +219,7 @@ def main():
         print(f"# warmup {w} done", file=sys.stderr, flush=True)
 
     model.timing = []
+    model.call_timing = {}
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for s in range(args.steps):
@@ -196,7 +228,8 @@ def main():
     torch.cuda.synchronize()
     dt = time.perf_counter() - t0
     timing = model.timing
-    model.timing = None
+    call_timing = model.call_timing
+    model.timing = model.call_timing = None
     value = args.batch * args.steps / dt
 
     # plain path: one cosmology per launch, same FD gradient
@@ -205,12 +238,35 @@ def main():
     torch.cuda.synchronize()
     plain = args.plain / (time.perf_counter() - t0)
     gb = grad[:args.plain]
-    # agreement of the batched gradient with the plain path's, per parameter, relative to the
-    # gradient's norm in units of each parameter's magnitude (components near zero stay meaningful)
-    scale = np.linalg.norm(gp * thetas[:args.plain], axis=1, keepdims=True) / np.abs(thetas[:args.plain])
-    agree = float(np.max(np.abs(gp - gb) / scale))
+    np_ = args.plain
+
+    def err_to_norm(g, ref):
+        """max over parameters of |g - ref| relative to ref's norm, each component in
+        units of its parameter's magnitude (components near zero stay meaningful)"""
+        scale = np.linalg.norm(ref * thetas[:np_], axis=1, keepdims=True) / np.abs(thetas[:np_])
+        return float(np.max(np.abs(g - ref) / scale))
+
+    agree = err_to_norm(gb, gp)
     agree_rel = float(np.max(np.abs(gp - gb) / (np.abs(gp) + 1e-300)))
     print(f"# plain path {plain:.4f} grad-evals/s", file=sys.stderr, flush=True)
+
+    # accuracy of the timed path on the plain cosmologies: C_l and L against the plain
+    # path's forward evaluations, gradients against a converged FP64 reference (the
+    # tangent-linear path at 100x tighter ODE tolerances)
+    cl_b = []
+    LK.tangent_gradients(model, like, thetas[:np_], k_grids=k_grids[:np_], chunk=np_, cl_out=cl_b)
+    cl_b = torch.cat(cl_b).cpu().numpy()
+    plans_p = model.prepare_many([model.derived(LK.theta_to_params(t)) for t in thetas[:np_]], k_list=k_grids[:np_])
+    outs_p = [model.execute([pl])[0] for pl in plans_p]
+    cl_p = np.stack([np.stack([o.ClTT.cpu().numpy(), o.ClTE.cpu().numpy(), o.ClEE.cpu().numpy()]) for o in outs_p])
+    cl_diff_plain = max(max(cl_errors(cl_b[i], cl_p[i], model.ells.size)[x] for x in ("TT", "TE", "EE"))
+                        for i in range(np_))
+    L_diff_plain = float(np.max(np.abs(L[:np_] - Lp) / np.abs(Lp)))
+    mconv = build_model(tol_scale=0.01)
+    _, g_conv = LK.tangent_gradients(mconv, like, thetas[:np_], k_grids=k_grids[:np_], chunk=np_)
+    grad_err_conv = err_to_norm(gb, g_conv)
+    del mconv
+    print(f"# gradient error vs converged {grad_err_conv:.2e}", file=sys.stderr, flush=True)
 
     ms = sum(t["ms"] for t in timing) / len(timing)
     if args.gradient == "tangent":
@@ -226,12 +282,39 @@ def main():
     if args.gradient == "tangent" and os.path.exists(tfile):
         t = json.load(open(tfile))
         traffic = t["fetch_bytes_per_launch"] + t["write_bytes_per_launch"]
-    roof = {"bound": "mfma", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+    # algorithmic HBM bytes of an ODE launch: the perturbation table it produces
+    # (21 fields x saved points x k-modes, FP64, value and 5 tangents)
+    ncomp = 1 + (5 if args.gradient == "tangent" else 0)
+    table_bytes = sum(t["items"] for t in timing) / len(timing) * model.n_save * 21 * 8 * ncomp
+    roof = {"bound": "fp64-valu", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
             "frac": achieved / FP64_PEAK_TFLOPS, "traffic": traffic, "kernel": kname,
             "launch_ms": ms, "flops_per_launch": flops,
             "chord_iters_per_step": sum(t["iters"] for t in timing) / max(1, sum(t["steps"] for t in timing)),
             "traffic_source": "profiles/pert_tangent_traffic.json (FETCH_SIZE x2.0 + WRITE_SIZE x1.0, "
-                              "scaled by a 1 GiB copy)" if traffic else None}
+                              "scaled by a 1 GiB copy)" if traffic else None,
+            "hbm": {"algorithmic_bytes_per_launch": table_bytes,
+                    "algorithmic_GBps": table_bytes / (ms * 1e-3) / 1e9,
+                    "counter_GBps": traffic / (ms * 1e-3) / 1e9 if traffic else None,
+                    "peak_GBps": HBM_PEAK_GBPS,
+                    "frac": table_bytes / (ms * 1e-3) / 1e9 / HBM_PEAK_GBPS}}
+    others = []
+    for name in ("abcmb_los_tangent", "abcmb_lens"):
+        calls = call_timing.get(name, [])
+        if not calls:
+            continue
+        cms = [e0.elapsed_time(e1) for e0, e1, _ in calls]
+        if name == "abcmb_los_tangent":
+            fl = [w["n_cosmo"] * w["n_ell"] * w["nkt"] * w["n_tau"] * (LOS_T_FLOPS_VALUE + LOS_T_FLOPS_PER_DIR
+                                                                       * w["n_dir"]) for _, _, w in calls]
+            kern = "los_tangent_kernel (+ its partial-sum kernel)"
+        else:
+            fl = [w["n_cosmo"] * w["n_mu"] * w["n_l"] * LENS_FLOPS_PER_MU_L for _, _, w in calls]
+            kern = "cgl_kernel + ksi_kernel + back_kernel + back_final_kernel"
+        a = sum(fl) / (sum(cms) * 1e-3) / 1e12
+        others.append({"call": name, "kernel": kern, "bound": "fp64-valu", "calls": len(cms),
+                       "ms_per_call": sum(cms) / len(cms), "achieved": a, "peak": FP64_PEAK_TFLOPS,
+                       "unit": "TFLOP/s", "frac": a / FP64_PEAK_TFLOPS})
+    roof["other_kernels"] = others
 
     # batched forward spectra (64 Latin-hypercube cosmologies with their own k grids, lensed to l = 2500,
     # FP64 ODE + FP32 LOS): host preparation + one ODE launch + downstream
@@ -248,9 +331,18 @@ def main():
     spectra = nf / (time.perf_counter() - t0)
     print(f"# forward {spectra:.2f} spectra/s", file=sys.stderr, flush=True)
 
-    cpu = None
+    cpu, cl_err = None, None
     if not args.no_cpu_baseline:
-        cpu = cpu_baseline(model, thetas, k_grids)
+        plan0 = model.prepare(model.derived(LK.theta_to_params(thetas[0])), k=k_grids[0])
+        g0 = model.execute([plan0])[0]
+        gl = [x.cpu().numpy() for x in (g0.ClTT, g0.ClTE, g0.ClEE, g0.ClBB)]
+        cpu, cl_err = cpu_baseline(model, plan0, gl)
+        print(f"# C_l error vs oracle {cl_err}", file=sys.stderr, flush=True)
+    gates = {"grad_err_vs_converged<=1e-3": grad_err_conv <= 1e-3}
+    if cl_err is not None:
+        gates["cl_err_vs_oracle TT/TE/EE<=1e-4"] = max(cl_err["TT"], cl_err["TE"], cl_err["EE"]) <= 1e-4
+        gates["cl_err_vs_oracle BB<=1e-3"] = cl_err["BB"] <= 1e-3
+    gates["cl_diff_vs_plain<=1e-4"] = cl_diff_plain <= 1e-4
 
     out = {
         "metric": METRIC, "value": value, "unit": "grad-evals/s", "n_gpus": 1, "steps": args.steps,
@@ -267,6 +359,16 @@ def main():
                    "plain_path": "one cosmology per launch, FP64, central finite differences (15 evaluations)",
                    "plain_path_grad_evals_per_s": plain, "speedup_vs_plain": value / plain,
                    "grad_diff_vs_plain_rel_to_norm": agree, "grad_diff_vs_plain_max_rel": agree_rel,
+                   "grad_err_vs_converged": grad_err_conv,
+                   "grad_err_reference": "tangent-linear gradients of the same cosmologies at 100x tighter ODE "
+                                         "tolerances (max error over parameters relative to the gradient norm, "
+                                         "each component in units of its parameter)",
+                   "cl_err_vs_oracle": cl_err,
+                   "cl_err_reference": "lensed C_l (l=2..2500) of cosmology 0 on the GPU path vs the complete "
+                                       "FP64 oracle evaluation of the same inputs; TT/EE/BB relative per l, TE "
+                                       "relative to max|TE|",
+                   "cl_diff_vs_plain": cl_diff_plain, "L_diff_vs_plain": L_diff_plain,
+                   "gates": gates,
                    "forward_spectra_per_s": spectra,
                    "forward_workload": "64 Latin-hypercube cosmologies (seed 2), own 300-800 k grids, lensed "
                                        "TT/TE/EE/BB to l=2500, FP64 ODE + FP32 LOS, host preparation included",
