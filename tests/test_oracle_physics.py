@@ -292,3 +292,130 @@ def test_poisson_and_derived_fields(nu_model):
             got = pt[I[fp]][2:-2, j]
             sl = slice(150, None)     # after the photon-baryon oscillations have died away at these k
             assert np.max(np.abs(d[sl] - got[sl])) < 2e-3 * np.max(np.abs(got[sl])), (f, j)
+
+
+
+@pytest.mark.parametrize("l_hier,k_lo,k_hi,tol", [(40, 3e-4, 1.8e-3, 1e-2), (100, 2e-3, 5e-3, 3e-3)])
+def test_los_temperature_matches_hierarchy(fiducial, l_hier, k_lo, k_hi, tol):
+    """The line-of-sight transfer Delta_l(k) (sources S_T0 + S_T1 + S_T2 with
+    the Bessel tables, on the reference's 499-node ln a grid) equals the
+    photon temperature multipole F_l(k, tau0) / 4 that the Boltzmann hierarchy
+    itself carries to today (Seljak & Zaldarriaga 1996), where the hierarchy
+    is long enough to hold it (k tau0 < l_max).  The residual (0.1-0.7% of
+    the largest multipole, converged in the hierarchy's tolerance) is the LOS
+    quadrature: the reference's trapezoid on 499 nodes with its last row
+    dropped; a wrong source coefficient or sign moves Delta_l by more."""
+    kw = dict(l_max=1000, l_max_g=l_hier, l_max_pol_g=l_hier, l_max_massless_nu=l_hier, max_steps_PE=20000)
+    m = Model(k_axis_perturbations=np.geomspace(1e-5, 2 * k_hi, 140), **kw)
+    p = m.derived(fiducial)
+    plan = m.prepare(p)
+    o = pipeline.run(m, plan, lensing=False)
+    kt = m.k_transfer
+    sel = np.nonzero((kt > k_lo) & (kt < k_hi))[0][::4]
+    ell_idx = np.nonzero(m.bes_l <= 0.6 * l_hier)[0].astype(np.int32)
+    _, D = O.los_cl(o["src"], plan.bgv[0], plan.bg.tau0, kt, m.bes_tab, m.bes_rng, m.bes_l, ell_idx, p["A_s"],
+                    p["n_s"], m.specs["k_pivot"], want_delta=True)
+    m2 = Model(k_axis_perturbations=kt[sel], **kw)
+    plan2 = m2.prepare(p)
+    o2 = pipeline.run(m2, plan2, want_states=True, k_subset=np.arange(sel.size))
+    assert np.all(o2["status"] == 0)
+    lay = m2.layout
+    ls = m.bes_l[ell_idx]
+    assert ls[0] == 2 and ls.size >= 18 and sel.size >= 5
+    for jj, j in enumerate(sel):
+        assert kt[j] * plan.bg.tau0 < 0.75 * lay.l_g
+        F = o2["states"][jj][-1][lay.i_g:lay.i_g + lay.l_g + 1]
+        hier = np.array([F[l] if l >= 3 else 2.0 * F[2] for l in ls]) / 4.0    # row 2 holds sigma = F_2 / 2
+        los = D[0, :, j] + D[1, :, j] + D[2, :, j]
+        assert np.max(np.abs(los - hier)) < tol * np.max(np.abs(hier)), (kt[j], los, hier)
+
+
+@pytest.fixture(scope="module")
+def c1_full(fiducial):
+    m = Model(l_max=1000, k_axis_perturbations=np.geomspace(1e-5, 0.3, 200))
+    p = m.derived(fiducial)
+    plan = m.prepare(p)
+    return m, p, plan, pipeline.run(m, plan, lensing=False)
+
+
+def _los_parts(m, p, plan, pt, scales, ell_idx):
+    src = O.sources(pt, plan.k, m.n_save, plan.bgv, plan.lna_save, m.k_transfer, np.asarray(scales, float))
+    _, D = O.los_cl(src, plan.bgv[0], plan.bg.tau0, m.k_transfer, m.bes_tab, m.bes_rng, m.bes_l, ell_idx,
+                    p["A_s"], p["n_s"], m.specs["k_pivot"], want_delta=True)
+    return D
+
+
+def _on_los_nodes(plan, pt, field, k, n):
+    from scipy.interpolate import CubicSpline
+    lk = np.log10(plan.k)
+    return np.array([CubicSpline(lk, pt[I[field]][s], bc_type="not-a-knot")(math.log10(k)) for s in range(n)])
+
+
+def test_los_sources_are_integrations_by_parts(c1_full):
+    """The reference's sources are integrated by parts (abcmb/spectrum.py:681-696).
+    Undoing that on the table gives the textbook conformal-Newtonian forms
+    (Seljak & Zaldarriaga 1996), integrated here with exact spherical Bessel
+    functions on the same 499 nodes:
+      SW + ISW:  int dtau [ g (delta_g / 4 + psi) + e^{-kappa} d(phi + psi)/dtau ] j_l(k chi),
+                 phi + psi = eta + alpha', psi = alpha' + aH alpha (MB95 eq. 18);
+      Doppler:   int dtau g (theta_b + k^2 alpha) / k  j_l'(k chi)  (the Newtonian
+                 baryon velocity theta_b + k^2 alpha).
+    The Doppler form is compared at small k only: above k ~ 0.02 Mpc^-1 the
+    reference's 499 nodes resolve neither form's oscillation during
+    reionization, and the two quadratures differ."""
+    m, p, plan, o = c1_full
+    pt = o["pt"]
+    kt, lna, bgv = m.k_transfer, plan.lna_save, plan.bgv
+    tau, g, aH, em = bgv[0], bgv[1], bgv[3], bgv[4]
+    dl = lna[1] - lna[0]
+    w = np.full(lna.size - 1, dl)
+    w[0] *= 0.5
+    w /= aH                                         # d tau of the reference's trapezoid
+    chi = plan.bg.tau0 - tau
+    ell_idx = np.array([0, 8, 20, 30, 40, 50, 55], np.int32)
+    ls = m.bes_l[ell_idx]
+    aH_all = plan.bg.eval(lna)["aH"]
+    D = _los_parts(m, p, plan, pt, (1, 1, 0, 0), ell_idx)
+    for j in np.nonzero((kt > 2e-3) & (kt < 0.13))[0][::40]:
+        k = kt[j]
+        F = _on_los_nodes(plan, pt, "metric_eta", k, lna.size) + aH_all * _on_los_nodes(
+            plan, pt, "metric_alpha_prime", k, lna.size)
+        dF = np.gradient(F, dl)[:-1] * aH            # d(phi + psi) / dtau
+        f = g * (_on_los_nodes(plan, pt, "g_delta", k, lna.size - 1) / 4
+                 + aH * _on_los_nodes(plan, pt, "metric_alpha_prime", k, lna.size - 1)) + em * dF
+        ref = np.array([np.sum(w * f * spherical_jn(l, k * chi)) for l in ls])
+        got = D[0, :, j] + D[1, :, j]
+        assert np.max(np.abs(got - ref)) < 5e-3 * np.max(np.abs(ref)), ("sw+isw", k)
+    D = _los_parts(m, p, plan, pt, (0, 0, 1, 0), ell_idx)
+    for j in np.nonzero((kt > 2e-3) & (kt < 0.015))[0][::20]:
+        k = kt[j]
+        f = g * (_on_los_nodes(plan, pt, "b_theta", k, lna.size - 1) / k ** 2
+                 + _on_los_nodes(plan, pt, "metric_alpha", k, lna.size - 1)) * k
+        ref = np.array([np.sum(w * f * spherical_jn(l, k * chi, derivative=True)) for l in ls])
+        got = D[0, :, j]
+        assert np.max(np.abs(got - ref)) < 1e-2 * np.max(np.abs(ref)), ("doppler", k)
+
+
+
+def test_tight_coupling_polarization(fiducial):
+    """Deep in tight coupling (expansion rate / scattering rate aH tau_c < 1e-3)
+    the polarization hierarchy is slaved to the photon shear: G_0 = Pi / 2,
+    G_2 = Pi / 10 with Pi = 2 sigma + G_0 + G_2 gives G_0 = 5 sigma / 2 and
+    G_2 = sigma / 2 (pins the Thomson source weights of the polarization rows
+    and the quadrupole combination Pi that the S_T2 and S_E sources share).
+    The saved points start at ln a = -10.5 here instead of the transfer start."""
+    import dataclasses
+    m = Model(l_max=1000, k_axis_perturbations=np.geomspace(1e-5, 0.3, 200))
+    p = m.derived(fiducial)
+    plan = m.prepare(p)
+    ls = np.linspace(-10.5, 0.0, m.n_save)
+    plan = dataclasses.replace(plan, lna_save=ls, lna_start=np.minimum(plan.lna_start, -11.0))
+    ks = [40, 80, 120, 160]
+    pt = pipeline.run(m, plan, k_subset=ks)["pt"]
+    for j in ks:
+        for i in (0, 20, 60):
+            b = _bg(plan, ls[i])
+            assert b["aH"] * b["tau_c"] < 1e-3
+            sg = pt[I["g_sigma"]][i, j]
+            assert abs(pt[I["g_G0"]][i, j] / (2.5 * sg) - 1) < 1e-2
+            assert abs(pt[I["g_G2"]][i, j] / (0.5 * sg) - 1) < 1e-2
