@@ -419,3 +419,97 @@ def test_tight_coupling_polarization(fiducial):
             sg = pt[I["g_sigma"]][i, j]
             assert abs(pt[I["g_G0"]][i, j] / (2.5 * sg) - 1) < 1e-2
             assert abs(pt[I["g_G2"]][i, j] / (0.5 * sg) - 1) < 1e-2
+
+
+@pytest.fixture(scope="module")
+def c1_lens(fiducial):
+    m = Model(l_max=1000, lensing=True, k_axis_perturbations=np.geomspace(1e-5, 1.0, 300))
+    p = m.derived(fiducial)
+    plan = m.prepare(p)
+    return m, p, plan, pipeline.run(m, plan, lensing=True)
+
+
+def _weyl(plan, pt, lna):
+    """The Weyl potential (phi + psi) / 2 = (eta + alpha-dot) / 2 (MB95 eq. 18)
+    on the saved points, from the table's metric fields only."""
+    aH = plan.bg.eval(lna)["aH"]
+    return 0.5 * (pt[I["metric_eta"]] + aH[:, None] * pt[I["metric_alpha_prime"]])
+
+
+def test_limber_lensing_potential_from_weyl_potential(c1_lens):
+    """C_l^phiphi of the oracle (Limber on delta_m through the matter-only
+    Poisson equation, abcmb/spectrum.py:349-435) against the Limber integral of
+    the Weyl potential taken from the table's metric fields,
+    C_l = int dchi 4 (chi* - chi)^2 / (chi*^2 chi^4) P_Weyl(k = (l + 1/2) / chi),
+    P_Weyl = 2 pi^2 / k^3 A_s (k / k_p)^(n_s - 1) Psi_W^2: the two routes agree
+    to the radiation's share of the Poisson source near recombination (< 0.5%
+    here); a wrong power of aH, Omega_m or k, or of the window, fails."""
+    from scipy.interpolate import CubicSpline
+    m, p, plan, o = c1_lens
+    pt, lna, kk = o["pt"], plan.lna_save, plan.k
+    tau = plan.bg.eval(lna)["tau"]
+    chi = plan.bg.tau0 - tau
+    chis = plan.chi_rec
+    PW = _weyl(plan, pt, lna)
+    use = np.nonzero(lna >= plan.bg.lna_rec)[0]
+    for ell in (2, 10, 30, 100, 300, 600):
+        L = ell + 0.5
+        vals = np.full(use.size, np.nan)
+        for n, i in enumerate(use):
+            if chi[i] <= 0:        # today: the window's 1/chi^2 meets k -> infinity, no power
+                vals[n] = 0.0
+                continue
+            k = L / chi[i]
+            if kk[0] <= k <= kk[-1]:
+                T = CubicSpline(np.log(kk), PW[i])(math.log(k))
+                W = (chis - chi[i]) / (chis * chi[i])
+                vals[n] = (4 * W ** 2 / chi[i] ** 2 * 2 * math.pi ** 2 / k ** 3 * p["A_s"]
+                           * (k / m.specs["k_pivot"]) ** (p["n_s"] - 1) * T * T)
+        ok = np.isfinite(vals)
+        assert ok.sum() > 0.95 * use.size
+        weyl = -np.trapezoid(vals[ok], chi[use][ok])
+        assert abs(weyl / o["clpp"][ell - 2] - 1) < 5e-3, (ell, weyl, o["clpp"][ell - 2])
+
+
+def test_matter_power_from_weyl_potential(c1_lens):
+    """P(k, z = 0) of the oracle (abcmb/spectrum.py:268-302) against the Weyl
+    potential today through Poisson, delta = -2 k^2 Psi_W / (3 Omega_m H0^2):
+    P = 2 pi^2 / k^3 A_s (k / k_p)^(n_s - 1) delta^2 (pins the normalisation,
+    the k interpolation and delta_m against the metric)."""
+    m, p, plan, o = c1_lens
+    pt, lna, kk = o["pt"], plan.lna_save, plan.k
+    H0 = plan.bg.eval(np.array([0.0]))["aH"][0]
+    Om = p["omega_m"] / p["h"] ** 2
+    psi = _weyl(plan, pt, lna)[-1]
+    kq = kk[(kk > 1e-3) & (kk < 0.3)][::7]
+    pk = O.pk(pt, kk, m.n_save, lna, kq, p["A_s"], p["n_s"], m.specs["k_pivot"])
+    dl = -2 * kq ** 2 * np.interp(kq, kk, psi) / (3 * Om * H0 ** 2)
+    ref = 2 * math.pi ** 2 / kq ** 3 * p["A_s"] * (kq / m.specs["k_pivot"]) ** (p["n_s"] - 1) * dl ** 2
+    assert np.max(np.abs(pk / ref - 1)) < 2e-3
+
+
+@pytest.mark.parametrize("k", [1e-5, 1e-3, 0.05, 0.6, 3.0])
+def test_starting_time_conditions(nu_model, k):
+    """get_starting_time (abcmb/perturbations.py:119-159, :279): the start is
+    the earlier of the times where the Thomson time over the Hubble time
+    tau_c aH reaches R_tc and where k / aH reaches R_large, searched on
+    [-20, -10] and never later than -10.  Found here by root finding on the
+    background functions."""
+    from scipy.optimize import brentq
+    m, p, plan = nu_model
+    ev = lambda l: _bg(plan, l)
+    R_tc, R_large = m.specs["R_tc"], m.specs["R_large"]
+
+    def root(f, thr):
+        lo, hi = -20.0, -10.0
+        if f(lo) >= thr:
+            return lo
+        if f(hi) <= thr:
+            return hi
+        return brentq(lambda l: f(l) - thr, lo, hi, xtol=1e-12)
+
+    l1 = root(lambda l: ev(l)["tau_c"] * ev(l)["aH"], R_tc)
+    l2 = root(lambda l: k / ev(l)["aH"], R_large)
+    want = min(l1, l2, -10.0)
+    got = plan.bg.lna_start(np.array([k]), R_tc, R_large)[0]
+    assert abs(got - want) < 1e-6, (k, got, want, l1, l2)
