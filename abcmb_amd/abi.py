@@ -89,6 +89,7 @@ def host_lib():
         lib.abcmb_host_los_background.argtypes = [C.POINTER(AbcmbCosmo), _P, C.c_int, _P]
         lib.abcmb_host_lna_start.argtypes = [C.POINTER(AbcmbCosmo), _P, C.c_int, _D, _D, _P]
         lib.abcmb_host_eval.argtypes = [C.POINTER(AbcmbCosmo), _P, C.c_int, _P]
+        lib.abcmb_host_drag.argtypes = [C.POINTER(AbcmbCosmo), C.POINTER(_D), C.POINTER(_D)]
         _host = lib
     return _host
 
@@ -98,7 +99,9 @@ HIP_SYMBOLS = (
     "abcmb_sources_workspace_size", "abcmb_sources", "abcmb_los_workspace_size", "abcmb_los_cl", "abcmb_apply_operator",
     "abcmb_los_tangent_workspace_size", "abcmb_los_tangent",
     "abcmb_lensing_clpp", "abcmb_lens_workspace_size", "abcmb_lens", "abcmb_pk_lin", "abcmb_pert_fwd_noalloc",
+    "abcmb_pk",
 )
+PK_M, PK_CB = 0, 1
 
 
 def hip_lib():
@@ -136,6 +139,7 @@ def hip_lib():
         lib.abcmb_lens_workspace_size.argtypes = [C.c_int, C.c_int, C.c_int]
         lib.abcmb_lens.argtypes = [C.c_int, _P, _P, C.c_int, _P, _P, C.c_int, _P, _P, C.c_size_t, _P]
         lib.abcmb_pk_lin.argtypes = [_P, C.c_int, _P, C.c_int, _P, _P]
+        lib.abcmb_pk.argtypes = [_P, C.c_int, _P, C.c_int, _D, C.c_int, _P, _P]
         _hip = lib
     return _hip
 

@@ -100,3 +100,24 @@ class Background:
 
     def tau(self, lna):
         return self.eval(lna)["tau"]
+
+    def R_ratio_lna(self, lna):
+        """R = 3 rho_b / (4 rho_gamma) (abcmb/background.py:389-418)."""
+        return 1.0 / self.eval(lna)["R"]
+
+    def _drag(self):
+        if getattr(self, "_drag_cache", None) is None:
+            zd, rs = C.c_double(), C.c_double()
+            if abi.host_lib().abcmb_host_drag(C.byref(self.cosmo), C.byref(zd), C.byref(rs)) != 0:
+                raise RuntimeError("abcmb_host_drag failed")
+            self._drag_cache = (zd.value, rs.value)
+        return self._drag_cache
+
+    def z_d(self, params=None):
+        """Baryon drag redshift, where the baryon optical depth reaches 1
+        (abcmb/background.py:901-918)."""
+        return self._drag()[0]
+
+    def rs_d(self, params=None):
+        """Sound horizon at the drag redshift in Mpc (abcmb/background.py:920-936)."""
+        return self._drag()[1]

@@ -37,10 +37,20 @@ def build_host():
 
 
 def build_hip():
+    """Each .hip source to an object in parallel (the ODE kernels dominate the
+    compile time), then one link."""
+    from concurrent.futures import ThreadPoolExecutor
     out = os.path.join(HERE, "libabcmb_hip.so")
     srcs = sorted(glob.glob(os.path.join(CSRC, "*.hip")))
+    objdir = os.path.join(ROOT, "_build", "hip")
+    os.makedirs(objdir, exist_ok=True)
+    cflags = [f for f in HIP_FLAGS if f != "-shared"]
+    objs = [os.path.join(objdir, os.path.basename(s)[:-4] + ".o") for s in srcs]
+    with ThreadPoolExecutor(max(1, min(len(srcs), os.cpu_count() or 1))) as ex:
+        for f in [ex.submit(_run, [HIPCC] + cflags + ["-c", s, "-o", o]) for s, o in zip(srcs, objs)]:
+            f.result()
     tmp = out + ".tmp"
-    _run([HIPCC] + HIP_FLAGS + srcs + ["-o", tmp])
+    _run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC"] + objs + ["-o", tmp])
     os.replace(tmp, out)
     return out
 

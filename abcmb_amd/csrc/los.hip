@@ -504,8 +504,21 @@ __device__ double interp_clamped(double x, const double* __restrict__ xp, const 
   return fp[(size_t)a * stride] + t * (fp[(size_t)(a + 1) * stride] - fp[(size_t)a * stride]);
 }
 
-__global__ void pk_kernel(const AbcmbCosmo* __restrict__ cosmos, const double* __restrict__ kq, int nq,
-                          double* __restrict__ out_all) {
+// Linear power spectrum at ln a = lna (abcmb/spectrum.py:268-347): the density
+// contrast is interpolated linearly in ln a over the saved points, then
+// linearly in k (both clamped at the ends, as jnp.interp does), squared and
+// multiplied by the primordial spectrum.  kind 0: delta_m (Pk_lin); kind 1:
+// (omega_b delta_b + omega_cdm delta_cdm) / omega_m (Pk_cb, with the
+// reference's omega_m, massive neutrinos included, in the denominator).
+__device__ double column_at(const double* __restrict__ f, const double* __restrict__ lna_save, int n_save, int nk,
+                            int a, int b, double t, double lna) {
+  const double va = interp_clamped(lna, lna_save, f + a, n_save, nk);
+  const double vb = interp_clamped(lna, lna_save, f + b, n_save, nk);
+  return va + t * (vb - va);
+}
+
+__global__ void pk_kernel(const AbcmbCosmo* __restrict__ cosmos, const double* __restrict__ kq, int nq, double lna,
+                          int kind, double* __restrict__ out_all) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   int ci = blockIdx.y;
   if (i >= nq) return;
@@ -516,9 +529,9 @@ __global__ void pk_kernel(const AbcmbCosmo* __restrict__ cosmos, const double* _
   int nk = cc.nk, n_save = cc.n_save;
   double A_s = cc.A_s, n_s = cc.n_s, kp = cc.k_pivot;
   double* out = out_all + (size_t)ci * nq;
-  const double* dm = pt + (size_t)ABCMB_PT_DELTA_M * n_save * nk;
+  const size_t fs = (size_t)n_save * nk;
   double x = kq[i];
-  // delta_m at ln a = 0 for the two bracketing k columns, then linear in k
+  // the two bracketing k columns and the weight between them
   int a, b;
   double t;
   if (x <= k[0]) { a = b = 0; t = 0.0; }
@@ -528,9 +541,14 @@ __global__ void pk_kernel(const AbcmbCosmo* __restrict__ cosmos, const double* _
     while (b - a > 1) { int mid = (a + b) >> 1; if (k[mid] <= x) a = mid; else b = mid; }
     t = (x - k[a]) / (k[b] - k[a]);
   }
-  double va = interp_clamped(0.0, lna_save, dm + a, n_save, nk);
-  double vb = interp_clamped(0.0, lna_save, dm + b, n_save, nk);
-  double d = va + t * (vb - va);
+  double d;
+  if (kind == 0) {
+    d = column_at(pt + ABCMB_PT_DELTA_M * fs, lna_save, n_save, nk, a, b, t, lna);
+  } else {
+    const double dc = column_at(pt + ABCMB_PT_CDM_DELTA * fs, lna_save, n_save, nk, a, b, t, lna);
+    const double db = column_at(pt + ABCMB_PT_B_DELTA * fs, lna_save, n_save, nk, a, b, t, lna);
+    d = (cc.omega_b * db + cc.omega_cdm * dc) / cc.omega_m;
+  }
   out[i] = d * d * (A_s * pow(x / kp, n_s - 1.) * (2 * PI * PI / (x * x * x)));
 }
 
@@ -649,11 +667,18 @@ extern "C" int abcmb_apply_operator(const double* op_dev, int rows, int cols, co
   return ABCMB_OK;
 }
 
-extern "C" int abcmb_pk_lin(const AbcmbCosmo* cosmos_dev, int n_cosmo, const double* kq_dev, int nq, double* pk_dev,
-                            void* stream) {
+extern "C" int abcmb_pk(const AbcmbCosmo* cosmos_dev, int n_cosmo, const double* kq_dev, int nq, double z, int kind,
+                        double* pk_dev, void* stream) {
   ABCMB_REQUIRE(cosmos_dev && kq_dev && pk_dev && n_cosmo > 0 && nq > 0, "bad P(k) arguments");
+  ABCMB_REQUIRE(kind == ABCMB_PK_M || kind == ABCMB_PK_CB, "bad P(k) kind %d", kind);
+  ABCMB_REQUIRE(z > -1.0, "redshift %g must exceed -1", z);
   hipLaunchKernelGGL(pk_kernel, dim3((nq + 255) / 256, n_cosmo), dim3(256), 0, (hipStream_t)stream, cosmos_dev,
-                     kq_dev, nq, pk_dev);
+                     kq_dev, nq, -std::log1p(z), kind, pk_dev);
   ABCMB_HIP_CHECK(hipGetLastError());
   return ABCMB_OK;
+}
+
+extern "C" int abcmb_pk_lin(const AbcmbCosmo* cosmos_dev, int n_cosmo, const double* kq_dev, int nq, double* pk_dev,
+                            void* stream) {
+  return abcmb_pk(cosmos_dev, n_cosmo, kq_dev, nq, 0.0, ABCMB_PK_M, pk_dev, stream);
 }

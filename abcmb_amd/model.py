@@ -652,6 +652,45 @@ class Model:
                                                          stream)), "abcmb_lens")
         return lensed
 
+    def _pk(self, k, z, PT, params, kind):
+        torch = _torch()
+        lib = abi.hip_lib()
+        dev = torch.device("cuda")
+        _, _, stage = self._staging()
+        kq = stage.add(np.atleast_1d(np.asarray(k, dtype=np.float64)))
+        lna = stage.add(np.asarray(PT.lna, dtype=np.float64))
+        kd = getattr(PT, "k_dev", None)
+        if kd is None:
+            kd = stage.add(np.asarray(PT.k, dtype=np.float64))
+        table = PT.table if hasattr(PT.table, "data_ptr") else stage.add(np.asarray(PT.table))
+        c = abi.AbcmbCosmo()
+        c.A_s, c.n_s, c.k_pivot = params["A_s"], params["n_s"], self.specs["k_pivot"]
+        c.omega_b, c.omega_cdm, c.omega_m = params["omega_b"], params["omega_cdm"], params["omega_m"]
+        c.n_save, c.nk = int(lna.numel()), int(kd.numel())
+        c.lna_save, c.k, c.pt = lna.data_ptr(), kd.data_ptr(), table.data_ptr()
+        cos = self._struct_array(torch, dev, [c], stage.add)
+        out = torch.empty(kq.numel(), dtype=torch.float64, device=dev)
+        stage.flush()
+        abi.check(lib.abcmb_pk(cos.data_ptr(), 1, kq.data_ptr(), int(kq.numel()), float(z), kind, out.data_ptr(),
+                               torch.cuda.current_stream().cuda_stream), "abcmb_pk")
+        return out
+
+    def Pk_lin(self, k, z, PT, params):
+        """Linear matter power spectrum P(k, z) in Mpc^3 on the GPU (SpectrumSolver.Pk_lin,
+        abcmb/spectrum.py:268-302)."""
+        return self._pk(k, z, PT, params, abi.PK_M)
+
+    def Pk_cb(self, k, z, PT, params):
+        """Linear CDM + baryon power spectrum P_cb(k, z) in Mpc^3 on the GPU
+        (SpectrumSolver.Pk_cb, abcmb/spectrum.py:304-347)."""
+        return self._pk(k, z, PT, params, abi.PK_CB)
+
+    @property
+    def SS(self):
+        """The reference's spectrum-solver handle (``model.SS.Pk_lin(...)``): the
+        P(k) entry points live on the model here."""
+        return self
+
     def observed_cl(self, ds):
         """[B][3][n_l] TT, TE, EE on l = 2..l_max (lensed when lensing is on) from a downstream result."""
         n_l = self.ells.size

@@ -150,6 +150,11 @@ int abcmb_host_lna_start(const AbcmbCosmo* c, const double* k, int nk,
                          double R_tc, double R_large, double* out);
 /* Background at arbitrary ln a: out[0..6][n] = aH, tau, tau_c, cs2, x_e, T_m, R. */
 int abcmb_host_eval(const AbcmbCosmo* c, const double* lna, int n, double* out);
+/* Baryon drag redshift z_d (baryon optical depth kappa_d = 1) and the sound
+ * horizon r_s(z_d) in Mpc (Background.z_d / Background.rs_d,
+ * abcmb/background.py:774-936), for a cosmology filled by
+ * abcmb_host_background. */
+int abcmb_host_drag(const AbcmbCosmo* c, double* z_d, double* rs_d);
 
 /* ---------------- GPU library (libabcmb_hip.so) ---------------------------- */
 
@@ -310,6 +315,16 @@ int abcmb_lens(int n_cosmo, const double* cl_in_dev, const double* clpp_dev, int
  * pk_dev out [B][nq] on the query grid kq_dev [nq]. */
 int abcmb_pk_lin(const AbcmbCosmo* cosmos_dev, int n_cosmo, const double* kq_dev, int nq,
                  double* pk_dev, void* stream);
+
+/* Linear power spectrum at redshift z of the total matter (kind
+ * ABCMB_PK_M: SpectrumSolver.Pk_lin, abcmb/spectrum.py:268-302) or of CDM +
+ * baryons (ABCMB_PK_CB: SpectrumSolver.Pk_cb, abcmb/spectrum.py:304-347,
+ * normalised by omega_m as the reference does): delta interpolated linearly in
+ * ln a over the saved points, then linearly in k, clamped at both ends. */
+#define ABCMB_PK_M 0
+#define ABCMB_PK_CB 1
+int abcmb_pk(const AbcmbCosmo* cosmos_dev, int n_cosmo, const double* kq_dev, int nq,
+             double z, int kind, double* pk_dev, void* stream);
 
 #ifdef __cplusplus
 }

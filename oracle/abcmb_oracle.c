@@ -1124,6 +1124,29 @@ static double pk_at(const double* dm /* [n_save][nk] */, const double* lna_save,
   return d * d * primordial(kq, A_s, n_s, kp);
 }
 
+/* P(k, z) of the total matter (kind 0, SpectrumSolver.Pk_lin) or of CDM + baryons
+ * (kind 1, SpectrumSolver.Pk_cb, abcmb/spectrum.py:304-347: (omega_b delta_b +
+ * omega_cdm delta_cdm) / omega_m) at ln a = lna */
+int oracle_pk_z(const double* pt, const double* k, int nk, int n_save, const double* lna_save, const double* kq,
+                int nq, double A_s, double n_s, double kp, double lna, int kind, double omega_b, double omega_cdm,
+                double omega_m, double* out) {
+  const size_t fs = (size_t)n_save * nk;
+  double* col = (double*)malloc(sizeof(double) * nk);
+  double* dcb = NULL;
+  const double* f = pt + (size_t)ABCMB_PT_DELTA_M * fs;
+  if (kind == 1) {
+    dcb = (double*)malloc(sizeof(double) * fs);
+    for (size_t i = 0; i < fs; ++i)
+      dcb[i] = (omega_b * pt[(size_t)ABCMB_PT_B_DELTA * fs + i] + omega_cdm * pt[(size_t)ABCMB_PT_CDM_DELTA * fs + i]) /
+               omega_m;
+    f = dcb;
+  }
+  for (int i = 0; i < nq; ++i) out[i] = pk_at(f, lna_save, n_save, k, nk, lna, kq[i], A_s, n_s, kp, col);
+  free(col);
+  free(dcb);
+  return 0;
+}
+
 int oracle_pk(const double* pt, const double* k, int nk, int n_save, const double* lna_save, const double* kq,
               int nq, double A_s, double n_s, double kp, double* out) {
   double* col = (double*)malloc(sizeof(double) * nk);

@@ -52,6 +52,7 @@ def lib():
         L.oracle_wigner.argtypes = [_I, _I, _P, _I, _I, _P]
         L.oracle_lens.argtypes = [_P, _P, _I, _P, _P, _I, _P]
         L.oracle_set_threads.argtypes = [_I]
+        L.oracle_pk_z.argtypes = [_P, _P, _I, _I, _P, _P, _I, _D, _D, _D, _D, _I, _D, _D, _D, _P]
         _lib = L
     return _lib
 
@@ -180,6 +181,14 @@ def los_cl_branch(src, tau_los, tau0, kt, bes, rng, bes_l, ell_idx, A_s, n_s, k_
 def pk(pt, k, n_save, lna_save, kq, A_s, n_s, k_pivot):
     pt = _c(pt); k = _c(k); lna_save = _c(lna_save); kq = _c(kq); out = np.zeros(kq.size)
     lib().oracle_pk(_p(pt), _p(k), k.size, n_save, _p(lna_save), _p(kq), kq.size, A_s, n_s, k_pivot, _p(out))
+    return out
+
+
+def pk_z(pt, k, n_save, lna_save, kq, A_s, n_s, k_pivot, z, kind, omega_b, omega_cdm, omega_m):
+    """P(k, z): kind 0 total matter, kind 1 CDM + baryons."""
+    pt = _c(pt); k = _c(k); lna_save = _c(lna_save); kq = _c(kq); out = np.zeros(kq.size)
+    lib().oracle_pk_z(_p(pt), _p(k), k.size, n_save, _p(lna_save), _p(kq), kq.size, A_s, n_s, k_pivot,
+                      -np.log1p(z), kind, omega_b, omega_cdm, omega_m, _p(out))
     return out
 
 

@@ -14,7 +14,8 @@
     sources/LOS/lensing differenced along each direction, or the unlensed C_l
     moved along branch-frozen oracle derivatives and the lensing
     differenced): 1e-4 of the gradient norm (3e-4 for omega_cdm with the
-    tangent-linear LOS);
+    tangent-linear LOS), lensed to l = 1000 and, downstream differenced, to
+    l = 2500;
   - at 100x tighter ODE tolerances the tangent gradient (downstream
     differenced, as the finite differences do) and central finite differences
     of full evaluations agree to 5e-4 of the gradient norm (at the default
@@ -83,8 +84,8 @@ def test_tangent_tables_match_oracle(massive):
     assert err.max() < 1e-4
 
 
-@pytest.mark.parametrize("downstream", ["fd", "tangent"])
-def test_tangent_gradient_matches_oracle(downstream):
+@pytest.mark.parametrize("downstream,l_max", [("fd", 1000), ("tangent", 1000), ("fd", 2500)])
+def test_tangent_gradient_matches_oracle(downstream, l_max):
     """Both directional downstreams against the oracle built the same way: "fd"
     differences the oracle's sources / LOS / lensing along each direction;
     "tangent" moves the unlensed C_l along their pointwise derivatives and
@@ -103,13 +104,13 @@ def test_tangent_gradient_matches_oracle(downstream):
     from abcmb_amd.likelihood import with_primordial
     from oracle import pipeline
     kk = np.geomspace(1e-4, 0.6, 24)
-    m = Model(user_species=(MassiveNeutrino,), lensing=True, l_max=1000, k_axis_perturbations=kk)
+    m = Model(user_species=(MassiveNeutrino,), lensing=True, l_max=l_max, k_axis_perturbations=kk)
     thetas = LK.latin_hypercube(2, seed=3)
     base0 = m.prepare(m.derived(LK.theta_to_params(thetas[0])))
     g0 = m.execute([base0])[0]
     like = LK.LensedClLikelihood(m, [g0.ClTT.cpu().numpy(), g0.ClTE.cpu().numpy(), g0.ClEE.cpu().numpy()],
                                  n_mocks=2, seed=5)
-    _say(f"case: tangent gradients ({downstream} downstream), 2 cosmologies, lensed l<=1000, 24 k")
+    _say(f"case: tangent gradients ({downstream} downstream), 2 cosmologies, lensed l<={l_max}, 24 k")
     Lg, gg = LK.tangent_gradients(m, like, thetas, downstream=downstream)
     # L(theta) -- in "tangent" mode lensed from the tangent-linear LOS's value --
     # against a plain evaluation: the tables of the tangent kernel differ from

@@ -513,3 +513,44 @@ def test_starting_time_conditions(nu_model, k):
     want = min(l1, l2, -10.0)
     got = plan.bg.lna_start(np.array([k]), R_tc, R_large)[0]
     assert abs(got - want) < 1e-6, (k, got, want, l1, l2)
+
+
+def test_drag_epoch_and_sound_horizon(nu_model):
+    """z_d and r_s(z_d) (abcmb/background.py:774-936) against direct scipy
+    quadratures of the defining integrals on the same background, and against
+    the Planck 2018 LCDM values they must land near (z_drag 1059.9, r_drag
+    147.1 Mpc) with this recombination history."""
+    from scipy.optimize import brentq
+    m, p, plan = nu_model
+    bg = plan.bg
+    ev = lambda l: bg.eval(np.array([l]))
+    R = lambda l: 1.0 / ev(l)["R"][0]            # 3 rho_b / (4 rho_g)
+    dkap = lambda l: 1.0 / (ev(l)["tau_c"][0] * ev(l)["aH"][0] * R(l))
+    kap = lambda l: quad(dkap, l, 0.0, epsabs=0, epsrel=1e-10, limit=400, points=[-7.5, -7.0, -6.5])[0]
+    lna_d = brentq(lambda l: kap(l) - 1.0, -8.0, -6.0, xtol=1e-12)
+    zd = math.exp(-lna_d) - 1
+    rs = (1.0 / math.sqrt(3.0) / ev(-33.0)["aH"][0]
+          + quad(lambda l: 1.0 / math.sqrt(3 * (1 + R(l))) / ev(l)["aH"][0], -33.0, lna_d, epsabs=0,
+                 epsrel=1e-10, limit=400)[0])
+    assert abs(bg.z_d() / zd - 1) < 1e-7
+    assert abs(bg.rs_d() / rs - 1) < 1e-7
+    assert 1050 < bg.z_d() < 1070 and 145 < bg.rs_d() < 150
+
+
+def test_cdm_baryon_power(fiducial, c1_lens):
+    """P_cb (abcmb/spectrum.py:304-347) equals P_m when every matter species is
+    CDM or baryons; P(k, z) follows the growth factor at late times."""
+    m, p, plan, o = c1_lens
+    pt, lna, kk = o["pt"], plan.lna_save, plan.k
+    kq = np.geomspace(2e-4, 0.5, 40)
+    args = (pt, kk, m.n_save, lna, kq, p["A_s"], p["n_s"], m.specs["k_pivot"])
+    for z in (0.0, 0.7, 3.0):
+        pm = O.pk_z(*args, z, 0, p["omega_b"], p["omega_cdm"], p["omega_m"])
+        pcb = O.pk_z(*args, z, 1, p["omega_b"], p["omega_cdm"], p["omega_m"])
+        assert np.max(np.abs(pcb / pm - 1)) < 1e-12
+    assert np.array_equal(O.pk_z(*args, 0.0, 0, p["omega_b"], p["omega_cdm"], p["omega_m"]), O.pk(*args))
+    sel = (kq > 0.02) & (kq < 0.1)
+    for z in (0.5, 1.0):
+        ratio = O.pk_z(*args, z, 0, p["omega_b"], p["omega_cdm"], p["omega_m"]) / O.pk(*args)
+        g = (_growth(p, 1 / (1 + z)) / _growth(p, 1.0)) ** 2
+        assert np.max(np.abs(ratio[sel] / g - 1)) < 5e-3, z
