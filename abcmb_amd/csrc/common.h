@@ -9,6 +9,8 @@
 namespace abcmb {
 
 void set_error(const char* fmt, ...);
+// launch-shape settings chosen by the autotuner (abcmb_tune_set); 0 when unset
+int tune_get(int key);
 
 #define ABCMB_HIP_CHECK(expr)                                                   \
   do {                                                                          \

@@ -138,6 +138,10 @@ template <int N> __device__ __forceinline__ D<N> dpow(const D<N>& a, double p) {
   return chain(a, s, p * pow(a.v, p - 1.0));
 }
 template <int N> __device__ __forceinline__ D<N> dcos(const D<N>& a) { return chain(a, cos(a.v), -sin(a.v)); }
+template <int N> __device__ __forceinline__ D<N> dtanh(const D<N>& a) {
+  const double t = tanh(a.v);
+  return chain(a, t, 1.0 - t * t);
+}
 template <int N> __device__ __forceinline__ D<N> dasin(const D<N>& a) {
   return chain(a, asin(a.v), 1.0 / sqrt(1.0 - a.v * a.v));
 }

@@ -409,9 +409,9 @@ double expmkappa(const Kappa& K, double l) { return l < -10.0 ? 0.0 : std::exp(-
 
 extern "C" int abcmb_host_recomb_points(void) { return rec_points(); }
 
-extern "C" int abcmb_host_background(AbcmbCosmo* c, double tau_reion, double z_reion_in,
-                                     double Delta_z_reion, double z_reion_He,
-                                     double Delta_z_reion_He, double exp_reion, int32_t* rec_switch) {
+namespace {
+int background(AbcmbCosmo* c, double tau_reion, double z_reion_in, double Delta_z_reion, double z_reion_He,
+               double Delta_z_reion_He, double exp_reion, int32_t* rec_switch, bool skip_reion) {
   if (!c || !c->tau_tab || !c->xe_tab || !c->tm_tab) return ABCMB_EINVAL;
   int N = rec_points();
   if (c->xe_n < N || c->tm_n != c->xe_n) return ABCMB_EINVAL;
@@ -429,7 +429,7 @@ extern "C" int abcmb_host_background(AbcmbCosmo* c, double tau_reion, double z_r
   c->xe_lna_last = c->tm_lna_last = x0 + (N - 1) * REC_DLNA;
   // reionization: solve tau_reion(z) = target (Newton/bisection), or take z
   double zr = z_reion_in;
-  if (tau_reion > 0.0) {
+  if (tau_reion > 0.0 && !skip_reion) {
     ReionGrid G;
     G.build(*c, z_reion_He, Delta_z_reion_He, exp_reion);
     auto f = [&](double z) { return G.tau(z, Delta_z_reion, exp_reion) - tau_reion; };
@@ -449,9 +449,11 @@ extern "C" int abcmb_host_background(AbcmbCosmo* c, double tau_reion, double z_r
     zr = b;
   }
   c->z_reion = zr;
-  for (int j = 0; j < N; ++j) {
-    double l = x0 + j * REC_DLNA;
-    xe[j] += xe_reion(l, zr, Delta_z_reion, z_reion_He, Delta_z_reion_He, exp_reion, c->YHe);
+  if (!skip_reion) {
+    for (int j = 0; j < N; ++j) {
+      double l = x0 + j * REC_DLNA;
+      xe[j] += xe_reion(l, zr, Delta_z_reion, z_reion_He, Delta_z_reion_He, exp_reion, c->YHe);
+    }
   }
   for (int j = N; j < c->xe_n; ++j) { xe[j] = xe[N - 1]; tm[j] = tm[N - 1]; }
   c->xe_last = xe[N - 1];
@@ -484,6 +486,23 @@ extern "C" int abcmb_host_background(AbcmbCosmo* c, double tau_reion, double z_r
     c->lna_transfer_start = -15.0 + 9.0 * best / 4999.0;
   }
   return ABCMB_OK;
+}
+
+}  // namespace
+
+extern "C" int abcmb_host_background(AbcmbCosmo* c, double tau_reion, double z_reion_in,
+                                     double Delta_z_reion, double z_reion_He,
+                                     double Delta_z_reion_He, double exp_reion, int32_t* rec_switch) {
+  return background(c, tau_reion, z_reion_in, Delta_z_reion, z_reion_He, Delta_z_reion_He, exp_reion, rec_switch,
+                    false);
+}
+
+// The same tables without the reionization: x_e is the recombination history
+// alone and z_reion is left 0; the scans for lna_rec and lna_transfer_start
+// do not see reionization (no free electrons from it at z > 50, and its
+// constant factor e^-tau_reio does not move the visibility's peak).
+extern "C" int abcmb_host_recombination(AbcmbCosmo* c, int32_t* rec_switch) {
+  return background(c, 0.0, 0.0, 0.5, 3.5, 0.5, 1.5, rec_switch, true);
 }
 
 extern "C" int abcmb_host_los_background(const AbcmbCosmo* c, const double* lna, int n, double* out) {

@@ -423,6 +423,316 @@ __global__ void back_final_kernel(const double* __restrict__ part_all, int nblk,
   out[idx] = s * f;
 }
 
+
+// ---- lensing contractions on the matrix cores ----
+// The two passes of the correlation-function lensing that are plain
+// contractions over a cosmology-independent operand become batched GEMMs
+// across the cosmologies of a call, with the reduced Wigner d-matrices
+// generated once and kept in HBM:
+//   Cgl_c(mu)  = sum_l d11(mu, l)  F_c(l),  Cgl2_c(mu) = sum_l d1-1(mu, l) F_c(l),
+//                F_c(l) = (2l+1) l(l+1) C^phiphi_c(l) / 4pi;
+//   G_qc(l)    = sum_mu d_q(mu, l) w(mu) xi_qc(mu),  q = 00, 22, 2-2, 20,
+// then TT = 2pi G_00, TE = 2pi G_20, EE/BB = pi (G_22 +- G_2-2).  The
+// correlation pass between them (exp-weighted X functions of sigma^2(mu) per
+// cosmology) is not a product of shared operands and stays on the vector
+// units (ksi_kernel).
+//
+// d-matrix layout: [6][n_mu][n_l] of T (l = 2 .. n_l + 1 along rows), pairs in
+// DM_PAIR order; the FP64 recursion tables (coef, lfac) follow the matrices.
+constexpr int N_DM = 6;
+constexpr int GEMM_MAX_SPLIT = 16;   // most slices of K in a lensing GEMM
+__constant__ int DM_PAIR[N_DM] = {0, 1, 2, 3, 4, 9};   // d00 d11 d1-1 d2-2 d22 d20
+enum { DM_00 = 0, DM_11, DM_1M1, DM_2M2, DM_22, DM_20 };
+
+typedef double f64x4 __attribute__((ext_vector_type(4)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+
+template <typename T> __device__ __forceinline__ T to_t(double v);
+template <> __device__ __forceinline__ double to_t<double>(double v) { return v; }
+template <> __device__ __forceinline__ float to_t<float>(double v) { return (float)v; }
+template <> __device__ __forceinline__ __bf16 to_t<__bf16>(double v) { return (__bf16)(float)v; }
+
+// one thread per node mu: the six d's along l, normalised as the vector path
+// normalises them (the recursion value times sqrt(2 / (2l + 1)))
+template <typename T>
+__global__ void dmat_kernel(const double* __restrict__ mu_arr, int n_mu, int n_l, const double* __restrict__ coef,
+                            const double* __restrict__ lfac, int LT, T* __restrict__ dm) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int p = blockIdx.y;
+  if (i >= n_mu) return;
+  const int q = DM_PAIR[p];
+  const int m = PAIR_M[q], n = PAIR_N[q];
+  const double mu = mu_arr[i];
+  const double* cq = coef + (size_t)q * LT * 3;
+  Rec r{wigner_start(m, n, mu), 0.0};
+  for (int l = m; l < 2; ++l) rec_step(r, cq, l, mu);
+  T* row = dm + ((size_t)p * n_mu + i) * n_l;
+  for (int j = 0; j < n_l; ++j) {
+    const int l = j + 2;
+    row[j] = to_t<T>(r.dl * lfac[l * LFAC]);
+    rec_step(r, cq, l, mu);
+  }
+}
+
+// C(m, n) = sum_k A(m, k) B(k, n), A(m, k) = A[m sam + k sak], B(k, n) =
+// B[k sbk + n sbn], C(m, n) -> C[m scm + n scn] (FP64).  One wavefront per 32 x
+// 32 tile of C as 2 x 2 MFMA blocks of 16 x 16; operands straight from global
+// memory (the d-matrix streams once per 32 or 64 columns of C, the batch
+// operand stays in L2).  MFMA operand layout (16 x 16 x K blocks): lane l
+// holds A(l % 16, k-slice l / 16), B(k-slice l / 16, l % 16); the four
+// accumulator values of lane l hold column l % 16 and rows 4 (l / 16) + v
+// (FP32 accumulation) or l / 16 + 4 v (FP64), as measured on gfx950.
+template <typename T>
+struct Mfma;
+template <>
+struct Mfma<double> {
+  static constexpr int KS = 4, KV = 1;
+  static constexpr bool ROW_STRIDED = true;   // accumulator v of lane l: row l / 16 + 4 v
+  typedef double vec;
+  typedef f64x4 acc;
+  __device__ static acc run(vec a, vec b, acc c) { return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0); }
+};
+template <>
+struct Mfma<float> {
+  static constexpr int KS = 4, KV = 1;
+  static constexpr bool ROW_STRIDED = false;  // accumulator v of lane l: row 4 (l / 16) + v
+  typedef float vec;
+  typedef f32x4 acc;
+  __device__ static acc run(vec a, vec b, acc c) { return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0); }
+};
+template <>
+struct Mfma<__bf16> {
+  static constexpr int KS = 32, KV = 8;
+  static constexpr bool ROW_STRIDED = false;
+  typedef bf16x8 vec;
+  typedef f32x4 acc;
+  __device__ static acc run(vec a, vec b, acc c) { return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0); }
+};
+
+template <typename T>
+__device__ __forceinline__ typename Mfma<T>::vec load_op(const T* __restrict__ p, size_t s_row, size_t s_k, int row,
+                                                         int rows, int k0, int K) {
+  typename Mfma<T>::vec v;
+  if constexpr (Mfma<T>::KV == 1) {
+    v = (row < rows && k0 < K) ? p[(size_t)row * s_row + (size_t)k0 * s_k] : T(0);
+  } else {
+#pragma unroll
+    for (int e = 0; e < Mfma<T>::KV; ++e) {
+      const int k = k0 + e;
+      v[e] = (row < rows && k < K) ? p[(size_t)row * s_row + (size_t)k * s_k] : (__bf16)0.0f;
+    }
+  }
+  return v;
+}
+
+// K is cut into gridDim.z slices of kslice (a multiple of the MFMA depth); slice
+// z writes its partial product to Cm + z * szc, and a reduction pass adds the
+// slices in order (deterministic).
+template <typename T, int NC>
+__global__ void __launch_bounds__(64) gemm_mfma_kernel(int M, int N, int K, int kslice, const T* __restrict__ A,
+                                                       size_t sam, size_t sak, const T* __restrict__ B, size_t sbk,
+                                                       size_t sbn, double* __restrict__ Cm, size_t scm, size_t scn,
+                                                       size_t szc) {
+  using F = Mfma<T>;
+  const int lane = threadIdx.x;
+  const int li = lane & 15, lk = (lane >> 4) * F::KV;
+  const int m0 = blockIdx.x * 32, n0 = blockIdx.y * 16 * NC;
+  const int kb = blockIdx.z * kslice;
+  const int ke = min(K, kb + kslice);
+  Cm += (size_t)blockIdx.z * szc;
+  typename F::acc acc[2][NC];
+#pragma unroll
+  for (int r = 0; r < 2; ++r)
+#pragma unroll
+    for (int c = 0; c < NC; ++c) acc[r][c] = typename F::acc{0, 0, 0, 0};
+  for (int k0 = kb; k0 < ke; k0 += F::KS) {
+    const int k = k0 + lk;
+    typename F::vec a[2], b[NC];
+#pragma unroll
+    for (int r = 0; r < 2; ++r) a[r] = load_op(A, sam, sak, m0 + 16 * r + li, M, k, ke);
+#pragma unroll
+    for (int c = 0; c < NC; ++c) b[c] = load_op(B, sbn, sbk, n0 + 16 * c + li, N, k, ke);
+#pragma unroll
+    for (int r = 0; r < 2; ++r)
+#pragma unroll
+      for (int c = 0; c < NC; ++c) acc[r][c] = F::run(a[r], b[c], acc[r][c]);
+  }
+#pragma unroll
+  for (int r = 0; r < 2; ++r)
+#pragma unroll
+    for (int c = 0; c < NC; ++c)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int m = m0 + 16 * r + (F::ROW_STRIDED ? (lane >> 4) + 4 * v : 4 * (lane >> 4) + v);
+        const int n = n0 + 16 * c + li;
+        if (m < M && n < N) Cm[(size_t)m * scm + (size_t)n * scn] = (double)acc[r][c][v];
+      }
+}
+
+// F[j][c] = (2l+1) l(l+1) C^phiphi_c(l) / 4pi, l = j + 2
+template <typename T>
+__global__ void cgl_prep_kernel(const double* __restrict__ clpp, int n_l, int n_cosmo, T* __restrict__ F) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  const int c = blockIdx.y;
+  if (j >= n_l) return;
+  const double l = j + 2;
+  F[(size_t)j * n_cosmo + c] = to_t<T>((2. * l + 1) * l * (l + 1) * clpp[(size_t)c * n_l + j] / 4. / PI);
+}
+
+// X[q][mu][c] = w(mu) xi_qc(mu) with the correlation functions in GEMM order
+// (d00 <- xi, d22 <- xi+, d2-2 <- xi-, d20 <- xi_x)
+template <typename T>
+__global__ void back_prep_kernel(const double* __restrict__ ksi_all, const double* __restrict__ w, int n_mu,
+                                 int n_cosmo, T* __restrict__ X) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int c = blockIdx.y;
+  if (i >= n_mu) return;
+  const double* ksi = ksi_all + (size_t)c * 4 * n_mu;
+  const double wi = w[i];
+#pragma unroll
+  for (int q = 0; q < 4; ++q) X[((size_t)q * n_mu + i) * n_cosmo + c] = to_t<T>(wi * ksi[q * n_mu + i]);
+}
+
+// out[c][4][n_l]: TT = 2pi G0, TE = 2pi G3, EE = pi (G1 + G2), BB = pi (G1 - G2);
+// G[q][j][c] in the back_prep order (q: xi, xi+, xi-, xi_x)
+__global__ void back_combine_kernel(const double* __restrict__ G, int n_l, int n_cosmo, double* __restrict__ out) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  const int c = blockIdx.y;
+  if (j >= n_l) return;
+  const size_t L = (size_t)n_l * n_cosmo, o = (size_t)j * n_cosmo + c;
+  const double g0 = G[o], gp = G[L + o], gm = G[2 * L + o], gx = G[3 * L + o];
+  double* oc = out + (size_t)c * 4 * n_l;
+  oc[j] = 2 * PI * g0;
+  oc[n_l + j] = 2 * PI * gx;
+  oc[2 * n_l + j] = PI * (gp + gm);
+  oc[3 * n_l + j] = PI * (gp - gm);
+}
+
+size_t elem_bytes(int precision) { return precision == 0 ? 8 : precision == 1 ? 4 : 2; }
+
+// d-matrix buffer: matrices, then coef and lfac (FP64), 256-byte aligned
+size_t dmat_parts(int n_mu, int n_l, int precision, size_t* off) {
+  const int LT = n_l + 3;
+  size_t o = 0;
+  off[0] = 0;
+  o += (size_t)N_DM * n_mu * n_l * elem_bytes(precision);
+  o = (o + 255) / 256 * 256;
+  off[1] = o;
+  o += sizeof(double) * NPAIR * LT * 3;
+  off[2] = o;
+  o += sizeof(double) * LT * LFAC;
+  return o;
+}
+
+// workspace of abcmb_lens_mfma: cgl, ksi, F, X, G
+size_t lens_mfma_parts(int n_cosmo, int n_mu, int n_l, int precision, size_t* off) {  // off[6]
+  const size_t eb = elem_bytes(precision);
+  auto al = [](size_t v) { return (v + 255) / 256 * 256; };
+  size_t o = 0;
+  off[0] = o; o = al(o + sizeof(double) * n_cosmo * 2 * n_mu);         // cgl
+  off[1] = o; o = al(o + sizeof(double) * n_cosmo * 4 * n_mu);         // ksi
+  off[2] = o; o = al(o + eb * n_l * n_cosmo);                          // F
+  off[3] = o; o = al(o + eb * 4 * n_mu * n_cosmo);                     // X
+  off[4] = o; o = al(o + sizeof(double) * 4 * n_l * n_cosmo);          // G
+  const int mn = n_mu > n_l ? n_mu : n_l;
+  off[5] = o; o = al(o + sizeof(double) * GEMM_MAX_SPLIT * mn * n_cosmo);  // split-K partials
+  return o;
+}
+
+// C(m, n) = sum over the S slices of P[z][m][n] (row-major partials)
+__global__ void gemm_reduce_kernel(const double* __restrict__ P, int S, int M, int N, double* __restrict__ Cm,
+                                   size_t scm, size_t scn) {
+  const size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= (size_t)M * N) return;
+  const int m = (int)(idx / N), n = (int)(idx % N);
+  double v = 0.0;
+  for (int z = 0; z < S; ++z) v += P[(size_t)z * M * N + idx];
+  Cm[(size_t)m * scm + (size_t)n * scn] = v;
+}
+
+// slices of K for a skinny product (few cosmologies): enough workgroups to
+// cover the CUs, each slice a multiple of the MFMA depth
+inline int gemm_split(int tiles, int K, int ks) {
+  int S = (512 + tiles - 1) / tiles;
+  if (S > GEMM_MAX_SPLIT) S = GEMM_MAX_SPLIT;
+  const int kmax = (K + ks - 1) / ks;   // no empty slice
+  if (S > kmax) S = kmax;
+  return S < 1 ? 1 : S;
+}
+
+// the column tile (32 or 64 cosmologies) is a tuned launch shape; part holds
+// GEMM_MAX_SPLIT * M * N doubles for the split-K partials
+template <typename T>
+int launch_gemm(int M, int N, int K, const T* A, size_t sam, size_t sak, const T* B, size_t sbk, size_t sbn, double* C,
+                size_t scm, size_t scn, double* part, hipStream_t s) {
+  const bool wide = tune_get(ABCMB_TUNE_LENS_GEMM_COLS) == 64;
+  const int tn = wide ? 64 : 32;
+  const int tiles = ((M + 31) / 32) * ((N + tn - 1) / tn);
+  const int S = gemm_split(tiles, K, Mfma<T>::KS);
+  const int kslice = ((K + S - 1) / S + Mfma<T>::KS - 1) / Mfma<T>::KS * Mfma<T>::KS;
+  const int Sk = (K + kslice - 1) / kslice;
+  double* out = Sk > 1 ? part : C;
+  const size_t om = Sk > 1 ? (size_t)N : scm, on = Sk > 1 ? 1 : scn, oz = (size_t)M * N;
+  const dim3 grid((M + 31) / 32, (N + tn - 1) / tn, Sk);
+  if (wide)
+    hipLaunchKernelGGL((gemm_mfma_kernel<T, 4>), grid, dim3(64), 0, s, M, N, K, kslice, A, sam, sak, B, sbk, sbn, out,
+                       om, on, oz);
+  else
+    hipLaunchKernelGGL((gemm_mfma_kernel<T, 2>), grid, dim3(64), 0, s, M, N, K, kslice, A, sam, sak, B, sbk, sbn, out,
+                       om, on, oz);
+  ABCMB_HIP_CHECK(hipGetLastError());
+  if (Sk > 1) {
+    hipLaunchKernelGGL(gemm_reduce_kernel, dim3((unsigned)(((size_t)M * N + 255) / 256)), dim3(256), 0, s, part, Sk, M,
+                       N, C, scm, scn);
+    ABCMB_HIP_CHECK(hipGetLastError());
+  }
+  return ABCMB_OK;
+}
+
+template <typename T>
+int lens_mfma(int n_cosmo, const double* cl_in, const double* clpp, int n_l, const double* mu, const double* w,
+              int n_mu, const void* dmat, int precision, double* out, void* workspace, hipStream_t s) {
+  size_t doff[3], off[6];
+  dmat_parts(n_mu, n_l, precision, doff);
+  lens_mfma_parts(n_cosmo, n_mu, n_l, precision, off);
+  const T* dm = (const T*)dmat;
+  const double* coef = (const double*)((const char*)dmat + doff[1]);
+  const double* lfac = (const double*)((const char*)dmat + doff[2]);
+  char* ws = (char*)workspace;
+  double* cgl = (double*)(ws + off[0]);
+  double* ksi = (double*)(ws + off[1]);
+  T* F = (T*)(ws + off[2]);
+  T* X = (T*)(ws + off[3]);
+  double* G = (double*)(ws + off[4]);
+  double* part = (double*)(ws + off[5]);
+  const int LT = n_l + 3;
+  const size_t plane = (size_t)n_mu * n_l;
+  hipLaunchKernelGGL((cgl_prep_kernel<T>), dim3((n_l + 255) / 256, n_cosmo), dim3(256), 0, s, clpp, n_l, n_cosmo, F);
+  ABCMB_HIP_CHECK(hipGetLastError());
+  // Cgl, Cgl2: (mu, c) = sum_l d(mu, l) F(l, c), written into cgl[c][0 or 1][mu]
+  for (int h = 0; h < 2; ++h) {
+    int rc = launch_gemm<T>(n_mu, n_cosmo, n_l, dm + (h == 0 ? DM_11 : DM_1M1) * plane, n_l, 1, F, n_cosmo, 1,
+                            cgl + (size_t)h * n_mu, 1, (size_t)2 * n_mu, part, s);
+    if (rc != ABCMB_OK) return rc;
+  }
+  const int nblk = (n_mu + 63) / 64;
+  hipLaunchKernelGGL(ksi_kernel, dim3(nblk, n_cosmo), dim3(64), 0, s, cl_in, cgl, n_l, mu, n_mu, coef, lfac, LT, ksi);
+  hipLaunchKernelGGL((back_prep_kernel<T>), dim3((n_mu + 255) / 256, n_cosmo), dim3(256), 0, s, ksi, w, n_mu,
+                     n_cosmo, X);
+  ABCMB_HIP_CHECK(hipGetLastError());
+  // G[q](l, c) = sum_mu d_q(mu, l) X[q](mu, c)
+  const int dq[4] = {DM_00, DM_22, DM_2M2, DM_20};
+  for (int q = 0; q < 4; ++q) {
+    int rc = launch_gemm<T>(n_l, n_cosmo, n_mu, dm + dq[q] * plane, 1, n_l, X + (size_t)q * n_mu * n_cosmo, n_cosmo,
+                            1, G + (size_t)q * n_l * n_cosmo, n_cosmo, 1, part, s);
+    if (rc != ABCMB_OK) return rc;
+  }
+  hipLaunchKernelGGL(back_combine_kernel, dim3((n_l + 255) / 256, n_cosmo), dim3(256), 0, s, G, n_l, n_cosmo, out);
+  ABCMB_HIP_CHECK(hipGetLastError());
+  return ABCMB_OK;
+}
+
 }  // namespace
 }  // namespace abcmb
 
@@ -483,4 +793,55 @@ extern "C" int abcmb_lens(int n_cosmo, const double* cl_in_dev, const double* cl
                      out_dev);
   ABCMB_HIP_CHECK(hipGetLastError());
   return ABCMB_OK;
+}
+
+extern "C" size_t abcmb_lens_dmat_size(int n_mu, int n_l, int precision) {
+  size_t off[3];
+  if (precision < 0 || precision > 2 || n_mu <= 0 || n_l <= 0) return 0;
+  return dmat_parts(n_mu, n_l, precision, off);
+}
+
+extern "C" int abcmb_lens_dmat(const double* mu_dev, int n_mu, int n_l, int precision, void* dmat_dev,
+                               size_t dmat_bytes, void* stream) {
+  ABCMB_REQUIRE(mu_dev && dmat_dev, "null argument");
+  ABCMB_REQUIRE(n_mu >= 2 && n_l >= 3, "bad sizes");
+  ABCMB_REQUIRE(precision >= 0 && precision <= 2, "precision must be 0 (FP64), 1 (FP32) or 2 (BF16)");
+  ABCMB_REQUIRE(dmat_bytes >= abcmb_lens_dmat_size(n_mu, n_l, precision), "d-matrix buffer too small");
+  hipStream_t s = (hipStream_t)stream;
+  size_t off[3];
+  dmat_parts(n_mu, n_l, precision, off);
+  double* coef = (double*)((char*)dmat_dev + off[1]);
+  double* lfac = (double*)((char*)dmat_dev + off[2]);
+  const int LT = n_l + 3;
+  hipLaunchKernelGGL(wigner_coef_kernel, dim3((NPAIR * LT + 255) / 256), dim3(256), 0, s, LT, coef, lfac);
+  const dim3 grid((n_mu + 63) / 64, N_DM);
+  if (precision == 0)
+    hipLaunchKernelGGL(dmat_kernel<double>, grid, dim3(64), 0, s, mu_dev, n_mu, n_l, coef, lfac, LT, (double*)dmat_dev);
+  else if (precision == 1)
+    hipLaunchKernelGGL(dmat_kernel<float>, grid, dim3(64), 0, s, mu_dev, n_mu, n_l, coef, lfac, LT, (float*)dmat_dev);
+  else
+    hipLaunchKernelGGL(dmat_kernel<__bf16>, grid, dim3(64), 0, s, mu_dev, n_mu, n_l, coef, lfac, LT, (__bf16*)dmat_dev);
+  ABCMB_HIP_CHECK(hipGetLastError());
+  return ABCMB_OK;
+}
+
+extern "C" size_t abcmb_lens_mfma_workspace_size(int n_cosmo, int n_mu, int n_l, int precision) {
+  size_t off[6];
+  if (precision < 0 || precision > 2 || n_cosmo <= 0 || n_mu <= 0 || n_l <= 0) return 0;
+  return lens_mfma_parts(n_cosmo, n_mu, n_l, precision, off);
+}
+
+extern "C" int abcmb_lens_mfma(int n_cosmo, const double* cl_in_dev, const double* clpp_dev, int n_l,
+                               const double* mu_dev, const double* w_dev, int n_mu, const void* dmat_dev,
+                               int precision, double* out_dev, void* workspace, size_t ws_bytes, void* stream) {
+  ABCMB_REQUIRE(cl_in_dev && clpp_dev && mu_dev && w_dev && dmat_dev && out_dev && workspace, "null argument");
+  ABCMB_REQUIRE(n_cosmo > 0 && n_l >= 3 && n_mu >= 2, "bad sizes");
+  ABCMB_REQUIRE(precision >= 0 && precision <= 2, "precision must be 0 (FP64), 1 (FP32) or 2 (BF16)");
+  ABCMB_REQUIRE(ws_bytes >= abcmb_lens_mfma_workspace_size(n_cosmo, n_mu, n_l, precision), "workspace too small");
+  hipStream_t s = (hipStream_t)stream;
+  if (precision == 0)
+    return lens_mfma<double>(n_cosmo, cl_in_dev, clpp_dev, n_l, mu_dev, w_dev, n_mu, dmat_dev, 0, out_dev, workspace, s);
+  if (precision == 1)
+    return lens_mfma<float>(n_cosmo, cl_in_dev, clpp_dev, n_l, mu_dev, w_dev, n_mu, dmat_dev, 1, out_dev, workspace, s);
+  return lens_mfma<__bf16>(n_cosmo, cl_in_dev, clpp_dev, n_l, mu_dev, w_dev, n_mu, dmat_dev, 2, out_dev, workspace, s);
 }

@@ -1245,17 +1245,15 @@ __global__ void bg_save_kernel(const AbcmbCosmo* __restrict__ cosmos, const Abcm
   bgsave[(size_t)ci * n_save_max + i] = b;
 }
 
-// the value-only kernel with 32-row blocks fits three waves per SIMD; the
-// tangent kernel and 64-row blocks need the whole register file of one wave
+// One work item (one k-mode of one cosmology) on the workgroup.
 template <int NT, int BMAX, int N>
-__global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu((N == 0 && BMAX <= 32) ? 3 : 1)))
-pert_fwd_kernel(const AbcmbCosmo* __restrict__ cosmos, const AbcmbTangent* __restrict__ tangents,
-                const AbcmbPertItem* __restrict__ items, AbcmbOdeSettings st, const Bg<N>* __restrict__ bgsave,
-                int n_save_max, int32_t* __restrict__ status_out, int32_t* __restrict__ steps_out,
-                int32_t* __restrict__ iters_out, double* __restrict__ state_out) {
-  __shared__ Smem<NT, BMAX, N> sm;
+__device__ __forceinline__ void ode_item(const int item, Smem<NT, BMAX, N>& sm, const AbcmbCosmo* __restrict__ cosmos,
+                                         const AbcmbTangent* __restrict__ tangents,
+                                         const AbcmbPertItem* __restrict__ items, const AbcmbOdeSettings& st,
+                                         const Bg<N>* __restrict__ bgsave, int n_save_max,
+                                         int32_t* __restrict__ status_out, int32_t* __restrict__ steps_out,
+                                         int32_t* __restrict__ iters_out, double* __restrict__ state_out) {
   const int r = threadIdx.x;
-  const int item = blockIdx.x;
   const AbcmbPertItem I = items[item];
   const AbcmbCosmo& c = cosmos[I.cosmo];
   const AbcmbTangent* T = nullptr;
@@ -1476,12 +1474,61 @@ pert_fwd_kernel(const AbcmbCosmo* __restrict__ cosmos, const AbcmbTangent* __res
   }
 }
 
+// The value-only kernel with 32-row blocks fits three waves per SIMD; the
+// tangent kernel and 64-row blocks need the whole register file of one wave.
+// PERSIST = false: one workgroup per item (the hardware dispatches them in
+// item order, longest first).  PERSIST = true: a grid of resident workgroups
+// takes items from a device work queue (an atomic counter over the item list,
+// zeroed before the launch) until it is empty; every workgroup leaves when the
+// counter passes n_items, so the grid drains without any wait.
+template <int NT, int BMAX, int N, bool PERSIST>
+__global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu((N == 0 && BMAX <= 32) ? 3 : 1)))
+pert_fwd_kernel(const AbcmbCosmo* __restrict__ cosmos, const AbcmbTangent* __restrict__ tangents,
+                const AbcmbPertItem* __restrict__ items, AbcmbOdeSettings st, const Bg<N>* __restrict__ bgsave,
+                int n_save_max, int32_t* __restrict__ status_out, int32_t* __restrict__ steps_out,
+                int32_t* __restrict__ iters_out, double* __restrict__ state_out, int n_items,
+                int32_t* __restrict__ queue) {
+  __shared__ Smem<NT, BMAX, N> sm;
+  if constexpr (!PERSIST) {
+    ode_item<NT, BMAX, N>(blockIdx.x, sm, cosmos, tangents, items, st, bgsave, n_save_max, status_out, steps_out,
+                          iters_out, state_out);
+  } else {
+    __shared__ int next;
+    for (;;) {
+      __syncthreads();  // every thread has read the previous item and left its LDS work
+      if (threadIdx.x == 0) next = atomicAdd(queue, 1);
+      __syncthreads();
+      const int item = next;
+      if (item >= n_items) break;
+      ode_item<NT, BMAX, N>(item, sm, cosmos, tangents, items, st, bgsave, n_save_max, status_out, steps_out,
+                            iters_out, state_out);
+    }
+  }
+}
+
 template <int NT, int BMAX, int N>
 int launch(const AbcmbCosmo* cosmos, const AbcmbTangent* tangents, int n_cosmo, const AbcmbPertItem* items,
            int n_items, const AbcmbOdeSettings& st, const Bg<N>* bgsave, int n_save_max, int32_t* cnt,
-           int32_t* status, int32_t* steps, int32_t* iters, double* state, hipStream_t stream) {
-  hipLaunchKernelGGL((pert_fwd_kernel<NT, BMAX, N>), dim3(n_items), dim3(NT), 0, stream, cosmos, tangents, items,
-                     st, bgsave, n_save_max, status, steps, iters, state);
+           int32_t* queue, int32_t* status, int32_t* steps, int32_t* iters, double* state, hipStream_t stream) {
+  const int sched = tune_get(ABCMB_TUNE_PERT_PERSISTENT);
+  if (sched > 0) {
+    // resident workgroups: the occupancy of this instance times the CUs, times
+    // the tuned fraction (percent) of that
+    int dev = 0, per_cu = 0;
+    ABCMB_HIP_CHECK(hipGetDevice(&dev));
+    hipDeviceProp_t prop;
+    ABCMB_HIP_CHECK(hipGetDeviceProperties(&prop, dev));
+    ABCMB_HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, pert_fwd_kernel<NT, BMAX, N, true>, NT, 0));
+    long long g = (long long)(per_cu > 0 ? per_cu : 1) * prop.multiProcessorCount * sched / 100;
+    if (g < 1) g = 1;
+    if (g > n_items) g = n_items;
+    hipLaunchKernelGGL(zero_counts_kernel, dim3(1), dim3(64), 0, stream, queue, 1);
+    hipLaunchKernelGGL((pert_fwd_kernel<NT, BMAX, N, true>), dim3((unsigned)g), dim3(NT), 0, stream, cosmos,
+                       tangents, items, st, bgsave, n_save_max, status, steps, iters, state, n_items, queue);
+  } else {
+    hipLaunchKernelGGL((pert_fwd_kernel<NT, BMAX, N, false>), dim3(n_items), dim3(NT), 0, stream, cosmos, tangents,
+                       items, st, bgsave, n_save_max, status, steps, iters, state, n_items, queue);
+  }
   ABCMB_HIP_CHECK(hipGetLastError());
   if (12 + 3 * st.n_q <= ABCMB_N_PT_FIELDS) {  // the kernel stored rows: turn them into fields
     hipLaunchKernelGGL(zero_counts_kernel, dim3((n_cosmo + 255) / 256), dim3(256), 0, stream, cnt, n_cosmo);
@@ -1512,25 +1559,27 @@ int check_settings(const AbcmbOdeSettings& st, int& maxb) {
   return ABCMB_OK;
 }
 
-// the workspace: the saved-point background table, then one item count per
-// cosmology at the next 256-byte boundary
+// the workspace: the saved-point background table, then at the next 256-byte
+// boundary the work-queue counter (256 bytes), then one item count per cosmology
 size_t bg_table_bytes(size_t per, int n_cosmo, int n_save_max) {
   const size_t b = per * (size_t)(n_cosmo > 0 ? n_cosmo : 0) * (size_t)(n_save_max > 0 ? n_save_max : 0);
   return (b + 255) / 256 * 256;
 }
+constexpr size_t QUEUE_BYTES = 256;
 
 template <int N>
 int dispatch(const AbcmbCosmo* cosmos, const AbcmbTangent* tangents, int n_cosmo, const AbcmbPertItem* items,
              int n_items, const AbcmbOdeSettings& st, int maxb, int n_save_max, void* workspace, int32_t* status,
              int32_t* steps, int32_t* iters, double* state, hipStream_t s) {
   Bg<N>* bgs = (Bg<N>*)workspace;
-  int32_t* cnt = (int32_t*)((char*)workspace + bg_table_bytes(sizeof(Bg<N>), n_cosmo, n_save_max));
+  int32_t* queue = (int32_t*)((char*)workspace + bg_table_bytes(sizeof(Bg<N>), n_cosmo, n_save_max));
+  int32_t* cnt = (int32_t*)((char*)queue + QUEUE_BYTES);
   hipLaunchKernelGGL((bg_save_kernel<N>), dim3((n_save_max + 63) / 64, n_cosmo), dim3(64), 0, s, cosmos, tangents,
                      n_save_max, bgs);
   ABCMB_HIP_CHECK(hipGetLastError());
 #define ABCMB_LAUNCH(NT_, BM_) \
-  return launch<NT_, BM_, N>(cosmos, tangents, n_cosmo, items, n_items, st, bgs, n_save_max, cnt, status, steps, \
-                             iters, state, s)
+  return launch<NT_, BM_, N>(cosmos, tangents, n_cosmo, items, n_items, st, bgs, n_save_max, cnt, queue, status, \
+                             steps, iters, state, s)
   // 26 rows: the photon block of the default cut-offs (l_max_g 12, l_max_pol_g 10)
   if (maxb <= 26) {
     if (st.n <= 64) ABCMB_LAUNCH(64, 26);
@@ -1554,7 +1603,7 @@ int dispatch(const AbcmbCosmo* cosmos, const AbcmbTangent* tangents, int n_cosmo
 extern "C" size_t abcmb_pert_workspace_size(int n_cosmo, int n_save_max, int n_dir) {
   using namespace abcmb;
   const size_t per = n_dir > 0 ? sizeof(Bg<ABCMB_TANGENT_DIRS>) : sizeof(Bg<0>);
-  return bg_table_bytes(per, n_cosmo, n_save_max) + sizeof(int32_t) * (size_t)(n_cosmo > 0 ? n_cosmo : 0);
+  return bg_table_bytes(per, n_cosmo, n_save_max) + QUEUE_BYTES + sizeof(int32_t) * (size_t)(n_cosmo > 0 ? n_cosmo : 0);
 }
 
 extern "C" int abcmb_pert_fwd(const AbcmbCosmo* cosmos_dev, int n_cosmo, const AbcmbPertItem* items_dev,
@@ -1587,6 +1636,182 @@ extern "C" int abcmb_pert_tangent(const AbcmbCosmo* cosmos_dev, const AbcmbTange
   return dispatch<ABCMB_TANGENT_DIRS>(cosmos_dev, tangents_dev, n_cosmo, items_dev, n_items, *settings, maxb,
                                       n_save_max, workspace, status_dev, steps_dev, iters_dev, nullptr,
                                       (hipStream_t)stream);
+}
+
+// ---------------------------------------------------------------- reionization on the device
+// tanh reionization (abcmb/background.py:939-1017) of a cosmology whose x_e
+// table holds the recombination history alone: the reionization redshift is
+// solved on the device for the target optical depth (safeguarded Newton on the
+// 2000-point trapezoid over ln a in [-5, 0] that the host producer uses), the
+// hydrogen and helium reionization fractions are added to the table, and with
+// tangents the derivative of z_reion follows from the implicit-function
+// theorem, dz/dtheta = (dtau_target/dtheta - dtau/dtheta|_z) / (dtau/dz).
+namespace abcmb {
+namespace {
+
+constexpr int REION_GRID = 2000;
+constexpr int REION_THREADS = 256;
+
+template <int N>
+__device__ __forceinline__ D<N> xe_reion(double lna, const D<N>& zre, const AbcmbReion& R, const D<N>& YHe) {
+  const D<N> fHe = YHe * dinv(4.0 * (1.0 - YHe));
+  const double z = exp(-lna) - 1.0;
+  const double y = pow(1.0 + z, R.exp_reion);
+  const D<N> yre = dpow(1.0 + zre, R.exp_reion);
+  const D<N> dy = R.exp_reion * dpow(1.0 + zre, R.exp_reion - 1.0) * R.Delta_z_reion;
+  const D<N> xH = 0.5 * (1.0 + fHe) * (1.0 + dtanh((yre - y) * dinv(dy)));
+  const D<N> xHe = 0.5 * fHe * (1.0 + tanh((R.z_reion_He - z) / R.Delta_z_reion_He));
+  return xH + xHe;
+}
+
+// optical depth of the reionization fractions and its z derivative (value only,
+// or with the parameter tangents at fixed z when N > 0): a workgroup sum
+template <int N>
+__device__ void reion_tau(const Cosmo<N>& P, const AbcmbReion& R, double zre, D<N>& tau, double& dtau_dz,
+                          double* red) {
+  constexpr double K_NH = 3.0 * H0_OVER_H * H0_OVER_H / 8 / PI / GN / MH;
+  constexpr double K_G = SIGMA_T * C_CM / C_MPC;
+  const D<N> YHe = P(F_YHE);
+  const D<N> fHe = YHe * dinv(4.0 * (1.0 - YHe));
+  const double yre = pow(1.0 + zre, R.exp_reion);
+  const double dy = R.exp_reion * pow(1.0 + zre, R.exp_reion - 1.0) * R.Delta_z_reion;
+  // d(yre)/dz and d(dy)/dz for the z derivative of the tanh argument
+  const double dyre = R.exp_reion * pow(1.0 + zre, R.exp_reion - 1.0);
+  const double ddy = R.exp_reion * (R.exp_reion - 1.0) * pow(1.0 + zre, R.exp_reion - 2.0) * R.Delta_z_reion;
+  D<N> s(0.0);
+  double sz = 0.0;
+  for (int i = threadIdx.x; i < REION_GRID; i += blockDim.x) {
+    const double l = -5.0 + 5.0 * i / (REION_GRID - 1);
+    const double w = (i == 0 || i == REION_GRID - 1 ? 0.5 : 1.0) * (5.0 / (REION_GRID - 1));
+    Bg<N> b;
+    bg_rho(P, l, b);
+    const double a = b.a, z = 1.0 / a - 1.0;
+    const double y = pow(1.0 + z, R.exp_reion);
+    const D<N> nh = (1 - YHe) * P(F_OMEGA_B) * (K_NH / (a * a * a));
+    const double u = (yre - y) / dy;
+    const double th = tanh(u);
+    const D<N> xH = 0.5 * (1.0 + fHe) * (1.0 + th);
+    const D<N> xHe = 0.5 * fHe * (1.0 + tanh((R.z_reion_He - z) / R.Delta_z_reion_He));
+    const D<N> g = (a * K_G) * nh * dinv(b.aH);
+    s += w * (g * (xH + xHe));
+    const double du = (dyre * dy - (yre - y) * ddy) / (dy * dy);
+    sz += w * g.v * 0.5 * (1.0 + fHe.v) * (1.0 - th * th) * du;
+  }
+  // workgroup sums of s (1 + N values) and sz
+  double v[2 + N];
+  v[0] = s.v;
+  v[1] = sz;
+  if constexpr (N > 0) { ABCMB_DLOOP(N) v[2 + i_] = s.d[i_]; }
+  for (int q = 0; q < 2 + N; ++q) {
+    double x = v[q];
+    for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o);
+    if ((threadIdx.x & 63) == 0) red[(threadIdx.x >> 6) * (2 + N) + q] = x;
+  }
+  __syncthreads();
+  for (int q = 0; q < 2 + N; ++q) {
+    double x = 0.0;
+    for (int w8 = 0; w8 < REION_THREADS / 64; ++w8) x += red[w8 * (2 + N) + q];
+    v[q] = x;
+  }
+  __syncthreads();
+  tau = D<N>(v[0]);
+  if constexpr (N > 0) { ABCMB_DLOOP(N) tau.d[i_] = v[2 + i_]; }
+  dtau_dz = v[1];
+}
+
+template <int N>
+__global__ void __launch_bounds__(REION_THREADS) reion_kernel(const AbcmbCosmo* __restrict__ cosmos,
+                                                             const AbcmbTangent* __restrict__ tangents,
+                                                             const AbcmbReion* __restrict__ reion,
+                                                             const double* __restrict__ dtarget, int n_rec,
+                                                             double* __restrict__ xe_out, double* __restrict__ dxe_out,
+                                                             double* __restrict__ z_out, double* __restrict__ dz_out,
+                                                             int32_t* __restrict__ status) {
+  __shared__ double red[(REION_THREADS / 64) * (2 + ABCMB_TANGENT_DIRS)];
+  const int ci = blockIdx.x;
+  const AbcmbCosmo& c = cosmos[ci];
+  const AbcmbReion R = reion[ci];
+  const AbcmbTangent* T = nullptr;
+  if constexpr (N > 0) T = tangents + ci;
+  const Cosmo<0> P0{c, nullptr};
+  // bracket by bisection on [0, 50], then Newton kept inside the bracket
+  double lo = 0.0, hi = 50.0, z = 0.0, f = 0.0, fz = 0.0;
+  D<0> t0;
+  reion_tau<0>(P0, R, lo, t0, fz, red);
+  const double flo = t0.v - R.tau_reion;
+  bool ok = flo < 0.0;
+  for (int it = 0; it < 8; ++it) {
+    const double mid = 0.5 * (lo + hi);
+    reion_tau<0>(P0, R, mid, t0, fz, red);
+    if (t0.v - R.tau_reion < 0.0) lo = mid; else hi = mid;
+  }
+  z = 0.5 * (lo + hi);
+  int it = 0;
+  for (; it < 100; ++it) {
+    reion_tau<0>(P0, R, z, t0, fz, red);
+    f = t0.v - R.tau_reion;
+    if (f < 0.0) lo = z; else hi = z;
+    double zn = fz > 0.0 ? z - f / fz : 0.5 * (lo + hi);
+    if (!(zn > lo && zn < hi)) zn = 0.5 * (lo + hi);
+    const double dz = fabs(zn - z);
+    z = zn;
+    if (dz <= 1e-13 * fmax(1.0, z) || hi - lo <= 1e-13 * fmax(1.0, z)) break;
+  }
+  ok = ok && it < 100 && isfinite(z);
+  // implicit-function derivatives at the root
+  D<N> zd(z);
+  if constexpr (N > 0) {
+    const Cosmo<N> P{c, T};
+    D<N> tau;
+    double dtdz;
+    reion_tau<N>(P, R, z, tau, dtdz, red);
+    ABCMB_DLOOP(N) zd.d[i_] = (dtarget[(size_t)ci * N + i_] - tau.d[i_]) / dtdz;
+    if (threadIdx.x == 0) { ABCMB_DLOOP(N) dz_out[(size_t)ci * N + i_] = zd.d[i_]; }
+  }
+  if (threadIdx.x == 0) {
+    z_out[ci] = z;
+    status[ci] = ok ? ABCMB_OK : ABCMB_EINVAL;
+  }
+  // the table: reionization added on the recombination grid, the padding copies the last point
+  const int n = c.xe_n;
+  const Cosmo<N> P{c, T};
+  const D<N> YHe = P(F_YHE);
+  double* xo = xe_out + (size_t)ci * n;
+  for (int j = threadIdx.x; j < n_rec; j += blockDim.x) {
+    const double l = c.xe_x0 + j * c.xe_dx;
+    const D<N> x = tab_at<N>(c.xe_tab, P.dxe(), n, j) + xe_reion<N>(l, zd, R, YHe);
+    xo[j] = x.v;
+    if constexpr (N > 0) { ABCMB_DLOOP(N) dxe_out[((size_t)ci * N + i_) * n + j] = x.d[i_]; }
+  }
+  __syncthreads();
+  for (int j = n_rec + threadIdx.x; j < n; j += blockDim.x) {
+    xo[j] = xo[n_rec - 1];
+    if constexpr (N > 0) { ABCMB_DLOOP(N) dxe_out[((size_t)ci * N + i_) * n + j] = dxe_out[((size_t)ci * N + i_) * n + n_rec - 1]; }
+  }
+}
+
+}  // namespace
+}  // namespace abcmb
+
+extern "C" int abcmb_reionize(const AbcmbCosmo* cosmos_dev, const AbcmbTangent* tangents_dev, int n_cosmo,
+                              const AbcmbReion* reion_dev, const double* dtarget_dev, int n_rec, double* xe_out_dev,
+                              double* dxe_out_dev, double* z_out_dev, double* dz_out_dev, int32_t* status_dev,
+                              void* stream) {
+  using namespace abcmb;
+  ABCMB_REQUIRE(cosmos_dev && reion_dev && xe_out_dev && z_out_dev && status_dev, "null argument");
+  ABCMB_REQUIRE(n_cosmo > 0 && n_rec > 1, "bad sizes");
+  const bool tan = tangents_dev != nullptr;
+  ABCMB_REQUIRE(!tan || (dtarget_dev && dxe_out_dev && dz_out_dev), "tangents need dtarget, dxe_out and dz_out");
+  hipStream_t s = (hipStream_t)stream;
+  if (tan)
+    hipLaunchKernelGGL(reion_kernel<ABCMB_TANGENT_DIRS>, dim3(n_cosmo), dim3(REION_THREADS), 0, s, cosmos_dev,
+                       tangents_dev, reion_dev, dtarget_dev, n_rec, xe_out_dev, dxe_out_dev, z_out_dev, dz_out_dev,
+                       status_dev);
+  else
+    hipLaunchKernelGGL(reion_kernel<0>, dim3(n_cosmo), dim3(REION_THREADS), 0, s, cosmos_dev, nullptr, reion_dev,
+                       nullptr, n_rec, xe_out_dev, nullptr, z_out_dev, nullptr, status_dev);
+  ABCMB_HIP_CHECK(hipGetLastError());
+  return ABCMB_OK;
 }
 
 // ---------------------------------------------------------------- launch without allocation

@@ -154,7 +154,8 @@ class Output:
 class Model:
     """GPU drop-in for ``abcmb.Model`` on the perturbation + LOS path."""
 
-    def __init__(self, user_species=None, los_precision: str = "fp64", **kwargs):
+    def __init__(self, user_species=None, los_precision: str = "fp64", lens_mode: str = "vector",
+                 lens_precision: str = "fp64", **kwargs):
         kwargs.pop("adjoint", None)  # reference option for diffrax AD; not used here
         specs = S.load_specs(kwargs)
         self.specs = specs
@@ -208,6 +209,15 @@ class Model:
         if los_precision not in precisions:
             raise ValueError("los_precision must be 'fp64', 'fp32' or 'bf16'")
         self.los_precision = precisions[los_precision]
+        # lensing: "vector" (Wigner recursions in registers, FP64) or "mfma" (the Cgl
+        # and back-transform contractions as GEMMs on the matrix cores against
+        # d-matrices built once, in lens_precision)
+        if lens_mode not in ("vector", "mfma") or lens_precision not in precisions:
+            raise ValueError("lens_mode must be 'vector' or 'mfma', lens_precision 'fp64', 'fp32' or 'bf16'")
+        if lens_mode == "vector" and lens_precision != "fp64":
+            raise ValueError("the vector lensing path is FP64 only")
+        self.lens_mode = lens_mode
+        self.lens_precision = precisions[lens_precision]
         self.chunk = 128   # cosmologies per downstream launch (bounds the source buffers)
         self.timing = None  # list: per-launch pert_fwd event timing and work counts when set
         self.call_timing = None  # dict: entry-point name -> [(start event, end event, work)] when set
@@ -283,6 +293,22 @@ class Model:
                 scales=self.scales,
             )
         return self._dev
+
+    def _dmat(self):
+        """The lensing d-matrices for the MFMA path in ``lens_precision`` (built on
+        first use, then kept)."""
+        D = self._device_constants()
+        key = ("dmat", self.lens_precision)
+        if key not in D:
+            torch = _torch()
+            lib = abi.hip_lib()
+            nl = self.lensing_ells.size
+            nb = lib.abcmb_lens_dmat_size(self.mu.size, nl, self.lens_precision)
+            dm = torch.empty(nb, dtype=torch.uint8, device=torch.device("cuda"))
+            abi.check(lib.abcmb_lens_dmat(D["mu"].data_ptr(), self.mu.size, nl, self.lens_precision, dm.data_ptr(), nb,
+                                          torch.cuda.current_stream().cuda_stream), "abcmb_lens_dmat")
+            D[key] = dm
+        return D[key]
 
     def run_batch(self, params_list, derived: bool = False):
         """Outputs for several cosmologies; one ODE launch covers all their k-modes."""
@@ -439,49 +465,69 @@ class Model:
         [B][n_l] C_l^phiphi and [B][4][n_l] lensed TT, TE, EE, BB; [B][nq] P(k).
         """
         torch = _torch()
-        lib = abi.hip_lib()
-        D = self._device_constants()
         dev = torch.device("cuda")
-        stream = torch.cuda.current_stream().cuda_stream
         stage = _Stage(torch, dev)
-        dt = stage.add
-
-        ns = self.n_save
-        ncos = len(plans)
-        nkt = self.k_transfer.size
-        n_ell = self.ell_idx.size
-        n_tau = ns - 1
-        nl = self.lensing_ells.size
-        nq = self.k_pk.size
-        lensing = bool(self.specs["lensing"])
         nkmax = max(int(k.numel()) for k in kdev)
-        sc = self.scales.ctypes.data_as(C.c_void_p)
-        cos_dev = self._struct_array(torch, dev, self._cosmo_array(plans, pts, kdev, dt, tables=False), dt)
-        cl_raw = torch.empty((ncos, 3, n_ell), dtype=torch.float64, device=dev)
-        cl_full = torch.empty((ncos, 3, nl), dtype=torch.float64, device=dev)
-        pk = torch.empty((ncos, nq), dtype=torch.float64, device=dev)
-        clpp = lensed = None
-        bgv_all = dt(np.stack([pl.bgv for pl in plans]))
-        n_lens = plans[0].lens_lna.size if lensing else 0
-        if lensing:
+        inp = self._downstream_inputs(plans, pts, kdev, stage.add)
+        buf = self._downstream_buffers(len(plans), nkmax)
+        stage.flush()
+        self._downstream_launch(inp, buf, len(plans), nkmax)
+        return {key: buf[key] for key in ("cl_raw", "cl_full", "clpp", "lensed", "pk")}
+
+    def _downstream_inputs(self, plans, pts, kdev, dt):
+        """Device inputs of the downstream: the AbcmbCosmo array, the LOS background
+        [B][6][n_save-1] and the Limber axes [B][4][n_lens]."""
+        torch = _torch()
+        dev = torch.device("cuda")
+        ncos = len(plans)
+        inp = dict(cos=self._struct_array(torch, dev, self._cosmo_array(plans, pts, kdev, dt, tables=False), dt),
+                   bgv=dt(np.stack([pl.bgv for pl in plans])), lens=None)
+        if self.specs["lensing"]:
+            n_lens = plans[0].lens_lna.size
             lens_h = np.zeros((ncos, 4, n_lens))
             for i, pl in enumerate(plans):
                 lens_h[i, 0], lens_h[i, 1], lens_h[i, 2] = pl.lens_lna, pl.lens_aH, pl.lens_tau
                 lens_h[i, 3, 0] = pl.chi_rec
-            lens_all = dt(lens_h)
-            clpp = torch.empty((ncos, nl), dtype=torch.float64, device=dev)
-            lensed = torch.empty((ncos, 4, nl), dtype=torch.float64, device=dev)
+            inp["lens"] = dt(lens_h)
+        return inp
+
+    def _downstream_buffers(self, ncos, nkmax):
+        """Outputs and workspaces of the downstream of ``ncos`` cosmologies of at most ``nkmax`` k-modes."""
+        torch = _torch()
+        lib = abi.hip_lib()
+        dev = torch.device("cuda")
+        f64 = dict(dtype=torch.float64, device=dev)
+        ns, nkt, n_ell = self.n_save, self.k_transfer.size, self.ell_idx.size
+        nl, nq = self.lensing_ells.size, self.k_pk.size
+        chunk = max(1, min(ncos, self.chunk))
+        buf = dict(cl_raw=torch.empty((ncos, 3, n_ell), **f64), cl_full=torch.empty((ncos, 3, nl), **f64),
+                   pk=torch.empty((ncos, nq), **f64), clpp=None, lensed=None,
+                   ws_src=torch.empty(lib.abcmb_sources_workspace_size(chunk, nkmax, ns) // 8 + 1, **f64),
+                   ws_los=torch.empty(lib.abcmb_los_workspace_size(chunk, n_ell, nkt) // 8 + 1, **f64),
+                   src=torch.empty((chunk, 4, ns - 1, nkt), **f64), ws_lens=None)
+        if self.specs["lensing"]:
+            buf["clpp"] = torch.empty((ncos, nl), **f64)
+            buf["lensed"] = torch.empty((ncos, 4, nl), **f64)
+            buf["ws_lens"] = torch.empty(self._lens_ws_bytes(lib, chunk, nl) // 8 + 1, **f64)
+        return buf
+
+    def _downstream_launch(self, inp, buf, ncos, nkmax):
+        """The downstream kernels on the current stream (no allocation, no copy:
+        this sequence is what a HIP graph captures)."""
+        torch = _torch()
+        lib = abi.hip_lib()
+        D = self._device_constants()
+        stream = torch.cuda.current_stream().cuda_stream
+        ns, nkt, n_ell = self.n_save, self.k_transfer.size, self.ell_idx.size
+        n_tau, nl, nq = ns - 1, self.lensing_ells.size, self.k_pk.size
+        lensing = bool(self.specs["lensing"])
+        sc = self.scales.ctypes.data_as(C.c_void_p)
         csz = C.sizeof(abi.AbcmbCosmo)
         chunk = max(1, min(ncos, self.chunk))
-        ws_src = torch.empty(lib.abcmb_sources_workspace_size(chunk, nkmax, ns) // 8 + 1, dtype=torch.float64,
-                             device=dev)
-        ws_los = torch.empty(lib.abcmb_los_workspace_size(chunk, n_ell, nkt) // 8 + 1, dtype=torch.float64, device=dev)
-        src = torch.empty((chunk, 4, n_tau, nkt), dtype=torch.float64, device=dev)
-        ws_lens = None
-        if lensing:
-            ws_lens = torch.empty(lib.abcmb_lens_workspace_size(chunk, self.mu.size, nl) // 8 + 1,
-                                  dtype=torch.float64, device=dev)
-        stage.flush()
+        cos_dev, bgv_all, lens_all = inp["cos"], inp["bgv"], inp["lens"]
+        n_lens = lens_all.shape[2] if lensing else 0
+        cl_raw, cl_full, clpp, lensed, pk = buf["cl_raw"], buf["cl_full"], buf["clpp"], buf["lensed"], buf["pk"]
+        ws_src, ws_los, src, ws_lens = buf["ws_src"], buf["ws_los"], buf["src"], buf["ws_lens"]
         for c0 in range(0, ncos, chunk):
             nb = min(chunk, ncos - c0)
             cp = cos_dev.data_ptr() + c0 * csz
@@ -499,11 +545,32 @@ class Model:
                 abi.check(lib.abcmb_lensing_clpp(cp, nb, lens_all[c0].data_ptr(), n_lens,
                                                  D["lens_ells"].data_ptr(), nl, clpp[c0].data_ptr(), stream),
                           "abcmb_lensing_clpp")
-                abi.check(lib.abcmb_lens(nb, cl_full[c0].data_ptr(), clpp[c0].data_ptr(), nl, D["mu"].data_ptr(),
-                                         D["w"].data_ptr(), self.mu.size, lensed[c0].data_ptr(), ws_lens.data_ptr(),
-                                         ws_lens.numel() * 8, stream), "abcmb_lens")
+                abi.check(self._lens_call(lib, D, nb, cl_full[c0].data_ptr(), clpp[c0].data_ptr(), nl,
+                                          lensed[c0].data_ptr(), ws_lens, stream), "lensing")
             abi.check(lib.abcmb_pk_lin(cp, nb, D["k_pk"].data_ptr(), nq, pk[c0].data_ptr(), stream), "abcmb_pk_lin")
-        return dict(cl_raw=cl_raw, cl_full=cl_full, clpp=clpp, lensed=lensed, pk=pk)
+
+    def downstream_graph(self, n_cosmo: int, nk_max: int):
+        """A ``DownstreamGraph`` for batches of exactly ``n_cosmo`` cosmologies of at
+        most ``nk_max`` k-modes (built once per shape and kept)."""
+        key = (n_cosmo, nk_max)
+        cache = self.__dict__.setdefault("_graphs", {})
+        if key not in cache:
+            cache[key] = DownstreamGraph(self, n_cosmo, nk_max)
+        return cache[key]
+
+    def _lens_ws_bytes(self, lib, n_cosmo, nl):
+        if self.lens_mode == "mfma":
+            return lib.abcmb_lens_mfma_workspace_size(n_cosmo, self.mu.size, nl, self.lens_precision)
+        return lib.abcmb_lens_workspace_size(n_cosmo, self.mu.size, nl)
+
+    def _lens_call(self, lib, D, nb, cl_ptr, clpp_ptr, nl, out_ptr, ws, stream):
+        """One lensing call on ``nb`` cosmologies through the selected entry point."""
+        if self.lens_mode == "mfma":
+            return lib.abcmb_lens_mfma(nb, cl_ptr, clpp_ptr, nl, D["mu"].data_ptr(), D["w"].data_ptr(), self.mu.size,
+                                       self._dmat().data_ptr(), self.lens_precision, out_ptr, ws.data_ptr(),
+                                       ws.numel() * 8, stream)
+        return lib.abcmb_lens(nb, cl_ptr, clpp_ptr, nl, D["mu"].data_ptr(), D["w"].data_ptr(), self.mu.size, out_ptr,
+                              ws.data_ptr(), ws.numel() * 8, stream)
 
     def _timed(self, name, work, fn):
         """Run ``fn`` (one C-ABI call on the current stream), bracketed by HIP
@@ -636,8 +703,7 @@ class Model:
         clpp = torch.empty((ncos, nl), dtype=torch.float64, device=dev)
         lensed = torch.empty((ncos, 4, nl), dtype=torch.float64, device=dev)
         chunk = max(1, min(ncos, self.chunk))
-        ws = torch.empty(lib.abcmb_lens_workspace_size(chunk, self.mu.size, nl) // 8 + 1, dtype=torch.float64,
-                         device=dev)
+        ws = torch.empty(self._lens_ws_bytes(lib, chunk, nl) // 8 + 1, dtype=torch.float64, device=dev)
         csz = C.sizeof(abi.AbcmbCosmo)
         stage.flush()
         for c0 in range(0, ncos, chunk):
@@ -645,11 +711,9 @@ class Model:
             abi.check(lib.abcmb_lensing_clpp(cos_dev.data_ptr() + c0 * csz, nb, lens_all[c0].data_ptr(), n_lens,
                                              D["lens_ells"].data_ptr(), nl, clpp[c0].data_ptr(), stream),
                       "abcmb_lensing_clpp")
-            abi.check(self._timed("abcmb_lens", dict(n_cosmo=nb, n_mu=self.mu.size, n_l=nl),
-                                  lambda: lib.abcmb_lens(nb, cl_full[c0].data_ptr(), clpp[c0].data_ptr(), nl,
-                                                         D["mu"].data_ptr(), D["w"].data_ptr(), self.mu.size,
-                                                         lensed[c0].data_ptr(), ws.data_ptr(), ws.numel() * 8,
-                                                         stream)), "abcmb_lens")
+            abi.check(self._timed("lensing", dict(n_cosmo=nb, n_mu=self.mu.size, n_l=nl, mode=self.lens_mode),
+                                  lambda: self._lens_call(lib, D, nb, cl_full[c0].data_ptr(), clpp[c0].data_ptr(), nl,
+                                                          lensed[c0].data_ptr(), ws, stream)), "lensing")
         return lensed
 
     def _pk(self, k, z, PT, params, kind):
@@ -750,6 +814,82 @@ class Model:
 
     def __call__(self, params: dict = None) -> Output:
         return self.run_batch([params if params is not None else {}])[0]
+
+
+class DownstreamGraph:
+    """The downstream of a batch (sources, LOS, l-spline, C_l^phiphi, lensing,
+    P(k)) captured once as a HIP graph and replayed for new inputs.
+
+    Every input lives in a static device buffer (the cosmology structs, the
+    LOS and Limber background vectors, one perturbation-table slot and one
+    k-grid slot per cosmology); ``run`` copies a batch's inputs into them on
+    the current stream and replays the graph after the copies.  The captured
+    sequence holds kernels only: no allocation, no copy, no memset.  Results
+    are the static output buffers (overwritten by the next ``run``).
+    """
+
+    def __init__(self, model, n_cosmo, nk_max):
+        torch = _torch()
+        dev = torch.device("cuda")
+        self.model, self.n_cosmo, self.nk_max = model, n_cosmo, nk_max
+        ns = model.n_save
+        f64 = dict(dtype=torch.float64, device=dev)
+        model._device_constants()
+        if model.specs["lensing"] and model.lens_mode == "mfma":
+            model._dmat()        # built outside the capture
+        self.pt = torch.zeros((n_cosmo, abi.N_PT_FIELDS * ns * nk_max), **f64)
+        self.k = torch.ones((n_cosmo, nk_max), **f64)
+        self.lna = torch.zeros((n_cosmo, ns), **f64)
+        csz = C.sizeof(abi.AbcmbCosmo)
+        self.inp = dict(cos=torch.zeros(n_cosmo * csz, dtype=torch.uint8, device=dev),
+                        bgv=torch.zeros((n_cosmo, 6, ns - 1), **f64), lens=None)
+        if model.specs["lensing"]:
+            self.inp["lens"] = torch.zeros((n_cosmo, 4, 500), **f64)
+        self.buf = model._downstream_buffers(n_cosmo, nk_max)
+        torch.cuda.synchronize()
+        self.graph = torch.cuda.CUDAGraph()
+        side = torch.cuda.Stream()
+        side.wait_stream(torch.cuda.current_stream())
+        with torch.cuda.stream(side):
+            with torch.cuda.graph(self.graph, stream=side):
+                model._downstream_launch(self.inp, self.buf, n_cosmo, nk_max)
+        torch.cuda.current_stream().wait_stream(side)
+
+    def run(self, plans, pts, kdev):
+        """Downstream of ``plans`` (exactly ``n_cosmo`` of them) with tables ``pts``."""
+        torch = _torch()
+        m = self.model
+        if len(plans) != self.n_cosmo:
+            raise ValueError(f"this graph holds {self.n_cosmo} cosmologies, got {len(plans)}")
+        ns = m.n_save
+        stage = _Stage(torch, torch.device("cuda"))
+        slots_pt, slots_k = [], []
+        for i, (pl, kd) in enumerate(zip(plans, kdev)):
+            nk = int(kd.numel())
+            if nk > self.nk_max:
+                raise ValueError(f"cosmology {i} has {nk} k-modes; this graph holds {self.nk_max}")
+            if pl.lens_lna is not None and pl.lens_lna.size != 500:
+                raise ValueError("the Limber axis must have 500 points")
+            slots_pt.append(self.pt[i, :abi.N_PT_FIELDS * ns * nk])
+            slots_k.append(self.k[i, :nk])
+        # staged host inputs (their device copies are valid after the flush)
+        lna_st = stage.add(np.stack([pl.lna_save for pl in plans]))
+        cos = m._cosmo_array(plans, slots_pt, slots_k, stage.add, tables=False)
+        for i, c in enumerate(cos):
+            c.lna_save = self.lna[i].data_ptr()
+        cos_st = m._struct_array(torch, torch.device("cuda"), cos, stage.add)
+        vec = m._downstream_inputs(plans, pts, kdev, stage.add)
+        stage.flush()
+        for sp, sk, pt, kd in zip(slots_pt, slots_k, pts, kdev):
+            sp.copy_(pt.reshape(-1))
+            sk.copy_(kd)
+        self.lna.copy_(lna_st)
+        self.inp["cos"].copy_(cos_st)
+        self.inp["bgv"].copy_(vec["bgv"])
+        if self.inp["lens"] is not None:
+            self.inp["lens"].copy_(vec["lens"])
+        self.graph.replay()
+        return {key: self.buf[key] for key in ("cl_raw", "cl_full", "clpp", "lensed", "pk")}
 
 
 def _moved_tables(torch, pt, dpt, hs):

@@ -114,6 +114,14 @@ typedef struct AbcmbOdeSettings {
   int32_t n_q, n;                   /* massive bins (0 if absent), state size */
 } AbcmbOdeSettings;
 
+/* tanh reionization of one cosmology (abcmb/background.py:939-1017): the
+ * optical depth it must reach and the shape parameters of the hydrogen and
+ * helium steps. */
+typedef struct AbcmbReion {
+  double tau_reion;
+  double Delta_z_reion, z_reion_He, Delta_z_reion_He, exp_reion;
+} AbcmbReion;
+
 /* ---------------- host background producer (libabcmb_host.so) ---------- */
 
 /*
@@ -138,6 +146,9 @@ int abcmb_host_background(AbcmbCosmo* c, double tau_reion, double z_reion_in,
                           double Delta_z_reion_He, double exp_reion,
                           int32_t* rec_switch);
 int abcmb_host_recomb_points(void);
+/* abcmb_host_background without the reionization: x_e holds the recombination
+ * history alone and z_reion is 0 (abcmb_reionize adds the rest on the GPU). */
+int abcmb_host_recombination(AbcmbCosmo* c, int32_t* rec_switch);
 
 /*
  * Background vectors on the LOS grid lna[0..n-1] (abcmb/spectrum.py:640-651):
@@ -160,6 +171,22 @@ int abcmb_host_drag(const AbcmbCosmo* c, double* z_d, double* rs_d);
 
 const char* abcmb_last_error(void);
 int abcmb_device_init(void);
+
+/*
+ * Launch-shape settings that the autotuner (abcmb_amd/autotune.py) picks by
+ * timing and caches on disk; process-wide, 0 = the built-in default.  No
+ * counterpart in the reference (XLA chooses its own launch shapes).
+ *   ABCMB_TUNE_PERT_PERSISTENT: 0 = one workgroup per ODE work item; v > 0 =
+ *     a persistent grid of v% of the resident workgroups pulling items from a
+ *     device work queue (abcmb_pert_fwd / _tangent / _noalloc).
+ *   ABCMB_TUNE_LENS_GEMM_COLS: column tile of the lensing GEMMs
+ *     (abcmb_lens_mfma): 0 or 32 = 32 cosmologies per tile, 64 = 64.
+ */
+#define ABCMB_TUNE_PERT_PERSISTENT 0
+#define ABCMB_TUNE_LENS_GEMM_COLS 1
+#define ABCMB_TUNE_KEYS 8
+int abcmb_tune_set(int key, int value);
+int abcmb_tune_get(int key);
 
 /*
  * Perturbation ODE for every work item: adaptive ESDIRK (Kvaerno5) with
@@ -310,6 +337,48 @@ size_t abcmb_lens_workspace_size(int n_cosmo, int n_mu, int n_l);
 int abcmb_lens(int n_cosmo, const double* cl_in_dev, const double* clpp_dev, int n_l,
                const double* mu_dev, const double* w_dev, int n_mu,
                double* out_dev, void* workspace, size_t ws_bytes, void* stream);
+
+/*
+ * Correlation-function lensing with its two contractions over shared operands
+ * on the matrix cores: the Cgl pass (abcmb/spectrum.py:498-503) and the
+ * back-transform (:564-570) as batched GEMMs across the n_cosmo cosmologies
+ * of the call, against reduced Wigner d-matrices built once by
+ * abcmb_lens_dmat (cosmology independent, [6][n_mu][n_l] plus the recursion
+ * tables).  precision 0: FP64 operands and MFMA (the results of abcmb_lens to
+ * rounding); 1: FP32 operands, FP32 MFMA accumulation; 2: BF16 operands, FP32
+ * accumulation.  The correlation pass between them stays FP64 on the vector
+ * units.  Arguments and outputs are those of abcmb_lens; the d-matrices must
+ * be built for the same mu nodes, n_l and precision.
+ */
+size_t abcmb_lens_dmat_size(int n_mu, int n_l, int precision);
+int abcmb_lens_dmat(const double* mu_dev, int n_mu, int n_l, int precision, void* dmat_dev,
+                    size_t dmat_bytes, void* stream);
+size_t abcmb_lens_mfma_workspace_size(int n_cosmo, int n_mu, int n_l, int precision);
+int abcmb_lens_mfma(int n_cosmo, const double* cl_in_dev, const double* clpp_dev, int n_l,
+                    const double* mu_dev, const double* w_dev, int n_mu, const void* dmat_dev,
+                    int precision, double* out_dev, void* workspace, size_t ws_bytes, void* stream);
+
+/*
+ * Reionization on the GPU (Background reionization model and its optical-depth
+ * root, abcmb/background.py:939-1017; the reference solves z_reion with
+ * optimistix Newton, :1008-1017).  For each cosmology whose x_e table
+ * (cosmos_dev[i].xe_tab) holds the recombination history alone
+ * (abcmb_host_recombination), solve tau_reion(z_reion) = reion_dev[i].tau_reion
+ * on the 2000-point trapezoid over ln a in [-5, 0], and write the table with
+ * the hydrogen and helium reionization added (xe_out_dev [n_cosmo][xe_n]: the
+ * first n_rec entries, the padding copying entry n_rec - 1) and z_reion
+ * (z_out_dev [n_cosmo]).  With tangents_dev (the cosmologies' AbcmbTangent
+ * inputs, x_e tables of the recombination history), also the implicit-function
+ * derivatives dz_reion/dtheta = (dtau_target/dtheta - dtau/dtheta|_z) /
+ * (dtau/dz) (dz_out_dev [n_cosmo][ABCMB_TANGENT_DIRS]; dtarget_dev
+ * [n_cosmo][ABCMB_TANGENT_DIRS] holds dtau_target along each direction) and the
+ * tangents of the finished table (dxe_out_dev [n_cosmo][ABCMB_TANGENT_DIRS][xe_n]).
+ * status_dev [n_cosmo]: 0, or ABCMB_EINVAL if no root in z in [0, 50].
+ */
+int abcmb_reionize(const AbcmbCosmo* cosmos_dev, const AbcmbTangent* tangents_dev, int n_cosmo,
+                   const AbcmbReion* reion_dev, const double* dtarget_dev, int n_rec,
+                   double* xe_out_dev, double* dxe_out_dev, double* z_out_dev, double* dz_out_dev,
+                   int32_t* status_dev, void* stream);
 
 /* Linear matter power spectrum at z = 0 (abcmb/spectrum.py:268-302):
  * pk_dev out [B][nq] on the query grid kq_dev [nq]. */
