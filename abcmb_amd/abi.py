@@ -62,6 +62,10 @@ N_TANGENT_DIRS = 5
 COSMO_DOUBLES = 31
 
 
+class AbcmbReion(C.Structure):
+    _fields_ = [(n, _D) for n in ("tau_reion", "Delta_z_reion", "z_reion_He", "Delta_z_reion_He", "exp_reion")]
+
+
 class AbcmbTangent(C.Structure):
     _fields_ = [("par", _P), ("tau_tab", _P), ("xe_tab", _P), ("tm_tab", _P), ("pt", _P)]
 
@@ -90,6 +94,7 @@ def host_lib():
         lib.abcmb_host_lna_start.argtypes = [C.POINTER(AbcmbCosmo), _P, C.c_int, _D, _D, _P]
         lib.abcmb_host_eval.argtypes = [C.POINTER(AbcmbCosmo), _P, C.c_int, _P]
         lib.abcmb_host_drag.argtypes = [C.POINTER(AbcmbCosmo), C.POINTER(_D), C.POINTER(_D)]
+        lib.abcmb_host_recombination.argtypes = [C.POINTER(AbcmbCosmo), C.POINTER(_I)]
         _host = lib
     return _host
 
@@ -100,7 +105,7 @@ HIP_SYMBOLS = (
     "abcmb_los_tangent_workspace_size", "abcmb_los_tangent",
     "abcmb_lensing_clpp", "abcmb_lens_workspace_size", "abcmb_lens", "abcmb_pk_lin", "abcmb_pert_fwd_noalloc",
     "abcmb_pk", "abcmb_lens_dmat_size", "abcmb_lens_dmat", "abcmb_lens_mfma_workspace_size", "abcmb_lens_mfma",
-    "abcmb_tune_set", "abcmb_tune_get",
+    "abcmb_tune_set", "abcmb_tune_get", "abcmb_reionize",
 )
 TUNE_PERT_PERSISTENT, TUNE_LENS_GEMM_COLS = 0, 1
 PK_M, PK_CB = 0, 1
@@ -146,6 +151,7 @@ def hip_lib():
         lib.abcmb_lens_dmat_size.argtypes = [C.c_int, C.c_int, C.c_int]
         lib.abcmb_lens_dmat.argtypes = [_P, C.c_int, C.c_int, C.c_int, _P, C.c_size_t, _P]
         lib.abcmb_tune_set.argtypes = [C.c_int, C.c_int]
+        lib.abcmb_reionize.argtypes = [_P, _P, C.c_int, _P, _P, C.c_int, _P, _P, _P, _P, _P, _P]
         lib.abcmb_tune_get.argtypes = [C.c_int]
         lib.abcmb_lens_mfma_workspace_size.restype = C.c_size_t
         lib.abcmb_lens_mfma_workspace_size.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int]

@@ -26,10 +26,12 @@ class Background:
 
     def __init__(self, params: dict, has_ncdm: bool, n_q: int = 3, k_pivot: float = 0.05,
                  input_tau_reion: bool = True, xe_pad: int = 0, rec_switch: int = -1,
-                 lna_rec: float = None, lna_transfer_start: float = None):
+                 lna_rec: float = None, lna_transfer_start: float = None, reionization: bool = True):
         """``rec_switch``, ``lna_rec``, ``lna_transfer_start``: grid choices held at
         another cosmology's (finite differences with frozen grids); the producer
-        then keeps them instead of searching for its own."""
+        then keeps them instead of searching for its own.  ``reionization=False``:
+        the x_e table holds the recombination history alone and z_reion is 0
+        (``abcmb_reionize`` adds the reionization on the GPU)."""
         lib = abi.host_lib()
         self.params = params
         n_rec = lib.abcmb_host_recomb_points()
@@ -56,8 +58,13 @@ class Background:
         if lna_rec is not None and lna_transfer_start is not None:
             c.lna_rec, c.lna_transfer_start = lna_rec, lna_transfer_start
         sw = C.c_int32(rec_switch)
-        rc = lib.abcmb_host_background(C.byref(c), tau_re, z_re, p["Delta_z_reion"], p["z_reion_He"],
-                                       p["Delta_z_reion_He"], p["exp_reion"], C.byref(sw))
+        if reionization:
+            rc = lib.abcmb_host_background(C.byref(c), tau_re, z_re, p["Delta_z_reion"], p["z_reion_He"],
+                                           p["Delta_z_reion_He"], p["exp_reion"], C.byref(sw))
+        else:
+            rc = lib.abcmb_host_recombination(C.byref(c), C.byref(sw))
+        self.reion = abi.AbcmbReion(tau_reion=tau_re, Delta_z_reion=p["Delta_z_reion"], z_reion_He=p["z_reion_He"],
+                                    Delta_z_reion_He=p["Delta_z_reion_He"], exp_reion=p["exp_reion"])
         if rc != 0:
             raise RuntimeError(f"abcmb_host_background failed ({rc})")
         self.rec_switch = sw.value

@@ -1733,6 +1733,12 @@ __global__ void __launch_bounds__(REION_THREADS) reion_kernel(const AbcmbCosmo* 
   const AbcmbReion R = reion[ci];
   const AbcmbTangent* T = nullptr;
   if constexpr (N > 0) T = tangents + ci;
+  // every cosmology's table has the stride of the first, and the recombination
+  // points fit in it (uniform across the workgroup: no barrier is skipped by some threads only)
+  if (c.xe_n != cosmos[0].xe_n || n_rec > c.xe_n) {
+    if (threadIdx.x == 0) status[ci] = ABCMB_EINVAL;
+    return;
+  }
   const Cosmo<0> P0{c, nullptr};
   // bracket by bisection on [0, 50], then Newton kept inside the bracket
   double lo = 0.0, hi = 50.0, z = 0.0, f = 0.0, fz = 0.0;

@@ -373,7 +373,9 @@ int abcmb_lens_mfma(int n_cosmo, const double* cl_in_dev, const double* clpp_dev
  * (dtau/dz) (dz_out_dev [n_cosmo][ABCMB_TANGENT_DIRS]; dtarget_dev
  * [n_cosmo][ABCMB_TANGENT_DIRS] holds dtau_target along each direction) and the
  * tangents of the finished table (dxe_out_dev [n_cosmo][ABCMB_TANGENT_DIRS][xe_n]).
- * status_dev [n_cosmo]: 0, or ABCMB_EINVAL if no root in z in [0, 50].
+ * Every cosmology's xe_n equals the first's (the row stride of the outputs)
+ * and is at least n_rec.  status_dev [n_cosmo]: 0, or ABCMB_EINVAL if no root
+ * in z in [0, 50] or the sizes break that rule.
  */
 int abcmb_reionize(const AbcmbCosmo* cosmos_dev, const AbcmbTangent* tangents_dev, int n_cosmo,
                    const AbcmbReion* reion_dev, const double* dtarget_dev, int n_rec,
