@@ -105,7 +105,7 @@ HIP_SYMBOLS = (
     "abcmb_los_tangent_workspace_size", "abcmb_los_tangent",
     "abcmb_lensing_clpp", "abcmb_lens_workspace_size", "abcmb_lens", "abcmb_pk_lin", "abcmb_pert_fwd_noalloc",
     "abcmb_pk", "abcmb_lens_dmat_size", "abcmb_lens_dmat", "abcmb_lens_mfma_workspace_size", "abcmb_lens_mfma",
-    "abcmb_tune_set", "abcmb_tune_get", "abcmb_reionize",
+    "abcmb_tune_set", "abcmb_tune_get", "abcmb_reionize", "abcmb_los_background",
 )
 TUNE_PERT_PERSISTENT, TUNE_LENS_GEMM_COLS = 0, 1
 PK_M, PK_CB = 0, 1
@@ -152,6 +152,7 @@ def hip_lib():
         lib.abcmb_lens_dmat.argtypes = [_P, C.c_int, C.c_int, C.c_int, _P, C.c_size_t, _P]
         lib.abcmb_tune_set.argtypes = [C.c_int, C.c_int]
         lib.abcmb_reionize.argtypes = [_P, _P, C.c_int, _P, _P, C.c_int, _P, _P, _P, _P, _P, _P]
+        lib.abcmb_los_background.argtypes = [_P, _P, C.c_int, _P, C.c_int, _P, _P, _P]
         lib.abcmb_tune_get.argtypes = [C.c_int]
         lib.abcmb_lens_mfma_workspace_size.restype = C.c_size_t
         lib.abcmb_lens_mfma_workspace_size.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int]

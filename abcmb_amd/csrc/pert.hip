@@ -1820,6 +1820,157 @@ extern "C" int abcmb_reionize(const AbcmbCosmo* cosmos_dev, const AbcmbTangent* 
   return ABCMB_OK;
 }
 
+// ---------------------------------------------------------------- LOS background on the device
+// The six line-of-sight background vectors (abcmb/spectrum.py:640-651) of a
+// cosmology on its LOS grid: tau, g, dg/dlna, aH, exp(-kappa) and aH' aH, with
+// the optical depth kappa(lna) = int_lna^0 dl / (tau_c aH)
+// (abcmb/background.py:682-742) by 4-point Gauss-Legendre over the 20000
+// equal panels of [-10, 0] that the host producer uses.  One workgroup per
+// cosmology: each thread sums a run of 40 consecutive panels, a suffix scan over
+// the runs gives kappa at every run boundary, and each LOS point adds the
+// panels between it and the next boundary.  With tangents, every quantity
+// carries its derivatives along the parameter directions.
+namespace abcmb {
+namespace {
+
+constexpr int KAPPA_NODES = 20001;
+constexpr double KAPPA_LO = -10.0;
+constexpr int LOSBG_THREADS = 512;
+constexpr int KAPPA_RUN = (KAPPA_NODES - 1 + LOSBG_THREADS - 1) / LOSBG_THREADS;
+__constant__ double GL4_X[4] = {-0.8611363115940526, -0.3399810435848563, 0.3399810435848563, 0.8611363115940526};
+__constant__ double GL4_W[4] = {0.3478548451374538, 0.6521451548625461, 0.6521451548625461, 0.3478548451374538};
+
+__device__ __forceinline__ double kappa_node(int i) {
+  return i == KAPPA_NODES - 1 ? 0.0 : KAPPA_LO + (0.0 - KAPPA_LO) * i / (KAPPA_NODES - 1);
+}
+
+// the optical-depth rate dkappa/d(-lna) = a n_e sigma_T c / aH = 1 / (tau_c aH)
+template <int N>
+__device__ __forceinline__ D<N> kappa_rate(const Cosmo<N>& P, double l) {
+  constexpr double K_NH = 3.0 * H0_OVER_H * H0_OVER_H / 8 / PI / GN / MH;
+  constexpr double K_G = SIGMA_T * C_CM / C_MPC;
+  Bg<N> b;
+  bg_rho(P, l, b);
+  const D<N> nh = (1 - P(F_YHE)) * P(F_OMEGA_B) * (K_NH / (b.a * b.a * b.a));
+  return (b.a * K_G) * nh * bg_xe(P, l) * dinv(b.aH);
+}
+
+template <int N>
+__device__ __forceinline__ D<N> kappa_gl4(const Cosmo<N>& P, double lo, double hi) {
+  const double m = 0.5 * (lo + hi), hw = 0.5 * (hi - lo);
+  D<N> s(0.0);
+#pragma unroll
+  for (int q = 0; q < 4; ++q) s += GL4_W[q] * kappa_rate<N>(P, m + hw * GL4_X[q]);
+  return s * hw;
+}
+
+template <int N>
+__global__ void __launch_bounds__(LOSBG_THREADS) losbg_kernel(const AbcmbCosmo* __restrict__ cosmos,
+                                                              const AbcmbTangent* __restrict__ tangents,
+                                                              const double* __restrict__ lna, int n,
+                                                              double* __restrict__ out, double* __restrict__ dout) {
+  constexpr int NV = 1 + N;
+  constexpr double K_NH = 3.0 * H0_OVER_H * H0_OVER_H / 8 / PI / GN / MH;
+  constexpr double K_TC = C_MPC / SIGMA_T / C_CM;
+  constexpr double K_RHO = 3.0 * H0_OVER_H * H0_OVER_H / 8.0 / PI / GN;
+  constexpr double K_AHP = 4.0 * PI * GN / 3.0 / (C_MPC * C_MPC);
+  __shared__ double runs[LOSBG_THREADS * NV];
+  __shared__ double total[NV];
+  const int ci = blockIdx.x;
+  const AbcmbCosmo& c = cosmos[ci];
+  const AbcmbTangent* T = nullptr;
+  if constexpr (N > 0) T = tangents + ci;
+  const Cosmo<N> P{c, T};
+  // this thread's run of panels
+  {
+    const int p0 = min(threadIdx.x * KAPPA_RUN, KAPPA_NODES - 1), p1 = min(p0 + KAPPA_RUN, KAPPA_NODES - 1);
+    D<N> s(0.0);
+    for (int p = p0; p < p1; ++p) s += kappa_gl4<N>(P, kappa_node(p), kappa_node(p + 1));
+    runs[threadIdx.x * NV] = s.v;
+    if constexpr (N > 0) { ABCMB_DLOOP(N) runs[threadIdx.x * NV + 1 + i_] = s.d[i_]; }
+  }
+  __syncthreads();
+  // exclusive suffix sums over the runs, one component per thread
+  if (threadIdx.x < NV) {
+    double acc = 0.0;
+    for (int r = LOSBG_THREADS - 1; r >= 0; --r) {
+      const double v = runs[r * NV + threadIdx.x];
+      runs[r * NV + threadIdx.x] = acc;
+      acc += v;
+    }
+    total[threadIdx.x] = acc;
+  }
+  __syncthreads();
+  const double h = kappa_node(1) - kappa_node(0);
+  for (int j = threadIdx.x; j < n; j += blockDim.x) {
+    const double l = lna[(size_t)ci * n + j];
+    D<N> kap(0.0);
+    if (l <= KAPPA_LO) {
+      kap.v = total[0];
+      if constexpr (N > 0) { ABCMB_DLOOP(N) kap.d[i_] = total[1 + i_]; }
+    } else if (l < 0.0) {
+      const int i = min((int)((l - KAPPA_LO) / h), KAPPA_NODES - 2);
+      // kappa at node i + 1: the suffix of its run plus the panels to that run's end
+      const int r = (i + 1) / KAPPA_RUN;
+      kap.v = runs[r * NV];
+      if constexpr (N > 0) { ABCMB_DLOOP(N) kap.d[i_] = runs[r * NV + 1 + i_]; }
+      const int pend = min((r + 1) * KAPPA_RUN, KAPPA_NODES - 1);
+      for (int p = i + 1; p < pend; ++p) kap += kappa_gl4<N>(P, kappa_node(p), kappa_node(p + 1));
+      kap += kappa_gl4<N>(P, l, kappa_node(i + 1));
+    }
+    const D<N> em = l < KAPPA_LO ? D<N>(0.0) : dexp(-kap);
+    Bg<N> b;
+    bg_rho(P, l, b);
+    const double ia = 1.0 / b.a;
+    const D<N> xe = bg_xe(P, l);
+    const D<N> nh = (1 - P(F_YHE)) * P(F_OMEGA_B) * (K_NH * ia * ia * ia);
+    const D<N> tc = dinv(nh * xe) * (ia * K_TC);
+    // slope of the x_e interpolant (zero off the table and where the index is clipped)
+    D<N> dxe(0.0);
+    if (l >= c.xe_x0 && l < c.xe_lna_last) {
+      const double hi = c.xe_x0 + c.xe_n * c.xe_dx;
+      const double fi = (l - c.xe_x0) / (hi - c.xe_x0) * (c.xe_n - 1);
+      if (fi >= 1e-6 && fi <= c.xe_n - 1.0 - 1e-6) {
+        const int il = (int)floor(fi), iu = min(il + 1, c.xe_n - 1);
+        const double sc = (c.xe_n - 1) / (hi - c.xe_x0);
+        dxe = (tab_at<N>(c.xe_tab, P.dxe(), c.xe_n, iu) - tab_at<N>(c.xe_tab, P.dxe(), c.xe_n, il)) * sc;
+      }
+    }
+    const D<N> g = em * dinv(tc);
+    const D<N> gp = l < KAPPA_LO ? D<N>(0.0) : g * (dinv(tc * b.aH) - (2.0 - dxe * dinv(xe)));
+    const D<N> rde = P(F_OMEGA_LAMBDA) * K_RHO;
+    const D<N> rp3 = -2.0 * rde + b.rc + b.rb + 2.0 * (b.rg + b.ru) + b.rn + 3.0 * b.pn;
+    const D<N> ahp = -(K_AHP * b.a * b.a) * rp3;
+    const D<N> v[6] = {fast_interp<N>(l, ABCMB_TAU_TAB_LO, 0.0, c.tau_tab, P.dtau(), ABCMB_TAU_TAB_N), g, gp, b.aH,
+                       em, ahp};
+#pragma unroll
+    for (int f = 0; f < 6; ++f) {
+      out[((size_t)ci * 6 + f) * n + j] = v[f].v;
+      if constexpr (N > 0) { ABCMB_DLOOP(N) dout[(((size_t)ci * N + i_) * 6 + f) * n + j] = v[f].d[i_]; }
+    }
+  }
+}
+
+}  // namespace
+}  // namespace abcmb
+
+extern "C" int abcmb_los_background(const AbcmbCosmo* cosmos_dev, const AbcmbTangent* tangents_dev, int n_cosmo,
+                                    const double* lna_dev, int n, double* out_dev, double* dout_dev, void* stream) {
+  using namespace abcmb;
+  ABCMB_REQUIRE(cosmos_dev && lna_dev && out_dev, "null argument");
+  ABCMB_REQUIRE(n_cosmo > 0 && n > 0, "bad sizes");
+  ABCMB_REQUIRE(!tangents_dev || dout_dev, "tangents need dout");
+  hipStream_t s = (hipStream_t)stream;
+  if (tangents_dev)
+    hipLaunchKernelGGL(losbg_kernel<ABCMB_TANGENT_DIRS>, dim3(n_cosmo), dim3(LOSBG_THREADS), 0, s, cosmos_dev,
+                       tangents_dev, lna_dev, n, out_dev, dout_dev);
+  else
+    hipLaunchKernelGGL(losbg_kernel<0>, dim3(n_cosmo), dim3(LOSBG_THREADS), 0, s, cosmos_dev, nullptr, lna_dev, n,
+                       out_dev, nullptr);
+  ABCMB_HIP_CHECK(hipGetLastError());
+  return ABCMB_OK;
+}
+
 // ---------------------------------------------------------------- launch without allocation
 // A fixed scratch array per device for callers that cannot allocate (graph
 // capture, tight serving loops): it holds what abcmb_pert_fwd's workspace

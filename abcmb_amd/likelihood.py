@@ -36,6 +36,8 @@ import math
 
 import numpy as np
 
+from . import abi
+
 NAMES = ("omega_b", "omega_cdm", "h", "ln10As", "n_s", "tau_reion", "m_nu")
 RANGES = np.array([(0.020, 0.024), (0.10, 0.14), (0.62, 0.74), (2.9, 3.2), (0.92, 1.0), (0.03, 0.09), (0.06, 0.3)])
 FIDUCIAL = np.array([0.02237, 0.1200, 0.6736, math.log(1e10 * 2.1e-9), 0.9649, 0.0544, 0.06])
@@ -152,7 +154,15 @@ def _tangent_inputs(model, thetas, k_grids, steps):
     nd = len(ODE_DIRS)
     pm = [pm[c * 2 * nd:(c + 1) * 2 * nd] for c in range(nc)]
     seeds = [tangent_seeds(pm[c][0::2], pm[c][1::2], [steps[c, p] for p in ODE_DIRS]) for c in range(nc)]
+    if getattr(model, "device_background", False):
+        # recombination-only tables: the GPU adds the reionization, its
+        # implicit-function derivatives replace the x_e and z_reion seeds
+        dtarget = np.zeros((nc, abi.N_TANGENT_DIRS))
+        dtarget[:, ODE_DIRS.index(NAMES.index("tau_reion"))] = 1.0
+        model.complete_backgrounds(base_plans, seeds, dtarget)
+        model.complete_backgrounds([q for row in pm for q in row])
     return base_plans, pm, seeds
+
 
 
 def _directional_losses(model, like, thetas, steps, base_plans, pm, pts_b, kdev_b, dpts, mock_index):

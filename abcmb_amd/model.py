@@ -155,7 +155,7 @@ class Model:
     """GPU drop-in for ``abcmb.Model`` on the perturbation + LOS path."""
 
     def __init__(self, user_species=None, los_precision: str = "fp64", lens_mode: str = "vector",
-                 lens_precision: str = "fp64", **kwargs):
+                 lens_precision: str = "fp64", device_background: bool = False, **kwargs):
         kwargs.pop("adjoint", None)  # reference option for diffrax AD; not used here
         specs = S.load_specs(kwargs)
         self.specs = specs
@@ -218,6 +218,10 @@ class Model:
             raise ValueError("the vector lensing path is FP64 only")
         self.lens_mode = lens_mode
         self.lens_precision = precisions[lens_precision]
+        # device_background: the host producer stops after recombination; the
+        # reionization root (abcmb_reionize) and the LOS background vectors with the
+        # optical depth (abcmb_los_background) run on the GPU before the ODE launch
+        self.device_background = bool(device_background)
         self.chunk = 128   # cosmologies per downstream launch (bounds the source buffers)
         self.timing = None  # list: per-launch pert_fwd event timing and work counts when set
         self.call_timing = None  # dict: entry-point name -> [(start event, end event, work)] when set
@@ -241,7 +245,8 @@ class Model:
                         input_tau_reion=sp["input_tau_reion"],
                         rec_switch=freeze.bg.rec_switch if freeze is not None else -1,
                         lna_rec=freeze.bg.lna_rec if freeze is not None else None,
-                        lna_transfer_start=freeze.bg.lna_transfer_start if freeze is not None else None)
+                        lna_transfer_start=freeze.bg.lna_transfer_start if freeze is not None else None,
+                        reionization=not self.device_background)
         if freeze is not None:
             bg.cosmo.lna_rec = freeze.bg.lna_rec
             bg.cosmo.lna_transfer_start = freeze.bg.lna_transfer_start
@@ -253,7 +258,7 @@ class Model:
         big = kk > sp["k_split_PE"]
         rtol = np.where(big, sp["rtol_large_k_PE"], sp["rtol_small_k_PE"])
         atol = np.where(big, sp["atol_large_k_PE"], sp["atol_small_k_PE"])
-        bgv = bg.los_vectors(lna_save[:-1])
+        bgv = None if self.device_background else bg.los_vectors(lna_save[:-1])
         plan = CosmoPlan(full_params, bg, lna_save, ls, rtol, atol, bgv, k=kk)
         if sp["lensing"]:
             lens_lna = np.linspace(bg.lna_rec, 0.0, 500)
@@ -310,6 +315,120 @@ class Model:
             D[key] = dm
         return D[key]
 
+    def complete_backgrounds(self, plans, seeds=None, dtarget=None):
+        """Finish recombination-only plans (``device_background``) on the GPU.
+
+        abcmb_reionize solves each plan's reionization redshift for its
+        tau_reion and adds the reionization to the x_e table; abcmb_los_background
+        then evaluates the LOS background vectors (tau, g, g', aH, exp(-kappa),
+        aH' aH) on the plan's save grid from the finished table.  Both run on a
+        stream of their own, and the results land in the plans' host
+        backgrounds (x_e table, z_reion, x_e today) and ``bgv``, as the host
+        producer would have left them.  ``seeds`` (one ``TangentSeeds`` per plan,
+        of recombination-only tables) and ``dtarget`` ([n][N_TANGENT_DIRS]:
+        d tau_reion along each direction) add the implicit-function derivative
+        of z_reion and the tangents of the finished table to the seeds.
+        Returns the LOS vectors' tangents [n][N_TANGENT_DIRS][6][n_save-1] with
+        seeds, else None.
+        """
+        if not plans:
+            return None
+        torch = _torch()
+        lib = abi.hip_lib()
+        self._device_constants()
+        dev = torch.device("cuda")
+        n, nd = len(plans), abi.N_TANGENT_DIRS
+        n_rec = abi.host_lib().abcmb_host_recomb_points()
+        xe_n = plans[0].bg.xe_tab.size
+        if any(pl.bg.xe_tab.size != xe_n for pl in plans):
+            raise ValueError("complete_backgrounds: the plans' x_e tables differ in length")
+        nl = self.n_save - 1
+        f_xe_last = abi.AbcmbCosmo.xe_last.offset // 8
+        f_z = abi.AbcmbCosmo.z_reion.offset // 8
+        side = torch.cuda.Stream()
+        with torch.cuda.stream(side):
+            stage = _Stage(torch, dev)
+            dt = stage.add
+            stream = side.cuda_stream
+            xe_out = torch.empty((n, xe_n), dtype=torch.float64, device=dev)
+            z_out = torch.empty(n, dtype=torch.float64, device=dev)
+            status = torch.empty(n, dtype=torch.int32, device=dev)
+            bgv = torch.empty((n, 6, nl), dtype=torch.float64, device=dev)
+            rec, fin = [], []
+            for i, pl in enumerate(plans):
+                c = abi.AbcmbCosmo.from_buffer_copy(pl.bg.cosmo)
+                c.tau_tab = dt(pl.bg.tau_tab).data_ptr()
+                c.xe_tab = dt(pl.bg.xe_tab).data_ptr()
+                c.tm_tab = dt(pl.bg.tm_tab).data_ptr()
+                rec.append(c)
+                f = abi.AbcmbCosmo.from_buffer_copy(c)
+                f.xe_tab = xe_out[i].data_ptr()
+                fin.append(f)
+            rec_dev = self._struct_array(torch, dev, rec, dt)
+            fin_dev = self._struct_array(torch, dev, fin, dt)
+            reion_dev = self._struct_array(torch, dev, [pl.bg.reion for pl in plans], dt)
+            lna = dt(np.stack([pl.lna_save[:-1] for pl in plans]))
+            tan_rec = tan_fin = None
+            if seeds is not None:
+                if dtarget is None or len(seeds) != n:
+                    raise ValueError("complete_backgrounds: seeds need one entry per plan and dtarget")
+                par = dt(np.stack([sd.par for sd in seeds]))
+                par_fin = torch.empty_like(par)
+                dxe_out = torch.empty((n, nd, xe_n), dtype=torch.float64, device=dev)
+                dz_out = torch.empty((n, nd), dtype=torch.float64, device=dev)
+                dbgv = torch.empty((n, nd, 6, nl), dtype=torch.float64, device=dev)
+                dtar = dt(np.asarray(dtarget, dtype=np.float64).reshape(n, nd))
+                tr, tf = [], []
+                for i, sd in enumerate(seeds):
+                    tau_s, tm_s = dt(sd.tau), dt(sd.tm)
+                    tr.append(abi.AbcmbTangent(par=par[i].data_ptr(), tau_tab=tau_s.data_ptr(),
+                                               xe_tab=dt(sd.xe).data_ptr(), tm_tab=tm_s.data_ptr(), pt=None))
+                    tf.append(abi.AbcmbTangent(par=par_fin[i].data_ptr(), tau_tab=tau_s.data_ptr(),
+                                               xe_tab=dxe_out[i].data_ptr(), tm_tab=tm_s.data_ptr(), pt=None))
+                tan_rec = self._struct_array(torch, dev, tr, dt)
+                tan_fin = self._struct_array(torch, dev, tf, dt)
+            stage.flush()
+            abi.check(lib.abcmb_reionize(rec_dev.data_ptr(), tan_rec.data_ptr() if seeds is not None else None, n,
+                                         reion_dev.data_ptr(), dtar.data_ptr() if seeds is not None else None, n_rec,
+                                         xe_out.data_ptr(), dxe_out.data_ptr() if seeds is not None else None,
+                                         z_out.data_ptr(), dz_out.data_ptr() if seeds is not None else None,
+                                         status.data_ptr(), stream), "abcmb_reionize")
+            # the finished structs take z_reion and x_e today from the device results
+            fv = fin_dev.view(torch.float64).view(n, -1)
+            fv[:, f_z] = z_out
+            fv[:, f_xe_last] = xe_out[:, n_rec - 1]
+            if seeds is not None:
+                par_fin.copy_(par)
+                par_fin[:, :, f_z] = dz_out
+                par_fin[:, :, f_xe_last] = dxe_out[:, :, n_rec - 1]
+            abi.check(lib.abcmb_los_background(fin_dev.data_ptr(), tan_fin.data_ptr() if seeds is not None else None,
+                                               n, lna.data_ptr(), nl, bgv.data_ptr(),
+                                               dbgv.data_ptr() if seeds is not None else None, stream),
+                      "abcmb_los_background")
+            st = status.cpu().numpy()
+            if np.any(st != 0):
+                raise RuntimeError(f"abcmb_reionize: no reionization redshift in [0, 50] for plans "
+                                   f"{np.nonzero(st)[0][:8].tolist()}")
+            xe_h, z_h, bgv_h = xe_out.cpu().numpy(), z_out.cpu().numpy(), bgv.cpu().numpy()
+            if seeds is not None:
+                dxe_h, dz_h, dbgv_h = dxe_out.cpu().numpy(), dz_out.cpu().numpy(), dbgv.cpu().numpy()
+        for i, pl in enumerate(plans):
+            pl.bg.xe_tab[:] = xe_h[i]
+            pl.bg.cosmo.z_reion = float(z_h[i])
+            pl.bg.cosmo.xe_last = float(xe_h[i, n_rec - 1])
+            pl.bgv = bgv_h[i]
+            if seeds is not None:
+                sd = seeds[i]
+                sd.xe[:] = dxe_h[i]
+                sd.par[:, f_z] = dz_h[i]
+                sd.par[:, f_xe_last] = dxe_h[i, :, n_rec - 1]
+        return dbgv_h if seeds is not None else None
+
+    def _complete_pending(self, plans):
+        todo = [pl for pl in plans if pl.bgv is None]
+        if todo:
+            self.complete_backgrounds(todo)
+
     def run_batch(self, params_list, derived: bool = False):
         """Outputs for several cosmologies; one ODE launch covers all their k-modes."""
         plist = [p if derived else self.derived(p) for p in params_list]
@@ -353,6 +472,7 @@ class Model:
         """Enqueue the ODE launch of ``_run_ode`` on the current stream and return
         a handle for ``_finish_ode``; the handle keeps every buffer the launch
         reads or writes alive."""
+        self._complete_pending(plans)
         torch = _torch()
         lib = abi.hip_lib()
         D = self._device_constants()

@@ -382,6 +382,19 @@ int abcmb_reionize(const AbcmbCosmo* cosmos_dev, const AbcmbTangent* tangents_de
                    double* xe_out_dev, double* dxe_out_dev, double* z_out_dev, double* dz_out_dev,
                    int32_t* status_dev, void* stream);
 
+/*
+ * The six LOS background vectors on the GPU (what abcmb_host_los_background
+ * returns; abcmb/spectrum.py:640-651 with kappa by abcmb/background.py:682-742):
+ * out_dev [n_cosmo][6][n] = tau, g, dg/dlna, aH, exp(-kappa), aH' * aH on each
+ * cosmology's grid lna_dev [n_cosmo][n], kappa(lna) = int_lna^0 dl / (tau_c aH)
+ * by 4-point Gauss-Legendre on the 20000 panels of [-10, 0].  With
+ * tangents_dev, also their derivatives dout_dev
+ * [n_cosmo][ABCMB_TANGENT_DIRS][6][n].  The x_e table may be the output of
+ * abcmb_reionize (a cosmology copied with xe_tab, xe_last and z_reion set).
+ */
+int abcmb_los_background(const AbcmbCosmo* cosmos_dev, const AbcmbTangent* tangents_dev, int n_cosmo,
+                         const double* lna_dev, int n, double* out_dev, double* dout_dev, void* stream);
+
 /* Linear matter power spectrum at z = 0 (abcmb/spectrum.py:268-302):
  * pk_dev out [B][nq] on the query grid kq_dev [nq]. */
 int abcmb_pk_lin(const AbcmbCosmo* cosmos_dev, int n_cosmo, const double* kq_dev, int nq,
