@@ -1,12 +1,16 @@
 """Benchmark: gradient evaluations per second of the 7-parameter lensed-C_l
 likelihood (l <= 2500) on one MI355X.
 
-Workload (batched gradients): ``--batch`` Latin-hypercube cosmologies
+Workload (batched gradients): ``--batch`` (64) Latin-hypercube cosmologies
 (omega_b, omega_cdm, h, ln 1e10 A_s, n_s, tau_reio, sum m_nu over the stated
 ranges, seed 0), each with its own uneven log-k grid of 300-800 modes in
 [7.05e-6, 0.62] Mpc^-1 (seed 0), one massive neutrino species (3 momentum
 bins x l <= 17, state size 100), lensed TT/TE/EE to l = 2500 in FP64 (ODE
-and LOS), Gaussian log-likelihood against a per-cosmology seeded mock.
+and LOS), Gaussian log-likelihood against a per-cosmology seeded mock.  The
+ODE runs at the reference's tolerances with the large-k ones tightened
+(``--tolerances accurate``, the default) so that the gradients are within
+1e-3 of their converged values; the same batch at the reference's own
+tolerances and a ``--batch-large`` (256) batch are timed once beside it.
 One step = the gradients of all ``--batch`` cosmologies: tangent-linear ODE
 launches of ``--chunk`` cosmologies each (perturbation tables and their
 derivatives along omega_b, omega_cdm, h, tau_reio, m_nu), each followed by
@@ -16,9 +20,11 @@ next chunk's host preparation overlapping the GPU work (``--gradient fd`` instea
 central finite differences, 15 full model evaluations per gradient, all in
 one ODE launch).
 
-Also reported: the plain path (one cosmology per launch, FP64, central
-finite differences) on a few of the same cosmologies and its agreement with
-the batched gradients, the roofline of the dominant kernel (the perturbation
+Also reported: the launch shapes the autotuner picked (untimed, at setup), the
+plain path (one cosmology per launch, FP64, central finite differences, the
+reference's tolerances) on a few of the same cosmologies, the gradient errors
+of the timed path, of the reference-tolerance path and of the plain path
+against converged gradients, the roofline of the dominant kernel (the perturbation
 ODE, FP64 FLOP/s against the FP64 peak, timed with HIP events on its stream),
 and the CPU oracle on a bounded sample of the same work.
 """
@@ -180,9 +186,17 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=1)
     ap.add_argument("--warmup", type=int, default=0)
-    ap.add_argument("--batch", type=int, default=256)
+    ap.add_argument("--batch", type=int, default=64,
+                    help="cosmologies per step (the gradient target's 64-cosmology batch)")
+    ap.add_argument("--batch-large", type=int, default=256,
+                    help="cosmologies of the larger batched-gradient workload timed once beside the headline "
+                         "(0: skip)")
+    ap.add_argument("--tolerances", choices=("accurate", "reference"), default="accurate",
+                    help="ODE tolerances of the timed path: the reference's, or the large-k ones tightened "
+                         "until the gradient is within 1e-3 of its converged value")
     ap.add_argument("--plain", type=int, default=2, help="cosmologies timed on the plain path")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--chunk", type=int, default=64, help="cosmologies per tangent-linear ODE launch")
     ap.add_argument("--gradient", choices=("tangent", "fd"), default="tangent",
                     help="batched gradient method (the plain path is always finite differences)")
@@ -201,26 +215,43 @@ def main():
                                    "is designed but not launched"}))
         sys.exit(2)
     import torch
+    from abcmb_amd import autotune
     from abcmb_amd import likelihood as LK
 
     t_setup = time.perf_counter()
-    model = build_model()
-    thetas = LK.latin_hypercube(args.batch, seed=0)
-    k_grids = LK.uneven_k_grids(args.batch, seed=0)
+    tol = TOL_ACCURATE if args.tolerances == "accurate" else None
+    model = build_model(tol)
+    mref = build_model() if tol else model          # the reference's tolerances
+    n_all = max(args.batch, args.batch_large)
+    thetas_all = LK.latin_hypercube(n_all, seed=0)
+    k_all = LK.uneven_k_grids(n_all, seed=0)
+    thetas, k_grids = thetas_all[:args.batch], k_all[:args.batch]
     # mock: C_l at the fiducial point, noise per cosmology slot
     fid = model.execute([model.prepare(model.derived(LK.theta_to_params(LK.FIDUCIAL)))])[0]
     like = LK.LensedClLikelihood(model, [fid.ClTT.cpu().numpy(), fid.ClTE.cpu().numpy(), fid.ClEE.cpu().numpy()],
-                                 n_mocks=args.batch, seed=1)
+                                 n_mocks=n_all, seed=1)
+    tuned = None
+    if not args.no_autotune and args.gradient == "tangent":
+        # launch shapes picked on a 16-cosmology sample of the workload (untimed)
+        ns = min(16, args.batch)
+        bp, _, seeds = LK._tangent_inputs(model, thetas[:ns], k_grids[:ns], LK.FD_REL_STEP * np.abs(thetas[:ns]))
+        tuned = autotune.tune(model, bp, seeds)
+        if mref is not model:
+            autotune.apply(mref, tuned)
+        del bp, seeds
+        print(f"# autotune {tuned}", file=sys.stderr, flush=True)
     print(f"# setup {time.perf_counter() - t_setup:.1f} s", file=sys.stderr, flush=True)
 
     phases = {}
-    if args.gradient == "tangent":
-        batched = lambda: LK.tangent_gradients(model, like, thetas, k_grids=k_grids, phases=phases, chunk=args.chunk,
-                                               downstream="fd" if args.fd_downstream else "tangent")
-    else:
-        batched = lambda: LK.fd_gradients(model, like, thetas, k_grids=k_grids, batched=True)
+
+    def grads(m, th, kg, ph=None, chunk=None):
+        if args.gradient == "tangent":
+            return LK.tangent_gradients(m, like, th, k_grids=kg, phases=ph, chunk=chunk or args.chunk,
+                                        downstream="fd" if args.fd_downstream else "tangent")
+        return LK.fd_gradients(m, like, th, k_grids=kg, batched=True)
+
     for w in range(args.warmup):
-        batched()
+        grads(model, thetas, k_grids)
         print(f"# warmup {w} done", file=sys.stderr, flush=True)
 
     model.timing = []
@@ -228,7 +259,7 @@ def main():
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for s in range(args.steps):
-        L, grad = batched()
+        L, grad = grads(model, thetas, k_grids, phases)
         print(f"# step {s} done", file=sys.stderr, flush=True)
     torch.cuda.synchronize()
     dt = time.perf_counter() - t0
@@ -237,13 +268,31 @@ def main():
     model.timing = model.call_timing = None
     value = args.batch * args.steps / dt
 
-    # plain path: one cosmology per launch, same FD gradient
-    t0 = time.perf_counter()
-    Lp, gp = LK.fd_gradients(model, like, thetas[:args.plain], k_grids=k_grids[:args.plain], batched=False)
-    torch.cuda.synchronize()
-    plain = args.plain / (time.perf_counter() - t0)
-    gb = grad[:args.plain]
+    def timed_once(m, n):
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        _, g = grads(m, thetas_all[:n], k_all[:n])
+        torch.cuda.synchronize()
+        return n / (time.perf_counter() - t), g
+
+    # the same batch at the reference's tolerances, and the larger batch (one step each)
+    ref_rate, g_ref = timed_once(mref, args.batch) if mref is not model else (value, grad)
+    print(f"# reference tolerances {ref_rate:.2f} grad-evals/s", file=sys.stderr, flush=True)
+    large = None
+    if args.batch_large:
+        large = {"cosmologies": args.batch_large, "steps": 1}
+        large["grad_evals_per_s"], _ = timed_once(model, args.batch_large)
+        if mref is not model:
+            large["reference_tolerances_grad_evals_per_s"], _ = timed_once(mref, args.batch_large)
+        print(f"# {args.batch_large} cosmologies {large}", file=sys.stderr, flush=True)
+
+    # plain path: one cosmology per launch, central finite differences, the reference's tolerances
     np_ = args.plain
+    t0 = time.perf_counter()
+    Lp, gp = LK.fd_gradients(mref, like, thetas[:np_], k_grids=k_grids[:np_], batched=False)
+    torch.cuda.synchronize()
+    plain = np_ / (time.perf_counter() - t0)
+    gb = grad[:np_]
 
     def err_to_norm(g, ref):
         """max over parameters of |g - ref| relative to ref's norm, each component in
@@ -251,13 +300,11 @@ def main():
         scale = np.linalg.norm(ref * thetas[:np_], axis=1, keepdims=True) / np.abs(thetas[:np_])
         return float(np.max(np.abs(g - ref) / scale))
 
-    agree = err_to_norm(gb, gp)
-    agree_rel = float(np.max(np.abs(gp - gb) / (np.abs(gp) + 1e-300)))
     print(f"# plain path {plain:.4f} grad-evals/s", file=sys.stderr, flush=True)
 
-    # accuracy of the timed path on the plain cosmologies: C_l and L against the plain
-    # path's forward evaluations, gradients against a converged FP64 reference (the
-    # tangent-linear path at 100x tighter ODE tolerances)
+    # accuracy on the plain cosmologies: C_l and L of the timed path against one-cosmology
+    # launches of the same model, gradients against a converged FP64 reference (the
+    # tangent-linear path at far tighter ODE tolerances)
     cl_b = []
     LK.tangent_gradients(model, like, thetas[:np_], k_grids=k_grids[:np_], chunk=np_, cl_out=cl_b)
     cl_b = torch.cat(cl_b).cpu().numpy()
@@ -266,26 +313,18 @@ def main():
     cl_p = np.stack([np.stack([o.ClTT.cpu().numpy(), o.ClTE.cpu().numpy(), o.ClEE.cpu().numpy()]) for o in outs_p])
     cl_diff_plain = max(max(cl_errors(cl_b[i], cl_p[i], model.ells.size)[x] for x in ("TT", "TE", "EE"))
                         for i in range(np_))
-    L_diff_plain = float(np.max(np.abs(L[:np_] - Lp) / np.abs(Lp)))
     mconv = build_model(TOL_CONVERGED)
+    if tuned:
+        autotune.apply(mconv, tuned)
     _, g_conv = LK.tangent_gradients(mconv, like, thetas[:np_], k_grids=k_grids[:np_], chunk=np_)
-    grad_err_conv = err_to_norm(gb, g_conv)
     del mconv
-    print(f"# gradient error vs converged {grad_err_conv:.2e}", file=sys.stderr, flush=True)
-    # the same workload at the tightened large-k tolerances: gradient rate on the
-    # first 64 cosmologies (one step of 64, after one warm-up chunk) and the error
-    # against the converged gradients
-    macc = build_model(TOL_ACCURATE)
-    LK.tangent_gradients(macc, like, thetas[:8], k_grids=k_grids[:8], chunk=8)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    _, g_acc = LK.tangent_gradients(macc, like, thetas[:64], k_grids=k_grids[:64], chunk=args.chunk)
-    torch.cuda.synchronize()
-    acc_rate = 64 / (time.perf_counter() - t0)
-    acc_err = err_to_norm(g_acc[:np_], g_conv)
-    del macc
-    print(f"# accurate tolerances: {acc_rate:.2f} grad-evals/s, error vs converged {acc_err:.2e}", file=sys.stderr,
-          flush=True)
+    grad_err_conv = err_to_norm(gb, g_conv)
+    ref_err_conv = err_to_norm(g_ref[:np_], g_conv)
+    plain_err_conv = err_to_norm(gp, g_conv)
+    agree = err_to_norm(gb, gp)
+    agree_rel = float(np.max(np.abs(gp - gb) / (np.abs(gp) + 1e-300)))
+    print(f"# gradient error vs converged: timed {grad_err_conv:.2e}, reference tolerances {ref_err_conv:.2e}, "
+          f"plain {plain_err_conv:.2e}", file=sys.stderr, flush=True)
 
     ms = sum(t["ms"] for t in timing) / len(timing)
     if args.gradient == "tangent":
@@ -300,17 +339,18 @@ def main():
     tfile = os.path.join(ROOT, "profiles", "pert_tangent_traffic.json")
     if args.gradient == "tangent" and os.path.exists(tfile):
         t = json.load(open(tfile))
-        traffic = t["fetch_bytes_per_launch"] + t["write_bytes_per_launch"]
+        if t.get("tolerances", "reference") == args.tolerances and t.get("cosmologies_per_launch") == args.chunk:
+            traffic = t["fetch_bytes_per_launch"] + t["write_bytes_per_launch"]
     # algorithmic HBM bytes of an ODE launch: the perturbation table it produces
     # (21 fields x saved points x k-modes, FP64, value and 5 tangents)
     ncomp = 1 + (5 if args.gradient == "tangent" else 0)
     table_bytes = sum(t["items"] for t in timing) / len(timing) * model.n_save * 21 * 8 * ncomp
     roof = {"bound": "fp64-valu", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
             "frac": achieved / FP64_PEAK_TFLOPS, "traffic": traffic, "kernel": kname,
-            "launch_ms": ms, "flops_per_launch": flops,
+            "launch_ms": ms, "flops_per_launch": flops, "launches_per_step": len(timing) / args.steps,
             "chord_iters_per_step": sum(t["iters"] for t in timing) / max(1, sum(t["steps"] for t in timing)),
-            "traffic_source": "profiles/pert_tangent_traffic.json (FETCH_SIZE x2.0 + WRITE_SIZE x1.0, "
-                              "scaled by a 1 GiB copy)" if traffic else None,
+            "traffic_source": "profiles/pert_tangent_traffic.json (FETCH_SIZE + WRITE_SIZE per launch of the same "
+                              "workload, scaled by a plain copy)" if traffic else None,
             "hbm": {"algorithmic_bytes_per_launch": table_bytes,
                     "algorithmic_GBps": table_bytes / (ms * 1e-3) / 1e9,
                     "counter_GBps": traffic / (ms * 1e-3) / 1e9 if traffic else None,
@@ -340,6 +380,8 @@ def main():
     # FP64 ODE + FP32 LOS): host preparation + one ODE launch + downstream
     from abcmb_amd.model import Model, MassiveNeutrino
     mf = Model(user_species=(MassiveNeutrino,), lensing=True, l_max=2500, max_steps_PE=4096, los_precision="fp32")
+    if tuned:
+        autotune.apply(mf, dict(tuned, lens_mode="vector"))
     nf = 64
     tf_thetas = LK.latin_hypercube(nf, seed=2)
     tf_k = LK.uneven_k_grids(nf, seed=2)
@@ -359,11 +401,12 @@ def main():
         cpu, cl_err = cpu_baseline(model, plan0, gl)
         print(f"# C_l error vs oracle {cl_err}", file=sys.stderr, flush=True)
     gates = {"grad_err_vs_converged<=1e-3": grad_err_conv <= 1e-3,
-             "accurate_tolerances_grad_err_vs_converged<=1e-3": acc_err <= 1e-3}
+             f"speedup_vs_plain>=10": value / plain >= 10.0}
     if cl_err is not None:
         gates["cl_err_vs_oracle TT/TE/EE<=1e-4"] = max(cl_err["TT"], cl_err["TE"], cl_err["EE"]) <= 1e-4
         gates["cl_err_vs_oracle BB<=1e-3"] = cl_err["BB"] <= 1e-3
     gates["cl_diff_vs_plain<=1e-4"] = cl_diff_plain <= 1e-4
+    tol_used = dict(TOL_ACCURATE) if tol else {}
 
     out = {
         "metric": METRIC, "value": value, "unit": "grad-evals/s", "n_gpus": 1, "steps": args.steps,
@@ -371,27 +414,34 @@ def main():
         "vs_baseline": None, "dtype": "f64", "data": "synthetic",
         "config": {"workload": f"batched lensed-Cl likelihood gradients, {args.batch} Latin-hypercube cosmologies "
                                f"(massive nu, uneven 300-800 k-grids, l<=2500, FP64 ODE + FP64 LOS)",
+                   "ode_tolerances": ("reference's (rtol 1e-5 / 1e-4, atol 1e-10 / 1e-6 below / above k=0.01)"
+                                      if not tol else f"reference's with the large-k ones tightened to {tol_used}"),
                    "gradient": (f"tangent-linear ODE (5 directions, one launch per chunk: {args.chunk // 8}, "
                                 f"{args.chunk // 4}, {args.chunk // 2}, then {args.chunk} cosmologies) + "
                                 + ("downstream differenced along 7 directions" if args.fd_downstream
                                    else "tangent-linear LOS, lensing differenced along 7 directions")
                                 if args.gradient == "tangent" else
                                 "central finite differences, 15 evaluations, grids frozen, all in one ODE launch"),
-                   "plain_path": "one cosmology per launch, FP64, central finite differences (15 evaluations)",
+                   "autotuned": tuned,
+                   "plain_path": "one cosmology per launch, FP64, central finite differences (15 evaluations), "
+                                 "the reference's ODE tolerances",
                    "plain_path_grad_evals_per_s": plain, "speedup_vs_plain": value / plain,
                    "grad_diff_vs_plain_rel_to_norm": agree, "grad_diff_vs_plain_max_rel": agree_rel,
                    "grad_err_vs_converged": grad_err_conv,
-                   "accurate_tolerances": {"ode_tolerances": TOL_ACCURATE, "grad_evals_per_s": acc_rate,
-                                           "workload": "the first 64 cosmologies of the same set, one step",
-                                           "grad_err_vs_converged": acc_err},
-                   "grad_err_reference": "tangent-linear gradients of the same cosmologies at 100x tighter ODE "
-                                         "tolerances (max error over parameters relative to the gradient norm, "
-                                         "each component in units of its parameter)",
+                   "plain_path_grad_err_vs_converged": plain_err_conv,
+                   "reference_tolerances": {"grad_evals_per_s": ref_rate, "grad_err_vs_converged": ref_err_conv,
+                                            "workload": f"the same {args.batch} cosmologies, one step"},
+                   "large_batch": large,
+                   "grad_err_reference": "tangent-linear gradients of the first plain-path cosmologies with the "
+                                         "large-k ODE tolerances / 300 and the small-k ones / 30 (max error over "
+                                         "parameters relative to the gradient norm, each component in units of "
+                                         "its parameter)",
                    "cl_err_vs_oracle": cl_err,
                    "cl_err_reference": "lensed C_l (l=2..2500) of cosmology 0 on the GPU path vs the complete "
                                        "FP64 oracle evaluation of the same inputs; TT/EE/BB relative per l, TE "
                                        "relative to max|TE|",
-                   "cl_diff_vs_plain": cl_diff_plain, "L_diff_vs_plain": L_diff_plain,
+                   "cl_diff_vs_plain": cl_diff_plain, "L_diff_vs_plain": float(np.max(np.abs(L[:np_] - Lp)
+                                                                                       / np.abs(Lp))),
                    "gates": gates,
                    "forward_spectra_per_s": spectra,
                    "forward_workload": "64 Latin-hypercube cosmologies (seed 2), own 300-800 k grids, lensed "
