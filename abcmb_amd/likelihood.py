@@ -207,14 +207,17 @@ def _directional_losses(model, like, thetas, steps, base_plans, pm, pts_b, kdev_
 
 
 def _directional_losses_linear(model, like, thetas, steps, base_plans, pm, pts, kdev, dpts, mock_index,
-                               cl_out=None):
+                               cl_out=None, jac_out=None):
     """L at theta ([nc]) and at theta +- step along all seven directions ([nc][7][2]),
     on the GPU, with the unlensed C_l of theta and their derivatives from the
     tangent-linear LOS (``Model.downstream_tangent``): the base C_l are its
     value, the moved C_l are C_l +- h dC_l; only C_l^phiphi and the lensing are
     evaluated at each point (the base one with theta's tables and plan).
     ``cl_out`` (a list, optional) receives the observed C_l at theta
-    ([n][3][n_l] per group of cosmologies)."""
+    ([n][3][n_l] per group of cosmologies), ``jac_out`` their derivatives along
+    the seven parameters ([n][7][3][n_l]: central differences of the lensed
+    spectra between C_l +- h dC_l, the unlensed spectra moved along their exact
+    derivatives)."""
     import torch
     nc = thetas.shape[0]
     nd = len(ODE_DIRS)
@@ -275,13 +278,16 @@ def _directional_losses_linear(model, like, thetas, steps, base_plans, pm, pts, 
         L.append(like.loglike_cl(obs, slots).reshape(n, per))
         if cl_out is not None:
             cl_out.append(obs.reshape(n, per, 3, -1)[:, 0])
+        if jac_out is not None:
+            ob = obs.reshape(n, per, 3, -1)
+            jac_out.append(torch.sub(ob[:, 1::2], ob[:, 2::2]).div_(2.0 * H_d[c0:c0 + n, :, 0, None, None]))
         del vpts, moved
     L = torch.cat(L)
     return L[:, 0], L[:, 1:].reshape(nc, 7, 2)
 
 
 def tangent_gradients(model, like, thetas, k_grids=None, mock_index=None, phases=None, chunk: int = 64,
-                      downstream: str = "tangent", cl_out=None):
+                      downstream: str = "tangent", cl_out=None, jac_out=None):
     """d L / d theta for each row of ``thetas`` from tangent-linear ODE launches.
 
     The cosmologies go through in chunks of ``chunk`` (one ODE launch each,
@@ -290,7 +296,9 @@ def tangent_gradients(model, like, thetas, k_grids=None, mock_index=None, phases
     main thread enqueues the current chunk's ODE launch and downstream on the
     current stream without waiting for the GPU, so the launches run back to
     back.  ``cl_out`` (a list, optional) receives the observed C_l at each
-    theta (tensors of [n][3][n_l], in order).  ``downstream``: "tangent"
+    theta (tensors of [n][3][n_l], in order), ``jac_out`` (with the default
+    downstream) their derivatives along the seven parameters ([n][7][3][n_l]).
+    ``downstream``: "tangent"
     (default) differentiates the LOS in one tangent-linear pass (the Bessel
     lookups as interpolated segments and asymptotic forms, as automatic
     differentiation of the reference would) and differences only the
@@ -302,6 +310,8 @@ def tangent_gradients(model, like, thetas, k_grids=None, mock_index=None, phases
     """
     import time
     from concurrent.futures import ThreadPoolExecutor
+    if jac_out is not None and downstream != "tangent":
+        raise ValueError("jac_out needs the tangent-linear downstream")
     t_start = time.perf_counter()
     thetas = np.atleast_2d(thetas)
     nc = thetas.shape[0]
@@ -351,7 +361,7 @@ def tangent_gradients(model, like, thetas, k_grids=None, mock_index=None, phases
             if downstream == "tangent":
                 # the tangent-linear LOS also gives theta's own C_l: L(theta) is
                 # one more lensing variant, no separate base downstream
-                lb, lp = _directional_losses_linear(*args, cl_out=cl_out)
+                lb, lp = _directional_losses_linear(*args, cl_out=cl_out, jac_out=jac_out)
             else:
                 ds = model._downstream(base_plans, run["pts"], run["kdev"])
                 obs = model.observed_cl(ds)
