@@ -139,7 +139,7 @@ def hip_lib():
         lib.abcmb_los_tangent_workspace_size.restype = C.c_size_t
         lib.abcmb_los_tangent_workspace_size.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int]
         lib.abcmb_los_tangent.argtypes = [_P, C.c_int, _P, _P, _P, C.c_int, C.c_int, _P, _P, C.c_int, _P, _P, _P,
-                                          C.c_int, C.c_int, _P, C.c_int, _P, _P, C.c_size_t, _P]
+                                          C.c_int, C.c_int, _P, C.c_int, C.c_int, _P, _P, C.c_size_t, _P]
         lib.abcmb_apply_operator.argtypes = [_P, C.c_int, C.c_int, _P, C.c_int, _P, _P]
         lib.abcmb_lensing_clpp.argtypes = [_P, C.c_int, _P, C.c_int, _P, C.c_int, _P, _P]
         lib.abcmb_lens_workspace_size.restype = C.c_size_t

@@ -301,41 +301,50 @@ __global__ void __launch_bounds__(NTH) los_cl_kernel(const AbcmbCosmo* __restric
 //
 // j_l(x) of the asymptotic form and its x-derivative: with nu = l + 1/2,
 // r = sqrt(x^2 - nu^2), Q = r - nu pi/2 + nu asin(nu/x): dQ/dx = r/x, dr/dx = x/r
-__device__ __forceinline__ void j_asym_d(double l, double x, double& j, double& dj) {
-  const double pi = PI;
-  constexpr double SQRT_2_PI = 0.79788456080286535588;   // sqrt(2 / pi)
-  const double nu = l + 0.5;
-  const double ix = 1.0 / x;
-  const double r = sqrt(x * x - nu * nu);
-  const double ir = 1.0 / r;
-  const double isr = 1.0 / sqrt(r);
-  const double Q = r - nu * pi / 2.0 + nu * asin(nu * ix);
-  double c, sn;
-  sincos(Q - pi / 4.0, &sn, &c);   // one argument reduction for both
-  const double J = SQRT_2_PI * isr * c;
-  const double pre = sqrt(pi / 2.0 * ix);
+__device__ __forceinline__ void sincos_t(double a, double* sn, double* c) { sincos(a, sn, c); }
+__device__ __forceinline__ void sincos_t(float a, float* sn, float* c) { sincosf(a, sn, c); }
+
+template <typename T>
+__device__ __forceinline__ void j_asym_d(T l, T x, T& j, T& dj) {
+  const T pi = (T)PI;
+  constexpr T SQRT_2_PI = (T)0.79788456080286535588;   // sqrt(2 / pi)
+  const T nu = l + (T)0.5;
+  const T ix = (T)1 / x;
+  const T r = sqrt(x * x - nu * nu);
+  const T ir = (T)1 / r;
+  const T isr = (T)1 / sqrt(r);
+  const T Q = r - nu * pi / (T)2 + nu * asin(nu * ix);
+  T c, sn;
+  sincos_t(Q - pi / (T)4, &sn, &c);   // one argument reduction for both
+  const T J = SQRT_2_PI * isr * c;
+  const T pre = sqrt(pi / (T)2 * ix);
   j = pre * J;
   // dJ/dx = sqrt(2/pi) (-c r^(-3/2) / 2 dr/dx - sin(.) r^(-1/2) dQ/dx)
-  const double dJ = SQRT_2_PI * isr * (-0.5 * ir * (x * ir) * c - sn * (r * ix));
-  dj = -0.5 * ix * j + pre * dJ;
+  const T dJ = SQRT_2_PI * isr * ((T)-0.5 * ir * (x * ir) * c - sn * (r * ix));
+  dj = (T)-0.5 * ix * j + pre * dJ;
 }
 
 // table value and slope of the interpolated segment (zero where the index is clipped)
-__device__ __forceinline__ void table_d(double x, double lo, double hi, const double* __restrict__ f, int n,
-                                        double& v, double& dv) {
-  double i0 = (x - lo) / (hi - lo) * (n - 1);
-  const double eps = 1e-6;
-  double i = fmin(fmax(i0, eps), n - 1.0 - eps);
+template <typename T>
+__device__ __forceinline__ void table_d(T x, T lo, T hi, const double* __restrict__ f, int n, T& v, T& dv) {
+  T i0 = (x - lo) / (hi - lo) * (T)(n - 1);
+  const T eps = (T)1e-6;
+  T i = fmin(fmax(i0, eps), (T)n - (T)1 - eps);
   int il = (int)floor(i);
   int iu = min(il + 1, n - 1);
-  double wu = i - il;
-  v = (1.0 - wu) * f[il] + wu * f[iu];
-  dv = (i == i0) ? (f[iu] - f[il]) * (n - 1) / (hi - lo) : 0.0;
+  T wu = i - (T)il;
+  const T fl = (T)f[il], fu = (T)f[iu];
+  v = ((T)1 - wu) * fl + wu * fu;
+  dv = (i == i0) ? (fu - fl) * (T)(n - 1) / (hi - lo) : (T)0;
 }
 
 // one block per (k chunk, block of LB sampled l, cosmology); a thread owns one
-// k and LB multipoles.  part: [B][NS][n_ell][n_chunk] with NS = 3 (1 + ND + 2)
-template <int NTH, int LB, int ND>
+// k and LB multipoles.  part: [B][NS][n_ell][n_chunk] with NS = 3 (1 + ND + 2).
+// T = double, or float with the Bessel arguments, lookups and asymptotic forms in
+// FP32 and the contraction over tau accumulated in FP32 (BF16: its operands, the
+// sources, their derivatives and the Bessel values and slopes, rounded to
+// bfloat16 first), as the value-only kernel's reduced-precision variants
+template <typename T, bool BF16, int NTH, int LB, int ND>
 __global__ void __launch_bounds__(NTH) __attribute__((amdgpu_waves_per_eu(2))) los_tangent_kernel(
     const AbcmbCosmo* __restrict__ cosmos, const double* __restrict__ src_all,
     const double* __restrict__ dsrc_all, const double* __restrict__ dchi_all, int n_tau,
@@ -358,76 +367,80 @@ __global__ void __launch_bounds__(NTH) __attribute__((amdgpu_waves_per_eu(2))) l
   const int nch = gridDim.x;
   double* part = part_all + (size_t)ci * NS * n_ell * nch;
 
-  double lv[LB], eps_l[LB];
+  T lv[LB], eps_l[LB];
   const double* tb[LB][3];
-  double lo[LB][3], hi[LB][3];
+  T lo[LB][3], hi[LB][3];
 #pragma unroll
   for (int b = 0; b < LB; ++b) {
     const int li = ell_idx[min(e0 + b, n_ell - 1)];
-    const double l = bes_l[li];
+    const T l = (T)bes_l[li];
     lv[b] = l;
-    eps_l[b] = sqrt(3.0 / 8.0 * (l + 2.0) * (l + 1.0) * l * (l - 1.0));
+    eps_l[b] = sqrt((T)3 / (T)8 * (l + (T)2) * (l + (T)1) * l * (l - (T)1));
 #pragma unroll
     for (int w = 0; w < 3; ++w) {
       tb[b][w] = bes + ((size_t)w * n_lt + li) * nx;
-      lo[b][w] = rng[((size_t)w * n_lt + li) * 2];
-      hi[b][w] = rng[((size_t)w * n_lt + li) * 2 + 1];
+      lo[b][w] = (T)rng[((size_t)w * n_lt + li) * 2];
+      hi[b][w] = (T)rng[((size_t)w * n_lt + li) * 2 + 1];
     }
   }
-  double T[LB], E[LB], Td[LB][ND], Ed[LB][ND];
+  T Tv[LB], E[LB], Td[LB][ND], Ed[LB][ND];
 #pragma unroll
   for (int b = 0; b < LB; ++b) {
-    T[b] = E[b] = 0.0;
+    Tv[b] = E[b] = (T)0;
 #pragma unroll
-    for (int d = 0; d < ND; ++d) Td[b][d] = Ed[b][d] = 0.0;
+    for (int d = 0; d < ND; ++d) Td[b][d] = Ed[b][d] = (T)0;
   }
   const double kk = j < nkt ? kt[j] : kt[nkt - 1];
+  const T kkt = (T)kk;
   const int jj = j < nkt ? j : nkt - 1;
   for (int s = 0; s < n_tau; ++s) {
     const size_t o = (size_t)s * nkt + jj;
-    const double x = (tau0 - tau_los[s]) * kk;
-    double S[4], dS[ND][4], dx[ND];
+    const T x = (T)(tau0 - tau_los[s]) * kkt;
+    T S[4], dS[ND][4], dx[ND];
 #pragma unroll
-    for (int q = 0; q < 4; ++q) S[q] = src[q * plane + o];
+    for (int q = 0; q < 4; ++q) S[q] = operand<BF16>((T)src[q * plane + o]);
 #pragma unroll
     for (int d = 0; d < ND; ++d) {
-      dx[d] = kk * dchi[d * n_tau + s];
+      dx[d] = operand<BF16>((T)(kk * dchi[d * n_tau + s]));
 #pragma unroll
-      for (int q = 0; q < 4; ++q) dS[d][q] = dsrc[((size_t)d * 4 + q) * plane + o];
+      for (int q = 0; q < 4; ++q) dS[d][q] = operand<BF16>((T)dsrc[((size_t)d * 4 + q) * plane + o]);
     }
 #pragma unroll
     for (int b = 0; b < LB; ++b) {
-      const double l = lv[b];
+      const T l = lv[b];
       const bool a0 = x >= hi[b][0], a1 = x >= hi[b][1], a2 = x >= hi[b][2];
-      double jl = 0.0, djl = 0.0, jl1 = 0.0, djl1 = 0.0;
-      if (a0 || a1 || a2) j_asym_d(l, x, jl, djl);
-      if (a1 || a2) j_asym_d(l + 1.0, x, jl1, djl1);
-      double p0 = 0.0, q0 = 0.0, p1 = 0.0, q1 = 0.0, p2 = 0.0, q2 = 0.0;  // phi_w and d phi_w / dx
+      T jl = 0, djl = 0, jl1 = 0, djl1 = 0;
+      if (a0 || a1 || a2) j_asym_d<T>(l, x, jl, djl);
+      if (a1 || a2) j_asym_d<T>(l + (T)1, x, jl1, djl1);
+      T p0 = 0, q0 = 0, p1 = 0, q1 = 0, p2 = 0, q2 = 0;  // phi_w and d phi_w / dx
       if (x >= lo[b][0]) {
         if (a0) { p0 = jl; q0 = djl; }
-        else table_d(x, lo[b][0], hi[b][0], tb[b][0], nx, p0, q0);
+        else table_d<T>(x, lo[b][0], hi[b][0], tb[b][0], nx, p0, q0);
       }
       if (x >= lo[b][1]) {
         if (a1) { p1 = l / x * jl - jl1; q1 = -l / (x * x) * jl + l / x * djl - djl1; }
-        else table_d(x, lo[b][1], hi[b][1], tb[b][1], nx, p1, q1);
+        else table_d<T>(x, lo[b][1], hi[b][1], tb[b][1], nx, p1, q1);
       }
       if (x >= lo[b][2]) {
         if (a2) {
-          const double c3 = 3.0 * l * (l - 1.0) - 2.0 * x * x;
-          const double Nn = c3 * jl + 6.0 * x * jl1;
-          const double dN = -4.0 * x * jl + c3 * djl + 6.0 * jl1 + 6.0 * x * djl1;
-          p2 = Nn / 2.0 / (x * x);
-          q2 = dN / (2.0 * x * x) - Nn / (x * x * x);
+          const T c3 = (T)3 * l * (l - (T)1) - (T)2 * x * x;
+          const T Nn = c3 * jl + (T)6 * x * jl1;
+          const T dN = (T)-4 * x * jl + c3 * djl + (T)6 * jl1 + (T)6 * x * djl1;
+          p2 = Nn / (T)2 / (x * x);
+          q2 = dN / ((T)2 * x * x) - Nn / (x * x * x);
         } else {
-          table_d(x, lo[b][2], hi[b][2], tb[b][2], nx, p2, q2);
+          table_d<T>(x, lo[b][2], hi[b][2], tb[b][2], nx, p2, q2);
         }
       }
-      const double psi = p0 / (x * x) * eps_l[b];
-      const double dpsi = eps_l[b] * (q0 / (x * x) - 2.0 * p0 / (x * x * x));
-      T[b] += S[0] * p0 + S[1] * p1 + S[2] * p2;
+      T psi = p0 / (x * x) * eps_l[b];
+      T dpsi = eps_l[b] * (q0 / (x * x) - (T)2 * p0 / (x * x * x));
+      p0 = operand<BF16>(p0); p1 = operand<BF16>(p1); p2 = operand<BF16>(p2);
+      q0 = operand<BF16>(q0); q1 = operand<BF16>(q1); q2 = operand<BF16>(q2);
+      psi = operand<BF16>(psi); dpsi = operand<BF16>(dpsi);
+      Tv[b] += S[0] * p0 + S[1] * p1 + S[2] * p2;
       E[b] += S[3] * psi;
-      const double G = S[0] * q0 + S[1] * q1 + S[2] * q2;
-      const double H = S[3] * dpsi;
+      const T G = S[0] * q0 + S[1] * q1 + S[2] * q2;
+      const T H = S[3] * dpsi;
 #pragma unroll
       for (int d = 0; d < ND; ++d) {
         Td[b][d] += dS[d][0] * p0 + dS[d][1] * p1 + dS[d][2] * p2 + G * dx[d];
@@ -444,14 +457,16 @@ __global__ void __launch_bounds__(NTH) __attribute__((amdgpu_waves_per_eu(2))) l
 #pragma unroll
   for (int b = 0; b < LB; ++b) {
     double cb[NS];
-    cb[0] = P * T[b] * T[b];
-    cb[1] = P * T[b] * E[b];
-    cb[2] = P * E[b] * E[b];
+    const double tv = Tv[b], ev = E[b];
+    cb[0] = P * tv * tv;
+    cb[1] = P * tv * ev;
+    cb[2] = P * ev * ev;
 #pragma unroll
     for (int d = 0; d < ND; ++d) {
-      cb[3 + 3 * d + 0] = P * 2.0 * T[b] * Td[b][d];
-      cb[3 + 3 * d + 1] = P * (Td[b][d] * E[b] + T[b] * Ed[b][d]);
-      cb[3 + 3 * d + 2] = P * 2.0 * E[b] * Ed[b][d];
+      const double td = Td[b][d], ed = Ed[b][d];
+      cb[3 + 3 * d + 0] = P * 2.0 * tv * td;
+      cb[3 + 3 * d + 1] = P * (td * ev + tv * ed);
+      cb[3 + 3 * d + 2] = P * 2.0 * ev * ed;
     }
 #pragma unroll
     for (int q = 0; q < 3; ++q) {
@@ -635,10 +650,11 @@ extern "C" int abcmb_los_tangent(const AbcmbCosmo* cosmos_dev, int n_cosmo, cons
                                  const double* dsrc_dev, const double* dchi_dev, int n_dir, int n_tau,
                                  const double* bgv_dev, const double* kt_dev, int nkt, const double* bes_dev,
                                  const double* bes_rng_dev, const int32_t* bes_l_dev, int n_lt, int nx,
-                                 const int32_t* ell_idx_dev, int n_ell, double* cl_dev, void* workspace,
-                                 size_t ws_bytes, void* stream) {
+                                 const int32_t* ell_idx_dev, int n_ell, int precision, double* cl_dev,
+                                 void* workspace, size_t ws_bytes, void* stream) {
   ABCMB_REQUIRE(cosmos_dev && src_dev && dsrc_dev && dchi_dev && bgv_dev && kt_dev && bes_dev && bes_rng_dev &&
                     bes_l_dev && ell_idx_dev && cl_dev, "null argument");
+  ABCMB_REQUIRE(precision >= 0 && precision <= 2, "precision must be 0 (FP64), 1 (FP32) or 2 (BF16 operands)");
   ABCMB_REQUIRE(n_dir == ABCMB_TANGENT_DIRS, "n_dir must be %d", ABCMB_TANGENT_DIRS);
   ABCMB_REQUIRE(n_cosmo > 0 && n_tau > 0 && nkt > 1 && n_lt > 0 && nx > 1 && n_ell > 0, "bad sizes");
   ABCMB_REQUIRE(workspace && ws_bytes >= abcmb_los_tangent_workspace_size(n_cosmo, n_dir, n_ell, nkt),
@@ -649,9 +665,14 @@ extern "C" int abcmb_los_tangent(const AbcmbCosmo* cosmos_dev, int n_cosmo, cons
   const int nch = (nkt + LOS_T_NTH - 1) / LOS_T_NTH;
   double* part = (double*)workspace;
   dim3 grid(nch, (n_ell + LOS_T_LB - 1) / LOS_T_LB, n_cosmo);
-  hipLaunchKernelGGL((los_tangent_kernel<LOS_T_NTH, LOS_T_LB, ND>), grid, dim3(LOS_T_NTH), 0, s, cosmos_dev, src_dev,
-                     dsrc_dev, dchi_dev, n_tau, bgv_dev, kt_dev, nkt, bes_dev, bes_rng_dev, bes_l_dev, n_lt, nx,
-                     ell_idx_dev, n_ell, part);
+#define ABCMB_LOS_T(T_, BF_)                                                                                  \
+  hipLaunchKernelGGL((los_tangent_kernel<T_, BF_, LOS_T_NTH, LOS_T_LB, ND>), grid, dim3(LOS_T_NTH), 0, s,            \
+                     cosmos_dev, src_dev, dsrc_dev, dchi_dev, n_tau, bgv_dev, kt_dev, nkt, bes_dev, bes_rng_dev,      \
+                     bes_l_dev, n_lt, nx, ell_idx_dev, n_ell, part)
+  if (precision == 0) ABCMB_LOS_T(double, false);
+  else if (precision == 1) ABCMB_LOS_T(float, false);
+  else ABCMB_LOS_T(float, true);
+#undef ABCMB_LOS_T
   hipLaunchKernelGGL(sum_parts_kernel, dim3((NS * n_ell * n_cosmo + 255) / 256), dim3(256), 0, s, part,
                      NS * n_ell * n_cosmo, nch, cl_dev);
   ABCMB_HIP_CHECK(hipGetLastError());

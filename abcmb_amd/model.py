@@ -798,8 +798,8 @@ class Model:
                                       cos_dev.data_ptr(), nb, src_b.data_ptr(), dsrc.data_ptr(), dchi_d.data_ptr(),
                                       nd, n_tau, bgv_b.data_ptr(), D["kt"].data_ptr(), nkt, D["bes"].data_ptr(),
                                       D["rng"].data_ptr(), D["bes_l"].data_ptr(), self.bes_l.size,
-                                      self.bes_tab.shape[2], D["ell_idx"].data_ptr(), n_ell, raw.data_ptr(),
-                                      ws.data_ptr(), ws.numel() * 8, stream)), "abcmb_los_tangent")
+                                      self.bes_tab.shape[2], D["ell_idx"].data_ptr(), n_ell, self.los_precision,
+                                      raw.data_ptr(), ws.data_ptr(), ws.numel() * 8, stream)), "abcmb_los_tangent")
             abi.check(lib.abcmb_apply_operator(D["ell_op"].data_ptr(), nl, n_ell, raw.data_ptr(), nb * nv * 3,
                                                out[c0].data_ptr(), stream), "abcmb_apply_operator")
         return out

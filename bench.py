@@ -108,13 +108,13 @@ TOL_CONVERGED = {"rtol_large_k_PE": 1e-4 / 300, "atol_large_k_PE": 1e-6 / 300, "
                  "atol_small_k_PE": 1e-10 / 30}
 
 
-def build_model(tol: dict = None):
+def build_model(tol: dict = None, los_precision: str = "fp64"):
     """The benchmark's model, at the reference's ODE tolerances unless ``tol``
     overrides some of them."""
     from abcmb_amd.model import Model, MassiveNeutrino
     # max_steps_PE raised from the reference default 2048: with massive neutrinos the
     # highest modes (k ~ 0.6 Mpc^-1) need up to ~2000 steps in this integrator
-    return Model(user_species=(MassiveNeutrino,), lensing=True, l_max=2500,
+    return Model(user_species=(MassiveNeutrino,), lensing=True, l_max=2500, los_precision=los_precision,
                  max_steps_PE=4096 if not tol else 32768, **(tol or {}))
 
 
@@ -326,6 +326,31 @@ def main():
     print(f"# gradient error vs converged: timed {grad_err_conv:.2e}, reference tolerances {ref_err_conv:.2e}, "
           f"plain {plain_err_conv:.2e}", file=sys.stderr, flush=True)
 
+    # the reduced-precision LOS contractions in the same gradient path (one step of the
+    # same batch each): rate, and spectra and gradients against the FP64 LOS
+    los_variants = {}
+    if args.gradient == "tangent" and not args.fd_downstream:
+        scale_all = np.linalg.norm(grad * thetas, axis=1, keepdims=True) / np.abs(thetas)
+        for prec in ("fp32", "bf16"):
+            mv = build_model(tol, los_precision=prec)
+            if tuned:
+                autotune.apply(mv, tuned)
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            cl_v = []
+            _, gv = LK.tangent_gradients(mv, like, thetas, k_grids=k_grids, chunk=args.chunk, cl_out=cl_v)
+            torch.cuda.synchronize()
+            rate_v = args.batch / (time.perf_counter() - t0)
+            cl_v = torch.cat(cl_v).cpu().numpy()[:np_]
+            los_variants[prec] = {
+                "grad_evals_per_s": rate_v,
+                "grad_diff_vs_fp64_los": float(np.max(np.abs(gv - grad) / scale_all)),
+                "grad_err_vs_converged": err_to_norm(gv[:np_], g_conv),
+                "cl_diff_vs_fp64_los": {x: max(cl_errors(cl_v[i], cl_b[i], model.ells.size)[x] for i in range(np_))
+                                        for x in ("TT", "TE", "EE")}}
+            del mv
+        print(f"# LOS variants {los_variants}", file=sys.stderr, flush=True)
+
     ms = sum(t["ms"] for t in timing) / len(timing)
     if args.gradient == "tangent":
         flops = sum(ode_flops_tangent(model, t["steps"], t["iters"]) for t in timing) / len(timing)
@@ -339,7 +364,8 @@ def main():
     tfile = os.path.join(ROOT, "profiles", "pert_tangent_traffic.json")
     if args.gradient == "tangent" and os.path.exists(tfile):
         t = json.load(open(tfile))
-        if t.get("tolerances", "reference") == args.tolerances and t.get("cosmologies_per_launch") == args.chunk:
+        if (t.get("tolerances", "reference"), t.get("batch"), t.get("chunk")) == (args.tolerances, args.batch,
+                                                                                   args.chunk):
             traffic = t["fetch_bytes_per_launch"] + t["write_bytes_per_launch"]
     # algorithmic HBM bytes of an ODE launch: the perturbation table it produces
     # (21 fields x saved points x k-modes, FP64, value and 5 tangents)
@@ -432,6 +458,11 @@ def main():
                    "reference_tolerances": {"grad_evals_per_s": ref_rate, "grad_err_vs_converged": ref_err_conv,
                                             "workload": f"the same {args.batch} cosmologies, one step"},
                    "large_batch": large,
+                   "los_variants": los_variants,
+                   "los_variants_note": "the tangent-linear LOS with FP32 arithmetic, or FP32 accumulation of "
+                                        "bfloat16-rounded operands, in the timed gradient path; errors against "
+                                        "the FP64 LOS (spectra of the plain-path cosmologies, gradients of the "
+                                        "whole batch relative to each norm) and against the converged gradients",
                    "grad_err_reference": "tangent-linear gradients of the first plain-path cosmologies with the "
                                          "large-k ODE tolerances / 300 and the small-k ones / 30 (max error over "
                                          "parameters relative to the gradient norm, each component in units of "

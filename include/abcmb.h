@@ -294,7 +294,7 @@ int abcmb_los_cl(const AbcmbCosmo* cosmos_dev, int n_cosmo, const double* src_de
 
 /*
  * Tangent-linear line-of-sight C_l: the sampled-l TT, TE, EE of abcmb_los_cl
- * (FP64) and their derivatives along n_dir (= ABCMB_TANGENT_DIRS) directions
+ * and their derivatives along n_dir (= ABCMB_TANGENT_DIRS) directions
  * of the sources and of the conformal distance chi_i = tau0 - tau_i at the
  * LOS nodes, plus the derivatives with respect to ln A_s and n_s.  The
  * Bessel lookups are differentiated exactly (slope of the interpolated
@@ -302,7 +302,10 @@ int abcmb_los_cl(const AbcmbCosmo* cosmos_dev, int n_cosmo, const double* src_de
  * SpectrumSolver.Cl_one_ell (abcmb/spectrum.py:616-797).
  * dsrc_dev: [B][n_dir][4][n_tau][nkt] derivatives of src_dev; dchi_dev:
  * [B][n_dir][n_tau] derivatives of chi; cl_dev out: [B][1 + n_dir + 2][3][n_ell]
- * (value, the n_dir directions, ln A_s, n_s; each TT, TE, EE).
+ * (value, the n_dir directions, ln A_s, n_s; each TT, TE, EE).  precision:
+ * 0 FP64; 1 FP32 (Bessel lookups and the contraction over tau in FP32,
+ * the k-quadrature in FP64); 2 the same with the contraction's operands
+ * rounded to bfloat16 (abcmb_los_cl's variants).
  */
 size_t abcmb_los_tangent_workspace_size(int n_cosmo, int n_dir, int n_ell, int nkt);
 int abcmb_los_tangent(const AbcmbCosmo* cosmos_dev, int n_cosmo, const double* src_dev,
@@ -310,7 +313,7 @@ int abcmb_los_tangent(const AbcmbCosmo* cosmos_dev, int n_cosmo, const double* s
                       const double* bgv_dev, const double* kt_dev, int nkt,
                       const double* bes_dev, const double* bes_rng_dev,
                       const int32_t* bes_l_dev, int n_lt, int nx,
-                      const int32_t* ell_idx_dev, int n_ell, double* cl_dev,
+                      const int32_t* ell_idx_dev, int n_ell, int precision, double* cl_dev,
                       void* workspace, size_t ws_bytes, void* stream);
 
 /* Dense linear map out[m][r] = sum_c op[r][c] in[m][c] (the l-spline of

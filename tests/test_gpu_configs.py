@@ -71,6 +71,37 @@ def test_config2_lensed_l2500_matches_oracle(bins):
     assert max(e) < 1e-6
 
 
+def test_config2_2000_los_samples():
+    """Configuration 2's 2000 conformal-time LOS samples (n_lna_save = 2001 saved
+    points: the ODE's dense output, sources, LOS and Limber on that grid), FP64 and
+    FP32 LOS, against a complete oracle evaluation on the same grid: FP64 within
+    1e-6, FP32 within the contract; the spectra move from the 499-sample ones by the
+    LOS quadrature's own error (printed)."""
+    from abcmb_amd.model import Model, MassiveNeutrino
+    from oracle import pipeline
+    kw = dict(user_species=(MassiveNeutrino,), lensing=True, k_axis_perturbations=np.geomspace(1e-5, 0.5, 500),
+              max_steps_PE=4096)
+    m = Model(n_lna_save=2001, **kw)
+    m32 = Model(n_lna_save=2001, los_precision="fp32", **kw)
+    m500 = Model(**kw)
+    p = m.derived(dict(P0, N_nu_massive=1, m_nu_massive=0.06))
+    plan = m.prepare(p)
+    assert plan.lna_save.size == 2001 and plan.bgv.shape == (6, 2000)
+    _say("case: config2, 2000 LOS samples, FP64 and FP32 LOS vs complete oracle")
+    g = m.execute([plan])[0]
+    g32 = m32.execute([plan])[0]
+    o = pipeline.run(m, plan)
+    assert np.all(o["status"] == 0)
+    e = _errors(_lensed(g), o["lensed"])
+    e32 = _errors(_lensed(g32), o["lensed"])
+    e500 = _errors(_lensed(m500.execute([m500.prepare(p)])[0]), o["lensed"])
+    _say(f"FP64 vs oracle: TT {e[0]:.1e} TE {e[1]:.1e} EE {e[2]:.1e} BB {e[3]:.1e}; FP32: TT {e32[0]:.1e} "
+         f"TE {e32[1]:.1e} EE {e32[2]:.1e} BB {e32[3]:.1e}; 499 samples vs 2000: TT {e500[0]:.1e} "
+         f"TE {e500[1]:.1e} EE {e500[2]:.1e} BB {e500[3]:.1e}")
+    assert max(e) < 1e-6
+    assert max(e32[:3]) < 1e-4 and e32[3] < 1e-3
+
+
 def test_config3_batched_forward():
     from abcmb_amd.model import Model, MassiveNeutrino
     from oracle import pipeline
