@@ -70,6 +70,10 @@ class AbcmbTangent(C.Structure):
     _fields_ = [("par", _P), ("tau_tab", _P), ("xe_tab", _P), ("tm_tab", _P), ("pt", _P)]
 
 
+class AbcmbCotangent(C.Structure):
+    _fields_ = [("pt", _P), ("par", _P), ("tau_tab", _P), ("xe_tab", _P), ("tm_tab", _P)]
+
+
 assert C.sizeof(AbcmbCosmo) == COSMO_DOUBLES * 8 + 6 * 4 + 6 * 8
 assert C.sizeof(AbcmbTangent) == 40
 assert C.sizeof(AbcmbPertItem) == 48
@@ -106,6 +110,7 @@ HIP_SYMBOLS = (
     "abcmb_lensing_clpp", "abcmb_lens_workspace_size", "abcmb_lens", "abcmb_pk_lin", "abcmb_pert_fwd_noalloc",
     "abcmb_pk", "abcmb_lens_dmat_size", "abcmb_lens_dmat", "abcmb_lens_mfma_workspace_size", "abcmb_lens_mfma",
     "abcmb_tune_set", "abcmb_tune_get", "abcmb_reionize", "abcmb_los_background",
+    "abcmb_pert_adjoint_workspace_size", "abcmb_pert_adjoint",
 )
 TUNE_PERT_PERSISTENT, TUNE_LENS_GEMM_COLS = 0, 1
 PK_M, PK_CB = 0, 1
@@ -129,6 +134,10 @@ def hip_lib():
                                                _P, _P, _P, _P, _P]
         lib.abcmb_pert_tangent.argtypes = [_P, _P, C.c_int, _P, C.c_int, C.POINTER(AbcmbOdeSettings), C.c_int,
                                            _P, _P, _P, _P, C.c_size_t, _P]
+        lib.abcmb_pert_adjoint_workspace_size.restype = C.c_size_t
+        lib.abcmb_pert_adjoint_workspace_size.argtypes = [C.c_int, C.c_int, C.c_int, C.POINTER(AbcmbOdeSettings)]
+        lib.abcmb_pert_adjoint.argtypes = [_P, _P, C.c_int, _P, C.c_int, C.POINTER(AbcmbOdeSettings), C.c_int,
+                                           C.c_int, _P, _P, _P, _P, C.c_size_t, _P]
         lib.abcmb_sources_workspace_size.restype = C.c_size_t
         lib.abcmb_sources_workspace_size.argtypes = [C.c_int, C.c_int, C.c_int]
         lib.abcmb_sources.argtypes = [_P, C.c_int, C.c_int, C.c_int, _P, _P, C.c_int, _P, _P, _P, C.c_size_t, _P]

@@ -932,6 +932,91 @@ class Model:
         ds = self._downstream(plans, run["pts"], run["kdev"])
         return self._outputs(plans, run, ds), run["dpts"]
 
+    def ode_adjoint(self, plans, pt_bars):
+        """Reverse-mode cotangents of the host inputs of ``plans``' perturbation tables.
+
+        ``pt_bars[i]``: dL/d(table of plans[i]), a [21][n_save][nk_i] float64 GPU
+        tensor.  One ``abcmb_pert_adjoint`` launch for all (cosmology, k) items
+        (the same items, order and step sequences as ``execute``).  Returns one
+        dict per plan with the GPU tensors ``par`` (dL/d of the 31 AbcmbCosmo
+        double fields), ``tau``, ``xe`` and ``tm`` (dL/d of the table entries),
+        after checking every item's status.
+        """
+        self._complete_pending(plans)
+        torch = _torch()
+        lib = abi.hip_lib()
+        D = self._device_constants()
+        dev = torch.device("cuda")
+        stream = torch.cuda.current_stream().cuda_stream
+        ns = self.n_save
+        stage = _Stage(torch, dev)
+        dt = stage.add
+        allk = [(pl.k if pl.k is not None else self.k_pt) for pl in plans]
+        kdev = [D["k_pt"] if kk is self.k_pt else dt(kk) for kk in allk]
+        for pb, kk in zip(pt_bars, allk):
+            if tuple(pb.shape) != (abi.N_PT_FIELDS, ns, kk.size) or pb.dtype != torch.float64 or not pb.is_cuda:
+                raise ValueError(f"table cotangent of shape {tuple(pb.shape)} / {pb.dtype}; expected "
+                                 f"({abi.N_PT_FIELDS}, {ns}, {kk.size}) float64 on the GPU")
+        pt_bars = [pb.contiguous() for pb in pt_bars]
+        cos = self._cosmo_array(plans, pt_bars, kdev, dt)
+        cos_dev = self._struct_array(torch, dev, cos, dt)
+        flat = [(-allk[ci][ik], ci, ik) for ci in range(len(plans)) for ik in range(allk[ci].size)]
+        flat.sort()
+        n_items = len(flat)
+        items = [abi.AbcmbPertItem(k=allk[ci][ik], lna_start=plans[ci].lna_start[ik], rtol=plans[ci].rtol[ik],
+                                   atol=plans[ci].atol[ik], cosmo=ci, ik=int(ik)) for _, ci, ik in flat]
+        items_dev = self._struct_array(torch, dev, items, dt)
+        out = []
+        for pl in plans:
+            out.append(dict(par=torch.zeros(abi.COSMO_DOUBLES, dtype=torch.float64, device=dev),
+                            tau=torch.zeros(abi.TAU_TAB_N, dtype=torch.float64, device=dev),
+                            xe=torch.zeros(pl.bg.xe_tab.size, dtype=torch.float64, device=dev),
+                            tm=torch.zeros(pl.bg.tm_tab.size, dtype=torch.float64, device=dev)))
+        cots = [abi.AbcmbCotangent(pt=pb.data_ptr(), par=o["par"].data_ptr(), tau_tab=o["tau"].data_ptr(),
+                                   xe_tab=o["xe"].data_ptr(), tm_tab=o["tm"].data_ptr())
+                for pb, o in zip(pt_bars, out)]
+        cot_dev = self._struct_array(torch, dev, cots, dt)
+        status = torch.zeros(n_items, dtype=torch.int32, device=dev)
+        steps = torch.zeros(n_items, dtype=torch.int32, device=dev)
+        iters = torch.zeros(n_items, dtype=torch.int32, device=dev)
+        tab_max = max([abi.TAU_TAB_N] + [max(pl.bg.xe_tab.size, pl.bg.tm_tab.size) for pl in plans])
+        st = self.settings
+        ws_bytes = lib.abcmb_pert_adjoint_workspace_size(len(plans), ns, tab_max, C.byref(st))
+        if ws_bytes == 0:
+            raise RuntimeError(f"abcmb_pert_adjoint_workspace_size: {lib.abcmb_last_error().decode()}")
+        ws = torch.empty(ws_bytes // 8 + 1, dtype=torch.float64, device=dev)
+        stage.flush()
+        ev0 = ev1 = None
+        if self.timing is not None:
+            ev0 = torch.cuda.Event(enable_timing=True)
+            ev1 = torch.cuda.Event(enable_timing=True)
+            ev0.record()
+        abi.check(lib.abcmb_pert_adjoint(cos_dev.data_ptr(), cot_dev.data_ptr(), len(plans), items_dev.data_ptr(),
+                                         n_items, C.byref(st), ns, tab_max, status.data_ptr(), steps.data_ptr(),
+                                         iters.data_ptr(), ws.data_ptr(), ws.numel() * 8, stream),
+                  "abcmb_pert_adjoint")
+        if ev1 is not None:
+            ev1.record()
+        torch.cuda.current_stream().synchronize()
+        if ev0 is not None:
+            self.timing.append(dict(ms=ev0.elapsed_time(ev1), steps=int(steps.sum().item()),
+                                    iters=int(iters.sum().item()), items=n_items, kernel="adjoint"))
+        st_h = status.cpu().numpy()
+        bad = np.nonzero(st_h != abi_k_ok())[0]
+        if bad.size:
+            where = [(flat[b][1], float(allk[flat[b][1]][flat[b][2]])) for b in bad[:8]]
+            raise RuntimeError(f"perturbation adjoint failed for {bad.size} (cosmology, k) items; status codes "
+                               f"{np.unique(st_h[bad]).tolist()} (5 = transposed stage solve did not converge); "
+                               f"first (cosmology, k): {where}")
+        del stage, ws
+        steps_ck = [np.zeros(a.size, dtype=np.int32) for a in allk]
+        steps_h = steps.cpu().numpy()
+        for m, (_, ci, ik) in enumerate(flat):
+            steps_ck[ci][ik] = steps_h[m]
+        for o, s_ in zip(out, steps_ck):
+            o["steps"] = s_
+        return out
+
     def __call__(self, params: dict = None) -> Output:
         return self.run_batch([params if params is not None else {}])[0]
 

@@ -37,6 +37,7 @@ extern "C" {
 #define ABCMB_K_NONFINITE 2   /* NaN/Inf in state or error norm */
 #define ABCMB_K_STEP_UNDERFLOW 3  /* step size fell below the spacing of ln a */
 #define ABCMB_K_BAD_INPUT 4   /* the cosmology has more saved points than n_save_max */
+#define ABCMB_K_ADJOINT 5     /* adjoint: a transposed stage solve did not converge */
 
 #define ABCMB_TAU_TAB_N 10000      /* tau(ln a) table on [-33, 0] */
 #define ABCMB_TAU_TAB_LO (-33.0)
@@ -102,6 +103,20 @@ typedef struct AbcmbTangent {
   const double* tm_tab;   /* [ABCMB_TANGENT_DIRS][tm_n] */
   double* pt;             /* out: [ABCMB_TANGENT_DIRS][ABCMB_N_PT_FIELDS][n_save][nk] */
 } AbcmbTangent;
+
+/*
+ * Reverse-mode cotangents of one cosmology (abcmb_pert_adjoint): pt is the
+ * input, dL/d(perturbation table); the others are outputs, accumulated into
+ * (the caller zeroes them): dL/d(AbcmbCosmo double fields, declaration
+ * order) and dL/d(table entries) of the tau(ln a), x_e and T_m tables.
+ */
+typedef struct AbcmbCotangent {
+  const double* pt;       /* in:  [ABCMB_N_PT_FIELDS][n_save][nk] */
+  double* par;            /* out: [ABCMB_COSMO_DOUBLES] */
+  double* tau_tab;        /* out: [ABCMB_TAU_TAB_N] */
+  double* xe_tab;         /* out: [xe_n] */
+  double* tm_tab;         /* out: [tm_n] */
+} AbcmbCotangent;
 
 /* Integrator settings shared by a launch (abcmb/model_specs.py:57-68,
  * abcmb/perturbations.py:285-312). */
@@ -246,6 +261,35 @@ int abcmb_pert_tangent(const AbcmbCosmo* cosmos_dev, const AbcmbTangent* tangent
  * results are those of abcmb_pert_fwd.
  */
 #define ABCMB_NOALLOC_SCRATCH_BYTES (8 << 20)
+/*
+ * Discrete adjoint of the perturbation ODE: the transpose of
+ * abcmb_pert_tangent's linear map, for one cotangent per cosmology.  Replaces
+ * reverse-mode differentiation through PerturbationEvolver.full_evolution with
+ * Model(adjoint=RecursiveCheckpointAdjoint) (abcmb/main.py:111,
+ * abcmb/perturbations.py:75-117, :250-404).  Each (cosmology, k) item is
+ * integrated again as abcmb_pert_fwd does (same step sequence), recording
+ * every accepted step and a checkpoint every 64 accepted steps; the accepted
+ * steps are then swept backwards segment by segment, each segment recomputed
+ * from its checkpoint, with transposed stage solves (the chord matrix as the
+ * preconditioner of an iterative refinement to 1e-6), the transposed cubic
+ * Hermite dense output injecting the table cotangents at the saved points,
+ * and the cotangents of the background scalars and of the initial conditions
+ * pulled back to the cosmology fields and table entries they are computed
+ * from.  The step sizes are held fixed, as in abcmb_pert_tangent.
+ * cot_dev: n_cosmo device-resident AbcmbCotangent.  tab_max: at least
+ * ABCMB_TAU_TAB_N and every cosmology's xe_n and tm_n.  iters_dev: transposed
+ * refinement iterations per item.  Item status ABCMB_K_ADJOINT: a transposed
+ * stage solve did not converge (its cotangents are still accumulated).
+ * Supports state sizes up to 192 with hierarchy blocks up to 32 rows.
+ */
+size_t abcmb_pert_adjoint_workspace_size(int n_cosmo, int n_save_max, int tab_max,
+                                         const AbcmbOdeSettings* settings);
+int abcmb_pert_adjoint(const AbcmbCosmo* cosmos_dev, const AbcmbCotangent* cot_dev, int n_cosmo,
+                       const AbcmbPertItem* items_dev, int n_items,
+                       const AbcmbOdeSettings* settings, int n_save_max, int tab_max,
+                       int32_t* status_dev, int32_t* steps_dev, int32_t* iters_dev,
+                       void* workspace, size_t ws_bytes, void* stream);
+
 int abcmb_pert_fwd_noalloc(const AbcmbCosmo* cosmos_dev, int n_cosmo,
                            const AbcmbPertItem* items_dev, int n_items,
                            const AbcmbOdeSettings* settings, int n_save_max,
