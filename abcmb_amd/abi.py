@@ -110,7 +110,8 @@ HIP_SYMBOLS = (
     "abcmb_lensing_clpp", "abcmb_lens_workspace_size", "abcmb_lens", "abcmb_pk_lin", "abcmb_pert_fwd_noalloc",
     "abcmb_pk", "abcmb_lens_dmat_size", "abcmb_lens_dmat", "abcmb_lens_mfma_workspace_size", "abcmb_lens_mfma",
     "abcmb_tune_set", "abcmb_tune_get", "abcmb_reionize", "abcmb_los_background",
-    "abcmb_pert_adjoint_workspace_size", "abcmb_pert_adjoint",
+    "abcmb_pert_adjoint_workspace_size", "abcmb_pert_adjoint", "abcmb_los_adjoint_workspace_size",
+    "abcmb_los_adjoint", "abcmb_lens_adjoint_workspace_size", "abcmb_lens_adjoint", "abcmb_lensing_clpp_adjoint",
 )
 TUNE_PERT_PERSISTENT, TUNE_LENS_GEMM_COLS = 0, 1
 PK_M, PK_CB = 0, 1
@@ -138,6 +139,14 @@ def hip_lib():
         lib.abcmb_pert_adjoint_workspace_size.argtypes = [C.c_int, C.c_int, C.c_int, C.POINTER(AbcmbOdeSettings)]
         lib.abcmb_pert_adjoint.argtypes = [_P, _P, C.c_int, _P, C.c_int, C.POINTER(AbcmbOdeSettings), C.c_int,
                                            C.c_int, _P, _P, _P, _P, C.c_size_t, _P]
+        lib.abcmb_los_adjoint_workspace_size.restype = C.c_size_t
+        lib.abcmb_los_adjoint_workspace_size.argtypes = [C.c_int] * 5
+        lib.abcmb_los_adjoint.argtypes = [_P, C.c_int, C.c_int, C.c_int, _P, _P, C.c_int, _P, _P, _P, _P, C.c_int,
+                                          C.c_int, _P, C.c_int, _P, _P, _P, _P, _P, C.c_size_t, _P]
+        lib.abcmb_lens_adjoint_workspace_size.restype = C.c_size_t
+        lib.abcmb_lens_adjoint_workspace_size.argtypes = [C.c_int] * 3
+        lib.abcmb_lens_adjoint.argtypes = [C.c_int, _P, _P, C.c_int, _P, _P, C.c_int, _P, _P, _P, _P, C.c_size_t, _P]
+        lib.abcmb_lensing_clpp_adjoint.argtypes = [_P, C.c_int, _P, C.c_int, _P, C.c_int, _P, _P, _P]
         lib.abcmb_sources_workspace_size.restype = C.c_size_t
         lib.abcmb_sources_workspace_size.argtypes = [C.c_int, C.c_int, C.c_int]
         lib.abcmb_sources.argtypes = [_P, C.c_int, C.c_int, C.c_int, _P, _P, C.c_int, _P, _P, _P, C.c_size_t, _P]

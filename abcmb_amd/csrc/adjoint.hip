@@ -366,7 +366,7 @@ __device__ __forceinline__ TInv transposed_setup(AdjSmem<NT, BMAX>& sm, const Ro
 }
 
 // mu = (I - g A(slot))^-T x by iterative refinement with P^-T; wt: the row's
-// weight atol + rtol |z|
+// weight atol + rtol |z|.  fail: a non-finite cotangent
 template <int NT, int BMAX>
 __device__ __forceinline__ double adj_solve(AdjSmem<NT, BMAX>& sm, const RowState& R, const ColT& T, int& rs,
                                             int& lb, const Inv<BMAX>& F, const TInv& X, double g, int slot,
@@ -383,16 +383,18 @@ __device__ __forceinline__ double adj_solve(AdjSmem<NT, BMAX>& sm, const RowStat
       if (!isfinite(mn)) fail = true;
       break;
     }
+    if (!isfinite(dn)) { fail = true; break; }
     if (it >= 1) {
       if (dn <= ADJ_RTOL * mn) break;   // the last update is below the tolerance
       if (it >= 2) {
         // remaining error ~ dn th / (1 - th) with the contraction rate th
         const double th = dn / prev;
         if (th < 1.0 && dn * th <= ADJ_RTOL * mn * (1.0 - th)) break;
-        if (th > 2.0) { fail = true; break; }
       }
     }
-    if (it >= maxit) { fail = true; break; }
+    // the iteration cap of the tangent-linear kernel's stage tangents (which
+    // stop there too): the transposed iteration has the same contraction rate
+    if (it >= maxit) break;
     prev = dn;
     d = pt_solve(sm, R, rs, F, X, g, x - mu + g * q);
     mu += d;

@@ -16,6 +16,7 @@
 #include <cstdint>
 
 #include "common.h"
+#include "dual.h"
 #include "wave.h"
 
 namespace abcmb {
@@ -733,6 +734,315 @@ int lens_mfma(int n_cosmo, const double* cl_in, const double* clpp, int n_l, con
   return ABCMB_OK;
 }
 
+
+// ---- reverse mode of the lensing (abcmb_lens_adjoint, abcmb_lensing_clpp_adjoint) ----
+// From the cotangents lb[4][n_l] of the lensed TT, TE, EE, BB, the passes of
+// abcmb_lens backwards:
+//   back-transform: xib_q(mu) = f w(mu) sum_l d_q(mu, l) lb(l) (TT: d00, TE: d20
+//     with f = 2 pi; xi+ with lb_EE + lb_BB against d22 and xi- with lb_EE - lb_BB
+//     against d2-2, f = pi), one thread per node;
+//   correlation pass: per node the xi's are linear in the unlensed spectra, so
+//     clb(l) gets pre(l) K(l, mu) xib(mu) / 4pi summed over the nodes (per-block
+//     sums, then across blocks), and the dependence on sigma^2(mu) and
+//     Cgl2(mu) is carried by two-direction dual numbers through the same X
+//     functions, giving their cotangents per node;
+//   Cgl pass: cgl_bar(mu) = -s2_bar(mu) (plus the sum of s2_bar at the last
+//     node, which every sigma^2 references) and cgl2_bar = C2_bar, then
+//     clppb(l) = (2l+1) l (l+1) / 4pi sum_mu (cgl_bar d11 + cgl2_bar d1-1).
+__global__ void __launch_bounds__(64) back_adj_kernel(const double* __restrict__ lb_all, int n_l,
+                                                      const double* __restrict__ mu_arr,
+                                                      const double* __restrict__ w_arr, int n_mu,
+                                                      const double* __restrict__ coef,
+                                                      const double* __restrict__ lfac, int LT,
+                                                      double* __restrict__ xib_all) {
+  const int i = blockIdx.x * 64 + threadIdx.x;
+  const int ci = blockIdx.y;
+  if (i >= n_mu) return;
+  const double* lb = lb_all + (size_t)ci * 4 * n_l;
+  double* xib = xib_all + (size_t)ci * 4 * n_mu;
+  const double mu = mu_arr[i], w = w_arr[i];
+  const double* c00 = coef;
+  const double* c2m2 = coef + (size_t)3 * LT * 3;
+  const double* c22 = coef + (size_t)4 * LT * 3;
+  const double* c20 = coef + (size_t)9 * LT * 3;
+  Rec r00{wigner_start(0, 0, mu), 0.0}, r20{wigner_start(2, 0, mu), 0.0};
+  Rec r22{wigner_start(2, 2, mu), 0.0}, r2m2{wigner_start(2, -2, mu), 0.0};
+  rec_step(r00, c00, 0, mu);
+  rec_step(r00, c00, 1, mu);
+  double s0 = 0.0, sx = 0.0, sp = 0.0, sm = 0.0;
+  for (int j = 0; j < n_l; ++j) {
+    const int l = j + 2;
+    const double nf = lfac[l * LFAC];
+    s0 += lb[j] * (r00.dl * nf);
+    sx += lb[n_l + j] * (r20.dl * nf);
+    sp += (lb[2 * n_l + j] + lb[3 * n_l + j]) * (r22.dl * nf);
+    sm += (lb[2 * n_l + j] - lb[3 * n_l + j]) * (r2m2.dl * nf);
+    rec_step(r00, c00, l, mu);
+    rec_step(r20, c20, l, mu);
+    rec_step(r22, c22, l, mu);
+    rec_step(r2m2, c2m2, l, mu);
+  }
+  xib[0 * n_mu + i] = 2. * PI * w * s0;
+  xib[1 * n_mu + i] = PI * w * sp;
+  xib[2 * n_mu + i] = PI * w * sm;
+  xib[3 * n_mu + i] = 2. * PI * w * sx;
+}
+
+// partials part[blk][3][n_l] of the unlensed spectra's cotangents (TT, TE, EE),
+// and s2c2b[2][n_mu]: the cotangents of sigma^2 and Cgl2 at each node
+__global__ void __launch_bounds__(64) ksi_adj_kernel(const double* __restrict__ cl_all,
+                                                     const double* __restrict__ cgl_all, int n_l,
+                                                     const double* __restrict__ mu_arr, int n_mu,
+                                                     const double* __restrict__ coef,
+                                                     const double* __restrict__ lfac, int LT,
+                                                     const double* __restrict__ xib_all, double* __restrict__ part_all,
+                                                     double* __restrict__ s2c2b_all) {
+  __shared__ CoefTile T;
+  const int i0 = blockIdx.x * blockDim.x + threadIdx.x;
+  const bool live = i0 < n_mu;
+  const int i = live ? i0 : n_mu - 1;   // spare lanes follow the last node with zero cotangents
+  const int ci = blockIdx.y;
+  const double* cl_in = cl_all + (size_t)ci * 3 * n_l;
+  const double* cgl = cgl_all + (size_t)ci * 2 * n_mu;
+  const double* xib = xib_all + (size_t)ci * 4 * n_mu;
+  double* part = part_all + ((size_t)ci * gridDim.x + blockIdx.x) * 3 * n_l;
+  const double* TT = cl_in;
+  const double* TE = cl_in + n_l;
+  const double* EE = cl_in + 2 * n_l;
+  const double b0 = live ? xib[0 * n_mu + i] / (4. * PI) : 0.0;
+  const double bp = live ? xib[1 * n_mu + i] / (4. * PI) : 0.0;
+  const double bm = live ? xib[2 * n_mu + i] / (4. * PI) : 0.0;
+  const double bx = live ? xib[3 * n_mu + i] / (4. * PI) : 0.0;
+  const double mu = mu_arr[i];
+  D<2> s2(cgl[n_mu - 1] - cgl[i]), C2(cgl[n_mu + i]);
+  s2.d[0] = 1.0;
+  C2.d[1] = 1.0;
+  Rec R[NPAIR];
+#pragma unroll
+  for (int q = 0; q < NPAIR; ++q) R[q] = Rec{0.0, 0.0};
+  R[0] = Rec{wigner_start(0, 0, mu), 0.0};
+  rec_step(R[0], coef + 0, 0, mu);
+  rec_step(R[0], coef + 0, 1, mu);
+  R[1] = Rec{wigner_start(1, 1, mu), 0.0};
+  rec_step(R[1], coef + (size_t)1 * LT * 3, 1, mu);
+  R[2] = Rec{wigner_start(1, -1, mu), 0.0};
+  rec_step(R[2], coef + (size_t)2 * LT * 3, 1, mu);
+  R[3] = Rec{wigner_start(2, -2, mu), 0.0};
+  R[4] = Rec{wigner_start(2, 2, mu), 0.0};
+  R[9] = Rec{wigner_start(2, 0, mu), 0.0};
+  const D<2> e2 = dexp(2. * s2 / 4.), e4 = dexp(4. * s2 / 4.), e83 = dexp(8. / 3. * s2 / 4.);
+  const D<2> e203 = dexp(20. / 3. * s2 / 4.), e10 = dexp(10. * s2 / 4.);
+  const D<2> g = dexp(-0.5 * s2);
+  D<2> E = dexp(-1.5 * s2), r = g * g * g;
+  const D<2> C22 = C2 * C2;
+  double s2b = 0.0, c2b = 0.0;
+  for (int j = 0; j < n_l; ++j) {
+    const int l = j + 2;
+    const int it = j % KT;
+    if (it == 0) load_tile(T, coef, lfac, LT, l);
+    if (l == 3) {
+      R[5] = Rec{wigner_start(3, 1, mu), 0.0};
+      R[7] = Rec{wigner_start(3, -3, mu), 0.0};
+      R[10] = Rec{wigner_start(3, -1, mu), 0.0};
+    }
+    if (l == 4) {
+      R[6] = Rec{wigner_start(4, 0, mu), 0.0};
+      R[8] = Rec{wigner_start(4, -4, mu), 0.0};
+      R[11] = Rec{wigner_start(4, -2, mu), 0.0};
+    }
+    const double* lf = T.f[it];
+    const double nf = lf[0];
+    double d[NPAIR];
+#pragma unroll
+    for (int q = 0; q < NPAIR; ++q) d[q] = R[q].dl * nf;
+    const double dl = l;
+    const double llp1 = dl * (dl + 1);
+    const D<2> X000 = E;
+    const D<2> X000p = -llp1 / 4. * X000;
+    const D<2> X220 = 1. / 4. * lf[1] * E * e2;
+    const D<2> X022 = E * e4;
+    const D<2> X022p = -(llp1 - 4) / 4 * X022;
+    const D<2> X121 = -1. / 2. * lf[2] * E * e83;
+    const D<2> X132 = -1. / 2. * lf[3] * E * e203;
+    const D<2> X242 = 1. / 4. * lf[4] * E * e10;
+    const double d00 = d[0], d11 = d[1], d1m1 = d[2], d2m2 = d[3], d22 = d[4], d31 = d[5];
+    const double d40 = d[6], d3m3 = d[7], d4m4 = d[8], d20 = d[9], d3m1 = d[10], d4m2 = d[11];
+    const double pre = 2. * dl + 1;
+    const D<2> K0 = X000 * X000 * d00 + lf[5] * C2 * X000p * X000p * d1m1 +
+                    C22 * (X000p * X000p * d00 + X220 * X220 * d2m2);
+    const D<2> Kp = X022 * X022 * d22 + 2 * C2 * X132 * X121 * d31 + C22 * (X022p * X022p * d22 + X242 * X220 * d40);
+    const D<2> Km = X022 * X022 * d2m2 + C2 * (X121 * X121 * d1m1 + X132 * X132 * d3m3) +
+                    1. / 2. * C22 * (2 * X022p * X022p * d2m2 + X220 * X220 * d00 + X242 * X242 * d4m4);
+    const D<2> Kx = X022 * X000 * d20 + C2 * lf[6] * X000p * (X121 * d11 + X132 * d3m1) +
+                    1. / 2. * C22 * ((2 * X022p * X000p + X220 * X220) * d20 + X220 * X242 * d4m2);
+    const double v3[3] = {pre * K0.v * b0, pre * Kx.v * bx, pre * (Kp.v * bp + Km.v * bm)};
+    double w[RowSums<3>::H2];
+    wave_sum_rows<3>(v3, w);
+    if ((threadIdx.x & 15) == 0 && (threadIdx.x >> 4) < 3) part[(threadIdx.x >> 4) * n_l + j] = w[0];
+    const double tt = TT[j], te = TE[j], ee = EE[j];
+    s2b += pre * (tt * K0.d[0] * b0 + te * Kx.d[0] * bx + ee * (Kp.d[0] * bp + Km.d[0] * bm));
+    c2b += pre * (tt * K0.d[1] * b0 + te * Kx.d[1] * bx + ee * (Kp.d[1] * bp + Km.d[1] * bm));
+    E = E * r;
+    r = r * g;
+    if (l >= 4) {
+#pragma unroll
+      for (int q = 0; q < NPAIR; ++q) rec_step_c(R[q], T.c[it][q][0], T.c[it][q][1], T.c[it][q][2], mu);
+    } else {
+#pragma unroll
+      for (int q = 0; q < NPAIR; ++q)
+        if (l >= PAIR_M[q]) rec_step_c(R[q], T.c[it][q][0], T.c[it][q][1], T.c[it][q][2], mu);
+    }
+  }
+  if (live) {
+    s2c2b_all[(size_t)ci * 2 * n_mu + i] = s2b;
+    s2c2b_all[(size_t)ci * 2 * n_mu + n_mu + i] = c2b;
+  }
+}
+
+// partials part[blk][n_l] of the C_l^phiphi cotangents
+__global__ void __launch_bounds__(64) cgl_adj_kernel(const double* __restrict__ s2c2b_all, int n_l,
+                                                     const double* __restrict__ mu_arr, int n_mu,
+                                                     const double* __restrict__ coef,
+                                                     const double* __restrict__ lfac, int LT,
+                                                     double* __restrict__ part_all) {
+  const int i0 = blockIdx.x * 64 + threadIdx.x;
+  const bool live = i0 < n_mu;
+  const int i = live ? i0 : n_mu - 1;
+  const int ci = blockIdx.y;
+  const double* s2b = s2c2b_all + (size_t)ci * 2 * n_mu;
+  const double* c2b = s2b + n_mu;
+  double* part = part_all + ((size_t)ci * gridDim.x + blockIdx.x) * n_l;
+  // every sigma^2(mu) = Cgl(mu_last) - Cgl(mu): the last node collects them all
+  double tot = 0.0;
+  for (int q = threadIdx.x; q < n_mu; q += 64) tot += s2b[q];
+  double tv[1] = {tot};
+  wave_sum(tv);
+  const double lcg = live ? -s2b[i] + (i == n_mu - 1 ? tv[0] : 0.0) : 0.0;
+  const double lcg2 = live ? c2b[i] : 0.0;
+  const double mu = mu_arr[i];
+  const double* c11 = coef + (size_t)1 * LT * 3;
+  const double* c1m = coef + (size_t)2 * LT * 3;
+  Rec a{wigner_start(1, 1, mu), 0.0}, b{wigner_start(1, -1, mu), 0.0};
+  rec_step(a, c11, 1, mu);
+  rec_step(b, c1m, 1, mu);
+  for (int j = 0; j < n_l; ++j) {
+    const int l = j + 2;
+    const double nf = lfac[l * LFAC];
+    double v[1] = {(2. * l + 1) * l * (l + 1) / (4. * PI) * (lcg * a.dl + lcg2 * b.dl) * nf};
+    wave_sum(v);
+    if (threadIdx.x == 0) part[j] = v[0];
+    rec_step(a, c11, l, mu);
+    rec_step(b, c1m, l, mu);
+  }
+}
+
+// out[c][r] = sum over blocks of part[c][blk][r]
+__global__ void sum_blocks_kernel(const double* __restrict__ part_all, int nblk, int n_row,
+                                  double* __restrict__ out_all) {
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  const int ci = blockIdx.y;
+  if (r >= n_row) return;
+  const double* part = part_all + (size_t)ci * nblk * n_row;
+  double s = 0.0;
+  for (int b = 0; b < nblk; ++b) s += part[(size_t)b * n_row + r];
+  out_all[(size_t)ci * n_row + r] = s;
+}
+
+// Limber C_l^phiphi backwards to the delta_m field of the perturbation table:
+// one thread per (ln a node, cosmology) walks the multipoles (the k bracket of
+// (l + 1/2) / chi moves forward with l) and adds 2 C d (trapezoid weight)
+// (l factor) clppb(l) times the bilinear weights into the four table entries,
+// kept in registers while the k bracket stays and added to the table (atomic
+// adds) when it moves.
+constexpr int CLPP_ADJ_THREADS = 64;
+__global__ void __launch_bounds__(CLPP_ADJ_THREADS) clpp_adj_kernel(const AbcmbCosmo* __restrict__ cosmos,
+                                                                    const double* __restrict__ lens_all, int n_lna,
+                                                                    const double* __restrict__ ells, int n_l,
+                                                                    const double* __restrict__ clppb_all,
+                                                                    double* const* __restrict__ ptbar) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int ci = blockIdx.y;
+  if (i >= n_lna) return;
+  const AbcmbCosmo& c = cosmos[ci];
+  const double* lna_ax = lens_all + (size_t)ci * 4 * n_lna;
+  const double* aH_ax = lna_ax + n_lna;
+  const double* tau_ax = lna_ax + 2 * n_lna;
+  const double chi_rec = lna_ax[3 * n_lna];
+  const double lna = lna_ax[i];
+  if (!(lna < 0.0)) return;
+  const double Om = c.omega_m / (c.h * c.h), OL = c.omega_Lambda / (c.h * c.h);
+  const double ns = c.n_s;
+  const double cnode = 9. / 8. / (PI * PI) * c.A_s * (2 * PI * PI) * pow(c.k_pivot, 1. - ns);
+  const int n_save = c.n_save, nk = c.nk;
+  const double* lna_save = c.lna_save;
+  const double* kgrid = c.k;
+  const double* dm = c.pt + (size_t)ABCMB_PT_DELTA_M * n_save * nk;
+  double* dmb = ptbar[ci] + (size_t)ABCMB_PT_DELTA_M * n_save * nk;
+  const double* clppb = clppb_all + (size_t)ci * n_l;
+  const double chi = c.tau0 - tau_ax[i];
+  const double win = (chi_rec - chi) / chi_rec / chi;
+  const double a = exp(lna);
+  const double z = 1. / a - 1.;
+  const double aH = aH_ax[i];
+  const double z3 = (1. + z) * (1. + z) * (1. + z);
+  const double omz = (Om * z3) / ((Om * z3) + OL);
+  const double aH2 = aH * aH;
+  const double C = chi / aH * win * win * omz * omz * (aH2 * aH2) * cnode * pow(chi, 5. - ns);
+  const double ichi = 1.0 / chi;
+  const double lq = -log(1. + z);
+  int ia, ib;
+  double u = 0.0;
+  if (lq <= lna_save[0]) { ia = ib = 0; }
+  else if (lq >= lna_save[n_save - 1]) { ia = ib = n_save - 1; }
+  else {
+    int lo = 0, hi = n_save - 1;
+    while (hi - lo > 1) { int mid = (lo + hi) >> 1; if (lna_save[mid] <= lq) lo = mid; else hi = mid; }
+    ia = lo; ib = lo + 1;
+    u = (lq - lna_save[lo]) / (lna_save[lo + 1] - lna_save[lo]);
+  }
+  // the trapezoid weight of node i (the sum runs over nodes 1 .. n_lna - 1)
+  const double tw = 0.5 * ((i > 0 ? lna - lna_ax[i - 1] : 0.0) + (i + 1 < n_lna ? lna_ax[i + 1] - lna : 0.0));
+  const double* ra = dm + (size_t)ia * nk;
+  const double* rb = dm + (size_t)ib * nk;
+  double* wa = dmb + (size_t)ia * nk;
+  double* wb = dmb + (size_t)ib * nk;
+  int ka = 0, cur0 = -1, cur1 = -1;
+  double acc[4] = {0.0, 0.0, 0.0, 0.0};   // (ia, k0), (ib, k0), (ia, k1), (ib, k1)
+  auto flush = [&]() {
+    if (cur0 >= 0) {
+      atomicAdd(wa + cur0, acc[0]);
+      atomicAdd(wb + cur0, acc[1]);
+      atomicAdd(wa + cur1, acc[2]);
+      atomicAdd(wb + cur1, acc[3]);
+    }
+    acc[0] = acc[1] = acc[2] = acc[3] = 0.0;
+  };
+  for (int il = 0; il < n_l; ++il) {
+    const double e5 = ells[il] + 0.5;
+    const double lfac = 8. * PI * PI / (e5 * e5 * e5) * pow(e5, ns - 5.);
+    const double kk = e5 * ichi;
+    while (ka < nk - 2 && kgrid[ka + 1] <= kk) ++ka;
+    int k0, k1;
+    double t;
+    if (kk <= kgrid[0]) { k0 = k1 = 0; t = 0.0; }
+    else if (kk >= kgrid[nk - 1]) { k0 = k1 = nk - 1; t = 0.0; }
+    else { k0 = ka; k1 = ka + 1; t = (kk - kgrid[k0]) / (kgrid[k1] - kgrid[k0]); }
+    const double da = ra[k0] + u * (rb[k0] - ra[k0]);
+    const double db = ra[k1] + u * (rb[k1] - ra[k1]);
+    const double d = da + t * (db - da);
+    const double ld = 2. * C * d * tw * lfac * clppb[il];
+    if (k0 != cur0 || k1 != cur1) {
+      flush();
+      cur0 = k0;
+      cur1 = k1;
+    }
+    acc[0] += (1. - t) * (1. - u) * ld;
+    acc[1] += (1. - t) * u * ld;
+    acc[2] += t * (1. - u) * ld;
+    acc[3] += t * u * ld;
+  }
+  flush();
+}
 }  // namespace
 }  // namespace abcmb
 
@@ -844,4 +1154,71 @@ extern "C" int abcmb_lens_mfma(int n_cosmo, const double* cl_in_dev, const doubl
   if (precision == 1)
     return lens_mfma<float>(n_cosmo, cl_in_dev, clpp_dev, n_l, mu_dev, w_dev, n_mu, dmat_dev, 1, out_dev, workspace, s);
   return lens_mfma<__bf16>(n_cosmo, cl_in_dev, clpp_dev, n_l, mu_dev, w_dev, n_mu, dmat_dev, 2, out_dev, workspace, s);
+}
+
+static size_t lens_adj_parts(int n_cosmo, int n_mu, int n_l, size_t* off) {
+  const int LT = n_l + 3;
+  const size_t nblk = (n_mu + 63) / 64;
+  size_t o = 0;
+  off[0] = o; o += (size_t)NPAIR * LT * 3;                 // coef
+  off[1] = o; o += (size_t)LT * LFAC;                       // lfac
+  off[2] = o; o += (size_t)n_cosmo * 2 * n_mu;             // cgl
+  off[3] = o; o += (size_t)n_cosmo * 4 * n_mu;             // xi cotangents
+  off[4] = o; o += (size_t)n_cosmo * 2 * n_mu;             // sigma^2, Cgl2 cotangents
+  off[5] = o; o += (size_t)n_cosmo * nblk * 3 * n_l;       // partials
+  return o;
+}
+
+extern "C" size_t abcmb_lens_adjoint_workspace_size(int n_cosmo, int n_mu, int n_l) {
+  size_t off[6];
+  return sizeof(double) * lens_adj_parts(n_cosmo, n_mu, n_l, off);
+}
+
+extern "C" int abcmb_lens_adjoint(int n_cosmo, const double* cl_in_dev, const double* clpp_dev, int n_l,
+                                  const double* mu_dev, const double* w_dev, int n_mu, const double* lensed_bar_dev,
+                                  double* cl_bar_dev, double* clpp_bar_dev, void* workspace, size_t ws_bytes,
+                                  void* stream) {
+  ABCMB_REQUIRE(cl_in_dev && clpp_dev && mu_dev && w_dev && lensed_bar_dev && cl_bar_dev && clpp_bar_dev &&
+                    workspace, "null argument");
+  ABCMB_REQUIRE(n_cosmo > 0 && n_l >= 3 && n_mu >= 2, "bad sizes");
+  ABCMB_REQUIRE(ws_bytes >= abcmb_lens_adjoint_workspace_size(n_cosmo, n_mu, n_l), "workspace too small");
+  hipStream_t s = (hipStream_t)stream;
+  size_t off[6];
+  lens_adj_parts(n_cosmo, n_mu, n_l, off);
+  double* ws = (double*)workspace;
+  double* coef = ws + off[0];
+  double* lfac = ws + off[1];
+  double* cgl = ws + off[2];
+  double* xib = ws + off[3];
+  double* s2c2b = ws + off[4];
+  double* part = ws + off[5];
+  const int LT = n_l + 3;
+  const int nblk = (n_mu + 63) / 64;
+  hipLaunchKernelGGL(wigner_coef_kernel, dim3((NPAIR * LT + 255) / 256), dim3(256), 0, s, LT, coef, lfac);
+  hipLaunchKernelGGL(cgl_kernel, dim3(nblk, n_cosmo), dim3(64), 0, s, clpp_dev, n_l, mu_dev, n_mu, coef, lfac, LT,
+                     cgl);
+  hipLaunchKernelGGL(back_adj_kernel, dim3(nblk, n_cosmo), dim3(64), 0, s, lensed_bar_dev, n_l, mu_dev, w_dev, n_mu,
+                     coef, lfac, LT, xib);
+  hipLaunchKernelGGL(ksi_adj_kernel, dim3(nblk, n_cosmo), dim3(64), 0, s, cl_in_dev, cgl, n_l, mu_dev, n_mu, coef,
+                     lfac, LT, xib, part, s2c2b);
+  hipLaunchKernelGGL(sum_blocks_kernel, dim3((3 * n_l + 255) / 256, n_cosmo), dim3(256), 0, s, part, nblk, 3 * n_l,
+                     cl_bar_dev);
+  hipLaunchKernelGGL(cgl_adj_kernel, dim3(nblk, n_cosmo), dim3(64), 0, s, s2c2b, n_l, mu_dev, n_mu, coef, lfac, LT,
+                     part);
+  hipLaunchKernelGGL(sum_blocks_kernel, dim3((n_l + 255) / 256, n_cosmo), dim3(256), 0, s, part, nblk, n_l,
+                     clpp_bar_dev);
+  ABCMB_HIP_CHECK(hipGetLastError());
+  return ABCMB_OK;
+}
+
+extern "C" int abcmb_lensing_clpp_adjoint(const AbcmbCosmo* cosmos_dev, int n_cosmo, const double* lens_dev,
+                                          int n_lna, const double* ells_dev, int n_l, const double* clpp_bar_dev,
+                                          double* const* pt_bar_dev, void* stream) {
+  ABCMB_REQUIRE(cosmos_dev && lens_dev && ells_dev && clpp_bar_dev && pt_bar_dev, "null argument");
+  ABCMB_REQUIRE(n_cosmo > 0 && n_lna >= 2 && n_l > 0, "bad sizes");
+  hipLaunchKernelGGL(clpp_adj_kernel, dim3((n_lna + CLPP_ADJ_THREADS - 1) / CLPP_ADJ_THREADS, n_cosmo),
+                     dim3(CLPP_ADJ_THREADS), 0, (hipStream_t)stream, cosmos_dev, lens_dev, n_lna, ells_dev, n_l,
+                     clpp_bar_dev, pt_bar_dev);
+  ABCMB_HIP_CHECK(hipGetLastError());
+  return ABCMB_OK;
 }

@@ -19,6 +19,7 @@
 #include <cstdint>
 
 #include "common.h"
+#include "dual.h"
 #include "wave.h"
 
 namespace abcmb {
@@ -106,6 +107,71 @@ __global__ void spline_m_kernel(const AbcmbCosmo* __restrict__ cosmos, int n_sav
   M[(size_t)(nk - 1) * n_tau] = Ma * (1.0 + h[nk - 2] / h[nk - 3]) - h[nk - 2] / h[nk - 3] * Mb;
 }
 
+// the ten LOS fields of the table at (kk, saved point s) by the not-a-knot
+// cubic spline in log10 k (abcmb/spectrum.py:664-678): the interval lo, the
+// weights A, B (values) and cA, cB (second derivatives M)
+struct SplineAt {
+  int lo;
+  double A, B, cA, cB;
+};
+
+__device__ __forceinline__ SplineAt spline_at(const double* __restrict__ x, int nk, double kk) {
+  const double xq = log10(kk);
+  int lo;
+  if (xq <= x[0]) lo = 0;
+  else if (xq >= x[nk - 1]) lo = nk - 2;
+  else {
+    int a = 0, b = nk - 1;
+    while (b - a > 1) { int mid = (a + b) >> 1; if (x[mid] <= xq) a = mid; else b = mid; }
+    lo = a;
+  }
+  SplineAt w;
+  w.lo = lo;
+  const double hh = x[lo + 1] - x[lo];
+  w.A = (x[lo + 1] - xq) / hh;
+  w.B = (xq - x[lo]) / hh;
+  w.cA = (w.A * w.A * w.A - w.A) * hh * hh / 6.0;
+  w.cB = (w.B * w.B * w.B - w.B) * hh * hh / 6.0;
+  return w;
+}
+
+__device__ __forceinline__ void los_spline(const AbcmbCosmo& c, const double* __restrict__ ws,
+                                           const double* __restrict__ Mb, int n_tau, int s, double kk,
+                                           double* v) {
+  const int nk = c.nk, n_save = c.n_save;
+  const SplineAt w = spline_at(ws, nk, kk);
+#pragma unroll
+  for (int f = 0; f < 10; ++f) {
+    const double* y = c.pt + ((size_t)LOS_FIELDS[f] * n_save + s) * nk;
+    const double* M = Mb + (size_t)f * nk * n_tau + s;
+    v[f] = w.A * y[w.lo] + w.B * y[w.lo + 1] + w.cA * M[(size_t)w.lo * n_tau] + w.cB * M[(size_t)(w.lo + 1) * n_tau];
+  }
+}
+
+// the four LOS sources at one (k, saved point) (abcmb/spectrum.py:681-696)
+// times the trapezoid-in-ln-a weight over aH (:755-772): v the spline values
+// of the LOS fields, (g, g', aH, e^-kappa, aH' aH) the background there, wdl
+// the ln a step (half at the first point).  V = double for the forward
+// kernel, a dual number for the adjoint's derivatives.
+template <typename V>
+__device__ __forceinline__ void src_terms(const V* v, const V& g, const V& gp, const V& aH, const V& em,
+                                          const V& aHd, double kk, double wdl, double sw, double isw, double dop,
+                                          double pol, V* out) {
+  const V wq = wdl / aH;
+  const V &dg = v[0], &tb = v[1], &tbp = v[2], &sg = v[3], &g0 = v[4], &g2 = v[5], &eta = v[6], &etap = v[7],
+          &al = v[8], &alp = v[9];
+  const V sT0 = sw * g * (dg / 4. + aH * alp) +
+                isw * (g * (eta - aH * alp - 2. * aH * al) + 2. * em * (aH * etap - aHd * al - aH * aH * alp)) +
+                dop * (aH * (g * ((tbp / (kk * kk)) + alp) + gp * ((tb / (kk * kk)) + al)));
+  const V sT1 = isw * em * ((aH * alp + 2. * aH * al - eta) * kk);
+  const V sT2 = pol * g * (2. * sg + g0 + g2) / 8.;
+  const V sE = sqrt(6.0) * g * (2. * sg + g0 + g2) / 8.;
+  out[0] = wq * sT0;
+  out[1] = wq * sT1;
+  out[2] = wq * sT2;
+  out[3] = wq * sE;
+}
+
 // sources at (s, kt).  bgv: [6][n_tau] (tau, g, g', aH, exp(-kappa), aH' aH)
 __global__ void sources_kernel(const AbcmbCosmo* __restrict__ cosmos, int n_save, size_t ws_stride,
                                const double* __restrict__ ws_all, size_t m_stride, const double* __restrict__ M_all,
@@ -117,50 +183,23 @@ __global__ void sources_kernel(const AbcmbCosmo* __restrict__ cosmos, int n_save
   int ci = blockIdx.z;
   if (j >= nkt) return;
   const AbcmbCosmo& c = cosmos[ci];
-  const double* pt = c.pt;
-  int nk = c.nk;
   const double* ws = ws_all + (size_t)ci * ws_stride;
   const double* Mb = M_all + (size_t)ci * m_stride;
   const double* bgv = bgv_all + (size_t)ci * 6 * n_tau;
   double* src = src_all + (size_t)ci * 4 * n_tau * nkt;
   double dl = c.lna_save[n_save - 1] - c.lna_save[n_save - 2];
-  const double* x = ws;
-  double kk = kt[j];
-  double xq = log10(kk);
-  int lo;
-  if (xq <= x[0]) lo = 0;
-  else if (xq >= x[nk - 1]) lo = nk - 2;
-  else {
-    int a = 0, b = nk - 1;
-    while (b - a > 1) { int mid = (a + b) >> 1; if (x[mid] <= xq) a = mid; else b = mid; }
-    lo = a;
-  }
-  double hh = x[lo + 1] - x[lo];
-  double A = (x[lo + 1] - xq) / hh, B = (xq - x[lo]) / hh;
-  double cA = (A * A * A - A) * hh * hh / 6.0, cB = (B * B * B - B) * hh * hh / 6.0;
+  const double kk = kt[j];
   double v[10];
-#pragma unroll
-  for (int f = 0; f < 10; ++f) {
-    const double* y = pt + ((size_t)LOS_FIELDS[f] * n_save + s) * nk;
-    const double* M = Mb + (size_t)f * nk * n_tau + s;
-    v[f] = A * y[lo] + B * y[lo + 1] + cA * M[(size_t)lo * n_tau] + cB * M[(size_t)(lo + 1) * n_tau];
-  }
-  double g = bgv[1 * n_tau + s], gp = bgv[2 * n_tau + s], aH = bgv[3 * n_tau + s], em = bgv[4 * n_tau + s],
-         aHd = bgv[5 * n_tau + s];
-  double wq = (s == 0 ? 0.5 * dl : dl) / aH;
-  double dg = v[0], tb = v[1], tbp = v[2], sg = v[3], g0 = v[4], g2 = v[5], eta = v[6], etap = v[7], al = v[8],
-         alp = v[9];
-  double sT0 = sw * g * (dg / 4. + aH * alp) +
-               isw * (g * (eta - aH * alp - 2. * aH * al) + 2. * em * (aH * etap - aHd * al - aH * aH * alp)) +
-               dop * (aH * (g * ((tbp / (kk * kk)) + alp) + gp * ((tb / (kk * kk)) + al)));
-  double sT1 = isw * em * ((aH * alp + 2. * aH * al - eta) * kk);
-  double sT2 = pol * g * (2 * sg + g0 + g2) / 8.;
-  double sE = sqrt(6.0) * g * (2 * sg + g0 + g2) / 8.;
+  los_spline(c, ws, Mb, n_tau, s, kk, v);
+  const double g = bgv[1 * n_tau + s], gp = bgv[2 * n_tau + s], aH = bgv[3 * n_tau + s], em = bgv[4 * n_tau + s],
+               aHd = bgv[5 * n_tau + s];
+  double out[4];
+  src_terms<double>(v, g, gp, aH, em, aHd, kk, s == 0 ? 0.5 * dl : dl, sw, isw, dop, pol, out);
   size_t o = (size_t)s * nkt + j, plane = (size_t)n_tau * nkt;
-  src[0 * plane + o] = wq * sT0;
-  src[1 * plane + o] = wq * sT1;
-  src[2 * plane + o] = wq * sT2;
-  src[3 * plane + o] = wq * sE;
+  src[0 * plane + o] = out[0];
+  src[1 * plane + o] = out[1];
+  src[2 * plane + o] = out[2];
+  src[3 * plane + o] = out[3];
 }
 
 // ---- Bessel lookups (abcmb/spectrum.py:48-109) ----
@@ -567,6 +606,260 @@ __global__ void pk_kernel(const AbcmbCosmo* __restrict__ cosmos, const double* _
   out[i] = d * d * (A_s * pow(x / kp, n_s - 1.) * (2 * PI * PI / (x * x * x)));
 }
 
+
+// ---- reverse mode of the sources, the LOS and the C_l quadrature (abcmb_los_adjoint) ----
+// From the cotangents cb[3][n_ell] of the sampled C_l (TT, TE, EE):
+//   lamd_kernel: per (l, k) the cotangents of Delta_T = D0 + D1 + D2 and
+//     Delta_E = D3 from the k trapezoid with the primordial spectrum
+//     (C_TT = sum_k w P T^2, C_TE = w P T E, C_EE = w P E^2), and the partial
+//     sums of the A_s and n_s cotangents;
+//   los_adj_kernel: per (k, saved point) the contraction over l: the sources'
+//     cotangents sum_l lamd phi_l(k chi) and the distance's sum_l lamd S
+//     phi_l'(k chi) k (the Bessel lookups differentiated as the tangent-linear
+//     kernel does), then the sources' cotangents pulled back through the
+//     source functions (one-direction duals of src_terms) to the ten spline
+//     values (lamv) and to the LOS background at that point (partial sums per
+//     block over k, with the distance's);
+//   spline_adj_kernel: per (field, saved point) the spline in log10 k
+//     transposed: the interpolation weights, the not-a-knot end rows and the
+//     tridiagonal solve (its LU factors transposed), into the table cotangent.
+template <int NTH>
+__global__ void __launch_bounds__(NTH) lamd_kernel(const AbcmbCosmo* __restrict__ cosmos,
+                                                   const double* __restrict__ delta_all,
+                                                   const double* __restrict__ kt, int nkt,
+                                                   const double* __restrict__ cb_all, int n_ell,
+                                                   double* __restrict__ lamd_all, double* __restrict__ part_all) {
+  __shared__ double red[2][NTH / 64];
+  const int e = blockIdx.y, ci = blockIdx.z;
+  const int j = blockIdx.x * NTH + threadIdx.x;
+  const AbcmbCosmo& cc = cosmos[ci];
+  const double* delta = delta_all + (size_t)ci * 4 * n_ell * nkt;
+  const double* cb = cb_all + (size_t)ci * 3 * n_ell;
+  double* lamd = lamd_all + (size_t)ci * 2 * n_ell * nkt;
+  double qa = 0.0, qn = 0.0;
+  if (j < nkt) {
+    const double Tv = delta[((size_t)0 * n_ell + e) * nkt + j] + delta[((size_t)1 * n_ell + e) * nkt + j] +
+                      delta[((size_t)2 * n_ell + e) * nkt + j];
+    const double Ev = delta[((size_t)3 * n_ell + e) * nkt + j];
+    const double kd = kt[j];
+    const double wj = 0.5 * ((j + 1 < nkt ? kt[j + 1] : kd) - (j > 0 ? kt[j - 1] : kd));
+    const double P = 4. * PI * cc.A_s * pow(kd / cc.k_pivot, cc.n_s - 1.) / kd;
+    const double lt = cb[e], lte = cb[n_ell + e], le = cb[2 * n_ell + e];
+    lamd[(size_t)e * nkt + j] = wj * P * (2.0 * Tv * lt + Ev * lte);
+    lamd[((size_t)n_ell + e) * nkt + j] = wj * P * (Tv * lte + 2.0 * Ev * le);
+    const double q = wj * P * (lt * Tv * Tv + lte * Tv * Ev + le * Ev * Ev);
+    qa = q / cc.A_s;
+    qn = q * log(kd / cc.k_pivot);
+  }
+  double v[2] = {qa, qn};
+  wave_sum(v);
+  const int wv = threadIdx.x >> 6;
+  if ((threadIdx.x & 63) == 0) { red[0][wv] = v[0]; red[1][wv] = v[1]; }
+  __syncthreads();
+  if (threadIdx.x < 2) {
+    double t = 0.0;
+    for (int w = 0; w < NTH / 64; ++w) t += red[threadIdx.x][w];
+    part_all[(((size_t)ci * 2 + threadIdx.x) * n_ell + e) * gridDim.x + blockIdx.x] = t;
+  }
+}
+
+template <int NTH>
+__global__ void __launch_bounds__(NTH) los_adj_kernel(
+    const AbcmbCosmo* __restrict__ cosmos, int n_save, size_t ws_stride, const double* __restrict__ ws_all,
+    size_t m_stride, const double* __restrict__ M_all, const double* __restrict__ src_all,
+    const double* __restrict__ bgv_all, const double* __restrict__ kt, int nkt, double sw, double isw, double dop,
+    double pol, const double* __restrict__ bes, const double* __restrict__ rng, const int32_t* __restrict__ bes_l,
+    int n_lt, int nx, const int32_t* __restrict__ ell_idx, int n_ell, const double* __restrict__ lamd_all,
+    double* __restrict__ lamv_all, double* __restrict__ part_all) {
+  constexpr int NW = NTH / 64;
+  __shared__ double red[6][NW];
+  const int n_tau = n_save - 1;
+  const int s = blockIdx.y, ci = blockIdx.z;
+  const int j0 = blockIdx.x * NTH + threadIdx.x;
+  const bool live = j0 < nkt;
+  const int j = live ? j0 : nkt - 1;
+  const AbcmbCosmo& c = cosmos[ci];
+  const size_t plane = (size_t)n_tau * nkt;
+  const double* src = src_all + (size_t)ci * 4 * plane;
+  const double* bgv = bgv_all + (size_t)ci * 6 * n_tau;
+  const double* lamd = lamd_all + (size_t)ci * 2 * n_ell * nkt;
+  const double kk = kt[j];
+  const double x = (c.tau0 - bgv[s]) * kk;
+  const size_t o = (size_t)s * nkt + j;
+  const double S0 = src[o], S1 = src[plane + o], S2 = src[2 * plane + o], S3 = src[3 * plane + o];
+  double ls[4] = {0.0, 0.0, 0.0, 0.0};
+  double lx = 0.0;
+  for (int e = 0; e < n_ell; ++e) {
+    const int li = ell_idx[e];
+    const double l = bes_l[li];
+    const double eps = sqrt(3. / 8. * (l + 2.) * (l + 1.) * l * (l - 1.));
+    double lo[3], hi[3];
+    const double* tb[3];
+#pragma unroll
+    for (int w = 0; w < 3; ++w) {
+      tb[w] = bes + ((size_t)w * n_lt + li) * nx;
+      lo[w] = rng[((size_t)w * n_lt + li) * 2];
+      hi[w] = rng[((size_t)w * n_lt + li) * 2 + 1];
+    }
+    const bool a0 = x >= hi[0], a1 = x >= hi[1], a2 = x >= hi[2];
+    double jl = 0, djl = 0, jl1 = 0, djl1 = 0;
+    if (a0 || a1 || a2) j_asym_d<double>(l, x, jl, djl);
+    if (a1 || a2) j_asym_d<double>(l + 1.0, x, jl1, djl1);
+    double p0 = 0, q0 = 0, p1 = 0, q1 = 0, p2 = 0, q2 = 0;
+    if (x >= lo[0]) {
+      if (a0) { p0 = jl; q0 = djl; }
+      else table_d<double>(x, lo[0], hi[0], tb[0], nx, p0, q0);
+    }
+    if (x >= lo[1]) {
+      if (a1) { p1 = l / x * jl - jl1; q1 = -l / (x * x) * jl + l / x * djl - djl1; }
+      else table_d<double>(x, lo[1], hi[1], tb[1], nx, p1, q1);
+    }
+    if (x >= lo[2]) {
+      if (a2) {
+        const double c3 = 3. * l * (l - 1.) - 2. * x * x;
+        const double Nn = c3 * jl + 6. * x * jl1;
+        const double dN = -4. * x * jl + c3 * djl + 6. * jl1 + 6. * x * djl1;
+        p2 = Nn / 2. / (x * x);
+        q2 = dN / (2. * x * x) - Nn / (x * x * x);
+      } else {
+        table_d<double>(x, lo[2], hi[2], tb[2], nx, p2, q2);
+      }
+    }
+    const double psi = p0 / (x * x) * eps;
+    const double dpsi = eps * (q0 / (x * x) - 2. * p0 / (x * x * x));
+    const double lT = lamd[(size_t)e * nkt + j], lE = lamd[((size_t)n_ell + e) * nkt + j];
+    ls[0] += lT * p0;
+    ls[1] += lT * p1;
+    ls[2] += lT * p2;
+    ls[3] += lE * psi;
+    lx += lT * (S0 * q0 + S1 * q1 + S2 * q2) + lE * S3 * dpsi;
+  }
+  // the source functions backwards: ten spline values, then the background
+  const double* ws = ws_all + (size_t)ci * ws_stride;
+  const double* Mb = M_all + (size_t)ci * m_stride;
+  double v[10];
+  los_spline(c, ws, Mb, n_tau, s, kk, v);
+  const double dl = c.lna_save[n_save - 1] - c.lna_save[n_save - 2];
+  const double wdl = s == 0 ? 0.5 * dl : dl;
+  double bg[5] = {bgv[1 * n_tau + s], bgv[2 * n_tau + s], bgv[3 * n_tau + s], bgv[4 * n_tau + s],
+                  bgv[5 * n_tau + s]};
+  double* lamv = lamv_all + (size_t)ci * 10 * plane;
+  double red_v[6];
+#pragma unroll 1
+  for (int d = 0; d < 15; ++d) {
+    D<1> vd[10], bd[5], out[4];
+#pragma unroll
+    for (int f = 0; f < 10; ++f) { vd[f] = D<1>(v[f]); vd[f].d[0] = f == d ? 1.0 : 0.0; }
+#pragma unroll
+    for (int q = 0; q < 5; ++q) { bd[q] = D<1>(bg[q]); bd[q].d[0] = 10 + q == d ? 1.0 : 0.0; }
+    src_terms<D<1>>(vd, bd[0], bd[1], bd[2], bd[3], bd[4], kk, wdl, sw, isw, dop, pol, out);
+    const double dv = live ? ls[0] * out[0].d[0] + ls[1] * out[1].d[0] + ls[2] * out[2].d[0] + ls[3] * out[3].d[0]
+                           : 0.0;
+    if (d < 10) {
+      if (live) lamv[(size_t)d * plane + o] = dv;
+    } else {
+      red_v[d - 10] = dv;
+    }
+  }
+  red_v[5] = live ? lx * kk : 0.0;   // the distance chi = tau0 - tau(s)
+  wave_sum(red_v);
+  const int wv = threadIdx.x >> 6;
+  if ((threadIdx.x & 63) == 0) {
+#pragma unroll
+    for (int q = 0; q < 6; ++q) red[q][wv] = red_v[q];
+  }
+  __syncthreads();
+  if (threadIdx.x < 6) {
+    double t = 0.0;
+    for (int w = 0; w < NW; ++w) t += red[threadIdx.x][w];
+    part_all[(((size_t)ci * 6 + threadIdx.x) * n_tau + s) * gridDim.x + blockIdx.x] = t;
+  }
+}
+
+// one thread per (saved point, field, cosmology)
+__global__ void spline_adj_kernel(const AbcmbCosmo* __restrict__ cosmos, int n_save, size_t ws_stride,
+                                  const double* __restrict__ ws_all, const double* __restrict__ kt, int nkt,
+                                  const double* __restrict__ lamv_all, size_t lm_stride, double* __restrict__ lm_all,
+                                  double* const* __restrict__ ptbar) {
+  const int n_tau = n_save - 1;
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  const int f = blockIdx.y, ci = blockIdx.z;
+  if (s >= n_tau) return;
+  const AbcmbCosmo& c = cosmos[ci];
+  const int nk = c.nk;
+  if (nk < 4) return;   // the not-a-knot rows need two intervals at each end
+  const double* ws = ws_all + (size_t)ci * ws_stride;
+  const double* x = ws;
+  const double* h = x + nk;
+  const double* w = h + nk;
+  const double* bp = w + nk;
+  const double* cc = bp + nk;
+  const double* lv = lamv_all + ((size_t)ci * 10 + f) * n_tau * nkt + (size_t)s * nkt;
+  double* yb = ptbar[ci] + ((size_t)LOS_FIELDS[f] * n_save + s) * nk;
+  double* Lm = lm_all + (size_t)ci * lm_stride + ((size_t)f * n_tau + s) * nk;
+  for (int i = 0; i < nk; ++i) Lm[i] = 0.0;
+  // the interpolation weights (the interval walks forward with k)
+  int lo = 0;
+  for (int jj = 0; jj < nkt; ++jj) {
+    const double xq = log10(kt[jj]);
+    while (lo < nk - 2 && x[lo + 1] <= xq) ++lo;
+    const double hh = x[lo + 1] - x[lo];
+    const double A = (x[lo + 1] - xq) / hh, B = (xq - x[lo]) / hh;
+    const double cA = (A * A * A - A) * hh * hh / 6.0, cB = (B * B * B - B) * hh * hh / 6.0;
+    const double l = lv[jj];
+    yb[lo] += A * l;
+    yb[lo + 1] += B * l;
+    Lm[lo] += cA * l;
+    Lm[lo + 1] += cB * l;
+  }
+  // M_0 and M_{nk-1} from the not-a-knot rows onto the interior unknowns z_r = M_{r+1}
+  const int m = nk - 2;
+  const double e0 = Lm[0], e1 = Lm[nk - 1];
+  for (int r = 0; r < m; ++r) Lm[r] = Lm[r + 1];
+  Lm[0] += (1.0 + h[0] / h[1]) * e0;
+  Lm[1] += -h[0] / h[1] * e0;
+  Lm[m - 1] += (1.0 + h[nk - 2] / h[nk - 3]) * e1;
+  Lm[m - 2] += -h[nk - 2] / h[nk - 3] * e1;
+  // the tridiagonal solve transposed: U^T q = zbar (forward), L^T dbar = q (backward)
+  Lm[0] /= bp[0];
+  for (int r = 1; r < m; ++r) Lm[r] = (Lm[r] - cc[r - 1] * Lm[r - 1]) / bp[r];
+  for (int r = m - 2; r >= 0; --r) Lm[r] -= w[r + 1] * Lm[r + 1];
+  // d_r = 6 ((y_{i+1} - y_i) / h_i - (y_i - y_{i-1}) / h_{i-1}), i = r + 1
+  for (int r = 0; r < m; ++r) {
+    const int i = r + 1;
+    const double db = 6.0 * Lm[r];
+    yb[i + 1] += db / h[i];
+    yb[i] -= db * (1.0 / h[i] + 1.0 / h[i - 1]);
+    yb[i - 1] += db / h[i - 1];
+  }
+}
+
+// bgv_bar[B][6][n_tau] (row 0 the cotangent of tau(s) = -that of chi(s)) and
+// scal_bar[B][3] = (A_s, n_s, tau0) from the partial sums
+__global__ void los_adj_final_kernel(const double* __restrict__ part6, int nch6, const double* __restrict__ part2,
+                                     int nch2, int n_tau, int n_ell, double* __restrict__ bgv_bar,
+                                     double* __restrict__ scal_bar) {
+  const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+  const int ci = blockIdx.y;
+  if (idx < 6 * n_tau) {
+    const int q = idx / n_tau, s = idx % n_tau;
+    const double* p = part6 + (((size_t)ci * 6 + (q == 0 ? 5 : q - 1)) * n_tau + s) * nch6;
+    double t = 0.0;
+    for (int b = 0; b < nch6; ++b) t += p[b];
+    bgv_bar[(size_t)ci * 6 * n_tau + idx] = q == 0 ? -t : t;
+  } else if (idx < 6 * n_tau + 3) {
+    const int q = idx - 6 * n_tau;
+    double t = 0.0;
+    if (q < 2) {
+      const double* p = part2 + ((size_t)ci * 2 + q) * n_ell * nch2;
+      for (int b = 0; b < n_ell * nch2; ++b) t += p[b];
+    } else {
+      const double* p = part6 + ((size_t)ci * 6 + 5) * n_tau * nch6;
+      for (int b = 0; b < n_tau * nch6; ++b) t += p[b];
+    }
+    scal_bar[(size_t)ci * 3 + q] = t;
+  }
+}
 }  // namespace
 }  // namespace abcmb
 
@@ -702,4 +995,75 @@ extern "C" int abcmb_pk(const AbcmbCosmo* cosmos_dev, int n_cosmo, const double*
 extern "C" int abcmb_pk_lin(const AbcmbCosmo* cosmos_dev, int n_cosmo, const double* kq_dev, int nq, double* pk_dev,
                             void* stream) {
   return abcmb_pk(cosmos_dev, n_cosmo, kq_dev, nq, 0.0, ABCMB_PK_M, pk_dev, stream);
+}
+
+namespace {
+constexpr int LOS_A_NTH = 256;
+struct LosAdjParts {
+  size_t src_ws, src, delta, lpart, lcl, lamd, part2, lamv, part6, lm, total;
+};
+LosAdjParts los_adj_parts(int n_cosmo, int nk_max, int n_save, int n_ell, int nkt) {
+  LosAdjParts P;
+  const size_t B = n_cosmo, n_tau = n_save - 1, nch = (nkt + LOS_A_NTH - 1) / LOS_A_NTH;
+  size_t o = 0;
+  P.src_ws = o; o += abcmb_sources_workspace_size(n_cosmo, nk_max, n_save) / sizeof(double);
+  P.src = o; o += B * 4 * n_tau * nkt;
+  P.delta = o; o += B * 4 * n_ell * nkt;
+  P.lpart = o; o += abcmb_los_workspace_size(n_cosmo, n_ell, nkt) / sizeof(double);
+  P.lcl = o; o += B * 3 * n_ell;
+  P.lamd = o; o += B * 2 * n_ell * nkt;
+  P.part2 = o; o += B * 2 * n_ell * nch;
+  P.lamv = o; o += B * 10 * n_tau * nkt;
+  P.part6 = o; o += B * 6 * n_tau * nch;
+  P.lm = o; o += B * 10 * n_tau * nk_max;
+  P.total = o * sizeof(double);
+  return P;
+}
+}  // namespace
+
+extern "C" size_t abcmb_los_adjoint_workspace_size(int n_cosmo, int nk_max, int n_save, int n_ell, int nkt) {
+  if (n_cosmo <= 0 || nk_max <= 0 || n_save < 3 || n_ell <= 0 || nkt < 2) return 0;
+  return los_adj_parts(n_cosmo, nk_max, n_save, n_ell, nkt).total;
+}
+
+extern "C" int abcmb_los_adjoint(const AbcmbCosmo* cosmos_dev, int n_cosmo, int nk_max, int n_save,
+                                 const double* bgv_dev, const double* kt_dev, int nkt, const double* scales4,
+                                 const double* bes_dev, const double* bes_rng_dev, const int32_t* bes_l_dev, int n_lt,
+                                 int nx, const int32_t* ell_idx_dev, int n_ell, const double* cl_bar_dev,
+                                 double* const* pt_bar_dev, double* bgv_bar_dev, double* scal_bar_dev,
+                                 void* workspace, size_t ws_bytes, void* stream) {
+  ABCMB_REQUIRE(cosmos_dev && bgv_dev && kt_dev && bes_dev && bes_rng_dev && bes_l_dev && ell_idx_dev && cl_bar_dev &&
+                    pt_bar_dev && bgv_bar_dev && scal_bar_dev && workspace, "null argument");
+  ABCMB_REQUIRE(n_cosmo > 0 && nk_max >= 5 && n_save >= 3 && nkt >= 2 && n_lt > 0 && nx > 1 && n_ell > 0,
+                "bad sizes");
+  const LosAdjParts P = los_adj_parts(n_cosmo, nk_max, n_save, n_ell, nkt);
+  ABCMB_REQUIRE(ws_bytes >= P.total, "workspace too small (%zu < %zu)", ws_bytes, P.total);
+  double sc[4] = {1, 1, 1, 1};
+  if (scales4) for (int i = 0; i < 4; ++i) sc[i] = scales4[i];
+  hipStream_t s = (hipStream_t)stream;
+  double* ws = (double*)workspace;
+  const int n_tau = n_save - 1;
+  const size_t wss = src_ws_stride(nk_max), ms = src_m_stride(nk_max, n_save);
+  double* spl = ws + P.src_ws;
+  double* Mb = spl + (size_t)n_cosmo * wss;
+  // the forward sources and LOS with the transfer functions kept
+  int rc = abcmb_sources(cosmos_dev, n_cosmo, nk_max, n_save, bgv_dev, kt_dev, nkt, sc, ws + P.src, spl,
+                         abcmb_sources_workspace_size(n_cosmo, nk_max, n_save), stream);
+  if (rc != ABCMB_OK) return rc;
+  rc = abcmb_los_cl(cosmos_dev, n_cosmo, ws + P.src, n_tau, bgv_dev, kt_dev, nkt, bes_dev, bes_rng_dev, bes_l_dev,
+                    n_lt, nx, ell_idx_dev, n_ell, 0, ws + P.lcl, ws + P.delta, ws + P.lpart,
+                    abcmb_los_workspace_size(n_cosmo, n_ell, nkt), stream);
+  if (rc != ABCMB_OK) return rc;
+  const int nch = (nkt + LOS_A_NTH - 1) / LOS_A_NTH;
+  hipLaunchKernelGGL((lamd_kernel<LOS_A_NTH>), dim3(nch, n_ell, n_cosmo), dim3(LOS_A_NTH), 0, s, cosmos_dev,
+                     ws + P.delta, kt_dev, nkt, cl_bar_dev, n_ell, ws + P.lamd, ws + P.part2);
+  hipLaunchKernelGGL((los_adj_kernel<LOS_A_NTH>), dim3(nch, n_tau, n_cosmo), dim3(LOS_A_NTH), 0, s, cosmos_dev,
+                     n_save, wss, spl, ms, Mb, ws + P.src, bgv_dev, kt_dev, nkt, sc[0], sc[1], sc[2], sc[3], bes_dev,
+                     bes_rng_dev, bes_l_dev, n_lt, nx, ell_idx_dev, n_ell, ws + P.lamd, ws + P.lamv, ws + P.part6);
+  hipLaunchKernelGGL(spline_adj_kernel, dim3((n_tau + 63) / 64, 10, n_cosmo), dim3(64), 0, s, cosmos_dev, n_save, wss,
+                     spl, kt_dev, nkt, ws + P.lamv, (size_t)10 * n_tau * nk_max, ws + P.lm, pt_bar_dev);
+  hipLaunchKernelGGL(los_adj_final_kernel, dim3((6 * n_tau + 3 + 255) / 256, n_cosmo), dim3(256), 0, s,
+                     ws + P.part6, nch, ws + P.part2, nch, n_tau, n_ell, bgv_bar_dev, scal_bar_dev);
+  ABCMB_HIP_CHECK(hipGetLastError());
+  return ABCMB_OK;
 }

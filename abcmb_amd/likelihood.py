@@ -391,3 +391,123 @@ def tangent_gradients(model, like, thetas, k_grids=None, mock_index=None, phases
         phases.update(wait_inputs_s=t_host, enqueue_s=t_enq, gpu_drain_s=t_drain,
                       total_s=time.perf_counter() - t_start, chunk=chunk)
     return Lb, grad
+
+
+def _clpp_directional(model, thetas, steps, base_plans, pm, pts, kdev):
+    """d C_l^phiphi / d theta_p with the perturbation tables held at theta's
+    ([nc][7][n_l]): the Limber integral's host inputs (distances, aH, the
+    cosmology fields, the primordial spectrum) differenced between the plans
+    at theta +- step, on theta's own delta_m table."""
+    import torch
+    nc = thetas.shape[0]
+    plans, tabs, ks = [], [], []
+    for c in range(nc):
+        for p in range(7):
+            for si, sgn in enumerate((+1, -1)):
+                if p in ODE_DIRS:
+                    plans.append(pm[c][2 * ODE_DIRS.index(p) + si])
+                else:
+                    t = thetas[c].copy()
+                    t[p] += sgn * steps[c, p]
+                    q = theta_to_params(t)
+                    plans.append(with_primordial(base_plans[c], q["A_s"], q["n_s"]))
+                tabs.append(pts[c])
+                ks.append(kdev[c])
+    clpp = model.clpp_of(plans, tabs, ks).view(nc, 7, 2, -1)
+    h2 = torch.as_tensor(2.0 * steps, dtype=torch.float64, device=clpp.device)
+    return (clpp[:, :, 0] - clpp[:, :, 1]) / h2[:, :, None]
+
+
+def adjoint_gradients(model, like, thetas, k_grids=None, mock_index=None, chunk: int = 64, phases=None,
+                      cl_out=None):
+    """d L / d theta for each row of ``thetas`` by reverse mode on the GPU.
+
+    Per chunk of ``chunk`` cosmologies: one primal ODE launch and the
+    downstream give L; the cotangent of the observed spectra runs backwards
+    through the lensing, the l-spline, the sources/LOS/k-quadrature and the
+    Limber integral (``Model.downstream_adjoint``) to the cotangents of the
+    perturbation tables, and one discrete-adjoint ODE launch
+    (``Model.ode_adjoint``) takes those to the cotangents of the host inputs
+    (the cosmology fields and the tau, x_e, T_m tables).  The host inputs'
+    derivatives along the parameters are the central differences of the host
+    producer that the tangent-linear path seeds with (``tangent_seeds``, the
+    LOS background vectors and the Limber inputs of the same plans), so the
+    gradient is the contraction of each cotangent with them: the reference's
+    jax.vjp through the background and recombination, here by the host's
+    directional derivatives.  ``cl_out`` (optional list) receives the
+    observed spectra of each chunk.  ``phases`` (optional dict) receives the
+    time spent in each part.
+    """
+    import time
+    import torch
+    t_start = time.perf_counter()
+    thetas = np.atleast_2d(thetas)
+    nc = thetas.shape[0]
+    mock_index = list(range(nc)) if mock_index is None else list(mock_index)
+    k_grids = list(k_grids) if k_grids is not None else [None] * nc
+    steps = FD_REL_STEP * np.abs(thetas)
+    nd = len(ODE_DIRS)
+    from . import abi
+    f_tau0 = [f[0] for f in abi.AbcmbCosmo._fields_].index("tau0")
+    f_as = [f[0] for f in abi.AbcmbCosmo._fields_].index("A_s")
+    f_ns = [f[0] for f in abi.AbcmbCosmo._fields_].index("n_s")
+    Lb, grads = [], []
+    t_host = t_fwd = t_dsa = t_ode = t_asm = 0.0
+    for a in range(0, nc, max(1, chunk)):
+        b = min(nc, a + max(1, chunk))
+        t0 = time.perf_counter()
+        base_plans, pm, seeds = _tangent_inputs(model, thetas[a:b], k_grids[a:b], steps[a:b])
+        t1 = time.perf_counter()
+        run = model._run_ode(base_plans)
+        ds = model._downstream(base_plans, run["pts"], run["kdev"])
+        obs = model.observed_cl(ds).detach().clone().requires_grad_(True)
+        L = like.loglike_cl(obs, mock_index[a:b])
+        (obs_bar,) = torch.autograd.grad(L.sum(), obs)
+        if cl_out is not None:
+            cl_out.append(obs.detach())
+        t2 = time.perf_counter()
+        adj = model.downstream_adjoint(base_plans, run["pts"], run["kdev"], ds, obs_bar)
+        t3 = time.perf_counter()
+        cot = model.ode_adjoint(base_plans, adj["pt_bar"])
+        t4 = time.perf_counter()
+        n = b - a
+        dev = obs.device
+        clpp_d = None
+        if model.specs["lensing"]:
+            clpp_d = _clpp_directional(model, thetas[a:b], steps[a:b], base_plans, pm, run["pts"], run["kdev"])
+        g = np.zeros((n, 7))
+        bgv_bar = adj["bgv_bar"].cpu().numpy()
+        scal = adj["scal_bar"].cpu().numpy()
+        lp = (torch.einsum("cl,cpl->cp", adj["clpp_bar"], clpp_d).cpu().numpy()
+              if clpp_d is not None else np.zeros((n, 7)))
+        for i in range(n):
+            c = a + i
+            par = cot[i]["par"].cpu().numpy().copy()
+            par[f_tau0] += scal[i, 2]
+            par[f_as] += scal[i, 0]
+            par[f_ns] += scal[i, 1]
+            tau, xe, tm = (cot[i][key].cpu().numpy() for key in ("tau", "xe", "tm"))
+            sd = seeds[i]
+            for p in range(7):
+                if p in ODE_DIRS:
+                    d = ODE_DIRS.index(p)
+                    pp, pq = pm[i][2 * d], pm[i][2 * d + 1]
+                    dbgv = (pp.bgv - pq.bgv) / (2.0 * steps[c, p])
+                    g[i, p] = (par @ sd.par[d] + tau @ sd.tau[d] + xe @ sd.xe[d] + tm @ sd.tm[d] +
+                               np.sum(bgv_bar[i] * dbgv))
+                elif NAMES[p] == "ln10As":
+                    g[i, p] = par[f_as] * base_plans[i].bg.cosmo.A_s
+                else:
+                    g[i, p] = par[f_ns]
+            g[i] += lp[i]
+        Lb.append(L.detach().cpu().numpy())
+        grads.append(g)
+        t_host += t1 - t0
+        t_fwd += t2 - t1
+        t_dsa += t3 - t2
+        t_ode += t4 - t3
+        t_asm += time.perf_counter() - t4
+    if phases is not None:
+        phases.update(host_s=t_host, forward_s=t_fwd, downstream_adjoint_s=t_dsa, ode_adjoint_s=t_ode,
+                      assemble_s=t_asm, total_s=time.perf_counter() - t_start, chunk=chunk)
+    return np.concatenate(Lb), np.concatenate(grads)

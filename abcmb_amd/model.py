@@ -804,6 +804,107 @@ class Model:
                                                out[c0].data_ptr(), stream), "abcmb_apply_operator")
         return out
 
+    def downstream_adjoint(self, plans, pts, kdev, ds, obs_bar):
+        """Reverse mode of ``_downstream`` for ``plans`` with tables ``pts``.
+
+        ``ds``: the forward result of ``_downstream`` on the same inputs;
+        ``obs_bar``: [B][3][n_l] cotangents of ``observed_cl(ds)``.  The lensing
+        (``abcmb_lens_adjoint``), the l-spline (its operator transposed), the
+        sources, LOS and k-quadrature (``abcmb_los_adjoint``) and the table part
+        of C_l^phiphi (``abcmb_lensing_clpp_adjoint``) run backwards on the GPU.
+        Returns dict(pt_bar=[21][n_save][nk] per plan (dL/d table), bgv_bar
+        [B][6][n_save-1] (dL/d LOS background vectors), scal_bar [B][3] (dL/d
+        A_s, n_s, tau0 through the unlensed C_l), clpp_bar [B][n_l] (dL/d
+        C_l^phiphi; None without lensing)).
+        """
+        torch, dev, stage = self._staging()
+        dt = stage.add
+        lib = abi.hip_lib()
+        D = self._device_constants()
+        stream = torch.cuda.current_stream().cuda_stream
+        f64 = dict(dtype=torch.float64, device=dev)
+        ncos, nl, n_l = len(plans), self.lensing_ells.size, self.ells.size
+        ns, nkt, n_ell = self.n_save, self.k_transfer.size, self.ell_idx.size
+        n_tau = ns - 1
+        nkmax = max(int(k.numel()) for k in kdev)
+        lensing = bool(self.specs["lensing"])
+        inp = self._downstream_inputs(plans, pts, kdev, dt)
+        pt_bar = [torch.zeros_like(pt) for pt in pts]
+        ptrs = dt(np.array([p.data_ptr() for p in pt_bar], dtype=np.int64), torch.int64)
+        if "ell_op_t" not in D:
+            D["ell_op_t"] = torch.as_tensor(np.ascontiguousarray(self.ell_op.T), **f64)
+        stage.flush()
+        obs_bar = obs_bar.to(torch.float64).contiguous()
+        cl_full_bar = torch.zeros((ncos, 3, nl), **f64)
+        clpp_bar = None
+        csz = C.sizeof(abi.AbcmbCosmo)
+        chunk = max(1, min(ncos, self.chunk))
+        cos_dev = inp["cos"]
+        if lensing:
+            lensed_bar = torch.zeros((ncos, 4, nl), **f64)
+            lensed_bar[:, :3, :n_l] = obs_bar
+            clpp_bar = torch.empty((ncos, nl), **f64)
+            ws = torch.empty(lib.abcmb_lens_adjoint_workspace_size(chunk, self.mu.size, nl) // 8 + 1, **f64)
+            cl_full, clpp = ds["cl_full"].contiguous(), ds["clpp"].contiguous()
+            n_lens = inp["lens"].shape[2]
+            for c0 in range(0, ncos, chunk):
+                nb = min(chunk, ncos - c0)
+                abi.check(lib.abcmb_lens_adjoint(nb, cl_full[c0].data_ptr(), clpp[c0].data_ptr(), nl,
+                                                 D["mu"].data_ptr(), D["w"].data_ptr(), self.mu.size,
+                                                 lensed_bar[c0].data_ptr(), cl_full_bar[c0].data_ptr(),
+                                                 clpp_bar[c0].data_ptr(), ws.data_ptr(), ws.numel() * 8, stream),
+                          "abcmb_lens_adjoint")
+            del ws
+            abi.check(lib.abcmb_lensing_clpp_adjoint(cos_dev.data_ptr(), ncos, inp["lens"].data_ptr(), n_lens,
+                                                     D["lens_ells"].data_ptr(), nl, clpp_bar.data_ptr(),
+                                                     ptrs.data_ptr(), stream), "abcmb_lensing_clpp_adjoint")
+        else:
+            cl_full_bar[:, :, :n_l] = obs_bar
+        cl_raw_bar = torch.empty((ncos, 3, n_ell), **f64)
+        abi.check(lib.abcmb_apply_operator(D["ell_op_t"].data_ptr(), n_ell, nl, cl_full_bar.data_ptr(), 3 * ncos,
+                                           cl_raw_bar.data_ptr(), stream), "abcmb_apply_operator")
+        bgv_bar = torch.empty((ncos, 6, n_tau), **f64)
+        scal_bar = torch.empty((ncos, 3), **f64)
+        # the LOS adjoint holds ~10 n_save (n_kt + nk) doubles per cosmology: groups of a few
+        per = lib.abcmb_los_adjoint_workspace_size(1, nkmax, ns, n_ell, nkt)
+        group = int(max(1, min(chunk, (4 << 30) // max(per, 1))))
+        ws = torch.empty(lib.abcmb_los_adjoint_workspace_size(group, nkmax, ns, n_ell, nkt) // 8 + 1, **f64)
+        sc = self.scales.ctypes.data_as(C.c_void_p)
+        for c0 in range(0, ncos, group):
+            nb = min(group, ncos - c0)
+            abi.check(lib.abcmb_los_adjoint(cos_dev.data_ptr() + c0 * csz, nb, nkmax, ns, inp["bgv"][c0].data_ptr(),
+                                            D["kt"].data_ptr(), nkt, sc, D["bes"].data_ptr(), D["rng"].data_ptr(),
+                                            D["bes_l"].data_ptr(), self.bes_l.size, self.bes_tab.shape[2],
+                                            D["ell_idx"].data_ptr(), n_ell, cl_raw_bar[c0].data_ptr(),
+                                            ptrs[c0:].data_ptr(), bgv_bar[c0].data_ptr(), scal_bar[c0].data_ptr(),
+                                            ws.data_ptr(), ws.numel() * 8, stream), "abcmb_los_adjoint")
+        del ws
+        return dict(pt_bar=pt_bar, bgv_bar=bgv_bar, scal_bar=scal_bar, clpp_bar=clpp_bar, keep=(stage, ptrs, inp))
+
+    def clpp_of(self, plans, pts, kdev):
+        """C_l^phiphi [B][n_l] of ``plans`` with the delta_m fields of tables ``pts``
+        (any plan may share a table: the host inputs move, the table does not)."""
+        torch, dev, stage = self._staging()
+        dt = stage.add
+        lib = abi.hip_lib()
+        D = self._device_constants()
+        stream = torch.cuda.current_stream().cuda_stream
+        ncos, nl = len(plans), self.lensing_ells.size
+        cos_dev = self._struct_array(torch, dev, self._cosmo_array(plans, pts, kdev, dt, tables=False), dt)
+        n_lens = plans[0].lens_lna.size
+        lens_h = np.zeros((ncos, 4, n_lens))
+        for i, pl in enumerate(plans):
+            lens_h[i, 0], lens_h[i, 1], lens_h[i, 2] = pl.lens_lna, pl.lens_aH, pl.lens_tau
+            lens_h[i, 3, 0] = pl.chi_rec
+        lens_all = dt(lens_h)
+        clpp = torch.empty((ncos, nl), dtype=torch.float64, device=dev)
+        stage.flush()
+        abi.check(lib.abcmb_lensing_clpp(cos_dev.data_ptr(), ncos, lens_all.data_ptr(), n_lens,
+                                         D["lens_ells"].data_ptr(), nl, clpp.data_ptr(), stream),
+                  "abcmb_lensing_clpp")
+        torch.cuda.current_stream().synchronize()
+        return clpp
+
     def lensed_from_cl(self, plans, pts, kdev, cl_full):
         """Lensed [B][4][n_l] from unlensed cl_full [B][3][n_l], with C_l^phiphi of ``plans`` / ``pts``."""
         torch, dev, stage = self._staging()
@@ -1006,7 +1107,7 @@ class Model:
         if bad.size:
             where = [(flat[b][1], float(allk[flat[b][1]][flat[b][2]])) for b in bad[:8]]
             raise RuntimeError(f"perturbation adjoint failed for {bad.size} (cosmology, k) items; status codes "
-                               f"{np.unique(st_h[bad]).tolist()} (5 = transposed stage solve did not converge); "
+                               f"{np.unique(st_h[bad]).tolist()} (5 = non-finite cotangent); "
                                f"first (cosmology, k): {where}")
         del stage, ws
         steps_ck = [np.zeros(a.size, dtype=np.int32) for a in allk]

@@ -37,7 +37,7 @@ extern "C" {
 #define ABCMB_K_NONFINITE 2   /* NaN/Inf in state or error norm */
 #define ABCMB_K_STEP_UNDERFLOW 3  /* step size fell below the spacing of ln a */
 #define ABCMB_K_BAD_INPUT 4   /* the cosmology has more saved points than n_save_max */
-#define ABCMB_K_ADJOINT 5     /* adjoint: a transposed stage solve did not converge */
+#define ABCMB_K_ADJOINT 5     /* adjoint: a non-finite cotangent */
 
 #define ABCMB_TAU_TAB_N 10000      /* tau(ln a) table on [-33, 0] */
 #define ABCMB_TAU_TAB_LO (-33.0)
@@ -278,8 +278,8 @@ int abcmb_pert_tangent(const AbcmbCosmo* cosmos_dev, const AbcmbTangent* tangent
  * from.  The step sizes are held fixed, as in abcmb_pert_tangent.
  * cot_dev: n_cosmo device-resident AbcmbCotangent.  tab_max: at least
  * ABCMB_TAU_TAB_N and every cosmology's xe_n and tm_n.  iters_dev: transposed
- * refinement iterations per item.  Item status ABCMB_K_ADJOINT: a transposed
- * stage solve did not converge (its cotangents are still accumulated).
+ * refinement iterations per item.  Item status ABCMB_K_ADJOINT: a cotangent
+ * became non-finite (what was accumulated before is kept).
  * Supports state sizes up to 192 with hierarchy blocks up to 32 rows.
  */
 size_t abcmb_pert_adjoint_workspace_size(int n_cosmo, int n_save_max, int tab_max,
@@ -360,6 +360,27 @@ int abcmb_los_tangent(const AbcmbCosmo* cosmos_dev, int n_cosmo, const double* s
                       const int32_t* ell_idx_dev, int n_ell, int precision, double* cl_dev,
                       void* workspace, size_t ws_bytes, void* stream);
 
+/*
+ * Reverse mode of abcmb_sources + abcmb_los_cl (FP64) and of the C_l
+ * k-quadrature: from the cotangents cl_bar [B][3][n_ell] of the sampled TT,
+ * TE, EE, the cotangents of the ten LOS fields of every perturbation table
+ * (added into pt_bar_dev[c], [ABCMB_N_PT_FIELDS][n_save][nk] device pointers;
+ * the caller zeroes them), of the LOS background vectors bgv_bar
+ * [B][6][n_save-1] (overwritten; same rows as bgv_dev) and scal_bar [B][3] =
+ * (A_s, n_s, tau0) (overwritten).  Replaces reverse-mode differentiation
+ * through SpectrumSolver.Cl_one_ell (abcmb/spectrum.py:616-797).  The
+ * forward sources and transfer functions are recomputed inside; the Bessel
+ * lookups are differentiated as abcmb_los_tangent does.  Every cosmology
+ * needs nk >= 4.
+ */
+size_t abcmb_los_adjoint_workspace_size(int n_cosmo, int nk_max, int n_save, int n_ell, int nkt);
+int abcmb_los_adjoint(const AbcmbCosmo* cosmos_dev, int n_cosmo, int nk_max, int n_save,
+                      const double* bgv_dev, const double* kt_dev, int nkt, const double* scales4,
+                      const double* bes_dev, const double* bes_rng_dev, const int32_t* bes_l_dev, int n_lt,
+                      int nx, const int32_t* ell_idx_dev, int n_ell, const double* cl_bar_dev,
+                      double* const* pt_bar_dev, double* bgv_bar_dev, double* scal_bar_dev,
+                      void* workspace, size_t ws_bytes, void* stream);
+
 /* Dense linear map out[m][r] = sum_c op[r][c] in[m][c] (the l-spline of
  * abcmb/spectrum.py:598-601 applied as a precomputed operator). */
 int abcmb_apply_operator(const double* op_dev, int rows, int cols,
@@ -397,6 +418,34 @@ int abcmb_lens(int n_cosmo, const double* cl_in_dev, const double* clpp_dev, int
  * units.  Arguments and outputs are those of abcmb_lens; the d-matrices must
  * be built for the same mu nodes, n_l and precision.
  */
+/*
+ * Reverse mode of abcmb_lens (vector or matrix-core path, FP64): from the
+ * cotangents lensed_bar [B][4][n_l] of the lensed TT, TE, EE, BB, the
+ * cotangents of the unlensed spectra cl_bar [B][3][n_l] and of C_l^phiphi
+ * clpp_bar [B][n_l] (both overwritten).  Replaces reverse-mode
+ * differentiation through SpectrumSolver.lensed_Cls (abcmb/spectrum.py:437-572).
+ * The back-transform and the Cgl pass are transposed as contractions over the
+ * nodes; the correlation pass is linear in the unlensed spectra and
+ * differentiated in sigma^2(mu) and Cgl2(mu) with dual numbers.
+ */
+size_t abcmb_lens_adjoint_workspace_size(int n_cosmo, int n_mu, int n_l);
+int abcmb_lens_adjoint(int n_cosmo, const double* cl_in_dev, const double* clpp_dev, int n_l,
+                       const double* mu_dev, const double* w_dev, int n_mu, const double* lensed_bar_dev,
+                       double* cl_bar_dev, double* clpp_bar_dev, void* workspace, size_t ws_bytes,
+                       void* stream);
+
+/*
+ * Reverse mode of abcmb_lensing_clpp with respect to the perturbation table:
+ * clpp_bar [B][n_l] into the ABCMB_PT_DELTA_M field of the table cotangents
+ * pt_bar_dev[c] ([ABCMB_N_PT_FIELDS][n_save][nk], device pointers, added to:
+ * the caller zeroes them).  Replaces the table part of reverse-mode
+ * differentiation through SpectrumSolver.Cl_phiphi (abcmb/spectrum.py:349-435);
+ * the background inputs (lens_dev, the cosmology fields) are held fixed.
+ */
+int abcmb_lensing_clpp_adjoint(const AbcmbCosmo* cosmos_dev, int n_cosmo, const double* lens_dev,
+                               int n_lna, const double* ells_dev, int n_l, const double* clpp_bar_dev,
+                               double* const* pt_bar_dev, void* stream);
+
 size_t abcmb_lens_dmat_size(int n_mu, int n_l, int precision);
 int abcmb_lens_dmat(const double* mu_dev, int n_mu, int n_l, int precision, void* dmat_dev,
                     size_t dmat_bytes, void* stream);
