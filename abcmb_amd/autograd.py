@@ -7,12 +7,21 @@ as a [n][3][n_l] tensor that takes part in torch's autograd: any loss built
 from it with torch operations back-propagates to ``theta``.  The reference gets
 the same from ``jax.grad`` through ``Model.__call__`` (abcmb/main.py:165-184).
 
-The forward pass runs ``tangent_gradients``' sequence (tangent-linear ODE and
-LOS launches, the lensing differenced along each parameter on the moved
-spectra) and keeps the Jacobian d C / d theta ([n][7][3][n_l], on the GPU);
-the backward pass is its transpose applied to the incoming cotangent, one
-contraction.  Cotangents are returned for ``theta`` only (the host tables'
-derivatives are not formed).
+``mode="adjoint"`` (default): the forward pass is one primal ODE launch and
+the downstream (``likelihood.adjoint_forward``); the backward pass is reverse
+mode on the GPU (``likelihood.adjoint_backward``): the lensing, l-spline,
+LOS and Limber adjoints, the discrete-adjoint ODE kernel (checkpoints every 64
+accepted steps, transposed stage solves), and the contraction of the host
+inputs' cotangents with their derivatives along the parameters.  The
+backward also leaves those cotangents (cosmology fields, tau / x_e / T_m
+tables, LOS background vectors, C_l^phiphi) in ``host_cotangents`` of the
+returned tensor's function node, for callers that chain them through their
+own background (the reference's jax.vjp).
+
+``mode="jacobian"``: the forward pass runs ``tangent_gradients``' sequence
+(tangent-linear ODE and LOS, the lensing differenced along each parameter)
+and keeps d C / d theta ([n][7][3][n_l]); the backward is its transpose
+applied to the cotangent, one contraction.
 """
 from __future__ import annotations
 
@@ -29,10 +38,10 @@ class _NoLoss:
         return torch.zeros(cl.shape[0], dtype=cl.dtype, device=cl.device)
 
 
-def _function():
+def _functions():
     import torch
 
-    class LensedCl(torch.autograd.Function):
+    class LensedClJacobian(torch.autograd.Function):
         @staticmethod
         def forward(ctx, theta, model, k_grids, chunk):
             th = np.ascontiguousarray(theta.detach().cpu().numpy(), dtype=np.float64)
@@ -50,19 +59,49 @@ def _function():
             g = torch.einsum("nxl,npxl->np", grad_cl.to(jac.device, jac.dtype), jac)
             return g.to(ctx.theta_device, ctx.theta_dtype), None, None, None
 
-    return LensedCl
+    class LensedClAdjoint(torch.autograd.Function):
+        @staticmethod
+        def forward(ctx, theta, model, k_grids, chunk):
+            th = np.ascontiguousarray(theta.detach().cpu().numpy(), dtype=np.float64)
+            ctx.model, ctx.chunk = model, max(1, chunk)
+            ctx.theta_device, ctx.theta_dtype = theta.device, theta.dtype
+            k_grids = list(k_grids) if k_grids is not None else [None] * th.shape[0]
+            ctx.states, obs = [], []
+            for a in range(0, th.shape[0], ctx.chunk):
+                st, o = LK.adjoint_forward(model, th[a:a + ctx.chunk], k_grids[a:a + ctx.chunk])
+                ctx.states.append(st)
+                obs.append(o.detach().clone())
+            ctx.host_cotangents = []
+            return torch.cat(obs).to(theta.device)
+
+        @staticmethod
+        def backward(ctx, grad_cl):
+            grads, a = [], 0
+            for st in ctx.states:
+                n = st["thetas"].shape[0]
+                gb = grad_cl[a:a + n].to("cuda", torch.float64).contiguous()
+                grads.append(LK.adjoint_backward(ctx.model, st, gb, cotangents_out=ctx.host_cotangents))
+                a += n
+            ctx.states = None
+            g = torch.as_tensor(np.concatenate(grads))
+            return g.to(ctx.theta_device, ctx.theta_dtype), None, None, None
+
+    return {"jacobian": LensedClJacobian, "adjoint": LensedClAdjoint}
 
 
-_LensedCl = None
+_FUNCS = None
 
 
-def lensed_cl(model, theta, k_grids=None, chunk: int = 64):
+def lensed_cl(model, theta, k_grids=None, chunk: int = 64, mode: str = "adjoint"):
     """Observed spectra of each row of ``theta`` ([n][7] tensor, float64), differentiable
     with respect to ``theta``; ``k_grids``: one perturbation k grid per row (default:
-    the model's)."""
-    global _LensedCl
-    if _LensedCl is None:
-        _LensedCl = _function()
+    the model's); ``mode``: "adjoint" (reverse mode on the GPU in the backward
+    pass) or "jacobian" (the forward-mode Jacobian kept from the forward pass)."""
+    global _FUNCS
+    if _FUNCS is None:
+        _FUNCS = _functions()
+    if mode not in _FUNCS:
+        raise ValueError(f"mode must be one of {tuple(_FUNCS)}")
     if theta.dim() != 2 or theta.shape[1] != len(LK.NAMES):
         raise ValueError(f"theta must be [n][{len(LK.NAMES)}] ({', '.join(LK.NAMES)})")
-    return _LensedCl.apply(theta, model, k_grids, chunk)
+    return _FUNCS[mode].apply(theta, model, k_grids, chunk)

@@ -135,12 +135,11 @@ def with_primordial(plan, A_s: float, n_s: float):
     return dataclasses.replace(plan, params=dict(plan.params, A_s=A_s, n_s=n_s), bg=bg)
 
 
-def _tangent_inputs(model, thetas, k_grids, steps):
-    """Host side of the tangent gradients: base plans, plans at theta +- step along
-    the ODE directions (grids frozen at the base), and the tangent seeds."""
+def _seed_inputs(model, thetas, base_plans, k_grids, steps):
+    """Plans at theta +- step along the ODE directions (grids frozen at the base
+    plans) and the tangent seeds of the host inputs from them."""
     from .model import tangent_seeds
     nc = thetas.shape[0]
-    base_plans = model.prepare_many([model.derived(theta_to_params(t)) for t in thetas], k_list=k_grids)
     pm_params, freeze, ks = [], [], []
     for c in range(nc):
         for p in ODE_DIRS:
@@ -154,6 +153,15 @@ def _tangent_inputs(model, thetas, k_grids, steps):
     nd = len(ODE_DIRS)
     pm = [pm[c * 2 * nd:(c + 1) * 2 * nd] for c in range(nc)]
     seeds = [tangent_seeds(pm[c][0::2], pm[c][1::2], [steps[c, p] for p in ODE_DIRS]) for c in range(nc)]
+    return pm, seeds
+
+
+def _tangent_inputs(model, thetas, k_grids, steps):
+    """Host side of the tangent gradients: base plans, plans at theta +- step along
+    the ODE directions (grids frozen at the base), and the tangent seeds."""
+    nc = thetas.shape[0]
+    base_plans = model.prepare_many([model.derived(theta_to_params(t)) for t in thetas], k_list=k_grids)
+    pm, seeds = _seed_inputs(model, thetas, base_plans, k_grids, steps)
     if getattr(model, "device_background", False):
         # recombination-only tables: the GPU adds the reionization, its
         # implicit-function derivatives replace the x_e and z_reion seeds
@@ -418,25 +426,89 @@ def _clpp_directional(model, thetas, steps, base_plans, pm, pts, kdev):
     return (clpp[:, :, 0] - clpp[:, :, 1]) / h2[:, :, None]
 
 
+def adjoint_forward(model, thetas, k_grids=None):
+    """The forward half of the reverse-mode gradient: plans, one primal ODE launch
+    and the downstream of every row of ``thetas``.  Returns a state for
+    ``adjoint_backward`` and the observed spectra [n][3][n_l]."""
+    if getattr(model, "device_background", False):
+        raise NotImplementedError("the reverse-mode gradient takes the host background (device_background=False)")
+    thetas = np.atleast_2d(np.asarray(thetas, dtype=np.float64))
+    k_grids = list(k_grids) if k_grids is not None else [None] * thetas.shape[0]
+    plans = model.prepare_many([model.derived(theta_to_params(t)) for t in thetas], k_list=k_grids)
+    run = model._run_ode(plans)
+    ds = model._downstream(plans, run["pts"], run["kdev"])
+    state = dict(thetas=thetas, k_grids=k_grids, plans=plans, run=run, ds=ds)
+    return state, model.observed_cl(ds)
+
+
+def adjoint_backward(model, state, obs_bar, cotangents_out=None):
+    """d L / d theta ([n][7]) for the cotangent ``obs_bar`` ([n][3][n_l]) of the
+    observed spectra of ``adjoint_forward``: the downstream backwards
+    (``Model.downstream_adjoint``) to the cotangents of the perturbation
+    tables, the discrete-adjoint ODE launch (``Model.ode_adjoint``) to those of
+    the host inputs (cosmology fields, tau / x_e / T_m tables), then their
+    contraction with the host inputs' derivatives along the parameters: the
+    tangent seeds of the producer, the LOS background vectors and the Limber
+    inputs of the plans at theta +- step (central differences of the host
+    producer, as the tangent-linear path seeds them).  ``cotangents_out``
+    (optional list) receives per cosmology the host-input cotangents before
+    that contraction (dict: par, tau, xe, tm, bgv, clpp)."""
+    thetas, plans, run, ds = state["thetas"], state["plans"], state["run"], state["ds"]
+    n = thetas.shape[0]
+    steps = FD_REL_STEP * np.abs(thetas)
+    pm, seeds = _seed_inputs(model, thetas, plans, state["k_grids"], steps)
+    adj = model.downstream_adjoint(plans, run["pts"], run["kdev"], ds, obs_bar)
+    cot = model.ode_adjoint(plans, adj["pt_bar"])
+    from . import abi
+    names = [f[0] for f in abi.AbcmbCosmo._fields_]
+    f_tau0, f_as, f_ns = names.index("tau0"), names.index("A_s"), names.index("n_s")
+    lp = np.zeros((n, 7))
+    if model.specs["lensing"]:
+        import torch
+        clpp_d = _clpp_directional(model, thetas, steps, plans, pm, run["pts"], run["kdev"])
+        lp = torch.einsum("cl,cpl->cp", adj["clpp_bar"], clpp_d).cpu().numpy()
+    bgv_bar = adj["bgv_bar"].cpu().numpy()
+    scal = adj["scal_bar"].cpu().numpy()
+    g = np.zeros((n, 7))
+    for i in range(n):
+        par = cot[i]["par"].cpu().numpy().copy()
+        par[f_tau0] += scal[i, 2]
+        par[f_as] += scal[i, 0]
+        par[f_ns] += scal[i, 1]
+        tau, xe, tm = (cot[i][key].cpu().numpy() for key in ("tau", "xe", "tm"))
+        if cotangents_out is not None:
+            cotangents_out.append(dict(par=par, tau=tau, xe=xe, tm=tm, bgv=bgv_bar[i],
+                                       clpp=adj["clpp_bar"][i].cpu().numpy() if adj["clpp_bar"] is not None
+                                       else None))
+        sd = seeds[i]
+        for p in range(7):
+            if p in ODE_DIRS:
+                d = ODE_DIRS.index(p)
+                dbgv = (pm[i][2 * d].bgv - pm[i][2 * d + 1].bgv) / (2.0 * steps[i, p])
+                g[i, p] = (par @ sd.par[d] + tau @ sd.tau[d] + xe @ sd.xe[d] + tm @ sd.tm[d] +
+                           np.sum(bgv_bar[i] * dbgv))
+            elif NAMES[p] == "ln10As":
+                g[i, p] = par[f_as] * plans[i].bg.cosmo.A_s
+            else:
+                g[i, p] = par[f_ns]
+        g[i] += lp[i]
+    return g
+
+
 def adjoint_gradients(model, like, thetas, k_grids=None, mock_index=None, chunk: int = 64, phases=None,
                       cl_out=None):
     """d L / d theta for each row of ``thetas`` by reverse mode on the GPU.
 
-    Per chunk of ``chunk`` cosmologies: one primal ODE launch and the
-    downstream give L; the cotangent of the observed spectra runs backwards
-    through the lensing, the l-spline, the sources/LOS/k-quadrature and the
-    Limber integral (``Model.downstream_adjoint``) to the cotangents of the
-    perturbation tables, and one discrete-adjoint ODE launch
-    (``Model.ode_adjoint``) takes those to the cotangents of the host inputs
-    (the cosmology fields and the tau, x_e, T_m tables).  The host inputs'
-    derivatives along the parameters are the central differences of the host
-    producer that the tangent-linear path seeds with (``tangent_seeds``, the
-    LOS background vectors and the Limber inputs of the same plans), so the
-    gradient is the contraction of each cotangent with them: the reference's
+    Per chunk of ``chunk`` cosmologies: ``adjoint_forward`` (one primal ODE
+    launch and the downstream) gives L; the cotangent of the observed spectra
+    (torch autograd through the likelihood) goes through ``adjoint_backward``:
+    the lensing, l-spline, sources/LOS/k-quadrature and Limber adjoints, one
+    discrete-adjoint ODE launch, and the contraction of the host inputs'
+    cotangents with their derivatives along the parameters -- the reference's
     jax.vjp through the background and recombination, here by the host's
     directional derivatives.  ``cl_out`` (optional list) receives the
-    observed spectra of each chunk.  ``phases`` (optional dict) receives the
-    time spent in each part.
+    observed spectra of each chunk; ``phases`` (optional dict) the time spent
+    in the forward and backward halves.
     """
     import time
     import torch
@@ -445,69 +517,23 @@ def adjoint_gradients(model, like, thetas, k_grids=None, mock_index=None, chunk:
     nc = thetas.shape[0]
     mock_index = list(range(nc)) if mock_index is None else list(mock_index)
     k_grids = list(k_grids) if k_grids is not None else [None] * nc
-    steps = FD_REL_STEP * np.abs(thetas)
-    nd = len(ODE_DIRS)
-    from . import abi
-    f_tau0 = [f[0] for f in abi.AbcmbCosmo._fields_].index("tau0")
-    f_as = [f[0] for f in abi.AbcmbCosmo._fields_].index("A_s")
-    f_ns = [f[0] for f in abi.AbcmbCosmo._fields_].index("n_s")
     Lb, grads = [], []
-    t_host = t_fwd = t_dsa = t_ode = t_asm = 0.0
+    t_fwd = t_bwd = 0.0
     for a in range(0, nc, max(1, chunk)):
         b = min(nc, a + max(1, chunk))
         t0 = time.perf_counter()
-        base_plans, pm, seeds = _tangent_inputs(model, thetas[a:b], k_grids[a:b], steps[a:b])
-        t1 = time.perf_counter()
-        run = model._run_ode(base_plans)
-        ds = model._downstream(base_plans, run["pts"], run["kdev"])
-        obs = model.observed_cl(ds).detach().clone().requires_grad_(True)
+        state, obs = adjoint_forward(model, thetas[a:b], k_grids[a:b])
+        obs = obs.detach().clone().requires_grad_(True)
         L = like.loglike_cl(obs, mock_index[a:b])
         (obs_bar,) = torch.autograd.grad(L.sum(), obs)
         if cl_out is not None:
             cl_out.append(obs.detach())
-        t2 = time.perf_counter()
-        adj = model.downstream_adjoint(base_plans, run["pts"], run["kdev"], ds, obs_bar)
-        t3 = time.perf_counter()
-        cot = model.ode_adjoint(base_plans, adj["pt_bar"])
-        t4 = time.perf_counter()
-        n = b - a
-        dev = obs.device
-        clpp_d = None
-        if model.specs["lensing"]:
-            clpp_d = _clpp_directional(model, thetas[a:b], steps[a:b], base_plans, pm, run["pts"], run["kdev"])
-        g = np.zeros((n, 7))
-        bgv_bar = adj["bgv_bar"].cpu().numpy()
-        scal = adj["scal_bar"].cpu().numpy()
-        lp = (torch.einsum("cl,cpl->cp", adj["clpp_bar"], clpp_d).cpu().numpy()
-              if clpp_d is not None else np.zeros((n, 7)))
-        for i in range(n):
-            c = a + i
-            par = cot[i]["par"].cpu().numpy().copy()
-            par[f_tau0] += scal[i, 2]
-            par[f_as] += scal[i, 0]
-            par[f_ns] += scal[i, 1]
-            tau, xe, tm = (cot[i][key].cpu().numpy() for key in ("tau", "xe", "tm"))
-            sd = seeds[i]
-            for p in range(7):
-                if p in ODE_DIRS:
-                    d = ODE_DIRS.index(p)
-                    pp, pq = pm[i][2 * d], pm[i][2 * d + 1]
-                    dbgv = (pp.bgv - pq.bgv) / (2.0 * steps[c, p])
-                    g[i, p] = (par @ sd.par[d] + tau @ sd.tau[d] + xe @ sd.xe[d] + tm @ sd.tm[d] +
-                               np.sum(bgv_bar[i] * dbgv))
-                elif NAMES[p] == "ln10As":
-                    g[i, p] = par[f_as] * base_plans[i].bg.cosmo.A_s
-                else:
-                    g[i, p] = par[f_ns]
-            g[i] += lp[i]
+        t1 = time.perf_counter()
+        grads.append(adjoint_backward(model, state, obs_bar))
         Lb.append(L.detach().cpu().numpy())
-        grads.append(g)
-        t_host += t1 - t0
-        t_fwd += t2 - t1
-        t_dsa += t3 - t2
-        t_ode += t4 - t3
-        t_asm += time.perf_counter() - t4
+        t_fwd += t1 - t0
+        t_bwd += time.perf_counter() - t1
+        del state
     if phases is not None:
-        phases.update(host_s=t_host, forward_s=t_fwd, downstream_adjoint_s=t_dsa, ode_adjoint_s=t_ode,
-                      assemble_s=t_asm, total_s=time.perf_counter() - t_start, chunk=chunk)
+        phases.update(forward_s=t_fwd, backward_s=t_bwd, total_s=time.perf_counter() - t_start, chunk=chunk)
     return np.concatenate(Lb), np.concatenate(grads)
