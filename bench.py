@@ -21,6 +21,8 @@ central finite differences, 15 full model evaluations per gradient, all in
 one ODE launch).
 
 Also reported: the launch shapes the autotuner picked (untimed, at setup), the
+reverse-mode gradient of the same batch (``adjoint_gradients``: downstream and
+ODE adjoints, one step; ``--no-adjoint`` skips it), the
 plain path (one cosmology per launch, FP64, central finite differences, the
 reference's tolerances) on a few of the same cosmologies, the gradient errors
 of the timed path, of the reference-tolerance path and of the plain path
@@ -203,6 +205,8 @@ def main():
     ap.add_argument("--fd-downstream", action="store_true",
                     help="difference the whole downstream along each direction instead of differentiating the "
                          "LOS in one tangent-linear pass")
+    ap.add_argument("--no-adjoint", dest="adjoint", action="store_false",
+                    help="skip the one-step timing of the reverse-mode (adjoint) gradient beside the headline")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="only run a device-to-device copy of BYTES (5 times): the reference for "
                          "scaling HBM byte counters")
@@ -325,6 +329,35 @@ def main():
     agree_rel = float(np.max(np.abs(gp - gb) / (np.abs(gp) + 1e-300)))
     print(f"# gradient error vs converged: timed {grad_err_conv:.2e}, reference tolerances {ref_err_conv:.2e}, "
           f"plain {plain_err_conv:.2e}", file=sys.stderr, flush=True)
+
+    # the reverse-mode gradient (downstream and ODE adjoints) on the same batch, one step:
+    # rate, the adjoint launch's time, agreement with the timed path and error vs converged
+    adjoint = None
+    if args.gradient == "tangent" and args.adjoint:
+        model.timing = []
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        ph_a = {}
+        _, g_adj = LK.adjoint_gradients(model, like, thetas, k_grids=k_grids, chunk=args.chunk, phases=ph_a)
+        torch.cuda.synchronize()
+        rate_a = args.batch / (time.perf_counter() - t0)
+        t_adj = [t for t in model.timing if t["kernel"] == "adjoint"]
+        t_pri = [t for t in model.timing if t["kernel"] == "primal"]
+        model.timing = None
+        scale_b = np.linalg.norm(grad * thetas, axis=1, keepdims=True) / np.abs(thetas)
+        adjoint = {"grad_evals_per_s": rate_a, "speedup_vs_plain": rate_a / plain,
+                   "ode_adjoint_ms_per_launch": sum(t["ms"] for t in t_adj) / max(1, len(t_adj)),
+                   "ode_primal_ms_per_launch": sum(t["ms"] for t in t_pri) / max(1, len(t_pri)),
+                   "launches": len(t_adj),
+                   "grad_diff_vs_timed_rel_to_norm": float(np.max(np.abs(g_adj - grad) / scale_b)),
+                   "grad_diff_vs_timed_median": float(np.median(np.max(np.abs(g_adj - grad) / scale_b, axis=1))),
+                   "grad_err_vs_converged": err_to_norm(g_adj[:np_], g_conv),
+                   "phases_s": ph_a,
+                   "method": "one primal ODE launch + downstream, then the lensing / l-spline / LOS / Limber "
+                             "adjoints and one discrete-adjoint ODE launch (checkpoints every 64 accepted "
+                             "steps, segments recomputed, transposed stage solves), contracted with the host "
+                             "seeds"}
+        print(f"# adjoint {adjoint}", file=sys.stderr, flush=True)
 
     # the reduced-precision LOS contractions in the same gradient path (one step of the
     # same batch each): rate, and spectra and gradients against the FP64 LOS
@@ -458,6 +491,7 @@ def main():
                    "reference_tolerances": {"grad_evals_per_s": ref_rate, "grad_err_vs_converged": ref_err_conv,
                                             "workload": f"the same {args.batch} cosmologies, one step"},
                    "large_batch": large,
+                   "adjoint": adjoint,
                    "los_variants": los_variants,
                    "los_variants_note": "the tangent-linear LOS with FP32 arithmetic, or FP32 accumulation of "
                                         "bfloat16-rounded operands, in the timed gradient path; errors against "
