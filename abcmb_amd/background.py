@@ -26,13 +26,18 @@ class Background:
 
     def __init__(self, params: dict, has_ncdm: bool, n_q: int = 3, k_pivot: float = 0.05,
                  input_tau_reion: bool = True, xe_pad: int = 0, rec_switch: int = -1,
-                 lna_rec: float = None, lna_transfer_start: float = None, reionization: bool = True):
+                 lna_rec: float = None, lna_transfer_start: float = None, reionization: bool = True,
+                 species_off: int = 0, fluid_par=None, lib=None):
         """``rec_switch``, ``lna_rec``, ``lna_transfer_start``: grid choices held at
         another cosmology's (finite differences with frozen grids); the producer
         then keeps them instead of searching for its own.  ``reionization=False``:
         the x_e table holds the recombination history alone and z_reion is 0
-        (``abcmb_reionize`` adds the reionization on the GPU)."""
-        lib = abi.host_lib()
+        (``abcmb_reionize`` adds the reionization on the GPU).  ``species_off``: the
+        AbcmbCosmo.species_off bits; ``fluid_par``: the user fluids' parameter slots;
+        ``lib``: the host producer built for the model's user fluids (default: the
+        standard one)."""
+        lib = lib if lib is not None else abi.host_lib()
+        self.lib = lib
         self.params = params
         n_rec = lib.abcmb_host_recomb_points()
         self.tau_tab = np.zeros(abi.TAU_TAB_N)
@@ -46,6 +51,9 @@ class Background:
         c.N_ncdm, c.T_ncdm, c.m_ncdm = p["N_nu_massive"], p["T_nu_massive"], p["m_nu_massive"]
         c.R_nu, c.R_b, c.om, c.omega_r, c.omega_m = p["R_nu"], p["R_b"], p["om"], p["omega_r"], p["omega_m"]
         c.has_ncdm = 1 if has_ncdm else 0
+        c.species_off = int(species_off)
+        if fluid_par is not None:
+            c.fluid_par[:] = [float(v) for v in fluid_par]
         c.n_q = n_q if has_ncdm else 0
         c.xe_n = c.tm_n = n_rec + xe_pad
         c.tau_tab = _ptr(self.tau_tab).value
@@ -85,7 +93,7 @@ class Background:
         """[6, n]: tau, g, g', aH, exp(-kappa), aH' aH on ``lna``."""
         lna = np.ascontiguousarray(lna, dtype=np.float64)
         out = np.zeros((6, lna.size))
-        rc = abi.host_lib().abcmb_host_los_background(C.byref(self.cosmo), _ptr(lna), lna.size, _ptr(out))
+        rc = self.lib.abcmb_host_los_background(C.byref(self.cosmo), _ptr(lna), lna.size, _ptr(out))
         if rc != 0:
             raise RuntimeError("abcmb_host_los_background failed")
         return out
@@ -93,7 +101,7 @@ class Background:
     def lna_start(self, k: np.ndarray, R_tc: float, R_large: float) -> np.ndarray:
         k = np.ascontiguousarray(k, dtype=np.float64)
         out = np.zeros(k.size)
-        rc = abi.host_lib().abcmb_host_lna_start(C.byref(self.cosmo), _ptr(k), k.size, R_tc, R_large, _ptr(out))
+        rc = self.lib.abcmb_host_lna_start(C.byref(self.cosmo), _ptr(k), k.size, R_tc, R_large, _ptr(out))
         if rc != 0:
             raise RuntimeError("abcmb_host_lna_start failed")
         return out
@@ -102,7 +110,7 @@ class Background:
         """aH, tau, tau_c, cs2, x_e, T_m, R at ``lna`` (host evaluation)."""
         lna = np.ascontiguousarray(np.atleast_1d(lna), dtype=np.float64)
         out = np.zeros((7, lna.size))
-        abi.host_lib().abcmb_host_eval(C.byref(self.cosmo), _ptr(lna), lna.size, _ptr(out))
+        self.lib.abcmb_host_eval(C.byref(self.cosmo), _ptr(lna), lna.size, _ptr(out))
         return dict(zip(("aH", "tau", "tau_c", "cs2", "xe", "Tm", "R"), out))
 
     def tau(self, lna):
@@ -115,7 +123,7 @@ class Background:
     def _drag(self):
         if getattr(self, "_drag_cache", None) is None:
             zd, rs = C.c_double(), C.c_double()
-            if abi.host_lib().abcmb_host_drag(C.byref(self.cosmo), C.byref(zd), C.byref(rs)) != 0:
+            if self.lib.abcmb_host_drag(C.byref(self.cosmo), C.byref(zd), C.byref(rs)) != 0:
                 raise RuntimeError("abcmb_host_drag failed")
             self._drag_cache = (zd.value, rs.value)
         return self._drag_cache

@@ -55,6 +55,37 @@ def build_hip():
     return out
 
 
+def build_fluid_variant(directory: str, host: bool = True, hip: bool = True):
+    """Kernel and host libraries for one model's user fluids: the same sources
+    compiled against the generated ``fluid_dev.h`` / ``fluid_host.h`` in
+    ``directory`` (abcmb_amd/fluidgen.py), linked into that directory."""
+    from concurrent.futures import ThreadPoolExecutor
+    directory = os.path.abspath(directory)
+    dev_h = os.path.join(directory, "fluid_dev.h")
+    host_h = os.path.join(directory, "fluid_host.h")
+    if not (os.path.exists(dev_h) and os.path.exists(host_h)):
+        raise FileNotFoundError(f"{directory} has no generated fluid headers")
+    if host:
+        tmp = os.path.join(directory, "libabcmb_host.so.tmp")
+        _run(["g++", "-O2", "-fPIC", "-shared", "-std=c++17", "-fopenmp", f"-I{os.path.join(ROOT, 'include')}",
+              f'-DABCMB_FLUID_HOST="{host_h}"', os.path.join(CSRC, "host_background.cpp"), "-o", tmp])
+        os.replace(tmp, os.path.join(directory, "libabcmb_host.so"))
+    if not hip:
+        return None
+    srcs = sorted(glob.glob(os.path.join(CSRC, "*.hip")))
+    objdir = os.path.join(ROOT, "_build", "fluids", os.path.basename(directory))
+    os.makedirs(objdir, exist_ok=True)
+    cflags = [f for f in HIP_FLAGS if f != "-shared"] + [f'-DABCMB_FLUID_DEV="{dev_h}"']
+    objs = [os.path.join(objdir, os.path.basename(s)[:-4] + ".o") for s in srcs]
+    with ThreadPoolExecutor(max(1, min(len(srcs), os.cpu_count() or 1))) as ex:
+        for f in [ex.submit(_run, [HIPCC] + cflags + ["-c", s, "-o", o]) for s, o in zip(srcs, objs)]:
+            f.result()
+    out = os.path.join(directory, "libabcmb_hip.so")
+    _run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC"] + objs + ["-o", out + ".tmp"])
+    os.replace(out + ".tmp", out)
+    return out
+
+
 def build_all():
     build_host()
     build_hip()

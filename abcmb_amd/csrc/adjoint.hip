@@ -157,6 +157,7 @@ struct RowsSeed {
   __device__ __forceinline__ D<1> pol(int m) const { return at(L.ip + m); }
   __device__ __forceinline__ D<1> ur(int m) const { return at(L.iu + m); }
   __device__ __forceinline__ D<1> nc(int b, int m) const { return at(L.inc + b * (L.ln + 1) + m); }
+  __device__ __forceinline__ D<1> fl(int j) const { return at(L.ifl + j); }
 };
 
 // sum_f lam_f d field_f / d (state row `seed`) at a saved point (background b0
@@ -502,9 +503,7 @@ __device__ __forceinline__ void adj_item(const int item, AdjSmem<NT, BMAX>& sm, 
   const AbcmbCotangent& C = cots[I.cosmo];
   const Cosmo<0> P{c, nullptr};
   const double k = I.k;
-  Layout L;
-  L.lg = st.l_g; L.lp = st.l_pol; L.lu = st.l_ur; L.ln = st.l_ncdm; L.nq = st.n_q; L.n = st.n;
-  L.ig = 4; L.ip = 4 + L.lg + 1; L.iu = L.ip + L.lp + 1; L.inc = L.iu + L.lu + 1;
+  const Layout L = make_layout(st);
   const int n = L.n;
   RowState R;
   row_desc_call(L, k, r, R.w);
@@ -800,6 +799,8 @@ extern "C" int abcmb_pert_adjoint(const AbcmbCosmo* cosmos_dev, const AbcmbCotan
   ABCMB_REQUIRE(tab_max >= ABCMB_TAU_TAB_N && tab_max >= ABCMB_COSMO_DOUBLES, "tab_max %d below the tau table",
                 tab_max);
   int maxb = 0;
+  ABCMB_REQUIRE(!fluid::kActive, "abcmb_pert_adjoint covers the standard species; this build has user fluids "
+                "(signature '%s'): use abcmb_pert_tangent", fluid::kSignature);
   int rc = check_settings(*settings, maxb);
   if (rc != ABCMB_OK) return rc;
   AdjLayout A;
@@ -819,6 +820,8 @@ extern "C" int abcmb_pert_adjoint(const AbcmbCosmo* cosmos_dev, const AbcmbCotan
                      nullptr, n_save_max, bgs);
   hipLaunchKernelGGL(zero_counts_kernel, dim3(1), dim3(64), 0, s, queue, 1);
   ABCMB_HIP_CHECK(hipGetLastError());
+  // a build with user fluids returned above: its adjoint kernels are never instantiated
+  if constexpr (!fluid::kActive) {
   if (A.nt == 128) {
     if (A.bmax == 26)
       adj_launch<128, 26>(cosmos_dev, cot_dev, items_dev, n_items, *settings, bgs, n_save_max, tab_max, S, scratch,
@@ -833,6 +836,7 @@ extern "C" int abcmb_pert_adjoint(const AbcmbCosmo* cosmos_dev, const AbcmbCotan
     else
       adj_launch<192, 32>(cosmos_dev, cot_dev, items_dev, n_items, *settings, bgs, n_save_max, tab_max, S, scratch,
                           A.grid, queue, status_dev, steps_dev, iters_dev, s);
+  }
   }
   ABCMB_HIP_CHECK(hipGetLastError());
   return ABCMB_OK;

@@ -53,16 +53,24 @@ const double Q5[5] = {0.583165, 2.0, 4.0, 7.26582, 13.0};
 const double W5[5] = {0.0081201, 0.689407, 2.8063, 2.05156, 0.12681};
 const double EQ5[5] = {std::exp(-Q5[0]), std::exp(-Q5[1]), std::exp(-Q5[2]), std::exp(-Q5[3]), std::exp(-Q5[4])};
 
+#ifndef ABCMB_FLUID_HOST
+#define ABCMB_FLUID_HOST "fluid_none_host.h"
+#endif
+#include ABCMB_FLUID_HOST
+
 struct Rho {
   double de, cdm, b, g, ur, ncdm, p_ncdm;
+  double fl, p_fl;  // user fluids (abcmb_amd/fluidgen.py), summed
 };
 
 inline Rho rho_all(const AbcmbCosmo& c, double lna) {
   Rho r;
   double a = std::exp(lna);
   double a3 = a * a * a, a4 = a3 * a;
-  r.de = c.omega_Lambda * RHO_UNIT;
-  r.cdm = c.omega_cdm * RHO_UNIT / a3;
+  r.de = (c.species_off & ABCMB_NO_LAMBDA) ? 0.0 : c.omega_Lambda * RHO_UNIT;
+  r.cdm = (c.species_off & ABCMB_NO_CDM) ? 0.0 : c.omega_cdm * RHO_UNIT / a3;
+  r.fl = r.p_fl = 0.0;
+  if (fluid_host::kActive) fluid_host::bg(c, lna, a, r.fl, r.p_fl);
   r.b = c.omega_b * RHO_UNIT / a3;
   double t04 = c.TCMB0 * c.TCMB0 * c.TCMB0 * c.TCMB0;
   r.g = PI * PI / 15.0 * t04 / a4 / HC3;
@@ -87,8 +95,8 @@ inline Rho rho_all(const AbcmbCosmo& c, double lna) {
   return r;
 }
 
-inline double rho_tot(const Rho& r) { return r.de + r.cdm + r.b + r.g + r.ur + r.ncdm; }
-inline double p_tot(const Rho& r) { return -r.de + r.g / 3.0 + r.ur / 3.0 + r.p_ncdm; }
+inline double rho_tot(const Rho& r) { return r.de + r.cdm + r.b + r.g + r.ur + r.ncdm + r.fl; }
+inline double p_tot(const Rho& r) { return -r.de + r.g / 3.0 + r.ur / 3.0 + r.p_ncdm + r.p_fl; }
 
 inline double hubble(const AbcmbCosmo& c, double lna) {
   return std::sqrt(8.0 * PI * GN * rho_tot(rho_all(c, lna)) / 3.0);
@@ -408,6 +416,8 @@ double expmkappa(const Kappa& K, double l) { return l < -10.0 ? 0.0 : std::exp(-
 }  // namespace
 
 extern "C" int abcmb_host_recomb_points(void) { return rec_points(); }
+
+extern "C" const char* abcmb_host_fluid_signature(void) { return fluid_host::kSignature; }
 
 namespace {
 int background(AbcmbCosmo* c, double tau_reion, double z_reion_in, double Delta_z_reion, double z_reion_He,

@@ -143,6 +143,8 @@ int dispatch(const AbcmbCosmo* cosmos, const AbcmbTangent* tangents, int n_cosmo
 }  // namespace
 }  // namespace abcmb
 
+extern "C" const char* abcmb_fluid_signature(void) { return abcmb::fluid::kSignature; }
+
 extern "C" size_t abcmb_pert_workspace_size(int n_cosmo, int n_save_max, int n_dir) {
   using namespace abcmb;
   const size_t per = n_dir > 0 ? sizeof(Bg<ABCMB_TANGENT_DIRS>) : sizeof(Bg<0>);
@@ -481,8 +483,13 @@ __global__ void __launch_bounds__(LOSBG_THREADS) losbg_kernel(const AbcmbCosmo* 
     }
     const D<N> g = em * dinv(tc);
     const D<N> gp = l < KAPPA_LO ? D<N>(0.0) : g * (dinv(tc * b.aH) - (2.0 - dxe * dinv(xe)));
-    const D<N> rde = P(F_OMEGA_LAMBDA) * K_RHO;
-    const D<N> rp3 = -2.0 * rde + b.rc + b.rb + 2.0 * (b.rg + b.ru) + b.rn + 3.0 * b.pn;
+    const D<N> rde = (c.species_off & ABCMB_NO_LAMBDA) ? D<N>(0.0) : P(F_OMEGA_LAMBDA) * K_RHO;
+    D<N> rp3 = -2.0 * rde + b.rc + b.rb + 2.0 * (b.rg + b.ru) + b.rn + 3.0 * b.pn;
+    if constexpr (fluid::kActive) {
+      D<N> rf, pf, rm;
+      fluid::bg<N>(P, l, b.a, rf, pf, rm);
+      rp3 += rf + 3.0 * pf;
+    }
     const D<N> ahp = -(K_AHP * b.a * b.a) * rp3;
     const D<N> v[6] = {fast_interp<N>(l, ABCMB_TAU_TAB_LO, 0.0, c.tau_tab, P.dtau(), ABCMB_TAU_TAB_N), g, gp, b.aH,
                        em, ahp};

@@ -88,17 +88,6 @@ __constant__ double WP5[5] = {E5V[0] * W5V[0], E5V[1] * W5V[1], E5V[2] * W5V[2],
 __constant__ double WR3[3] = {ABCMB_WR(Q3V, W3V, E3V, 0), ABCMB_WR(Q3V, W3V, E3V, 1), ABCMB_WR(Q3V, W3V, E3V, 2)};
 #undef ABCMB_WR
 
-// background scalar slots (per evaluation time)
-enum {
-  SC_ONE = 0, SC_IAH, SC_ITC, SC_ITAU, SC_CS2, SC_RTC,
-  SC_H_ETA, SC_H_C, SC_H_B, SC_H_G, SC_H_U,
-  SC_E_B, SC_E_G, SC_E_U,
-  SC_P0,            // 5 slots: q_i / (eps_i aH)
-  SC_HN0 = SC_P0 + 5,  // 5 slots: h'-weight of Psi0_i
-  SC_EN = SC_HN0 + 5,  // eta'-weight base of kPsi1_i
-  SC_ZERO,
-  SC_COUNT
-};
 
 // ---------------------------------------------------------------- cosmology fields
 // AbcmbCosmo's double fields in declaration order; the tangent inputs index
@@ -107,10 +96,48 @@ enum CosmoField {
   F_H = 0, F_OMEGA_B, F_OMEGA_CDM, F_OMEGA_LAMBDA, F_A_S, F_N_S, F_K_PIVOT, F_TCMB0, F_YHE,
   F_N_UR, F_T_UR, F_N_NCDM, F_T_NCDM, F_M_NCDM, F_R_NU, F_R_B, F_OM, F_OMEGA_R, F_OMEGA_M, F_TAU0,
   F_XE_X0, F_XE_DX, F_XE_LNA_LAST, F_XE_LAST, F_TM_X0, F_TM_DX, F_TM_LNA_LAST, F_TM_LAST,
-  F_LNA_REC, F_LNA_TRANSFER_START, F_Z_REION, F_COUNT
+  F_LNA_REC, F_LNA_TRANSFER_START, F_Z_REION, F_FLUID0,
+  F_COUNT = F_FLUID0 + ABCMB_FLUID_PARAMS
 };
 static_assert(F_COUNT == ABCMB_COSMO_DOUBLES, "field list out of step with AbcmbCosmo");
-static_assert(offsetof(AbcmbCosmo, z_reion) == (F_COUNT - 1) * sizeof(double), "AbcmbCosmo doubles are not packed");
+static_assert(offsetof(AbcmbCosmo, fluid_par) == F_FLUID0 * sizeof(double), "AbcmbCosmo doubles are not packed");
+
+// ---------------------------------------------------------------- user fluids
+// The model's user-defined fluids (abcmb/species.py:11-249) arrive as
+// generated code (abcmb_amd/fluidgen.py) in namespace fluid: their summed
+// density and pressure, their rows' terms (columns, static coefficients and
+// background scalar slots), their h'/eta' couplings, initial conditions and
+// contributions to the metric sums.  The default header has none, and every
+// use below is behind `if constexpr (fluid::kActive)` / `fluid::kRows > 0`,
+// so the standard kernels compile exactly as without this hook.  The helpers
+// give the generated expressions one spelling for plain doubles and duals.
+namespace fluid {
+__device__ __forceinline__ double fexp(double x) { return exp(x); }
+__device__ __forceinline__ double flog(double x) { return log(x); }
+__device__ __forceinline__ double fsqrt(double x) { return sqrt(x); }
+__device__ __forceinline__ double fpow(double x, double p) { return pow(x, p); }
+template <int N> __device__ __forceinline__ D<N> fexp(const D<N>& x) { return dexp(x); }
+template <int N> __device__ __forceinline__ D<N> flog(const D<N>& x) { return dlog(x); }
+template <int N> __device__ __forceinline__ D<N> fsqrt(const D<N>& x) { return dsqrt(x); }
+template <int N> __device__ __forceinline__ D<N> fpow(const D<N>& x, double p) { return dpow(x, p); }
+}  // namespace fluid
+#ifndef ABCMB_FLUID_DEV
+#define ABCMB_FLUID_DEV "fluid_none_dev.h"
+#endif
+#include ABCMB_FLUID_DEV
+
+// background scalar slots (per evaluation time); the user fluids' slots follow
+enum {
+  SC_ONE = 0, SC_IAH, SC_ITC, SC_ITAU, SC_CS2, SC_RTC,
+  SC_H_ETA, SC_H_C, SC_H_B, SC_H_G, SC_H_U,
+  SC_E_B, SC_E_G, SC_E_U,
+  SC_P0,            // 5 slots: q_i / (eps_i aH)
+  SC_HN0 = SC_P0 + 5,  // 5 slots: h'-weight of Psi0_i
+  SC_EN = SC_HN0 + 5,  // eta'-weight base of kPsi1_i
+  SC_ZERO,
+  SC_FL0,              // fluid::kSlots slots of the user fluids' rows
+  SC_COUNT = SC_FL0 + fluid::kSlots
+};
 
 // A cosmology seen by the kernels: its fields and tables, with their
 // derivatives along N parameter directions when N > 0.
@@ -181,8 +208,9 @@ __device__ void bg_rho(const Cosmo<N>& P, double lna, Bg<N>& b) {
   const D<N> t04 = T0 * T0 * T0 * T0;
   const D<N> tu4 = Tu * Tu * Tu * Tu;
   b.a = a;
-  const D<N> rde = P(F_OMEGA_LAMBDA) * K_RHO;
-  b.rc = P(F_OMEGA_CDM) * (K_RHO * ia3);
+  const int off = P.c.species_off;
+  const D<N> rde = (off & ABCMB_NO_LAMBDA) ? D<N>(0.0) : P(F_OMEGA_LAMBDA) * K_RHO;
+  b.rc = (off & ABCMB_NO_CDM) ? D<N>(0.0) : P(F_OMEGA_CDM) * (K_RHO * ia3);
   b.rb = P(F_OMEGA_B) * (K_RHO * ia3);
   b.rg = t04 * (K_G * ia4);
   b.ru = P(F_N_UR) * tu4 * t04 * (K_U * ia4);
@@ -205,6 +233,11 @@ __device__ void bg_rho(const Cosmo<N>& P, double lna, Bg<N>& b) {
     b.pn = P(F_N_NCDM) * (4.0 / 3.0) * pre * ip;
   }
   D<N> rt = rde + b.rc + b.rb + b.rg + b.ru + b.rn;
+  if constexpr (fluid::kActive) {
+    D<N> rf, pf, rm;
+    fluid::bg<N>(P, lna, a, rf, pf, rm);
+    rt += rf;
+  }
   b.aH = a * dsqrt(K_H * rt) * (1.0 / C_MPC);
 }
 
@@ -265,7 +298,16 @@ struct Row {
 struct Layout {
   int lg, lp, lu, ln, nq, n;
   int ig, ip, iu, inc;
+  int ifl, nfl;  // user-fluid rows: the last nfl rows, from ifl
 };
+
+__device__ __forceinline__ Layout make_layout(const AbcmbOdeSettings& st) {
+  Layout L;
+  L.lg = st.l_g; L.lp = st.l_pol; L.lu = st.l_ur; L.ln = st.l_ncdm; L.nq = st.n_q; L.n = st.n;
+  L.ig = 4; L.ip = 4 + L.lg + 1; L.iu = L.ip + L.lp + 1; L.inc = L.iu + L.lu + 1;
+  L.nfl = st.n_fluid; L.ifl = L.n - L.nfl;
+  return L;
+}
 
 __device__ __forceinline__ void row_desc(const Layout& L, double k, int r, Row& w) {
   for (int j = 0; j < 5; ++j) { w.col[j] = r < L.n ? r : 0; w.cs[j] = 0.0; w.si[j] = SC_ZERO; }
@@ -332,7 +374,7 @@ __device__ __forceinline__ void row_desc(const Layout& L, double k, int r, Row& 
     } else { add(b + l - 1, k, SC_IAH); add(b + l, -(double)(L.lu + 1), SC_ITAU); }
     return;
   }
-  if (r >= L.inc && r < L.n) {  // massive neutrinos, bin i
+  if (r >= L.inc && r < L.ifl) {  // massive neutrinos, bin i
     int nl = L.ln + 1;
     int i = (r - L.inc) / nl, l = (r - L.inc) % nl, b = L.inc + i * nl;
     double q = L.nq == 5 ? Q5[i] : Q3[i];
@@ -349,6 +391,15 @@ __device__ __forceinline__ void row_desc(const Layout& L, double k, int r, Row& 
       add(b + l - 1, l * k / (2.0 * l + 1.0), sp); add(b + l + 1, -(l + 1) * k / (2.0 * l + 1.0), sp);
     } else { add(b + l - 1, k, sp); add(b + l, -(double)(L.ln + 1), SC_ITAU); }
     return;
+  }
+  if constexpr (fluid::kRows > 0) {
+    if (r >= L.ifl && r < L.n) {  // user fluids: columns local to their block, slots after SC_FL0
+      int hs = -1, es = -1;
+      fluid::row(r - L.ifl, k, [&](int c, double v, int sl) { add(L.ifl + c, v, sl < 0 ? SC_ONE : SC_FL0 + sl); },
+                 w.uh, w.ue, hs, es);
+      if (hs >= 0) { w.whi = SC_FL0 + hs; w.whs = 1.0; }
+      if (es >= 0) { w.wei = SC_FL0 + es; w.wes = 1.0; }
+    }
   }
 }
 
@@ -411,6 +462,12 @@ __device__ __forceinline__ void fill_scalars(const Cosmo<N>& P, const Layout& L,
   }
   put(SC_EN, ke * 4.0 * pre);
   put(SC_ZERO, D<N>(0.0));
+  if constexpr (fluid::kSlots > 0) {
+    D<N> fs[fluid::kSlots > 0 ? fluid::kSlots : 1];
+    fluid::slots<N>(P, log(b.a), b.a, k, b.aH, kh, ke, fs);
+#pragma unroll
+    for (int q = 0; q < fluid::kSlots; ++q) put(SC_FL0 + q, fs[q]);
+  }
 }
 
 // the background and the scalars of A at one time, out of line: one call per
@@ -480,7 +537,9 @@ constexpr int JAC_SLOT = 6;
 __constant__ double TIME_C[NTIMES - 1] = {0.0, 0.52, 1.230333209967908, 0.8957659843500759, 0.43639360985864756,
                                           1.0};
 __constant__ int STAGE_SLOT[7] = {0, 1, 2, 3, 4, 5, 5};
-constexpr int NBLK = 10;
+// blocks: eta, delta_c, baryon-photon, massless nu, up to 5 massive-nu bins,
+// the user fluids (when the build has any), and a last id for rows past n
+constexpr int NBLK = fluid::kRows > 0 ? 11 : 10;
 
 struct Blk {
   int id, start, size, lr;
@@ -492,10 +551,11 @@ __device__ Blk row_block(const Layout& L, int r) {
   else if (r == 1) { b.id = 1; b.start = 1; b.size = 1; }
   else if (r < L.iu) { b.id = 2; b.start = 2; b.size = L.iu - 2; }
   else if (r < L.inc) { b.id = 3; b.start = L.iu; b.size = L.lu + 1; }
-  else if (r < L.n) {
+  else if (r < L.ifl) {
     int i = (r - L.inc) / (L.ln + 1);
     b.id = 4 + i; b.start = L.inc + i * (L.ln + 1); b.size = L.ln + 1;
-  } else { b.id = NBLK - 1; b.start = r; b.size = 0; }
+  } else if (r < L.n) { b.id = 9; b.start = L.ifl; b.size = L.nfl; }
+  else { b.id = NBLK - 1; b.start = r; b.size = 0; }
   b.lr = r - b.start;
   return b;
 }
@@ -1004,7 +1064,7 @@ __device__ __noinline__ D<N> ic_row(const Cosmo<N>& P, const Layout& L, double k
   if (r == L.iu) return dg;
   if (r == L.iu + 1) return tn;
   if (r == L.iu + 2) return sn;
-  if (r >= L.inc && r < L.n) {
+  if (r >= L.inc && r < L.ifl) {
     int nl = L.ln + 1;
     int i = (r - L.inc) / nl, l = (r - L.inc) % nl;
     double q = L.nq == 5 ? Q5[i] : Q3[i];
@@ -1012,6 +1072,9 @@ __device__ __noinline__ D<N> ic_row(const Cosmo<N>& P, const Layout& L, double k
     if (l == 0) return dg / 4. * f;
     if (l == 1) return tn / 3. * f;
     if (l == 2) return sn / 2. * f;
+  }
+  if constexpr (fluid::kRows > 0) {
+    if (r >= L.ifl && r < L.n) return fluid::ic<N>(r - L.ifl, P, k, ti);
   }
   return D<N>(0.0);
 }
@@ -1031,7 +1094,7 @@ __device__ __forceinline__ int out_slot(const Layout& L, int r) {
   if (r == L.ip) return 7;
   if (r == L.ip + 2) return 8;
   if (r >= L.iu && r < L.iu + 3) return 9 + r - L.iu;
-  if (r >= L.inc && r < L.n) {
+  if (r >= L.inc && r < L.ifl) {
     const int b = (r - L.inc) / (L.ln + 1), m = (r - L.inc) % (L.ln + 1);
     if (m < 3) return 12 + 3 * b + m;
   }
@@ -1056,12 +1119,14 @@ struct RowsLds {
   __device__ __forceinline__ D<N> pol(int m) const { return at(L.ip + m); }
   __device__ __forceinline__ D<N> ur(int m) const { return at(L.iu + m); }
   __device__ __forceinline__ D<N> nc(int b, int m) const { return at(L.inc + b * (L.ln + 1) + m); }
+  __device__ __forceinline__ D<N> fl(int j) const { return at(L.ifl + j); }
 };
 
 // the rows in registers, by slot (bins beyond the layout's read as zero)
 template <int N>
 struct RowsReg {
   D<N> Y[27];
+  D<N> F[fluid::kRows > 0 ? fluid::kRows : 1];  // user-fluid rows
   __device__ __forceinline__ D<N> eta() const { return Y[0]; }
   __device__ __forceinline__ D<N> dc() const { return Y[1]; }
   __device__ __forceinline__ D<N> db() const { return Y[2]; }
@@ -1070,6 +1135,7 @@ struct RowsReg {
   __device__ __forceinline__ D<N> pol(int m) const { return m == 0 ? Y[7] : Y[8]; }
   __device__ __forceinline__ D<N> ur(int m) const { return Y[9 + m]; }
   __device__ __forceinline__ D<N> nc(int b, int m) const { return Y[12 + 3 * b + m]; }
+  __device__ __forceinline__ D<N> fl(int j) const { return F[j]; }
 };
 
 // every field of the perturbation table at one saved point
@@ -1118,6 +1184,7 @@ __device__ __forceinline__ void all_fields(const Cosmo<N>& P, const Layout& L, d
   D<N> srs = (b.rg + b.rg / 3.) * y.g(2) + (b.ru + b.ru / 3.) * y.ur(2);
   D<N> srdm = b.rc * y.dc() + b.rb * y.db(), srm = b.rc + b.rb;
   if (L.nq > 0) { srd += nrd; srt += nrt; srs += nrs; srdm += nrd; srm += b.rn; }
+  if constexpr (fluid::kActive) fluid::sums<N>(P, log(a), a, y, srd, srt, srs, srdm, srm);
   emit(ABCMB_PT_DELTA_M, srdm / srm);
   double KG = 4. * PI * GN * a * a / (C_MPC * C_MPC);
   D<N> hp = 2. / (aH * aH) * (k * k * y.eta() + KG * srd);
@@ -1151,9 +1218,7 @@ template <int N>
 __device__ __forceinline__ void fields_at(const AbcmbCosmo& c, const AbcmbTangent* T, const AbcmbOdeSettings& st,
                                           const Bg<N>& bg, double k, int ik, int isave) {
   const Cosmo<N> P{c, T};
-  Layout L;
-  L.lg = st.l_g; L.lp = st.l_pol; L.lu = st.l_ur; L.ln = st.l_ncdm; L.nq = st.n_q; L.n = st.n;
-  L.ig = 4; L.ip = 4 + L.lg + 1; L.iu = L.ip + L.lp + 1; L.inc = L.iu + L.lu + 1;
+  const Layout L = make_layout(st);
   const int nr = out_rows(L);
   const int n_save = c.n_save;
   const size_t dstride = (size_t)ABCMB_N_PT_FIELDS * n_save * c.nk;
@@ -1166,6 +1231,15 @@ __device__ __forceinline__ void fields_at(const AbcmbCosmo& c, const AbcmbTangen
       const size_t o = at(s);
       y.Y[s].v = c.pt[o];
       if constexpr (N > 0) { ABCMB_DLOOP(N) y.Y[s].d[i_] = T->pt[i_ * dstride + o]; }
+    }
+  }
+  if constexpr (fluid::kRows > 0) {
+    const size_t fstride = (size_t)fluid::kRows * n_save * c.nk;
+#pragma unroll
+    for (int j = 0; j < fluid::kRows; ++j) {
+      const size_t o = ((size_t)j * n_save + isave) * c.nk + ik;
+      y.F[j] = D<N>(c.pt_fluid[o]);
+      if constexpr (N > 0) { ABCMB_DLOOP(N) y.F[j].d[i_] = T->pt_fluid[i_ * fstride + o]; }
     }
   }
   all_fields(P, L, k, bg, y, [&](int f, const D<N>& v) {
@@ -1385,9 +1459,7 @@ __device__ __forceinline__ void ode_item(const int item, Smem<NT, BMAX, N>& sm, 
   if constexpr (N > 0) T = tangents + I.cosmo;
   const Cosmo<N> P{c, T};
   const double k = I.k;
-  Layout L;
-  L.lg = st.l_g; L.lp = st.l_pol; L.lu = st.l_ur; L.ln = st.l_ncdm; L.nq = st.n_q; L.n = st.n;
-  L.ig = 4; L.ip = 4 + L.lg + 1; L.iu = L.ip + L.lp + 1; L.inc = L.iu + L.lu + 1;
+  const Layout L = make_layout(st);
   const int n = L.n;
   RowState R;
   if constexpr (N > 0) row_desc(L, k, r, R.w);
@@ -1396,11 +1468,12 @@ __device__ __forceinline__ void ode_item(const int item, Smem<NT, BMAX, N>& sm, 
   R.n = n;
 #pragma unroll
   for (int j = 0; j < 5; ++j) R.lc[j] = R.w.col[j] - R.B.start;
-  const int maxb = max(L.iu - 2, max(L.lu + 1, L.nq > 0 ? L.ln + 1 : 1));
+  const int maxb = max(max(L.iu - 2, L.nfl), max(L.lu + 1, L.nq > 0 ? L.ln + 1 : 1));
   // every hierarchy block inside one wavefront (rows lo..hi)
   auto one_wave = [](int lo, int hi) { return (lo >> 6) == (hi >> 6); };
   bool wl = one_wave(2, L.iu - 1) && one_wave(L.iu, L.inc - 1);
   for (int i = 0; i < L.nq; ++i) wl = wl && one_wave(L.inc + i * (L.ln + 1), L.inc + (i + 1) * (L.ln + 1) - 1);
+  if (L.nfl > 0) wl = wl && one_wave(L.ifl, L.n - 1);
   for (int q = 0; q < 1 + N; ++q) {
     sm.zl[q][r] = 0.0; sm.rl[r * (1 + N) + q] = 0.0;
     if (r < BMAX) { sm.zl[q][NT + r] = 0.0; sm.rl[(NT + r) * (1 + N) + q] = 0.0; }
@@ -1436,6 +1509,16 @@ __device__ __forceinline__ void ode_item(const int item, Smem<NT, BMAX, N>& sm, 
   const bool rows_out = out_rows(L) <= ABCMB_N_PT_FIELDS;
   const int oslot = rows_out && r < n ? out_slot(L, r) : -1;
   auto save_point = [&](const D<N>& ys) {
+    if constexpr (fluid::kRows > 0) {  // the user-fluid rows, as they are
+      if (r >= L.ifl && r < n) {
+        const size_t o = ((size_t)(r - L.ifl) * n_save + isave) * c.nk + I.ik;
+        c.pt_fluid[o] = ys.v;
+        if constexpr (N > 0) {
+          const size_t fstride = (size_t)fluid::kRows * n_save * c.nk;
+          ABCMB_DLOOP(N) T->pt_fluid[i_ * fstride + o] = ys.d[i_];
+        }
+      }
+    }
     if (rows_out) {
       if (state_out && r < n) state_out[((size_t)item * n_save + isave) * n + r] = ys.v;
       if (oslot >= 0) {
@@ -1528,12 +1611,16 @@ __device__ __forceinline__ void ode_item(const int item, Smem<NT, BMAX, N>& sm, 
 int check_settings(const AbcmbOdeSettings& st, int& maxb) {
   ABCMB_REQUIRE(st.l_g >= 5 && st.l_pol >= 4 && st.l_ur >= 5, "hierarchy cut-offs too small");
   ABCMB_REQUIRE(st.n_q == 0 || ((st.n_q == 3 || st.n_q == 5) && st.l_ncdm >= 4), "bad massive-nu layout");
-  int n_expect = 4 + (st.l_g + 1) + (st.l_pol + 1) + (st.l_ur + 1) + st.n_q * (st.l_ncdm + 1);
+  ABCMB_REQUIRE(st.n_fluid == fluid::kRows,
+                "settings.n_fluid=%d but this library's fluid code has %d rows (signature '%s')", st.n_fluid,
+                fluid::kRows, fluid::kSignature);
+  int n_expect = 4 + (st.l_g + 1) + (st.l_pol + 1) + (st.l_ur + 1) + st.n_q * (st.l_ncdm + 1) + st.n_fluid;
   ABCMB_REQUIRE(st.n == n_expect, "settings.n=%d does not match the layout (%d)", st.n, n_expect);
   ABCMB_REQUIRE(st.max_newton > 0 && st.max_steps > 0, "bad iteration limits");
   maxb = 2 + (st.l_g + 1) + (st.l_pol + 1);
   if (st.l_ur + 1 > maxb) maxb = st.l_ur + 1;
   if (st.n_q > 0 && st.l_ncdm + 1 > maxb) maxb = st.l_ncdm + 1;
+  if (st.n_fluid > maxb) maxb = st.n_fluid;
   ABCMB_REQUIRE(maxb <= 64, "a hierarchy block of %d rows exceeds 64", maxb);
   ABCMB_REQUIRE(st.n <= 192, "state size %d exceeds 192", st.n);
   return ABCMB_OK;

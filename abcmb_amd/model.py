@@ -21,16 +21,13 @@ from dataclasses import dataclass, field
 import numpy as np
 
 from . import abi, bessel, splines
+from . import species as SPC
 from . import specs as S
 from .background import Background
 from .params import add_derived_parameters
 
-LCDM_SPECIES = ("DarkEnergy", "ColdDarkMatter", "Baryon", "Photon", "MasslessNeutrino")
-
-
-class MassiveNeutrino:
-    """Marker for the massive-neutrino user species (abcmb/species.py:762-1090)."""
-    name = "MassiveNeutrino"
+LCDM_SPECIES = tuple(c.name for c in SPC.LCDM_SPECIES)
+MassiveNeutrino = SPC.MassiveNeutrino
 
 
 def _torch():
@@ -104,10 +101,27 @@ def tangent_seeds(plus, minus, steps) -> TangentSeeds:
     return TangentSeeds(par, tau, xe, tm)
 
 
+class _Modes:
+    """``modes`` of a species' output_perturbations (abcmb/perturbations.py:318-404):
+    modes[i] is state row i on (Nlna, Nk) for the user-fluid rows (kept by the
+    kernels as they are) and the standard rows the table stores."""
+
+    def __init__(self, pt_fluid, first_row, std):
+        self.pf, self.first, self.std = pt_fluid, first_row, std
+
+    def __getitem__(self, i):
+        i = int(i)
+        if self.pf is not None and self.first <= i < self.first + self.pf.shape[0]:
+            return self.pf[i - self.first]
+        if i in self.std:
+            return self.std[i]
+        raise KeyError(f"state row {i} is not kept in the perturbation table")
+
+
 class PerturbationTable:
     """Perturbation table (abcmb/perturbations.py:406-450); arrays are (Nlna, Nk) views."""
 
-    def __init__(self, k, lna, pt, has_ncdm):
+    def __init__(self, k, lna, pt, has_ncdm, species_off=0, fluids=None, pt_fluid=None, args=None):
         self.k = k
         self.lna = lna
         self.table = pt  # [21][Nlna][Nk]
@@ -131,6 +145,16 @@ class PerturbationTable:
         if has_ncdm:
             sp["MassiveNeutrino"] = {"delta": pt[I["ncdm_delta"]], "theta": pt[I["ncdm_theta"]],
                                      "sigma": pt[I["ncdm_sigma"]]}
+        if species_off & abi.NO_LAMBDA:
+            del sp["DarkEnergy"]
+        if species_off & abi.NO_CDM:
+            del sp["ColdDarkMatter"]
+        if fluids is not None:
+            modes = _Modes(pt_fluid, fluids.first_row, {1: pt[I["cdm_delta"]], 2: pt[I["b_delta"]],
+                                                        3: pt[I["b_theta"]]})
+            for f in fluids.user:
+                sp[f.name] = f.output_perturbations(lna, modes, args)
+        self.pt_fluid = pt_fluid
         self.species_perturbations = sp
 
 
@@ -159,16 +183,7 @@ class Model:
         kwargs.pop("adjoint", None)  # reference option for diffrax AD; not used here
         specs = S.load_specs(kwargs)
         self.specs = specs
-        names = list(LCDM_SPECIES)
-        self.has_ncdm = False
-        for s in (user_species or ()):
-            nm = getattr(s, "name", s if isinstance(s, str) else getattr(s, "__name__", ""))
-            if nm != "MassiveNeutrino":
-                raise NotImplementedError(f"user species {nm!r}: only MassiveNeutrino is supported")
-            names.append("MassiveNeutrino")
-            self.has_ncdm = True
-        self.species = tuple(names)
-        self.layout = S.state_layout(specs, self.has_ncdm)
+        self._species_setup(user_species)
         k_pt, k_pk = S.get_k_axis_perturbations(specs)
         if specs.get("k_axis_perturbations") is not None:
             k_pt = np.asarray(specs["k_axis_perturbations"], dtype=np.float64)
@@ -204,7 +219,8 @@ class Model:
             pcoeff=specs["pcoeff_PE"], icoeff=specs["icoeff_PE"], dcoeff=specs["dcoeff_PE"], dt0=1e-2,
             safety=0.9, factormin=0.2, factormax=10.0, newton_kappa=1e-2, jac_offset=0.6,
             max_steps=int(specs["max_steps_PE"]), max_newton=10,
-            l_g=lay.l_g, l_pol=lay.l_pol, l_ur=lay.l_ur, l_ncdm=lay.l_ncdm, n_q=lay.n_q, n=lay.n)
+            l_g=lay.l_g, l_pol=lay.l_pol, l_ur=lay.l_ur, l_ncdm=lay.l_ncdm, n_q=lay.n_q,
+            n=lay.n + self.n_fluid, n_fluid=self.n_fluid)
         precisions = {"fp64": 0, "fp32": 1, "bf16": 2}
         if los_precision not in precisions:
             raise ValueError("los_precision must be 'fp64', 'fp32' or 'bf16'")
@@ -227,6 +243,79 @@ class Model:
         self.call_timing = None  # dict: entry-point name -> [(start event, end event, work)] when set
         self._dev = None
 
+    def _species_setup(self, user_species):
+        """The model's species (abcmb/model_specs.py:84-118): the LCDM set unless
+        use_LCDM_species is off, then the user's classes, instantiated in that
+        order.  The standard species' rows are the kernels' fixed layout; user
+        fluids (abcmb_amd.species) take the rows after it and are compiled into
+        a build of the libraries for their set (abcmb_amd.fluidgen)."""
+        specs = self.specs
+        classes = list(SPC.LCDM_SPECIES) if specs.get("use_LCDM_species", True) else []
+        for s in (user_species or ()):
+            if isinstance(s, str) and s == "MassiveNeutrino":
+                s = SPC.MassiveNeutrino
+            if not isinstance(s, type):
+                raise TypeError(f"user_species takes Fluid classes, not {s!r} (the model instantiates them)")
+            if not issubclass(s, SPC.Fluid):
+                raise TypeError(f"{s.__name__} is not a subclass of abcmb_amd.species.Fluid")
+            classes.append(s)
+        std = {c.name for c in classes if SPC.is_standard(c)}
+        missing = [nm for nm in ("Baryon", "Photon", "MasslessNeutrino") if nm not in std]
+        if missing:
+            raise NotImplementedError(f"the kernels need the standard species {missing} in the model")
+        self.has_ncdm = "MassiveNeutrino" in std
+        self.layout = lay = S.state_layout(specs, self.has_ncdm)
+        rows = {"DarkEnergy": 1, "ColdDarkMatter": lay.i_cdm, "Baryon": lay.i_b, "Photon": lay.i_g,
+                "MasslessNeutrino": lay.i_ur, "MassiveNeutrino": lay.i_ncdm}
+        objs, nxt = [], lay.n
+        for c in classes:
+            if SPC.is_standard(c):
+                objs.append(c(rows[c.name], specs))
+            else:
+                o = c(nxt, specs)
+                o.first_idx = nxt
+                nxt += int(o.num_equations)
+                objs.append(o)
+        names = [o.name for o in objs]
+        if len(set(names)) != len(names):
+            raise ValueError(f"species names must be distinct: {names}")
+        self.species_objs = tuple(objs)
+        self.species = tuple(o.name if SPC.is_standard(o) else o for o in objs)
+        self.species_off = (0 if "DarkEnergy" in std else abi.NO_LAMBDA) | (0 if "ColdDarkMatter" in std
+                                                                           else abi.NO_CDM)
+        self.n_fluid = nxt - lay.n
+        user = [o for o in objs if not SPC.is_standard(o)]
+        self.fluids = None
+        self._hip_path = self._host_path = None
+        if user:
+            from .fluidgen import FluidSet
+            self.fluids = FluidSet(objs, user, lay.n, nxt, self.species_off)
+            self.fluids.build(hip=False)   # the host producer (seconds); the kernels on first GPU use
+            self._hip_path, self._host_path = self.fluids.hip_path, self.fluids.host_path
+            for lib, fn in ((self._hlib(), "abcmb_host_fluid_signature"), ):
+                sig = getattr(lib, fn)().decode()
+                if sig != self.fluids.signature:
+                    raise RuntimeError(f"{self._host_path} was built for fluids {sig!r}, "
+                                       f"not {self.fluids.signature!r}")
+
+    def _lib(self):
+        """The kernel library of this model's species (a fluid build when it has user fluids,
+        compiled here the first time: the ODE kernels take a few minutes)."""
+        if self.fluids is not None and not self.fluids.built():
+            if not self.specs.get("build_fluids", True):
+                raise abi.HipLibraryMissing(f"the kernels for fluids {[f.name for f in self.fluids.user]} are not "
+                                            f"built ({self.fluids.directory})")
+            print(f"building the kernels for user fluids {[f.name for f in self.fluids.user]} "
+                  f"({self.fluids.signature}); a few minutes, once", flush=True)
+            self.fluids.build()
+        lib = abi.hip_lib(self._hip_path)
+        if self.fluids is not None and lib.abcmb_fluid_signature().decode() != self.fluids.signature:
+            raise RuntimeError(f"{self._hip_path} was built for other fluids")
+        return lib
+
+    def _hlib(self):
+        return abi.host_lib(self._host_path)
+
     # ------------------------------------------------------------------ host side
     def derived(self, params: dict) -> dict:
         return add_derived_parameters(params, self.species, self.specs)
@@ -242,7 +331,9 @@ class Model:
         sp = self.specs
         kk = np.ascontiguousarray(self.k_pt if k is None else k, dtype=np.float64)
         bg = Background(full_params, self.has_ncdm, n_q=self.layout.n_q, k_pivot=sp["k_pivot"],
-                        input_tau_reion=sp["input_tau_reion"],
+                        input_tau_reion=sp["input_tau_reion"], species_off=self.species_off,
+                        fluid_par=self.fluids.fluid_par(full_params) if self.fluids is not None else None,
+                        lib=self._hlib(),
                         rec_switch=freeze.bg.rec_switch if freeze is not None else -1,
                         lna_rec=freeze.bg.lna_rec if freeze is not None else None,
                         lna_transfer_start=freeze.bg.lna_transfer_start if freeze is not None else None,
@@ -284,7 +375,7 @@ class Model:
     def _device_constants(self):
         if self._dev is None:
             torch = _torch()
-            lib = abi.hip_lib()
+            lib = self._lib()
             abi.check(lib.abcmb_device_init(), "abcmb_device_init")
             dev = torch.device("cuda")
             t = lambda a, dt=torch.float64: torch.as_tensor(np.ascontiguousarray(a), dtype=dt, device=dev)
@@ -306,7 +397,7 @@ class Model:
         key = ("dmat", self.lens_precision)
         if key not in D:
             torch = _torch()
-            lib = abi.hip_lib()
+            lib = self._lib()
             nl = self.lensing_ells.size
             nb = lib.abcmb_lens_dmat_size(self.mu.size, nl, self.lens_precision)
             dm = torch.empty(nb, dtype=torch.uint8, device=torch.device("cuda"))
@@ -334,11 +425,11 @@ class Model:
         if not plans:
             return None
         torch = _torch()
-        lib = abi.hip_lib()
+        lib = self._lib()
         self._device_constants()
         dev = torch.device("cuda")
         n, nd = len(plans), abi.N_TANGENT_DIRS
-        n_rec = abi.host_lib().abcmb_host_recomb_points()
+        n_rec = self._hlib().abcmb_host_recomb_points()
         xe_n = plans[0].bg.xe_tab.size
         if any(pl.bg.xe_tab.size != xe_n for pl in plans):
             raise ValueError("complete_backgrounds: the plans' x_e tables differ in length")
@@ -434,11 +525,12 @@ class Model:
         plist = [p if derived else self.derived(p) for p in params_list]
         return self.execute(self.prepare_many(plist))
 
-    def _cosmo_array(self, plans, pts, kdev, dt, tables=True):
-        """AbcmbCosmo structs (device pointers) for ``plans`` with perturbation tables ``pts``."""
+    def _cosmo_array(self, plans, pts, kdev, dt, tables=True, pfs=None):
+        """AbcmbCosmo structs (device pointers) for ``plans`` with perturbation tables
+        ``pts`` (and user-fluid row tables ``pfs``)."""
         ns = self.n_save
         cos = []
-        for pl, pt, kd in zip(plans, pts, kdev):
+        for i, (pl, pt, kd) in enumerate(zip(plans, pts, kdev)):
             bg = pl.bg
             c = abi.AbcmbCosmo.from_buffer_copy(bg.cosmo)
             if tables:
@@ -451,6 +543,7 @@ class Model:
             c.k = kd.data_ptr()
             c.n_save, c.nk = ns, int(kd.numel())
             c.pt = pt.data_ptr()
+            c.pt_fluid = pfs[i].data_ptr() if pfs is not None else None
             cos.append(c)
         return cos
 
@@ -474,7 +567,7 @@ class Model:
         reads or writes alive."""
         self._complete_pending(plans)
         torch = _torch()
-        lib = abi.hip_lib()
+        lib = self._lib()
         D = self._device_constants()
         dev = torch.device("cuda")
         stream = torch.cuda.current_stream().cuda_stream
@@ -486,7 +579,9 @@ class Model:
         allk = [(pl.k if pl.k is not None else self.k_pt) for pl in plans]
         kdev = [D["k_pt"] if kk is self.k_pt else dt(kk) for kk in allk]
         pts = [torch.empty((abi.N_PT_FIELDS, ns, kk.size), dtype=torch.float64, device=dev) for kk in allk]
-        cos = self._cosmo_array(plans, pts, kdev, dt)
+        nf = self.n_fluid
+        pfs = [torch.zeros((nf, ns, kk.size), dtype=torch.float64, device=dev) for kk in allk] if nf else None
+        cos = self._cosmo_array(plans, pts, kdev, dt, pfs=pfs)
         cos_dev = self._struct_array(torch, dev, cos, dt)
         # work items: all (cosmology, k), largest k first
         flat = [(-allk[ci][ik], ci, ik) for ci in range(len(plans)) for ik in range(allk[ci].size)]
@@ -506,9 +601,13 @@ class Model:
             nd = abi.N_TANGENT_DIRS
             dpts = [torch.empty((nd, abi.N_PT_FIELDS, ns, kk.size), dtype=torch.float64, device=dev)
                     for kk in allk]
+            dpfs = [torch.zeros((nd, nf, ns, kk.size), dtype=torch.float64, device=dev) for kk in allk] if nf \
+                else [None] * len(allk)
+            keep.append(dpfs)
             tans = [abi.AbcmbTangent(par=dt(sd.par).data_ptr(), tau_tab=dt(sd.tau).data_ptr(),
-                                     xe_tab=dt(sd.xe).data_ptr(), tm_tab=dt(sd.tm).data_ptr(), pt=dp.data_ptr())
-                    for sd, dp in zip(tangents, dpts)]
+                                     xe_tab=dt(sd.xe).data_ptr(), tm_tab=dt(sd.tm).data_ptr(), pt=dp.data_ptr(),
+                                     pt_fluid=dq.data_ptr() if dq is not None else None)
+                    for sd, dp, dq in zip(tangents, dpts, dpfs)]
             tan_dev = self._struct_array(torch, dev, tans, dt)
         nd = abi.N_TANGENT_DIRS if tangents is not None else 0
         ws_bytes = lib.abcmb_pert_workspace_size(len(plans), ns, nd)
@@ -535,7 +634,8 @@ class Model:
             ev1.record()
         done = torch.cuda.Event()
         done.record()
-        return dict(pts=pts, dpts=dpts, kdev=kdev, allk=allk, state=state, flat=flat, keep=keep, status=status,
+        return dict(pts=pts, pfs=pfs, dpts=dpts, kdev=kdev, allk=allk, state=state, flat=flat, keep=keep,
+                    status=status,
                     steps=steps, iters=iters, ws=ws, items_dev=items_dev, cos_dev=cos_dev, ev=(ev0, ev1), done=done,
                     tangent=tangents is not None, tan_dev=tan_dev if tangents is not None else None)
 
@@ -550,8 +650,8 @@ class Model:
     def _ode_result(h):
         """The tables of a ``_launch_ode`` handle without waiting for them: work
         enqueued on the launch's stream after this sees them complete."""
-        return dict(pts=h["pts"], dpts=h["dpts"], kdev=h["kdev"], allk=h["allk"], steps=None, state=h["state"],
-                    flat=h["flat"], keep=h["keep"])
+        return dict(pts=h["pts"], pfs=h["pfs"], dpts=h["dpts"], kdev=h["kdev"], allk=h["allk"], steps=None,
+                    state=h["state"], flat=h["flat"], keep=h["keep"])
 
     def _check_ode(self, h):
         """Wait for a ``_launch_ode`` handle and raise on any failed (cosmology, k)
@@ -614,7 +714,7 @@ class Model:
     def _downstream_buffers(self, ncos, nkmax):
         """Outputs and workspaces of the downstream of ``ncos`` cosmologies of at most ``nkmax`` k-modes."""
         torch = _torch()
-        lib = abi.hip_lib()
+        lib = self._lib()
         dev = torch.device("cuda")
         f64 = dict(dtype=torch.float64, device=dev)
         ns, nkt, n_ell = self.n_save, self.k_transfer.size, self.ell_idx.size
@@ -635,7 +735,7 @@ class Model:
         """The downstream kernels on the current stream (no allocation, no copy:
         this sequence is what a HIP graph captures)."""
         torch = _torch()
-        lib = abi.hip_lib()
+        lib = self._lib()
         D = self._device_constants()
         stream = torch.cuda.current_stream().cuda_stream
         ns, nkt, n_ell = self.n_save, self.k_transfer.size, self.ell_idx.size
@@ -714,7 +814,7 @@ class Model:
         """LOS source functions [B][4][n_save-1][n_kt] of ``plans`` with tables ``pts``."""
         torch, dev, stage = self._staging()
         dt = stage.add
-        lib = abi.hip_lib()
+        lib = self._lib()
         D = self._device_constants()
         stream = torch.cuda.current_stream().cuda_stream
         ns, ncos, nkt = self.n_save, len(plans), self.k_transfer.size
@@ -748,7 +848,7 @@ class Model:
         """
         torch, dev, stage = self._staging()
         dt = stage.add
-        lib = abi.hip_lib()
+        lib = self._lib()
         D = self._device_constants()
         stream = torch.cuda.current_stream().cuda_stream
         nd = abi.N_TANGENT_DIRS
@@ -819,7 +919,7 @@ class Model:
         """
         torch, dev, stage = self._staging()
         dt = stage.add
-        lib = abi.hip_lib()
+        lib = self._lib()
         D = self._device_constants()
         stream = torch.cuda.current_stream().cuda_stream
         f64 = dict(dtype=torch.float64, device=dev)
@@ -886,7 +986,7 @@ class Model:
         (any plan may share a table: the host inputs move, the table does not)."""
         torch, dev, stage = self._staging()
         dt = stage.add
-        lib = abi.hip_lib()
+        lib = self._lib()
         D = self._device_constants()
         stream = torch.cuda.current_stream().cuda_stream
         ncos, nl = len(plans), self.lensing_ells.size
@@ -909,7 +1009,7 @@ class Model:
         """Lensed [B][4][n_l] from unlensed cl_full [B][3][n_l], with C_l^phiphi of ``plans`` / ``pts``."""
         torch, dev, stage = self._staging()
         dt = stage.add
-        lib = abi.hip_lib()
+        lib = self._lib()
         D = self._device_constants()
         stream = torch.cuda.current_stream().cuda_stream
         ncos, nl = len(plans), self.lensing_ells.size
@@ -939,7 +1039,7 @@ class Model:
 
     def _pk(self, k, z, PT, params, kind):
         torch = _torch()
-        lib = abi.hip_lib()
+        lib = self._lib()
         dev = torch.device("cuda")
         _, _, stage = self._staging()
         kq = stage.add(np.atleast_1d(np.asarray(k, dtype=np.float64)))
@@ -994,7 +1094,8 @@ class Model:
             else:
                 F = ds["cl_full"]
                 TT, TE, EE, BB = F[i, 0, :n_l], F[i, 1, :n_l], F[i, 2, :n_l], None
-            PT = PerturbationTable(run["allk"][i], pl.lna_save, run["pts"][i], self.has_ncdm)
+            PT = PerturbationTable(run["allk"][i], pl.lna_save, run["pts"][i], self.has_ncdm, self.species_off,
+                                   self.fluids, run["pfs"][i] if run.get("pfs") else None, (pl.bg, pl.params))
             PT.k_dev = run["kdev"][i]
             out = Output(TT, TE, EE, ds["pk"][i], self.ells, self.k_pk, pl.bg, PT, pl.params, ClBB=BB,
                          Clpp=ds["clpp"][i] if lensing else None,
@@ -1045,7 +1146,7 @@ class Model:
         """
         self._complete_pending(plans)
         torch = _torch()
-        lib = abi.hip_lib()
+        lib = self._lib()
         D = self._device_constants()
         dev = torch.device("cuda")
         stream = torch.cuda.current_stream().cuda_stream

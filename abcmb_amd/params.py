@@ -49,8 +49,26 @@ def species_rho(name: str, lna: float, p: dict) -> float:
 MATTER = {"ColdDarkMatter", "Baryon", "MassiveNeutrino"}
 
 
+def _name(s) -> str:
+    return s if isinstance(s, str) else s.name
+
+
+def _rho(s, lna: float, p: dict) -> float:
+    """A standard species by name, or a user fluid (abcmb_amd.species) through its own rho."""
+    if isinstance(s, str):
+        return species_rho(s, lna, p)
+    return float(s.rho(lna, p))
+
+
+def _matter(s) -> bool:
+    return s in MATTER if isinstance(s, str) else bool(s.is_matter)
+
+
 def add_derived_parameters(param_in: dict, species: tuple, specs: dict) -> dict:
     """Fill defaults and derived keys (``abcmb/main.py:368-612``).
+
+    ``species``: the model's species in order, standard ones by name and user
+    fluids as their instances (whose rho is called with numbers).
 
     Raises ``ValueError`` where the reference calls ``sys.exit()``
     (``abcmb/main.py:409-420``).
@@ -95,10 +113,10 @@ def add_derived_parameters(param_in: dict, species: tuple, specs: dict) -> dict:
     if input_N:
         rho_g = rho_nu = rho_extra = 0.0
         for s in species:
-            rho = species_rho(s, lna_early, p)
-            if s == "Photon":
+            rho = _rho(s, lna_early, p)
+            if _name(s) == "Photon":
                 rho_g += rho
-            elif "neutrino" in s.lower():
+            elif "neutrino" in _name(s).lower():
                 rho_nu += rho
             else:
                 rho_extra += rho
@@ -113,22 +131,22 @@ def add_derived_parameters(param_in: dict, species: tuple, specs: dict) -> dict:
     if input_Neff:
         rho_g = rho_extra = 0.0
         for s in species:
-            if s == "Photon":
-                rho_g += species_rho(s, lna_early, p)
-            elif s != "MasslessNeutrino":
-                rho_extra += species_rho(s, lna_early, p)
+            if _name(s) == "Photon":
+                rho_g += _rho(s, lna_early, p)
+            elif _name(s) != "MasslessNeutrino":
+                rho_extra += _rho(s, lna_early, p)
         rho1nu = 7 / 8 * (4 / 11) ** (4 / 3) * rho_g
         p["N_nu_massless"] = (p["Neff"] - rho_extra / rho1nu) * ((4 / 11) ** (1 / 3) / p["T_nu_massless"]) ** 4
 
-    rho_m = sum(species_rho(s, 0.0, p) for s in species if s in MATTER)
+    rho_m = sum(_rho(s, 0.0, p) for s in species if _matter(s))
     p["omega_m"] = rho_m / cnst.RHO_UNIT
     p["R_b"] = p["omega_b"] / p["omega_m"]
     a_early = math.exp(-23.0)
     rho_r = rho_nu = 0.0
     for s in species:
-        r = species_rho(s, math.log(a_early), p)
+        r = _rho(s, math.log(a_early), p)
         rho_r += r
-        if "neutrino" in s.lower():
+        if "neutrino" in _name(s).lower():
             rho_nu += r
     p["omega_r"] = rho_r * a_early ** 4 / cnst.RHO_UNIT
     p["R_nu"] = rho_nu / rho_r

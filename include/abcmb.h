@@ -59,6 +59,13 @@ enum {
  * goes.  Pointers are device pointers for the HIP entry points and host
  * pointers for the host producer.
  */
+#define ABCMB_FLUID_PARAMS 8  /* parameter slots of user-defined fluids */
+/* AbcmbCosmo.species_off bits: a standard species the model leaves out
+ * (use_LCDM_species=False with a user fluid in its place,
+ * abcmb/model_specs.py:89-116) */
+#define ABCMB_NO_LAMBDA 1
+#define ABCMB_NO_CDM 2
+
 typedef struct AbcmbCosmo {
   double h, omega_b, omega_cdm, omega_Lambda;
   double A_s, n_s, k_pivot, TCMB0, YHe;
@@ -68,16 +75,23 @@ typedef struct AbcmbCosmo {
   double xe_x0, xe_dx, xe_lna_last, xe_last;     /* x_e table metadata */
   double tm_x0, tm_dx, tm_lna_last, tm_last;     /* T_m table metadata */
   double lna_rec, lna_transfer_start, z_reion;
+  /* parameters of the model's user fluids (abcmb/species.py:11-249), in the
+   * order the model's generated fluid code reads them; zero when unused */
+  double fluid_par[ABCMB_FLUID_PARAMS];
   int32_t has_ncdm;   /* MassiveNeutrino species present */
   int32_t n_q;        /* momentum bins of its hierarchy (3 or 5) */
   int32_t xe_n, tm_n; /* allocated table lengths (lookups span all of them) */
   int32_t n_save, nk; /* saved ln a points, PT k-modes */
+  int32_t species_off; /* ABCMB_NO_* bits */
+  int32_t pad0;
   const double* tau_tab;   /* ABCMB_TAU_TAB_N */
   const double* xe_tab;    /* xe_n */
   const double* tm_tab;    /* tm_n */
   const double* lna_save;  /* n_save, ascending, last = 0 */
   const double* k;         /* nk, Mpc^-1 */
   double* pt;              /* [ABCMB_N_PT_FIELDS][n_save][nk] */
+  double* pt_fluid;        /* [n_fluid][n_save][nk]: the user-fluid rows at the
+                              saved points (AbcmbOdeSettings.n_fluid > 0 only) */
 } AbcmbCosmo;
 
 /* One (cosmology, k-mode) ODE work item. */
@@ -95,13 +109,14 @@ typedef struct AbcmbPertItem {
  * tangent of the perturbation table comes back in pt.
  */
 #define ABCMB_TANGENT_DIRS 5
-#define ABCMB_COSMO_DOUBLES 31
+#define ABCMB_COSMO_DOUBLES (31 + ABCMB_FLUID_PARAMS)
 typedef struct AbcmbTangent {
   const double* par;      /* [ABCMB_TANGENT_DIRS][ABCMB_COSMO_DOUBLES] */
   const double* tau_tab;  /* [ABCMB_TANGENT_DIRS][ABCMB_TAU_TAB_N] */
   const double* xe_tab;   /* [ABCMB_TANGENT_DIRS][xe_n] */
   const double* tm_tab;   /* [ABCMB_TANGENT_DIRS][tm_n] */
   double* pt;             /* out: [ABCMB_TANGENT_DIRS][ABCMB_N_PT_FIELDS][n_save][nk] */
+  double* pt_fluid;       /* out: [ABCMB_TANGENT_DIRS][n_fluid][n_save][nk] (n_fluid > 0) */
 } AbcmbTangent;
 
 /*
@@ -127,6 +142,10 @@ typedef struct AbcmbOdeSettings {
   int32_t max_steps, max_newton;
   int32_t l_g, l_pol, l_ur, l_ncdm; /* hierarchy cut-offs */
   int32_t n_q, n;                   /* massive bins (0 if absent), state size */
+  int32_t n_fluid;                  /* rows of user-defined fluids, the last rows of
+                                       the state; nonzero only with a library built
+                                       for that fluid set (abcmb_fluid_signature) */
+  int32_t pad0;
 } AbcmbOdeSettings;
 
 /* tanh reionization of one cosmology (abcmb/background.py:939-1017): the
@@ -181,11 +200,23 @@ int abcmb_host_eval(const AbcmbCosmo* c, const double* lna, int n, double* out);
  * abcmb/background.py:774-936), for a cosmology filled by
  * abcmb_host_background. */
 int abcmb_host_drag(const AbcmbCosmo* c, double* z_d, double* rs_d);
+/* Signature of the user-fluid code this build of the host producer was
+ * compiled with ("" for the standard species only).  Replaces the
+ * reference's registration of Fluid classes (Model(user_species=...),
+ * abcmb/model_specs.py:89-116; abcmb/species.py:11-249): a model's fluids
+ * are turned into C++ once and the producer built for them. */
+const char* abcmb_host_fluid_signature(void);
 
 /* ---------------- GPU library (libabcmb_hip.so) ---------------------------- */
 
 const char* abcmb_last_error(void);
 int abcmb_device_init(void);
+/* Signature of the user-fluid code this build of the kernels was compiled
+ * with ("" for the standard species only); the ODE entry points accept
+ * settings.n_fluid > 0 only from a build whose fluid code has that many rows
+ * (the model's Fluid classes, abcmb/species.py:11-249, registered by
+ * Model(user_species=...), abcmb/model_specs.py:89-116). */
+const char* abcmb_fluid_signature(void);
 
 /*
  * Launch-shape settings that the autotuner (abcmb_amd/autotune.py) picks by

@@ -68,7 +68,36 @@ static double fast_interp(double x, double lo, double hi, const double* f, int n
 
 typedef struct {
   double a, de, cdm, b, g, ur, ncdm, p_ncdm, T_nu, x_nu;
+  double fl, p_fl, m_fl;  /* user fluids: density, pressure, matter density (summed) */
 } Rho;
+
+/* User-defined fluids (abcmb/species.py:11-249): the species methods are
+ * evaluated as written by the caller (tests: the model's Fluid objects called
+ * with numbers, their parameters read from the cosmology c) through these
+ * callbacks; their rows are the last n_rows rows of the state.  NULL: the
+ * standard species only. */
+typedef struct OracleFluids {
+  int n_rows;
+  void* ctx;
+  /* out[0..2]: summed rho, P and matter rho of the user fluids at ln a */
+  void (*bg)(void* ctx, const AbcmbCosmo* c, double lna, double* out);
+  /* out[0..3]: sum rho delta, (rho + P) theta, (rho + P) sigma and the matter
+   * rho delta of the user fluids in state y (n rows) at ln a */
+  void (*sums)(void* ctx, const AbcmbCosmo* c, double lna, const double* y, int n, double* out);
+  /* dy[0..n_rows): y_prime of the user rows at (k, ln a) given aH, h', eta' */
+  void (*rhs)(void* ctx, const AbcmbCosmo* c, double k, double lna, double aH, double hp, double ep,
+              const double* y, int n, double* dy);
+  /* y[0..n_rows): y_ini of the user rows at (k, tau) */
+  void (*ic)(void* ctx, const AbcmbCosmo* c, double k, double tau, double* y);
+} OracleFluids;
+static const OracleFluids* g_fl = NULL;
+
+/* installs (or, with NULL, removes) the user-fluid callbacks; the caller keeps
+ * them alive and runs the oracle single-threaded while they are set */
+int oracle_set_fluids(const OracleFluids* f) {
+  g_fl = f;
+  return 0;
+}
 
 /* rho of each species in species order (abcmb/species.py:454-1456) */
 static Rho rho_all(const AbcmbCosmo* c, double lna) {
@@ -76,8 +105,14 @@ static Rho rho_all(const AbcmbCosmo* c, double lna) {
   double a = exp(lna);
   double hc3 = pow(C_CM * HBAR, 3.0);
   r.a = a;
-  r.de = c->omega_Lambda * RHO_UNIT();
-  r.cdm = c->omega_cdm * RHO_UNIT() / pow(a, 3.0);
+  r.de = (c->species_off & ABCMB_NO_LAMBDA) ? 0.0 : c->omega_Lambda * RHO_UNIT();
+  r.cdm = (c->species_off & ABCMB_NO_CDM) ? 0.0 : c->omega_cdm * RHO_UNIT() / pow(a, 3.0);
+  r.fl = r.p_fl = r.m_fl = 0.0;
+  if (g_fl) {
+    double o[3];
+    g_fl->bg(g_fl->ctx, c, lna, o);
+    r.fl = o[0]; r.p_fl = o[1]; r.m_fl = o[2];
+  }
   r.b = c->omega_b * RHO_UNIT() / pow(a, 3.0);
   r.g = PI * PI / 15.0 * pow(c->TCMB0, 4.0) / pow(a, 4.0) / hc3;
   r.ur = c->N_ur * 2.0 * 7.0 / 8.0 * PI * PI / 30.0 * pow(c->T_ur, 4.0) * pow(c->TCMB0, 4.0) / pow(a, 4.0) / hc3;
@@ -99,7 +134,7 @@ static Rho rho_all(const AbcmbCosmo* c, double lna) {
 
 static double rho_tot(const Rho* r) {
   double s = 0.0;
-  s += r->de; s += r->cdm; s += r->b; s += r->g; s += r->ur; s += r->ncdm;
+  s += r->de; s += r->cdm; s += r->b; s += r->g; s += r->ur; s += r->ncdm; s += r->fl;
   return s;
 }
 
@@ -222,9 +257,16 @@ static void rhs(const Ctx* x, double lna, const double* y, double* dy) {
   srt += (r.g + r.g / 3.0) * y[x->ig + 1];
   srt += (r.ur + r.ur / 3.0) * y[x->iu + 1];
   if (c->has_ncdm && x->nq > 0) srt += nrt;
+  if (g_fl && g_fl->n_rows > 0) {
+    double o[4];
+    g_fl->sums(g_fl->ctx, c, lna, y, x->n, o);
+    srd += o[0];
+    srt += o[1];
+  }
   double hp = 2.0 / (aH * aH) * (k * k * y[0] + 4.0 * PI * GN * a * a / (C_MPC * C_MPC) * srd);
   double ep = 4.0 * PI * GN * a * a / aH / (k * k) * srt / (C_MPC * C_MPC);
   dy[0] = ep;
+  if (g_fl && g_fl->n_rows > 0) g_fl->rhs(g_fl->ctx, c, k, lna, aH, hp, ep, y, x->n, dy + x->n - g_fl->n_rows);
   /* CDM */
   dy[1] = -0.5 * hp;
   /* Baryon */
@@ -320,6 +362,7 @@ static void initial_conditions(const Ctx* x, double lna0, double* y) {
     P[1] = tn / 3. * q / (1. + exp(-q));
     P[2] = sn / 2. * q / (1. + exp(-q));
   }
+  if (g_fl && g_fl->n_rows > 0) g_fl->ic(g_fl->ctx, c, k, ti, y + x->n - g_fl->n_rows);
 }
 
 /* ---------- dense linear algebra ---------- */
@@ -709,6 +752,13 @@ static void output_fields(const Ctx* x, double lna, const double* y, double* o /
     srd += nrd; srt += nrt; srs += nrs;
     srdm += nrd; srm += r.ncdm;
   }
+  /* user fluids */
+  if (g_fl) {
+    double u[4] = {0.0, 0.0, 0.0, 0.0};
+    if (g_fl->n_rows > 0) g_fl->sums(g_fl->ctx, c, lna, y, x->n, u);
+    srd += u[0]; srt += u[1]; srs += u[2];
+    srdm += u[3]; srm += r.m_fl;
+  }
   double dm = srdm / srm;
   double hp = 2. / (aH * aH) * (k * k * y[0] + 4. * PI * GN * a * a / (C_MPC * C_MPC) * srd);
   double ep = 4. * PI * GN * a * a / aH * srt / (C_MPC * C_MPC) / (k * k);
@@ -762,6 +812,9 @@ int oracle_pert(const AbcmbCosmo* cosmos, const AbcmbPertItem* items, int n_item
     for (int s = 0; s < ns; ++s) {
       output_fields(&x, c->lna_save[s], S + (size_t)s * st->n, o);
       for (int f = 0; f < ABCMB_N_PT_FIELDS; ++f) c->pt[((size_t)f * ns + s) * c->nk + I->ik] = o[f];
+      /* the user-fluid rows as they are */
+      for (int j = 0; j < st->n_fluid && c->pt_fluid; ++j)
+        c->pt_fluid[((size_t)j * ns + s) * c->nk + I->ik] = S[(size_t)s * st->n + st->n - st->n_fluid + j];
     }
     if (!states) free(S);
   }

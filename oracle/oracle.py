@@ -53,8 +53,142 @@ def lib():
         L.oracle_lens.argtypes = [_P, _P, _I, _P, _P, _I, _P]
         L.oracle_set_threads.argtypes = [_I]
         L.oracle_pk_z.argtypes = [_P, _P, _I, _I, _P, _P, _I, _D, _D, _D, _D, _I, _D, _D, _D, _P]
+        L.oracle_set_fluids.argtypes = [_P]
         _lib = L
     return _lib
+
+
+_BG_CB = C.CFUNCTYPE(None, _P, _P, _D, C.POINTER(_D))
+_SUMS_CB = C.CFUNCTYPE(None, _P, _P, _D, C.POINTER(_D), _I, C.POINTER(_D))
+_RHS_CB = C.CFUNCTYPE(None, _P, _P, _D, _D, _D, _D, _D, C.POINTER(_D), _I, C.POINTER(_D))
+_IC_CB = C.CFUNCTYPE(None, _P, _P, _D, _D, C.POINTER(_D))
+
+
+class _OracleFluids(C.Structure):
+    _fields_ = [("n_rows", _I), ("ctx", _P), ("bg", _BG_CB), ("sums", _SUMS_CB), ("rhs", _RHS_CB), ("ic", _IC_CB)]
+
+
+class _NumBG:
+    """The background argument of y_prime in the numeric calls: aH as the oracle has it."""
+
+    def __init__(self, aH):
+        self.v = aH
+
+    def aH(self, lna, params=None):
+        return self.v
+
+
+class fluids:
+    """``with fluids(model, params):`` -- the oracle evaluates the model's user
+    fluids (abcmb_amd.species subclasses) by calling their methods with
+    numbers, as the reference calls them (abcmb/perturbations.py:201-248,
+    :318-404; abcmb/main.py:540-594 for the background), through C callbacks.
+    The oracle runs single-threaded meanwhile.  The parameters the methods see
+    are ``params`` with the cosmology fields (and the user fluids' parameter
+    slots) of the cosmology the oracle evaluates, so that the tangent-linear
+    integration's neighbouring cosmologies move them."""
+
+    def __init__(self, model, params):
+        from abcmb_amd import constants, fluidgen
+        self.model, self.p = model, dict(params)
+        fs = model.fluids
+        self.user = [f for f in fs.user] if fs is not None else []
+        self.rows = [f for f in self.user if int(f.num_equations) > 0]
+        self.n_rows = model.n_fluid
+        self.names = list(fs.params) if fs is not None else []
+        self.fields = fluidgen.PARAM_FIELDS
+        self.H0 = constants.H0_over_h
+        self.args_tail = (model.species_objs, {o.name: i for i, o in enumerate(model.species_objs)})
+        self._cache = {}
+
+    def _params(self, cptr):
+        from abcmb_amd import abi
+        c = abi.AbcmbCosmo.from_address(cptr)
+        key = bytes(c)[:abi.COSMO_DOUBLES * 8]
+        p = self._cache.get(key)
+        if p is None:
+            p = dict(self.p)
+            for name, fld in self.fields.items():
+                p[name] = getattr(c, fld)
+            p["H0"] = p["h"] * self.H0
+            for j, nm in enumerate(self.names):
+                p[nm] = c.fluid_par[j]
+            if len(self._cache) > 64:
+                self._cache.clear()
+            self._cache[key] = p
+        return p
+
+    def _bg(self, ctx, cptr, lna, out):
+        p = self._params(cptr)
+        r = pr = m = 0.0
+        for f in self.user:
+            rv = float(f.rho(lna, p))
+            r += rv
+            pr += float(f.P(lna, p))
+            if f.is_matter:
+                m += rv
+        out[0], out[1], out[2] = r, pr, m
+
+    def _sums(self, ctx, cptr, lna, y, n, out):
+        p = self._params(cptr)
+        yv = np.ctypeslib.as_array(y, (n,))
+        s = [0.0, 0.0, 0.0, 0.0]
+        for f in self.rows:
+            d = float(f.rho_delta(lna, yv, p))
+            s[0] += d
+            s[1] += float(f.rho_plus_P_theta(lna, yv, p))
+            s[2] += float(f.rho_plus_P_sigma(lna, yv, p))
+            if f.is_matter:
+                s[3] += d
+        for i in range(4):
+            out[i] = s[i]
+
+    def _rhs(self, ctx, cptr, k, lna, aH, hp, ep, y, n, dy):
+        p = self._params(cptr)
+        yv = np.ctypeslib.as_array(y, (n,))
+        args = (_NumBG(aH), p) + self.args_tail
+        vals = np.concatenate([np.asarray(f.y_prime(k, lna, hp, ep, yv, args), dtype=np.float64)
+                               for f in self.rows])
+        for i, v in enumerate(vals):
+            dy[i] = v
+
+    def _ic(self, ctx, cptr, k, tau, y):
+        p = self._params(cptr)
+        vals = np.concatenate([np.asarray(f.y_ini(k, tau, p), dtype=np.float64) for f in self.rows])
+        for i, v in enumerate(vals):
+            y[i] = v
+
+    def _guard(self, fn, out_len):
+        """A callback that records an exception (raised again on exit) and hands
+        the oracle NaNs instead of returning half-written outputs."""
+        def cb(*a):
+            try:
+                fn(*a)
+            except Exception as e:   # noqa: BLE001 -- re-raised in __exit__
+                if self.error is None:
+                    self.error = e
+                out, n = a[-1], out_len(a)
+                for i in range(n):
+                    out[i] = float("nan")
+        return cb
+
+    def __enter__(self):
+        self.error = None
+        self._cbs = (_BG_CB(self._guard(self._bg, lambda a: 3)), _SUMS_CB(self._guard(self._sums, lambda a: 4)),
+                     _RHS_CB(self._guard(self._rhs, lambda a: self.n_rows)),
+                     _IC_CB(self._guard(self._ic, lambda a: self.n_rows)))
+        self._st = _OracleFluids(self.n_rows, None, *self._cbs)
+        self._threads = lib().oracle_max_threads()
+        lib().oracle_set_threads(1)
+        lib().oracle_set_fluids(C.byref(self._st))
+        return self
+
+    def __exit__(self, *exc):
+        lib().oracle_set_fluids(None)
+        lib().oracle_set_threads(self._threads)
+        if self.error is not None and exc[0] is None:
+            raise RuntimeError("a user-fluid callback of the oracle failed") from self.error
+        return False
 
 
 def _p(a):

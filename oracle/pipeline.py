@@ -20,11 +20,13 @@ def run(model, plan, want_states=False, lensing=None, k_subset=None):
     kk = np.ascontiguousarray(plan.k if plan.k is not None else model.k_pt)
     nk, ns = kk.size, model.n_save
     pt = np.zeros((abi.N_PT_FIELDS, ns, nk))
+    pf = np.zeros((model.n_fluid, ns, nk)) if getattr(model, "n_fluid", 0) else None
     c = abi.AbcmbCosmo.from_buffer_copy(plan.bg.cosmo)
     lna = np.ascontiguousarray(plan.lna_save)
     c.lna_save = lna.ctypes.data
     c.k = kk.ctypes.data
     c.pt = pt.ctypes.data
+    c.pt_fluid = pf.ctypes.data if pf is not None else None
     c.n_save, c.nk = ns, nk
     cos = (abi.AbcmbCosmo * 1)(c)
     ks = np.arange(nk) if k_subset is None else np.asarray(k_subset)
@@ -32,8 +34,12 @@ def run(model, plan, want_states=False, lensing=None, k_subset=None):
     for m, ik in enumerate(ks):
         items[m] = abi.AbcmbPertItem(k=kk[ik], lna_start=plan.lna_start[ik], rtol=plan.rtol[ik],
                                      atol=plan.atol[ik], cosmo=0, ik=int(ik))
-    status, steps, states = O.pert(cos, items, model.settings, want_states=want_states)
-    out = dict(pt=pt, status=status, steps=steps, states=states, cosmo=c, keep=(lna, kk))
+    if getattr(model, "fluids", None) is not None:
+        with O.fluids(model, plan.params):
+            status, steps, states = O.pert(cos, items, model.settings, want_states=want_states)
+    else:
+        status, steps, states = O.pert(cos, items, model.settings, want_states=want_states)
+    out = dict(pt=pt, pt_fluid=pf, status=status, steps=steps, states=states, cosmo=c, keep=(lna, kk))
     if k_subset is not None:
         return out
     out.update(downstream(model, plan, pt, lensing=lensing))
@@ -152,7 +158,12 @@ def tangent_tables(model, base, plus, minus, steps, k_subset):
     parr = (abi.AbcmbCosmo * nd)(*pcs)
     marr = (abi.AbcmbCosmo * nd)(*mcs)
     dpt = np.zeros((nd, abi.N_PT_FIELDS, ns, nk))
-    st, nst = O.pert_tangent(cos, parr, marr, np.asarray(steps, dtype=np.float64), items, model.settings, dpt)
+    if getattr(model, "fluids", None) is not None:
+        with O.fluids(model, base.params):
+            st, nst = O.pert_tangent(cos, parr, marr, np.asarray(steps, dtype=np.float64), items, model.settings,
+                                     dpt)
+    else:
+        st, nst = O.pert_tangent(cos, parr, marr, np.asarray(steps, dtype=np.float64), items, model.settings, dpt)
     if np.any(st != 0):
         raise RuntimeError(f"oracle tangent integration failed: status {np.unique(st)}")
     return pt, dpt, nst

@@ -41,9 +41,11 @@ def test_libraries_load_without_gpu():
 
 def test_struct_layouts_match_header():
     # the ctypes mirrors must match the C sizes used by the kernels
-    assert C.sizeof(abi.AbcmbCosmo) == 320
+    assert C.sizeof(abi.AbcmbCosmo) == 400
+    assert abi.AbcmbCosmo.fluid_par.offset == 31 * 8 and abi.AbcmbCosmo.pt_fluid.offset == 392
     assert C.sizeof(abi.AbcmbPertItem) == 48
-    assert C.sizeof(abi.AbcmbOdeSettings) == 9 * 8 + 8 * 4
+    assert C.sizeof(abi.AbcmbOdeSettings) == 9 * 8 + 10 * 4
+    assert C.sizeof(abi.AbcmbTangent) == 48
 
 
 def test_hot_path_refuses_missing_library(monkeypatch, tmp_path):
