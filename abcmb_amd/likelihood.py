@@ -432,6 +432,9 @@ def adjoint_forward(model, thetas, k_grids=None):
     ``adjoint_backward`` and the observed spectra [n][3][n_l]."""
     if getattr(model, "device_background", False):
         raise NotImplementedError("the reverse-mode gradient takes the host background (device_background=False)")
+    if getattr(model, "fluids", None) is not None:
+        raise NotImplementedError("the reverse-mode gradient covers the standard species; a model with user "
+                                  "fluids takes forward-mode gradients (tangent_gradients, lensed_cl(mode='jacobian'))")
     thetas = np.atleast_2d(np.asarray(thetas, dtype=np.float64))
     k_grids = list(k_grids) if k_grids is not None else [None] * thetas.shape[0]
     plans = model.prepare_many([model.derived(theta_to_params(t)) for t in thetas], k_list=k_grids)

@@ -183,6 +183,7 @@ class Model:
         kwargs.pop("adjoint", None)  # reference option for diffrax AD; not used here
         specs = S.load_specs(kwargs)
         self.specs = specs
+        self.fixed_params = {}
         self._species_setup(user_species)
         k_pt, k_pk = S.get_k_axis_perturbations(specs)
         if specs.get("k_axis_perturbations") is not None:
@@ -316,9 +317,16 @@ class Model:
     def _hlib(self):
         return abi.host_lib(self._host_path)
 
+    def fix_params(self, **params):
+        """Parameters merged into every parameter set this model derives (a call's
+        own keys win): the user fluids' parameters when the caller varies only the
+        standard ones (the likelihood and gradient helpers)."""
+        self.fixed_params.update(params)
+        return self
+
     # ------------------------------------------------------------------ host side
     def derived(self, params: dict) -> dict:
-        return add_derived_parameters(params, self.species, self.specs)
+        return add_derived_parameters({**self.fixed_params, **params}, self.species, self.specs)
 
     def prepare(self, full_params: dict, freeze: CosmoPlan = None, k: np.ndarray = None) -> CosmoPlan:
         """Host background and ODE inputs of one cosmology.

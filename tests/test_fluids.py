@@ -238,3 +238,14 @@ def test_user_species_must_be_classes():
         Model(user_species=(EX.SelfInteractingDR(0, {}),), **SMALL)
     with pytest.raises(NotImplementedError, match="Photon"):
         Model(user_species=(SP.Baryon, SP.MasslessNeutrino), use_LCDM_species=False, **SMALL)
+
+
+def test_fixed_fluid_parameters_reach_every_parameter_set():
+    from abcmb_amd import likelihood as LK
+    m = Model(user_species=(EX.SelfInteractingDR,), **SMALL).fix_params(N_idr=0.3)
+    p = m.derived(LK.theta_to_params(LK.FIDUCIAL))
+    assert p["N_idr"] == 0.3
+    assert m.derived(dict(LK.theta_to_params(LK.FIDUCIAL), N_idr=0.1))["N_idr"] == 0.1
+    assert m.fluids.fluid_par(p)[0] == 0.3
+    with pytest.raises(NotImplementedError, match="forward-mode"):
+        LK.adjoint_forward(m, LK.FIDUCIAL[None])

@@ -148,3 +148,27 @@ def test_tangent_tables_with_fluids_match_oracle(which):
     err = np.abs(dg - do) / sc
     _say(f"max tangent error vs oracle {err.max():.2e}")
     assert err.max() < 1e-4
+
+
+def test_forward_mode_gradients_with_fluids():
+    """Likelihood gradients of a model with SIDR (its parameter held by
+    fix_params) from the tangent-linear path against central finite differences
+    of full evaluations at 100x the reference's ODE tolerances, where the two
+    converge together (5e-4 of the gradient norm for the standard build)."""
+    from abcmb_amd import likelihood as LK
+    tol = 1e-2
+    m = _model("sidr", lensing=True, l_max=1000, k_axis_perturbations=None, max_steps_PE=30000,
+               rtol_small_k_PE=1e-5 * tol, rtol_large_k_PE=1e-4 * tol, atol_small_k_PE=1e-10 * tol,
+               atol_large_k_PE=1e-6 * tol).fix_params(**PARAMS["sidr"])
+    thetas = LK.latin_hypercube(1, seed=2)
+    fid = m.execute([m.prepare(m.derived(LK.theta_to_params(LK.FIDUCIAL)))])[0]
+    like = LK.LensedClLikelihood(m, [fid.ClTT.cpu().numpy(), fid.ClTE.cpu().numpy(), fid.ClEE.cpu().numpy()],
+                                 n_mocks=1, seed=1)
+    _say("case: tangent vs finite-difference gradients with SIDR")
+    Lf, gf = LK.fd_gradients(m, like, thetas, batched=True)
+    Lt, gt = LK.tangent_gradients(m, like, thetas, downstream="fd")
+    assert np.allclose(Lf, Lt, rtol=1e-6, atol=0)
+    scale = np.linalg.norm(gf * thetas, axis=1, keepdims=True) / np.abs(thetas)
+    err = np.abs(gt - gf) / scale
+    _say(f"tangent vs FD, relative to the norm: {err.max():.2e}")
+    assert err.max() < 5e-4
