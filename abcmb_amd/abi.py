@@ -64,6 +64,7 @@ N_TANGENT_DIRS = 5
 FLUID_PARAMS = 8
 COSMO_DOUBLES = 31 + FLUID_PARAMS
 NO_LAMBDA, NO_CDM = 1, 2    # AbcmbCosmo.species_off bits
+REC_STAGES = 7              # recombination stage ends (ABCMB_REC_STAGES)
 
 
 class AbcmbReion(C.Structure):
@@ -110,13 +111,14 @@ def host_lib(path: str = None):
 
 
 def _host_setup(lib):
-    lib.abcmb_host_background.argtypes = [C.POINTER(AbcmbCosmo)] + [_D] * 6 + [C.POINTER(_I)]
+    lib.abcmb_host_background.argtypes = [C.POINTER(AbcmbCosmo)] + [_D] * 7 + [_P]
     lib.abcmb_host_recomb_points.argtypes = []
+    lib.abcmb_host_tm_points.argtypes = []
     lib.abcmb_host_los_background.argtypes = [C.POINTER(AbcmbCosmo), _P, C.c_int, _P]
     lib.abcmb_host_lna_start.argtypes = [C.POINTER(AbcmbCosmo), _P, C.c_int, _D, _D, _P]
     lib.abcmb_host_eval.argtypes = [C.POINTER(AbcmbCosmo), _P, C.c_int, _P]
     lib.abcmb_host_drag.argtypes = [C.POINTER(AbcmbCosmo), C.POINTER(_D), C.POINTER(_D)]
-    lib.abcmb_host_recombination.argtypes = [C.POINTER(AbcmbCosmo), C.POINTER(_I)]
+    lib.abcmb_host_recombination.argtypes = [C.POINTER(AbcmbCosmo), _D, _P]
     lib.abcmb_host_fluid_signature.restype = C.c_char_p
     lib.abcmb_host_fluid_signature.argtypes = []
     return lib

@@ -2,11 +2,9 @@
 
 ``Background`` plays the role of the reference's ``Background`` object
 (``abcmb/background.py:420-768``) for this library: it owns the conformal
-time table, the x_e / T_m histories, tau_0, lna_rec, lna_transfer_start and
-z_reion, produced by the native host producer ``libabcmb_host.so``.  The
-recombination history there is a Saha + Peebles stand-in for HyRex (the
-reference's HyRex cannot run in this environment); everything downstream of
-the tables follows the reference's formulas.
+time table, the x_e / T_m histories (HyRex, ``abcmb/hyrex/*.py``, restated in
+``csrc/hyrex.cpp``), tau_0, lna_rec, lna_transfer_start and z_reion, produced
+by the native host producer ``libabcmb_host.so``.
 """
 from __future__ import annotations
 
@@ -25,12 +23,13 @@ class Background:
     """Background of one cosmology (host-resident FP64 tables)."""
 
     def __init__(self, params: dict, has_ncdm: bool, n_q: int = 3, k_pivot: float = 0.05,
-                 input_tau_reion: bool = True, xe_pad: int = 0, rec_switch: int = -1,
+                 input_tau_reion: bool = True, xe_pad: int = 0, rec_switch=None,
                  lna_rec: float = None, lna_transfer_start: float = None, reionization: bool = True,
                  species_off: int = 0, fluid_par=None, lib=None):
-        """``rec_switch``, ``lna_rec``, ``lna_transfer_start``: grid choices held at
-        another cosmology's (finite differences with frozen grids); the producer
-        then keeps them instead of searching for its own.  ``reionization=False``:
+        """``rec_switch`` (the recombination stages' grid counts, ``REC_STAGES``
+        ints), ``lna_rec``, ``lna_transfer_start``: grid choices held at another
+        cosmology's (finite differences with frozen grids); the producer then
+        keeps them instead of searching for its own.  ``reionization=False``:
         the x_e table holds the recombination history alone and z_reion is 0
         (``abcmb_reionize`` adds the reionization on the GPU).  ``species_off``: the
         AbcmbCosmo.species_off bits; ``fluid_par``: the user fluids' parameter slots;
@@ -40,9 +39,10 @@ class Background:
         self.lib = lib
         self.params = params
         n_rec = lib.abcmb_host_recomb_points()
+        n_tm = lib.abcmb_host_tm_points()
         self.tau_tab = np.zeros(abi.TAU_TAB_N)
         self.xe_tab = np.zeros(n_rec + xe_pad)
-        self.tm_tab = np.zeros(n_rec + xe_pad)
+        self.tm_tab = np.zeros(n_tm + xe_pad)
         c = abi.AbcmbCosmo()
         p = params
         c.h, c.omega_b, c.omega_cdm, c.omega_Lambda = p["h"], p["omega_b"], p["omega_cdm"], p["omega_Lambda"]
@@ -55,7 +55,8 @@ class Background:
         if fluid_par is not None:
             c.fluid_par[:] = [float(v) for v in fluid_par]
         c.n_q = n_q if has_ncdm else 0
-        c.xe_n = c.tm_n = n_rec + xe_pad
+        c.xe_n = n_rec + xe_pad
+        c.tm_n = n_tm + xe_pad
         c.tau_tab = _ptr(self.tau_tab).value
         c.xe_tab = _ptr(self.xe_tab).value
         c.tm_tab = _ptr(self.tm_tab).value
@@ -65,17 +66,19 @@ class Background:
             tau_re, z_re = -1.0, p["z_reion"]
         if lna_rec is not None and lna_transfer_start is not None:
             c.lna_rec, c.lna_transfer_start = lna_rec, lna_transfer_start
-        sw = C.c_int32(rec_switch)
+        sw = np.full(abi.REC_STAGES, -1, dtype=np.int32)
+        if rec_switch is not None:
+            sw[:] = rec_switch
         if reionization:
             rc = lib.abcmb_host_background(C.byref(c), tau_re, z_re, p["Delta_z_reion"], p["z_reion_He"],
-                                           p["Delta_z_reion_He"], p["exp_reion"], C.byref(sw))
+                                           p["Delta_z_reion_He"], p["exp_reion"], p["Neff"], _ptr(sw))
         else:
-            rc = lib.abcmb_host_recombination(C.byref(c), C.byref(sw))
+            rc = lib.abcmb_host_recombination(C.byref(c), p["Neff"], _ptr(sw))
         self.reion = abi.AbcmbReion(tau_reion=tau_re, Delta_z_reion=p["Delta_z_reion"], z_reion_He=p["z_reion_He"],
                                     Delta_z_reion_He=p["Delta_z_reion_He"], exp_reion=p["exp_reion"])
         if rc != 0:
             raise RuntimeError(f"abcmb_host_background failed ({rc})")
-        self.rec_switch = sw.value
+        self.rec_switch = sw.copy()
         self.cosmo = c
         self.n_rec = n_rec
 

@@ -28,11 +28,54 @@ def _run(cmd):
     subprocess.run(cmd, check=True)
 
 
+HOST_SRCS = ("host_background.cpp", "hyrex.cpp")
+
+
+def sources(kind: str):
+    """The files a library is compiled from: "host" (the producer, its headers and
+    the HyRex tables) or "hip" (every kernel source and header)."""
+    inc = [os.path.join(ROOT, "include", "abcmb.h")]
+    if kind == "host":
+        return ([os.path.join(CSRC, f) for f in HOST_SRCS + ("hyrex.h", "fluid_none_host.h")] + inc +
+                sorted(glob.glob(os.path.join(HERE, "data", "hyrex", "*.dat"))))
+    return sorted(glob.glob(os.path.join(CSRC, "*.hip")) + glob.glob(os.path.join(CSRC, "*.h"))) + inc
+GEN = os.path.join(ROOT, "_build", "gen")
+
+
+def hyrex_tables():
+    """The HyRec tables (abcmb_amd/data/hyrex/*.dat) as a C++ include for hyrex.cpp,
+    written to _build/gen/hyrex_tables.inc."""
+    import numpy as np
+    d = os.path.join(HERE, "data", "hyrex")
+    alpha = np.loadtxt(os.path.join(d, "Alpha_inf.dat"))
+    r = np.loadtxt(os.path.join(d, "R_inf.dat"))
+    sw = np.loadtxt(os.path.join(d, "fit_swift.dat"))
+    if alpha.shape != (4000, 4) or r.shape != (100,) or sw.ndim != 2 or sw.shape[1] != 5:
+        raise ValueError("unexpected HyRex table shapes")
+    fmt = lambda a: ",".join(repr(float(v)) for v in np.asarray(a).ravel())
+    os.makedirs(GEN, exist_ok=True)
+    src = (f"// generated by abcmb_amd/build.py from abcmb_amd/data/hyrex/*.dat\n"
+           f"const double HX_ALPHA[4000][4] = {{{fmt(alpha)}}};\n"
+           f"const double HX_R[100] = {{{fmt(r)}}};\n"
+           f"constexpr int HX_SWIFT_N = {sw.shape[0]};\n"
+           f"const double HX_SWIFT[{sw.shape[0]}][5] = {{{fmt(sw)}}};\n")
+    path = os.path.join(GEN, "hyrex_tables.inc")
+    if not os.path.exists(path) or open(path).read() != src:
+        with open(path, "w") as fh:
+            fh.write(src)
+    return path
+
+
+def _host_cmd(out, extra=()):
+    hyrex_tables()
+    return (["g++", "-O2", "-fPIC", "-shared", "-std=c++17", "-fopenmp", f"-I{os.path.join(ROOT, 'include')}",
+             f"-I{GEN}"] + list(extra) + [os.path.join(CSRC, f) for f in HOST_SRCS] + ["-o", out])
+
+
 def build_host():
     out = os.path.join(HERE, "libabcmb_host.so")
-    _run(["g++", "-O2", "-fPIC", "-shared", "-std=c++17", "-fopenmp",
-          f"-I{os.path.join(ROOT, 'include')}",
-          os.path.join(CSRC, "host_background.cpp"), "-o", out])
+    _run(_host_cmd(out + ".tmp"))
+    os.replace(out + ".tmp", out)
     return out
 
 
@@ -67,8 +110,7 @@ def build_fluid_variant(directory: str, host: bool = True, hip: bool = True):
         raise FileNotFoundError(f"{directory} has no generated fluid headers")
     if host:
         tmp = os.path.join(directory, "libabcmb_host.so.tmp")
-        _run(["g++", "-O2", "-fPIC", "-shared", "-std=c++17", "-fopenmp", f"-I{os.path.join(ROOT, 'include')}",
-              f'-DABCMB_FLUID_HOST="{host_h}"', os.path.join(CSRC, "host_background.cpp"), "-o", tmp])
+        _run(_host_cmd(tmp, [f'-DABCMB_FLUID_HOST="{host_h}"']))
         os.replace(tmp, os.path.join(directory, "libabcmb_host.so"))
     if not hip:
         return None

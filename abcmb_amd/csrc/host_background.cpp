@@ -16,15 +16,15 @@
 //   kappa, exp(-kappa), visibility  abcmb/background.py:682-768
 //   lna_rec, lna_transfer_start     abcmb/background.py:555-564
 //   start time per k                abcmb/perturbations.py:119-159, :279
-// The recombination history is NOT HyRex (abcmb/hyrex/*): it is a Saha +
-// Peebles three-level-atom stand-in on HyRex's own ln a grid
-// (uniform 5e-4 from -ln 8001), because the reference cannot run here.
+//   recombination (x_e, T_m)        abcmb/hyrex/*.py, restated in hyrex.cpp,
+//                                   on the tables' reference layout
 #include <cmath>
 #include <cstring>
 #include <vector>
 #include <algorithm>
 
 #include "../../include/abcmb.h"
+#include "hyrex.h"
 
 namespace {
 
@@ -221,121 +221,8 @@ void tau_table(AbcmbCosmo& c, double* tab) {
   }
 }
 
-// ---- recombination stand-in: Saha + Peebles TLA on HyRex's grid ----
-const double REC_DLNA = 5.0e-4;
-const double REC_Z0 = 8000.0;
-int rec_points() {
-  // jnp.arange(-ln(1+z0), 0, 5e-4) (abcmb/hyrex/hyrex.py:139)
-  double lo = -std::log(1.0 + REC_Z0);
-  return (int)std::ceil((0.0 - lo) / REC_DLNA);
-}
-
-// Saha ionization: given T (eV), nH (cm^-3), returns x_e including helium.
-double saha_xe(double T, double nh, double fhe, double* xp_out) {
-  double lam = std::pow(2.0 * PI * MU_E * T, 1.5) / std::pow(2.0 * PI * HBAR * C_CM, 3.0);
-  double lame = std::pow(2.0 * PI * ME * T, 1.5) / std::pow(2.0 * PI * HBAR * C_CM, 3.0);
-  double sH = lam * std::exp(-RYD / T);
-  double sHe1 = 4.0 * lame * std::exp(-24.587387 / T);
-  double sHe2 = lame * std::exp(-54.417760 / T);
-  double lo = 0.0, hi = 1.0 + 2.0 * fhe;
-  double xp = 1.0;
-  for (int it = 0; it < 64; ++it) {
-    double xe = 0.5 * (lo + hi);
-    double ne = xe * nh;
-    xp = sH / (ne + sH);
-    double r1 = sHe1 / ne, r2 = sHe2 / ne;   // HeII/HeI, HeIII/HeII
-    double y0 = 1.0 / (1.0 + r1 + r1 * r2);
-    double yII = r1 * y0, yIII = r1 * r2 * y0;
-    double f = xp + fhe * (yII + 2.0 * yIII) - xe;
-    if (f > 0) lo = xe; else hi = xe;
-  }
-  if (xp_out) *xp_out = xp;
-  return 0.5 * (lo + hi);
-}
-
-double alpha_b(double T) {  // case-B recombination coefficient with RECFAST fudge 1.14
-  double t4 = T / KB * 1.0e-4;
-  return 1.14 * 4.309e-13 * std::pow(t4, -0.6166) / (1.0 + 0.6703 * std::pow(t4, 0.5300));
-}
-double beta_b(double T) {
-  double lam3 = std::pow(2.0 * PI * MU_E * T, 1.5) / std::pow(2.0 * PI * HBAR * C_CM, 3.0);
-  return alpha_b(T) * lam3 * std::exp(-RYD / 4.0 / T);
-}
-
-void recombination(const AbcmbCosmo& c, int N, double* xe, double* tm, int* switch_io) {
-  const double hc_pi3 = std::pow(2.0 * PI * HBAR * C_CM, 3.0);
-  const double k_lya = std::pow(C_CM / LYA_FREQ, 3);
-  double lo = -std::log(1.0 + REC_Z0);
-  double fhe = c.YHe * MH / MHE / (1.0 - c.YHe);
-  bool tla = false;
-  double xp = 1.0, T_m = 0.0;
-  for (int j = 0; j < N; ++j) {
-    double lna = lo + j * REC_DLNA;
-    double a = std::exp(lna);
-    double Tg = c.TCMB0 / a;
-    double nh = nH(c, lna);
-    double H = hubble(c, lna);
-    if (!tla) {
-      double xp_saha;
-      double xe_saha = saha_xe(Tg, nh, fhe, &xp_saha);
-      xe[j] = xe_saha;
-      xp = xp_saha;
-      double gc = gamma_compton(xe_saha, Tg, c.YHe);
-      T_m = Tg * (1.0 - H / gc);
-      tm[j] = T_m;
-      if ((*switch_io < 0 && xp_saha < 0.99) || (*switch_io >= 0 && j + 1 >= *switch_io)) {
-        tla = true;
-        if (*switch_io < 0) *switch_io = j + 1;
-      }
-      continue;
-    }
-    // advance x_p and T_m from lna - dlna to lna with backward-Euler substeps
-    const int NS = 2;
-    double h = REC_DLNA / NS;
-    double lprev = lna - REC_DLNA;
-    for (int s = 1; s <= NS; ++s) {
-      double l = lprev + s * h;
-      double al = std::exp(l);
-      double tg = c.TCMB0 / al;
-      double n1 = nH(c, l);
-      double Hs = hubble(c, l);
-      // helium from Saha given the current electron density (lagged)
-      double ne_guess = (xp + 0.0) * n1;
-      double lame = std::pow(2.0 * PI * ME * tg, 1.5) / hc_pi3;
-      double r1 = 4.0 * lame * std::exp(-24.587387 / tg) / std::max(ne_guess, 1e-300);
-      double xhe = fhe * r1 / (1.0 + r1);
-      // Newton on F(x) = x - xp_old - h * dxdlna(x); rates depend on T only
-      const double K = k_lya / (8.0 * PI * Hs);
-      const double bb = beta_b(tg);
-      const double ab = alpha_b(T_m);
-      const double boltz = std::exp(-E21 / tg);
-      double x = xp, xold = xp;
-      for (int it = 0; it < 30; ++it) {
-        double xe_ = x + xhe;
-        double x1s = std::max(1.0 - x, 1e-30);
-        double lam_a = 1.0 / (K * n1 * x1s);
-        double Cp = (R2S1S + lam_a) / (R2S1S + lam_a + bb);
-        double f = -Cp * (ab * n1 * xe_ * x - bb * (1.0 - x) * boltz) / Hs;
-        double d = -Cp * (ab * n1 * (xe_ + x) + bb * boltz) / Hs;
-        double F = x - xold - h * f;
-        double dF = 1.0 - h * d;
-        double dx = -F / dF;
-        x += dx;
-        if (x < 0) x = 1e-12;
-        if (std::fabs(dx) < 1e-14 * std::max(x, 1e-10)) break;
-      }
-      xp = x;
-      double xe_ = xp + xhe;
-      double gc = gamma_compton(xe_, tg, c.YHe) / Hs;
-      // backward Euler for dT/dlna = -2T + gc (Tg - T)
-      T_m = (T_m + h * gc * tg) / (1.0 + h * (2.0 + gc));
-    }
-    double lame = std::pow(2.0 * PI * ME * Tg, 1.5) / hc_pi3;
-    double r1 = 4.0 * lame * std::exp(-24.587387 / Tg) / std::max(xp * nh, 1e-300);
-    xe[j] = xp + fhe * r1 / (1.0 + r1);
-    tm[j] = T_m;
-  }
-}
+// ---- recombination: HyRex (hyrex.cpp) on its grid ----
+int rec_points() { return hyrex::XE_ALLOC; }
 
 // ---- reionization (abcmb/background.py:939-1017) ----
 double xe_reion(double lna, double z_re, double dz, double zHe, double dzHe, double ex, double YHe) {
@@ -416,27 +303,42 @@ double expmkappa(const Kappa& K, double l) { return l < -10.0 ? 0.0 : std::exp(-
 }  // namespace
 
 extern "C" int abcmb_host_recomb_points(void) { return rec_points(); }
+extern "C" int abcmb_host_tm_points(void) { return hyrex::TM_ALLOC; }
 
 extern "C" const char* abcmb_host_fluid_signature(void) { return fluid_host::kSignature; }
 
 namespace {
 int background(AbcmbCosmo* c, double tau_reion, double z_reion_in, double Delta_z_reion, double z_reion_He,
-               double Delta_z_reion_He, double exp_reion, int32_t* rec_switch, bool skip_reion) {
+               double Delta_z_reion_He, double exp_reion, double n_eff, int32_t* rec_switch, bool skip_reion) {
   if (!c || !c->tau_tab || !c->xe_tab || !c->tm_tab) return ABCMB_EINVAL;
-  int N = rec_points();
-  if (c->xe_n < N || c->tm_n != c->xe_n) return ABCMB_EINVAL;
+  if (c->xe_n < hyrex::XE_ALLOC || c->tm_n < hyrex::TM_ALLOC) return ABCMB_EINVAL;
   double* tau = const_cast<double*>(c->tau_tab);
   double* xe = const_cast<double*>(c->xe_tab);
   double* tm = const_cast<double*>(c->tm_tab);
   tau_table(*c, tau);
   c->tau0 = fast_interp(0.0, ABCMB_TAU_TAB_LO, 0.0, c->tau_tab, ABCMB_TAU_TAB_N);
-  int sw = rec_switch ? *rec_switch : -1;
-  recombination(*c, N, xe, tm, &sw);
-  if (rec_switch) *rec_switch = sw;
-  double x0 = -std::log(1.0 + REC_Z0);
-  c->xe_x0 = c->tm_x0 = x0;
-  c->xe_dx = c->tm_dx = REC_DLNA;
-  c->xe_lna_last = c->tm_lna_last = x0 + (N - 1) * REC_DLNA;
+  // HyRex's inputs on its grid (abcmb/background.py:88-99)
+  const double x0 = -std::log(1.0 + hyrex::Z0), dx = hyrex::DLNA;
+  std::vector<double> Tg(hyrex::N_GRID), nhg(hyrex::N_GRID), Hg(hyrex::N_GRID);
+  for (int j = 0; j < hyrex::N_GRID; ++j) {
+    const double l = x0 + j * dx;
+    Tg[j] = c->TCMB0 / std::exp(l);
+    nhg[j] = nH(*c, l);
+    Hg[j] = hubble(*c, l);
+  }
+  int32_t sw[hyrex::N_STAGES];
+  for (int s = 0; s < hyrex::N_STAGES; ++s) sw[s] = rec_switch ? rec_switch[s] : -1;
+  hyrex::History hist;
+  const int rc = hyrex::recombination(*c, n_eff, Tg.data(), nhg.data(), Hg.data(), hyrex::N_GRID, xe, c->xe_n, tm,
+                                      c->tm_n, sw, hist);
+  if (rc != ABCMB_OK) return rc;
+  if (rec_switch) for (int s = 0; s < hyrex::N_STAGES; ++s) rec_switch[s] = sw[s];
+  const int N = hist.n_xe;
+  c->xe_x0 = x0;
+  c->xe_dx = c->tm_dx = dx;
+  c->tm_x0 = hist.tm_x0;
+  c->xe_lna_last = x0 + (N - 1) * dx;
+  c->tm_lna_last = hist.tm_x0 + (hist.n_tm - 1) * dx;
   // reionization: solve tau_reion(z) = target (Newton/bisection), or take z
   double zr = z_reion_in;
   if (tau_reion > 0.0 && !skip_reion) {
@@ -461,13 +363,13 @@ int background(AbcmbCosmo* c, double tau_reion, double z_reion_in, double Delta_
   c->z_reion = zr;
   if (!skip_reion) {
     for (int j = 0; j < N; ++j) {
-      double l = x0 + j * REC_DLNA;
+      double l = x0 + j * dx;
       xe[j] += xe_reion(l, zr, Delta_z_reion, z_reion_He, Delta_z_reion_He, exp_reion, c->YHe);
     }
   }
-  for (int j = N; j < c->xe_n; ++j) { xe[j] = xe[N - 1]; tm[j] = tm[N - 1]; }
+  for (int j = N; j < c->xe_n; ++j) xe[j] = xe[N - 1];
   c->xe_last = xe[N - 1];
-  c->tm_last = tm[N - 1];
+  c->tm_last = tm[hist.n_tm - 1];
   // a caller that freezes the grids at another cosmology's (both set, as
   // negative ln a, on input) keeps them and skips the two scans below
   if (c->lna_rec < 0.0 && c->lna_transfer_start < 0.0) return ABCMB_OK;
@@ -502,17 +404,18 @@ int background(AbcmbCosmo* c, double tau_reion, double z_reion_in, double Delta_
 
 extern "C" int abcmb_host_background(AbcmbCosmo* c, double tau_reion, double z_reion_in,
                                      double Delta_z_reion, double z_reion_He,
-                                     double Delta_z_reion_He, double exp_reion, int32_t* rec_switch) {
-  return background(c, tau_reion, z_reion_in, Delta_z_reion, z_reion_He, Delta_z_reion_He, exp_reion, rec_switch,
-                    false);
+                                     double Delta_z_reion_He, double exp_reion, double n_eff,
+                                     int32_t* rec_switch) {
+  return background(c, tau_reion, z_reion_in, Delta_z_reion, z_reion_He, Delta_z_reion_He, exp_reion, n_eff,
+                    rec_switch, false);
 }
 
 // The same tables without the reionization: x_e is the recombination history
 // alone and z_reion is left 0; the scans for lna_rec and lna_transfer_start
 // do not see reionization (no free electrons from it at z > 50, and its
 // constant factor e^-tau_reio does not move the visibility's peak).
-extern "C" int abcmb_host_recombination(AbcmbCosmo* c, int32_t* rec_switch) {
-  return background(c, 0.0, 0.0, 0.5, 3.5, 0.5, 1.5, rec_switch, true);
+extern "C" int abcmb_host_recombination(AbcmbCosmo* c, double n_eff, int32_t* rec_switch) {
+  return background(c, 0.0, 0.0, 0.5, 3.5, 0.5, 1.5, n_eff, rec_switch, true);
 }
 
 extern "C" int abcmb_host_los_background(const AbcmbCosmo* c, const double* lna, int n, double* out) {

@@ -1,0 +1,568 @@
+// HyRex recombination (abcmb/hyrex/*.py) restated for the host producer.
+//
+// The reference's recombination module: helium (Saha HeIII/HeII, a
+// post-Saha HeII expansion, then the HeII rate equation; helium.py), then
+// hydrogen (a post-Saha expansion, the effective multilevel atom with the
+// two-photon correction fit, the EMLA with the matter temperature, and a
+// three-level atom at late times; hydrogen.py), with the effective rates of
+// recomb_functions.py interpolated from HyRec's tables (Alpha_inf, R_inf and
+// the fit_swift correction, shipped in abcmb_amd/data/hyrex/).  Every stage
+// writes the free-electron fraction on the uniform grid ln a = x0 + j dx,
+// x0 = -ln 8001, dx = 5e-4 (hyrex.py:139); stages hand over at the last
+// point they stored; the matter temperature is tabulated from the start of
+// the EMLA-with-T_m stage.  The tables are laid out as the reference's
+// concatenated arrays (array_with_padding.py): the same allocated lengths,
+// padded with the last value, so the lookups' span quirk
+// (abcmb/background.py:568-657) lands on the same points.
+//
+// Integration: the reference runs diffrax's adaptive Kvaerno3/PID at
+// rtol 1e-6 (1e-7 for the three-level atom) and stops each stage at the end
+// of the step where its event turns true.  Here each stage advances on the
+// save grid itself, one step per 5e-4 in ln a (two in the two-photon stage,
+// where hydrogen recombines) with Alexander's three-stage L-stable SDIRK of
+// order 3 (R. Alexander, SIAM J. Numer. Anal. 14, 1006, 1977) and chord
+// Newton to rounding, and stops at the first grid point where the event is
+// true.  The integration error is ~1e-8 of x_e, below the reference's
+// tolerance, and the result is a smooth function of the parameters with the
+// stage boundaries held (the tangent seeds difference it), at the price of the
+// reference's exact boundary points (its adaptive steps overshoot an event by
+// up to one step).
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "../../include/abcmb.h"
+#include "hyrex.h"
+
+namespace hyrex {
+namespace {
+
+// abcmb/constants.py
+constexpr double C_CM = 29979245800.0;
+constexpr double HBAR = 6.582119569509e-16;
+constexpr double KB = 8.617343e-5;
+constexpr double MP = 938271999.0;
+constexpr double ME = 510998.9461;
+constexpr double MH = MP + ME + 13.5982860719383;
+constexpr double MHE = 3.72839e9;
+constexpr double MU_E = MP * ME / MH;
+constexpr double E21 = 10.198714553953742;
+constexpr double RYD = 13.598286071938324;
+constexpr double SIGMA_T = 6.652458734e-25;
+constexpr double R2S1S = 8.2206;
+const double PI = 3.14159265358979323846;
+const double STEF = PI * PI / (60.0 * HBAR * HBAR * HBAR * C_CM * C_CM);
+const double LYA_FREQ = RYD * 3.0 / 4.0 / (2.0 * PI * HBAR);
+
+#include "hyrex_tables.inc"  // HX_ALPHA[4000][4], HX_R[100], HX_SWIFT[HX_SWIFT_N][5] (generated at build)
+
+// ---------------------------------------------------------------- dual numbers
+// value and one derivative, for the reference's jax.grad calls
+struct Dl {
+  double v, d;
+  Dl(double x = 0.0, double dx = 0.0) : v(x), d(dx) {}
+};
+inline Dl operator+(Dl a, Dl b) { return Dl(a.v + b.v, a.d + b.d); }
+inline Dl operator-(Dl a, Dl b) { return Dl(a.v - b.v, a.d - b.d); }
+inline Dl operator*(Dl a, Dl b) { return Dl(a.v * b.v, a.d * b.v + a.v * b.d); }
+inline Dl operator/(Dl a, Dl b) { return Dl(a.v / b.v, (a.d * b.v - a.v * b.d) / (b.v * b.v)); }
+inline Dl operator-(Dl a) { return Dl(-a.v, -a.d); }
+inline Dl operator+(Dl a, double b) { return Dl(a.v + b, a.d); }
+inline Dl operator+(double b, Dl a) { return Dl(a.v + b, a.d); }
+inline Dl operator-(Dl a, double b) { return Dl(a.v - b, a.d); }
+inline Dl operator-(double b, Dl a) { return Dl(b - a.v, -a.d); }
+inline Dl operator*(Dl a, double b) { return Dl(a.v * b, a.d * b); }
+inline Dl operator*(double b, Dl a) { return Dl(a.v * b, a.d * b); }
+inline Dl operator/(Dl a, double b) { return Dl(a.v / b, a.d / b); }
+inline Dl operator/(double b, Dl a) { return Dl(b / a.v, -b * a.d / (a.v * a.v)); }
+inline Dl exp(Dl a) { const double e = std::exp(a.v); return Dl(e, e * a.d); }
+inline Dl log(Dl a) { return Dl(std::log(a.v), a.d / a.v); }
+inline Dl sqrt(Dl a) { const double s = std::sqrt(a.v); return Dl(s, 0.5 * a.d / s); }
+inline Dl pow(Dl a, double p) { const double s = std::pow(a.v, p); return Dl(s, p * std::pow(a.v, p - 1.0) * a.d); }
+inline double val(double x) { return x; }
+inline double val(Dl x) { return x.v; }
+using std::exp;
+using std::log;
+using std::pow;
+using std::sqrt;
+
+// ---------------------------------------------------------------- inputs
+// RecombInputs (abcmb/hyrex/hyrex.py:16-90): T_CMB, n_H, H on the grid,
+// linearly interpolated (fast_interp with the grid's own end, no span quirk)
+struct Inputs {
+  double x0, x1;
+  int n;
+  std::vector<double> T, nh, H;
+  template <class V>
+  V at(const std::vector<double>& f, V x) const {
+    V i = (x - x0) / (x1 - x0) * (double)(n - 1);
+    const double eps = 1e-6;
+    if (val(i) < eps) i = V(eps);
+    if (val(i) > n - 1.0 - eps) i = V(n - 1.0 - eps);
+    const int il = (int)std::floor(val(i));
+    const int iu = il + 1 < n - 1 ? il + 1 : n - 1;
+    const V wu = i - (double)il;
+    return (1.0 - wu) * f[il] + wu * f[iu];
+  }
+};
+
+struct Par {
+  double YHe, fHe_num;    // fHe_num: YHe / (1 - YHe) / 3.97153 (helium.py)
+  double omega_b, omega_cdm, neff;
+};
+
+// ---------------------------------------------------------------- rates (recomb_functions.py)
+constexpr int NTR = 100, NTM = 40;
+const double TR_MIN = 0.004, TR_MAX = 0.4, T_RATIO_MIN = 0.1, T_RATIO_MAX = 1.0;
+
+struct Tables {
+  double lnA2s[NTR][NTM], lnA2p[NTR][NTM], lnR[NTR];
+  double swT[HX_SWIFT_N];
+  Tables() {
+    for (int i = 0; i < NTR; ++i) {
+      for (int j = 0; j < NTM; ++j) {
+        lnA2s[i][j] = std::log(HX_ALPHA[i * NTM + j][0]);
+        lnA2p[i][j] = std::log(HX_ALPHA[i * NTM + j][1]);
+      }
+      lnR[i] = std::log(HX_R[i]);
+    }
+    for (int i = 0; i < HX_SWIFT_N; ++i) swT[i] = KB * HX_SWIFT[i][0];
+  }
+};
+const Tables& tables() {
+  static const Tables t;
+  return t;
+}
+
+// jnp.interp(x, linspace(lo, hi, n), arange(n)): the fractional index, clamped
+template <class V>
+V axis_index(V x, double lo, double hi, int n) {
+  V i = (x - lo) / (hi - lo) * (double)(n - 1);
+  if (val(i) < 0.0) return V(0.0);
+  if (val(i) > n - 1.0) return V(n - 1.0);
+  return i;
+}
+
+// map_coordinates(log table, [i, j], order=1): bilinear, neighbours past the
+// last index weigh zero
+template <class V, class W>
+V bilinear(const double (*t)[NTM], V i, W j) {
+  const int i0 = (int)std::floor(val(i)), j0 = (int)std::floor(val(j));
+  const V fi = i - (double)i0;
+  const W fj = j - (double)j0;
+  auto g = [&](int a, int b) { return (a < NTR && b < NTM) ? t[a][b] : 0.0; };
+  return (1.0 - fi) * ((1.0 - fj) * g(i0, j0) + fj * g(i0, j0 + 1)) + fi * ((1.0 - fj) * g(i0 + 1, j0) + fj * g(i0 + 1, j0 + 1));
+}
+
+// jnp.interp on an ascending table, clamped
+template <class V>
+V interp1(V x, const double* xp, const double* fp, int n, int stride = 1) {
+  if (val(x) <= xp[0]) return V(fp[0]);
+  if (val(x) >= xp[n - 1]) return V(fp[(n - 1) * stride]);
+  int lo = 0, hi = n - 1;
+  while (hi - lo > 1) {
+    const int m = (lo + hi) / 2;
+    if (xp[m] <= val(x)) lo = m; else hi = m;
+  }
+  const V w = (x - xp[lo]) / (xp[hi] - xp[lo]);
+  return (1.0 - w) * fp[lo * stride] + w * fp[hi * stride];
+}
+
+template <class V>
+V alpha_H(V Tm) {
+  const V t4 = Tm / KB * 1.0e-4;
+  return 4.309e-13 * pow(t4, -0.6166) / (1.0 + 0.6703 * pow(t4, 0.5300));
+}
+
+template <class V>
+V beta_H(V T) {
+  const V lam = 2.0 * PI * HBAR * C_CM / sqrt(2.0 * PI * MU_E * T);
+  return 1.0 / (lam * lam * lam) * exp(-RYD / 4.0 / T) * alpha_H(T) / 4.0;
+}
+
+template <class V>
+V gamma_compton(V xe, double T, double YHe) {
+  const double FHe = YHe * MH / MHE / (1.0 - YHe);
+  const double T2 = T * T;
+  return xe / (1.0 + FHe + xe) * (8.0 * SIGMA_T * (4.0 * STEF) * T2 * T2 / (3.0 * ME));
+}
+
+struct Eff {
+  double A2s, A2p, B2s, B2p, R2p2s, R2s2p;
+};
+
+// the parts of the EMLA rate that depend on the time only (T, n_H, H)
+struct EmlaTime {
+  double t = NAN, T, nh, H, iT, boltz;
+  Eff e;
+};
+
+// effective_coefficients (recomb_functions.py:180-258): the ones the rate
+// equation reads (A2s, A2p, B2s, B2p, R2p2s, R2s2p)
+template <class W>
+void effective(double T, W Tm, Eff& e, W& A2s, W& A2p, double* iT_out = nullptr) {
+  const Tables& tb = tables();
+  const double lt = std::log(T);
+  const double iT = axis_index(lt, std::log(TR_MIN), std::log(TR_MAX), NTR);
+  const W ir = axis_index(Tm / T, T_RATIO_MIN, T_RATIO_MAX, NTM);
+  A2s = exp(bilinear(tb.lnA2s, iT, ir));
+  A2p = exp(bilinear(tb.lnA2p, iT, ir));
+  const double E2 = -RYD / 4.0;
+  const double q = std::pow(2.0 * PI * MU_E * T / ((2.0 * PI * HBAR) * (2.0 * PI * HBAR)), 1.5) /
+                   (C_CM * C_CM * C_CM);
+  const double last = NTM - 1.0;
+  e.B2s = std::exp(E2 / T) * q * std::exp(bilinear(tb.lnA2s, iT, last));
+  e.B2p = (1.0 / 3.0) * std::exp(E2 / T) * q * std::exp(bilinear(tb.lnA2p, iT, last));
+  // jnp.interp(log T, TR_axis, log R_tab): TR_axis is uniform
+  const double lo = std::log(TR_MIN), hi = std::log(TR_MAX);
+  double fr = (lt - lo) / (hi - lo) * (NTR - 1);
+  double lnr;
+  if (fr <= 0.0) lnr = tb.lnR[0];
+  else if (fr >= NTR - 1) lnr = tb.lnR[NTR - 1];
+  else {
+    const int k = (int)std::floor(fr);
+    const double w = fr - k;
+    lnr = (1.0 - w) * tb.lnR[k] + w * tb.lnR[k + 1];
+  }
+  e.R2p2s = std::exp(lnr);
+  e.R2s2p = 3.0 * e.R2p2s;
+  if (iT_out) *iT_out = iT;
+}
+
+void emla_time(const Inputs& in, double t, EmlaTime& tc) {
+  if (t == tc.t) return;
+  tc.t = t;
+  tc.T = in.at(in.T, t);
+  tc.nh = in.at(in.nh, t);
+  tc.H = in.at(in.H, t);
+  double a2s, a2p;
+  effective(tc.T, tc.T, tc.e, a2s, a2p, &tc.iT);
+  tc.boltz = std::exp(-E21 / tc.T);
+}
+
+// dxe/dlna of the EMLA with the two-photon correction Delta
+// (hydrogen.py:497-536, steady-state 2s/2p populations :577-636)
+template <class V, class W>
+V dxe_emla(V xe, double T, W Tm, double H, double nh, double Delta, const EmlaTime* tc = nullptr) {
+  Eff e;
+  W A2s, A2p;
+  if (tc) {
+    const Tables& tb = tables();
+    e = tc->e;
+    const W ir = axis_index(Tm / T, T_RATIO_MIN, T_RATIO_MAX, NTM);
+    A2s = exp(bilinear(tb.lnA2s, tc->iT, ir));
+    A2p = exp(bilinear(tb.lnA2p, tc->iT, ir));
+  } else {
+    effective(T, Tm, e, A2s, A2p);
+  }
+  const V x1s = 1.0 - xe;
+  const double lam = 2.0 * PI * HBAR * C_CM / RYD * 4.0 / 3.0;
+  const V RLya = 8.0 * PI * H / 3.0 / nh / x1s / (lam * lam * lam);
+  const V R2p1s = RLya / (1.0 + Delta);
+  const double boltz = tc ? tc->boltz : std::exp(-E21 / T);
+  const double R1s2s = boltz * R2S1S;
+  const V R1s2p = 3.0 * boltz * R2p1s;
+  // [[a, b], [c, d]] (x2s, x2p) = (s0, s1)
+  const double a = e.B2s + e.R2s2p + R2S1S, b = -e.R2p2s, c = -e.R2s2p;
+  const V d = e.B2p + e.R2p2s + R2p1s;
+  const V xe2n = xe * xe * nh;
+  const V s0 = xe2n * A2s + x1s * R1s2s;
+  const V s1 = xe2n * A2p + x1s * R1s2p;
+  const V det = a * d - b * c;
+  const V x2s = (s0 * d - b * s1) / det;
+  const V x2p = (a * s1 - c * s0) / det;
+  return (x2s * e.B2s + x2p * e.B2p - xe2n * (A2s + A2p)) / H;
+}
+
+// get_current_correction_func (hydrogen.py:538-575)
+double swift_delta(double T, const Par& p) {
+  const Tables& tb = tables();
+  const double omH = p.omega_b * (1.0 - p.YHe), omcb = p.omega_b + p.omega_cdm;
+  const double* col = &HX_SWIFT[0][0];
+  const double D = interp1(T, tb.swT, col + 1, HX_SWIFT_N, 5);
+  const double Dcb = interp1(T, tb.swT, col + 2, HX_SWIFT_N, 5);
+  const double DH = interp1(T, tb.swT, col + 3, HX_SWIFT_N, 5);
+  const double DN = interp1(T, tb.swT, col + 4, HX_SWIFT_N, 5);
+  return D + (omcb - 0.14175) * Dcb + (omH - 0.01689) * DH + (p.neff - 3.046) * DN;
+}
+
+// ---------------------------------------------------------------- helium (helium.py)
+template <class V>
+V xHeII_post_saha(const Inputs& in, const Par& p, V lna) {
+  const V T = in.at(in.T, lna), nh = in.at(in.nh, lna);
+  const V Tk = T / KB;
+  const V s = 4.0 * 2.414194e15 * Tk * sqrt(Tk) * exp(-285325.0 / Tk) / nh;
+  const double f = p.fHe_num;
+  return 2.0 * s * f / (1.0 + s) / (1.0 + sqrt(1.0 + 4.0 * s * f / ((1.0 + s) * (1.0 + s))));
+}
+
+template <class V>
+V xH1_saha(const Inputs& in, const Par& p, V lna) {
+  const V T = in.at(in.T, lna), nh = in.at(in.nh, lna);
+  const V xHeII = xHeII_post_saha(in, p, lna);
+  const V Tk = T / KB;
+  const V s = 2.4127161187130e15 * Tk * sqrt(Tk) * exp(-157801.37882 / Tk) / nh;
+  if (val(s) > 1e5) return (1.0 + xHeII) / s - (xHeII * xHeII + 3.0 * xHeII + 2.0) / (s * s);
+  if (val(s) == 0.0) return V(1.0);
+  const V u = 1.0 + xHeII / s;
+  return 1.0 - 2.0 / (u + sqrt(u * u + 4.0 / s));
+}
+
+// d x_e / d ln a from HeII recombination (helium.py:375-453)
+template <class V>
+V he_rate(const Inputs& in, const Par& p, V xe, double lna) {
+  const double f = p.fHe_num;
+  const double T = in.at(in.T, lna), nh = in.at(in.nh, lna), H = in.at(in.H, lna);
+  const double xH1 = xH1_saha(in, p, lna);
+  const V xHeII = xe - (1.0 - xH1);
+  const double Tk = T / KB;
+  const double s0 = 2.414194e15 * Tk * std::sqrt(Tk) / nh * 4.0;
+  const double s = s0 * std::exp(-285325.0 / Tk);
+  const double y2s = std::exp(46090.0 / Tk) / s0;
+  const double y2p = std::exp(39101.0 / Tk) / s0 * 3.0;
+  const double etacinv = 9.15776e22 * H / (nh * xH1);
+  const double g2pinc = 1.976e6 / (1.0 - std::exp(-6989.0 / Tk)) + 6.03e6 / (std::exp(19754.0 / Tk) - 1.0) +
+                        1.06e8 / (std::exp(21539.0 / Tk) - 1.0) + 2.18e6 / (std::exp(28496.0 / Tk) - 1.0) +
+                        3.37e7 / (std::exp(29224.0 / Tk) - 1.0) + 1.04e6 / (std::exp(32414.0 / Tk) - 1.0) +
+                        1.51e7 / (std::exp(32781.0 / Tk) - 1.0);
+  const V tau2p = 4.277e-8 * nh / H * (f - xHeII);
+  const V dnuline = g2pinc * tau2p / (4.0 * PI * PI);
+  const V tauc = dnuline / etacinv;
+  const V enh = sqrt(1.0 + PI * PI * tauc) + 7.74 * tauc / (1.0 + 70.0 * tauc);
+  V pesc = enh / tau2p;
+  pesc = pesc + (1.0 - exp(-1.023e-7 * tau2p)) *
+                    (0.964525 * std::exp(2947.0 / Tk) - enh * std::exp(-6.14e13 / etacinv)) / tau2p;
+  const V ydown = 50.94 * y2s + 1.7989e9 * y2p * pesc;
+  return ydown * ((f - xHeII) * s - xHeII * (xHeII + 1.0 - xH1)) / H;
+}
+
+// ---------------------------------------------------------------- integrator
+// Alexander's L-stable SDIRK3 on dy/dlna = f(lna, y), y of size D, one step of
+// h; stage equations by chord Newton with a difference Jacobian at the step
+// start, to rounding
+const double SG = 0.43586652150845899;
+const double SB1 = -(6.0 * SG * SG - 16.0 * SG + 1.0) / 4.0;
+const double SB2 = (6.0 * SG * SG - 20.0 * SG + 5.0) / 4.0;
+const double SC[3] = {SG, (1.0 + SG) / 2.0, 1.0};
+const double SA[3][3] = {{SG, 0.0, 0.0}, {(1.0 - SG) / 2.0, SG, 0.0}, {SB1, SB2, SG}};
+
+template <int D, class F>
+bool sdirk3_step(F f, double t, double h, double* y) {
+  double J[D][D], f0[D], k[3][D];
+  f(t, y, f0);
+  for (int j = 0; j < D; ++j) {
+    double yp[D];
+    std::memcpy(yp, y, sizeof(yp));
+    const double dy = 1e-7 * std::fabs(y[j]) + 1e-20;
+    yp[j] += dy;
+    double fp[D];
+    f(t, yp, fp);
+    for (int i = 0; i < D; ++i) J[i][j] = (fp[i] - f0[i]) / dy;
+  }
+  // M = I - h g J, inverted once (D <= 2)
+  double M[D][D], Mi[D][D];
+  for (int i = 0; i < D; ++i)
+    for (int j = 0; j < D; ++j) M[i][j] = (i == j ? 1.0 : 0.0) - h * SG * J[i][j];
+  if (D == 1) {
+    Mi[0][0] = 1.0 / M[0][0];
+  } else {
+    const double det = M[0][0] * M[D - 1][D - 1] - M[0][D - 1] * M[D - 1][0];
+    Mi[0][0] = M[D - 1][D - 1] / det; Mi[D - 1][D - 1] = M[0][0] / det;
+    Mi[0][D - 1] = -M[0][D - 1] / det; Mi[D - 1][0] = -M[D - 1][0] / det;
+  }
+  for (int s = 0; s < 3; ++s) {
+    double base[D], z[D];
+    for (int i = 0; i < D; ++i) {
+      base[i] = y[i];
+      for (int j = 0; j < s; ++j) base[i] += h * SA[s][j] * k[j][i];
+      z[i] = base[i] + h * SG * (s > 0 ? k[s - 1][i] : f0[i]);
+    }
+    const double ts = t + SC[s] * h;
+    bool conv = false;
+    for (int it = 0; it < 40; ++it) {
+      double fz[D], r[D], dz[D];
+      f(ts, z, fz);
+      for (int i = 0; i < D; ++i) r[i] = base[i] + h * SG * fz[i] - z[i];
+      double big = 0.0;
+      for (int i = 0; i < D; ++i) {
+        dz[i] = 0.0;
+        for (int j = 0; j < D; ++j) dz[i] += Mi[i][j] * r[j];
+        z[i] += dz[i];
+        big = std::fmax(big, std::fabs(dz[i]) / (std::fabs(z[i]) + 1e-300));
+      }
+      if (!(big == big)) return false;
+      if (big < 4e-16) { conv = true; break; }
+      if (it >= 2 && big < 1e-13) { conv = true; break; }
+    }
+    if (!conv) return false;
+    f(ts, z, k[s]);
+    if (s == 2) for (int i = 0; i < D; ++i) y[i] = z[i];
+  }
+  return true;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------- the history
+int recombination(const AbcmbCosmo& c, double neff, const double* Tgrid, const double* nhgrid,
+                  const double* Hgrid, int n_grid, double* xe, int xe_alloc, double* tm, int tm_alloc,
+                  int32_t* sw, History& out) {
+  const double x0 = -std::log(1.0 + Z0), dx = DLNA;
+  Inputs in;
+  in.x0 = x0;
+  in.n = n_grid;
+  in.x1 = x0 + (n_grid - 1) * dx;
+  in.T.assign(Tgrid, Tgrid + n_grid);
+  in.nh.assign(nhgrid, nhgrid + n_grid);
+  in.H.assign(Hgrid, Hgrid + n_grid);
+  Par p;
+  p.YHe = c.YHe;
+  p.fHe_num = c.YHe / (1.0 - c.YHe) / 3.97153;
+  p.omega_b = c.omega_b;
+  p.omega_cdm = c.omega_cdm;
+  p.neff = neff;
+  if (xe_alloc < XE_ALLOC || tm_alloc < TM_ALLOC) return ABCMB_EINVAL;
+  auto grid = [&](int j) { return x0 + j * dx; };
+  // stage ends: a frozen end (sw[s] >= 0) is kept, else the event decides
+  auto frozen = [&](int s) { return sw[s] >= 0; };
+  int j = 0;  // next grid index to store
+  // 1. HeIII -> HeII Saha on the grid up to z = 3601 (helium.py:156-233)
+  const int n4 = (int)std::floor((-std::log(3601.0) - x0) / dx + 1e-9) + 1;
+  for (int i = 0; i < n4; ++i) {
+    const double lna = grid(i);
+    const double T = in.at(in.T, lna), nh = in.at(in.nh, lna);
+    const double Tk = T / KB;
+    const double s = 2.414194e15 * Tk * std::sqrt(Tk) * std::exp(-631462.7 / Tk) / nh;
+    const double f = p.fHe_num;
+    const double xHeIII = 2.0 * s * f / (1.0 + s + f) / (1.0 + std::sqrt(1.0 + 4.0 * s * f / ((1.0 + s + f) * (1.0 + s + f))));
+    xe[j++] = 1.0 + f + xHeIII;
+    const bool stop = frozen(0) ? (i + 1 >= sw[0]) : (xHeIII < 1e-9);
+    if (stop) break;
+  }
+  sw[0] = j;
+  // 2. post-Saha HeII (helium.py:297-373), at most 4000 points
+  for (int iz = 0; iz < 4000; ++iz) {
+    const double lna = grid(j);
+    const double xHeII = xHeII_post_saha(in, p, lna);
+    const double xH1 = xH1_saha(in, p, lna);
+    const double xs = 1.0 - xH1 + xHeII;
+    const double g = he_rate(in, p, Dl(xs, 1.0), lna).d;
+    const double dsaha = xHeII_post_saha(in, p, Dl(lna, 1.0)).d;
+    const double x = xs + dsaha / g;
+    xe[j++] = x;
+    const bool stop = frozen(1) ? (j >= sw[1]) : (std::fabs(xs - x) > 1e-5);
+    if (stop) break;
+  }
+  sw[1] = j;
+  // 3. HeII rate equation until x_HeII < 1e-4 (helium.py:455-556), at most 4096 points
+  {
+    double y[1] = {xe[j - 1]};
+    auto f = [&](double t, const double* yy, double* dy) { dy[0] = he_rate(in, p, yy[0] + xH1_saha(in, p, t), t); };
+    for (int k = 0; k < 4096; ++k) {
+      const double t = grid(j - 1);
+      if (!sdirk3_step<1>(f, t, dx, y)) return ABCMB_EINVAL;
+      xe[j++] = y[0];
+      const double lna = grid(j - 1);
+      const bool stop = frozen(2) ? (j >= sw[2]) : (y[0] - (1.0 - xH1_saha(in, p, lna)) < 1e-4);
+      if (stop) break;
+    }
+  }
+  sw[2] = j;
+  // 4. hydrogen post-Saha expansion (hydrogen.py:190-276), at most 800 points
+  for (int iz = 0; iz < 800; ++iz) {
+    const double lna = grid(j);
+    const double T = in.at(in.T, lna), nh = in.at(in.nh, lna), H = in.at(in.H, lna);
+    const double ge = std::pow(2.0 * PI * MU_E * T, 1.5) / std::pow(2.0 * PI * HBAR * C_CM, 3.0) / nh;
+    const double s = ge * std::exp(-RYD / T);
+    const double xs = (std::sqrt(s * s + 4.0 * s) - s) / 2.0;
+    const double dsaha = -(RYD / T - 1.5) * xs * xs / (2.0 * xs + s);
+    const double g = dxe_emla(Dl(xs, 1.0), T, T, H, nh, 0.0).d;
+    const double x = xs + dsaha / g;
+    xe[j++] = x;
+    const bool stop = frozen(3) ? (j >= sw[3]) : (std::fabs(xs - x) > 1e-5);
+    if (stop) break;
+  }
+  sw[3] = j;
+  // 5. EMLA with the two-photon correction, T_m in equilibrium, to z = 701
+  //    (hydrogen.py:278-383), at most 2048 points
+  {
+    double y[1] = {xe[j - 1]};
+    EmlaTime tc;
+    double dl = 0.0, dl_t = NAN;
+    auto f = [&](double t, const double* yy, double* dy) {
+      emla_time(in, t, tc);
+      if (t != dl_t) { dl = swift_delta(tc.T, p); dl_t = t; }
+      const double Tm = tc.T * (1.0 - tc.H / gamma_compton(yy[0], tc.T, p.YHe));
+      dy[0] = dxe_emla(yy[0], tc.T, Tm, tc.H, tc.nh, dl, &tc);
+    };
+    const double t_end = -std::log(701.0);
+    for (int k = 0; k < 2048; ++k) {
+      // two steps per grid interval: hydrogen recombines here, x_e falls fastest
+      if (!sdirk3_step<1>(f, grid(j - 1), 0.5 * dx, y) || !sdirk3_step<1>(f, grid(j - 1) + 0.5 * dx, 0.5 * dx, y))
+        return ABCMB_EINVAL;
+      xe[j++] = y[0];
+      const bool stop = frozen(4) ? (j >= sw[4]) : (grid(j - 1) > t_end);
+      if (stop) break;
+    }
+  }
+  sw[4] = j;
+  // 6. EMLA with the matter temperature (hydrogen.py:385-495): T_m tabulated
+  //    from here; until T_CMB < 0.004 eV or the temperature ratio < 0.1; at most
+  //    8192 points
+  const int jt0 = j;  // first grid point of the T_m table
+  int jt = 0;
+  double y2[2];
+  {
+    const double t0 = grid(j - 1);
+    const double T0 = in.at(in.T, t0), H0 = in.at(in.H, t0);
+    y2[0] = xe[j - 1];
+    y2[1] = T0 * (1.0 - H0 / gamma_compton(y2[0], T0, p.YHe));
+    EmlaTime tc;
+    auto f = [&](double t, const double* yy, double* dy) {
+      emla_time(in, t, tc);
+      dy[0] = dxe_emla(yy[0], tc.T, yy[1], tc.H, tc.nh, 0.0, &tc);
+      dy[1] = (-2.0 * tc.H * yy[1] + gamma_compton(yy[0], tc.T, p.YHe) * (tc.T - yy[1])) / tc.H;
+    };
+    for (int k = 0; k < 8192; ++k) {
+      if (!sdirk3_step<2>(f, grid(j - 1), dx, y2)) return ABCMB_EINVAL;
+      xe[j++] = y2[0];
+      tm[jt++] = y2[1];
+      const double lna = grid(j - 1);
+      const double T = in.at(in.T, lna);
+      const double ratio = std::fmin(y2[1] / T, T / y2[1]);
+      const bool stop = frozen(5) ? (j >= sw[5]) : (T < TR_MIN || ratio < T_RATIO_MIN);
+      if (stop) break;
+    }
+  }
+  sw[5] = j;
+  // 7. three-level atom to ln a = 0 (hydrogen.py:664-766), at most 8192 points
+  {
+    auto f = [&](double t, const double* yy, double* dy) {
+      const double T = in.at(in.T, t), nh = in.at(in.nh, t), H = in.at(in.H, t);
+      const double xe_ = yy[0], Tm = yy[1];
+      const double rate = 8.0 * PI * H / (3.0 * (nh * std::pow(C_CM / LYA_FREQ, 3.0)));
+      const double exc = 3.0 * rate / 4.0 + (1.0 - xe_) * R2S1S / 4.0;
+      const double ion = (1.0 - xe_) * beta_H(in.at(in.T, std::log(1.0 / (1.0 + (std::exp(-t) - 1.0)))));
+      const double C = exc / (exc + ion);
+      dy[0] = C * (beta_H(Tm) * (1.0 - xe_) - alpha_H(Tm) * nh * xe_ * xe_) / H;
+      dy[1] = -2.0 * Tm + (gamma_compton(xe_, T, p.YHe) / H) * (T - Tm);
+    };
+    for (int k = 0; k < 8192; ++k) {
+      if (!sdirk3_step<2>(f, grid(j - 1), dx, y2)) return ABCMB_EINVAL;
+      xe[j++] = y2[0];
+      tm[jt++] = y2[1];
+      const bool stop = frozen(6) ? (j >= sw[6]) : (grid(j - 1) > 0.0);
+      if (stop) break;
+    }
+  }
+  sw[6] = j;
+  // padding with the last values (abcmb/background.py:541-549)
+  for (int i = j; i < xe_alloc; ++i) xe[i] = xe[j - 1];
+  for (int i = jt; i < tm_alloc; ++i) tm[i] = tm[jt - 1];
+  out.n_xe = j;
+  out.n_tm = jt;
+  out.tm_x0 = grid(jt0);
+  return ABCMB_OK;
+}
+
+}  // namespace hyrex

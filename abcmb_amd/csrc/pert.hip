@@ -323,21 +323,23 @@ __global__ void __launch_bounds__(REION_THREADS) reion_kernel(const AbcmbCosmo* 
     z_out[ci] = z;
     status[ci] = ok ? ABCMB_OK : ABCMB_EINVAL;
   }
-  // the table: reionization added on the recombination grid, the padding copies the last point
+  // the table: reionization added on the recombination grid's valid points
+  // (through xe_lna_last, at most n_rec), the padding copies the last of them
   const int n = c.xe_n;
   const Cosmo<N> P{c, T};
   const D<N> YHe = P(F_YHE);
   double* xo = xe_out + (size_t)ci * n;
-  for (int j = threadIdx.x; j < n_rec; j += blockDim.x) {
+  const int nv = max(1, min(n_rec, (int)llround((c.xe_lna_last - c.xe_x0) / c.xe_dx) + 1));
+  for (int j = threadIdx.x; j < nv; j += blockDim.x) {
     const double l = c.xe_x0 + j * c.xe_dx;
     const D<N> x = tab_at<N>(c.xe_tab, P.dxe(), n, j) + xe_reion<N>(l, zd, R, YHe);
     xo[j] = x.v;
     if constexpr (N > 0) { ABCMB_DLOOP(N) dxe_out[((size_t)ci * N + i_) * n + j] = x.d[i_]; }
   }
   __syncthreads();
-  for (int j = n_rec + threadIdx.x; j < n; j += blockDim.x) {
-    xo[j] = xo[n_rec - 1];
-    if constexpr (N > 0) { ABCMB_DLOOP(N) dxe_out[((size_t)ci * N + i_) * n + j] = dxe_out[((size_t)ci * N + i_) * n + n_rec - 1]; }
+  for (int j = nv + threadIdx.x; j < n; j += blockDim.x) {
+    xo[j] = xo[nv - 1];
+    if constexpr (N > 0) { ABCMB_DLOOP(N) dxe_out[((size_t)ci * N + i_) * n + j] = dxe_out[((size_t)ci * N + i_) * n + nv - 1]; }
   }
 }
 

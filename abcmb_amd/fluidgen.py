@@ -663,19 +663,39 @@ class FluidSet:
         with open(os.path.join(self.directory, "meta.json"), "w") as fh:
             json.dump(meta, fh, indent=1)
 
+    def _stamp(self, kind):
+        """Hash of everything a library of this set is compiled from (sources,
+        headers, tables and the generated header), so that a source change
+        rebuilds it whatever the files' times."""
+        from . import build as B
+        h = hashlib.sha256((self.host_src if kind == "host" else self.dev_src).encode())
+        for path in B.sources(kind):
+            with open(path, "rb") as fh:
+                h.update(fh.read())
+        return h.hexdigest()
+
+    def _fresh(self, kind):
+        lib = self.host_path if kind == "host" else self.hip_path
+        stamp = os.path.join(self.directory, kind + ".stamp")
+        return os.path.exists(lib) and os.path.exists(stamp) and open(stamp).read() == self._stamp(kind)
+
     def built(self, hip: bool = True) -> bool:
-        return os.path.exists(self.host_path) and (not hip or os.path.exists(self.hip_path))
+        return self._fresh("host") and (not hip or self._fresh("hip"))
 
     def build(self, force: bool = False, hip: bool = True):
         """Compile the host producer (seconds) and, with ``hip``, the kernel library
         (minutes: the ODE kernels are compiled again) for this fluid set, under
-        ``abcmb_amd/_fluids/<signature>/``."""
+        ``abcmb_amd/_fluids/<signature>/``, unless built from the same sources."""
         from . import build as B
         self.write()
-        if force or not os.path.exists(self.host_path):
-            B.build_fluid_variant(self.directory, hip=False)
-        if hip and (force or not os.path.exists(self.hip_path)):
-            B.build_fluid_variant(self.directory, host=False)
+        for kind, run in (("host", lambda: B.build_fluid_variant(self.directory, hip=False)),
+                          ("hip", lambda: B.build_fluid_variant(self.directory, host=False))):
+            if kind == "hip" and not hip:
+                continue
+            if force or not self._fresh(kind):
+                run()
+                with open(os.path.join(self.directory, kind + ".stamp"), "w") as fh:
+                    fh.write(self._stamp(kind))
 
 
 class _Emitter:

@@ -342,7 +342,7 @@ class Model:
                         input_tau_reion=sp["input_tau_reion"], species_off=self.species_off,
                         fluid_par=self.fluids.fluid_par(full_params) if self.fluids is not None else None,
                         lib=self._hlib(),
-                        rec_switch=freeze.bg.rec_switch if freeze is not None else -1,
+                        rec_switch=freeze.bg.rec_switch if freeze is not None else None,
                         lna_rec=freeze.bg.lna_rec if freeze is not None else None,
                         lna_transfer_start=freeze.bg.lna_transfer_start if freeze is not None else None,
                         reionization=not self.device_background)
@@ -437,8 +437,8 @@ class Model:
         self._device_constants()
         dev = torch.device("cuda")
         n, nd = len(plans), abi.N_TANGENT_DIRS
-        n_rec = self._hlib().abcmb_host_recomb_points()
         xe_n = plans[0].bg.xe_tab.size
+        n_rec = xe_n
         if any(pl.bg.xe_tab.size != xe_n for pl in plans):
             raise ValueError("complete_backgrounds: the plans' x_e tables differ in length")
         nl = self.n_save - 1

@@ -160,29 +160,37 @@ typedef struct AbcmbReion {
 
 /*
  * Fill the background tables and scalars of `c` from its parameters:
- * conformal time table (abcmb/background.py:252-319), a recombination
- * history x_e/T_m (stand-in for HyRex, abcmb/hyrex/hyrex.py:92-207) with the
- * tanh reionization added (abcmb/background.py:939-1017), lna_rec and
- * lna_transfer_start (abcmb/background.py:555-564).  c->tau_tab, xe_tab and
- * tm_tab must point to writable host arrays of ABCMB_TAU_TAB_N, xe_n and
- * tm_n doubles (xe_n = tm_n >= abcmb_host_recomb_points()).
- * tau_reion > 0 selects the tau-input reionization branch.  rec_switch
- * (in/out, may be NULL): grid index where the hydrogen history leaves Saha
- * equilibrium; < 0 on input lets the producer choose it (and reports it),
- * >= 0 forces it (finite differences hold it fixed, like every other
+ * conformal time table (abcmb/background.py:252-319), the recombination
+ * history x_e/T_m of HyRex (abcmb/hyrex/*.py) with the tanh reionization
+ * added (abcmb/background.py:939-1017), lna_rec and lna_transfer_start
+ * (abcmb/background.py:555-564).  c->tau_tab, xe_tab and tm_tab must point
+ * to writable host arrays of ABCMB_TAU_TAB_N, xe_n and tm_n doubles
+ * (xe_n >= abcmb_host_recomb_points(), tm_n >= abcmb_host_tm_points(): the
+ * reference's allocated lengths, which its lookups' span depends on; x_e and
+ * T_m are written from their first entries, the tails padded with the last
+ * value, and xe_lna_last / tm_x0 / tm_lna_last say where).  tau_reion > 0
+ * selects the tau-input reionization branch.  n_eff: N_eff of the
+ * cosmology, read by the two-photon correction fit (abcmb/hyrex/hydrogen.py:
+ * 538-575).  rec_switch (in/out, may be NULL): ABCMB_REC_STAGES grid counts,
+ * where each recombination stage handed over (HeIII Saha, post-Saha HeII,
+ * HeII rate equation, post-Saha H, two-photon EMLA, EMLA with T_m, three-level
+ * atom); < 0 on input lets the stage's own rule decide (and reports it),
+ * >= 0 forces it (finite differences hold them fixed, like every other
  * discrete grid choice).  c->lna_rec and c->lna_transfer_start are outputs,
  * unless both are negative on input: then they are kept (a caller that holds
  * the grids at another cosmology's passes that cosmology's values) and their
  * scans are skipped.
  */
+#define ABCMB_REC_STAGES 7
 int abcmb_host_background(AbcmbCosmo* c, double tau_reion, double z_reion_in,
                           double Delta_z_reion, double z_reion_He,
-                          double Delta_z_reion_He, double exp_reion,
+                          double Delta_z_reion_He, double exp_reion, double n_eff,
                           int32_t* rec_switch);
 int abcmb_host_recomb_points(void);
+int abcmb_host_tm_points(void);
 /* abcmb_host_background without the reionization: x_e holds the recombination
  * history alone and z_reion is 0 (abcmb_reionize adds the rest on the GPU). */
-int abcmb_host_recombination(AbcmbCosmo* c, int32_t* rec_switch);
+int abcmb_host_recombination(AbcmbCosmo* c, double n_eff, int32_t* rec_switch);
 
 /*
  * Background vectors on the LOS grid lna[0..n-1] (abcmb/spectrum.py:640-651):
@@ -493,7 +501,8 @@ int abcmb_lens_mfma(int n_cosmo, const double* cl_in_dev, const double* clpp_dev
  * (abcmb_host_recombination), solve tau_reion(z_reion) = reion_dev[i].tau_reion
  * on the 2000-point trapezoid over ln a in [-5, 0], and write the table with
  * the hydrogen and helium reionization added (xe_out_dev [n_cosmo][xe_n]: the
- * first n_rec entries, the padding copying entry n_rec - 1) and z_reion
+ * entries up to xe_lna_last (at most n_rec), the padding copying the last of
+ * them) and z_reion
  * (z_out_dev [n_cosmo]).  With tangents_dev (the cosmologies' AbcmbTangent
  * inputs, x_e tables of the recombination history), also the implicit-function
  * derivatives dz_reion/dtheta = (dtau_target/dtheta - dtau/dtheta|_z) /

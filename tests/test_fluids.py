@@ -131,23 +131,39 @@ def _split_models():
 def test_split_cdm_derived_parameters_and_background():
     """Cold dark matter split between the standard species and a user clone: the
     closure (abcmb/main.py:567-594) and the host producer's tables are those of
-    the unsplit model."""
+    the unsplit model, x_e up to the two-photon stage (HyRex's two-photon fit
+    reads params['omega_cdm'] itself, abcmb/hyrex/hydrogen.py:538-575, so from
+    there on the split model's history differs, in the reference too)."""
     lcdm, clone, pa, pb = _split_models()
     for key in ("omega_m", "omega_r", "omega_Lambda", "R_b", "R_nu", "om"):
         assert pb[key] == pytest.approx(pa[key], rel=1e-14)
     ba, bb = lcdm.prepare(pa).bg, clone.prepare(pb).bg
     assert bb.tau0 == pytest.approx(ba.tau0, rel=1e-12)
     np.testing.assert_allclose(bb.tau_tab, ba.tau_tab, rtol=1e-12)
-    np.testing.assert_allclose(bb.xe_tab, ba.xe_tab, rtol=1e-10, atol=1e-14)
+    assert np.array_equal(ba.rec_switch[:4], bb.rec_switch[:4])
+    n2g = ba.rec_switch[3]
+    np.testing.assert_allclose(bb.xe_tab[:n2g], ba.xe_tab[:n2g], rtol=1e-10, atol=1e-14)
 
 
 def test_oracle_fluid_callbacks_restate_the_standard_cdm():
     """The oracle's user-fluid hooks (metric sums, rows, initial conditions) with a
     clone of cold dark matter: every standard row's derivative is the unsplit
     model's and the clone's row is delta_c's, at random states (the clone's
-    entry set to delta_c)."""
+    entry set to delta_c).  Both read the unsplit model's background tables
+    (the split one's recombination differs through HyRex's omega_cdm-dependent
+    two-photon fit)."""
+    import ctypes as C
     lcdm, clone, pa, pb = _split_models()
-    plan_a, plan_b = lcdm.prepare(pa), clone.prepare(pb)
+    plan_a = lcdm.prepare(pa)
+    cb = abi.AbcmbCosmo.from_buffer_copy(plan_a.bg.cosmo)
+    cb.omega_cdm = pb["omega_cdm"]
+    cb.fluid_par[:] = clone.fluids.fluid_par(pb)
+
+    class _B:
+        pass
+    plan_b = _B()
+    plan_b.bg = _B()
+    plan_b.bg.cosmo = cb
     n = lcdm.settings.n
     assert clone.settings.n == n + 1 and clone.fluids.first_row == n
     rng = np.random.default_rng(5)
