@@ -68,7 +68,7 @@ def hyrex_tables():
 
 def _host_cmd(out, extra=()):
     hyrex_tables()
-    return (["g++", "-O2", "-fPIC", "-shared", "-std=c++17", "-fopenmp", f"-I{os.path.join(ROOT, 'include')}",
+    return (["g++", "-O3", "-fno-math-errno", "-fPIC", "-shared", "-std=c++17", "-fopenmp", f"-I{os.path.join(ROOT, 'include')}",
              f"-I{GEN}"] + list(extra) + [os.path.join(CSRC, f) for f in HOST_SRCS] + ["-o", out])
 
 

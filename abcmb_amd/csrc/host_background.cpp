@@ -452,19 +452,24 @@ extern "C" int abcmb_host_lna_start(const AbcmbCosmo* c, const double* k, int nk
     ah[i] = aH(*c, grid[i]);
     f1[i] = tau_c_of(*c, grid[i]) * ah[i];
   }
-  // numpy/jax interp: clamp outside, linear inside (xp ascending)
-  auto interp = [&](double x, const std::vector<double>& xp) {
-    if (x <= xp[0]) return grid[0];
-    if (x >= xp[n - 1]) return grid[n - 1];
-    int i = (int)(std::upper_bound(xp.begin(), xp.end(), x) - xp.begin()) - 1;
-    double t = (x - xp[i]) / (xp[i + 1] - xp[i]);
+  // numpy/jax interp: clamp outside, linear inside (xp ascending); xp(i) is
+  // evaluated where the search probes it
+  auto interp = [&](double x, auto xp) {
+    if (x <= xp(0)) return grid[0];
+    if (x >= xp(n - 1)) return grid[n - 1];
+    int lo = 0, hi = n;  // first index with xp > x
+    while (lo < hi) {
+      const int m = (lo + hi) / 2;
+      if (xp(m) > x) hi = m; else lo = m + 1;
+    }
+    const int i = lo - 1;
+    double t = (x - xp(i)) / (xp(i + 1) - xp(i));
     return grid[i] + t * (grid[i + 1] - grid[i]);
   };
-  double l1 = interp(R_tc, f1);
-  std::vector<double> f2(n);
+  double l1 = interp(R_tc, [&](int i) { return f1[i]; });
   for (int j = 0; j < nk; ++j) {
-    for (int i = 0; i < n; ++i) f2[i] = k[j] / ah[i];
-    double l2 = interp(R_large, f2);
+    const double kj = k[j];
+    double l2 = interp(R_large, [&](int i) { return kj / ah[i]; });
     double l = std::min(l1, l2);
     out[j] = std::min(l, -10.0);
   }

@@ -171,13 +171,20 @@ V interp1(V x, const double* xp, const double* fp, int n, int stride = 1) {
 template <class V>
 V alpha_H(V Tm) {
   const V t4 = Tm / KB * 1.0e-4;
-  return 4.309e-13 * pow(t4, -0.6166) / (1.0 + 0.6703 * pow(t4, 0.5300));
+  const V lt = log(t4);
+  return 4.309e-13 * exp(-0.6166 * lt) / (1.0 + 0.6703 * exp(0.5300 * lt));
+}
+
+// beta_H(T) given alpha_H(T)
+template <class V>
+V beta_H_from(V T, V alpha) {
+  const V lam = 2.0 * PI * HBAR * C_CM / sqrt(2.0 * PI * MU_E * T);
+  return 1.0 / (lam * lam * lam) * exp(-RYD / 4.0 / T) * alpha / 4.0;
 }
 
 template <class V>
 V beta_H(V T) {
-  const V lam = 2.0 * PI * HBAR * C_CM / sqrt(2.0 * PI * MU_E * T);
-  return 1.0 / (lam * lam * lam) * exp(-RYD / 4.0 / T) * alpha_H(T) / 4.0;
+  return beta_H_from(T, alpha_H(T));
 }
 
 template <class V>
@@ -198,21 +205,19 @@ struct EmlaTime {
 };
 
 // effective_coefficients (recomb_functions.py:180-258): the ones the rate
-// equation reads (A2s, A2p, B2s, B2p, R2p2s, R2s2p)
-template <class W>
-void effective(double T, W Tm, Eff& e, W& A2s, W& A2p, double* iT_out = nullptr) {
+// equation reads that depend on T alone (B2s, B2p, R2p2s, R2s2p), and the
+// fractional T index of the A2s / A2p table lookups
+void effective_T(double T, Eff& e, double& iT) {
   const Tables& tb = tables();
   const double lt = std::log(T);
-  const double iT = axis_index(lt, std::log(TR_MIN), std::log(TR_MAX), NTR);
-  const W ir = axis_index(Tm / T, T_RATIO_MIN, T_RATIO_MAX, NTM);
-  A2s = exp(bilinear(tb.lnA2s, iT, ir));
-  A2p = exp(bilinear(tb.lnA2p, iT, ir));
+  iT = axis_index(lt, std::log(TR_MIN), std::log(TR_MAX), NTR);
   const double E2 = -RYD / 4.0;
-  const double q = std::pow(2.0 * PI * MU_E * T / ((2.0 * PI * HBAR) * (2.0 * PI * HBAR)), 1.5) /
-                   (C_CM * C_CM * C_CM);
+  const double q0 = 2.0 * PI * MU_E * T / ((2.0 * PI * HBAR) * (2.0 * PI * HBAR));
+  const double q = q0 * std::sqrt(q0) / (C_CM * C_CM * C_CM);
   const double last = NTM - 1.0;
-  e.B2s = std::exp(E2 / T) * q * std::exp(bilinear(tb.lnA2s, iT, last));
-  e.B2p = (1.0 / 3.0) * std::exp(E2 / T) * q * std::exp(bilinear(tb.lnA2p, iT, last));
+  const double boltz2 = std::exp(E2 / T) * q;
+  e.B2s = boltz2 * std::exp(bilinear(tb.lnA2s, iT, last));
+  e.B2p = (1.0 / 3.0) * boltz2 * std::exp(bilinear(tb.lnA2p, iT, last));
   // jnp.interp(log T, TR_axis, log R_tab): TR_axis is uniform
   const double lo = std::log(TR_MIN), hi = std::log(TR_MAX);
   double fr = (lt - lo) / (hi - lo) * (NTR - 1);
@@ -226,7 +231,15 @@ void effective(double T, W Tm, Eff& e, W& A2s, W& A2p, double* iT_out = nullptr)
   }
   e.R2p2s = std::exp(lnr);
   e.R2s2p = 3.0 * e.R2p2s;
-  if (iT_out) *iT_out = iT;
+}
+
+// A2s, A2p at (T, T_m): bilinear in (log T, T_m / T) on the log tables
+template <class W>
+void effective_A(double T, double iT, W Tm, W& A2s, W& A2p) {
+  const Tables& tb = tables();
+  const W ir = axis_index(Tm / T, T_RATIO_MIN, T_RATIO_MAX, NTM);
+  A2s = exp(bilinear(tb.lnA2s, iT, ir));
+  A2p = exp(bilinear(tb.lnA2p, iT, ir));
 }
 
 void emla_time(const Inputs& in, double t, EmlaTime& tc) {
@@ -235,8 +248,7 @@ void emla_time(const Inputs& in, double t, EmlaTime& tc) {
   tc.T = in.at(in.T, t);
   tc.nh = in.at(in.nh, t);
   tc.H = in.at(in.H, t);
-  double a2s, a2p;
-  effective(tc.T, tc.T, tc.e, a2s, a2p, &tc.iT);
+  effective_T(tc.T, tc.e, tc.iT);
   tc.boltz = std::exp(-E21 / tc.T);
 }
 
@@ -245,16 +257,15 @@ void emla_time(const Inputs& in, double t, EmlaTime& tc) {
 template <class V, class W>
 V dxe_emla(V xe, double T, W Tm, double H, double nh, double Delta, const EmlaTime* tc = nullptr) {
   Eff e;
+  double iT;
   W A2s, A2p;
   if (tc) {
-    const Tables& tb = tables();
     e = tc->e;
-    const W ir = axis_index(Tm / T, T_RATIO_MIN, T_RATIO_MAX, NTM);
-    A2s = exp(bilinear(tb.lnA2s, tc->iT, ir));
-    A2p = exp(bilinear(tb.lnA2p, tc->iT, ir));
+    iT = tc->iT;
   } else {
-    effective(T, Tm, e, A2s, A2p);
+    effective_T(T, e, iT);
   }
+  effective_A(T, iT, Tm, A2s, A2p);
   const V x1s = 1.0 - xe;
   const double lam = 2.0 * PI * HBAR * C_CM / RYD * 4.0 / 3.0;
   const V RLya = 8.0 * PI * H / 3.0 / nh / x1s / (lam * lam * lam);
@@ -308,97 +319,159 @@ V xH1_saha(const Inputs& in, const Par& p, V lna) {
   return 1.0 - 2.0 / (u + sqrt(u * u + 4.0 / s));
 }
 
-// d x_e / d ln a from HeII recombination (helium.py:375-453)
-template <class V>
-V he_rate(const Inputs& in, const Par& p, V xe, double lna) {
-  const double f = p.fHe_num;
-  const double T = in.at(in.T, lna), nh = in.at(in.nh, lna), H = in.at(in.H, lna);
-  const double xH1 = xH1_saha(in, p, lna);
-  const V xHeII = xe - (1.0 - xH1);
+// the parts of the HeII rate that depend on the time only (helium.py:375-453)
+struct HeTime {
+  double t = NAN, xH1, H, s, y2s, y2p, etacinv, g2pinc, tau_pre, intercom, leak;
+};
+
+void he_time(const Inputs& in, const Par& p, double lna, HeTime& h) {
+  if (lna == h.t) return;
+  h.t = lna;
+  const double T = in.at(in.T, lna), nh = in.at(in.nh, lna);
+  h.H = in.at(in.H, lna);
+  h.xH1 = xH1_saha(in, p, lna);
   const double Tk = T / KB;
   const double s0 = 2.414194e15 * Tk * std::sqrt(Tk) / nh * 4.0;
-  const double s = s0 * std::exp(-285325.0 / Tk);
-  const double y2s = std::exp(46090.0 / Tk) / s0;
-  const double y2p = std::exp(39101.0 / Tk) / s0 * 3.0;
-  const double etacinv = 9.15776e22 * H / (nh * xH1);
-  const double g2pinc = 1.976e6 / (1.0 - std::exp(-6989.0 / Tk)) + 6.03e6 / (std::exp(19754.0 / Tk) - 1.0) +
-                        1.06e8 / (std::exp(21539.0 / Tk) - 1.0) + 2.18e6 / (std::exp(28496.0 / Tk) - 1.0) +
-                        3.37e7 / (std::exp(29224.0 / Tk) - 1.0) + 1.04e6 / (std::exp(32414.0 / Tk) - 1.0) +
-                        1.51e7 / (std::exp(32781.0 / Tk) - 1.0);
-  const V tau2p = 4.277e-8 * nh / H * (f - xHeII);
-  const V dnuline = g2pinc * tau2p / (4.0 * PI * PI);
-  const V tauc = dnuline / etacinv;
+  h.s = s0 * std::exp(-285325.0 / Tk);
+  h.y2s = std::exp(46090.0 / Tk) / s0;
+  h.y2p = std::exp(39101.0 / Tk) / s0 * 3.0;
+  h.etacinv = 9.15776e22 * h.H / (nh * h.xH1);
+  h.g2pinc = 1.976e6 / (1.0 - std::exp(-6989.0 / Tk)) + 6.03e6 / (std::exp(19754.0 / Tk) - 1.0) +
+             1.06e8 / (std::exp(21539.0 / Tk) - 1.0) + 2.18e6 / (std::exp(28496.0 / Tk) - 1.0) +
+             3.37e7 / (std::exp(29224.0 / Tk) - 1.0) + 1.04e6 / (std::exp(32414.0 / Tk) - 1.0) +
+             1.51e7 / (std::exp(32781.0 / Tk) - 1.0);
+  h.tau_pre = 4.277e-8 * nh / h.H;
+  h.intercom = 0.964525 * std::exp(2947.0 / Tk);
+  h.leak = std::exp(-6.14e13 / h.etacinv);
+}
+
+// d x_e / d ln a from HeII recombination at total x_e (helium.py:375-453)
+template <class V>
+V he_rate(const Par& p, const HeTime& h, V xe) {
+  const double f = p.fHe_num;
+  const V xHeII = xe - (1.0 - h.xH1);
+  const V tau2p = h.tau_pre * (f - xHeII);
+  const V dnuline = h.g2pinc * tau2p / (4.0 * PI * PI);
+  const V tauc = dnuline / h.etacinv;
   const V enh = sqrt(1.0 + PI * PI * tauc) + 7.74 * tauc / (1.0 + 70.0 * tauc);
   V pesc = enh / tau2p;
-  pesc = pesc + (1.0 - exp(-1.023e-7 * tau2p)) *
-                    (0.964525 * std::exp(2947.0 / Tk) - enh * std::exp(-6.14e13 / etacinv)) / tau2p;
-  const V ydown = 50.94 * y2s + 1.7989e9 * y2p * pesc;
-  return ydown * ((f - xHeII) * s - xHeII * (xHeII + 1.0 - xH1)) / H;
+  pesc = pesc + (1.0 - exp(-1.023e-7 * tau2p)) * (h.intercom - enh * h.leak) / tau2p;
+  const V ydown = 50.94 * h.y2s + 1.7989e9 * h.y2p * pesc;
+  return ydown * ((f - xHeII) * h.s - xHeII * (xHeII + 1.0 - h.xH1)) / h.H;
 }
 
 // ---------------------------------------------------------------- integrator
-// Alexander's L-stable SDIRK3 on dy/dlna = f(lna, y), y of size D, one step of
-// h; stage equations by chord Newton with a difference Jacobian at the step
-// start, to rounding
+// Alexander's L-stable SDIRK3 on dy/dlna = f(lna, y), y of size D, from t0 to
+// t1 (the last stage sits at t1 itself, so the next step's start reuses the
+// rate's time-only parts); stage equations by chord Newton to rounding,
+// stopped when the contraction rate puts the remaining error below 1e-16 of
+// the state.  The iteration matrix I - h g J, J by forward differences, is
+// kept for REUSE steps (it only sets the contraction rate, not the converged
+// stages) and rebuilt when a stage does not converge.  The method is stiffly
+// accurate, so the last stage's derivative is f at the step end and starts
+// the next step's predictor.
 const double SG = 0.43586652150845899;
 const double SB1 = -(6.0 * SG * SG - 16.0 * SG + 1.0) / 4.0;
 const double SB2 = (6.0 * SG * SG - 20.0 * SG + 5.0) / 4.0;
-const double SC[3] = {SG, (1.0 + SG) / 2.0, 1.0};
+const double SC[2] = {SG, (1.0 + SG) / 2.0};
 const double SA[3][3] = {{SG, 0.0, 0.0}, {(1.0 - SG) / 2.0, SG, 0.0}, {SB1, SB2, SG}};
 
 template <int D, class F>
-bool sdirk3_step(F f, double t, double h, double* y) {
-  double J[D][D], f0[D], k[3][D];
-  f(t, y, f0);
-  for (int j = 0; j < D; ++j) {
-    double yp[D];
-    std::memcpy(yp, y, sizeof(yp));
-    const double dy = 1e-7 * std::fabs(y[j]) + 1e-20;
-    yp[j] += dy;
-    double fp[D];
-    f(t, yp, fp);
-    for (int i = 0; i < D; ++i) J[i][j] = (fp[i] - f0[i]) / dy;
+class Sdirk3 {
+ public:
+  static constexpr int REUSE = 4;
+  explicit Sdirk3(F& f) : f_(f) {}
+
+  bool step(double t0, double t1, double* y) {
+    if (age_ < REUSE && attempt(t0, t1, y, false)) return true;
+    return attempt(t0, t1, y, true);
   }
-  // M = I - h g J, inverted once (D <= 2)
-  double M[D][D], Mi[D][D];
-  for (int i = 0; i < D; ++i)
-    for (int j = 0; j < D; ++j) M[i][j] = (i == j ? 1.0 : 0.0) - h * SG * J[i][j];
-  if (D == 1) {
-    Mi[0][0] = 1.0 / M[0][0];
-  } else {
-    const double det = M[0][0] * M[D - 1][D - 1] - M[0][D - 1] * M[D - 1][0];
-    Mi[0][0] = M[D - 1][D - 1] / det; Mi[D - 1][D - 1] = M[0][0] / det;
-    Mi[0][D - 1] = -M[0][D - 1] / det; Mi[D - 1][0] = -M[D - 1][0] / det;
-  }
-  for (int s = 0; s < 3; ++s) {
-    double base[D], z[D];
-    for (int i = 0; i < D; ++i) {
-      base[i] = y[i];
-      for (int j = 0; j < s; ++j) base[i] += h * SA[s][j] * k[j][i];
-      z[i] = base[i] + h * SG * (s > 0 ? k[s - 1][i] : f0[i]);
+
+ private:
+  F& f_;
+  double Mi_[D][D], fy_[D];
+  bool have_f_ = false;
+  int age_ = REUSE;
+
+  void refresh(double t0, double h, const double* y) {
+    double J[D][D];
+    f_(t0, y, fy_);
+    have_f_ = true;
+    for (int j = 0; j < D; ++j) {
+      double yp[D], fp[D];
+      std::memcpy(yp, y, sizeof(yp));
+      const double dy = 1e-7 * std::fabs(y[j]) + 1e-20;
+      yp[j] += dy;
+      f_(t0, yp, fp);
+      for (int i = 0; i < D; ++i) J[i][j] = (fp[i] - fy_[i]) / dy;
     }
-    const double ts = t + SC[s] * h;
-    bool conv = false;
-    for (int it = 0; it < 40; ++it) {
-      double fz[D], r[D], dz[D];
-      f(ts, z, fz);
-      for (int i = 0; i < D; ++i) r[i] = base[i] + h * SG * fz[i] - z[i];
-      double big = 0.0;
+    double M[D][D];
+    for (int i = 0; i < D; ++i)
+      for (int j = 0; j < D; ++j) M[i][j] = (i == j ? 1.0 : 0.0) - h * SG * J[i][j];
+    if (D == 1) {
+      Mi_[0][0] = 1.0 / M[0][0];
+    } else {
+      const double det = M[0][0] * M[D - 1][D - 1] - M[0][D - 1] * M[D - 1][0];
+      Mi_[0][0] = M[D - 1][D - 1] / det; Mi_[D - 1][D - 1] = M[0][0] / det;
+      Mi_[0][D - 1] = -M[0][D - 1] / det; Mi_[D - 1][0] = -M[D - 1][0] / det;
+    }
+    age_ = 0;
+  }
+
+  bool attempt(double t0, double t1, double* y, bool fresh) {
+    const double h = t1 - t0;
+    if (fresh) refresh(t0, h, y);
+    if (!have_f_) {
+      f_(t0, y, fy_);
+      have_f_ = true;
+    }
+    double k[3][D], z[D];
+    for (int s = 0; s < 3; ++s) {
+      double base[D];
       for (int i = 0; i < D; ++i) {
-        dz[i] = 0.0;
-        for (int j = 0; j < D; ++j) dz[i] += Mi[i][j] * r[j];
-        z[i] += dz[i];
-        big = std::fmax(big, std::fabs(dz[i]) / (std::fabs(z[i]) + 1e-300));
+        base[i] = y[i];
+        for (int j = 0; j < s; ++j) base[i] += h * SA[s][j] * k[j][i];
+        z[i] = base[i] + h * SG * (s > 0 ? k[s - 1][i] : fy_[i]);
       }
-      if (!(big == big)) return false;
-      if (big < 4e-16) { conv = true; break; }
-      if (it >= 2 && big < 1e-13) { conv = true; break; }
+      const double ts = s < 2 ? t0 + SC[s] * h : t1;
+      bool conv = false;
+      double prev = 0.0;
+      for (int it = 0; it < 40; ++it) {
+        double fz[D], r[D], dz[D];
+        f_(ts, z, fz);
+        for (int i = 0; i < D; ++i) r[i] = base[i] + h * SG * fz[i] - z[i];
+        double big = 0.0;
+        for (int i = 0; i < D; ++i) {
+          dz[i] = 0.0;
+          for (int j = 0; j < D; ++j) dz[i] += Mi_[i][j] * r[j];
+          z[i] += dz[i];
+          big = std::fmax(big, std::fabs(dz[i]) / (std::fabs(z[i]) + 1e-300));
+        }
+        if (!(big == big)) break;
+        if (big < 4e-16) { conv = true; break; }
+        // what remains after this update is about theta / (1 - theta) of it
+        if (it >= 1) {
+          const double theta = big / prev;
+          if (theta < 0.5 && theta / (1.0 - theta) * big < 1e-16) { conv = true; break; }
+        }
+        prev = big;
+      }
+      if (!conv) return false;
+      // the stage derivative from the converged stage equation z = base + h g k
+      for (int i = 0; i < D; ++i) k[s][i] = (z[i] - base[i]) / (h * SG);
     }
-    if (!conv) return false;
-    f(ts, z, k[s]);
-    if (s == 2) for (int i = 0; i < D; ++i) y[i] = z[i];
+    for (int i = 0; i < D; ++i) {
+      y[i] = z[i];
+      fy_[i] = k[2][i];
+    }
+    ++age_;
+    return true;
   }
-  return true;
+};
+
+template <int D, class F>
+Sdirk3<D, F> stepper(F& f) {
+  return Sdirk3<D, F>(f);
 }
 
 }  // namespace
@@ -446,7 +519,9 @@ int recombination(const AbcmbCosmo& c, double neff, const double* Tgrid, const d
     const double xHeII = xHeII_post_saha(in, p, lna);
     const double xH1 = xH1_saha(in, p, lna);
     const double xs = 1.0 - xH1 + xHeII;
-    const double g = he_rate(in, p, Dl(xs, 1.0), lna).d;
+    HeTime ht;
+    he_time(in, p, lna, ht);
+    const double g = he_rate(p, ht, Dl(xs, 1.0)).d;
     const double dsaha = xHeII_post_saha(in, p, Dl(lna, 1.0)).d;
     const double x = xs + dsaha / g;
     xe[j++] = x;
@@ -457,10 +532,16 @@ int recombination(const AbcmbCosmo& c, double neff, const double* Tgrid, const d
   // 3. HeII rate equation until x_HeII < 1e-4 (helium.py:455-556), at most 4096 points
   {
     double y[1] = {xe[j - 1]};
-    auto f = [&](double t, const double* yy, double* dy) { dy[0] = he_rate(in, p, yy[0] + xH1_saha(in, p, t), t); };
+    // the state is advanced with x_e = state + x_H1(Saha) handed to the rate, as
+    // xe_derivative_HeII does (helium.py:458-482)
+    HeTime ht;
+    auto f = [&](double t, const double* yy, double* dy) {
+      he_time(in, p, t, ht);
+      dy[0] = he_rate(p, ht, yy[0] + ht.xH1);
+    };
+    auto st = stepper<1>(f);
     for (int k = 0; k < 4096; ++k) {
-      const double t = grid(j - 1);
-      if (!sdirk3_step<1>(f, t, dx, y)) return ABCMB_EINVAL;
+      if (!st.step(grid(j - 1), grid(j), y)) return ABCMB_EINVAL;
       xe[j++] = y[0];
       const double lna = grid(j - 1);
       const bool stop = frozen(2) ? (j >= sw[2]) : (y[0] - (1.0 - xH1_saha(in, p, lna)) < 1e-4);
@@ -496,10 +577,11 @@ int recombination(const AbcmbCosmo& c, double neff, const double* Tgrid, const d
       dy[0] = dxe_emla(yy[0], tc.T, Tm, tc.H, tc.nh, dl, &tc);
     };
     const double t_end = -std::log(701.0);
+    auto st = stepper<1>(f);
     for (int k = 0; k < 2048; ++k) {
       // two steps per grid interval: hydrogen recombines here, x_e falls fastest
-      if (!sdirk3_step<1>(f, grid(j - 1), 0.5 * dx, y) || !sdirk3_step<1>(f, grid(j - 1) + 0.5 * dx, 0.5 * dx, y))
-        return ABCMB_EINVAL;
+      const double tm = grid(j - 1) + 0.5 * dx;
+      if (!st.step(grid(j - 1), tm, y) || !st.step(tm, grid(j), y)) return ABCMB_EINVAL;
       xe[j++] = y[0];
       const bool stop = frozen(4) ? (j >= sw[4]) : (grid(j - 1) > t_end);
       if (stop) break;
@@ -523,8 +605,9 @@ int recombination(const AbcmbCosmo& c, double neff, const double* Tgrid, const d
       dy[0] = dxe_emla(yy[0], tc.T, yy[1], tc.H, tc.nh, 0.0, &tc);
       dy[1] = (-2.0 * tc.H * yy[1] + gamma_compton(yy[0], tc.T, p.YHe) * (tc.T - yy[1])) / tc.H;
     };
+    auto st = stepper<2>(f);
     for (int k = 0; k < 8192; ++k) {
-      if (!sdirk3_step<2>(f, grid(j - 1), dx, y2)) return ABCMB_EINVAL;
+      if (!st.step(grid(j - 1), grid(j), y2)) return ABCMB_EINVAL;
       xe[j++] = y2[0];
       tm[jt++] = y2[1];
       const double lna = grid(j - 1);
@@ -537,18 +620,33 @@ int recombination(const AbcmbCosmo& c, double neff, const double* Tgrid, const d
   sw[5] = j;
   // 7. three-level atom to ln a = 0 (hydrogen.py:664-766), at most 8192 points
   {
+    // time-only parts cached per time: T, n_H, H, the Lyman-alpha rate and the
+    // photo-ionisation rate from n = 2 at the radiation temperature
+    struct {
+      double t = NAN, T, nh, H, rate, bT;
+    } tt;
+    const double lya3 = std::pow(C_CM / LYA_FREQ, 3.0);
     auto f = [&](double t, const double* yy, double* dy) {
-      const double T = in.at(in.T, t), nh = in.at(in.nh, t), H = in.at(in.H, t);
+      if (t != tt.t) {
+        tt.t = t;
+        tt.T = in.at(in.T, t);
+        tt.nh = in.at(in.nh, t);
+        tt.H = in.at(in.H, t);
+        tt.rate = 8.0 * PI * tt.H / (3.0 * (tt.nh * lya3));
+        tt.bT = beta_H(in.at(in.T, std::log(1.0 / (1.0 + (std::exp(-t) - 1.0)))));
+      }
+      const double T = tt.T, nh = tt.nh, H = tt.H;
       const double xe_ = yy[0], Tm = yy[1];
-      const double rate = 8.0 * PI * H / (3.0 * (nh * std::pow(C_CM / LYA_FREQ, 3.0)));
-      const double exc = 3.0 * rate / 4.0 + (1.0 - xe_) * R2S1S / 4.0;
-      const double ion = (1.0 - xe_) * beta_H(in.at(in.T, std::log(1.0 / (1.0 + (std::exp(-t) - 1.0)))));
+      const double exc = 3.0 * tt.rate / 4.0 + (1.0 - xe_) * R2S1S / 4.0;
+      const double ion = (1.0 - xe_) * tt.bT;
       const double C = exc / (exc + ion);
-      dy[0] = C * (beta_H(Tm) * (1.0 - xe_) - alpha_H(Tm) * nh * xe_ * xe_) / H;
+      const double aTm = alpha_H(Tm);
+      dy[0] = C * (beta_H_from(Tm, aTm) * (1.0 - xe_) - aTm * nh * xe_ * xe_) / H;
       dy[1] = -2.0 * Tm + (gamma_compton(xe_, T, p.YHe) / H) * (T - Tm);
     };
+    auto st = stepper<2>(f);
     for (int k = 0; k < 8192; ++k) {
-      if (!sdirk3_step<2>(f, grid(j - 1), dx, y2)) return ABCMB_EINVAL;
+      if (!st.step(grid(j - 1), grid(j), y2)) return ABCMB_EINVAL;
       xe[j++] = y2[0];
       tm[jt++] = y2[1];
       const bool stop = frozen(6) ? (j >= sw[6]) : (grid(j - 1) > 0.0);

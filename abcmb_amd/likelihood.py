@@ -135,6 +135,18 @@ def with_primordial(plan, A_s: float, n_s: float):
     return dataclasses.replace(plan, params=dict(plan.params, A_s=A_s, n_s=n_s), bg=bg)
 
 
+def _pm_params(model, theta, step):
+    """Derived parameters at theta +- step along each ODE direction, in the order
+    (direction, +), (direction, -)."""
+    out = []
+    for p in ODE_DIRS:
+        for sgn in (+1, -1):
+            t = theta.copy()
+            t[p] += sgn * step[p]
+            out.append(model.derived(theta_to_params(t)))
+    return out
+
+
 def _seed_inputs(model, thetas, base_plans, k_grids, steps):
     """Plans at theta +- step along the ODE directions (grids frozen at the base
     plans) and the tangent seeds of the host inputs from them."""
@@ -142,13 +154,10 @@ def _seed_inputs(model, thetas, base_plans, k_grids, steps):
     nc = thetas.shape[0]
     pm_params, freeze, ks = [], [], []
     for c in range(nc):
-        for p in ODE_DIRS:
-            for sgn in (+1, -1):
-                t = thetas[c].copy()
-                t[p] += sgn * steps[c, p]
-                pm_params.append(model.derived(theta_to_params(t)))
-                freeze.append(base_plans[c])
-                ks.append(k_grids[c])
+        q = _pm_params(model, thetas[c], steps[c])
+        pm_params += q
+        freeze += [base_plans[c]] * len(q)
+        ks += [k_grids[c]] * len(q)
     pm = model.prepare_many(pm_params, freeze_list=freeze, k_list=ks)
     nd = len(ODE_DIRS)
     pm = [pm[c * 2 * nd:(c + 1) * 2 * nd] for c in range(nc)]
@@ -156,12 +165,30 @@ def _seed_inputs(model, thetas, base_plans, k_grids, steps):
     return pm, seeds
 
 
-def _tangent_inputs(model, thetas, k_grids, steps):
+def _tangent_inputs(model, thetas, k_grids, steps, threads: int = None):
     """Host side of the tangent gradients: base plans, plans at theta +- step along
-    the ODE directions (grids frozen at the base), and the tangent seeds."""
+    the ODE directions (grids frozen at the base), and the tangent seeds.  One
+    thread pool: each cosmology's frozen plans are queued as soon as its base
+    plan is ready, so no thread waits for the slowest base."""
+    import os
+    from concurrent.futures import ThreadPoolExecutor, as_completed
+    from .model import tangent_seeds
     nc = thetas.shape[0]
-    base_plans = model.prepare_many([model.derived(theta_to_params(t)) for t in thetas], k_list=k_grids)
-    pm, seeds = _seed_inputs(model, thetas, base_plans, k_grids, steps)
+    k_grids = list(k_grids) if k_grids is not None else [None] * nc
+    threads = threads or min(16, os.cpu_count() or 1)
+    base_plans, pm = [None] * nc, [None] * nc
+    with ThreadPoolExecutor(threads) as ex:
+        fb = {ex.submit(model.prepare, model.derived(theta_to_params(thetas[c])), None, k_grids[c]): c
+              for c in range(nc)}
+        fz = {}
+        for f in as_completed(fb):
+            c = fb[f]
+            base_plans[c] = f.result()
+            fz[c] = [ex.submit(model.prepare, q, base_plans[c], k_grids[c])
+                     for q in _pm_params(model, thetas[c], steps[c])]
+        for c in range(nc):
+            pm[c] = [f.result() for f in fz[c]]
+    seeds = [tangent_seeds(pm[c][0::2], pm[c][1::2], [steps[c, p] for p in ODE_DIRS]) for c in range(nc)]
     if getattr(model, "device_background", False):
         # recombination-only tables: the GPU adds the reionization, its
         # implicit-function derivatives replace the x_e and z_reion seeds
