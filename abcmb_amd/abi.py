@@ -119,6 +119,10 @@ def _host_setup(lib):
     lib.abcmb_host_eval.argtypes = [C.POINTER(AbcmbCosmo), _P, C.c_int, _P]
     lib.abcmb_host_drag.argtypes = [C.POINTER(AbcmbCosmo), C.POINTER(_D), C.POINTER(_D)]
     lib.abcmb_host_recombination.argtypes = [C.POINTER(AbcmbCosmo), _D, _P]
+    lib.abcmb_host_background_tangent.argtypes = ([C.POINTER(AbcmbCosmo), _D, _P] + [_D] * 6 + [_P, _P, C.c_int]
+                                                  + [_P] * 4)
+    lib.abcmb_host_los_background_tangent.argtypes = [C.POINTER(AbcmbCosmo)] + [_P] * 5 + [C.c_int, _P, _P]
+    lib.abcmb_host_eval_tangent.argtypes = [C.POINTER(AbcmbCosmo)] + [_P] * 5 + [C.c_int, _P, _P]
     lib.abcmb_host_fluid_signature.restype = C.c_char_p
     lib.abcmb_host_fluid_signature.argtypes = []
     return lib

@@ -36,7 +36,7 @@ def sources(kind: str):
     the HyRex tables) or "hip" (every kernel source and header)."""
     inc = [os.path.join(ROOT, "include", "abcmb.h")]
     if kind == "host":
-        return ([os.path.join(CSRC, f) for f in HOST_SRCS + ("hyrex.h", "fluid_none_host.h")] + inc +
+        return ([os.path.join(CSRC, f) for f in HOST_SRCS + ("hyrex.h", "host_dual.h", "fluid_none_host.h")] + inc +
                 sorted(glob.glob(os.path.join(HERE, "data", "hyrex", "*.dat"))))
     return sorted(glob.glob(os.path.join(CSRC, "*.hip")) + glob.glob(os.path.join(CSRC, "*.h"))) + inc
 GEN = os.path.join(ROOT, "_build", "gen")

@@ -29,13 +29,23 @@
 // up to one step).
 #include <cmath>
 #include <cstring>
+#include <type_traits>
 #include <vector>
 
 #include "../../include/abcmb.h"
+#include "host_dual.h"
 #include "hyrex.h"
 
 namespace hyrex {
 namespace {
+
+using abcmb_ad::dbl;
+using abcmb_ad::Dl;
+using abcmb_ad::is_dl;
+using std::exp;
+using std::log;
+using std::pow;
+using std::sqrt;
 
 // abcmb/constants.py
 constexpr double C_CM = 29979245800.0;
@@ -56,59 +66,40 @@ const double LYA_FREQ = RYD * 3.0 / 4.0 / (2.0 * PI * HBAR);
 
 #include "hyrex_tables.inc"  // HX_ALPHA[4000][4], HX_R[100], HX_SWIFT[HX_SWIFT_N][5] (generated at build)
 
-// ---------------------------------------------------------------- dual numbers
-// value and one derivative, for the reference's jax.grad calls
-struct Dl {
-  double v, d;
-  Dl(double x = 0.0, double dx = 0.0) : v(x), d(dx) {}
-};
-inline Dl operator+(Dl a, Dl b) { return Dl(a.v + b.v, a.d + b.d); }
-inline Dl operator-(Dl a, Dl b) { return Dl(a.v - b.v, a.d - b.d); }
-inline Dl operator*(Dl a, Dl b) { return Dl(a.v * b.v, a.d * b.v + a.v * b.d); }
-inline Dl operator/(Dl a, Dl b) { return Dl(a.v / b.v, (a.d * b.v - a.v * b.d) / (b.v * b.v)); }
-inline Dl operator-(Dl a) { return Dl(-a.v, -a.d); }
-inline Dl operator+(Dl a, double b) { return Dl(a.v + b, a.d); }
-inline Dl operator+(double b, Dl a) { return Dl(a.v + b, a.d); }
-inline Dl operator-(Dl a, double b) { return Dl(a.v - b, a.d); }
-inline Dl operator-(double b, Dl a) { return Dl(b - a.v, -a.d); }
-inline Dl operator*(Dl a, double b) { return Dl(a.v * b, a.d * b); }
-inline Dl operator*(double b, Dl a) { return Dl(a.v * b, a.d * b); }
-inline Dl operator/(Dl a, double b) { return Dl(a.v / b, a.d / b); }
-inline Dl operator/(double b, Dl a) { return Dl(b / a.v, -b * a.d / (a.v * a.v)); }
-inline Dl exp(Dl a) { const double e = std::exp(a.v); return Dl(e, e * a.d); }
-inline Dl log(Dl a) { return Dl(std::log(a.v), a.d / a.v); }
-inline Dl sqrt(Dl a) { const double s = std::sqrt(a.v); return Dl(s, 0.5 * a.d / s); }
-inline Dl pow(Dl a, double p) { const double s = std::pow(a.v, p); return Dl(s, p * std::pow(a.v, p - 1.0) * a.d); }
-inline double val(double x) { return x; }
-inline double val(Dl x) { return x.v; }
-using std::exp;
-using std::log;
-using std::pow;
-using std::sqrt;
+// The code below is written over a scalar type S for everything that depends
+// on the cosmological parameters: double for the values, abcmb_ad::Dual<N>
+// for values with their parameter derivatives.  Times (ln a on the grid) stay
+// double.  Where a local derivative is taken (the post-Saha expansions), the
+// quantity is an abcmb_ad::Dl over S; Prom<S, V> is the type of a function of
+// S inputs at a time or state of type V (S itself, or Dl<S>).
+template <class S, class V>
+using Prom = std::conditional_t<is_dl<V>::value, V, S>;
 
 // ---------------------------------------------------------------- inputs
 // RecombInputs (abcmb/hyrex/hyrex.py:16-90): T_CMB, n_H, H on the grid,
 // linearly interpolated (fast_interp with the grid's own end, no span quirk)
+template <class S>
 struct Inputs {
   double x0, x1;
   int n;
-  std::vector<double> T, nh, H;
+  std::vector<S> T, nh, H;
   template <class V>
-  V at(const std::vector<double>& f, V x) const {
+  Prom<S, V> at(const std::vector<S>& f, V x) const {
     V i = (x - x0) / (x1 - x0) * (double)(n - 1);
     const double eps = 1e-6;
-    if (val(i) < eps) i = V(eps);
-    if (val(i) > n - 1.0 - eps) i = V(n - 1.0 - eps);
-    const int il = (int)std::floor(val(i));
+    if (dbl(i) < eps) i = V(eps);
+    if (dbl(i) > n - 1.0 - eps) i = V(n - 1.0 - eps);
+    const int il = (int)std::floor(dbl(i));
     const int iu = il + 1 < n - 1 ? il + 1 : n - 1;
     const V wu = i - (double)il;
     return (1.0 - wu) * f[il] + wu * f[iu];
   }
 };
 
+template <class S>
 struct Par {
-  double YHe, fHe_num;    // fHe_num: YHe / (1 - YHe) / 3.97153 (helium.py)
-  double omega_b, omega_cdm, neff;
+  S YHe, fHe_num;    // fHe_num: YHe / (1 - YHe) / 3.97153 (helium.py)
+  S omega_b, omega_cdm, neff;
 };
 
 // ---------------------------------------------------------------- rates (recomb_functions.py)
@@ -138,16 +129,16 @@ const Tables& tables() {
 template <class V>
 V axis_index(V x, double lo, double hi, int n) {
   V i = (x - lo) / (hi - lo) * (double)(n - 1);
-  if (val(i) < 0.0) return V(0.0);
-  if (val(i) > n - 1.0) return V(n - 1.0);
+  if (dbl(i) < 0.0) return V(0.0);
+  if (dbl(i) > n - 1.0) return V(n - 1.0);
   return i;
 }
 
 // map_coordinates(log table, [i, j], order=1): bilinear, neighbours past the
 // last index weigh zero
 template <class V, class W>
-V bilinear(const double (*t)[NTM], V i, W j) {
-  const int i0 = (int)std::floor(val(i)), j0 = (int)std::floor(val(j));
+auto bilinear(const double (*t)[NTM], V i, W j) {
+  const int i0 = (int)std::floor(dbl(i)), j0 = (int)std::floor(dbl(j));
   const V fi = i - (double)i0;
   const W fj = j - (double)j0;
   auto g = [&](int a, int b) { return (a < NTR && b < NTM) ? t[a][b] : 0.0; };
@@ -157,12 +148,12 @@ V bilinear(const double (*t)[NTM], V i, W j) {
 // jnp.interp on an ascending table, clamped
 template <class V>
 V interp1(V x, const double* xp, const double* fp, int n, int stride = 1) {
-  if (val(x) <= xp[0]) return V(fp[0]);
-  if (val(x) >= xp[n - 1]) return V(fp[(n - 1) * stride]);
+  if (dbl(x) <= xp[0]) return V(fp[0]);
+  if (dbl(x) >= xp[n - 1]) return V(fp[(n - 1) * stride]);
   int lo = 0, hi = n - 1;
   while (hi - lo > 1) {
     const int m = (lo + hi) / 2;
-    if (xp[m] <= val(x)) lo = m; else hi = m;
+    if (xp[m] <= dbl(x)) lo = m; else hi = m;
   }
   const V w = (x - xp[lo]) / (xp[hi] - xp[lo]);
   return (1.0 - w) * fp[lo * stride] + w * fp[hi * stride];
@@ -187,77 +178,83 @@ V beta_H(V T) {
   return beta_H_from(T, alpha_H(T));
 }
 
-template <class V>
-V gamma_compton(V xe, double T, double YHe) {
-  const double FHe = YHe * MH / MHE / (1.0 - YHe);
-  const double T2 = T * T;
+template <class V, class S>
+V gamma_compton(V xe, S T, S YHe) {
+  const S FHe = YHe * MH / MHE / (1.0 - YHe);
+  const S T2 = T * T;
   return xe / (1.0 + FHe + xe) * (8.0 * SIGMA_T * (4.0 * STEF) * T2 * T2 / (3.0 * ME));
 }
 
+template <class S>
 struct Eff {
-  double A2s, A2p, B2s, B2p, R2p2s, R2s2p;
+  S A2s, A2p, B2s, B2p, R2p2s, R2s2p;
 };
 
 // the parts of the EMLA rate that depend on the time only (T, n_H, H)
+template <class S>
 struct EmlaTime {
-  double t = NAN, T, nh, H, iT, boltz;
-  Eff e;
+  double t = NAN;
+  S T, nh, H, iT, boltz;
+  Eff<S> e;
 };
 
 // effective_coefficients (recomb_functions.py:180-258): the ones the rate
 // equation reads that depend on T alone (B2s, B2p, R2p2s, R2s2p), and the
 // fractional T index of the A2s / A2p table lookups
-void effective_T(double T, Eff& e, double& iT) {
+template <class S>
+void effective_T(S T, Eff<S>& e, S& iT) {
   const Tables& tb = tables();
-  const double lt = std::log(T);
+  const S lt = log(T);
   iT = axis_index(lt, std::log(TR_MIN), std::log(TR_MAX), NTR);
   const double E2 = -RYD / 4.0;
-  const double q0 = 2.0 * PI * MU_E * T / ((2.0 * PI * HBAR) * (2.0 * PI * HBAR));
-  const double q = q0 * std::sqrt(q0) / (C_CM * C_CM * C_CM);
+  const S q0 = 2.0 * PI * MU_E * T / ((2.0 * PI * HBAR) * (2.0 * PI * HBAR));
+  const S q = q0 * sqrt(q0) / (C_CM * C_CM * C_CM);
   const double last = NTM - 1.0;
-  const double boltz2 = std::exp(E2 / T) * q;
-  e.B2s = boltz2 * std::exp(bilinear(tb.lnA2s, iT, last));
-  e.B2p = (1.0 / 3.0) * boltz2 * std::exp(bilinear(tb.lnA2p, iT, last));
+  const S boltz2 = exp(E2 / T) * q;
+  e.B2s = boltz2 * exp(bilinear(tb.lnA2s, iT, last));
+  e.B2p = (1.0 / 3.0) * boltz2 * exp(bilinear(tb.lnA2p, iT, last));
   // jnp.interp(log T, TR_axis, log R_tab): TR_axis is uniform
   const double lo = std::log(TR_MIN), hi = std::log(TR_MAX);
-  double fr = (lt - lo) / (hi - lo) * (NTR - 1);
-  double lnr;
-  if (fr <= 0.0) lnr = tb.lnR[0];
-  else if (fr >= NTR - 1) lnr = tb.lnR[NTR - 1];
+  S fr = (lt - lo) / (hi - lo) * (NTR - 1);
+  S lnr;
+  if (dbl(fr) <= 0.0) lnr = S(tb.lnR[0]);
+  else if (dbl(fr) >= NTR - 1) lnr = S(tb.lnR[NTR - 1]);
   else {
-    const int k = (int)std::floor(fr);
-    const double w = fr - k;
+    const int k = (int)std::floor(dbl(fr));
+    const S w = fr - (double)k;
     lnr = (1.0 - w) * tb.lnR[k] + w * tb.lnR[k + 1];
   }
-  e.R2p2s = std::exp(lnr);
+  e.R2p2s = exp(lnr);
   e.R2s2p = 3.0 * e.R2p2s;
 }
 
 // A2s, A2p at (T, T_m): bilinear in (log T, T_m / T) on the log tables
-template <class W>
-void effective_A(double T, double iT, W Tm, W& A2s, W& A2p) {
+template <class S, class W>
+void effective_A(S T, S iT, W Tm, W& A2s, W& A2p) {
   const Tables& tb = tables();
   const W ir = axis_index(Tm / T, T_RATIO_MIN, T_RATIO_MAX, NTM);
   A2s = exp(bilinear(tb.lnA2s, iT, ir));
   A2p = exp(bilinear(tb.lnA2p, iT, ir));
 }
 
-void emla_time(const Inputs& in, double t, EmlaTime& tc) {
+template <class S>
+void emla_time(const Inputs<S>& in, double t, EmlaTime<S>& tc) {
   if (t == tc.t) return;
   tc.t = t;
   tc.T = in.at(in.T, t);
   tc.nh = in.at(in.nh, t);
   tc.H = in.at(in.H, t);
   effective_T(tc.T, tc.e, tc.iT);
-  tc.boltz = std::exp(-E21 / tc.T);
+  tc.boltz = exp(-E21 / tc.T);
 }
 
 // dxe/dlna of the EMLA with the two-photon correction Delta
-// (hydrogen.py:497-536, steady-state 2s/2p populations :577-636)
-template <class V, class W>
-V dxe_emla(V xe, double T, W Tm, double H, double nh, double Delta, const EmlaTime* tc = nullptr) {
-  Eff e;
-  double iT;
+// (hydrogen.py:497-536, steady-state 2s/2p populations :577-636); V: the
+// type of x_e, W: that of T_m (S, or Dl<S> for a local derivative)
+template <class V, class S, class W>
+V dxe_emla(V xe, S T, W Tm, S H, S nh, S Delta, const EmlaTime<S>* tc = nullptr) {
+  Eff<S> e;
+  S iT;
   W A2s, A2p;
   if (tc) {
     e = tc->e;
@@ -270,11 +267,11 @@ V dxe_emla(V xe, double T, W Tm, double H, double nh, double Delta, const EmlaTi
   const double lam = 2.0 * PI * HBAR * C_CM / RYD * 4.0 / 3.0;
   const V RLya = 8.0 * PI * H / 3.0 / nh / x1s / (lam * lam * lam);
   const V R2p1s = RLya / (1.0 + Delta);
-  const double boltz = tc ? tc->boltz : std::exp(-E21 / T);
-  const double R1s2s = boltz * R2S1S;
+  const S boltz = tc ? tc->boltz : exp(-E21 / T);
+  const S R1s2s = boltz * R2S1S;
   const V R1s2p = 3.0 * boltz * R2p1s;
   // [[a, b], [c, d]] (x2s, x2p) = (s0, s1)
-  const double a = e.B2s + e.R2s2p + R2S1S, b = -e.R2p2s, c = -e.R2s2p;
+  const S a = e.B2s + e.R2s2p + R2S1S, b = -e.R2p2s, c = -e.R2s2p;
   const V d = e.B2p + e.R2p2s + R2p1s;
   const V xe2n = xe * xe * nh;
   const V s0 = xe2n * A2s + x1s * R1s2s;
@@ -286,69 +283,75 @@ V dxe_emla(V xe, double T, W Tm, double H, double nh, double Delta, const EmlaTi
 }
 
 // get_current_correction_func (hydrogen.py:538-575)
-double swift_delta(double T, const Par& p) {
+template <class S>
+S swift_delta(S T, const Par<S>& p) {
   const Tables& tb = tables();
-  const double omH = p.omega_b * (1.0 - p.YHe), omcb = p.omega_b + p.omega_cdm;
+  const S omH = p.omega_b * (1.0 - p.YHe), omcb = p.omega_b + p.omega_cdm;
   const double* col = &HX_SWIFT[0][0];
-  const double D = interp1(T, tb.swT, col + 1, HX_SWIFT_N, 5);
-  const double Dcb = interp1(T, tb.swT, col + 2, HX_SWIFT_N, 5);
-  const double DH = interp1(T, tb.swT, col + 3, HX_SWIFT_N, 5);
-  const double DN = interp1(T, tb.swT, col + 4, HX_SWIFT_N, 5);
+  const S D = interp1(T, tb.swT, col + 1, HX_SWIFT_N, 5);
+  const S Dcb = interp1(T, tb.swT, col + 2, HX_SWIFT_N, 5);
+  const S DH = interp1(T, tb.swT, col + 3, HX_SWIFT_N, 5);
+  const S DN = interp1(T, tb.swT, col + 4, HX_SWIFT_N, 5);
   return D + (omcb - 0.14175) * Dcb + (omH - 0.01689) * DH + (p.neff - 3.046) * DN;
 }
 
 // ---------------------------------------------------------------- helium (helium.py)
-template <class V>
-V xHeII_post_saha(const Inputs& in, const Par& p, V lna) {
-  const V T = in.at(in.T, lna), nh = in.at(in.nh, lna);
-  const V Tk = T / KB;
-  const V s = 4.0 * 2.414194e15 * Tk * sqrt(Tk) * exp(-285325.0 / Tk) / nh;
-  const double f = p.fHe_num;
+template <class S, class V>
+Prom<S, V> xHeII_post_saha(const Inputs<S>& in, const Par<S>& p, V lna) {
+  using R = Prom<S, V>;
+  const R T = in.at(in.T, lna), nh = in.at(in.nh, lna);
+  const R Tk = T / KB;
+  const R s = 4.0 * 2.414194e15 * Tk * sqrt(Tk) * exp(-285325.0 / Tk) / nh;
+  const S f = p.fHe_num;
   return 2.0 * s * f / (1.0 + s) / (1.0 + sqrt(1.0 + 4.0 * s * f / ((1.0 + s) * (1.0 + s))));
 }
 
-template <class V>
-V xH1_saha(const Inputs& in, const Par& p, V lna) {
-  const V T = in.at(in.T, lna), nh = in.at(in.nh, lna);
-  const V xHeII = xHeII_post_saha(in, p, lna);
-  const V Tk = T / KB;
-  const V s = 2.4127161187130e15 * Tk * sqrt(Tk) * exp(-157801.37882 / Tk) / nh;
-  if (val(s) > 1e5) return (1.0 + xHeII) / s - (xHeII * xHeII + 3.0 * xHeII + 2.0) / (s * s);
-  if (val(s) == 0.0) return V(1.0);
-  const V u = 1.0 + xHeII / s;
+template <class S, class V>
+Prom<S, V> xH1_saha(const Inputs<S>& in, const Par<S>& p, V lna) {
+  using R = Prom<S, V>;
+  const R T = in.at(in.T, lna), nh = in.at(in.nh, lna);
+  const R xHeII = xHeII_post_saha(in, p, lna);
+  const R Tk = T / KB;
+  const R s = 2.4127161187130e15 * Tk * sqrt(Tk) * exp(-157801.37882 / Tk) / nh;
+  if (dbl(s) > 1e5) return (1.0 + xHeII) / s - (xHeII * xHeII + 3.0 * xHeII + 2.0) / (s * s);
+  if (dbl(s) == 0.0) return R(1.0);
+  const R u = 1.0 + xHeII / s;
   return 1.0 - 2.0 / (u + sqrt(u * u + 4.0 / s));
 }
 
 // the parts of the HeII rate that depend on the time only (helium.py:375-453)
+template <class S>
 struct HeTime {
-  double t = NAN, xH1, H, s, y2s, y2p, etacinv, g2pinc, tau_pre, intercom, leak;
+  double t = NAN;
+  S xH1, H, s, y2s, y2p, etacinv, g2pinc, tau_pre, intercom, leak;
 };
 
-void he_time(const Inputs& in, const Par& p, double lna, HeTime& h) {
+template <class S>
+void he_time(const Inputs<S>& in, const Par<S>& p, double lna, HeTime<S>& h) {
   if (lna == h.t) return;
   h.t = lna;
-  const double T = in.at(in.T, lna), nh = in.at(in.nh, lna);
+  const S T = in.at(in.T, lna), nh = in.at(in.nh, lna);
   h.H = in.at(in.H, lna);
   h.xH1 = xH1_saha(in, p, lna);
-  const double Tk = T / KB;
-  const double s0 = 2.414194e15 * Tk * std::sqrt(Tk) / nh * 4.0;
-  h.s = s0 * std::exp(-285325.0 / Tk);
-  h.y2s = std::exp(46090.0 / Tk) / s0;
-  h.y2p = std::exp(39101.0 / Tk) / s0 * 3.0;
+  const S Tk = T / KB;
+  const S s0 = 2.414194e15 * Tk * sqrt(Tk) / nh * 4.0;
+  h.s = s0 * exp(-285325.0 / Tk);
+  h.y2s = exp(46090.0 / Tk) / s0;
+  h.y2p = exp(39101.0 / Tk) / s0 * 3.0;
   h.etacinv = 9.15776e22 * h.H / (nh * h.xH1);
-  h.g2pinc = 1.976e6 / (1.0 - std::exp(-6989.0 / Tk)) + 6.03e6 / (std::exp(19754.0 / Tk) - 1.0) +
-             1.06e8 / (std::exp(21539.0 / Tk) - 1.0) + 2.18e6 / (std::exp(28496.0 / Tk) - 1.0) +
-             3.37e7 / (std::exp(29224.0 / Tk) - 1.0) + 1.04e6 / (std::exp(32414.0 / Tk) - 1.0) +
-             1.51e7 / (std::exp(32781.0 / Tk) - 1.0);
+  h.g2pinc = 1.976e6 / (1.0 - exp(-6989.0 / Tk)) + 6.03e6 / (exp(19754.0 / Tk) - 1.0) +
+             1.06e8 / (exp(21539.0 / Tk) - 1.0) + 2.18e6 / (exp(28496.0 / Tk) - 1.0) +
+             3.37e7 / (exp(29224.0 / Tk) - 1.0) + 1.04e6 / (exp(32414.0 / Tk) - 1.0) +
+             1.51e7 / (exp(32781.0 / Tk) - 1.0);
   h.tau_pre = 4.277e-8 * nh / h.H;
-  h.intercom = 0.964525 * std::exp(2947.0 / Tk);
-  h.leak = std::exp(-6.14e13 / h.etacinv);
+  h.intercom = 0.964525 * exp(2947.0 / Tk);
+  h.leak = exp(-6.14e13 / h.etacinv);
 }
 
 // d x_e / d ln a from HeII recombination at total x_e (helium.py:375-453)
-template <class V>
-V he_rate(const Par& p, const HeTime& h, V xe) {
-  const double f = p.fHe_num;
+template <class S, class V>
+V he_rate(const Par<S>& p, const HeTime<S>& h, V xe) {
+  const S f = p.fHe_num;
   const V xHeII = xe - (1.0 - h.xH1);
   const V tau2p = h.tau_pre * (f - xHeII);
   const V dnuline = h.g2pinc * tau2p / (4.0 * PI * PI);
@@ -365,45 +368,48 @@ V he_rate(const Par& p, const HeTime& h, V xe) {
 // t1 (the last stage sits at t1 itself, so the next step's start reuses the
 // rate's time-only parts); stage equations by chord Newton to rounding,
 // stopped when the contraction rate puts the remaining error below 1e-16 of
-// the state.  The iteration matrix I - h g J, J by forward differences, is
-// kept for REUSE steps (it only sets the contraction rate, not the converged
-// stages) and rebuilt when a stage does not converge.  The method is stiffly
-// accurate, so the last stage's derivative is f at the step end and starts
-// the next step's predictor.
+// the state.  The iteration matrix I - h g J, J by forward differences of the
+// values, is kept for REUSE steps (it only sets the contraction rate, not the
+// converged stages) and rebuilt when a stage does not converge.  The method
+// is stiffly accurate, so the last stage's derivative is f at the step end
+// and starts the next step's predictor.  Over dual numbers the iteration
+// carries the derivatives along: their fixed-point map contracts at the same
+// rate, so they converge with the values.
 const double SG = 0.43586652150845899;
 const double SB1 = -(6.0 * SG * SG - 16.0 * SG + 1.0) / 4.0;
 const double SB2 = (6.0 * SG * SG - 20.0 * SG + 5.0) / 4.0;
 const double SC[2] = {SG, (1.0 + SG) / 2.0};
 const double SA[3][3] = {{SG, 0.0, 0.0}, {(1.0 - SG) / 2.0, SG, 0.0}, {SB1, SB2, SG}};
 
-template <int D, class F>
+template <int D, class S, class F>
 class Sdirk3 {
  public:
   static constexpr int REUSE = 4;
   explicit Sdirk3(F& f) : f_(f) {}
 
-  bool step(double t0, double t1, double* y) {
+  bool step(double t0, double t1, S* y) {
     if (age_ < REUSE && attempt(t0, t1, y, false)) return true;
     return attempt(t0, t1, y, true);
   }
 
  private:
   F& f_;
-  double Mi_[D][D], fy_[D];
+  double Mi_[D][D];
+  S fy_[D];
   bool have_f_ = false;
   int age_ = REUSE;
 
-  void refresh(double t0, double h, const double* y) {
+  void refresh(double t0, double h, const S* y) {
     double J[D][D];
     f_(t0, y, fy_);
     have_f_ = true;
     for (int j = 0; j < D; ++j) {
-      double yp[D], fp[D];
-      std::memcpy(yp, y, sizeof(yp));
-      const double dy = 1e-7 * std::fabs(y[j]) + 1e-20;
-      yp[j] += dy;
+      S yp[D], fp[D];
+      for (int i = 0; i < D; ++i) yp[i] = S(dbl(y[i]));
+      const double dy = 1e-7 * std::fabs(dbl(y[j])) + 1e-20;
+      yp[j] = S(dbl(y[j]) + dy);
       f_(t0, yp, fp);
-      for (int i = 0; i < D; ++i) J[i][j] = (fp[i] - fy_[i]) / dy;
+      for (int i = 0; i < D; ++i) J[i][j] = (dbl(fp[i]) - dbl(fy_[i])) / dy;
     }
     double M[D][D];
     for (int i = 0; i < D; ++i)
@@ -418,16 +424,16 @@ class Sdirk3 {
     age_ = 0;
   }
 
-  bool attempt(double t0, double t1, double* y, bool fresh) {
+  bool attempt(double t0, double t1, S* y, bool fresh) {
     const double h = t1 - t0;
     if (fresh) refresh(t0, h, y);
     if (!have_f_) {
       f_(t0, y, fy_);
       have_f_ = true;
     }
-    double k[3][D], z[D];
+    S k[3][D], z[D];
     for (int s = 0; s < 3; ++s) {
-      double base[D];
+      S base[D];
       for (int i = 0; i < D; ++i) {
         base[i] = y[i];
         for (int j = 0; j < s; ++j) base[i] += h * SA[s][j] * k[j][i];
@@ -437,15 +443,15 @@ class Sdirk3 {
       bool conv = false;
       double prev = 0.0;
       for (int it = 0; it < 40; ++it) {
-        double fz[D], r[D], dz[D];
+        S fz[D], r[D];
         f_(ts, z, fz);
         for (int i = 0; i < D; ++i) r[i] = base[i] + h * SG * fz[i] - z[i];
         double big = 0.0;
         for (int i = 0; i < D; ++i) {
-          dz[i] = 0.0;
-          for (int j = 0; j < D; ++j) dz[i] += Mi_[i][j] * r[j];
-          z[i] += dz[i];
-          big = std::fmax(big, std::fabs(dz[i]) / (std::fabs(z[i]) + 1e-300));
+          S dz = Mi_[i][0] * r[0];
+          for (int j = 1; j < D; ++j) dz += Mi_[i][j] * r[j];
+          z[i] += dz;
+          big = std::fmax(big, std::fabs(dbl(dz)) / (std::fabs(dbl(z[i])) + 1e-300));
         }
         if (!(big == big)) break;
         if (big < 4e-16) { conv = true; break; }
@@ -469,30 +475,30 @@ class Sdirk3 {
   }
 };
 
-template <int D, class F>
-Sdirk3<D, F> stepper(F& f) {
-  return Sdirk3<D, F>(f);
+template <int D, class S, class F>
+Sdirk3<D, S, F> stepper(F& f) {
+  return Sdirk3<D, S, F>(f);
 }
 
 }  // namespace
 
 // ---------------------------------------------------------------- the history
-int recombination(const AbcmbCosmo& c, double neff, const double* Tgrid, const double* nhgrid,
-                  const double* Hgrid, int n_grid, double* xe, int xe_alloc, double* tm, int tm_alloc,
-                  int32_t* sw, History& out) {
+template <class S>
+int recombination(S YHe, S omega_b, S omega_cdm, S neff, const S* Tgrid, const S* nhgrid, const S* Hgrid,
+                  int n_grid, S* xe, int xe_alloc, S* tm, int tm_alloc, int32_t* sw, History& out) {
   const double x0 = -std::log(1.0 + Z0), dx = DLNA;
-  Inputs in;
+  Inputs<S> in;
   in.x0 = x0;
   in.n = n_grid;
   in.x1 = x0 + (n_grid - 1) * dx;
   in.T.assign(Tgrid, Tgrid + n_grid);
   in.nh.assign(nhgrid, nhgrid + n_grid);
   in.H.assign(Hgrid, Hgrid + n_grid);
-  Par p;
-  p.YHe = c.YHe;
-  p.fHe_num = c.YHe / (1.0 - c.YHe) / 3.97153;
-  p.omega_b = c.omega_b;
-  p.omega_cdm = c.omega_cdm;
+  Par<S> p;
+  p.YHe = YHe;
+  p.fHe_num = YHe / (1.0 - YHe) / 3.97153;
+  p.omega_b = omega_b;
+  p.omega_cdm = omega_cdm;
   p.neff = neff;
   if (xe_alloc < XE_ALLOC || tm_alloc < TM_ALLOC) return ABCMB_EINVAL;
   auto grid = [&](int j) { return x0 + j * dx; };
@@ -503,48 +509,48 @@ int recombination(const AbcmbCosmo& c, double neff, const double* Tgrid, const d
   const int n4 = (int)std::floor((-std::log(3601.0) - x0) / dx + 1e-9) + 1;
   for (int i = 0; i < n4; ++i) {
     const double lna = grid(i);
-    const double T = in.at(in.T, lna), nh = in.at(in.nh, lna);
-    const double Tk = T / KB;
-    const double s = 2.414194e15 * Tk * std::sqrt(Tk) * std::exp(-631462.7 / Tk) / nh;
-    const double f = p.fHe_num;
-    const double xHeIII = 2.0 * s * f / (1.0 + s + f) / (1.0 + std::sqrt(1.0 + 4.0 * s * f / ((1.0 + s + f) * (1.0 + s + f))));
+    const S T = in.at(in.T, lna), nh = in.at(in.nh, lna);
+    const S Tk = T / KB;
+    const S s = 2.414194e15 * Tk * sqrt(Tk) * exp(-631462.7 / Tk) / nh;
+    const S f = p.fHe_num;
+    const S xHeIII = 2.0 * s * f / (1.0 + s + f) / (1.0 + sqrt(1.0 + 4.0 * s * f / ((1.0 + s + f) * (1.0 + s + f))));
     xe[j++] = 1.0 + f + xHeIII;
-    const bool stop = frozen(0) ? (i + 1 >= sw[0]) : (xHeIII < 1e-9);
+    const bool stop = frozen(0) ? (i + 1 >= sw[0]) : (dbl(xHeIII) < 1e-9);
     if (stop) break;
   }
   sw[0] = j;
   // 2. post-Saha HeII (helium.py:297-373), at most 4000 points
   for (int iz = 0; iz < 4000; ++iz) {
     const double lna = grid(j);
-    const double xHeII = xHeII_post_saha(in, p, lna);
-    const double xH1 = xH1_saha(in, p, lna);
-    const double xs = 1.0 - xH1 + xHeII;
-    HeTime ht;
+    const S xHeII = xHeII_post_saha(in, p, lna);
+    const S xH1 = xH1_saha(in, p, lna);
+    const S xs = 1.0 - xH1 + xHeII;
+    HeTime<S> ht;
     he_time(in, p, lna, ht);
-    const double g = he_rate(p, ht, Dl(xs, 1.0)).d;
-    const double dsaha = xHeII_post_saha(in, p, Dl(lna, 1.0)).d;
-    const double x = xs + dsaha / g;
+    const S g = he_rate(p, ht, Dl<S>(xs, S(1.0))).d;
+    const S dsaha = xHeII_post_saha(in, p, Dl<S>(S(lna), S(1.0))).d;
+    const S x = xs + dsaha / g;
     xe[j++] = x;
-    const bool stop = frozen(1) ? (j >= sw[1]) : (std::fabs(xs - x) > 1e-5);
+    const bool stop = frozen(1) ? (j >= sw[1]) : (std::fabs(dbl(xs) - dbl(x)) > 1e-5);
     if (stop) break;
   }
   sw[1] = j;
   // 3. HeII rate equation until x_HeII < 1e-4 (helium.py:455-556), at most 4096 points
   {
-    double y[1] = {xe[j - 1]};
+    S y[1] = {xe[j - 1]};
     // the state is advanced with x_e = state + x_H1(Saha) handed to the rate, as
     // xe_derivative_HeII does (helium.py:458-482)
-    HeTime ht;
-    auto f = [&](double t, const double* yy, double* dy) {
+    HeTime<S> ht;
+    auto f = [&](double t, const S* yy, S* dy) {
       he_time(in, p, t, ht);
       dy[0] = he_rate(p, ht, yy[0] + ht.xH1);
     };
-    auto st = stepper<1>(f);
+    auto st = stepper<1, S>(f);
     for (int k = 0; k < 4096; ++k) {
       if (!st.step(grid(j - 1), grid(j), y)) return ABCMB_EINVAL;
       xe[j++] = y[0];
       const double lna = grid(j - 1);
-      const bool stop = frozen(2) ? (j >= sw[2]) : (y[0] - (1.0 - xH1_saha(in, p, lna)) < 1e-4);
+      const bool stop = frozen(2) ? (j >= sw[2]) : (dbl(y[0]) - (1.0 - dbl(xH1_saha(in, p, lna))) < 1e-4);
       if (stop) break;
     }
   }
@@ -552,32 +558,33 @@ int recombination(const AbcmbCosmo& c, double neff, const double* Tgrid, const d
   // 4. hydrogen post-Saha expansion (hydrogen.py:190-276), at most 800 points
   for (int iz = 0; iz < 800; ++iz) {
     const double lna = grid(j);
-    const double T = in.at(in.T, lna), nh = in.at(in.nh, lna), H = in.at(in.H, lna);
-    const double ge = std::pow(2.0 * PI * MU_E * T, 1.5) / std::pow(2.0 * PI * HBAR * C_CM, 3.0) / nh;
-    const double s = ge * std::exp(-RYD / T);
-    const double xs = (std::sqrt(s * s + 4.0 * s) - s) / 2.0;
-    const double dsaha = -(RYD / T - 1.5) * xs * xs / (2.0 * xs + s);
-    const double g = dxe_emla(Dl(xs, 1.0), T, T, H, nh, 0.0).d;
-    const double x = xs + dsaha / g;
+    const S T = in.at(in.T, lna), nh = in.at(in.nh, lna), H = in.at(in.H, lna);
+    const S ge = pow(2.0 * PI * MU_E * T, 1.5) / std::pow(2.0 * PI * HBAR * C_CM, 3.0) / nh;
+    const S s = ge * exp(-RYD / T);
+    const S xs = (sqrt(s * s + 4.0 * s) - s) / 2.0;
+    const S dsaha = -(RYD / T - 1.5) * xs * xs / (2.0 * xs + s);
+    const S g = dxe_emla(Dl<S>(xs, S(1.0)), T, T, H, nh, S(0.0)).d;
+    const S x = xs + dsaha / g;
     xe[j++] = x;
-    const bool stop = frozen(3) ? (j >= sw[3]) : (std::fabs(xs - x) > 1e-5);
+    const bool stop = frozen(3) ? (j >= sw[3]) : (std::fabs(dbl(xs) - dbl(x)) > 1e-5);
     if (stop) break;
   }
   sw[3] = j;
   // 5. EMLA with the two-photon correction, T_m in equilibrium, to z = 701
   //    (hydrogen.py:278-383), at most 2048 points
   {
-    double y[1] = {xe[j - 1]};
-    EmlaTime tc;
-    double dl = 0.0, dl_t = NAN;
-    auto f = [&](double t, const double* yy, double* dy) {
+    S y[1] = {xe[j - 1]};
+    EmlaTime<S> tc;
+    S dl = S(0.0);
+    double dl_t = NAN;
+    auto f = [&](double t, const S* yy, S* dy) {
       emla_time(in, t, tc);
       if (t != dl_t) { dl = swift_delta(tc.T, p); dl_t = t; }
-      const double Tm = tc.T * (1.0 - tc.H / gamma_compton(yy[0], tc.T, p.YHe));
+      const S Tm = tc.T * (1.0 - tc.H / gamma_compton(yy[0], tc.T, p.YHe));
       dy[0] = dxe_emla(yy[0], tc.T, Tm, tc.H, tc.nh, dl, &tc);
     };
     const double t_end = -std::log(701.0);
-    auto st = stepper<1>(f);
+    auto st = stepper<1, S>(f);
     for (int k = 0; k < 2048; ++k) {
       // two steps per grid interval: hydrogen recombines here, x_e falls fastest
       const double tm = grid(j - 1) + 0.5 * dx;
@@ -593,26 +600,26 @@ int recombination(const AbcmbCosmo& c, double neff, const double* Tgrid, const d
   //    8192 points
   const int jt0 = j;  // first grid point of the T_m table
   int jt = 0;
-  double y2[2];
+  S y2[2];
   {
     const double t0 = grid(j - 1);
-    const double T0 = in.at(in.T, t0), H0 = in.at(in.H, t0);
+    const S T0 = in.at(in.T, t0), H0 = in.at(in.H, t0);
     y2[0] = xe[j - 1];
     y2[1] = T0 * (1.0 - H0 / gamma_compton(y2[0], T0, p.YHe));
-    EmlaTime tc;
-    auto f = [&](double t, const double* yy, double* dy) {
+    EmlaTime<S> tc;
+    auto f = [&](double t, const S* yy, S* dy) {
       emla_time(in, t, tc);
-      dy[0] = dxe_emla(yy[0], tc.T, yy[1], tc.H, tc.nh, 0.0, &tc);
+      dy[0] = dxe_emla(yy[0], tc.T, yy[1], tc.H, tc.nh, S(0.0), &tc);
       dy[1] = (-2.0 * tc.H * yy[1] + gamma_compton(yy[0], tc.T, p.YHe) * (tc.T - yy[1])) / tc.H;
     };
-    auto st = stepper<2>(f);
+    auto st = stepper<2, S>(f);
     for (int k = 0; k < 8192; ++k) {
       if (!st.step(grid(j - 1), grid(j), y2)) return ABCMB_EINVAL;
       xe[j++] = y2[0];
       tm[jt++] = y2[1];
       const double lna = grid(j - 1);
-      const double T = in.at(in.T, lna);
-      const double ratio = std::fmin(y2[1] / T, T / y2[1]);
+      const double T = dbl(in.at(in.T, lna));
+      const double ratio = std::fmin(dbl(y2[1]) / T, T / dbl(y2[1]));
       const bool stop = frozen(5) ? (j >= sw[5]) : (T < TR_MIN || ratio < T_RATIO_MIN);
       if (stop) break;
     }
@@ -623,10 +630,11 @@ int recombination(const AbcmbCosmo& c, double neff, const double* Tgrid, const d
     // time-only parts cached per time: T, n_H, H, the Lyman-alpha rate and the
     // photo-ionisation rate from n = 2 at the radiation temperature
     struct {
-      double t = NAN, T, nh, H, rate, bT;
+      double t = NAN;
+      S T, nh, H, rate, bT;
     } tt;
     const double lya3 = std::pow(C_CM / LYA_FREQ, 3.0);
-    auto f = [&](double t, const double* yy, double* dy) {
+    auto f = [&](double t, const S* yy, S* dy) {
       if (t != tt.t) {
         tt.t = t;
         tt.T = in.at(in.T, t);
@@ -635,16 +643,16 @@ int recombination(const AbcmbCosmo& c, double neff, const double* Tgrid, const d
         tt.rate = 8.0 * PI * tt.H / (3.0 * (tt.nh * lya3));
         tt.bT = beta_H(in.at(in.T, std::log(1.0 / (1.0 + (std::exp(-t) - 1.0)))));
       }
-      const double T = tt.T, nh = tt.nh, H = tt.H;
-      const double xe_ = yy[0], Tm = yy[1];
-      const double exc = 3.0 * tt.rate / 4.0 + (1.0 - xe_) * R2S1S / 4.0;
-      const double ion = (1.0 - xe_) * tt.bT;
-      const double C = exc / (exc + ion);
-      const double aTm = alpha_H(Tm);
+      const S T = tt.T, nh = tt.nh, H = tt.H;
+      const S xe_ = yy[0], Tm = yy[1];
+      const S exc = 3.0 * tt.rate / 4.0 + (1.0 - xe_) * R2S1S / 4.0;
+      const S ion = (1.0 - xe_) * tt.bT;
+      const S C = exc / (exc + ion);
+      const S aTm = alpha_H(Tm);
       dy[0] = C * (beta_H_from(Tm, aTm) * (1.0 - xe_) - aTm * nh * xe_ * xe_) / H;
       dy[1] = -2.0 * Tm + (gamma_compton(xe_, T, p.YHe) / H) * (T - Tm);
     };
-    auto st = stepper<2>(f);
+    auto st = stepper<2, S>(f);
     for (int k = 0; k < 8192; ++k) {
       if (!st.step(grid(j - 1), grid(j), y2)) return ABCMB_EINVAL;
       xe[j++] = y2[0];
@@ -662,5 +670,10 @@ int recombination(const AbcmbCosmo& c, double neff, const double* Tgrid, const d
   out.tm_x0 = grid(jt0);
   return ABCMB_OK;
 }
+
+template int recombination<double>(double, double, double, double, const double*, const double*, const double*,
+                                   int, double*, int, double*, int, int32_t*, History&);
+template int recombination<Tan>(Tan, Tan, Tan, Tan, const Tan*, const Tan*, const Tan*, int, Tan*, int, Tan*, int,
+                                int32_t*, History&);
 
 }  // namespace hyrex

@@ -3,8 +3,12 @@
 #include <cstdint>
 
 #include "../../include/abcmb.h"
+#include "host_dual.h"
 
 namespace hyrex {
+
+// values with their derivatives along the tangent directions
+using Tan = abcmb_ad::Dual<ABCMB_TANGENT_DIRS>;
 
 constexpr double Z0 = 8000.0;    // the grid starts at ln a = -ln(1 + Z0)   (hyrex.py:115-139)
 constexpr double DLNA = 5.0e-4;  // and steps by DLNA
@@ -23,12 +27,15 @@ struct History {
   double tm_x0;   // ln a of the first T_m point
 };
 
-// The history of cosmology c on the grid; T, n_H and H sampled on the N_GRID
-// grid points (RecombInputs).  xe [xe_alloc], tm [tm_alloc] are filled, their
-// tails padded with the last values.  sw [N_STAGES]: per stage, the grid
-// count after it (< 0 on input: the stage's own stopping rule decides; >= 0:
-// forced, for finite differences with the grids held).
-int recombination(const AbcmbCosmo& c, double neff, const double* T, const double* nh, const double* H, int n_grid,
-                  double* xe, int xe_alloc, double* tm, int tm_alloc, int32_t* sw, History& out);
+// The history on the grid for helium mass fraction YHe, omega_b, omega_cdm and
+// N_eff, from T, n_H and H sampled on the N_GRID grid points (RecombInputs).
+// xe [xe_alloc], tm [tm_alloc] are filled, their tails padded with the last
+// values.  sw [N_STAGES]: per stage, the grid count after it (< 0 on input:
+// the stage's own stopping rule decides; >= 0: forced, for derivatives with
+// the grids held).  S: double, or Tan for values with their derivatives
+// (then every stage end must be forced).
+template <class S>
+int recombination(S YHe, S omega_b, S omega_cdm, S neff, const S* T, const S* nh, const S* H, int n_grid,
+                  S* xe, int xe_alloc, S* tm, int tm_alloc, int32_t* sw, History& out);
 
 }  // namespace hyrex

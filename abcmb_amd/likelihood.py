@@ -158,11 +158,24 @@ def _seed_inputs(model, thetas, base_plans, k_grids, steps):
         pm_params += q
         freeze += [base_plans[c]] * len(q)
         ks += [k_grids[c]] * len(q)
-    pm = model.prepare_many(pm_params, freeze_list=freeze, k_list=ks)
     nd = len(ODE_DIRS)
-    pm = [pm[c * 2 * nd:(c + 1) * 2 * nd] for c in range(nc)]
+    if _forward_host(model):
+        from concurrent.futures import ThreadPoolExecutor
+        import os
+        with ThreadPoolExecutor(min(16, os.cpu_count() or 1)) as ex:
+            pm = list(ex.map(lambda c: model.tangent_plans(base_plans[c], pm_params[c * 2 * nd:(c + 1) * 2 * nd],
+                                                           [steps[c, p] for p in ODE_DIRS]), range(nc)))
+    else:
+        pm = model.prepare_many(pm_params, freeze_list=freeze, k_list=ks)
+        pm = [pm[c * 2 * nd:(c + 1) * 2 * nd] for c in range(nc)]
     seeds = [tangent_seeds(pm[c][0::2], pm[c][1::2], [steps[c, p] for p in ODE_DIRS]) for c in range(nc)]
     return pm, seeds
+
+
+def _forward_host(model) -> bool:
+    """Tangent seeds from the producer's forward mode (else from frozen-grid
+    central differences: user fluids, or host_tangents="fd")."""
+    return getattr(model, "host_tangents", "fd") == "forward" and getattr(model, "fluids", None) is None
 
 
 def _tangent_inputs(model, thetas, k_grids, steps, threads: int = None):
@@ -181,13 +194,17 @@ def _tangent_inputs(model, thetas, k_grids, steps, threads: int = None):
         fb = {ex.submit(model.prepare, model.derived(theta_to_params(thetas[c])), None, k_grids[c]): c
               for c in range(nc)}
         fz = {}
+        forward = _forward_host(model)
         for f in as_completed(fb):
             c = fb[f]
             base_plans[c] = f.result()
-            fz[c] = [ex.submit(model.prepare, q, base_plans[c], k_grids[c])
-                     for q in _pm_params(model, thetas[c], steps[c])]
+            q = _pm_params(model, thetas[c], steps[c])
+            if forward:
+                fz[c] = ex.submit(model.tangent_plans, base_plans[c], q, [steps[c, p] for p in ODE_DIRS])
+            else:
+                fz[c] = [ex.submit(model.prepare, qq, base_plans[c], k_grids[c]) for qq in q]
         for c in range(nc):
-            pm[c] = [f.result() for f in fz[c]]
+            pm[c] = fz[c].result() if forward else [f.result() for f in fz[c]]
     seeds = [tangent_seeds(pm[c][0::2], pm[c][1::2], [steps[c, p] for p in ODE_DIRS]) for c in range(nc)]
     if getattr(model, "device_background", False):
         # recombination-only tables: the GPU adds the reionization, its

@@ -16,6 +16,7 @@ before any kernel runs.
 from __future__ import annotations
 
 import ctypes as C
+import dataclasses
 from dataclasses import dataclass, field
 
 import numpy as np
@@ -58,7 +59,7 @@ class CosmoPlan:
 class TangentSeeds:
     """Derivatives of one cosmology's host inputs along the tangent directions.
 
-    par: [N_TANGENT_DIRS][31] derivatives of the AbcmbCosmo double fields;
+    par: [N_TANGENT_DIRS][COSMO_DOUBLES] derivatives of the AbcmbCosmo double fields;
     tau, xe, tm: [N_TANGENT_DIRS][len] derivatives of the tables.  Directions
     beyond the ones a caller asks for are zero.
     """
@@ -68,8 +69,11 @@ class TangentSeeds:
     tm: np.ndarray
 
 
+_TAU0_FIELD = [f for f, _ in abi.AbcmbCosmo._fields_].index("tau0")   # its index among the double fields
+
+
 def cosmo_doubles(c) -> np.ndarray:
-    """The 31 double fields of an AbcmbCosmo, in declaration order."""
+    """The COSMO_DOUBLES double fields of an AbcmbCosmo, in declaration order."""
     return np.frombuffer(bytes(c), dtype=np.float64, count=abi.COSMO_DOUBLES).copy()
 
 
@@ -239,6 +243,11 @@ class Model:
         # reionization root (abcmb_reionize) and the LOS background vectors with the
         # optical depth (abcmb_los_background) run on the GPU before the ODE launch
         self.device_background = bool(device_background)
+        # host_tangents: how the tangent seeds of the host inputs are made --
+        # "forward": the producer's forward-mode derivatives (tangent_plans);
+        # "fd": central differences of producer runs with the grids held (the
+        # only way for user fluids, whose generated host code has no tangents)
+        self.host_tangents = "forward"
         self.chunk = 128   # cosmologies per downstream launch (bounds the source buffers)
         self.timing = None  # list: per-launch pert_fwd event timing and work counts when set
         self.call_timing = None  # dict: entry-point name -> [(start event, end event, work)] when set
@@ -367,6 +376,51 @@ class Model:
             plan.lens_tau = ev["tau"]
             plan.chi_rec = bg.tau0 - float(bg.eval(np.array([bg.lna_rec]))["tau"][0])
         return plan
+
+    def tangent_plans(self, base: CosmoPlan, pm_params, steps) -> list:
+        """Plans at theta +- step along each direction, from the host producer's
+        forward-mode derivatives at ``base`` (every grid choice held): every
+        table, double field, LOS vector and lensing background of plan (d, +-)
+        is base's +- step[d] times its exact derivative, and its parameters are
+        ``pm_params`` (the derived parameters at theta +- step, in the order
+        (d, +), (d, -)).  Differences of these plans are the producer's
+        derivatives to rounding; the derived parameters themselves (closed forms
+        and the BBN helium table, ``params.add_derived_parameters``) enter by
+        their central differences.  Same order and meaning as ``prepare`` with
+        ``freeze=base`` at each of ``pm_params``."""
+        from .background import input_doubles
+        nd = len(steps)
+        N = abi.N_TANGENT_DIRS
+        if len(pm_params) != 2 * nd or nd > N:
+            raise ValueError(f"need a plus and a minus parameter set for each of at most {N} directions")
+        kp = self.specs["k_pivot"]
+        dpar, dtr, dne = np.zeros((N, abi.COSMO_DOUBLES)), np.zeros(N), np.zeros(N)
+        for d in range(nd):
+            pp, pm = pm_params[2 * d], pm_params[2 * d + 1]
+            s = 1.0 / (2.0 * steps[d])
+            dpar[d] = (input_doubles(pp, kp) - input_doubles(pm, kp)) * s
+            dtr[d] = (pp["tau_reion"] - pm["tau_reion"]) * s
+            dne[d] = (pp["Neff"] - pm["Neff"]) * s
+        bg = base.bg
+        t = bg.tangents(dpar, dtr, dne, reionization=not self.device_background)
+        dbgv = None if base.bgv is None else bg.los_vectors_tangent(t, base.lna_save[:-1])[1]
+        lensing = bool(self.specs["lensing"])
+        if lensing:
+            dev = bg.eval_tangent(t, np.append(base.lens_lna, bg.lna_rec))[1]
+        tau0 = _TAU0_FIELD
+        out = []
+        for d in range(nd):
+            for si, sgn in enumerate((1.0, -1.0)):
+                h = sgn * steps[d]
+                params = pm_params[2 * d + si]
+                plan = dataclasses.replace(base, params=params, bg=bg.shifted(params, h, t, d),
+                                           bgv=None if dbgv is None else base.bgv + h * dbgv[d])
+                if lensing:
+                    plan.lens_aH = base.lens_aH + h * dev[d, 0, :-1]
+                    plan.lens_tau = base.lens_tau + h * dev[d, 1, :-1]
+                    plan.chi_rec = base.chi_rec + h * (t.par[d, tau0] - dev[d, 1, -1])
+                out.append(plan)
+        return out
 
     def prepare_many(self, params_list, freeze_list=None, k_list=None, threads: int = None):
         """``prepare`` for many cosmologies on a host thread pool (the producer releases the GIL)."""

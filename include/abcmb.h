@@ -203,6 +203,40 @@ int abcmb_host_lna_start(const AbcmbCosmo* c, const double* k, int nk,
                          double R_tc, double R_large, double* out);
 /* Background at arbitrary ln a: out[0..6][n] = aH, tau, tau_c, cs2, x_e, T_m, R. */
 int abcmb_host_eval(const AbcmbCosmo* c, const double* lna, int n, double* out);
+/*
+ * Forward-mode derivatives of the producer along ABCMB_TANGENT_DIRS
+ * parameter directions, with every grid choice held.  Replaces the
+ * derivatives of the reference's Background construction that jax.jacfwd /
+ * jax.grad take through abcmb/background.py:420-768 and abcmb/hyrex/
+ * (Model(adjoint=...), abcmb/main.py:111): the same arithmetic is carried on
+ * dual numbers (the value parts equal abcmb_host_background's results bit for
+ * bit).  c: a cosmology filled by abcmb_host_background (or
+ * abcmb_host_recombination when reionization == 0) with lna_rec and
+ * lna_transfer_start negative (held); its tables are rewritten with the same
+ * values.  rec_switch: that call's ABCMB_REC_STAGES stage ends (all >= 0).
+ * dpar [ABCMB_TANGENT_DIRS][ABCMB_COSMO_DOUBLES], in/out: on input the
+ * derivatives of the parameter fields (h .. omega_m), on output also those of
+ * tau0, xe_last, tm_last and z_reion (zero for the held grid metadata).
+ * dtau_reion, dn_eff [ABCMB_TANGENT_DIRS]: derivatives of the tau_reion and
+ * n_eff arguments.  dtau [dirs][ABCMB_TAU_TAB_N], dxe [dirs][xe_n],
+ * dtm [dirs][tm_n]: the tables' derivatives.  z_reion's derivative is the
+ * implicit-function derivative of the tau_reion root.  Returns ABCMB_EINVAL
+ * for a build with user fluids (their generated host code has no tangents).
+ */
+int abcmb_host_background_tangent(AbcmbCosmo* c, double tau_reion, const double* dtau_reion,
+                                  double z_reion_in, double Delta_z_reion, double z_reion_He,
+                                  double Delta_z_reion_He, double exp_reion, double n_eff,
+                                  const double* dn_eff, const int32_t* rec_switch, int reionization,
+                                  double* dpar, double* dtau, double* dxe, double* dtm);
+/* abcmb_host_los_background (out [6][n]) and its derivatives dout
+ * [ABCMB_TANGENT_DIRS][6][n], given the derivatives dpar, dtau, dxe, dtm of
+ * the cosmology's fields and tables (from abcmb_host_background_tangent). */
+int abcmb_host_los_background_tangent(const AbcmbCosmo* c, const double* dpar, const double* dtau,
+                                      const double* dxe, const double* dtm, const double* lna, int n,
+                                      double* out, double* dout);
+/* abcmb_host_eval (out [7][n]) and its derivatives dout [ABCMB_TANGENT_DIRS][7][n]. */
+int abcmb_host_eval_tangent(const AbcmbCosmo* c, const double* dpar, const double* dtau, const double* dxe,
+                            const double* dtm, const double* lna, int n, double* out, double* dout);
 /* Baryon drag redshift z_d (baryon optical depth kappa_d = 1) and the sound
  * horizon r_s(z_d) in Mpc (Background.z_d / Background.rs_d,
  * abcmb/background.py:774-936), for a cosmology filled by

@@ -43,14 +43,14 @@ def _say(msg):
 
 
 def _pm(m, th, base, k=None, rel=LK.FD_REL_STEP):
+    """Plans at theta +- step along the ODE directions, made as the gradient
+    pipeline makes them (the host producer's forward-mode derivatives, or
+    frozen-grid producer runs for a model with user fluids)."""
     steps = rel * np.abs(th)
-    plus, minus = [], []
-    for p in LK.ODE_DIRS:
-        for sgn, lst in ((1, plus), (-1, minus)):
-            t = th.copy()
-            t[p] += sgn * steps[p]
-            lst.append(m.prepare(m.derived(LK.theta_to_params(t)), freeze=base, k=k))
-    return plus, minus, [steps[p] for p in LK.ODE_DIRS]
+    q = LK._pm_params(m, th, steps)
+    sd = [steps[p] for p in LK.ODE_DIRS]
+    pm = m.tangent_plans(base, q, sd) if LK._forward_host(m) else [m.prepare(x, freeze=base, k=k) for x in q]
+    return pm[0::2], pm[1::2], sd
 
 
 def _norm_err(g, ref, theta):
