@@ -389,6 +389,9 @@ class Model:
         their central differences.  Same order and meaning as ``prepare`` with
         ``freeze=base`` at each of ``pm_params``."""
         from .background import input_doubles
+        if self.fluids is not None:
+            raise ValueError("tangent_plans: the host code generated for user fluids has no forward mode; "
+                             "use prepare(..., freeze=base) at each of pm_params")
         nd = len(steps)
         N = abi.N_TANGENT_DIRS
         if len(pm_params) != 2 * nd or nd > N:
@@ -399,7 +402,7 @@ class Model:
             pp, pm = pm_params[2 * d], pm_params[2 * d + 1]
             s = 1.0 / (2.0 * steps[d])
             dpar[d] = (input_doubles(pp, kp) - input_doubles(pm, kp)) * s
-            dtr[d] = (pp["tau_reion"] - pm["tau_reion"]) * s
+            dtr[d] = (pp.get("tau_reion", 0.0) - pm.get("tau_reion", 0.0)) * s
             dne[d] = (pp["Neff"] - pm["Neff"]) * s
         bg = base.bg
         t = bg.tangents(dpar, dtr, dne, reionization=not self.device_background)
