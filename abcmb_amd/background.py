@@ -33,11 +33,13 @@ def _doubles(c) -> np.ndarray:
     return np.frombuffer((C.c_double * abi.COSMO_DOUBLES).from_buffer(c), dtype=np.float64)
 
 
-def input_doubles(params: dict, k_pivot: float) -> np.ndarray:
-    """The COSMO_DOUBLES fields an AbcmbCosmo takes from ``params`` (the
-    producer's results left zero)."""
+def input_doubles(params: dict, k_pivot: float, fluid_par=None) -> np.ndarray:
+    """The COSMO_DOUBLES fields an AbcmbCosmo takes from ``params`` and the
+    user fluids' parameter slots (the producer's results left zero)."""
     c = abi.AbcmbCosmo()
     _set_inputs(c, params, k_pivot)
+    if fluid_par is not None:
+        c.fluid_par[:] = [float(v) for v in fluid_par]
     return _doubles(c).copy()
 
 

@@ -86,6 +86,7 @@ struct CosmoT {
   S tm_last;
   double lna_rec, lna_transfer_start;
   S z_reion;
+  S fluid_par[ABCMB_FLUID_PARAMS];
   int32_t has_ncdm, n_q, xe_n, tm_n, species_off;
   const S* tau_tab;
   const S* xe_tab;
@@ -110,9 +111,7 @@ inline Rho<Sc<C>> rho_all(const C& c, double lna) {
   r.de = (c.species_off & ABCMB_NO_LAMBDA) ? S(0.0) : c.omega_Lambda * RHO_UNIT;
   r.cdm = (c.species_off & ABCMB_NO_CDM) ? S(0.0) : c.omega_cdm * RHO_UNIT / a3;
   r.fl = r.p_fl = S(0.0);
-  if constexpr (std::is_same_v<C, AbcmbCosmo>) {
-    if (fluid_host::kActive) fluid_host::bg(c, lna, a, r.fl, r.p_fl);
-  }
+  if (fluid_host::kActive) fluid_host::bg(c, lna, a, r.fl, r.p_fl);
   r.b = c.omega_b * RHO_UNIT / a3;
   S t04 = c.TCMB0 * c.TCMB0 * c.TCMB0 * c.TCMB0;
   r.g = PI * PI / 15.0 * t04 / a4 / HC3;
@@ -544,7 +543,7 @@ void eval_all(const C& c, const double* lna, int n, Sc<C>* out) {
 }
 
 // ---- the tangent producer's cosmology: AbcmbCosmo's doubles (h .. z_reion,
-// in declaration order) with their derivatives dpar[d][f]
+// then fluid_par, in declaration order) with their derivatives dpar[d][f]
 #define ABCMB_COSMO_SCALARS(X) \
   X(h) X(omega_b) X(omega_cdm) X(omega_Lambda) X(A_s) X(n_s) X(k_pivot) X(TCMB0) X(YHe) X(N_ur) X(T_ur) \
   X(N_ncdm) X(T_ncdm) X(m_ncdm) X(R_nu) X(R_b) X(om) X(omega_r) X(omega_m) X(tau0) X(xe_x0) X(xe_dx)    \
@@ -572,6 +571,7 @@ CosmoT<Tan> with_tangents(const AbcmbCosmo& c, const double* dpar) {
 #define X(name) set_field(t.name, dpar, f++, c.name);
   ABCMB_COSMO_SCALARS(X)
 #undef X
+  for (int i = 0; i < ABCMB_FLUID_PARAMS; ++i) set_field(t.fluid_par[i], dpar, f++, c.fluid_par[i]);
   t.has_ncdm = c.has_ncdm;
   t.n_q = c.n_q;
   t.xe_n = c.xe_n;
@@ -586,6 +586,7 @@ void store_tangents(const CosmoT<Tan>& t, AbcmbCosmo& c, double* dpar) {
 #define X(name) get_field(t.name, &c.name, dpar, f++);
   ABCMB_COSMO_SCALARS(X)
 #undef X
+  for (int i = 0; i < ABCMB_FLUID_PARAMS; ++i) get_field(t.fluid_par[i], &c.fluid_par[i], dpar, f++);
 }
 
 // the cosmology of c with tables tau, xe, tm (values and their derivatives)
@@ -660,7 +661,6 @@ extern "C" int abcmb_host_background_tangent(AbcmbCosmo* c, double tau_reion, co
   if (!c || !c->tau_tab || !c->xe_tab || !c->tm_tab || !rec_switch || !dpar || !dtau || !dxe || !dtm ||
       !dtau_reion || !dn_eff)
     return ABCMB_EINVAL;
-  if (fluid_host::kActive) return ABCMB_EINVAL;  // user fluids: their host code has no tangents
   if (!(c->lna_rec < 0.0 && c->lna_transfer_start < 0.0)) return ABCMB_EINVAL;
   int32_t sw[hyrex::N_STAGES];
   for (int s = 0; s < hyrex::N_STAGES; ++s) {

@@ -117,6 +117,7 @@ template <int N> inline Dual<N> sqrt(const Dual<N>& a) {
 template <int N> inline Dual<N> pow(const Dual<N>& a, double p) {
   return chain(a, std::pow(a.v, p), p * std::pow(a.v, p - 1.0));
 }
+template <int N> inline Dual<N> cos(const Dual<N>& a) { return chain(a, std::cos(a.v), -std::sin(a.v)); }
 template <int N> inline Dual<N> tanh(const Dual<N>& a) {
   const double t = std::tanh(a.v);
   return chain(a, t, 1.0 - t * t);

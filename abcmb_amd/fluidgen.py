@@ -618,7 +618,10 @@ class FluidSet:
         h = []
         h.append("constexpr bool kActive = true;")
         h.append("constexpr const char* kSignature = \"@SIG@\";")
-        h.append("inline void bg(const AbcmbCosmo& c, double lna, double a, double& rho, double& pres) {")
+        h.append("// C: AbcmbCosmo, or the producer's cosmology over dual numbers; S: its scalar")
+        h.append("template <class C, class S>")
+        h.append("inline void bg(const C& c, double lna, double a, S& rho, S& pres) {")
+        h.append("  using std::cos; using std::exp; using std::log; using std::pow; using std::sqrt; using std::tanh;")
         h += host.block([("rho", self.rho), ("pres", self.pres)], assign=True)
         h.append("}")
         names = ", ".join(s.name for s in self.user)
@@ -702,7 +705,9 @@ class _Emitter:
     """C++ for a set of expressions: one ``const auto`` temporary per shared node."""
 
     FN_DEV = {"exp": "fluid::fexp", "log": "fluid::flog", "sqrt": "fluid::fsqrt", "tanh": None, "cos": None}
-    FN_HOST = {"exp": "std::exp", "log": "std::log", "sqrt": "std::sqrt", "tanh": "std::tanh", "cos": "std::cos"}
+    # host code is a template over its scalar (double, or dual numbers with the
+    # parameter derivatives): unqualified calls, std:: brought in by using-declarations
+    FN_HOST = {"exp": "exp", "log": "log", "sqrt": "sqrt", "tanh": "tanh", "cos": "cos"}
 
     def __init__(self, style):
         self.style = style
@@ -780,8 +785,8 @@ class _Emitter:
             if p == -1.0:
                 return f"(1.0 / {x})"
             if p == 0.5:
-                return f"{'fluid::fsqrt' if self.style == 'dev' else 'std::sqrt'}({x})"
-            return f"{'fluid::fpow' if self.style == 'dev' else 'std::pow'}({x}, {self._num(p)})"
+                return f"{'fluid::fsqrt' if self.style == 'dev' else 'sqrt'}({x})"
+            return f"{'fluid::fpow' if self.style == 'dev' else 'pow'}({x}, {self._num(p)})"
         fns = self.FN_DEV if self.style == "dev" else self.FN_HOST
         if op in fns:
             if fns[op] is None:

@@ -173,9 +173,9 @@ def _seed_inputs(model, thetas, base_plans, k_grids, steps):
 
 
 def _forward_host(model) -> bool:
-    """Tangent seeds from the producer's forward mode (else from frozen-grid
-    central differences: user fluids, or host_tangents="fd")."""
-    return getattr(model, "host_tangents", "fd") == "forward" and getattr(model, "fluids", None) is None
+    """Tangent seeds from the producer's forward mode (else, with
+    host_tangents="fd", from frozen-grid central differences)."""
+    return getattr(model, "host_tangents", "fd") == "forward"
 
 
 def _tangent_inputs(model, thetas, k_grids, steps, threads: int = None):

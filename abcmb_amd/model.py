@@ -245,8 +245,7 @@ class Model:
         self.device_background = bool(device_background)
         # host_tangents: how the tangent seeds of the host inputs are made --
         # "forward": the producer's forward-mode derivatives (tangent_plans);
-        # "fd": central differences of producer runs with the grids held (the
-        # only way for user fluids, whose generated host code has no tangents)
+        # "fd": central differences of producer runs with the grids held
         self.host_tangents = "forward"
         self.chunk = 128   # cosmologies per downstream launch (bounds the source buffers)
         self.timing = None  # list: per-launch pert_fwd event timing and work counts when set
@@ -389,9 +388,7 @@ class Model:
         their central differences.  Same order and meaning as ``prepare`` with
         ``freeze=base`` at each of ``pm_params``."""
         from .background import input_doubles
-        if self.fluids is not None:
-            raise ValueError("tangent_plans: the host code generated for user fluids has no forward mode; "
-                             "use prepare(..., freeze=base) at each of pm_params")
+        fpar = (lambda q: self.fluids.fluid_par(q)) if self.fluids is not None else (lambda q: None)
         nd = len(steps)
         N = abi.N_TANGENT_DIRS
         if len(pm_params) != 2 * nd or nd > N:
@@ -401,7 +398,7 @@ class Model:
         for d in range(nd):
             pp, pm = pm_params[2 * d], pm_params[2 * d + 1]
             s = 1.0 / (2.0 * steps[d])
-            dpar[d] = (input_doubles(pp, kp) - input_doubles(pm, kp)) * s
+            dpar[d] = (input_doubles(pp, kp, fpar(pp)) - input_doubles(pm, kp, fpar(pm))) * s
             dtr[d] = (pp.get("tau_reion", 0.0) - pm.get("tau_reion", 0.0)) * s
             dne[d] = (pp["Neff"] - pm["Neff"]) * s
         bg = base.bg

@@ -215,13 +215,14 @@ int abcmb_host_eval(const AbcmbCosmo* c, const double* lna, int n, double* out);
  * lna_transfer_start negative (held); its tables are rewritten with the same
  * values.  rec_switch: that call's ABCMB_REC_STAGES stage ends (all >= 0).
  * dpar [ABCMB_TANGENT_DIRS][ABCMB_COSMO_DOUBLES], in/out: on input the
- * derivatives of the parameter fields (h .. omega_m), on output also those of
+ * derivatives of the parameter fields (h .. omega_m, fluid_par), on output also those of
  * tau0, xe_last, tm_last and z_reion (zero for the held grid metadata).
  * dtau_reion, dn_eff [ABCMB_TANGENT_DIRS]: derivatives of the tau_reion and
  * n_eff arguments.  dtau [dirs][ABCMB_TAU_TAB_N], dxe [dirs][xe_n],
  * dtm [dirs][tm_n]: the tables' derivatives.  z_reion's derivative is the
- * implicit-function derivative of the tau_reion root.  Returns ABCMB_EINVAL
- * for a build with user fluids (their generated host code has no tangents).
+ * implicit-function derivative of the tau_reion root.  A build for user
+ * fluids differentiates their generated host code the same way (the fluid_par
+ * fields take part like the others).
  */
 int abcmb_host_background_tangent(AbcmbCosmo* c, double tau_reion, const double* dtau_reion,
                                   double z_reion_in, double Delta_z_reion, double z_reion_He,

@@ -265,3 +265,39 @@ def test_fixed_fluid_parameters_reach_every_parameter_set():
     assert m.fluids.fluid_par(p)[0] == 0.3
     with pytest.raises(NotImplementedError, match="forward-mode"):
         LK.adjoint_forward(m, LK.FIDUCIAL[None])
+
+
+@pytest.mark.parametrize("which", ["w0wa", "nadm"])
+def test_forward_host_derivatives_through_fluid_code(which):
+    """The host producer's forward mode through the generated fluid code: the
+    tangent seeds of tau(ln a), x_e and the LOS background vectors within 1e-6 of
+    each derivative's maximum of central differences of frozen-grid producer
+    runs at a relative step of 1e-5 (their truncation is ~1e-10 there), with
+    the fluids' parameters held; w0wa's density follows omega_Lambda, so h moves
+    it, NADM's dark matter and radiation carry omega_cdm's and T_CMB's."""
+    from abcmb_amd import likelihood as LK
+    from abcmb_amd.model import tangent_seeds
+    m = (_w0wa() if which == "w0wa" else _nadm()).fix_params(**EX.PARAMS[which])
+    th = LK.FIDUCIAL.copy()
+    base = m.prepare(m.derived(LK.theta_to_params(th)))
+
+    def plans(rel, forward):
+        st = rel * np.abs(th)
+        q = LK._pm_params(m, th, st)
+        sd = [st[p] for p in LK.ODE_DIRS]
+        pm = m.tangent_plans(base, q, sd) if forward else [m.prepare(x, freeze=base) for x in q]
+        return pm, sd
+
+    fwd, sf = plans(1e-4, True)
+    fd, s5 = plans(1e-5, False)
+    for get in (lambda p: p.bg.tau_tab, lambda p: p.bg.xe_tab, lambda p: p.bgv):
+        a = np.stack([(get(fwd[2 * d]) - get(fwd[2 * d + 1])) / (2 * sf[d]) for d in range(len(sf))])
+        b = np.stack([(get(fd[2 * d]) - get(fd[2 * d + 1])) / (2 * s5[d]) for d in range(len(s5))])
+        a, b = a.reshape(a.shape[0], -1), b.reshape(b.shape[0], -1)
+        sc = np.abs(b).max(axis=1)
+        for d in range(len(sf)):
+            if sc[d] > 0:
+                assert np.abs(a[d] - b[d]).max() < 1e-6 * sc[d], (which, d)
+    # the seeds' fluid-parameter slots are zero along the standard directions
+    seeds = tangent_seeds(fwd[0::2], fwd[1::2], sf)
+    assert np.all(seeds.par[:, 31:] == 0.0)

@@ -14,7 +14,9 @@ Pins, on a cosmology with a massive neutrino and lensing:
     tau_reion root) against the central difference of the root;
   - closed forms of the input tangents: d omega_b / d omega_b = 1,
     d omega_Lambda / d h = 2 h, zeros where a parameter does not enter;
-  - models with user fluids keep the central-difference seeds.
+  - a model with user fluids (w0wa dark energy in place of the cosmological
+    constant): the forward derivatives through the generated host code against
+    fine central differences (tests/test_fluids.py).
 """
 import numpy as np
 import pytest
@@ -130,8 +132,7 @@ def test_plans_keep_the_base_grids(setup):
         assert p.bg.cosmo.tau_tab == p.bg.tau_tab.ctypes.data
 
 
-def test_user_fluids_keep_central_differences():
+def test_host_tangents_switch():
     import types
-    assert not LK._forward_host(types.SimpleNamespace(host_tangents="forward", fluids=object()))
-    assert not LK._forward_host(types.SimpleNamespace(host_tangents="fd", fluids=None))
-    assert LK._forward_host(types.SimpleNamespace(host_tangents="forward", fluids=None))
+    assert not LK._forward_host(types.SimpleNamespace(host_tangents="fd"))
+    assert LK._forward_host(types.SimpleNamespace(host_tangents="forward"))
