@@ -161,7 +161,7 @@ typedef struct AbcmbReion {
 /*
  * Fill the background tables and scalars of `c` from its parameters:
  * conformal time table (abcmb/background.py:252-319), the recombination
- * history x_e/T_m of HyRex (abcmb/hyrex/*.py) with the tanh reionization
+ * history x_e/T_m of HyRex (abcmb/hyrex/) with the tanh reionization
  * added (abcmb/background.py:939-1017), lna_rec and lna_transfer_start
  * (abcmb/background.py:555-564).  c->tau_tab, xe_tab and tm_tab must point
  * to writable host arrays of ABCMB_TAU_TAB_N, xe_n and tm_n doubles
