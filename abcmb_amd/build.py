@@ -35,10 +35,12 @@ def sources(kind: str):
     """The files a library is compiled from: "host" (the producer, its headers and
     the HyRex tables) or "hip" (every kernel source and header)."""
     inc = [os.path.join(ROOT, "include", "abcmb.h")]
+    host_headers = ("hyrex.h", "host_dual.h", "fluid_none_host.h")
     if kind == "host":
-        return ([os.path.join(CSRC, f) for f in HOST_SRCS + ("hyrex.h", "host_dual.h", "fluid_none_host.h")] + inc +
+        return ([os.path.join(CSRC, f) for f in HOST_SRCS + host_headers] + inc +
                 sorted(glob.glob(os.path.join(HERE, "data", "hyrex", "*.dat"))))
-    return sorted(glob.glob(os.path.join(CSRC, "*.hip")) + glob.glob(os.path.join(CSRC, "*.h"))) + inc
+    return sorted(glob.glob(os.path.join(CSRC, "*.hip")) +
+                  [h for h in glob.glob(os.path.join(CSRC, "*.h")) if os.path.basename(h) not in host_headers]) + inc
 GEN = os.path.join(ROOT, "_build", "gen")
 
 

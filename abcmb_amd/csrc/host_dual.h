@@ -198,7 +198,4 @@ inline double dbl(double x) { return x; }
 template <int N> inline double dbl(const Dual<N>& x) { return x.v; }
 template <class S> inline double dbl(const Dl<S>& x) { return dbl(x.v); }
 
-// derivative part of a Dl (the local derivative)
-template <class S> inline S dpart(const Dl<S>& x) { return x.d; }
-
 }  // namespace abcmb_ad
