@@ -1176,6 +1176,58 @@ class Model:
             return outs, run["state"], run["flat"]
         return outs
 
+    def cl_jacobian(self, params: dict, names, rel_step: float = 1e-4) -> dict:
+        """Unlensed C_l of one cosmology and their derivatives along the
+        parameters ``names`` -- input parameters (of ``params`` or of
+        ``fix_params``) that reach the perturbation equations, the user fluids'
+        own parameters included (what ``jax.jacfwd`` of the reference's
+        ``Model.__call__`` gives per parameter), at most N_TANGENT_DIRS of them
+        -- plus d/d ln A_s and d/d n_s.  The host inputs' derivatives come from the producer's forward
+        mode (``tangent_plans``, the derived parameters by central differences
+        at a relative step ``rel_step``), the tables' from one tangent-linear ODE
+        launch, the spectra's from the tangent-linear LOS.  Returns
+        ``{"l": l, "cl": [3][n_l] TT, TE, EE, name: [3][n_l] d C_l / d name,
+        "ln10As" and "n_s": [3][n_l]}`` (numpy, l = 2..l_max)."""
+        names = list(names)
+        nd = abi.N_TANGENT_DIRS
+        if not names or len(names) > nd or len(set(names)) != len(names):
+            raise ValueError(f"cl_jacobian: between 1 and {nd} distinct parameter names")
+        if any(n in ("A_s", "ln10As", "n_s") for n in names):
+            raise ValueError("cl_jacobian: A_s and n_s do not reach the perturbations; "
+                             "their derivatives are always returned")
+        full = self.derived(params)
+        for n in names:
+            if n not in full:
+                raise KeyError(f"cl_jacobian: {n!r} is not a parameter of this model")
+        base = self.prepare(full)
+        steps, pm_params = [], []
+        for n in names:
+            h = rel_step * abs(float(full[n])) or rel_step
+            steps.append(h)
+            src = dict(self.fixed_params, **params)
+            for sgn in (1.0, -1.0):
+                q = dict(src)
+                q[n] = float(full[n]) + sgn * h
+                pm_params.append(self.derived(q))
+        pm = self.tangent_plans(base, pm_params, steps)
+        plus, minus = pm[0::2], pm[1::2]
+        seeds = tangent_seeds(plus, minus, steps)
+        h = self.launch_tangent([base], [seeds])
+        run = self._ode_result(h)
+        # directions beyond ``names`` carry zero seeds: their plans are the base
+        # plan itself, at a unit step
+        pad = nd - len(names)
+        cl_t = self.downstream_tangent([base], run["pts"], run["kdev"], run["dpts"], [plus + [base] * pad],
+                                       [minus + [base] * pad], [steps + [1.0] * pad])
+        self._check_ode(h)
+        n_l = self.ells.size
+        c = cl_t[0, :, :, :n_l].cpu().numpy()
+        out = {"l": np.arange(2, 2 + n_l), "cl": c[0]}
+        for i, n in enumerate(names):
+            out[n] = c[1 + i]
+        out["ln10As"], out["n_s"] = c[1 + nd], c[2 + nd]
+        return out
+
     def execute_tangent(self, plans, seeds):
         """Outputs of ``plans`` plus the derivatives of their perturbation tables.
 

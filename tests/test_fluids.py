@@ -301,3 +301,15 @@ def test_forward_host_derivatives_through_fluid_code(which):
     # the seeds' fluid-parameter slots are zero along the standard directions
     seeds = tangent_seeds(fwd[0::2], fwd[1::2], sf)
     assert np.all(seeds.par[:, 31:] == 0.0)
+
+
+def test_cl_jacobian_refuses_bad_directions():
+    m = _w0wa().fix_params(**EX.PARAMS["w0wa"])
+    with pytest.raises(ValueError):
+        m.cl_jacobian(P0, [])
+    with pytest.raises(ValueError):
+        m.cl_jacobian(P0, ["h", "omega_b", "omega_cdm", "w0DE", "waDE", "tau_reion"])
+    with pytest.raises(ValueError):
+        m.cl_jacobian(P0, ["n_s"])
+    with pytest.raises(KeyError):
+        m.cl_jacobian(P0, ["no_such_parameter"])
