@@ -1187,7 +1187,9 @@ class Model:
         at a relative step ``rel_step``), the tables' from one tangent-linear ODE
         launch, the spectra's from the tangent-linear LOS.  Returns
         ``{"l": l, "cl": [3][n_l] TT, TE, EE, name: [3][n_l] d C_l / d name,
-        "ln10As" and "n_s": [3][n_l]}`` (numpy, l = 2..l_max)."""
+        "ln10As" and "n_s": [3][n_l]}`` (numpy, l = 2..l_max); with lensing also
+        ``"lensed"`` ([4][n_l] TT, TE, EE, BB) and ``"lensed_" + name``, the
+        lensing differenced at C_l +- h dC_l as ``tangent_gradients`` does."""
         names = list(names)
         nd = abi.N_TANGENT_DIRS
         if not names or len(names) > nd or len(set(names)) != len(names):
@@ -1219,13 +1221,33 @@ class Model:
         pad = nd - len(names)
         cl_t = self.downstream_tangent([base], run["pts"], run["kdev"], run["dpts"], [plus + [base] * pad],
                                        [minus + [base] * pad], [steps + [1.0] * pad])
-        self._check_ode(h)
         n_l = self.ells.size
+        lens = None
+        if self.specs["lensing"]:
+            # lensed spectra at theta +- h along each name: the lensing of
+            # C_l +- h dC_l with C_l^phiphi of the tables moved along their
+            # tangents and the plans at theta +- h; central differences
+            torch = _torch()
+            pt0, kd0 = run["pts"][0], run["kdev"][0]
+            hs = torch.tensor([[st, -st] for st in steps], dtype=torch.float64, device=pt0.device)
+            moved = _moved_tables(torch, pt0[:1], run["dpts"][0][:len(names), :1], hs)   # delta_m alone
+            vplans, vpts, cls = [base], [pt0[:1]], [cl_t[0, 0]]
+            for d in range(len(names)):
+                for si, sgn in enumerate((1.0, -1.0)):
+                    vplans.append(pm[2 * d + si])
+                    vpts.append(moved[2 * d + si])
+                    cls.append(cl_t[0, 0] + sgn * steps[d] * cl_t[0, 1 + d])
+            lens = self.lensed_from_cl(vplans, vpts, [kd0] * len(vplans), torch.stack(cls))[:, :, :n_l].cpu().numpy()
+        self._check_ode(h)
         c = cl_t[0, :, :, :n_l].cpu().numpy()
         out = {"l": np.arange(2, 2 + n_l), "cl": c[0]}
         for i, n in enumerate(names):
             out[n] = c[1 + i]
         out["ln10As"], out["n_s"] = c[1 + nd], c[2 + nd]
+        if lens is not None:
+            out["lensed"] = lens[0]
+            for i, n in enumerate(names):
+                out["lensed_" + n] = (lens[1 + 2 * i] - lens[2 + 2 * i]) / (2.0 * steps[i])
         return out
 
     def execute_tangent(self, plans, seeds):
