@@ -22,6 +22,10 @@ own background (the reference's jax.vjp).
 (tangent-linear ODE and LOS, the lensing differenced along each parameter)
 and keeps d C / d theta ([n][7][3][n_l]); the backward is its transpose
 applied to the cotangent, one contraction.
+
+``spectra(model, values, names, params)``: one cosmology's spectra as a
+function of any named input parameters (user fluids' own included), through
+``Model.cl_jacobian``.
 """
 from __future__ import annotations
 
@@ -86,7 +90,26 @@ def _functions():
             g = torch.as_tensor(np.concatenate(grads))
             return g.to(ctx.theta_device, ctx.theta_dtype), None, None, None
 
-    return {"jacobian": LensedClJacobian, "adjoint": LensedClAdjoint}
+    class SpectraJacobian(torch.autograd.Function):
+        @staticmethod
+        def forward(ctx, values, model, names, params):
+            v = values.detach().cpu().numpy().astype(np.float64)
+            p = dict(params)
+            p.update({n: float(x) for n, x in zip(names, v)})
+            J = model.cl_jacobian(p, names)
+            pre = "lensed_" if model.specs["lensing"] else ""
+            obs = J["lensed"][:3] if pre else J["cl"]
+            ctx.save_for_backward(torch.as_tensor(np.stack([J[pre + n][:3] for n in names])))
+            ctx.values_device, ctx.values_dtype = values.device, values.dtype
+            return torch.as_tensor(obs).to(values.device, values.dtype)
+
+        @staticmethod
+        def backward(ctx, grad_cl):
+            (jac,) = ctx.saved_tensors                                    # [n_names][3][n_l]
+            g = torch.tensordot(jac, grad_cl.to(jac.device, jac.dtype), dims=([1, 2], [0, 1]))
+            return g.to(ctx.values_device, ctx.values_dtype), None, None, None
+
+    return {"jacobian": LensedClJacobian, "adjoint": LensedClAdjoint, "spectra": SpectraJacobian}
 
 
 _FUNCS = None
@@ -100,8 +123,25 @@ def lensed_cl(model, theta, k_grids=None, chunk: int = 64, mode: str = "adjoint"
     global _FUNCS
     if _FUNCS is None:
         _FUNCS = _functions()
-    if mode not in _FUNCS:
-        raise ValueError(f"mode must be one of {tuple(_FUNCS)}")
+    if mode not in ("jacobian", "adjoint"):
+        raise ValueError("mode must be one of ('jacobian', 'adjoint')")
     if theta.dim() != 2 or theta.shape[1] != len(LK.NAMES):
         raise ValueError(f"theta must be [n][{len(LK.NAMES)}] ({', '.join(LK.NAMES)})")
     return _FUNCS[mode].apply(theta, model, k_grids, chunk)
+
+
+def spectra(model, values, names, params: dict):
+    """Observed spectra of one cosmology ([3][n_l]: lensed TT, TE, EE on
+    l = 2..l_max, or the unlensed ones without lensing), differentiable with
+    respect to ``values`` (a 1-D tensor): the values of the parameters ``names``
+    -- any inputs that reach the perturbations, at most five, the user fluids'
+    own included (``Model.cl_jacobian``) -- the other parameters taken from
+    ``params``.  The backward pass contracts the cotangent with the forward-mode
+    Jacobian kept from the forward pass."""
+    global _FUNCS
+    if _FUNCS is None:
+        _FUNCS = _functions()
+    names = list(names)
+    if values.dim() != 1 or values.shape[0] != len(names):
+        raise ValueError(f"values must be a 1-D tensor of {len(names)} entries, one per name")
+    return _FUNCS["spectra"].apply(values, model, names, params)

@@ -209,3 +209,21 @@ def test_cl_jacobian_along_a_fluid_parameter():
     c0 = m.execute([base])[0].cl_unlensed.cpu().numpy()[:, :n_l]
     assert np.max(np.abs(J["cl"] - c0) / np.abs(c0).max(axis=1, keepdims=True)) < 1e-5
     assert np.all(np.isfinite(J["N_idr"])) and np.abs(J["N_idr"][0]).max() > 0
+
+
+def test_autograd_spectra_along_fluid_parameters():
+    """autograd.spectra: a loss of SIDR's spectra back-propagates to N_idr and
+    omega_cdm as the contraction of Model.cl_jacobian's columns (1e-12)."""
+    import torch
+    from abcmb_amd.autograd import spectra
+    m = _model("sidr", l_max=500, k_axis_perturbations=np.geomspace(1e-4, 0.3, 8)).fix_params(**PARAMS["sidr"])
+    names = ["N_idr", "omega_cdm"]
+    vals = torch.tensor([0.3, 0.12], dtype=torch.float64, requires_grad=True)
+    _say("case: autograd.spectra along N_idr and omega_cdm")
+    cl = spectra(m, vals, names, P0)
+    w = torch.linspace(0.0, 1.0, cl.shape[1], dtype=torch.float64)
+    (cl[0].cpu() * w).sum().backward()
+    J = m.cl_jacobian(dict(P0, N_idr=0.3, omega_cdm=0.12), names)
+    ref = np.array([(J[n][0] * w.numpy()).sum() for n in names])
+    assert np.allclose(vals.grad.numpy(), ref, rtol=1e-12, atol=0)
+    assert np.array_equal(cl.detach().cpu().numpy(), J["cl"])
